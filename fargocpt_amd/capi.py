@@ -1,0 +1,299 @@
+"""ctypes mirror of include/fcpt.h and a thin Python handle over libfcpt_hip.so.
+
+The structures below must stay field-for-field identical to include/fcpt.h.
+Loading the library fails loudly when the HIP build is missing: there is no
+CPU fallback in the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfcpt_hip.so")
+
+FIELDS = [
+    "SIGMA", "VRAD", "VAZI", "ENERGY", "PRESSURE", "SOUNDSPEED", "SCALE_HEIGHT",
+    "VISCOSITY", "POTENTIAL", "Q_R", "Q_PHI", "DIV_V", "TAU_R_R", "TAU_R_PHI",
+    "TAU_PHI_PHI", "QPLUS", "QMINUS", "TEMPERATURE", "SIGMA0", "VRAD0", "VAZI0", "ENERGY0",
+]
+FIELD_ID = {name: i for i, name in enumerate(FIELDS)}
+VECTOR_FIELDS = {"VRAD", "VRAD0", "TAU_R_PHI"}
+
+ARTVISC = {"none": 0, "tw": 1, "sn": 2}
+BC = {"reflecting": 0, "zerogradient": 1, "outflow": 2}
+MAX_PLANETS = 8
+
+
+class DiskDesc(C.Structure):
+    _fields_ = [
+        ("nr", C.c_int32), ("nphi", C.c_int32), ("radii", C.POINTER(C.c_double)),
+        ("rmin", C.c_double), ("rmax", C.c_double),
+        ("adiabatic", C.c_int32), ("precision", C.c_int32),
+        ("G", C.c_double), ("R", C.c_double), ("sigma_sb", C.c_double), ("c_light", C.c_double),
+        ("hydro_center_mass", C.c_double), ("mu", C.c_double), ("gamma", C.c_double),
+        ("aspect_ratio", C.c_double), ("flaring_index", C.c_double), ("alpha", C.c_double),
+        ("thickness_smoothing", C.c_double),
+        ("cfl", C.c_double), ("cfl_max_var", C.c_double), ("heating_cooling_cfl_limit", C.c_double),
+        ("sigma0", C.c_double), ("sigma_floor", C.c_double), ("t_min", C.c_double),
+        ("t_max", C.c_double),
+        ("artificial_viscosity", C.c_int32), ("artvisc_dissipation", C.c_int32),
+        ("artvisc_factor", C.c_double),
+        ("heating_viscous", C.c_int32), ("heating_viscous_factor", C.c_double),
+        ("cooling_beta_enabled", C.c_int32), ("cooling_beta_ref", C.c_int32),
+        ("cooling_beta", C.c_double), ("cooling_beta_ramp_up", C.c_double),
+        ("radial_viscosity_factor", C.c_double),
+        ("bc_inner", C.c_int32), ("bc_outer", C.c_int32), ("damping", C.c_int32),
+        ("damping_inner_limit", C.c_double), ("damping_outer_limit", C.c_double),
+        ("damping_time_factor", C.c_double), ("damping_time_radius_outer", C.c_double),
+        ("damp_inner_scalar", C.c_int32), ("damp_inner_vector", C.c_int32),
+        ("damp_outer_scalar", C.c_int32), ("damp_outer_vector", C.c_int32),
+        ("omega_frame", C.c_double), ("monitor_timestep", C.c_double),
+    ]
+
+
+class Planet(C.Structure):
+    _fields_ = [("x", C.c_double), ("y", C.c_double), ("mass", C.c_double),
+                ("r_cubic", C.c_double)]
+
+
+class StepParams(C.Structure):
+    _fields_ = [("n_planets", C.c_int32), ("pad_", C.c_int32),
+                ("planet", Planet * MAX_PLANETS),
+                ("indirect_x", C.c_double), ("indirect_y", C.c_double)]
+
+
+def make_step_params(planets, indirect=(0.0, 0.0)) -> StepParams:
+    """planets: iterable of (x, y, mass, r_cubic); body 0 is the central object."""
+    sp = StepParams()
+    planets = list(planets)
+    if len(planets) > MAX_PLANETS:
+        raise ValueError("too many planets")
+    sp.n_planets = len(planets)
+    for k, (x, y, m, rc) in enumerate(planets):
+        sp.planet[k] = Planet(float(x), float(y), float(m), float(rc))
+    sp.indirect_x, sp.indirect_y = float(indirect[0]), float(indirect[1])
+    return sp
+
+
+def desc_from_dict(cfg: dict) -> tuple[DiskDesc, np.ndarray]:
+    """Build a DiskDesc from a flat parameter dict (keys as in DiskDesc).
+
+    ``cfg['radii']`` must hold nr+2 interface radii.  Returns the struct and the
+    radii array, which the caller must keep alive while the struct is used.
+    """
+    radii = np.ascontiguousarray(cfg["radii"], dtype=np.float64)
+    d = DiskDesc()
+    defaults = {
+        "precision": 0, "heating_cooling_cfl_limit": 1.0, "radial_viscosity_factor": 1.0,
+        "cooling_beta_ramp_up": 0.0, "bc_inner": 0, "bc_outer": 0,
+        "damp_inner_scalar": -1, "damp_inner_vector": -1, "damp_outer_scalar": -1,
+        "damp_outer_vector": -1,
+    }
+    for name, _ in DiskDesc._fields_:
+        if name == "radii":
+            continue
+        if name in cfg:
+            setattr(d, name, cfg[name])
+        elif name in defaults:
+            setattr(d, name, defaults[name])
+        else:
+            raise KeyError(f"disk description lacks {name!r}")
+    if len(radii) != d.nr + 2:
+        raise ValueError("radii must hold nr+2 values")
+    d.radii = radii.ctypes.data_as(C.POINTER(C.c_double))
+    return d, radii
+
+
+def field_shape(name: str, nr: int, nphi: int) -> tuple[int, int]:
+    return (nr + 1, nphi) if name in VECTOR_FIELDS else (nr, nphi)
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libfcpt_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with __graft_entry__.build()")
+    lib = C.CDLL(path)
+    P = C.c_void_p
+    D = C.c_double
+    I = C.c_int32
+    dp = C.POINTER(C.c_double)
+    sig = {
+        "fcpt_create": [C.POINTER(DiskDesc), C.POINTER(P)],
+        "fcpt_destroy": [P],
+        "fcpt_synchronize": [P],
+        "fcpt_upload_state": [P, dp, dp, dp, dp],
+        "fcpt_download_state": [P, dp, dp, dp, dp],
+        "fcpt_upload_reference": [P, dp, dp, dp, dp],
+        "fcpt_upload_field": [P, I, dp],
+        "fcpt_download_field": [P, I, dp],
+        "fcpt_set_time": [P, D, D, I],
+        "fcpt_get_time": [P, dp, dp, C.POINTER(I)],
+        "fcpt_set_step_params": [P, C.POINTER(StepParams)],
+        "fcpt_condition_cfl": [P, dp],
+        "fcpt_calculate_timestep": [P, dp],
+        "fcpt_nbody_potential": [P],
+        "fcpt_update_with_sourceterms": [P, D],
+        "fcpt_artificial_viscosity": [P, D],
+        "fcpt_recalculate_viscosity": [P],
+        "fcpt_viscous_stress_tensor": [P],
+        "fcpt_update_velocities_with_viscosity": [P, D],
+        "fcpt_substep3": [P, D, D],
+        "fcpt_apply_boundary_condition": [P, D, I],
+        "fcpt_transport": [P, D],
+        "fcpt_recalculate_derived": [P],
+        "fcpt_get_nshift": [P, C.POINTER(I)],
+        "fcpt_step_euler": [P, D],
+        "fcpt_run_step": [P, dp],
+        "fcpt_totals": [P, D, dp, dp],
+    }
+    for name, args in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = C.c_int
+    lib.fcpt_last_error.argtypes = [P]
+    lib.fcpt_last_error.restype = C.c_char_p
+    _lib = lib
+    return lib
+
+
+def exported_symbols() -> list[str]:
+    """Function names declared in include/fcpt.h."""
+    import re
+    hdr = os.path.join(os.path.dirname(HERE), "include", "fcpt.h")
+    return re.findall(r"^\w[\w\s\*]*?\b(fcpt_\w+)\s*\(", open(hdr).read(), re.M)
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Disk:
+    """One disk resident on the GPU, driven through the C ABI."""
+
+    def __init__(self, cfg: dict):
+        self.lib = load_library()
+        self.desc, self._radii = desc_from_dict(cfg)
+        self.nr, self.nphi = self.desc.nr, self.desc.nphi
+        h = C.c_void_p()
+        self._check(self.lib.fcpt_create(C.byref(self.desc), C.byref(h)), None)
+        self.h = h
+
+    def _check(self, rc, h=0):
+        if rc != 0:
+            msg = self.lib.fcpt_last_error(self.h if h == 0 else h)
+            raise RuntimeError(f"fcpt call failed ({rc}): {msg.decode() if msg else ''}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.fcpt_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # state
+    def upload_state(self, sigma, vrad, vazi, energy=None):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (sigma, vrad, vazi)]
+        e = None if energy is None else np.ascontiguousarray(energy, dtype=np.float64)
+        self._check(self.lib.fcpt_upload_state(self.h, *[_dptr(a) for a in arrs],
+                                               _dptr(e) if e is not None else None))
+
+    def upload_reference(self, sigma0, vrad0, vazi0, energy0):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (sigma0, vrad0, vazi0, energy0)]
+        self._check(self.lib.fcpt_upload_reference(self.h, *[_dptr(a) for a in arrs]))
+
+    def upload_field(self, name, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        assert a.shape == field_shape(name, self.nr, self.nphi), (name, a.shape)
+        self._check(self.lib.fcpt_upload_field(self.h, FIELD_ID[name], _dptr(a)))
+
+    def download_field(self, name):
+        a = np.empty(field_shape(name, self.nr, self.nphi))
+        self._check(self.lib.fcpt_download_field(self.h, FIELD_ID[name], _dptr(a)))
+        return a
+
+    def set_time(self, time, last_dt, n_monitor=0):
+        self._check(self.lib.fcpt_set_time(self.h, time, last_dt, n_monitor))
+
+    def get_time(self):
+        t, l, n = C.c_double(), C.c_double(), C.c_int32()
+        self._check(self.lib.fcpt_get_time(self.h, C.byref(t), C.byref(l), C.byref(n)))
+        return t.value, l.value, n.value
+
+    def set_step_params(self, planets, indirect=(0.0, 0.0)):
+        sp = make_step_params(planets, indirect)
+        self._check(self.lib.fcpt_set_step_params(self.h, C.byref(sp)))
+
+    # hydro step functions (names follow the reference)
+    def condition_cfl(self):
+        v = C.c_double()
+        self._check(self.lib.fcpt_condition_cfl(self.h, C.byref(v)))
+        return v.value
+
+    def calculate_timestep(self):
+        v = C.c_double()
+        self._check(self.lib.fcpt_calculate_timestep(self.h, C.byref(v)))
+        return v.value
+
+    def nbody_potential(self):
+        self._check(self.lib.fcpt_nbody_potential(self.h))
+
+    def update_with_sourceterms(self, dt):
+        self._check(self.lib.fcpt_update_with_sourceterms(self.h, dt))
+
+    def artificial_viscosity(self, dt):
+        self._check(self.lib.fcpt_artificial_viscosity(self.h, dt))
+
+    def recalculate_viscosity(self):
+        self._check(self.lib.fcpt_recalculate_viscosity(self.h))
+
+    def viscous_stress_tensor(self):
+        self._check(self.lib.fcpt_viscous_stress_tensor(self.h))
+
+    def update_velocities_with_viscosity(self, dt):
+        self._check(self.lib.fcpt_update_velocities_with_viscosity(self.h, dt))
+
+    def substep3(self, time, dt):
+        self._check(self.lib.fcpt_substep3(self.h, time, dt))
+
+    def apply_boundary_condition(self, dt, final):
+        self._check(self.lib.fcpt_apply_boundary_condition(self.h, dt, int(final)))
+
+    def transport(self, dt):
+        self._check(self.lib.fcpt_transport(self.h, dt))
+
+    def recalculate_derived(self):
+        self._check(self.lib.fcpt_recalculate_derived(self.h))
+
+    def nshift(self):
+        a = np.empty(self.nr, dtype=np.int32)
+        self._check(self.lib.fcpt_get_nshift(self.h, a.ctypes.data_as(C.POINTER(C.c_int32))))
+        return a
+
+    def step_euler(self, dt):
+        self._check(self.lib.fcpt_step_euler(self.h, dt))
+
+    def run_step(self):
+        v = C.c_double()
+        self._check(self.lib.fcpt_run_step(self.h, C.byref(v)))
+        return v.value
+
+    def totals(self, radius):
+        m, l = C.c_double(), C.c_double()
+        self._check(self.lib.fcpt_totals(self.h, radius, C.byref(m), C.byref(l)))
+        return m.value, l.value
+
+    def synchronize(self):
+        self._check(self.lib.fcpt_synchronize(self.h))

@@ -1,0 +1,188 @@
+/*
+ * fcpt.h -- C ABI of the MI355X (gfx950) hydro-step library libfcpt_hip.so.
+ *
+ * One context (fcpt_ctx) holds one 2-D polar disk resident in HBM: the evolved
+ * fields (Sigma, v_r, v_phi, E), the fields derived from them, the damping
+ * reference fields and the per-ring metric arrays.  Every entry point below
+ * replaces one function of the reference's per-timestep hydro update; the
+ * reference function and its declaration (file:line in the reference tree)
+ * are named in the comment above each declaration.
+ *
+ * Layout contract (identical to the reference's t_polargrid, polargrid.h:111-123):
+ *   scalar fields   [nr][nphi]      row-major, phi contiguous
+ *   v_r, tau_r_phi  [nr + 1][nphi]  (the reference's "vector" grids)
+ *   ring 0 and ring nr-1 are ghost rings; Radii[1] = Rmin, Radii[nr-1] = Rmax.
+ * Host arrays passed to upload/download use exactly this layout.
+ *
+ * Errors: every function returns 0 on success or a negative fcpt_status; the
+ * message of the last error on a context is returned by fcpt_last_error().
+ * No entry point aborts the process.  A context may be used by one host
+ * thread at a time; distinct contexts may be used concurrently.
+ */
+#ifndef FCPT_H
+#define FCPT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fcpt_ctx fcpt_ctx;
+
+typedef enum fcpt_status {
+    FCPT_OK = 0,
+    FCPT_ERR_ARG = -1,      /* invalid argument or size */
+    FCPT_ERR_DEVICE = -2,   /* HIP runtime error */
+    FCPT_ERR_STATE = -3,    /* call order / missing state */
+    FCPT_ERR_NONFINITE = -4 /* CFL reduction met a NaN or non-positive dt */
+} fcpt_status;
+
+/* artificial viscosity (reference parameters.h:156-161) */
+enum { FCPT_ARTVISC_NONE = 0, FCPT_ARTVISC_TW = 1, FCPT_ARTVISC_SN = 2 };
+/* composite inner/outer boundary (reference boundary_conditions/config.cpp) */
+enum { FCPT_BC_REFLECTING = 0, FCPT_BC_ZEROGRADIENT = 1, FCPT_BC_OUTFLOW = 2 };
+/* beta-cooling reference energy (reference parameters.cpp:443-457) */
+enum { FCPT_BETA_REF_ZERO = 0, FCPT_BETA_REF_REFERENCE = 1, FCPT_BETA_REF_FLOOR = 3 };
+/* arithmetic of the bulk stencil */
+enum { FCPT_PREC_FP64 = 0 };
+
+/* field identifiers for fcpt_upload_field / fcpt_download_field */
+typedef enum fcpt_field {
+    FCPT_SIGMA = 0, FCPT_VRAD, FCPT_VAZI, FCPT_ENERGY,
+    FCPT_PRESSURE, FCPT_SOUNDSPEED, FCPT_SCALE_HEIGHT, FCPT_VISCOSITY,
+    FCPT_POTENTIAL, FCPT_Q_R, FCPT_Q_PHI, FCPT_DIV_V,
+    FCPT_TAU_R_R, FCPT_TAU_R_PHI, FCPT_TAU_PHI_PHI, FCPT_QPLUS,
+    FCPT_QMINUS, FCPT_TEMPERATURE,
+    FCPT_SIGMA0, FCPT_VRAD0, FCPT_VAZI0, FCPT_ENERGY0,
+    FCPT_NUM_FIELDS
+} fcpt_field;
+
+/*
+ * Disk description.  Numerical and physical parameters in code units, named
+ * after the reference's parameters:: / constants:: globals they stand for.
+ */
+typedef struct fcpt_disk_desc {
+    int32_t nr;                 /* NRadial, including both ghost rings (>= 8) */
+    int32_t nphi;               /* NAzimuthal (>= 4) */
+    const double *radii;        /* Radii[0 .. nr+1]: nr+2 interface radii; the last one
+                                   is only read through the reference's Rmed[nr] */
+    double rmin, rmax;          /* configured domain edges (the reference's RMIN, RMAX
+                                   globals; Radii[nr-1] may differ from RMAX in the last bit) */
+    int32_t adiabatic;          /* 0 locally isothermal (AspectRatioMode 0), 1 ideal gas */
+    int32_t precision;          /* FCPT_PREC_* */
+    /* constants (constants.cpp: code values) */
+    double G, R, sigma_sb, c_light;
+    double hydro_center_mass;   /* mass the Keplerian quantities refer to */
+    double mu, gamma;           /* parameters::MU, ADIABATICINDEX */
+    double aspect_ratio, flaring_index; /* isothermal sound speed profile */
+    double alpha;               /* parameters::viscous_alpha (CONST_ALPHA) */
+    double thickness_smoothing; /* potential smoothing in units of H */
+    double cfl, cfl_max_var, heating_cooling_cfl_limit;
+    double sigma0, sigma_floor; /* floor = sigma_floor * sigma0 */
+    double t_min, t_max;        /* temperature range (code units) */
+    int32_t artificial_viscosity;  /* FCPT_ARTVISC_* */
+    int32_t artvisc_dissipation;
+    double artvisc_factor;
+    int32_t heating_viscous; double heating_viscous_factor;
+    int32_t cooling_beta_enabled; int32_t cooling_beta_ref; /* FCPT_BETA_REF_* */
+    double cooling_beta, cooling_beta_ramp_up;
+    double radial_viscosity_factor;
+    int32_t bc_inner, bc_outer; /* FCPT_BC_*; v_phi ghost rings are Keplerian */
+    int32_t damping;            /* initial-value damping of all four fields */
+    double damping_inner_limit, damping_outer_limit, damping_time_factor,
+        damping_time_radius_outer;
+    /* last ring index damped at the inner edge / first at the outer edge, as the
+       reference computes them (boundary_conditions/damping.cpp:321-327,377-383);
+       -1 lets the library compute them for a logarithmic mesh */
+    int32_t damp_inner_scalar, damp_inner_vector, damp_outer_scalar, damp_outer_vector;
+    double omega_frame;         /* refframe::OmegaFrame (fixed frame rotation) */
+    double monitor_timestep;    /* parameters::monitor_timestep for the dt clip */
+} fcpt_disk_desc;
+
+#define FCPT_MAX_PLANETS 8
+
+/* Host-side N-body state consumed by one step (reference Pframeforce.cpp:27-35). */
+typedef struct fcpt_planet {
+    double x, y;        /* position in the hydro frame */
+    double mass;        /* ramp-up mass at the current time */
+    double r_cubic;     /* Klahr-Kley cubic smoothing radius; 0 disables */
+} fcpt_planet;
+
+typedef struct fcpt_step_params {
+    int32_t n_planets;                 /* body 0 is the central object */
+    int32_t pad_;
+    fcpt_planet planet[FCPT_MAX_PLANETS];
+    double indirect_x, indirect_y;     /* refframe::IndirectTerm */
+} fcpt_step_params;
+
+/* ---- context ---------------------------------------------------------- */
+int fcpt_create(const fcpt_disk_desc *desc, fcpt_ctx **out);
+int fcpt_destroy(fcpt_ctx *ctx);
+const char *fcpt_last_error(const fcpt_ctx *ctx);
+int fcpt_synchronize(fcpt_ctx *ctx);
+
+/* ---- state transfer (host arrays in the reference layout) -------------- */
+/* Sets Sigma, v_r, v_phi (and E when adiabatic), then recomputes the derived
+   fields like the reference's init_euler (SourceEuler.cpp:251-284). */
+int fcpt_upload_state(fcpt_ctx *ctx, const double *sigma, const double *vrad,
+                      const double *vazi, const double *energy);
+int fcpt_download_state(fcpt_ctx *ctx, double *sigma, double *vrad, double *vazi,
+                        double *energy);
+/* Damping reference values (reference copy_initial_values, damping.cpp:301-309). */
+int fcpt_upload_reference(fcpt_ctx *ctx, const double *sigma0, const double *vrad0,
+                          const double *vazi0, const double *energy0);
+int fcpt_upload_field(fcpt_ctx *ctx, int32_t field, const double *host);
+int fcpt_download_field(fcpt_ctx *ctx, int32_t field, double *host);
+int fcpt_set_time(fcpt_ctx *ctx, double time, double last_dt, int32_t n_monitor);
+int fcpt_get_time(fcpt_ctx *ctx, double *time, double *last_dt, int32_t *n_monitor);
+int fcpt_set_step_params(fcpt_ctx *ctx, const fcpt_step_params *params);
+
+/* ---- the hydro step, one entry point per reference function ------------ */
+/* cfl::condition_cfl(t_data&, double)                      cfl.h:8, cfl.cpp:185-382 */
+int fcpt_condition_cfl(fcpt_ctx *ctx, double *dt_out);
+/* sim::CalculateTimeStep(t_data&)               simulation.h:19, simulation.cpp:100-118 */
+int fcpt_calculate_timestep(fcpt_ctx *ctx, double *dt_out);
+/* CalculateNbodyPotential(t_data&, double)               Pframeforce.h, Pframeforce.cpp:21-86 */
+int fcpt_nbody_potential(fcpt_ctx *ctx);
+/* update_with_sourceterms(t_data&, double)             SourceEuler.h:16, SourceEuler.cpp:435-452 */
+int fcpt_update_with_sourceterms(fcpt_ctx *ctx, double dt);
+/* art_visc::update_with_artificial_viscosity(t_data&, double)  artificial_viscosity.cpp:11-26 */
+int fcpt_artificial_viscosity(fcpt_ctx *ctx, double dt);
+/* recalculate_viscosity(t_data&, double)               SourceEuler.h:12, SourceEuler.cpp:205-223 */
+int fcpt_recalculate_viscosity(fcpt_ctx *ctx);
+/* viscosity::compute_viscous_stress_tensor(t_data&)    viscosity.h:9, viscosity.cpp:139-350 */
+int fcpt_viscous_stress_tensor(fcpt_ctx *ctx);
+/* viscosity::update_velocities_with_viscosity(t_data&, double)  viscosity.cpp:355-426 */
+int fcpt_update_velocities_with_viscosity(fcpt_ctx *ctx, double dt);
+/* SubStep3(t_data&, double, double)                    SourceEuler.h:18, SourceEuler.cpp:859-954 */
+int fcpt_substep3(fcpt_ctx *ctx, double time, double dt);
+/* boundary_conditions::apply_boundary_condition(t_data&, double, double, bool)
+                                          boundary_conditions.h:106, boundary_conditions.cpp:65-114 */
+int fcpt_apply_boundary_condition(fcpt_ctx *ctx, double dt, int32_t final_call);
+/* Transport(t_data&, PolarGrid*, PolarGrid*, PolarGrid*, PolarGrid*, double)
+                                                         TransportEuler.h:5-6, TransportEuler.cpp:112-136 */
+int fcpt_transport(fcpt_ctx *ctx, double dt);
+/* recalculate_derived_disk_quantities(t_data&, double) SourceEuler.h:11, SourceEuler.cpp:225-249 */
+int fcpt_recalculate_derived(fcpt_ctx *ctx);
+/* FARGO integer shifts of the last fcpt_transport call (TransportEuler.cpp:215-220) */
+int fcpt_get_nshift(fcpt_ctx *ctx, int32_t *nshift);
+
+/* ---- fused driver ------------------------------------------------------ */
+/* step_Euler(t_data&, double) (simulation.cpp:148-267) for the gas: potential,
+   sources, artificial viscosity, viscosity, SubStep3, boundaries, transport,
+   final boundaries and derived fields, with the step parameters last set.
+   Advances the context time by dt. */
+int fcpt_step_euler(fcpt_ctx *ctx, double dt);
+/* One iteration of sim::run (simulation.cpp:515-556): CalculateTimeStep, the
+   clip to the next monitor time, step_Euler.  *dt_out receives the step size. */
+int fcpt_run_step(fcpt_ctx *ctx, double *dt_out);
+
+/* Compensated (Neumaier) totals over active rings with Rmed <= radius
+   (quantities::gas_total_mass / gas_angular_momentum, quantities.cpp:51-78,242-279). */
+int fcpt_totals(fcpt_ctx *ctx, double radius, double *mass, double *angular_momentum);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FCPT_H */

@@ -1,0 +1,129 @@
+"""Helpers that turn the committed reference fixtures into driver calls.
+
+Each tests/golden/<case>.npz was written by tests/golden/make_golden.py from
+the reference FargoCPT build.  It holds the reference's code-unit constants and
+parameters (meta), the geometry it computed, the per-step host inputs it fed
+its potential (planet x, y, mass, cubic radius and the indirect term), the
+per-step dt values and FARGO shifts, and checkpointed states.
+"""
+from __future__ import annotations
+
+import json
+import os
+
+import numpy as np
+
+from fargocpt_amd.ic import DiskSetup, initial_state
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+# stage order of the reference's step_Euler as dumped by refdump.cpp
+STAGES_ISO = ["start", "potential", "sources", "artvisc", "viscosity", "stress",
+              "viscupdate", "bc", "transport", "bcfinal", "derived"]
+STAGES_ADI = ["start", "potential", "sources", "artvisc", "viscosity", "stress",
+              "viscupdate", "substep3", "bc", "transport", "bcfinal", "derived"]
+STAGE_FIELDS = {
+    "sigma": "SIGMA", "vrad": "VRAD", "vazi": "VAZI", "energy": "ENERGY",
+    "pressure": "PRESSURE", "soundspeed": "SOUNDSPEED", "scaleheight": "SCALE_HEIGHT",
+    "viscosity": "VISCOSITY", "potential": "POTENTIAL", "qr": "Q_R", "qphi": "Q_PHI",
+    "divv": "DIV_V", "taurr": "TAU_R_R", "taurp": "TAU_R_PHI", "taupp": "TAU_PHI_PHI",
+    "qplus": "QPLUS", "qminus": "QMINUS", "temperature": "TEMPERATURE",
+}
+FIRST_DT = 0.1  # FirstDT of every fixture config
+
+
+def load_case(name: str):
+    data = np.load(os.path.join(GOLDEN, f"{name}.npz"))
+    meta = json.loads(bytes(data["meta_json"]).decode())
+    return meta, data
+
+
+def disk_config(meta: dict) -> dict:
+    """Flat parameter dict for fargocpt_amd.capi.desc_from_dict."""
+    from fargocpt_amd.ic import log_radii
+    nr = meta["Nr"]
+    radii = log_radii(nr, meta["RMIN"], meta["RMAX"])
+    return {
+        "nr": nr, "nphi": meta["Nphi"], "radii": radii, "rmin": meta["RMIN"], "rmax": meta["RMAX"],
+        "adiabatic": meta["adiabatic"], "precision": 0,
+        "G": meta["G"], "R": meta["R"], "sigma_sb": meta["sigma_sb"], "c_light": meta["c"],
+        "hydro_center_mass": meta["hydro_center_mass"], "mu": meta["mu"], "gamma": meta["gamma"],
+        "aspect_ratio": meta["aspectratio"], "flaring_index": meta["flaring"],
+        "alpha": meta["alpha"], "thickness_smoothing": meta["thickness_smoothing"],
+        "cfl": meta["cfl"], "cfl_max_var": meta["cfl_max_var"],
+        "heating_cooling_cfl_limit": 1.0,
+        "sigma0": meta["sigma0"], "sigma_floor": meta["sigma_floor"],
+        "t_min": meta["Tmin"], "t_max": meta["Tmax"],
+        "artificial_viscosity": meta["artvisc"],
+        "artvisc_dissipation": meta["artvisc_dissipation"],
+        "artvisc_factor": meta["artvisc_factor"],
+        "heating_viscous": meta["heating_viscous"],
+        "heating_viscous_factor": meta["heating_viscous_factor"],
+        "cooling_beta_enabled": meta["cooling_beta_enabled"],
+        "cooling_beta_ref": 3 if meta["cooling_beta_floor"] else 0,
+        "cooling_beta": meta["cooling_beta"], "cooling_beta_ramp_up": 0.0,
+        "radial_viscosity_factor": meta["radial_viscosity_factor"],
+        "bc_inner": 0, "bc_outer": 0, "damping": 1,
+        "damping_inner_limit": meta["damping_inner_limit"],
+        "damping_outer_limit": meta["damping_outer_limit"],
+        "damping_time_factor": meta["damping_time_factor"],
+        "damping_time_radius_outer": meta["damping_time_radius_outer"],
+        "damp_inner_scalar": -1, "damp_inner_vector": -1,
+        "damp_outer_scalar": -1, "damp_outer_vector": -1,
+        "omega_frame": meta["OmegaFrame"], "monitor_timestep": meta["monitor_timestep"],
+    }
+
+
+def initial_condition(meta: dict) -> dict:
+    setup = DiskSetup(nr=meta["Nr"], nphi=meta["Nphi"], rmin=meta["RMIN"], rmax=meta["RMAX"],
+                      sigma0=meta["sigma0"], sigma_slope=meta["sigma_slope"],
+                      aspect_ratio=meta["aspectratio"], flaring=meta["flaring"], G=meta["G"],
+                      mstar=meta["hydro_center_mass"], omega_frame=meta["OmegaFrame"],
+                      adiabatic=bool(meta["adiabatic"]), gamma=meta["gamma"],
+                      noise_amp=meta["noise_amp"], seed=meta["seed"])
+    return initial_state(setup)
+
+
+def start_run(disk, meta: dict):
+    """Upload the initial condition and replay the reference's start-up calls
+    (main.cpp:117 CalculateTimeStep, then sim::init, simulation.cpp:462-482)."""
+    ic = initial_condition(meta)
+    disk.upload_reference(ic["sigma"], ic["vrad"], ic["vazi"], ic["energy"])
+    disk.upload_state(ic["sigma"], ic["vrad"], ic["vazi"], ic["energy"])
+    disk.set_time(0.0, FIRST_DT, 0)
+    disk.calculate_timestep()
+    disk.apply_boundary_condition(0.0, False)
+    disk.calculate_timestep()
+    return ic
+
+
+def step_inputs(data, step: int):
+    """(planets, indirect) the reference's potential saw at 1-based step."""
+    pl = data["planets"][step - 1]
+    planets = [tuple(p) for p in pl]
+    rec = data["steps"][step - 1]
+    return planets, (rec[6], rec[7])
+
+
+def stage_states(data, meta: dict, step: int):
+    """Yield (stage_name, full_state_dict) of the per-stage dumps of one step."""
+    stages = STAGES_ADI if meta["adiabatic"] else STAGES_ISO
+    cur = {}
+    # the first stage of step 1 holds every field; later stages only changes
+    for s in range(1, step + 1):
+        for k, name in enumerate(stages):
+            for f in STAGE_FIELDS:
+                key = f"stage_{s}_{k:02d}_{name}_{f}"
+                if key in data.files:
+                    cur[f] = data[key]
+            if s == step:
+                yield name, dict(cur)
+
+
+def rel_err(a: np.ndarray, b: np.ndarray) -> float:
+    """max |a-b| / max |b| (0 when both are zero)."""
+    scale = float(np.max(np.abs(b)))
+    diff = float(np.max(np.abs(a - b)))
+    if scale == 0.0:
+        return diff
+    return diff / scale
