@@ -1,0 +1,554 @@
+// C ABI of libfcpt_hip.so (include/fcpt.h): context management, state
+// transfer and one entry point per reference hydro-step function.  All device
+// work of a context is enqueued on the context's own HIP stream (default
+// priority); functions that hand a value back to the host synchronise it.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "fcpt_device.hpp"
+#include "fcpt_kernels.hpp"
+
+using namespace fcpt;
+
+struct fcpt_ctx {
+    fcpt_disk_desc desc{};
+    KParams P{};
+    hipStream_t stream = nullptr;
+    int device = 0;
+    std::vector<void *> allocs;
+    DevScalars *s_dev = nullptr;
+    DevScalars *s_host = nullptr;  // pinned mirror for read-back
+    double *totals_ring = nullptr;
+    double *totals_out = nullptr;
+    double *totals_host = nullptr;
+    std::string err;
+};
+
+namespace {
+
+int fail(fcpt_ctx *c, int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                            \
+    do {                                                                              \
+        hipError_t e_ = (expr);                                                       \
+        if (e_ != hipSuccess)                                                         \
+            return fail((ctx), FCPT_ERR_DEVICE, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+#define LAUNCH_CHECK(ctx)                                                             \
+    do {                                                                              \
+        hipError_t e_ = hipGetLastError();                                            \
+        if (e_ != hipSuccess)                                                         \
+            return fail((ctx), FCPT_ERR_DEVICE, "kernel launch: %s", hipGetErrorString(e_)); \
+    } while (0)
+
+#define CTX_CHECK(ctx)                                   \
+    do {                                                 \
+        if (!(ctx)) return FCPT_ERR_ARG;                 \
+        hipError_t e_ = hipSetDevice((ctx)->device);     \
+        if (e_ != hipSuccess)                            \
+            return fail((ctx), FCPT_ERR_DEVICE, "hipSetDevice: %s", hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename T>
+int dev_alloc(fcpt_ctx *c, T **p, size_t count)
+{
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, count * sizeof(T) + 256);
+    if (e != hipSuccess)
+        return fail(c, FCPT_ERR_DEVICE, "hipMalloc(%zu): %s", count * sizeof(T), hipGetErrorString(e));
+    c->allocs.push_back(q);
+    *p = static_cast<T *>(q);
+    return FCPT_OK;
+}
+
+size_t field_rows(int field, int nr)
+{
+    return (field == FCPT_VRAD || field == FCPT_VRAD0 || field == FCPT_TAU_R_PHI) ? (size_t)nr + 1
+                                                                                : (size_t)nr;
+}
+
+// log-mesh ring lookups with the reference's NDEBUG clamps
+// (find_cell_id.cpp:218-224, 259-264, 107-215; damping.cpp:321-327, 377-383)
+int rmed_id_log(const fcpt_disk_desc &d, double r)
+{
+    const double g = std::pow(d.rmax / d.rmin, 1.0 / ((double)d.nr - 2.0));
+    const double opt = 3.0 / 2.0 / d.rmin * (1 - std::pow(g, 2.0)) / (1 - std::pow(g, 3.0));
+    const double inv_log = 1.0 / std::log(g);
+    int id = (int)std::floor(std::log(r * opt) * inv_log) + 1;
+    if (id < 0) id = 0;
+    if (id > d.nr - 1) id = d.nr - 1;
+    return id;
+}
+int rinf_id_log(const fcpt_disk_desc &d, double r)
+{
+    const double g = std::pow(d.rmax / d.rmin, 1.0 / ((double)d.nr - 2.0));
+    const double inv_log = 1.0 / std::log(g);
+    int id = (int)std::floor(std::log(r / d.rmin) * inv_log) + 1;
+    if (id < 0) id = 0;
+    if (id > d.nr - 1) id = d.nr - 1;
+    return id;
+}
+int clampi(int v, int hi) { return v < 0 ? 0 : (v > hi ? hi : v); }
+
+}  // namespace
+
+extern "C" {
+
+const char *fcpt_last_error(const fcpt_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int fcpt_create(const fcpt_disk_desc *desc, fcpt_ctx **out)
+{
+    if (!desc || !out) return FCPT_ERR_ARG;
+    *out = nullptr;
+    if (desc->nr < 8 || desc->nr > 16384 || desc->nphi < 4 || !desc->radii)
+        return FCPT_ERR_ARG;
+    if (desc->precision != FCPT_PREC_FP64) return FCPT_ERR_ARG;
+    for (int i = 0; i + 1 < desc->nr + 2; ++i)
+        if (!(desc->radii[i + 1] > desc->radii[i]) || !(desc->radii[i] > 0.0)) return FCPT_ERR_ARG;
+    fcpt_ctx *c = new fcpt_ctx();
+    c->desc = *desc;
+    c->desc.radii = nullptr;
+    hipError_t e = hipGetDevice(&c->device);
+    if (e != hipSuccess) {
+        delete c;
+        return FCPT_ERR_DEVICE;
+    }
+    const int nr = desc->nr, ns = desc->nphi;
+    KParams &P = c->P;
+    P.nr = nr;
+    P.ns = ns;
+    // Interpret.cpp:230-231
+    P.dphi = 2.0 * M_PI / (double)ns;
+    P.invdphi = (double)ns / (2.0 * M_PI);
+    P.rmin = desc->rmin;
+    P.rmax = desc->rmax;
+    P.adiabatic = desc->adiabatic;
+    P.artvisc = desc->artificial_viscosity;
+    P.artvisc_dissipation = desc->artvisc_dissipation;
+    P.heating_viscous = desc->heating_viscous;
+    P.cooling_beta_enabled = desc->cooling_beta_enabled;
+    P.cooling_beta_ref = desc->cooling_beta_ref;
+    P.bc_inner = desc->bc_inner;
+    P.bc_outer = desc->bc_outer;
+    P.damping = desc->damping;
+    P.G = desc->G;
+    P.Rgas = desc->R;
+    P.sigma_sb = desc->sigma_sb;
+    P.c_light = desc->c_light;
+    P.mstar = desc->hydro_center_mass;
+    P.mu = desc->mu;
+    P.gamma = desc->gamma;
+    P.aspect_ratio = desc->aspect_ratio;
+    P.flaring = desc->flaring_index;
+    P.alpha = desc->alpha;
+    P.thickness_smoothing = desc->thickness_smoothing;
+    P.cfl = desc->cfl;
+    P.cfl_max_var = desc->cfl_max_var;
+    P.heating_cooling_cfl_limit = desc->heating_cooling_cfl_limit;
+    P.sigma0 = desc->sigma0;
+    P.sigma_floor = desc->sigma_floor;
+    P.t_min = desc->t_min;
+    P.t_max = desc->t_max;
+    P.artvisc_factor = desc->artvisc_factor;
+    P.heating_viscous_factor = desc->heating_viscous_factor;
+    P.cooling_beta = desc->cooling_beta;
+    P.cooling_beta_ramp_up = desc->cooling_beta_ramp_up;
+    P.radial_viscosity_factor = desc->radial_viscosity_factor;
+    P.damping_inner_limit = desc->damping_inner_limit;
+    P.damping_outer_limit = desc->damping_outer_limit;
+    P.damping_time_factor = desc->damping_time_factor;
+    P.damping_time_radius_outer = desc->damping_time_radius_outer;
+    P.omega_frame = desc->omega_frame;
+    P.monitor_timestep = desc->monitor_timestep;
+    int ids[4] = {desc->damp_inner_scalar, desc->damp_inner_vector, desc->damp_outer_scalar,
+                  desc->damp_outer_vector};
+    if (ids[0] < 0) ids[0] = clampi(rmed_id_log(*desc, desc->rmin * desc->damping_inner_limit), nr - 1);
+    if (ids[1] < 0) ids[1] = clampi(rinf_id_log(*desc, desc->rmin * desc->damping_inner_limit), nr);
+    if (ids[2] < 0)
+        ids[2] = clampi(rmed_id_log(*desc, desc->rmax * desc->damping_outer_limit) + 1, nr - 1);
+    if (ids[3] < 0)
+        ids[3] = clampi(rinf_id_log(*desc, desc->rmax * desc->damping_outer_limit) + 1, nr);
+    memcpy(P.damp_ids, ids, sizeof(ids));
+    if (desc->damping && (ids[0] >= ids[2] || ids[1] >= ids[3])) {
+        delete c;
+        return FCPT_ERR_ARG;  // inner and outer damping zones overlap
+    }
+
+    // per-ring metric arrays, init_radialarrays (init.cpp:188-225), in FP64 on the host
+    const int n1 = nr + 2;
+    std::vector<double> Rinf(n1, 0), Rsup(n1, 0), Rmed(n1, 0), Surf(n1, 0), InvSurf(n1, 0),
+        InvRmed(n1, 0), InvRinf(n1, 0), InvDiffRmed(n1, 0), InvDiffRsup(n1, 0),
+        InvDiffRsupRb(n1, 0), cosphi(ns), sinphi(ns);
+    for (int i = 0; i < nr + 1; ++i) {
+        Rinf[i] = desc->radii[i];
+        Rsup[i] = desc->radii[i + 1];
+        Rmed[i] = 2.0 / 3.0 * (std::pow(Rsup[i], 3) - std::pow(Rinf[i], 3));
+        Rmed[i] = Rmed[i] / (std::pow(Rsup[i], 2) - std::pow(Rinf[i], 2));
+        Surf[i] = M_PI * (std::pow(Rsup[i], 2) - std::pow(Rinf[i], 2)) / (double)ns;
+        InvRmed[i] = 1.0 / Rmed[i];
+        InvSurf[i] = 1.0 / Surf[i];
+        InvDiffRsup[i] = 1.0 / (Rsup[i] - Rinf[i]);
+        InvDiffRsupRb[i] = 1.0 / ((Rsup[i] - Rinf[i]) * Rmed[i]);
+        InvRinf[i] = 1.0 / Rinf[i];
+    }
+    for (int i = 1; i < nr + 1; ++i) InvDiffRmed[i] = 1.0 / (Rmed[i] - Rmed[i - 1]);
+    for (int j = 0; j < ns; ++j) {
+        // cell centres, SideEuler.cpp:57-66
+        cosphi[j] = std::cos(P.dphi * (double)j);
+        sinphi[j] = std::sin(P.dphi * (double)j);
+    }
+
+    int rc = FCPT_OK;
+    auto upload = [&](const double **dst, const std::vector<double> &src) -> int {
+        double *d = nullptr;
+        int r = dev_alloc(c, &d, src.size());
+        if (r) return r;
+        hipError_t e2 = hipMemcpy(d, src.data(), src.size() * sizeof(double), hipMemcpyHostToDevice);
+        if (e2 != hipSuccess) return fail(c, FCPT_ERR_DEVICE, "hipMemcpy: %s", hipGetErrorString(e2));
+        *dst = d;
+        return FCPT_OK;
+    };
+    if ((rc = upload(&P.Rinf, Rinf)) || (rc = upload(&P.Rsup, Rsup)) || (rc = upload(&P.Rmed, Rmed)) ||
+        (rc = upload(&P.Surf, Surf)) || (rc = upload(&P.InvSurf, InvSurf)) ||
+        (rc = upload(&P.InvRmed, InvRmed)) || (rc = upload(&P.InvRinf, InvRinf)) ||
+        (rc = upload(&P.InvDiffRmed, InvDiffRmed)) || (rc = upload(&P.InvDiffRsup, InvDiffRsup)) ||
+        (rc = upload(&P.InvDiffRsupRb, InvDiffRsupRb)) || (rc = upload(&P.cosphi, cosphi)) ||
+        (rc = upload(&P.sinphi, sinphi))) {
+        fcpt_destroy(c);
+        return rc;
+    }
+    const size_t ncell = (size_t)nr * ns;
+    for (int k = 0; k < FCPT_NUM_FIELDS && !rc; ++k) rc = dev_alloc(c, &P.f[k], field_rows(k, nr) * ns);
+    for (int k = 0; k < 4 && !rc; ++k) rc = dev_alloc(c, &P.mom[k], ncell);
+    for (int k = 0; k < 4 && !rc; ++k) rc = dev_alloc(c, &P.mom_b[k], ncell);
+    if (!rc) rc = dev_alloc(c, &P.sig_b, ncell);
+    if (!rc) rc = dev_alloc(c, &P.en_b, ncell);
+    if (!rc) rc = dev_alloc(c, &P.vmean, (size_t)nr + 1);
+    if (!rc) rc = dev_alloc(c, &P.vuni, (size_t)nr + 1);
+    if (!rc) rc = dev_alloc(c, &P.nshift, (size_t)nr + 1);
+    if (!rc) rc = dev_alloc(c, &P.ring_min, (size_t)nr + 1);
+    if (!rc) rc = dev_alloc(c, &c->s_dev, 1);
+    if (!rc) rc = dev_alloc(c, &c->totals_ring, 4 * (size_t)nr);
+    if (!rc) rc = dev_alloc(c, &c->totals_out, 2);
+    if (rc) {
+        fcpt_destroy(c);
+        return rc;
+    }
+    P.s = c->s_dev;
+    e = hipHostMalloc((void **)&c->s_host, sizeof(DevScalars), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&c->totals_host, 2 * sizeof(double), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        fcpt_destroy(c);
+        return FCPT_ERR_DEVICE;
+    }
+    // zero every field (the reference's t_polargrid::set_size clears its arrays)
+    for (int k = 0; k < FCPT_NUM_FIELDS; ++k)
+        e = e == hipSuccess ? hipMemset(P.f[k], 0, field_rows(k, nr) * ns * sizeof(double)) : e;
+    std::memset(c->s_host, 0, sizeof(DevScalars));
+    if (e == hipSuccess) e = hipMemcpy(c->s_dev, c->s_host, sizeof(DevScalars), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        fcpt_destroy(c);
+        return FCPT_ERR_DEVICE;
+    }
+    *out = c;
+    return FCPT_OK;
+}
+
+int fcpt_destroy(fcpt_ctx *ctx)
+{
+    if (!ctx) return FCPT_OK;
+    hipSetDevice(ctx->device);
+    if (ctx->stream) {
+        hipStreamSynchronize(ctx->stream);
+        hipStreamDestroy(ctx->stream);
+    }
+    for (void *p : ctx->allocs) hipFree(p);
+    if (ctx->s_host) hipHostFree(ctx->s_host);
+    if (ctx->totals_host) hipHostFree(ctx->totals_host);
+    delete ctx;
+    return FCPT_OK;
+}
+
+int fcpt_synchronize(fcpt_ctx *ctx)
+{
+    CTX_CHECK(ctx);
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return FCPT_OK;
+}
+
+// read back the device scalars; flags a non-finite CFL result
+static int read_scalars(fcpt_ctx *ctx)
+{
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->s_host, ctx->s_dev, sizeof(DevScalars), hipMemcpyDeviceToHost,
+                                ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return FCPT_OK;
+}
+
+int fcpt_upload_field(fcpt_ctx *ctx, int32_t field, const double *host)
+{
+    CTX_CHECK(ctx);
+    if (field < 0 || field >= FCPT_NUM_FIELDS || !host) return fail(ctx, FCPT_ERR_ARG, "bad field");
+    const size_t n = field_rows(field, ctx->P.nr) * ctx->P.ns;
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->P.f[field], host, n * sizeof(double), hipMemcpyHostToDevice,
+                                ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return FCPT_OK;
+}
+
+int fcpt_download_field(fcpt_ctx *ctx, int32_t field, double *host)
+{
+    CTX_CHECK(ctx);
+    if (field < 0 || field >= FCPT_NUM_FIELDS || !host) return fail(ctx, FCPT_ERR_ARG, "bad field");
+    const size_t n = field_rows(field, ctx->P.nr) * ctx->P.ns;
+    HIP_TRY(ctx, hipMemcpyAsync(host, ctx->P.f[field], n * sizeof(double), hipMemcpyDeviceToHost,
+                                ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return FCPT_OK;
+}
+
+int fcpt_upload_reference(fcpt_ctx *ctx, const double *sigma0, const double *vrad0,
+                          const double *vazi0, const double *energy0)
+{
+    int rc;
+    if ((rc = fcpt_upload_field(ctx, FCPT_SIGMA0, sigma0))) return rc;
+    if ((rc = fcpt_upload_field(ctx, FCPT_VRAD0, vrad0))) return rc;
+    if ((rc = fcpt_upload_field(ctx, FCPT_VAZI0, vazi0))) return rc;
+    if (energy0) return fcpt_upload_field(ctx, FCPT_ENERGY0, energy0);
+    return FCPT_OK;
+}
+
+int fcpt_upload_state(fcpt_ctx *ctx, const double *sigma, const double *vrad, const double *vazi,
+                      const double *energy)
+{
+    int rc;
+    if ((rc = fcpt_upload_field(ctx, FCPT_SIGMA, sigma))) return rc;
+    if ((rc = fcpt_upload_field(ctx, FCPT_VRAD, vrad))) return rc;
+    if ((rc = fcpt_upload_field(ctx, FCPT_VAZI, vazi))) return rc;
+    if (energy && (rc = fcpt_upload_field(ctx, FCPT_ENERGY, energy))) return rc;
+    if (ctx->P.adiabatic && !energy) return fail(ctx, FCPT_ERR_ARG, "adiabatic disk needs energy");
+    // init_euler recalculation (SourceEuler.cpp:264-284)
+    const KParams &P = ctx->P;
+    if (!P.adiabatic) {
+        launch_iso_profile(P, ctx->stream);
+        launch_recalc_viscosity(P, ctx->stream);
+    } else {
+        launch_derived(P, ctx->stream);
+        // compute_heating_cooling_for_CFL (SourceEuler.cpp:1410-1450)
+        launch_recalc_viscosity(P, ctx->stream);
+        launch_stress(P, ctx->stream);
+        launch_heatcool_cfl(P, ctx->stream);
+    }
+    LAUNCH_CHECK(ctx);
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return FCPT_OK;
+}
+
+int fcpt_download_state(fcpt_ctx *ctx, double *sigma, double *vrad, double *vazi, double *energy)
+{
+    int rc;
+    if (sigma && (rc = fcpt_download_field(ctx, FCPT_SIGMA, sigma))) return rc;
+    if (vrad && (rc = fcpt_download_field(ctx, FCPT_VRAD, vrad))) return rc;
+    if (vazi && (rc = fcpt_download_field(ctx, FCPT_VAZI, vazi))) return rc;
+    if (energy && (rc = fcpt_download_field(ctx, FCPT_ENERGY, energy))) return rc;
+    return FCPT_OK;
+}
+
+int fcpt_set_time(fcpt_ctx *ctx, double time, double last_dt, int32_t n_monitor)
+{
+    CTX_CHECK(ctx);
+    launch_set_time(ctx->P, ctx->stream, time, last_dt, n_monitor);
+    LAUNCH_CHECK(ctx);
+    return FCPT_OK;
+}
+
+int fcpt_get_time(fcpt_ctx *ctx, double *time, double *last_dt, int32_t *n_monitor)
+{
+    CTX_CHECK(ctx);
+    int rc = read_scalars(ctx);
+    if (rc) return rc;
+    if (time) *time = ctx->s_host->time;
+    if (last_dt) *last_dt = ctx->s_host->last_dt;
+    if (n_monitor) *n_monitor = ctx->s_host->n_monitor;
+    return FCPT_OK;
+}
+
+int fcpt_set_step_params(fcpt_ctx *ctx, const fcpt_step_params *params)
+{
+    CTX_CHECK(ctx);
+    if (!params || params->n_planets < 0 || params->n_planets > FCPT_MAX_PLANETS)
+        return fail(ctx, FCPT_ERR_ARG, "bad step parameters");
+    launch_set_params(ctx->P, ctx->stream, *params);
+    LAUNCH_CHECK(ctx);
+    return FCPT_OK;
+}
+
+static int cfl_common(fcpt_ctx *ctx, int mode, double *dt_out)
+{
+    CTX_CHECK(ctx);
+    launch_cfl(ctx->P, ctx->stream, mode);
+    LAUNCH_CHECK(ctx);
+    int rc = read_scalars(ctx);
+    if (rc) return rc;
+    if (ctx->s_host->error_flags & 1) {
+        // clear the flag for the next call
+        HIP_TRY(ctx, hipMemsetAsync(&ctx->s_dev->error_flags, 0, sizeof(int), ctx->stream));
+        return fail(ctx, FCPT_ERR_NONFINITE, "CFL time step is not finite and positive (%g)",
+                    ctx->s_host->cfl_dt);
+    }
+    if (dt_out) *dt_out = ctx->s_host->cfl_dt;
+    return FCPT_OK;
+}
+
+int fcpt_condition_cfl(fcpt_ctx *ctx, double *dt_out) { return cfl_common(ctx, 0, dt_out); }
+int fcpt_calculate_timestep(fcpt_ctx *ctx, double *dt_out) { return cfl_common(ctx, 1, dt_out); }
+
+int fcpt_nbody_potential(fcpt_ctx *ctx)
+{
+    CTX_CHECK(ctx);
+    launch_potential(ctx->P, ctx->stream);
+    LAUNCH_CHECK(ctx);
+    return FCPT_OK;
+}
+
+#define DT_ENTRY(name, body)                         \
+    int name(fcpt_ctx *ctx, double dt)               \
+    {                                                \
+        CTX_CHECK(ctx);                              \
+        launch_set_dt(ctx->P, ctx->stream, dt);      \
+        body;                                        \
+        LAUNCH_CHECK(ctx);                           \
+        return FCPT_OK;                              \
+    }
+
+DT_ENTRY(fcpt_update_with_sourceterms, launch_sources(ctx->P, ctx->stream))
+DT_ENTRY(fcpt_artificial_viscosity, launch_artificial_viscosity(ctx->P, ctx->stream))
+DT_ENTRY(fcpt_update_velocities_with_viscosity, launch_visc_update(ctx->P, ctx->stream))
+DT_ENTRY(fcpt_transport, launch_transport(ctx->P, ctx->stream))
+
+int fcpt_recalculate_viscosity(fcpt_ctx *ctx)
+{
+    CTX_CHECK(ctx);
+    launch_recalc_viscosity(ctx->P, ctx->stream);
+    LAUNCH_CHECK(ctx);
+    return FCPT_OK;
+}
+
+int fcpt_viscous_stress_tensor(fcpt_ctx *ctx)
+{
+    CTX_CHECK(ctx);
+    launch_stress(ctx->P, ctx->stream);
+    LAUNCH_CHECK(ctx);
+    return FCPT_OK;
+}
+
+int fcpt_substep3(fcpt_ctx *ctx, double time, double dt)
+{
+    CTX_CHECK(ctx);
+    if (!ctx->P.adiabatic) return FCPT_OK;
+    launch_set_clock(ctx->P, ctx->stream, time);
+    launch_set_dt(ctx->P, ctx->stream, dt);
+    launch_substep3(ctx->P, ctx->stream);
+    LAUNCH_CHECK(ctx);
+    return FCPT_OK;
+}
+
+int fcpt_apply_boundary_condition(fcpt_ctx *ctx, double dt, int32_t final_call)
+{
+    CTX_CHECK(ctx);
+    launch_set_dt(ctx->P, ctx->stream, dt);
+    launch_boundaries(ctx->P, ctx->stream, final_call);
+    LAUNCH_CHECK(ctx);
+    return FCPT_OK;
+}
+
+int fcpt_recalculate_derived(fcpt_ctx *ctx)
+{
+    CTX_CHECK(ctx);
+    launch_derived(ctx->P, ctx->stream);
+    LAUNCH_CHECK(ctx);
+    return FCPT_OK;
+}
+
+int fcpt_get_nshift(fcpt_ctx *ctx, int32_t *nshift)
+{
+    CTX_CHECK(ctx);
+    if (!nshift) return fail(ctx, FCPT_ERR_ARG, "null output");
+    HIP_TRY(ctx, hipMemcpyAsync(nshift, ctx->P.nshift, sizeof(int32_t) * ctx->P.nr,
+                                hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return FCPT_OK;
+}
+
+// the gas part of step_Euler (simulation.cpp:148-267); dt already in device memory
+static void enqueue_euler(const KParams &P, hipStream_t st, int count_monitor)
+{
+    launch_potential(P, st);
+    launch_sources(P, st);
+    launch_artificial_viscosity(P, st);
+    launch_recalc_viscosity(P, st);
+    launch_stress(P, st);
+    launch_visc_update(P, st);
+    if (P.adiabatic) launch_substep3(P, st);
+    launch_boundaries(P, st, 0);
+    launch_transport(P, st);
+    launch_advance_time(P, st, count_monitor);
+    launch_boundaries(P, st, 1);
+    launch_derived(P, st);
+}
+
+int fcpt_step_euler(fcpt_ctx *ctx, double dt)
+{
+    CTX_CHECK(ctx);
+    launch_set_dt(ctx->P, ctx->stream, dt);
+    enqueue_euler(ctx->P, ctx->stream, 0);
+    LAUNCH_CHECK(ctx);
+    return FCPT_OK;
+}
+
+int fcpt_run_step(fcpt_ctx *ctx, double *dt_out)
+{
+    CTX_CHECK(ctx);
+    const KParams &P = ctx->P;
+    launch_cfl(P, ctx->stream, 2);
+    enqueue_euler(P, ctx->stream, 1);
+    LAUNCH_CHECK(ctx);
+    int rc = read_scalars(ctx);
+    if (rc) return rc;
+    if (ctx->s_host->error_flags & 1)
+        return fail(ctx, FCPT_ERR_NONFINITE, "CFL time step is not finite and positive");
+    if (dt_out) *dt_out = ctx->s_host->dt;
+    return FCPT_OK;
+}
+
+int fcpt_totals(fcpt_ctx *ctx, double radius, double *mass, double *angular_momentum)
+{
+    CTX_CHECK(ctx);
+    launch_totals(ctx->P, ctx->stream, radius, ctx->totals_ring, ctx->totals_out);
+    LAUNCH_CHECK(ctx);
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->totals_host, ctx->totals_out, 2 * sizeof(double),
+                                hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    if (mass) *mass = ctx->totals_host[0];
+    if (angular_momentum) *angular_momentum = ctx->totals_host[1];
+    return FCPT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,80 @@
+// Device-side data structures shared by the gfx950 kernels and the host glue.
+//
+// One disk = one KParams block passed by value to every kernel: grid sizes,
+// physical scalars, pointers to the per-ring metric arrays and to every field
+// in HBM.  Quantities that change from step to step (dt, time, planets, the
+// CFL minimum) live in a DevScalars block in device memory so that a captured
+// graph of the step can be replayed without host involvement.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "../../include/fcpt.h"
+
+namespace fcpt {
+
+constexpr int kMaxPlanets = FCPT_MAX_PLANETS;
+
+struct DevPlanet {
+    double x, y, mass, r_cubic;
+};
+
+// Step-varying scalars, resident in device memory.
+struct DevScalars {
+    double dt;            // step size used by the current step's kernels
+    double time;          // simulation time at the start of the step
+    double last_dt;       // sim::last_dt (CFLmaxVar limiter)
+    double cfl_dt;        // result of the last CFL reduction
+    double dt_core_init;  // CFL seed from rings 0/1 (cfl.cpp:200-201)
+    unsigned long long dt_min_bits;  // atomic min over non-negative doubles
+    int n_monitor;
+    int error_flags;      // bit 0: non-finite or non-positive dt
+    int n_planets;
+    int pad_;
+    double indirect_x, indirect_y;
+    DevPlanet planet[kMaxPlanets];
+};
+
+struct KParams {
+    int nr, ns;
+    double dphi, invdphi;
+    double rmin, rmax;
+    // physics
+    int adiabatic;
+    int artvisc;
+    int artvisc_dissipation;
+    int heating_viscous;
+    int cooling_beta_enabled;
+    int cooling_beta_ref;
+    int bc_inner, bc_outer;
+    int damping;
+    int damp_ids[4];
+    double G, Rgas, sigma_sb, c_light, mstar, mu, gamma;
+    double aspect_ratio, flaring, alpha, thickness_smoothing;
+    double cfl, cfl_max_var, heating_cooling_cfl_limit;
+    double sigma0, sigma_floor, t_min, t_max;
+    double artvisc_factor, heating_viscous_factor;
+    double cooling_beta, cooling_beta_ramp_up, radial_viscosity_factor;
+    double damping_inner_limit, damping_outer_limit, damping_time_factor,
+        damping_time_radius_outer;
+    double omega_frame, monitor_timestep;
+    // per-ring metric arrays (length nr + 2)
+    const double *Rinf, *Rsup, *Rmed, *Surf, *InvSurf, *InvRmed, *InvRinf, *InvDiffRmed,
+        *InvDiffRsup, *InvDiffRsupRb;
+    // per-sector tables: cos / sin of dphi * j (cell-centre coordinates)
+    const double *cosphi, *sinphi;
+    // fields in HBM (indexed by fcpt_field)
+    double *f[FCPT_NUM_FIELDS];
+    // transport work arrays
+    double *mom[4];      // radial momentum +/-, angular momentum +/-
+    double *mom_b[4];    // ping-pong partners
+    double *sig_b, *en_b;
+    double *vmean;       // per-ring mean azimuthal velocity
+    double *vuni;        // per-ring uniform residual velocity (FARGO)
+    int *nshift;         // per-ring integer shift
+    double *ring_min;    // per-ring partial CFL minima
+    DevScalars *s;
+};
+
+}  // namespace fcpt

@@ -1,0 +1,1125 @@
+// gfx950 kernels of the FargoCPT hydro step (FP64).
+//
+// Every kernel restates one loop nest of the reference (file:line given above
+// it) with the same expression order; the library is compiled with
+// -ffp-contract=off so that add/mul/div/sqrt round exactly like the
+// reference's host build.  Threads map x -> azimuthal index j (contiguous in
+// HBM, so every wavefront reads 64 consecutive doubles) and y -> ring index i.
+#include "fcpt_device.hpp"
+#include "fcpt_kernels.hpp"
+
+namespace fcpt {
+
+#define IX(i, j) ((size_t)(i) * (size_t)ns + (size_t)(j))
+
+__device__ __forceinline__ double std_min(double a, double b) { return (b < a) ? b : a; }
+__device__ __forceinline__ double std_max(double a, double b) { return (a < b) ? b : a; }
+__device__ __forceinline__ double van_leer_lim(double a, double b)
+{
+    // TransportEuler.cpp:306-312
+    if (a * b > 0.0) return 2.0 * a * b / (a + b);
+    return 0;
+}
+__device__ __forceinline__ double omega_kepler(const KParams &P, double r)
+{
+    return sqrt(P.G * P.mstar / (r * r * r));  // Theo.cpp:246-249
+}
+
+// 2-D launch over rows [r0, r1) x all sectors
+#define CELL_LOOP_HEAD(r0, r1)                                   \
+    const int ns = P.ns;                                         \
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;         \
+    const int i = (r0) + blockIdx.y * blockDim.y + threadIdx.y;  \
+    if (j >= ns || i >= (r1)) return;
+
+// ---------------------------------------------------------------------------
+// ring sums (cfl.cpp:196-205, TransportEuler.cpp:174-190): one block per ring,
+// pairwise tree order
+template <int BS>
+__device__ double block_sum(double v, double *sh)
+{
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+#pragma unroll
+    for (int s = BS / 2; s > 0; s >>= 1) {
+        if (t < s) sh[t] = sh[t] + sh[t + s];
+        __syncthreads();
+    }
+    return sh[0];
+}
+
+// mode 0: CFL ring mean (also re-arms the CFL minimum)
+// mode 1: transport ring mean + FARGO shift + uniform residual
+//         (TransportEuler.cpp:207-236)
+__global__ void __launch_bounds__(256) k_ring_mean(KParams P, int mode)
+{
+    __shared__ double sh[256];
+    const int ns = P.ns;
+    const int i = blockIdx.x;
+    const double *v = P.f[FCPT_VAZI];
+    double acc = 0.0;
+    for (int j = threadIdx.x; j < ns; j += 256) acc += v[IX(i, j)];
+    const double sum = block_sum<256>(acc, sh);
+    if (threadIdx.x == 0) {
+        const double vm = sum / (double)ns;
+        P.vmean[i] = vm;
+        if (mode == 1) {
+            const double dt = P.s->dt;
+            const double invdt = 1.0 / dt;
+            const double Ntilde = vm * P.InvRmed[i] * dt * P.invdphi;
+            const double Nround = floor(Ntilde + 0.5);
+            P.nshift[i] = (int)Nround;
+            P.vuni[i] = (Ntilde - Nround) * P.Rmed[i] * invdt * P.dphi;
+        } else if (i == 0) {
+            P.s->dt_min_bits = 0x7FF0000000000000ull;  // +inf
+        }
+    }
+}
+
+// cfl.cpp:210-376, per cell dt and the per-ring shear limit; min reduced with
+// wavefront shuffles and one 64-bit atomic per wavefront (valid for doubles >= 0)
+__global__ void __launch_bounds__(256) k_cfl_cells(KParams P)
+{
+    const int ns = P.ns;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+    double cand = __longlong_as_double(0x7FF0000000000000ll);
+    if (j < ns && i < P.nr - 1) {
+        const double *vr = P.f[FCPT_VRAD], *vaz = P.f[FCPT_VAZI];
+        const double CFL = P.cfl, dphi = P.dphi;
+        const double *vm = P.vmean;
+        if (j == 0) {
+            const double denom =
+                fabs(vm[i] * P.InvRmed[i] - vm[i + 1] * P.InvRmed[i + 1]) + 1.0e-100;
+            cand = CFL * dphi / denom;
+            if (i == 1) {
+                const double denom0 =
+                    fabs(vm[0] * P.InvRmed[0] - vm[1] * P.InvRmed[1]) + 1.0e-100;
+                cand = std_min(cand, CFL * dphi / denom0);
+            }
+        }
+        const double dxRadial = P.Rsup[i] - P.Rinf[i];
+        const double dxAzimuthal = P.Rmed[i] * dphi;
+        const double cell_size = std_min(dxRadial, dxAzimuthal);
+        const double v_residual = vaz[IX(i, j)] - vm[i];
+        const double invdt1 = P.f[FCPT_SOUNDSPEED][IX(i, j)] / cell_size;
+        const double invdt2 = vr[IX(i, j)] / dxRadial;
+        const double invdt3 = v_residual / dxAzimuthal;
+        const int jn = j == ns - 1 ? 0 : j + 1;
+        const double fac2 = P.artvisc_factor * P.artvisc_factor;
+        double invdt4;
+        if (P.artvisc == FCPT_ARTVISC_SN) {
+            double dvRadial = vr[IX(i + 1, j)] - vr[IX(i, j)];
+            double dvAzimuthal = vaz[IX(i, jn)] - vaz[IX(i, j)];
+            if (dvRadial > 0.0) dvRadial = 0.0;
+            else dvRadial = -dvRadial;
+            if (dvAzimuthal > 0.0) dvAzimuthal = 0.0;
+            else dvAzimuthal = -dvAzimuthal;
+            invdt4 = 4.0 * fac2 * std_max(dvRadial / dxRadial, dvAzimuthal / dxAzimuthal) * 1.0;
+        } else {
+            const double eps_rr = (vr[IX(i + 1, j)] - vr[IX(i, j)]) * P.InvDiffRsup[i];
+            const double eps_pp = P.InvRmed[i] * ((vaz[IX(i, jn)] - vaz[IX(i, j)]) * P.invdphi +
+                                                  0.5 * (vr[IX(i + 1, j)] + vr[IX(i, j)]));
+            const double mdiv_V = -std_min(eps_rr + eps_pp, 0.0);
+            invdt4 = 4.0 * fac2 * mdiv_V * 1.0;
+        }
+        const double invdt5 = 4.0 * P.f[FCPT_VISCOSITY][IX(i, j)] / (cell_size * cell_size) * 1.0;
+        double invdt6 = 0.0;
+        if (P.adiabatic) {
+            const double inv_limit = 1.0 / P.heating_cooling_cfl_limit;
+            invdt6 = inv_limit *
+                     fabs((P.f[FCPT_QPLUS][IX(i, j)] - P.f[FCPT_QMINUS][IX(i, j)]) /
+                          P.f[FCPT_ENERGY][IX(i, j)]) *
+                     1.0;
+        }
+        const double dt_cell =
+            CFL / sqrt(invdt1 * invdt1 + invdt2 * invdt2 + invdt3 * invdt3 + invdt4 * invdt4 +
+                       invdt5 * invdt5 + invdt6 * invdt6);
+        // NaN never wins a comparison, exactly like the reference's "if (dt_cell < dt_core)"
+        cand = std_min(cand, dt_cell);
+    }
+    // wavefront min (64 lanes)
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(cand, off, 64);
+        cand = std_min(cand, o);
+    }
+    const int lane = (threadIdx.y * blockDim.x + threadIdx.x) & 63;
+    if (lane == 0) atomicMin(&P.s->dt_min_bits, (unsigned long long)__double_as_longlong(cand));
+}
+
+// mode 0: condition_cfl only; 1: sim::CalculateTimeStep (simulation.cpp:100-118);
+// 2: CalculateTimeStep + the monitor clip of sim::run (simulation.cpp:526-542)
+__global__ void k_cfl_finalize(KParams P, int mode)
+{
+    DevScalars *s = P.s;
+    const double cfl_dt = __longlong_as_double((long long)s->dt_min_bits);
+    s->cfl_dt = cfl_dt;
+    if (!(cfl_dt > 0.0) || !isfinite(cfl_dt)) s->error_flags |= 1;
+    if (mode >= 1) {
+        const double rv = std_min(P.cfl_max_var * s->last_dt, cfl_dt);
+        s->last_dt = rv;
+        s->cfl_dt = rv;
+        if (mode == 2) {
+            const double time_next_monitor = (s->n_monitor + 1) * P.monitor_timestep;
+            const double time_left_till_write = time_next_monitor - s->time;
+            const bool overshoot = rv > time_left_till_write;
+            const double dt_stretch_factor = 0.05;
+            const bool almost_there = time_left_till_write < rv * (1 + dt_stretch_factor);
+            s->dt = (overshoot || almost_there) ? time_left_till_write : rv;
+        }
+    }
+}
+
+// end of a sim::run iteration: time += dt and the monitor counter
+// (simulation.cpp:244, 545-551)
+__global__ void k_advance_time(KParams P, int count_monitor)
+{
+    DevScalars *s = P.s;
+    const double time_next_monitor = (s->n_monitor + 1) * P.monitor_timestep;
+    s->time = s->time + s->dt;
+    if (count_monitor && fabs(time_next_monitor - s->time) < 1e-6 * s->last_dt) s->n_monitor++;
+}
+
+__global__ void k_set_dt(KParams P, double dt) { P.s->dt = dt; }
+__global__ void k_set_clock(KParams P, double time) { P.s->time = time; }
+
+__global__ void k_set_time(KParams P, double time, double last_dt, int n_monitor)
+{
+    P.s->time = time;
+    P.s->last_dt = last_dt;
+    P.s->n_monitor = n_monitor;
+}
+
+__global__ void k_set_params(KParams P, fcpt_step_params sp)
+{
+    DevScalars *s = P.s;
+    s->n_planets = sp.n_planets;
+    for (int k = 0; k < sp.n_planets; ++k) {
+        s->planet[k].x = sp.planet[k].x;
+        s->planet[k].y = sp.planet[k].y;
+        s->planet[k].mass = sp.planet[k].mass;
+        s->planet[k].r_cubic = sp.planet[k].r_cubic;
+    }
+    s->indirect_x = sp.indirect_x;
+    s->indirect_y = sp.indirect_y;
+}
+
+// ---------------------------------------------------------------------------
+// CalculateNbodyPotential, Pframeforce.cpp:44-85 (+ compute_smoothing, Force.cpp:145-159)
+__global__ void __launch_bounds__(256) k_potential(KParams P)
+{
+    CELL_LOOP_HEAD(0, P.nr)
+    const DevScalars *s = P.s;
+    const double x = P.Rmed[i] * P.cosphi[j];
+    const double y = P.Rmed[i] * P.sinphi[j];
+    const double H = P.f[FCPT_SCALE_HEIGHT][IX(i, j)];
+    double acc = 0.0;
+    const int np = s->n_planets;
+    for (int k = 0; k < np; ++k) {
+        const double smooth = P.thickness_smoothing * H;
+        const double dx = x - s->planet[k].x;
+        const double dy = y - s->planet[k].y;
+        const double dist_2 = dx * dx + dy * dy;
+        const double d_smoothed = sqrt(dist_2 + smooth * smooth);
+        double smooth_factor_klahr = 1.0;
+        const double r_sm = s->planet[k].r_cubic;
+        if (r_sm > 0.0 && d_smoothed < r_sm) {
+            const double q = d_smoothed / r_sm;
+            smooth_factor_klahr = (pow(q, 4.0) - 2.0 * pow(q, 3.0) + 2.0 * d_smoothed / r_sm);
+        }
+        acc += -P.G * s->planet[k].mass / d_smoothed * smooth_factor_klahr;
+    }
+    acc += -s->indirect_x * x - s->indirect_y * y;
+    P.f[FCPT_POTENTIAL][IX(i, j)] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// momentum_update_radial, SourceEuler.cpp:325-372
+__global__ void __launch_bounds__(256) k_src_vr(KParams P)
+{
+    CELL_LOOP_HEAD(2, P.nr - 1)
+    const double *S = P.f[FCPT_SIGMA], *Pr = P.f[FCPT_PRESSURE], *Phi = P.f[FCPT_POTENTIAL],
+                 *vaz = P.f[FCPT_VAZI];
+    double *vr = P.f[FCPT_VRAD];
+    const double dt = P.s->dt;
+    double gradp = 2.0 / (S[IX(i, j)] + S[IX(i - 1, j)]);
+    gradp *= (Pr[IX(i, j)] - Pr[IX(i - 1, j)]);
+    gradp *= P.InvDiffRmed[i];
+    const double gradphi = (Phi[IX(i, j)] - Phi[IX(i - 1, j)]) * P.InvDiffRmed[i];
+    const int jn = j == ns - 1 ? 0 : j + 1;
+    const double vsum = vaz[IX(i, j)] + vaz[IX(i, jn)] + vaz[IX(i - 1, j)] + vaz[IX(i - 1, jn)];
+    const double vt = 0.25 * vsum + P.Rinf[i] * P.omega_frame;
+    const double vt2 = vt * vt;
+    const double centrifugal_accel = vt2 * P.InvRinf[i];
+    vr[IX(i, j)] += dt * (-gradp - gradphi + centrifugal_accel);
+}
+
+// momentum_update_azimuthal, SourceEuler.cpp:375-428
+__global__ void __launch_bounds__(256) k_src_vphi(KParams P)
+{
+    CELL_LOOP_HEAD(1, P.nr - 1)
+    const double *S = P.f[FCPT_SIGMA], *Pr = P.f[FCPT_PRESSURE], *Phi = P.f[FCPT_POTENTIAL];
+    double *vaz = P.f[FCPT_VAZI];
+    const double dt = P.s->dt;
+    const double invdxtheta = 2.0 / (P.dphi * (P.Rsup[i] + P.Rinf[i]));
+    const int jp = j == 0 ? ns - 1 : j - 1;
+    const double gradp =
+        2.0 / (S[IX(i, j)] + S[IX(i, jp)]) * (Pr[IX(i, j)] - Pr[IX(i, jp)]) * invdxtheta;
+    const double gradphi = (Phi[IX(i, j)] - Phi[IX(i, jp)]) * invdxtheta;
+    vaz[IX(i, j)] = vaz[IX(i, j)] + dt * (-gradp - gradphi);
+}
+
+// compression_heating, SourceEuler.cpp:459-493
+__global__ void __launch_bounds__(256) k_src_energy(KParams P)
+{
+    CELL_LOOP_HEAD(0, P.nr - 1)
+    const double *vr = P.f[FCPT_VRAD], *vaz = P.f[FCPT_VAZI];
+    double *E = P.f[FCPT_ENERGY];
+    const double dt = P.s->dt;
+    const int jn = j == ns - 1 ? 0 : j + 1;
+    const double DIV_V =
+        (vr[IX(i + 1, j)] * P.Rinf[i + 1] - vr[IX(i, j)] * P.Rinf[i]) * P.InvDiffRsupRb[i] +
+        (vaz[IX(i, jn)] - vaz[IX(i, j)]) * P.invdphi * P.InvRmed[i];
+    const double energy_old = E[IX(i, j)];
+    E[IX(i, j)] = energy_old * exp(-(P.gamma - 1.0) * dt * DIV_V);
+}
+
+// assure_temperature_range for one cell, SourceEuler.cpp:136-202
+__device__ __forceinline__ double temperature_floor(const KParams &P, double e, double sigma)
+{
+    const double minimum_energy = P.t_min * sigma / P.mu * P.Rgas / (P.gamma - 1.0);
+    const double maximum_energy = P.t_max * sigma / P.mu * P.Rgas / (P.gamma - 1.0);
+    if (!(e > minimum_energy)) e = P.t_min * sigma / P.mu * P.Rgas / (P.gamma - 1.0);
+    if (!(e < maximum_energy)) e = P.t_max * sigma / P.mu * P.Rgas / (P.gamma - 1.0);
+    return e;
+}
+
+__global__ void __launch_bounds__(256) k_temperature_floor(KParams P)
+{
+    CELL_LOOP_HEAD(0, P.nr)
+    double *E = P.f[FCPT_ENERGY];
+    E[IX(i, j)] = temperature_floor(P, E[IX(i, j)], P.f[FCPT_SIGMA][IX(i, j)]);
+}
+
+// ---------------------------------------------------------------------------
+// update_with_artificial_viscosity_TW, artificial_viscosity.cpp:35-88
+__global__ void __launch_bounds__(256) k_tw_q(KParams P)
+{
+    CELL_LOOP_HEAD(0, P.nr)
+    const double *vr = P.f[FCPT_VRAD], *vaz = P.f[FCPT_VAZI], *S = P.f[FCPT_SIGMA];
+    double *E = P.f[FCPT_ENERGY];
+    const double dt = P.s->dt;
+    const int jn = j == ns - 1 ? 0 : j + 1;
+    const double eps_rr = (vr[IX(i + 1, j)] - vr[IX(i, j)]) * P.InvDiffRsup[i];
+    const double eps_pp = P.InvRmed[i] * ((vaz[IX(i, jn)] - vaz[IX(i, j)]) * P.invdphi +
+                                          0.5 * (vr[IX(i + 1, j)] + vr[IX(i, j)]));
+    const double div_V = std_min(eps_rr + eps_pp, 0.0);
+    const double Dr = P.Rinf[i + 1] - P.Rinf[i];
+    const double rDphi = P.Rmed[i] * P.dphi;
+    const double dxm = (ns <= 16) ? std_min(Dr, rDphi) : std_max(Dr, rDphi);
+    const double dx_sq = dxm * dxm;
+    const double l_sq = (P.artvisc_factor * P.artvisc_factor) * dx_sq;
+    const double sig = S[IX(i, j)];
+    P.f[FCPT_Q_R][IX(i, j)] = l_sq * sig * -div_V * (eps_rr - 1.0 / 3.0 * div_V);
+    P.f[FCPT_Q_PHI][IX(i, j)] = l_sq * sig * -div_V * (eps_pp - 1.0 / 3.0 * div_V);
+    if (P.adiabatic && P.artvisc_dissipation) {
+        double e = E[IX(i, j)];
+        if (i > 1 && i < P.nr - 1) {
+            const double d = eps_rr - eps_pp;
+            const double Qplus = -l_sq * div_V * sig * 1.0 / 3.0 *
+                                 (eps_rr * eps_rr + eps_pp * eps_pp + d * d);
+            e += Qplus * dt;
+        }
+        // SetTemperatureFloorCeilValues after the art. viscosity (artificial_viscosity.cpp:19-21)
+        E[IX(i, j)] = temperature_floor(P, e, sig);
+    }
+}
+
+// artificial_viscosity.cpp:90-139
+__global__ void __launch_bounds__(256) k_tw_vel(KParams P)
+{
+    CELL_LOOP_HEAD(1, P.nr - 1)
+    const double *S = P.f[FCPT_SIGMA], *Qr = P.f[FCPT_Q_R], *Qp = P.f[FCPT_Q_PHI];
+    const double dt = P.s->dt;
+    const int jp = j == 0 ? ns - 1 : j - 1;
+    const double sigma_phi_avg = 0.5 * (S[IX(i, j)] + S[IX(i, jp)]);
+    const double dVp = 2.0 * dt / ((P.Rsup[i] + P.Rinf[i]) * sigma_phi_avg) *
+                       (Qp[IX(i, j)] - Qp[IX(i, jp)]) * P.invdphi;
+    P.f[FCPT_VAZI][IX(i, j)] += dVp;
+    if (i >= 2) {
+        const double sigma_r_avg = 0.5 * (S[IX(i, j)] + S[IX(i - 1, j)]);
+        const double r0 = P.Rmed[i], r1 = P.Rmed[i - 1];
+        const double dVr = P.radial_viscosity_factor * dt / sigma_r_avg * 2.0 / (r0 * r0 - r1 * r1) *
+                           ((Qr[IX(i, j)] * r0 - Qr[IX(i - 1, j)] * r1) -
+                            0.5 * (Qp[IX(i, j)] + Qp[IX(i - 1, j)]) * (r0 - r1));
+        P.f[FCPT_VRAD][IX(i, j)] += dVr;
+    }
+}
+
+// update_with_artificial_viscosity_SN, artificial_viscosity.cpp:148-211
+__global__ void __launch_bounds__(256) k_sn_q(KParams P)
+{
+    CELL_LOOP_HEAD(0, P.nr)
+    const double *vr = P.f[FCPT_VRAD], *vaz = P.f[FCPT_VAZI], *S = P.f[FCPT_SIGMA];
+    const double fac2 = P.artvisc_factor * P.artvisc_factor;
+    const double sig = S[IX(i, j)];
+    const double dv_r = vr[IX(i + 1, j)] - vr[IX(i, j)];
+    const double qr = dv_r < 0.0 ? fac2 * sig * (dv_r * dv_r) : 0.0;
+    const int jn = j == ns - 1 ? 0 : j + 1;
+    const double dv_phi = vaz[IX(i, jn)] - vaz[IX(i, j)];
+    const double qp = dv_phi < 0.0 ? fac2 * sig * (dv_phi * dv_phi) : 0.0;
+    P.f[FCPT_Q_R][IX(i, j)] = qr;
+    P.f[FCPT_Q_PHI][IX(i, j)] = qp;
+    if (P.adiabatic && P.artvisc_dissipation) {
+        double *E = P.f[FCPT_ENERGY];
+        double e = E[IX(i, j)];
+        if (i >= 1 && i < P.nr - 1) {
+            const double dt = P.s->dt;
+            const double dxtheta = P.dphi * P.Rmed[i];
+            const double invdxtheta = 1.0 / dxtheta;
+            e = e - dt * qr * dv_r * P.InvDiffRsup[i] - dt * qp * dv_phi * invdxtheta;
+        }
+        E[IX(i, j)] = temperature_floor(P, e, sig);
+    }
+}
+
+// artificial_viscosity.cpp:214-248
+__global__ void __launch_bounds__(256) k_sn_vel(KParams P)
+{
+    CELL_LOOP_HEAD(1, P.nr - 1)
+    const double *S = P.f[FCPT_SIGMA], *Qr = P.f[FCPT_Q_R], *Qp = P.f[FCPT_Q_PHI];
+    const double dt = P.s->dt;
+    if (i >= 2) {
+        double *vr = P.f[FCPT_VRAD];
+        vr[IX(i, j)] = vr[IX(i, j)] - dt * 2.0 / (S[IX(i, j)] + S[IX(i - 1, j)]) *
+                                          (Qr[IX(i, j)] - Qr[IX(i - 1, j)]) * P.InvDiffRmed[i];
+    }
+    const double dxtheta = P.dphi * P.Rmed[i];
+    const double invdxtheta = 1.0 / dxtheta;
+    const int jp = j == 0 ? ns - 1 : j - 1;
+    double *vaz = P.f[FCPT_VAZI];
+    vaz[IX(i, j)] = vaz[IX(i, j)] - dt * 2.0 / (S[IX(i, j)] + S[IX(i, jp)]) *
+                                        (Qp[IX(i, j)] - Qp[IX(i, jp)]) * invdxtheta;
+}
+
+// ---------------------------------------------------------------------------
+// compute_sound_speed / compute_scale_height / update_viscosity for one cell
+// (SourceEuler.cpp:957-995, 1121-1154; viscosity.cpp:98-137)
+__device__ __forceinline__ void derived_cell(const KParams &P, int i, size_t c, bool with_cs_h,
+                                            bool with_temperature, bool with_pressure)
+{
+    const int ns = P.ns;
+    (void)ns;
+    double *cs = P.f[FCPT_SOUNDSPEED], *H = P.f[FCPT_SCALE_HEIGHT];
+    if (P.adiabatic) {
+        const double e = P.f[FCPT_ENERGY][c];
+        const double sig = P.f[FCPT_SIGMA][c];
+        if (with_temperature) {
+            const double c_v_inv = P.mu / P.Rgas * (P.gamma - 1.0);
+            P.f[FCPT_TEMPERATURE][c] = c_v_inv * e / sig;
+        }
+        if (with_cs_h) {
+            const double g = P.gamma;
+            const double c_s = sqrt(g * (g - 1.0) * e / sig);
+            cs[c] = c_s;
+            const double inv_omega_kepler = 1.0 / omega_kepler(P, P.Rmed[i]);
+            H[c] = c_s / (sqrt(g)) * inv_omega_kepler;
+        }
+        if (with_pressure) P.f[FCPT_PRESSURE][c] = (P.gamma - 1.0) * e;
+    } else if (with_pressure) {
+        const double c_s = cs[c];
+        P.f[FCPT_PRESSURE][c] = P.f[FCPT_SIGMA][c] * (c_s * c_s);
+    }
+    const double alpha = P.alpha;
+    P.f[FCPT_VISCOSITY][c] = alpha * H[c] * cs[c];
+}
+
+// recalculate_viscosity, SourceEuler.cpp:205-223
+__global__ void __launch_bounds__(256) k_recalc_viscosity(KParams P)
+{
+    CELL_LOOP_HEAD(0, P.nr)
+    derived_cell(P, i, IX(i, j), true, false, false);
+}
+
+// recalculate_derived_disk_quantities, SourceEuler.cpp:225-249
+__global__ void __launch_bounds__(256) k_derived(KParams P)
+{
+    CELL_LOOP_HEAD(0, P.nr)
+    derived_cell(P, i, IX(i, j), true, true, true);
+}
+
+// locally isothermal sound speed and scale height from the initial setup
+// (compute_sound_speed_normal / compute_scale_height_old, isothermal branches)
+__global__ void __launch_bounds__(256) k_iso_profile(KParams P)
+{
+    CELL_LOOP_HEAD(0, P.nr)
+    const double r = P.Rmed[i];
+    const double vK = sqrt(P.G * P.mstar / r);
+    const double h = P.aspect_ratio * pow(r, P.flaring);
+    const double c_s = h * vK;
+    P.f[FCPT_SOUNDSPEED][IX(i, j)] = c_s;
+    const double inv_omega_kepler = 1.0 / omega_kepler(P, r);
+    P.f[FCPT_SCALE_HEIGHT][IX(i, j)] = c_s * inv_omega_kepler;
+    const double sig = P.f[FCPT_SIGMA][IX(i, j)];
+    const double pr = sig * (c_s * c_s);
+    P.f[FCPT_PRESSURE][IX(i, j)] = pr;
+    P.f[FCPT_TEMPERATURE][IX(i, j)] = P.mu / P.Rgas * pr / sig;
+}
+
+// ---------------------------------------------------------------------------
+// compute_viscous_stress_tensor, viscosity.cpp:139-254
+__global__ void __launch_bounds__(256) k_stress(KParams P)
+{
+    CELL_LOOP_HEAD(0, P.nr)
+    const double *vr = P.f[FCPT_VRAD], *vaz = P.f[FCPT_VAZI], *nu = P.f[FCPT_VISCOSITY],
+                 *S = P.f[FCPT_SIGMA];
+    const int jn = j == ns - 1 ? 0 : j + 1;
+    const double divv =
+        (vr[IX(i + 1, j)] * P.Rinf[i + 1] - vr[IX(i, j)] * P.Rinf[i]) * P.InvDiffRsupRb[i] +
+        (vaz[IX(i, jn)] - vaz[IX(i, j)]) * P.invdphi * P.InvRmed[i];
+    P.f[FCPT_DIV_V][IX(i, j)] = divv;
+    const double n = nu[IX(i, j)], s = S[IX(i, j)];
+    const double drr = (vr[IX(i + 1, j)] - vr[IX(i, j)]) * P.InvDiffRsup[i];
+    P.f[FCPT_TAU_R_R][IX(i, j)] = 2.0 * n * s * (drr - 1.0 / 3.0 * divv);
+    const double dpp = (vaz[IX(i, jn)] - vaz[IX(i, j)]) * P.invdphi * P.InvRmed[i] +
+                       0.5 * (vr[IX(i + 1, j)] + vr[IX(i, j)]) * P.InvRmed[i];
+    P.f[FCPT_TAU_PHI_PHI][IX(i, j)] = 2.0 * n * s * (dpp - 1.0 / 3.0 * divv);
+    if (i >= 1) {
+        const int jp = j == 0 ? ns - 1 : j - 1;
+        const double dvazirdr =
+            (vaz[IX(i, j)] * P.InvRmed[i] - vaz[IX(i - 1, j)] * P.InvRmed[i - 1]) * P.InvDiffRmed[i];
+        const double dvrdphi = (vr[IX(i, j)] - vr[IX(i, jp)]) * P.invdphi;
+        const double drp = P.Rinf[i] * dvazirdr + dvrdphi * P.InvRinf[i];
+        const double n4 = 0.25 * (nu[IX(i, j)] + nu[IX(i - 1, j)] + nu[IX(i, jp)] + nu[IX(i - 1, jp)]);
+        const double s4 = 0.25 * (S[IX(i, j)] + S[IX(i - 1, j)] + S[IX(i, jp)] + S[IX(i - 1, jp)]);
+        P.f[FCPT_TAU_R_PHI][IX(i, j)] = n4 * s4 * drp;
+    }
+}
+
+// update_velocities_with_viscosity, viscosity.cpp:355-426
+__global__ void __launch_bounds__(256) k_visc_update(KParams P)
+{
+    CELL_LOOP_HEAD(1, P.nr - 1)
+    const double *S = P.f[FCPT_SIGMA], *trr = P.f[FCPT_TAU_R_R], *tpp = P.f[FCPT_TAU_PHI_PHI],
+                 *trp = P.f[FCPT_TAU_R_PHI];
+    const double dt = P.s->dt;
+    const int jp = j == 0 ? ns - 1 : j - 1;
+    {
+        const double sigma_avg = 0.5 * (S[IX(i, j)] + S[IX(i, jp)]);
+        const double a1 = P.Rinf[i + 1] * P.Rinf[i + 1], a0 = P.Rinf[i] * P.Rinf[i];
+        const double dVp = dt * P.InvRmed[i] / (sigma_avg) *
+                           ((2.0 / (a1 - a0)) * (a1 * trp[IX(i + 1, j)] - a0 * trp[IX(i, j)]) +
+                            (tpp[IX(i, j)] - tpp[IX(i, jp)]) * P.invdphi);
+        P.f[FCPT_VAZI][IX(i, j)] += dVp;
+    }
+    if (i >= 2) {
+        const int jn = j == ns - 1 ? 0 : j + 1;
+        const double sigma_avg = 0.5 * (S[IX(i, j)] + S[IX(i - 1, j)]);
+        const double dVr = dt / (sigma_avg)*P.radial_viscosity_factor * 2.0 /
+                           (P.Rmed[i] + P.Rmed[i - 1]) *
+                           ((P.Rmed[i] * trr[IX(i, j)] - P.Rmed[i - 1] * trr[IX(i - 1, j)]) *
+                                P.InvDiffRmed[i] +
+                            (trp[IX(i, jn)] - trp[IX(i, j)]) * P.invdphi -
+                            0.5 * (tpp[IX(i, j)] + tpp[IX(i - 1, j)]));
+        P.f[FCPT_VRAD][IX(i, j)] += dVr;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Q- (thermal_relaxation, SourceEuler.cpp:632-690) and Q+ (viscous_heating,
+// SourceEuler.cpp:496-536) for one active cell
+__device__ __forceinline__ void heating_cooling_cell(const KParams &P, int i, int j, double time,
+                                                     double e, double &qplus, double &qminus)
+{
+    const int ns = P.ns;
+    const double sig = P.f[FCPT_SIGMA][IX(i, j)];
+    qminus = 0.0;
+    if (P.cooling_beta_enabled) {
+        const double r = P.Rmed[i];
+        const double omega_k = omega_kepler(P, r);
+        double beta_inv = 1 / P.cooling_beta;
+        if (P.cooling_beta_ramp_up > 0.0) {
+            const double x = 2 * time / P.cooling_beta_ramp_up;
+            const double ramp_factor = 1 - exp(-(x * x));
+            beta_inv = beta_inv * ramp_factor;
+        }
+        double delta_E = e;
+        if (P.cooling_beta_ref == FCPT_BETA_REF_REFERENCE)
+            delta_E -= P.f[FCPT_ENERGY0][IX(i, j)] / P.f[FCPT_SIGMA0][IX(i, j)] * sig;
+        if (P.cooling_beta_ref == FCPT_BETA_REF_FLOOR)
+            delta_E -= P.t_min * sig / P.mu * P.Rgas / (P.gamma - 1.0);
+        qminus = 0.0 + delta_E * omega_k * beta_inv;
+    }
+    qplus = 0.0;
+    if (P.heating_viscous) {
+        const double nu = P.f[FCPT_VISCOSITY][IX(i, j)];
+        if (nu != 0.0) {
+            const double *trp = P.f[FCPT_TAU_R_PHI];
+            const int jn = j == ns - 1 ? 0 : j + 1;
+            const double tau_r_phi =
+                0.25 * (trp[IX(i, j)] + trp[IX(i + 1, j)] + trp[IX(i, jn)] + trp[IX(i + 1, jn)]);
+            const double trr = P.f[FCPT_TAU_R_R][IX(i, j)], tpp = P.f[FCPT_TAU_PHI_PHI][IX(i, j)];
+            const double dv = P.f[FCPT_DIV_V][IX(i, j)];
+            double q = 1.0 / (2.0 * nu * sig) *
+                       (trr * trr + 2 * (tau_r_phi * tau_r_phi) + tpp * tpp);
+            q += (2.0 / 9.0) * nu * sig * (dv * dv);
+            q *= P.heating_viscous_factor;
+            qplus = 0.0 + q;
+        }
+    }
+}
+
+__device__ __forceinline__ double rad_alpha(const KParams &P, double h, double sig, double e)
+{
+    const double inv_pow4 = pow(P.mu * (P.gamma - 1.0) / (P.Rgas * sig), 4.0);
+    return 1.0 + 2.0 * h * 4.0 * P.sigma_sb / P.c_light * inv_pow4 * pow(e, 3.0);
+}
+
+// SubStep3, SourceEuler.cpp:859-954
+__global__ void __launch_bounds__(256) k_substep3(KParams P)
+{
+    CELL_LOOP_HEAD(0, P.nr)
+    const size_t c = IX(i, j);
+    double *E = P.f[FCPT_ENERGY];
+    const double sig = P.f[FCPT_SIGMA][c];
+    double e = E[c];
+    // compute_temperature (SourceEuler.cpp:1378-1408) before the update
+    const double c_v_inv = P.mu / P.Rgas * (P.gamma - 1.0);
+    P.f[FCPT_TEMPERATURE][c] = c_v_inv * e / sig;
+    double qplus = 0.0, qminus = 0.0;
+    if (i >= 1 && i < P.nr - 1) {
+        const DevScalars *s = P.s;
+        heating_cooling_cell(P, i, j, s->time, e, qplus, qminus);
+        const double alpha = rad_alpha(P, P.f[FCPT_SCALE_HEIGHT][c], sig, e);
+        qplus /= alpha;
+        qminus /= alpha;
+        double energy_new = e + s->dt * (qplus - qminus);
+        const double SigmaFloor = 10.0 * P.sigma0 * P.sigma_floor;
+        if ((sig < SigmaFloor)) {
+            // TAU_EFF is never computed without surface cooling: it holds 0
+            const double tau_eff = 0.0;
+            const double e4 = qplus * tau_eff / (2.0 * P.sigma_sb);
+            const double constant = (P.Rgas / P.mu * sig / (P.gamma - 1.0));
+            energy_new = pow(e4, 1.0 / 4.0) * constant;
+            qminus = qplus;
+        }
+        e = energy_new;
+    }
+    P.f[FCPT_QPLUS][c] = qplus;
+    P.f[FCPT_QMINUS][c] = qminus;
+    E[c] = temperature_floor(P, e, sig);
+}
+
+// compute_heating_cooling_for_CFL, SourceEuler.cpp:1410-1450 (after update_viscosity
+// and the stress tensor)
+__global__ void __launch_bounds__(256) k_heatcool_cfl(KParams P)
+{
+    CELL_LOOP_HEAD(0, P.nr)
+    const size_t c = IX(i, j);
+    double qplus = 0.0, qminus = 0.0;
+    if (i >= 1 && i < P.nr - 1) {
+        const double e = P.f[FCPT_ENERGY][c];
+        heating_cooling_cell(P, i, j, P.s->time, e, qplus, qminus);
+        const double alpha = rad_alpha(P, P.f[FCPT_SCALE_HEIGHT][c], P.f[FCPT_SIGMA][c], e);
+        qplus /= alpha;
+        qminus /= alpha;
+    }
+    P.f[FCPT_QPLUS][c] = qplus;
+    P.f[FCPT_QMINUS][c] = qminus;
+}
+
+// ---------------------------------------------------------------------------
+// damping (boundary_conditions/damping.cpp:311-427, 754-776): block row y maps
+// to an inner-zone ring (y < n_inner) or an outer-zone ring
+__global__ void __launch_bounds__(256) k_damping(KParams P, int n_inner)
+{
+    const int ns = P.ns;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (j >= ns) return;
+    const double dt = P.s->dt;
+    const double lin = P.damping_inner_limit, lout = P.damping_outer_limit;
+    const double RMIN = P.rmin, RMAX = P.rmax;
+    int i;
+    bool inner;
+    if (y < n_inner) {
+        i = y;
+        inner = true;
+    } else {
+        // outer zone starts at the smaller of the scalar / vector limits
+        const int o0 = P.damp_ids[2] < P.damp_ids[3] ? P.damp_ids[2] : P.damp_ids[3];
+        i = o0 + (y - n_inner);
+        if (i > P.nr) return;
+        inner = false;
+    }
+    const double *F0[4] = {P.f[FCPT_VRAD0], P.f[FCPT_VAZI0], P.f[FCPT_SIGMA0], P.f[FCPT_ENERGY0]};
+    double *F[4] = {P.f[FCPT_VRAD], P.f[FCPT_VAZI], P.f[FCPT_SIGMA], P.f[FCPT_ENERGY]};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const bool vector = q == 0;
+        const double *radius = vector ? P.Rinf : P.Rmed;
+        const int size_radial = vector ? P.nr + 1 : P.nr;
+        if (i >= size_radial) continue;
+        if (inner) {
+            if (!((lin > 1.0) && (radius[0] < RMIN * lin))) continue;
+            const int limit = vector ? P.damp_ids[1] : P.damp_ids[0];
+            if (i > limit) continue;
+            const double tau = P.damping_time_factor * 2.0 * M_PI / omega_kepler(P, RMIN);
+            const double a = (radius[i] - RMIN * lin) / (RMIN - RMIN * lin);
+            const double exp_factor = exp(-dt * (a * a) / tau);
+            const double x = F[q][IX(i, j)], x0 = F0[q][IX(i, j)];
+            F[q][IX(i, j)] = (x - x0) * exp_factor + x0;
+        } else {
+            if (!((lout < 1.0) && (radius[size_radial - 1] > RMAX * lout))) continue;
+            const int limit = vector ? P.damp_ids[3] : P.damp_ids[2];
+            if (i < limit) continue;
+            const double tau =
+                P.damping_time_factor * 2.0 * M_PI / omega_kepler(P, P.damping_time_radius_outer);
+            const double a = (radius[i] - RMAX * lout) / (RMAX - RMAX * lout);
+            const double exp_factor = exp(-dt * (a * a) / tau);
+            const double x = F[q][IX(i, j)], x0 = F0[q][IX(i, j)];
+            F[q][IX(i, j)] = (x - x0) * exp_factor + x0;
+        }
+    }
+}
+
+// ghost rings: zero-gradient Sigma/E, reflecting / zero-gradient / outflow v_r,
+// Keplerian v_phi (boundary_conditions.cpp:67-97 and the BC files it dispatches to)
+__global__ void __launch_bounds__(256) k_ghost_bc(KParams P)
+{
+    const int ns = P.ns;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= ns) return;
+    const int nr = P.nr;
+    double *S = P.f[FCPT_SIGMA], *E = P.f[FCPT_ENERGY], *vr = P.f[FCPT_VRAD],
+           *vaz = P.f[FCPT_VAZI];
+    S[IX(0, j)] = S[IX(1, j)];
+    S[IX(nr - 1, j)] = S[IX(nr - 2, j)];
+    E[IX(0, j)] = E[IX(1, j)];
+    E[IX(nr - 1, j)] = E[IX(nr - 2, j)];
+    if (P.bc_inner == FCPT_BC_REFLECTING) {
+        vr[IX(0, j)] = -vr[IX(2, j)];
+        vr[IX(1, j)] = 0;
+    } else if (P.bc_inner == FCPT_BC_ZEROGRADIENT) {
+        vr[IX(0, j)] = vr[IX(2, j)];
+        vr[IX(1, j)] = vr[IX(2, j)];
+    } else {
+        if (vr[IX(2, j)] > 0.0) {
+            vr[IX(1, j)] = 0.0;
+            vr[IX(0, j)] = 0.0;
+        } else {
+            vr[IX(1, j)] = vr[IX(2, j)];
+            vr[IX(0, j)] = vr[IX(2, j)];
+        }
+    }
+    if (P.bc_outer == FCPT_BC_REFLECTING) {
+        vr[IX(nr, j)] = -vr[IX(nr - 2, j)];
+        vr[IX(nr - 1, j)] = 0;
+    } else if (P.bc_outer == FCPT_BC_ZEROGRADIENT) {
+        vr[IX(nr, j)] = vr[IX(nr - 2, j)];
+        vr[IX(nr - 1, j)] = vr[IX(nr - 2, j)];
+    } else {
+        if (vr[IX(nr - 2, j)] < 0.0) {
+            vr[IX(nr - 1, j)] = 0.0;
+            vr[IX(nr, j)] = 0.0;
+        } else {
+            vr[IX(nr - 1, j)] = vr[IX(nr - 2, j)];
+            vr[IX(nr, j)] = vr[IX(nr - 2, j)];
+        }
+    }
+    const double vk_in = sqrt(P.G * P.mstar / P.Rmed[0]);
+    const double vk_out = sqrt(P.G * P.mstar / P.Rmed[nr - 1]);
+    vaz[IX(0, j)] = 1.0 * vk_in - P.Rmed[0] * P.omega_frame;
+    vaz[IX(nr - 1, j)] = 1.0 * vk_out - P.Rmed[nr - 1] * P.omega_frame;
+}
+
+// ---------------------------------------------------------------------------
+// Transport
+
+// compute_momenta_from_velocities, TransportEuler.cpp:471-495
+__global__ void __launch_bounds__(256) k_momenta(KParams P)
+{
+    CELL_LOOP_HEAD(0, P.nr)
+    const double *S = P.f[FCPT_SIGMA], *vr = P.f[FCPT_VRAD], *vaz = P.f[FCPT_VAZI];
+    const double sig = S[IX(i, j)];
+    const int jn = j == ns - 1 ? 0 : j + 1;
+    const double r = P.Rmed[i];
+    const double OmegaF = P.omega_frame;
+    P.mom[0][IX(i, j)] = sig * vr[IX(i + 1, j)];
+    P.mom[1][IX(i, j)] = sig * vr[IX(i, j)];
+    P.mom[2][IX(i, j)] = sig * (vaz[IX(i, jn)] + r * OmegaF) * r;
+    P.mom[3][IX(i, j)] = sig * (vaz[IX(i, j)] + r * OmegaF) * r;
+}
+
+// van Leer slope of a radial profile at ring k (compute_star_radial first loop,
+// TransportEuler.cpp:355-371); q[] holds rings k-1..k+1
+__device__ __forceinline__ double dq_radial(const KParams &P, int k, double qm, double q0, double qp)
+{
+    if (k == 0 || k == P.nr - 1) return 0.0;
+    const double dqm = (q0 - qm) * P.InvDiffRmed[k];
+    const double dqp = (qp - q0) * P.InvDiffRmed[k + 1];
+    return van_leer_lim(dqp, dqm);
+}
+
+// upwind interface value at ring interface i (compute_star_radial second loop,
+// TransportEuler.cpp:376-405); w[0..4] = rings i-2..i+2 of the specific quantity
+__device__ __forceinline__ double star_radial(const KParams &P, int i, const double *w, double v,
+                                              double dt)
+{
+    if (i <= 0 || i >= P.nr) return 0.0;
+    if (v > 0.0) {
+        const double dq = dq_radial(P, i - 1, w[0], w[1], w[2]);
+        return w[1] + (P.Rmed[i] - P.Rmed[i - 1] - v * dt) * 0.5 * dq;
+    }
+    const double dq = dq_radial(P, i, w[1], w[2], w[3]);
+    return w[2] - (P.Rmed[i + 1] - P.Rmed[i] + v * dt) * 0.5 * dq;
+}
+
+// OneWindRad, TransportEuler.cpp:138-168 and VanLeerRadial :545-619: every
+// advected quantity of ring i is updated from rings i-2..i+2 of itself and of
+// the density before the sweep (DENSITY_INT), so all of them are done in one
+// pass that reads inputs and writes separate outputs.
+__global__ void __launch_bounds__(256) k_radial_sweep(KParams P)
+{
+    CELL_LOOP_HEAD(0, P.nr)
+    const int nr = P.nr;
+    const double dt = P.s->dt;
+    const double *S = P.f[FCPT_SIGMA], *vr = P.f[FCPT_VRAD];
+    // density rings i-2..i+2 (values outside [0, nr) are never used by dq)
+    double s5[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        const int r = i - 2 + k;
+        s5[k] = (r >= 0 && r < nr) ? S[IX(r, j)] : 1.0;
+    }
+    const double v0 = vr[IX(i, j)], v1 = vr[IX(i + 1, j)];
+    // DensityStar at interfaces i and i+1 (row nr is never written: 0)
+    const double ds0 = star_radial(P, i, s5, v0, dt);
+    const double ds1 = star_radial(P, i + 1, s5 + 1, v1, dt);
+    const double c_inf = dt * P.dphi * P.Rinf[i];
+    const double c_sup = dt * P.dphi * P.Rsup[i];
+    const int nq = P.adiabatic ? 6 : 5;
+    for (int q = 0; q < nq; ++q) {
+        const double *Q;
+        double *out;
+        if (q < 4) {
+            Q = P.mom[q];
+            out = P.mom_b[q];
+        } else if (q == 4 && P.adiabatic) {
+            Q = P.f[FCPT_ENERGY];
+            out = P.en_b;
+        } else {
+            Q = S;
+            out = P.sig_b;
+        }
+        double w[5];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            const int r = i - 2 + k;
+            w[k] = (r >= 0 && r < nr) ? Q[IX(r, j)] / s5[k] : 0.0;
+        }
+        const double qs0 = star_radial(P, i, w, v0, dt);
+        const double qs1 = star_radial(P, i + 1, w + 1, v1, dt);
+        const double varq_inf = c_inf * qs0 * ds0 * v0;
+        const double varq_sup = c_sup * qs1 * ds1 * v1;
+        out[IX(i, j)] = Q[IX(i, j)] + (varq_inf - varq_sup) * P.InvSurf[i];
+    }
+}
+
+// ComputeStarTheta for one interface j (TransportEuler.cpp:416-465);
+// q[0..3] = sectors j-2..j+1
+__device__ __forceinline__ double star_theta(const double *q, double dxtheta, double invdxtheta,
+                                             double ksi)
+{
+    if (ksi > 0.0) {
+        const double dq = 0.5 * van_leer_lim(q[2] - q[1], q[1] - q[0]) * invdxtheta;
+        return q[1] + (dxtheta - ksi) * dq;
+    }
+    const double dq = 0.5 * van_leer_lim(q[3] - q[2], q[2] - q[1]) * invdxtheta;
+    return q[2] - (dxtheta + ksi) * dq;
+}
+
+// QuantitiesAdvection (TransportEuler.cpp:292-304) with VanLeerTheta (:630-664).
+// pass 0: residual velocity v - <v>; inputs = radial-sweep outputs, outputs = fields
+// pass 1: uniform residual per ring, followed by AdvectSHIFT (:238-268): the value
+//         of sector j is written to sector j + Nshift of the ping-pong buffer
+__global__ void __launch_bounds__(256) k_azimuthal_pass(KParams P, int pass)
+{
+    CELL_LOOP_HEAD(0, P.nr)
+    const double dt = P.s->dt;
+    const double *Sin = pass == 0 ? P.sig_b : P.f[FCPT_SIGMA];
+    int js[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        int jj = j - 2 + k;
+        if (jj < 0) jj += ns;
+        if (jj >= ns) jj -= ns;
+        js[k] = jj;
+    }
+    // velocities at sectors j and j+1
+    double v0, v1;
+    if (pass == 0) {
+        const double *vaz = P.f[FCPT_VAZI];
+        const double vm = P.vmean[i];
+        v0 = vaz[IX(i, js[2])] - vm;
+        v1 = vaz[IX(i, js[3])] - vm;
+    } else {
+        v0 = P.vuni[i];
+        v1 = v0;
+    }
+    const double dxtheta = P.dphi * P.Rmed[i];
+    const double invdxtheta = 1.0 / dxtheta;
+    double s5[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) s5[k] = Sin[IX(i, js[k])];
+    const double ds0 = star_theta(s5, dxtheta, invdxtheta, v0 * dt);
+    const double ds1 = star_theta(s5 + 1, dxtheta, invdxtheta, v1 * dt);
+    const double dxrad = (P.Rsup[i] - P.Rinf[i]) * dt;
+    const double invsurf = P.InvSurf[i];
+    int jdst = j;
+    if (pass == 1) {
+        jdst = j + P.nshift[i];
+        while (jdst < 0) jdst += ns;
+        while (jdst >= ns) jdst -= ns;
+    }
+    const int nq = P.adiabatic ? 6 : 5;
+    for (int q = 0; q < nq; ++q) {
+        const double *Q;
+        double *out;
+        if (q < 4) {
+            Q = pass == 0 ? P.mom_b[q] : P.mom[q];
+            out = pass == 0 ? P.mom[q] : P.mom_b[q];
+        } else if (q == 4 && P.adiabatic) {
+            Q = pass == 0 ? P.en_b : P.f[FCPT_ENERGY];
+            out = pass == 0 ? P.f[FCPT_ENERGY] : P.en_b;
+        } else {
+            Q = Sin;
+            out = pass == 0 ? P.f[FCPT_SIGMA] : P.sig_b;
+        }
+        double w[5];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) w[k] = Q[IX(i, js[k])] / s5[k];
+        const double qs0 = star_theta(w, dxtheta, invdxtheta, v0 * dt);
+        const double qs1 = star_theta(w + 1, dxtheta, invdxtheta, v1 * dt);
+        double varq = dxrad * qs0 * ds0 * v0;
+        varq -= dxrad * qs1 * ds1 * v1;
+        out[IX(i, jdst)] = Q[IX(i, j)] + varq * invsurf;
+    }
+}
+
+// compute_velocities_from_momenta (TransportEuler.cpp:498-535), then
+// assure_minimum_value(Sigma) (SourceEuler.cpp:102-131) and, adiabatic,
+// SetTemperatureFloorCeilValues (TransportEuler.cpp:127-131)
+__global__ void __launch_bounds__(256) k_velocities(KParams P)
+{
+    CELL_LOOP_HEAD(0, P.nr)
+    const double *Sb = P.sig_b;
+    const int jm = j == 0 ? ns - 1 : j - 1;
+    if (i == 0) {
+        P.f[FCPT_VRAD][IX(i, j)] = 0.0;
+    } else {
+        P.f[FCPT_VRAD][IX(i, j)] = (P.mom_b[0][IX(i - 1, j)] + P.mom_b[1][IX(i, j)]) /
+                                   (Sb[IX(i - 1, j)] + Sb[IX(i, j)]);
+    }
+    P.f[FCPT_VAZI][IX(i, j)] = (P.mom_b[2][IX(i, jm)] + P.mom_b[3][IX(i, j)]) /
+                                   (Sb[IX(i, jm)] + Sb[IX(i, j)]) * P.InvRmed[i] -
+                               P.Rmed[i] * P.omega_frame;
+    const double floor_value = P.sigma_floor * P.sigma0;
+    double sig = Sb[IX(i, j)];
+    if (sig < floor_value) sig = floor_value;
+    P.f[FCPT_SIGMA][IX(i, j)] = sig;
+    if (P.adiabatic) P.f[FCPT_ENERGY][IX(i, j)] = temperature_floor(P, P.en_b[IX(i, j)], sig);
+}
+
+// ---------------------------------------------------------------------------
+// totals with Neumaier compensation: per ring (sum, compensation) pairs, then
+// one block folds the rings in order (quantities.cpp:51-78, 242-279)
+__device__ __forceinline__ void neumaier_add(double &s, double &c, double x)
+{
+    const double t = s + x;
+    c += fabs(s) >= fabs(x) ? (s - t) + x : (x - t) + s;
+    s = t;
+}
+
+__global__ void __launch_bounds__(64) k_totals_rings(KParams P, double radius, double *ring_out)
+{
+    const int ns = P.ns;
+    const int i = blockIdx.x;  // ring
+    const int t = threadIdx.x;
+    double m = 0.0, mc = 0.0, l = 0.0, lc = 0.0;
+    const bool active = i >= 1 && i < P.nr - 1 && P.Rmed[i] <= radius;
+    if (active) {
+        const double *S = P.f[FCPT_SIGMA], *vaz = P.f[FCPT_VAZI];
+        for (int j = t; j < ns; j += 64) {
+            const int jm = j == 0 ? ns - 1 : j - 1;
+            neumaier_add(m, mc, P.Surf[i] * S[IX(i, j)]);
+            neumaier_add(l, lc, P.Surf[i] * 0.5 * (S[IX(i, j)] + S[IX(i, jm)]) * P.Rmed[i] *
+                                    (vaz[IX(i, j)] + P.omega_frame * P.Rmed[i]));
+        }
+    }
+    __shared__ double sh[4][64];
+    sh[0][t] = m;
+    sh[1][t] = mc;
+    sh[2][t] = l;
+    sh[3][t] = lc;
+    __syncthreads();
+    if (t == 0) {
+        double M = 0.0, MC = 0.0, L = 0.0, LC = 0.0;
+        for (int k = 0; k < 64; ++k) {
+            neumaier_add(M, MC, sh[0][k]);
+            MC += sh[1][k];
+            neumaier_add(L, LC, sh[2][k]);
+            LC += sh[3][k];
+        }
+        ring_out[4 * i + 0] = M;
+        ring_out[4 * i + 1] = MC;
+        ring_out[4 * i + 2] = L;
+        ring_out[4 * i + 3] = LC;
+    }
+}
+
+__global__ void k_totals_final(KParams P, const double *ring_out, double *out2)
+{
+    double M = 0.0, MC = 0.0, L = 0.0, LC = 0.0;
+    for (int i = 0; i < P.nr; ++i) {
+        neumaier_add(M, MC, ring_out[4 * i + 0]);
+        MC += ring_out[4 * i + 1];
+        neumaier_add(L, LC, ring_out[4 * i + 2]);
+        LC += ring_out[4 * i + 3];
+    }
+    out2[0] = M + MC;
+    out2[1] = L + LC;
+}
+
+// ---------------------------------------------------------------------------
+// launch helpers
+
+static inline dim3 cell_grid(const KParams &P, int rows)
+{
+    return dim3((unsigned)((P.ns + 63) / 64), (unsigned)((rows + 3) / 4));
+}
+static const dim3 kBlock(64, 4);
+
+void launch_ring_mean(const KParams &P, hipStream_t st, int mode)
+{
+    hipLaunchKernelGGL(k_ring_mean, dim3(P.nr), dim3(256), 0, st, P, mode);
+}
+
+void launch_cfl(const KParams &P, hipStream_t st, int mode)
+{
+    launch_ring_mean(P, st, 0);
+    if (P.nr > 2) hipLaunchKernelGGL(k_cfl_cells, cell_grid(P, P.nr - 2), kBlock, 0, st, P);
+    hipLaunchKernelGGL(k_cfl_finalize, dim3(1), dim3(1), 0, st, P, mode);
+}
+
+void launch_set_dt(const KParams &P, hipStream_t st, double dt)
+{
+    hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(1), 0, st, P, dt);
+}
+void launch_set_clock(const KParams &P, hipStream_t st, double t)
+{
+    hipLaunchKernelGGL(k_set_clock, dim3(1), dim3(1), 0, st, P, t);
+}
+void launch_set_time(const KParams &P, hipStream_t st, double t, double last_dt, int n_monitor)
+{
+    hipLaunchKernelGGL(k_set_time, dim3(1), dim3(1), 0, st, P, t, last_dt, n_monitor);
+}
+void launch_set_params(const KParams &P, hipStream_t st, const fcpt_step_params &sp)
+{
+    hipLaunchKernelGGL(k_set_params, dim3(1), dim3(1), 0, st, P, sp);
+}
+void launch_advance_time(const KParams &P, hipStream_t st, int count_monitor)
+{
+    hipLaunchKernelGGL(k_advance_time, dim3(1), dim3(1), 0, st, P, count_monitor);
+}
+
+void launch_potential(const KParams &P, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_potential, cell_grid(P, P.nr), kBlock, 0, st, P);
+}
+
+void launch_sources(const KParams &P, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_src_vr, cell_grid(P, P.nr - 3), kBlock, 0, st, P);
+    hipLaunchKernelGGL(k_src_vphi, cell_grid(P, P.nr - 2), kBlock, 0, st, P);
+    if (P.adiabatic) hipLaunchKernelGGL(k_src_energy, cell_grid(P, P.nr - 1), kBlock, 0, st, P);
+}
+
+void launch_artificial_viscosity(const KParams &P, hipStream_t st)
+{
+    if (P.artvisc == FCPT_ARTVISC_TW) {
+        hipLaunchKernelGGL(k_tw_q, cell_grid(P, P.nr), kBlock, 0, st, P);
+        hipLaunchKernelGGL(k_tw_vel, cell_grid(P, P.nr - 2), kBlock, 0, st, P);
+    } else if (P.artvisc == FCPT_ARTVISC_SN) {
+        hipLaunchKernelGGL(k_sn_q, cell_grid(P, P.nr), kBlock, 0, st, P);
+        hipLaunchKernelGGL(k_sn_vel, cell_grid(P, P.nr - 2), kBlock, 0, st, P);
+    } else if (P.adiabatic && P.artvisc_dissipation) {
+        hipLaunchKernelGGL(k_temperature_floor, cell_grid(P, P.nr), kBlock, 0, st, P);
+    }
+}
+
+void launch_recalc_viscosity(const KParams &P, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_recalc_viscosity, cell_grid(P, P.nr), kBlock, 0, st, P);
+}
+
+void launch_stress(const KParams &P, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_stress, cell_grid(P, P.nr), kBlock, 0, st, P);
+}
+
+void launch_visc_update(const KParams &P, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_visc_update, cell_grid(P, P.nr - 2), kBlock, 0, st, P);
+}
+
+void launch_substep3(const KParams &P, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_substep3, cell_grid(P, P.nr), kBlock, 0, st, P);
+}
+
+void launch_heatcool_cfl(const KParams &P, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_heatcool_cfl, cell_grid(P, P.nr), kBlock, 0, st, P);
+}
+
+void launch_boundaries(const KParams &P, hipStream_t st, int final_call)
+{
+    if (final_call && P.damping) {
+        const int n_inner = (P.damp_ids[0] > P.damp_ids[1] ? P.damp_ids[0] : P.damp_ids[1]) + 1;
+        const int o0 = P.damp_ids[2] < P.damp_ids[3] ? P.damp_ids[2] : P.damp_ids[3];
+        const int n_outer = P.nr + 1 - o0;
+        hipLaunchKernelGGL(k_damping, cell_grid(P, n_inner + n_outer), kBlock, 0, st, P, n_inner);
+    }
+    hipLaunchKernelGGL(k_ghost_bc, dim3((unsigned)((P.ns + 255) / 256)), dim3(256), 0, st, P);
+}
+
+void launch_transport(const KParams &P, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_momenta, cell_grid(P, P.nr), kBlock, 0, st, P);
+    hipLaunchKernelGGL(k_radial_sweep, cell_grid(P, P.nr), kBlock, 0, st, P);
+    launch_ring_mean(P, st, 1);
+    hipLaunchKernelGGL(k_azimuthal_pass, cell_grid(P, P.nr), kBlock, 0, st, P, 0);
+    hipLaunchKernelGGL(k_azimuthal_pass, cell_grid(P, P.nr), kBlock, 0, st, P, 1);
+    hipLaunchKernelGGL(k_velocities, cell_grid(P, P.nr), kBlock, 0, st, P);
+}
+
+void launch_derived(const KParams &P, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_derived, cell_grid(P, P.nr), kBlock, 0, st, P);
+}
+
+void launch_iso_profile(const KParams &P, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_iso_profile, cell_grid(P, P.nr), kBlock, 0, st, P);
+}
+
+void launch_totals(const KParams &P, hipStream_t st, double radius, double *ring_scratch,
+                   double *out2)
+{
+    hipLaunchKernelGGL(k_totals_rings, dim3(P.nr), dim3(64), 0, st, P, radius, ring_scratch);
+    hipLaunchKernelGGL(k_totals_final, dim3(1), dim3(1), 0, st, P, ring_scratch, out2);
+}
+
+}  // namespace fcpt

@@ -127,3 +127,103 @@ def rel_err(a: np.ndarray, b: np.ndarray) -> float:
     if scale == 0.0:
         return diff
     return diff / scale
+
+
+def call_stage(disk, stage: str, dt: float, t_before: float):
+    """Invoke the driver function that produced one per-stage dump."""
+    if stage == "potential":
+        disk.nbody_potential()
+    elif stage == "sources":
+        disk.update_with_sourceterms(dt)
+    elif stage == "artvisc":
+        disk.artificial_viscosity(dt)
+    elif stage == "viscosity":
+        disk.recalculate_viscosity()
+    elif stage == "stress":
+        disk.viscous_stress_tensor()
+    elif stage == "viscupdate":
+        disk.update_velocities_with_viscosity(dt)
+    elif stage == "substep3":
+        disk.substep3(t_before, dt)
+    elif stage == "bc":
+        disk.apply_boundary_condition(0.0, False)
+    elif stage == "transport":
+        disk.transport(dt)
+    elif stage == "bcfinal":
+        disk.apply_boundary_condition(dt, True)
+    elif stage == "derived":
+        disk.recalculate_derived()
+    else:
+        raise ValueError(stage)
+
+
+def check_stages(disk, meta, data, step: int):
+    """Replay every stage of one reference step from the dumped inputs.
+
+    Returns (worst relative error per stage and field, nshift_equal)."""
+    disk.upload_reference(data["init_sigma0"], data["init_vrad0"], data["init_vazi0"],
+                          data["init_energy0"])
+    rec = data["steps"][step - 1]
+    dt, t = rec[3], rec[0]
+    planets, ind = step_inputs(data, step)
+    disk.set_step_params(planets, ind)
+    disk.set_time(t, rec[1], int(rec[4]))
+    errors = {}
+    nshift_ok = None
+    prev = None
+    for stage, state in stage_states(data, meta, step):
+        if prev is not None:
+            for f, name in STAGE_FIELDS.items():
+                disk.upload_field(name, prev[f])
+            call_stage(disk, stage, dt, t)
+            if stage == "transport":
+                nshift_ok = bool(np.array_equal(disk.nshift(), data["nshift"][step - 1]))
+            for f, name in STAGE_FIELDS.items():
+                errors[(stage, f)] = rel_err(disk.download_field(name), state[f])
+        prev = state
+    return errors, nshift_ok
+
+
+STATE_FIELDS = [("sigma", "SIGMA"), ("vrad", "VRAD"), ("vazi", "VAZI"), ("energy", "ENERGY"),
+                ("qplus", "QPLUS"), ("qminus", "QMINUS")]
+
+
+def compare_state(disk, data, tag: str) -> dict:
+    """Relative error of the driver's state against a checkpoint (whole arrays,
+    or per-ring sums / extrema / sample rings for large grids)."""
+    out = {}
+    for f, name in STATE_FIELDS:
+        a = disk.download_field(name)
+        key = f"state_{tag}_{f}"
+        if key in data.files:
+            out[f] = rel_err(a, data[key])
+        elif key + "_ringsum" in data.files:
+            out[f + "_ringsum"] = rel_err(a.sum(axis=1), data[key + "_ringsum"])
+            out[f + "_ringmin"] = rel_err(a.min(axis=1), data[key + "_ringmin"])
+            out[f + "_ringmax"] = rel_err(a.max(axis=1), data[key + "_ringmax"])
+            rows = data[key + "_rowidx"]
+            out[f + "_rows"] = rel_err(a[rows], data[key + "_rows"])
+    return out
+
+
+def run_case(disk, meta, data, nsteps=None):
+    """Replay a whole reference run: start-up calls, then sim::run iterations
+    with the reference's per-step planet inputs.  Returns
+    (max relative dt error, number of steps whose FARGO shifts differ,
+     {checkpoint: errors})."""
+    start_run(disk, meta)
+    nsteps = nsteps or meta["nsteps"]
+    res = {"0": compare_state(disk, data, "0")}
+    worst_dt = 0.0
+    bad_shift = 0
+    for s in range(1, nsteps + 1):
+        planets, ind = step_inputs(data, s)
+        disk.set_step_params(planets, ind)
+        dt = disk.run_step()
+        ref_dt = data["steps"][s - 1][3]
+        worst_dt = max(worst_dt, abs(dt - ref_dt) / ref_dt)
+        if not np.array_equal(disk.nshift(), data["nshift"][s - 1]):
+            bad_shift += 1
+        if f"state_{s}_sigma" in data.files or f"state_{s}_sigma_ringsum" in data.files:
+            res[str(s)] = compare_state(disk, data, str(s))
+    return worst_dt, bad_shift, res

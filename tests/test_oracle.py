@@ -1,0 +1,149 @@
+"""The CPU oracle against the reference FargoCPT fixtures and against closed
+forms.  The oracle is a C restatement of the reference's loops; built without
+fast-math and FMA contraction it must reproduce the reference bit for bit."""
+import numpy as np
+import pytest
+
+from fixtures import (check_stages, compare_state, disk_config, initial_condition, load_case,
+                      run_case)
+from oracle import OracleDisk
+
+SMALL = ["c1_small", "c2_small", "c5_small"]
+
+
+@pytest.mark.parametrize("case", ["c1_small", "c1_iso", "c5_small"])
+def test_geometry_bit_exact(case, oracle_lib):
+    meta, data = load_case(case)
+    d = OracleDisk(disk_config(meta))
+    nr = meta["Nr"]
+    for g, n in [("Radii", nr + 1), ("Rinf", nr + 1), ("Rsup", nr), ("Rmed", nr + 1),
+                 ("Surf", nr), ("InvSurf", nr), ("InvRmed", nr + 1), ("InvRinf", nr + 1),
+                 ("InvDiffRmed", nr + 1), ("InvDiffRsup", nr), ("InvDiffRsupRb", nr),
+                 ("TwoDiffRaSq", nr)]:
+        assert np.array_equal(d.geom(g, n), data["geom_" + g]), g
+    ids = [meta[k] for k in ("damp_inner_scalar", "damp_inner_vector", "damp_outer_scalar",
+                             "damp_outer_vector")]
+    assert d.damping_ids() == ids
+
+
+@pytest.mark.parametrize("case", SMALL)
+@pytest.mark.parametrize("step", [1, 2])
+def test_stages_bit_exact(case, step, oracle_lib):
+    meta, data = load_case(case)
+    if f"stage_{step}_00_start_sigma" not in data.files and step > 1:
+        if not any(k.startswith(f"stage_{step}_") for k in data.files):
+            pytest.skip("no dump of this step")
+    d = OracleDisk(disk_config(meta))
+    errors, nshift_ok = check_stages(d, meta, data, step)
+    bad = {k: v for k, v in errors.items() if v != 0.0}
+    assert not bad, bad
+    assert nshift_ok
+
+
+@pytest.mark.parametrize("case", SMALL + ["c1_iso"])
+def test_full_run_bit_exact(case, oracle_lib):
+    meta, data = load_case(case)
+    d = OracleDisk(disk_config(meta))
+    worst_dt, bad_shift, res = run_case(d, meta, data)
+    assert worst_dt == 0.0
+    assert bad_shift == 0
+    for tag, errs in res.items():
+        assert all(v == 0.0 for v in errs.values()), (tag, errs)
+
+
+def test_adiabatic_512x1536_500_steps(oracle_lib):
+    """The full adiabatic configuration: per-ring moments and sample rings of
+    all fields after 500 steps, every dt and every FARGO shift."""
+    meta, data = load_case("c2_adi")
+    d = OracleDisk(disk_config(meta))
+    worst_dt, bad_shift, res = run_case(d, meta, data)
+    assert worst_dt == 0.0 and bad_shift == 0
+    for tag, errs in res.items():
+        assert all(v == 0.0 for v in errs.values()), (tag, errs)
+
+
+# --- closed-form checks that do not depend on the reference fixtures --------
+
+def _small_disk(nr=24, nphi=48, adiabatic=False):
+    meta, _ = load_case("c2_small" if adiabatic else "c1_small")
+    meta = dict(meta, Nr=nr, Nphi=nphi)
+    return meta, OracleDisk(disk_config(meta))
+
+
+def test_pure_fargo_shift_is_a_rotation(oracle_lib):
+    """v_r = 0 and v_phi = N_i Rmed_i dphi / dt make the residual and the
+    uniform residual velocities vanish (up to the rounding of N_i Rmed_i dphi
+    / dt): transport is then np.roll by N_i, and the shifts are exact."""
+    meta, d = _small_disk()
+    nr, ns = meta["Nr"], meta["Nphi"]
+    rng = np.random.default_rng(5)
+    sigma = 1.0 + rng.random((nr, ns))
+    rmed = d.geom("Rmed", nr)
+    dt = 0.01
+    dphi = 2 * np.pi / ns
+    shifts = (np.arange(nr) % 7) - 3
+    vazi = np.repeat((shifts * rmed * dphi / dt)[:, None], ns, axis=1)
+    d.upload_state(sigma, np.zeros((nr + 1, ns)), vazi, np.zeros((nr, ns)))
+    d.transport(dt)
+    got = d.download_field("SIGMA")
+    want = np.stack([np.roll(sigma[i], shifts[i]) for i in range(nr)])
+    assert np.array_equal(d.nshift(), shifts)
+    np.testing.assert_allclose(got, want, rtol=1e-13, atol=0)
+
+
+def test_transport_conserves_mass(oracle_lib):
+    """With v_r = 0 on the boundary interfaces the flux form conserves
+    sum(Surf * Sigma) to rounding (TransportEuler.cpp:587-597, 652-661)."""
+    meta, d = _small_disk(adiabatic=True)
+    nr, ns = meta["Nr"], meta["Nphi"]
+    rng = np.random.default_rng(7)
+    sigma = 1.0 + 0.3 * rng.random((nr, ns))
+    vrad = 1e-3 * rng.standard_normal((nr + 1, ns))
+    vrad[[0, 1, nr - 1, nr]] = 0.0
+    rmed = d.geom("Rmed", nr)
+    vazi = (rmed ** -0.5)[:, None] - rmed[:, None] + 1e-3 * rng.standard_normal((nr, ns))
+    energy = sigma * 1e-3
+    d.upload_state(sigma, vrad, vazi, energy)
+    surf = d.geom("Surf", nr)
+    m0 = np.sum(surf[:, None] * sigma)
+    d.transport(0.01)
+    m1 = np.sum(surf[:, None] * d.download_field("SIGMA"))
+    assert abs(m1 - m0) / m0 < 1e-14
+
+
+def test_potential_closed_form(oracle_lib):
+    meta, d = _small_disk()
+    nr, ns = meta["Nr"], meta["Nphi"]
+    ic = initial_condition(dict(meta))
+    d.upload_state(ic["sigma"], ic["vrad"], ic["vazi"], ic["energy"])
+    planets = [(0.0, 0.0, 1.0, 0.0), (0.8, 0.3, 1e-3, 0.0)]
+    d.set_step_params(planets, (1e-4, -2e-4))
+    d.nbody_potential()
+    rmed = d.geom("Rmed", nr)
+    phi = 2 * np.pi * np.arange(ns) / ns
+    x = rmed[:, None] * np.cos(phi)[None, :]
+    y = rmed[:, None] * np.sin(phi)[None, :]
+    H = d.download_field("SCALE_HEIGHT")
+    want = np.zeros((nr, ns))
+    for (px, py, m, _) in planets:
+        want += -meta["G"] * m / np.sqrt((x - px) ** 2 + (y - py) ** 2 + (0.6 * H) ** 2)
+    want += -1e-4 * x + 2e-4 * y
+    np.testing.assert_allclose(d.download_field("POTENTIAL"), want, rtol=1e-13, atol=0)
+
+
+def test_totals_match_naive_sum(oracle_lib):
+    meta, data = load_case("c1_iso")
+    d = OracleDisk(disk_config(meta))
+    sigma = data["state_100_sigma"]
+    vazi = data["state_100_vazi"]
+    d.upload_state(sigma, data["state_100_vrad"], vazi, None)
+    m, l = d.totals(meta["RMAX"])
+    nr = meta["Nr"]
+    surf = d.geom("Surf", nr)
+    rmed = d.geom("Rmed", nr)
+    act = slice(1, nr - 1)
+    m_ref = np.sum(surf[act, None] * sigma[act])
+    sl = 0.5 * (sigma + np.roll(sigma, 1, axis=1))
+    l_ref = np.sum((surf * rmed)[act, None] * sl[act] * (vazi[act] + rmed[act, None]))
+    assert abs(m - m_ref) / m_ref < 1e-13
+    assert abs(l - l_ref) / abs(l_ref) < 1e-13
