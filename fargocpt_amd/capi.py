@@ -23,6 +23,8 @@ FIELD_ID = {name: i for i, name in enumerate(FIELDS)}
 VECTOR_FIELDS = {"VRAD", "VRAD0", "TAU_R_PHI"}
 
 ARTVISC = {"none": 0, "tw": 1, "sn": 2}
+KERNELS = {"radial_sweep": 0, "azimuthal_residual": 1, "azimuthal_shift": 2, "cfl": 3,
+           "stress": 4, "potential": 5}
 BC = {"reflecting": 0, "zerogradient": 1, "outflow": 2}
 MAX_PLANETS = 8
 
@@ -154,6 +156,10 @@ def load_library(path: str = LIB_PATH):
         "fcpt_step_euler": [P, D],
         "fcpt_run_step": [P, dp],
         "fcpt_totals": [P, D, dp, dp],
+        "fcpt_run_steps": [P, I, I],
+        "fcpt_time_steps": [P, I, I, dp],
+        "fcpt_time_kernel": [P, I, I, dp],
+        "fcpt_time_copy": [P, I, dp, dp],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
@@ -297,3 +303,21 @@ class Disk:
 
     def synchronize(self):
         self._check(self.lib.fcpt_synchronize(self.h))
+
+    def run_steps(self, n, use_graph=True):
+        self._check(self.lib.fcpt_run_steps(self.h, n, int(use_graph)))
+
+    def time_steps(self, n, use_graph=True):
+        ms = C.c_double()
+        self._check(self.lib.fcpt_time_steps(self.h, n, int(use_graph), C.byref(ms)))
+        return ms.value
+
+    def time_copy(self, reps):
+        ms, nb = C.c_double(), C.c_double()
+        self._check(self.lib.fcpt_time_copy(self.h, reps, C.byref(ms), C.byref(nb)))
+        return ms.value, nb.value
+
+    def time_kernel(self, kernel, reps):
+        ms = C.c_double()
+        self._check(self.lib.fcpt_time_kernel(self.h, kernel, reps, C.byref(ms)))
+        return ms.value

@@ -27,6 +27,8 @@ struct fcpt_ctx {
     double *totals_ring = nullptr;
     double *totals_out = nullptr;
     double *totals_host = nullptr;
+    hipGraphExec_t step_graph = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::string err;
 };
 
@@ -279,6 +281,9 @@ int fcpt_destroy(fcpt_ctx *ctx)
         hipStreamSynchronize(ctx->stream);
         hipStreamDestroy(ctx->stream);
     }
+    if (ctx->step_graph) hipGraphExecDestroy(ctx->step_graph);
+    if (ctx->ev0) hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) hipEventDestroy(ctx->ev1);
     for (void *p : ctx->allocs) hipFree(p);
     if (ctx->s_host) hipHostFree(ctx->s_host);
     if (ctx->totals_host) hipHostFree(ctx->totals_host);
@@ -535,6 +540,123 @@ int fcpt_run_step(fcpt_ctx *ctx, double *dt_out)
     if (ctx->s_host->error_flags & 1)
         return fail(ctx, FCPT_ERR_NONFINITE, "CFL time step is not finite and positive");
     if (dt_out) *dt_out = ctx->s_host->dt;
+    return FCPT_OK;
+}
+
+// capture one sim::run iteration into a graph (once per context)
+static int ensure_step_graph(fcpt_ctx *ctx)
+{
+    if (ctx->step_graph) return FCPT_OK;
+    hipGraph_t g = nullptr;
+    HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    launch_cfl(ctx->P, ctx->stream, 2);
+    enqueue_euler(ctx->P, ctx->stream, 1);
+    hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    if (e != hipSuccess) return fail(ctx, FCPT_ERR_DEVICE, "graph capture: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(&ctx->step_graph, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        ctx->step_graph = nullptr;
+        return fail(ctx, FCPT_ERR_DEVICE, "graph instantiate: %s", hipGetErrorString(e));
+    }
+    return FCPT_OK;
+}
+
+int fcpt_run_steps(fcpt_ctx *ctx, int32_t n_steps, int32_t use_graph)
+{
+    CTX_CHECK(ctx);
+    if (n_steps < 0) return fail(ctx, FCPT_ERR_ARG, "negative step count");
+    if (use_graph) {
+        int rc = ensure_step_graph(ctx);
+        if (rc) return rc;
+        for (int k = 0; k < n_steps; ++k) HIP_TRY(ctx, hipGraphLaunch(ctx->step_graph, ctx->stream));
+    } else {
+        for (int k = 0; k < n_steps; ++k) {
+            launch_cfl(ctx->P, ctx->stream, 2);
+            enqueue_euler(ctx->P, ctx->stream, 1);
+        }
+        LAUNCH_CHECK(ctx);
+    }
+    return FCPT_OK;
+}
+
+static int ensure_events(fcpt_ctx *ctx)
+{
+    if (!ctx->ev0) HIP_TRY(ctx, hipEventCreate(&ctx->ev0));
+    if (!ctx->ev1) HIP_TRY(ctx, hipEventCreate(&ctx->ev1));
+    return FCPT_OK;
+}
+
+int fcpt_time_steps(fcpt_ctx *ctx, int32_t n_steps, int32_t use_graph, double *ms_total)
+{
+    CTX_CHECK(ctx);
+    int rc = ensure_events(ctx);
+    if (!rc && use_graph) rc = ensure_step_graph(ctx);
+    if (rc) return rc;
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    rc = fcpt_run_steps(ctx, n_steps, use_graph);
+    if (rc) return rc;
+    HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
+    float ms = 0.f;
+    HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    rc = read_scalars(ctx);
+    if (rc) return rc;
+    if (ctx->s_host->error_flags & 1)
+        return fail(ctx, FCPT_ERR_NONFINITE, "CFL time step became non-finite during the run");
+    if (ms_total) *ms_total = (double)ms;
+    return FCPT_OK;
+}
+
+int fcpt_time_kernel(fcpt_ctx *ctx, int32_t kernel, int32_t reps, double *ms_per_launch)
+{
+    CTX_CHECK(ctx);
+    if (reps <= 0) return fail(ctx, FCPT_ERR_ARG, "reps must be positive");
+    int rc = ensure_events(ctx);
+    if (rc) return rc;
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    auto one = [&]() {
+        switch (kernel) {
+        case FCPT_KERNEL_RADIAL_SWEEP: launch_radial_sweep(ctx->P, ctx->stream); break;
+        case FCPT_KERNEL_AZIMUTHAL_RESIDUAL: launch_azimuthal_pass(ctx->P, ctx->stream, 0); break;
+        case FCPT_KERNEL_AZIMUTHAL_SHIFT: launch_azimuthal_pass(ctx->P, ctx->stream, 1); break;
+        case FCPT_KERNEL_CFL: launch_cfl_cells(ctx->P, ctx->stream); break;
+        case FCPT_KERNEL_STRESS: launch_stress(ctx->P, ctx->stream); break;
+        case FCPT_KERNEL_POTENTIAL: launch_potential(ctx->P, ctx->stream); break;
+        default: break;
+        }
+    };
+    if (kernel < 0 || kernel > FCPT_KERNEL_POTENTIAL) return fail(ctx, FCPT_ERR_ARG, "unknown kernel");
+    one();  // warm-up
+    HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    for (int k = 0; k < reps; ++k) one();
+    HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    LAUNCH_CHECK(ctx);
+    HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
+    float ms = 0.f;
+    HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    if (ms_per_launch) *ms_per_launch = (double)ms / reps;
+    return FCPT_OK;
+}
+
+int fcpt_time_copy(fcpt_ctx *ctx, int32_t reps, double *ms_per_launch, double *bytes_per_launch)
+{
+    CTX_CHECK(ctx);
+    if (reps <= 0) return fail(ctx, FCPT_ERR_ARG, "reps must be positive");
+    int rc = ensure_events(ctx);
+    if (rc) return rc;
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    launch_copy4(ctx->P, ctx->stream);
+    HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    for (int k = 0; k < reps; ++k) launch_copy4(ctx->P, ctx->stream);
+    HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    LAUNCH_CHECK(ctx);
+    HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
+    float ms = 0.f;
+    HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    if (ms_per_launch) *ms_per_launch = (double)ms / reps;
+    if (bytes_per_launch) *bytes_per_launch = 2.0 * 4.0 * 8.0 * (double)((size_t)ctx->P.nr * ctx->P.ns / 2 * 2);
     return FCPT_OK;
 }
 

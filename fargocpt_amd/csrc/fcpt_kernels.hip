@@ -992,6 +992,18 @@ __global__ void k_totals_final(KParams P, const double *ring_out, double *out2)
     out2[1] = L + LC;
 }
 
+// streaming copy of four work arrays (bandwidth probe), 16-byte accesses
+__global__ void __launch_bounds__(256) k_copy4(KParams P, size_t n2)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (int q = 0; q < 4; ++q) {
+        const double2 *src = reinterpret_cast<const double2 *>(P.mom[q]);
+        double2 *dst = reinterpret_cast<double2 *>(P.mom_b[q]);
+        for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n2; k += stride)
+            dst[k] = src[k];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // launch helpers
 
@@ -1093,6 +1105,28 @@ void launch_boundaries(const KParams &P, hipStream_t st, int final_call)
         hipLaunchKernelGGL(k_damping, cell_grid(P, n_inner + n_outer), kBlock, 0, st, P, n_inner);
     }
     hipLaunchKernelGGL(k_ghost_bc, dim3((unsigned)((P.ns + 255) / 256)), dim3(256), 0, st, P);
+}
+
+void launch_radial_sweep(const KParams &P, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_radial_sweep, cell_grid(P, P.nr), kBlock, 0, st, P);
+}
+void launch_azimuthal_pass(const KParams &P, hipStream_t st, int pass)
+{
+    hipLaunchKernelGGL(k_azimuthal_pass, cell_grid(P, P.nr), kBlock, 0, st, P, pass);
+}
+void launch_copy4(const KParams &P, hipStream_t st)
+{
+    const size_t n2 = (size_t)P.nr * P.ns / 2;
+    size_t blocks = (n2 + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_copy4, dim3((unsigned)blocks), dim3(256), 0, st, P, n2);
+}
+
+void launch_cfl_cells(const KParams &P, hipStream_t st)
+{
+    if (P.nr > 2) hipLaunchKernelGGL(k_cfl_cells, cell_grid(P, P.nr - 2), kBlock, 0, st, P);
 }
 
 void launch_transport(const KParams &P, hipStream_t st)

@@ -25,6 +25,10 @@ void launch_substep3(const KParams &P, hipStream_t st);
 void launch_heatcool_cfl(const KParams &P, hipStream_t st);
 void launch_boundaries(const KParams &P, hipStream_t st, int final_call);
 void launch_transport(const KParams &P, hipStream_t st);
+void launch_radial_sweep(const KParams &P, hipStream_t st);
+void launch_azimuthal_pass(const KParams &P, hipStream_t st, int pass);
+void launch_cfl_cells(const KParams &P, hipStream_t st);
+void launch_copy4(const KParams &P, hipStream_t st);
 void launch_derived(const KParams &P, hipStream_t st);
 void launch_iso_profile(const KParams &P, hipStream_t st);
 void launch_totals(const KParams &P, hipStream_t st, double radius, double *ring_scratch,

@@ -178,6 +178,30 @@ int fcpt_step_euler(fcpt_ctx *ctx, double dt);
    clip to the next monitor time, step_Euler.  *dt_out receives the step size. */
 int fcpt_run_step(fcpt_ctx *ctx, double *dt_out);
 
+/* n_steps iterations of sim::run entirely on the device: the CFL time step,
+   the monitor clip and the time stay in device memory and the planets keep the
+   last step parameters.  use_graph != 0 replays one captured HIP graph per
+   iteration.  No host synchronisation; errors surface at the next sync. */
+int fcpt_run_steps(fcpt_ctx *ctx, int32_t n_steps, int32_t use_graph);
+
+/* Measurement hooks (no counterpart in the reference): elapsed device time of
+   n_steps fcpt_run_steps iterations, from HIP events on the context's stream;
+   and the mean duration of one launch of a single kernel of the step, timed
+   the same way over `reps` back-to-back launches on the current state
+   (FCPT_KERNEL_* below; the listed kernels do not modify their inputs). */
+enum { FCPT_KERNEL_RADIAL_SWEEP = 0, FCPT_KERNEL_AZIMUTHAL_RESIDUAL = 1,
+       FCPT_KERNEL_AZIMUTHAL_SHIFT = 2, FCPT_KERNEL_CFL = 3, FCPT_KERNEL_STRESS = 4,
+       FCPT_KERNEL_POTENTIAL = 5 };
+int fcpt_time_steps(fcpt_ctx *ctx, int32_t n_steps, int32_t use_graph, double *ms_total);
+int fcpt_time_kernel(fcpt_ctx *ctx, int32_t kernel, int32_t reps, double *ms_per_launch);
+/* Streaming-copy probe (no counterpart in the reference): copies four
+   transport work arrays into their partners (8 * nr * nphi * 8 bytes moved
+   per launch, returned in *bytes_per_launch) and times it like
+   fcpt_time_kernel; it measures the achievable HBM bandwidth of the box and
+   calibrates the profiler's byte counters.  It overwrites transport scratch
+   only. */
+int fcpt_time_copy(fcpt_ctx *ctx, int32_t reps, double *ms_per_launch, double *bytes_per_launch);
+
 /* Compensated (Neumaier) totals over active rings with Rmed <= radius
    (quantities::gas_total_mass / gas_angular_momentum, quantities.cpp:51-78,242-279). */
 int fcpt_totals(fcpt_ctx *ctx, double radius, double *mass, double *angular_momentum);

@@ -30,6 +30,7 @@ STAGE_FIELDS = {
     "qplus": "QPLUS", "qminus": "QMINUS", "temperature": "TEMPERATURE",
 }
 FIRST_DT = 0.1  # FirstDT of every fixture config
+BC_IDS = {"reflecting": 0, "zerogradient": 1, "outflow": 2}
 
 
 def load_case(name: str):
@@ -51,7 +52,7 @@ def disk_config(meta: dict) -> dict:
         "aspect_ratio": meta["aspectratio"], "flaring_index": meta["flaring"],
         "alpha": meta["alpha"], "thickness_smoothing": meta["thickness_smoothing"],
         "cfl": meta["cfl"], "cfl_max_var": meta["cfl_max_var"],
-        "heating_cooling_cfl_limit": 1.0,
+        "heating_cooling_cfl_limit": meta.get("heating_cooling_cfl_limit", 1.0),
         "sigma0": meta["sigma0"], "sigma_floor": meta["sigma_floor"],
         "t_min": meta["Tmin"], "t_max": meta["Tmax"],
         "artificial_viscosity": meta["artvisc"],
@@ -60,10 +61,14 @@ def disk_config(meta: dict) -> dict:
         "heating_viscous": meta["heating_viscous"],
         "heating_viscous_factor": meta["heating_viscous_factor"],
         "cooling_beta_enabled": meta["cooling_beta_enabled"],
-        "cooling_beta_ref": 3 if meta["cooling_beta_floor"] else 0,
-        "cooling_beta": meta["cooling_beta"], "cooling_beta_ramp_up": 0.0,
+        "cooling_beta_ref": (3 if meta["cooling_beta_floor"]
+                             else 1 if meta.get("cooling_beta_reference") else 0),
+        "cooling_beta": meta["cooling_beta"],
+        "cooling_beta_ramp_up": meta.get("cooling_beta_ramp_up", 0.0),
         "radial_viscosity_factor": meta["radial_viscosity_factor"],
-        "bc_inner": 0, "bc_outer": 0, "damping": 1,
+        "bc_inner": BC_IDS[meta.get("bc_inner", "reflecting")],
+        "bc_outer": BC_IDS[meta.get("bc_outer", "reflecting")],
+        "damping": meta.get("damping_enabled", 1),
         "damping_inner_limit": meta["damping_inner_limit"],
         "damping_outer_limit": meta["damping_outer_limit"],
         "damping_time_factor": meta["damping_time_factor"],

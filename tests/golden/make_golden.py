@@ -64,6 +64,8 @@ BASE = {
 
 PLANET_1 = [(1.0, 1e-3)]
 PLANETS_3 = [(1.0, 1e-3), (1.6, 3e-4), (2.5, 1e-4)]
+# (a, q, cubic smoothing factor, ramp-up time in orbits)
+PLANETS_MIX = [(1.0, 1e-3, 1.5, 0.0), (1.5, 5e-4, 0.0, 0.5)]
 
 ADIABATIC = {"EquationOfState": "Ideal", "ArtificialViscosity": "SN"}
 
@@ -78,6 +80,18 @@ CASES = {
     # adiabatic with beta cooling (beta 10, floor reference) and three planets, 48 x 144
     "c5_small": (dict(ADIABATIC, Nrad="48", Naz="144", Rmax="4.0", DampingTimeRadiusOuter="4.0",
                       CoolingBetaLocal="yes"), PLANETS_3, 20, 1, [1, 5, 20], True),
+    # branches the cases above leave out: adiabatic with TW viscosity, outflow inner and
+    # zero-gradient outer boundaries, Klahr-Kley cubic smoothing, beta cooling towards
+    # the reference state with a ramp-up, a planet mass ramp-up; 32 x 64
+    "mix_small": (dict(EquationOfState="Ideal", ArtificialViscosity="TW", Nrad="32", Naz="64",
+                       InnerBoundary="Outflow", OuterBoundary="ZeroGradient",
+                       CoolingBetaLocal="yes", CoolingBetaReference="reference",
+                       CoolingBetaRampUp="1.0"), PLANETS_MIX, 20, 1, [1, 5, 20], True),
+    # locally isothermal without artificial viscosity and without damping, zero-gradient
+    # boundaries; 32 x 64
+    "none_small": (dict(ArtificialViscosity="none", Nrad="32", Naz="64", Damping="No",
+                        InnerBoundary="ZeroGradient", OuterBoundary="ZeroGradient"),
+                   PLANET_1, 20, 1, [1, 5, 20], True),
     # adiabatic 512 x 1536, 500 steps: only per-ring moments and a few full rings kept
     "c2_adi": (dict(ADIABATIC, Nrad="512", Naz="1536"), PLANET_1, 500, 0, [500], False),
 }
@@ -91,11 +105,14 @@ def write_config(path: str, overrides: dict, planets) -> None:
     lines.append("nbody:")
     lines += ["- name: Star", "  semi-major axis: 0.0 au", "  mass: 1.0 solMass",
               "  eccentricity: 0.0", "  radius: 1.0 solRadius", "  temperature: 0 K"]
-    for k, (a, q) in enumerate(planets):
+    for k, pl in enumerate(planets):
+        a, q = pl[0], pl[1]
+        csf = pl[2] if len(pl) > 2 else 0.0
+        ramp = pl[3] if len(pl) > 3 else 0.0
         lines += [f"- name: Planet{k}", f"  semi-major axis: {a} au", f"  mass: {q} solMass",
-                  "  cubic smoothing factor: 0.0", "  accretion efficiency: 0.0",
+                  f"  cubic smoothing factor: {csf}", "  accretion efficiency: 0.0",
                   "  accretion method: none", "  eccentricity: 0", "  radius: 0.01 solRadius",
-                  "  ramp-up time: 0.0"]
+                  f"  ramp-up time: {ramp}"]
     with open(path, "w") as f:
         f.write("\n".join(lines) + "\n")
 
@@ -180,7 +197,11 @@ def make_case(name, exe, addr, keep_dir=None):
         meta["noise_amp"] = 1e-4
         meta["seed"] = 1337
         meta["sigma_slope"] = float(BASE["SigmaSlope"])
-        meta["planets"] = [[a, q] for a, q in planets]
+        meta["planets"] = [list(pl) for pl in planets]
+        cfg = dict(BASE)
+        cfg.update(overrides)
+        meta["bc_inner"] = cfg["InnerBoundary"].lower()
+        meta["bc_outer"] = cfg["OuterBoundary"].lower()
         out = {"meta_json": np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8)}
         for g in ("Radii", "Rinf", "Rsup", "Rmed", "Surf", "InvSurf", "InvRmed", "InvRinf",
                   "InvDiffRmed", "InvDiffRsup", "InvDiffRsupRb", "TwoDiffRaSq"):

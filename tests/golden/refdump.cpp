@@ -347,6 +347,10 @@ int main(int argc, char **argv)
 		clamp_r_id_to_rmed_grid(get_rmed_id(RMAX * boundary_conditions::damping_outer_limit) + 1, false),
 		clamp_r_id_to_radii_grid(get_rinf_id(RMAX * boundary_conditions::damping_outer_limit) + 1, true));
 	fprintf(f, "n_planets = %u\n", data.get_planetary_system().get_number_of_planets());
+	fprintf(f, "damping_enabled = %d\n", boundary_conditions::damping_enabled ? 1 : 0);
+	fprintf(f, "cooling_beta_reference = %d\ncooling_beta_ramp_up = %.17g\n",
+		parameters::cooling_beta_reference ? 1 : 0, parameters::cooling_beta_ramp_up);
+	fprintf(f, "heating_cooling_cfl_limit = %.17g\n", parameters::HEATING_COOLING_CFL_LIMIT);
 	fclose(f);
     }
     write_raw("geom_Radii.bin", &Radii[0], sizeof(double) * (Nr + 1));

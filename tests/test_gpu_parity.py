@@ -23,7 +23,8 @@ def gpu_disk(meta):
 
 
 @pytest.mark.parametrize("case,step", [("c1_small", 1), ("c1_small", 2), ("c2_small", 1),
-                                       ("c2_small", 2), ("c5_small", 1)])
+                                       ("c2_small", 2), ("c5_small", 1), ("mix_small", 1),
+                                       ("none_small", 1)])
 def test_stages(case, step):
     meta, data = load_case(case)
     print(f"case={case} step={step}", flush=True)
@@ -34,7 +35,8 @@ def test_stages(case, step):
     assert nshift_ok
 
 
-@pytest.mark.parametrize("case", ["c1_small", "c2_small", "c5_small", "c1_iso"])
+@pytest.mark.parametrize("case", ["c1_small", "c2_small", "c5_small", "mix_small", "none_small",
+                                  "c1_iso"])
 def test_full_run(case):
     meta, data = load_case(case)
     print(f"case={case} nsteps={meta['nsteps']}", flush=True)

@@ -8,7 +8,7 @@ from fixtures import (check_stages, compare_state, disk_config, initial_conditio
                       run_case)
 from oracle import OracleDisk
 
-SMALL = ["c1_small", "c2_small", "c5_small"]
+SMALL = ["c1_small", "c2_small", "c5_small", "mix_small", "none_small"]
 
 
 @pytest.mark.parametrize("case", ["c1_small", "c1_iso", "c5_small"])
