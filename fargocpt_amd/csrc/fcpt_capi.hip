@@ -118,7 +118,7 @@ int fcpt_create(const fcpt_disk_desc *desc, fcpt_ctx **out)
 {
     if (!desc || !out) return FCPT_ERR_ARG;
     *out = nullptr;
-    if (desc->nr < 8 || desc->nr > 16384 || desc->nphi < 4 || !desc->radii)
+    if (desc->nr < 8 || desc->nr > 16384 || desc->nphi < 4 || desc->nphi > 262144 || !desc->radii)
         return FCPT_ERR_ARG;
     if (desc->precision != FCPT_PREC_FP64) return FCPT_ERR_ARG;
     for (int i = 0; i + 1 < desc->nr + 2; ++i)
@@ -244,7 +244,7 @@ int fcpt_create(const fcpt_disk_desc *desc, fcpt_ctx **out)
     if (!rc) rc = dev_alloc(c, &P.vmean, (size_t)nr + 1);
     if (!rc) rc = dev_alloc(c, &P.vuni, (size_t)nr + 1);
     if (!rc) rc = dev_alloc(c, &P.nshift, (size_t)nr + 1);
-    if (!rc) rc = dev_alloc(c, &P.ring_min, (size_t)nr + 1);
+    if (!rc) rc = dev_alloc(c, &P.ring_min, (size_t)kCflMaxBlocks);
     if (!rc) rc = dev_alloc(c, &c->s_dev, 1);
     if (!rc) rc = dev_alloc(c, &c->totals_ring, 4 * (size_t)nr);
     if (!rc) rc = dev_alloc(c, &c->totals_out, 2);
@@ -444,7 +444,7 @@ int fcpt_nbody_potential(fcpt_ctx *ctx)
     }
 
 DT_ENTRY(fcpt_update_with_sourceterms, launch_sources(ctx->P, ctx->stream))
-DT_ENTRY(fcpt_artificial_viscosity, launch_artificial_viscosity(ctx->P, ctx->stream))
+DT_ENTRY(fcpt_artificial_viscosity, launch_artificial_viscosity(ctx->P, ctx->stream, 0))
 DT_ENTRY(fcpt_update_velocities_with_viscosity, launch_visc_update(ctx->P, ctx->stream))
 DT_ENTRY(fcpt_transport, launch_transport(ctx->P, ctx->stream))
 
@@ -479,7 +479,7 @@ int fcpt_apply_boundary_condition(fcpt_ctx *ctx, double dt, int32_t final_call)
 {
     CTX_CHECK(ctx);
     launch_set_dt(ctx->P, ctx->stream, dt);
-    launch_boundaries(ctx->P, ctx->stream, final_call);
+    launch_boundaries(ctx->P, ctx->stream, final_call, 0);
     LAUNCH_CHECK(ctx);
     return FCPT_OK;
 }
@@ -507,15 +507,14 @@ static void enqueue_euler(const KParams &P, hipStream_t st, int count_monitor)
 {
     launch_potential(P, st);
     launch_sources(P, st);
-    launch_artificial_viscosity(P, st);
-    launch_recalc_viscosity(P, st);
+    if (!launch_artificial_viscosity(P, st, 1)) launch_recalc_viscosity(P, st);
     launch_stress(P, st);
     launch_visc_update(P, st);
     if (P.adiabatic) launch_substep3(P, st);
-    launch_boundaries(P, st, 0);
+    launch_boundaries(P, st, 0, 0);
     launch_transport(P, st);
-    launch_advance_time(P, st, count_monitor);
-    launch_boundaries(P, st, 1);
+    // the final boundary call also advances the time (and counts the monitor)
+    launch_boundaries(P, st, 1, count_monitor ? 2 : 1);
     launch_derived(P, st);
 }
 

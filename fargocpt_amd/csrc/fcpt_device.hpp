@@ -15,6 +15,7 @@
 namespace fcpt {
 
 constexpr int kMaxPlanets = FCPT_MAX_PLANETS;
+constexpr unsigned kCflMaxBlocks = 4096;  // slots of the block-minimum array
 
 struct DevPlanet {
     double x, y, mass, r_cubic;
@@ -26,8 +27,6 @@ struct DevScalars {
     double time;          // simulation time at the start of the step
     double last_dt;       // sim::last_dt (CFLmaxVar limiter)
     double cfl_dt;        // result of the last CFL reduction
-    double dt_core_init;  // CFL seed from rings 0/1 (cfl.cpp:200-201)
-    unsigned long long dt_min_bits;  // atomic min over non-negative doubles
     int n_monitor;
     int error_flags;      // bit 0: non-finite or non-positive dt
     int n_planets;
@@ -73,7 +72,7 @@ struct KParams {
     double *vmean;       // per-ring mean azimuthal velocity
     double *vuni;        // per-ring uniform residual velocity (FARGO)
     int *nshift;         // per-ring integer shift
-    double *ring_min;    // per-ring partial CFL minima
+    double *ring_min;    // per-block CFL minima (kCflMaxBlocks slots)
     DevScalars *s;
 };
 

@@ -49,7 +49,7 @@ __device__ double block_sum(double v, double *sh)
     return sh[0];
 }
 
-// mode 0: CFL ring mean (also re-arms the CFL minimum)
+// mode 0: CFL ring mean
 // mode 1: transport ring mean + FARGO shift + uniform residual
 //         (TransportEuler.cpp:207-236)
 __global__ void __launch_bounds__(256) k_ring_mean(KParams P, int mode)
@@ -71,28 +71,29 @@ __global__ void __launch_bounds__(256) k_ring_mean(KParams P, int mode)
             const double Nround = floor(Ntilde + 0.5);
             P.nshift[i] = (int)Nround;
             P.vuni[i] = (Ntilde - Nround) * P.Rmed[i] * invdt * P.dphi;
-        } else if (i == 0) {
-            P.s->dt_min_bits = 0x7FF0000000000000ull;  // +inf
         }
     }
 }
 
-// cfl.cpp:210-376, per cell dt and the per-ring shear limit; min reduced with
-// wavefront shuffles and one 64-bit atomic per wavefront (valid for doubles >= 0)
+// cfl.cpp:210-376, per cell dt and the per-ring shear limit.  Each thread
+// walks rows with a grid stride; the block minimum (wavefront shuffles, then
+// LDS) goes to its own slot of P.ring_min and k_cfl_finalize folds the slots,
+// so the result is deterministic and free of atomics.
 __global__ void __launch_bounds__(256) k_cfl_cells(KParams P)
 {
     const int ns = P.ns;
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = 1 + blockIdx.y * blockDim.y + threadIdx.y;
     double cand = __longlong_as_double(0x7FF0000000000000ll);
-    if (j < ns && i < P.nr - 1) {
-        const double *vr = P.f[FCPT_VRAD], *vaz = P.f[FCPT_VAZI];
-        const double CFL = P.cfl, dphi = P.dphi;
-        const double *vm = P.vmean;
+    const double *vr = P.f[FCPT_VRAD], *vaz = P.f[FCPT_VAZI];
+    const double CFL = P.cfl, dphi = P.dphi;
+    const double *vm = P.vmean;
+    const double fac2 = P.artvisc_factor * P.artvisc_factor;
+    for (int i = 1 + blockIdx.y * blockDim.y + threadIdx.y; i < P.nr - 1 && j < ns;
+         i += gridDim.y * blockDim.y) {
         if (j == 0) {
             const double denom =
                 fabs(vm[i] * P.InvRmed[i] - vm[i + 1] * P.InvRmed[i + 1]) + 1.0e-100;
-            cand = CFL * dphi / denom;
+            cand = std_min(cand, CFL * dphi / denom);
             if (i == 1) {
                 const double denom0 =
                     fabs(vm[0] * P.InvRmed[0] - vm[1] * P.InvRmed[1]) + 1.0e-100;
@@ -107,7 +108,6 @@ __global__ void __launch_bounds__(256) k_cfl_cells(KParams P)
         const double invdt2 = vr[IX(i, j)] / dxRadial;
         const double invdt3 = v_residual / dxAzimuthal;
         const int jn = j == ns - 1 ? 0 : j + 1;
-        const double fac2 = P.artvisc_factor * P.artvisc_factor;
         double invdt4;
         if (P.artvisc == FCPT_ARTVISC_SN) {
             double dvRadial = vr[IX(i + 1, j)] - vr[IX(i, j)];
@@ -139,21 +139,33 @@ __global__ void __launch_bounds__(256) k_cfl_cells(KParams P)
         // NaN never wins a comparison, exactly like the reference's "if (dt_cell < dt_core)"
         cand = std_min(cand, dt_cell);
     }
-    // wavefront min (64 lanes)
-    for (int off = 32; off > 0; off >>= 1) {
-        const double o = __shfl_xor(cand, off, 64);
-        cand = std_min(cand, o);
+    for (int off = 32; off > 0; off >>= 1) cand = std_min(cand, __shfl_xor(cand, off, 64));
+    __shared__ double wmin[4];
+    const int t = threadIdx.y * blockDim.x + threadIdx.x;
+    if ((t & 63) == 0) wmin[t >> 6] = cand;
+    __syncthreads();
+    if (t == 0) {
+        const double m = std_min(std_min(wmin[0], wmin[1]), std_min(wmin[2], wmin[3]));
+        P.ring_min[blockIdx.y * gridDim.x + blockIdx.x] = m;
     }
-    const int lane = (threadIdx.y * blockDim.x + threadIdx.x) & 63;
-    if (lane == 0) atomicMin(&P.s->dt_min_bits, (unsigned long long)__double_as_longlong(cand));
 }
 
 // mode 0: condition_cfl only; 1: sim::CalculateTimeStep (simulation.cpp:100-118);
 // 2: CalculateTimeStep + the monitor clip of sim::run (simulation.cpp:526-542)
-__global__ void k_cfl_finalize(KParams P, int mode)
+__global__ void __launch_bounds__(256) k_cfl_finalize(KParams P, int mode, int nblocks)
 {
+    __shared__ double sh[256];
+    double m = __longlong_as_double(0x7FF0000000000000ll);
+    for (int k = threadIdx.x; k < nblocks; k += 256) m = std_min(m, P.ring_min[k]);
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] = std_min(sh[threadIdx.x], sh[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
     DevScalars *s = P.s;
-    const double cfl_dt = __longlong_as_double((long long)s->dt_min_bits);
+    const double cfl_dt = sh[0];
     s->cfl_dt = cfl_dt;
     if (!(cfl_dt > 0.0) || !isfinite(cfl_dt)) s->error_flags |= 1;
     if (mode >= 1) {
@@ -169,16 +181,6 @@ __global__ void k_cfl_finalize(KParams P, int mode)
             s->dt = (overshoot || almost_there) ? time_left_till_write : rv;
         }
     }
-}
-
-// end of a sim::run iteration: time += dt and the monitor counter
-// (simulation.cpp:244, 545-551)
-__global__ void k_advance_time(KParams P, int count_monitor)
-{
-    DevScalars *s = P.s;
-    const double time_next_monitor = (s->n_monitor + 1) * P.monitor_timestep;
-    s->time = s->time + s->dt;
-    if (count_monitor && fabs(time_next_monitor - s->time) < 1e-6 * s->last_dt) s->n_monitor++;
 }
 
 __global__ void k_set_dt(KParams P, double dt) { P.s->dt = dt; }
@@ -303,6 +305,38 @@ __global__ void __launch_bounds__(256) k_temperature_floor(KParams P)
 }
 
 // ---------------------------------------------------------------------------
+// compute_sound_speed / compute_scale_height / update_viscosity for one cell
+// (SourceEuler.cpp:957-995, 1121-1154; viscosity.cpp:98-137)
+__device__ __forceinline__ void derived_cell(const KParams &P, int i, size_t c, bool with_cs_h,
+                                            bool with_temperature, bool with_pressure)
+{
+    const int ns = P.ns;
+    (void)ns;
+    double *cs = P.f[FCPT_SOUNDSPEED], *H = P.f[FCPT_SCALE_HEIGHT];
+    if (P.adiabatic) {
+        const double e = P.f[FCPT_ENERGY][c];
+        const double sig = P.f[FCPT_SIGMA][c];
+        if (with_temperature) {
+            const double c_v_inv = P.mu / P.Rgas * (P.gamma - 1.0);
+            P.f[FCPT_TEMPERATURE][c] = c_v_inv * e / sig;
+        }
+        if (with_cs_h) {
+            const double g = P.gamma;
+            const double c_s = sqrt(g * (g - 1.0) * e / sig);
+            cs[c] = c_s;
+            const double inv_omega_kepler = 1.0 / omega_kepler(P, P.Rmed[i]);
+            H[c] = c_s / (sqrt(g)) * inv_omega_kepler;
+        }
+        if (with_pressure) P.f[FCPT_PRESSURE][c] = (P.gamma - 1.0) * e;
+    } else if (with_pressure) {
+        const double c_s = cs[c];
+        P.f[FCPT_PRESSURE][c] = P.f[FCPT_SIGMA][c] * (c_s * c_s);
+    }
+    const double alpha = P.alpha;
+    P.f[FCPT_VISCOSITY][c] = alpha * H[c] * cs[c];
+}
+
+// ---------------------------------------------------------------------------
 // update_with_artificial_viscosity_TW, artificial_viscosity.cpp:35-88
 __global__ void __launch_bounds__(256) k_tw_q(KParams P)
 {
@@ -337,9 +371,13 @@ __global__ void __launch_bounds__(256) k_tw_q(KParams P)
 }
 
 // artificial_viscosity.cpp:90-139
-__global__ void __launch_bounds__(256) k_tw_vel(KParams P)
+__global__ void __launch_bounds__(256) k_tw_vel(KParams P, int with_viscosity)
 {
-    CELL_LOOP_HEAD(1, P.nr - 1)
+    CELL_LOOP_HEAD(0, P.nr)
+    // in the fused step, recalculate_viscosity (SourceEuler.cpp:205-223), which is
+    // pointwise in Sigma and E and follows this kernel, is done here for every ring
+    if (with_viscosity) derived_cell(P, i, IX(i, j), true, false, false);
+    if (i < 1 || i >= P.nr - 1) return;
     const double *S = P.f[FCPT_SIGMA], *Qr = P.f[FCPT_Q_R], *Qp = P.f[FCPT_Q_PHI];
     const double dt = P.s->dt;
     const int jp = j == 0 ? ns - 1 : j - 1;
@@ -385,9 +423,13 @@ __global__ void __launch_bounds__(256) k_sn_q(KParams P)
 }
 
 // artificial_viscosity.cpp:214-248
-__global__ void __launch_bounds__(256) k_sn_vel(KParams P)
+__global__ void __launch_bounds__(256) k_sn_vel(KParams P, int with_viscosity)
 {
-    CELL_LOOP_HEAD(1, P.nr - 1)
+    CELL_LOOP_HEAD(0, P.nr)
+    // in the fused step, recalculate_viscosity (SourceEuler.cpp:205-223), which is
+    // pointwise in Sigma and E and follows this kernel, is done here for every ring
+    if (with_viscosity) derived_cell(P, i, IX(i, j), true, false, false);
+    if (i < 1 || i >= P.nr - 1) return;
     const double *S = P.f[FCPT_SIGMA], *Qr = P.f[FCPT_Q_R], *Qp = P.f[FCPT_Q_PHI];
     const double dt = P.s->dt;
     if (i >= 2) {
@@ -401,38 +443,6 @@ __global__ void __launch_bounds__(256) k_sn_vel(KParams P)
     double *vaz = P.f[FCPT_VAZI];
     vaz[IX(i, j)] = vaz[IX(i, j)] - dt * 2.0 / (S[IX(i, j)] + S[IX(i, jp)]) *
                                         (Qp[IX(i, j)] - Qp[IX(i, jp)]) * invdxtheta;
-}
-
-// ---------------------------------------------------------------------------
-// compute_sound_speed / compute_scale_height / update_viscosity for one cell
-// (SourceEuler.cpp:957-995, 1121-1154; viscosity.cpp:98-137)
-__device__ __forceinline__ void derived_cell(const KParams &P, int i, size_t c, bool with_cs_h,
-                                            bool with_temperature, bool with_pressure)
-{
-    const int ns = P.ns;
-    (void)ns;
-    double *cs = P.f[FCPT_SOUNDSPEED], *H = P.f[FCPT_SCALE_HEIGHT];
-    if (P.adiabatic) {
-        const double e = P.f[FCPT_ENERGY][c];
-        const double sig = P.f[FCPT_SIGMA][c];
-        if (with_temperature) {
-            const double c_v_inv = P.mu / P.Rgas * (P.gamma - 1.0);
-            P.f[FCPT_TEMPERATURE][c] = c_v_inv * e / sig;
-        }
-        if (with_cs_h) {
-            const double g = P.gamma;
-            const double c_s = sqrt(g * (g - 1.0) * e / sig);
-            cs[c] = c_s;
-            const double inv_omega_kepler = 1.0 / omega_kepler(P, P.Rmed[i]);
-            H[c] = c_s / (sqrt(g)) * inv_omega_kepler;
-        }
-        if (with_pressure) P.f[FCPT_PRESSURE][c] = (P.gamma - 1.0) * e;
-    } else if (with_pressure) {
-        const double c_s = cs[c];
-        P.f[FCPT_PRESSURE][c] = P.f[FCPT_SIGMA][c] * (c_s * c_s);
-    }
-    const double alpha = P.alpha;
-    P.f[FCPT_VISCOSITY][c] = alpha * H[c] * cs[c];
 }
 
 // recalculate_viscosity, SourceEuler.cpp:205-223
@@ -686,10 +696,19 @@ __global__ void __launch_bounds__(256) k_damping(KParams P, int n_inner)
 
 // ghost rings: zero-gradient Sigma/E, reflecting / zero-gradient / outflow v_r,
 // Keplerian v_phi (boundary_conditions.cpp:67-97 and the BC files it dispatches to)
-__global__ void __launch_bounds__(256) k_ghost_bc(KParams P)
+__global__ void __launch_bounds__(256) k_ghost_bc(KParams P, int advance_time)
 {
     const int ns = P.ns;
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (advance_time && j == 0) {
+        // end of a sim::run iteration: time += dt and the monitor counter
+        // (simulation.cpp:244, 545-551); nothing else in this kernel reads them
+        DevScalars *s = P.s;
+        const double time_next_monitor = (s->n_monitor + 1) * P.monitor_timestep;
+        s->time = s->time + s->dt;
+        if (advance_time == 2 && fabs(time_next_monitor - s->time) < 1e-6 * s->last_dt)
+            s->n_monitor++;
+    }
     if (j >= ns) return;
     const int nr = P.nr;
     double *S = P.f[FCPT_SIGMA], *E = P.f[FCPT_ENERGY], *vr = P.f[FCPT_VRAD],
@@ -737,21 +756,6 @@ __global__ void __launch_bounds__(256) k_ghost_bc(KParams P)
 // ---------------------------------------------------------------------------
 // Transport
 
-// compute_momenta_from_velocities, TransportEuler.cpp:471-495
-__global__ void __launch_bounds__(256) k_momenta(KParams P)
-{
-    CELL_LOOP_HEAD(0, P.nr)
-    const double *S = P.f[FCPT_SIGMA], *vr = P.f[FCPT_VRAD], *vaz = P.f[FCPT_VAZI];
-    const double sig = S[IX(i, j)];
-    const int jn = j == ns - 1 ? 0 : j + 1;
-    const double r = P.Rmed[i];
-    const double OmegaF = P.omega_frame;
-    P.mom[0][IX(i, j)] = sig * vr[IX(i + 1, j)];
-    P.mom[1][IX(i, j)] = sig * vr[IX(i, j)];
-    P.mom[2][IX(i, j)] = sig * (vaz[IX(i, jn)] + r * OmegaF) * r;
-    P.mom[3][IX(i, j)] = sig * (vaz[IX(i, j)] + r * OmegaF) * r;
-}
-
 // van Leer slope of a radial profile at ring k (compute_star_radial first loop,
 // TransportEuler.cpp:355-371); q[] holds rings k-1..k+1
 __device__ __forceinline__ double dq_radial(const KParams &P, int k, double qm, double q0, double qp)
@@ -779,19 +783,36 @@ __device__ __forceinline__ double star_radial(const KParams &P, int i, const dou
 // OneWindRad, TransportEuler.cpp:138-168 and VanLeerRadial :545-619: every
 // advected quantity of ring i is updated from rings i-2..i+2 of itself and of
 // the density before the sweep (DENSITY_INT), so all of them are done in one
-// pass that reads inputs and writes separate outputs.
+// pass that reads inputs and writes separate outputs.  The four cell-centred
+// momenta (compute_momenta_from_velocities, :471-495) are formed on the fly
+// from Sigma, v_r and v_phi with the reference's expressions instead of being
+// stored and re-read.
 __global__ void __launch_bounds__(256) k_radial_sweep(KParams P)
 {
     CELL_LOOP_HEAD(0, P.nr)
     const int nr = P.nr;
     const double dt = P.s->dt;
-    const double *S = P.f[FCPT_SIGMA], *vr = P.f[FCPT_VRAD];
-    // density rings i-2..i+2 (values outside [0, nr) are never used by dq)
-    double s5[5];
+    const double *S = P.f[FCPT_SIGMA], *vr = P.f[FCPT_VRAD], *vaz = P.f[FCPT_VAZI];
+    const int jn = j == ns - 1 ? 0 : j + 1;
+    const double OmegaF = P.omega_frame;
+    // rings i-2..i+2 (values outside [0, nr) are never used by the slopes)
+    double s5[5], m5[4][5], e5[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
         const int r = i - 2 + k;
-        s5[k] = (r >= 0 && r < nr) ? S[IX(r, j)] : 1.0;
+        if (r >= 0 && r < nr) {
+            const double sig = S[IX(r, j)];
+            const double rr = P.Rmed[r];
+            s5[k] = sig;
+            m5[0][k] = sig * vr[IX(r + 1, j)];
+            m5[1][k] = sig * vr[IX(r, j)];
+            m5[2][k] = sig * (vaz[IX(r, jn)] + rr * OmegaF) * rr;
+            m5[3][k] = sig * (vaz[IX(r, j)] + rr * OmegaF) * rr;
+            e5[k] = P.adiabatic ? P.f[FCPT_ENERGY][IX(r, j)] : 0.0;
+        } else {
+            s5[k] = 1.0;
+            m5[0][k] = m5[1][k] = m5[2][k] = m5[3][k] = e5[k] = 0.0;
+        }
     }
     const double v0 = vr[IX(i, j)], v1 = vr[IX(i + 1, j)];
     // DensityStar at interfaces i and i+1 (row nr is never written: 0)
@@ -799,31 +820,34 @@ __global__ void __launch_bounds__(256) k_radial_sweep(KParams P)
     const double ds1 = star_radial(P, i + 1, s5 + 1, v1, dt);
     const double c_inf = dt * P.dphi * P.Rinf[i];
     const double c_sup = dt * P.dphi * P.Rsup[i];
+    const double inv_surf = P.InvSurf[i];
     const int nq = P.adiabatic ? 6 : 5;
-    for (int q = 0; q < nq; ++q) {
-        const double *Q;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        if (q >= nq) break;
+        const double *Q5;
         double *out;
         if (q < 4) {
-            Q = P.mom[q];
+            Q5 = m5[q];
             out = P.mom_b[q];
         } else if (q == 4 && P.adiabatic) {
-            Q = P.f[FCPT_ENERGY];
+            Q5 = e5;
             out = P.en_b;
         } else {
-            Q = S;
+            Q5 = s5;
             out = P.sig_b;
         }
         double w[5];
 #pragma unroll
         for (int k = 0; k < 5; ++k) {
             const int r = i - 2 + k;
-            w[k] = (r >= 0 && r < nr) ? Q[IX(r, j)] / s5[k] : 0.0;
+            w[k] = (r >= 0 && r < nr) ? Q5[k] / s5[k] : 0.0;
         }
         const double qs0 = star_radial(P, i, w, v0, dt);
         const double qs1 = star_radial(P, i + 1, w + 1, v1, dt);
         const double varq_inf = c_inf * qs0 * ds0 * v0;
         const double varq_sup = c_sup * qs1 * ds1 * v1;
-        out[IX(i, j)] = Q[IX(i, j)] + (varq_inf - varq_sup) * P.InvSurf[i];
+        out[IX(i, j)] = Q5[2] + (varq_inf - varq_sup) * inv_surf;
     }
 }
 
@@ -1018,11 +1042,24 @@ void launch_ring_mean(const KParams &P, hipStream_t st, int mode)
     hipLaunchKernelGGL(k_ring_mean, dim3(P.nr), dim3(256), 0, st, P, mode);
 }
 
+// CFL grid: at most kCflMaxBlocks blocks, rows walked with a grid stride
+static dim3 cfl_grid(const KParams &P)
+{
+    const unsigned gx = (unsigned)((P.ns + 63) / 64);
+    unsigned gy = (unsigned)((P.nr - 2 + 3) / 4);
+    unsigned cap = kCflMaxBlocks / gx;
+    if (cap < 1) cap = 1;
+    if (gy > cap) gy = cap;
+    if (gy < 1) gy = 1;
+    return dim3(gx, gy);
+}
+
 void launch_cfl(const KParams &P, hipStream_t st, int mode)
 {
     launch_ring_mean(P, st, 0);
-    if (P.nr > 2) hipLaunchKernelGGL(k_cfl_cells, cell_grid(P, P.nr - 2), kBlock, 0, st, P);
-    hipLaunchKernelGGL(k_cfl_finalize, dim3(1), dim3(1), 0, st, P, mode);
+    const dim3 g = cfl_grid(P);
+    hipLaunchKernelGGL(k_cfl_cells, g, kBlock, 0, st, P);
+    hipLaunchKernelGGL(k_cfl_finalize, dim3(1), dim3(256), 0, st, P, mode, (int)(g.x * g.y));
 }
 
 void launch_set_dt(const KParams &P, hipStream_t st, double dt)
@@ -1041,10 +1078,6 @@ void launch_set_params(const KParams &P, hipStream_t st, const fcpt_step_params 
 {
     hipLaunchKernelGGL(k_set_params, dim3(1), dim3(1), 0, st, P, sp);
 }
-void launch_advance_time(const KParams &P, hipStream_t st, int count_monitor)
-{
-    hipLaunchKernelGGL(k_advance_time, dim3(1), dim3(1), 0, st, P, count_monitor);
-}
 
 void launch_potential(const KParams &P, hipStream_t st)
 {
@@ -1058,17 +1091,22 @@ void launch_sources(const KParams &P, hipStream_t st)
     if (P.adiabatic) hipLaunchKernelGGL(k_src_energy, cell_grid(P, P.nr - 1), kBlock, 0, st, P);
 }
 
-void launch_artificial_viscosity(const KParams &P, hipStream_t st)
+// returns true when the viscosity recalculation was folded in
+bool launch_artificial_viscosity(const KParams &P, hipStream_t st, int with_viscosity)
 {
     if (P.artvisc == FCPT_ARTVISC_TW) {
         hipLaunchKernelGGL(k_tw_q, cell_grid(P, P.nr), kBlock, 0, st, P);
-        hipLaunchKernelGGL(k_tw_vel, cell_grid(P, P.nr - 2), kBlock, 0, st, P);
-    } else if (P.artvisc == FCPT_ARTVISC_SN) {
-        hipLaunchKernelGGL(k_sn_q, cell_grid(P, P.nr), kBlock, 0, st, P);
-        hipLaunchKernelGGL(k_sn_vel, cell_grid(P, P.nr - 2), kBlock, 0, st, P);
-    } else if (P.adiabatic && P.artvisc_dissipation) {
-        hipLaunchKernelGGL(k_temperature_floor, cell_grid(P, P.nr), kBlock, 0, st, P);
+        hipLaunchKernelGGL(k_tw_vel, cell_grid(P, P.nr), kBlock, 0, st, P, with_viscosity);
+        return with_viscosity != 0;
     }
+    if (P.artvisc == FCPT_ARTVISC_SN) {
+        hipLaunchKernelGGL(k_sn_q, cell_grid(P, P.nr), kBlock, 0, st, P);
+        hipLaunchKernelGGL(k_sn_vel, cell_grid(P, P.nr), kBlock, 0, st, P, with_viscosity);
+        return with_viscosity != 0;
+    }
+    if (P.adiabatic && P.artvisc_dissipation)
+        hipLaunchKernelGGL(k_temperature_floor, cell_grid(P, P.nr), kBlock, 0, st, P);
+    return false;
 }
 
 void launch_recalc_viscosity(const KParams &P, hipStream_t st)
@@ -1096,7 +1134,7 @@ void launch_heatcool_cfl(const KParams &P, hipStream_t st)
     hipLaunchKernelGGL(k_heatcool_cfl, cell_grid(P, P.nr), kBlock, 0, st, P);
 }
 
-void launch_boundaries(const KParams &P, hipStream_t st, int final_call)
+void launch_boundaries(const KParams &P, hipStream_t st, int final_call, int advance_time)
 {
     if (final_call && P.damping) {
         const int n_inner = (P.damp_ids[0] > P.damp_ids[1] ? P.damp_ids[0] : P.damp_ids[1]) + 1;
@@ -1104,7 +1142,8 @@ void launch_boundaries(const KParams &P, hipStream_t st, int final_call)
         const int n_outer = P.nr + 1 - o0;
         hipLaunchKernelGGL(k_damping, cell_grid(P, n_inner + n_outer), kBlock, 0, st, P, n_inner);
     }
-    hipLaunchKernelGGL(k_ghost_bc, dim3((unsigned)((P.ns + 255) / 256)), dim3(256), 0, st, P);
+    hipLaunchKernelGGL(k_ghost_bc, dim3((unsigned)((P.ns + 255) / 256)), dim3(256), 0, st, P,
+                       advance_time);
 }
 
 void launch_radial_sweep(const KParams &P, hipStream_t st)
@@ -1126,12 +1165,11 @@ void launch_copy4(const KParams &P, hipStream_t st)
 
 void launch_cfl_cells(const KParams &P, hipStream_t st)
 {
-    if (P.nr > 2) hipLaunchKernelGGL(k_cfl_cells, cell_grid(P, P.nr - 2), kBlock, 0, st, P);
+    hipLaunchKernelGGL(k_cfl_cells, cfl_grid(P), kBlock, 0, st, P);
 }
 
 void launch_transport(const KParams &P, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_momenta, cell_grid(P, P.nr), kBlock, 0, st, P);
     hipLaunchKernelGGL(k_radial_sweep, cell_grid(P, P.nr), kBlock, 0, st, P);
     launch_ring_mean(P, st, 1);
     hipLaunchKernelGGL(k_azimuthal_pass, cell_grid(P, P.nr), kBlock, 0, st, P, 0);

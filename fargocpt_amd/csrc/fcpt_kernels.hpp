@@ -14,16 +14,15 @@ void launch_set_dt(const KParams &P, hipStream_t st, double dt);
 void launch_set_clock(const KParams &P, hipStream_t st, double t);
 void launch_set_time(const KParams &P, hipStream_t st, double t, double last_dt, int n_monitor);
 void launch_set_params(const KParams &P, hipStream_t st, const fcpt_step_params &sp);
-void launch_advance_time(const KParams &P, hipStream_t st, int count_monitor);
 void launch_potential(const KParams &P, hipStream_t st);
 void launch_sources(const KParams &P, hipStream_t st);
-void launch_artificial_viscosity(const KParams &P, hipStream_t st);
+bool launch_artificial_viscosity(const KParams &P, hipStream_t st, int with_viscosity);
 void launch_recalc_viscosity(const KParams &P, hipStream_t st);
 void launch_stress(const KParams &P, hipStream_t st);
 void launch_visc_update(const KParams &P, hipStream_t st);
 void launch_substep3(const KParams &P, hipStream_t st);
 void launch_heatcool_cfl(const KParams &P, hipStream_t st);
-void launch_boundaries(const KParams &P, hipStream_t st, int final_call);
+void launch_boundaries(const KParams &P, hipStream_t st, int final_call, int advance_time);
 void launch_transport(const KParams &P, hipStream_t st);
 void launch_radial_sweep(const KParams &P, hipStream_t st);
 void launch_azimuthal_pass(const KParams &P, hipStream_t st, int pass);

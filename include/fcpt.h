@@ -63,8 +63,9 @@ typedef enum fcpt_field {
  * after the reference's parameters:: / constants:: globals they stand for.
  */
 typedef struct fcpt_disk_desc {
-    int32_t nr;                 /* NRadial, including both ghost rings (>= 8) */
-    int32_t nphi;               /* NAzimuthal (>= 4) */
+    int32_t nr;                 /* NRadial, including both ghost rings (8 .. 16384, the
+                                   reference's MAX1D) */
+    int32_t nphi;               /* NAzimuthal (4 .. 262144) */
     const double *radii;        /* Radii[0 .. nr+1]: nr+2 interface radii; the last one
                                    is only read through the reference's Rmed[nr] */
     double rmin, rmax;          /* configured domain edges (the reference's RMIN, RMAX
