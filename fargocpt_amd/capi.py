@@ -23,8 +23,7 @@ FIELD_ID = {name: i for i, name in enumerate(FIELDS)}
 VECTOR_FIELDS = {"VRAD", "VRAD0", "TAU_R_PHI"}
 
 ARTVISC = {"none": 0, "tw": 1, "sn": 2}
-KERNELS = {"radial_sweep": 0, "azimuthal_residual": 1, "azimuthal_shift": 2, "cfl": 3,
-           "stress": 4, "potential": 5}
+KERNELS = {"radial_sweep": 0, "azimuthal_sweep": 1, "cfl": 3, "stress": 4, "potential": 5}
 BC = {"reflecting": 0, "zerogradient": 1, "outflow": 2}
 MAX_PLANETS = 8
 

@@ -241,6 +241,8 @@ int fcpt_create(const fcpt_disk_desc *desc, fcpt_ctx **out)
     for (int k = 0; k < 4 && !rc; ++k) rc = dev_alloc(c, &P.mom_b[k], ncell);
     if (!rc) rc = dev_alloc(c, &P.sig_b, ncell);
     if (!rc) rc = dev_alloc(c, &P.en_b, ncell);
+    if (!rc) rc = dev_alloc(c, &P.sig_c, ncell);
+    if (!rc) rc = dev_alloc(c, &P.en_c, ncell);
     if (!rc) rc = dev_alloc(c, &P.vmean, (size_t)nr + 1);
     if (!rc) rc = dev_alloc(c, &P.vuni, (size_t)nr + 1);
     if (!rc) rc = dev_alloc(c, &P.nshift, (size_t)nr + 1);
@@ -618,15 +620,15 @@ int fcpt_time_kernel(fcpt_ctx *ctx, int32_t kernel, int32_t reps, double *ms_per
     auto one = [&]() {
         switch (kernel) {
         case FCPT_KERNEL_RADIAL_SWEEP: launch_radial_sweep(ctx->P, ctx->stream); break;
-        case FCPT_KERNEL_AZIMUTHAL_RESIDUAL: launch_azimuthal_pass(ctx->P, ctx->stream, 0); break;
-        case FCPT_KERNEL_AZIMUTHAL_SHIFT: launch_azimuthal_pass(ctx->P, ctx->stream, 1); break;
+        case FCPT_KERNEL_AZIMUTHAL: launch_azimuthal_sweep(ctx->P, ctx->stream); break;
         case FCPT_KERNEL_CFL: launch_cfl_cells(ctx->P, ctx->stream); break;
         case FCPT_KERNEL_STRESS: launch_stress(ctx->P, ctx->stream); break;
         case FCPT_KERNEL_POTENTIAL: launch_potential(ctx->P, ctx->stream); break;
         default: break;
         }
     };
-    if (kernel < 0 || kernel > FCPT_KERNEL_POTENTIAL) return fail(ctx, FCPT_ERR_ARG, "unknown kernel");
+    if (kernel < 0 || kernel > FCPT_KERNEL_POTENTIAL || kernel == 2)
+        return fail(ctx, FCPT_ERR_ARG, "unknown kernel");
     one();  // warm-up
     HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     for (int k = 0; k < reps; ++k) one();

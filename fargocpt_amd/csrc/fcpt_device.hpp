@@ -68,7 +68,8 @@ struct KParams {
     // transport work arrays
     double *mom[4];      // radial momentum +/-, angular momentum +/-
     double *mom_b[4];    // ping-pong partners
-    double *sig_b, *en_b;
+    double *sig_b, *en_b;   // radial-sweep outputs
+    double *sig_c, *en_c;   // azimuthal-sweep outputs
     double *vmean;       // per-ring mean azimuthal velocity
     double *vuni;        // per-ring uniform residual velocity (FARGO)
     int *nshift;         // per-ring integer shift

@@ -780,74 +780,84 @@ __device__ __forceinline__ double star_radial(const KParams &P, int i, const dou
     return w[2] - (P.Rmed[i + 1] - P.Rmed[i] + v * dt) * 0.5 * dq;
 }
 
-// OneWindRad, TransportEuler.cpp:138-168 and VanLeerRadial :545-619: every
-// advected quantity of ring i is updated from rings i-2..i+2 of itself and of
-// the density before the sweep (DENSITY_INT), so all of them are done in one
-// pass that reads inputs and writes separate outputs.  The four cell-centred
-// momenta (compute_momenta_from_velocities, :471-495) are formed on the fly
-// from Sigma, v_r and v_phi with the reference's expressions instead of being
-// stored and re-read.
+// OneWindRad, TransportEuler.cpp:138-168 and VanLeerRadial :545-619, as one
+// LDS-tiled pass.  Every advected quantity of ring i is updated from rings
+// i-2..i+2 of itself and of the density before the sweep (DENSITY_INT), so a
+// block stages 64 sectors x (TR + 4) rings of Sigma and of each quantity's
+// specific value Q/Sigma in LDS, forms every interface value once, and writes
+// the TR updated rings to separate outputs.  The four cell-centred momenta
+// (compute_momenta_from_velocities, :471-495) are formed on the fly from
+// Sigma, v_r and v_phi with the reference's expressions.
+template <int TR>
 __global__ void __launch_bounds__(256) k_radial_sweep(KParams P)
 {
-    CELL_LOOP_HEAD(0, P.nr)
-    const int nr = P.nr;
-    const double dt = P.s->dt;
-    const double *S = P.f[FCPT_SIGMA], *vr = P.f[FCPT_VRAD], *vaz = P.f[FCPT_VAZI];
+    constexpr int R = TR + 4;  // rings i0-2 .. i0+TR+1
+    __shared__ double sS[R][64], sW[R][64], sQ[R][64];
+    __shared__ double sDS[TR + 1][64], sQS[TR + 1][64], sV[TR + 1][64];
+    const int ns = P.ns, nr = P.nr;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int j = blockIdx.x * 64 + tx;
+    const int i0 = blockIdx.y * TR;
+    const bool col = j < ns;
     const int jn = j == ns - 1 ? 0 : j + 1;
+    const double dt = P.s->dt;
+    const double *S = P.f[FCPT_SIGMA], *vr = P.f[FCPT_VRAD], *vaz = P.f[FCPT_VAZI],
+                 *E = P.f[FCPT_ENERGY];
     const double OmegaF = P.omega_frame;
-    // rings i-2..i+2 (values outside [0, nr) are never used by the slopes)
-    double s5[5], m5[4][5], e5[5];
-#pragma unroll
-    for (int k = 0; k < 5; ++k) {
-        const int r = i - 2 + k;
-        if (r >= 0 && r < nr) {
-            const double sig = S[IX(r, j)];
-            const double rr = P.Rmed[r];
-            s5[k] = sig;
-            m5[0][k] = sig * vr[IX(r + 1, j)];
-            m5[1][k] = sig * vr[IX(r, j)];
-            m5[2][k] = sig * (vaz[IX(r, jn)] + rr * OmegaF) * rr;
-            m5[3][k] = sig * (vaz[IX(r, j)] + rr * OmegaF) * rr;
-            e5[k] = P.adiabatic ? P.f[FCPT_ENERGY][IX(r, j)] : 0.0;
-        } else {
-            s5[k] = 1.0;
-            m5[0][k] = m5[1][k] = m5[2][k] = m5[3][k] = e5[k] = 0.0;
-        }
+    for (int p = ty; p < R; p += 4) {
+        const int r = i0 - 2 + p;
+        sS[p][tx] = (col && r >= 0 && r < nr) ? S[IX(r, j)] : 1.0;
     }
-    const double v0 = vr[IX(i, j)], v1 = vr[IX(i + 1, j)];
-    // DensityStar at interfaces i and i+1 (row nr is never written: 0)
-    const double ds0 = star_radial(P, i, s5, v0, dt);
-    const double ds1 = star_radial(P, i + 1, s5 + 1, v1, dt);
-    const double c_inf = dt * P.dphi * P.Rinf[i];
-    const double c_sup = dt * P.dphi * P.Rsup[i];
-    const double inv_surf = P.InvSurf[i];
+    for (int p = ty; p < TR + 1; p += 4) {
+        const int r = i0 + p;
+        sV[p][tx] = (col && r <= nr) ? vr[IX(r, j)] : 0.0;
+    }
+    __syncthreads();
+    // DensityStar at interfaces i0 .. i0+TR (interface nr is never written: 0)
+    for (int p = ty; p < TR + 1; p += 4) {
+        const double w4[4] = {sS[p][tx], sS[p + 1][tx], sS[p + 2][tx], sS[p + 3][tx]};
+        sDS[p][tx] = star_radial(P, i0 + p, w4, sV[p][tx], dt);
+    }
     const int nq = P.adiabatic ? 6 : 5;
-#pragma unroll
-    for (int q = 0; q < 6; ++q) {
-        if (q >= nq) break;
-        const double *Q5;
+    for (int q = 0; q < nq; ++q) {
         double *out;
-        if (q < 4) {
-            Q5 = m5[q];
-            out = P.mom_b[q];
-        } else if (q == 4 && P.adiabatic) {
-            Q5 = e5;
-            out = P.en_b;
-        } else {
-            Q5 = s5;
-            out = P.sig_b;
+        if (q < 4) out = P.mom_b[q];
+        else if (q == 4 && P.adiabatic) out = P.en_b;
+        else out = P.sig_b;
+        // quantity and specific quantity on the staged rings
+        for (int p = ty; p < R; p += 4) {
+            const int r = i0 - 2 + p;
+            double Qv = 0.0, w = 0.0;
+            if (col && r >= 0 && r < nr) {
+                const double sig = sS[p][tx];
+                const double rr = P.Rmed[r];
+                if (q == 0) Qv = sig * vr[IX(r + 1, j)];
+                else if (q == 1) Qv = sig * vr[IX(r, j)];
+                else if (q == 2) Qv = sig * (vaz[IX(r, jn)] + rr * OmegaF) * rr;
+                else if (q == 3) Qv = sig * (vaz[IX(r, j)] + rr * OmegaF) * rr;
+                else if (q == 4 && P.adiabatic) Qv = E[IX(r, j)];
+                else Qv = sig;
+                w = Qv / sig;
+            }
+            sQ[p][tx] = Qv;
+            sW[p][tx] = w;
         }
-        double w[5];
-#pragma unroll
-        for (int k = 0; k < 5; ++k) {
-            const int r = i - 2 + k;
-            w[k] = (r >= 0 && r < nr) ? Q5[k] / s5[k] : 0.0;
+        __syncthreads();
+        for (int p = ty; p < TR + 1; p += 4) {
+            const double w4[4] = {sW[p][tx], sW[p + 1][tx], sW[p + 2][tx], sW[p + 3][tx]};
+            sQS[p][tx] = star_radial(P, i0 + p, w4, sV[p][tx], dt);
         }
-        const double qs0 = star_radial(P, i, w, v0, dt);
-        const double qs1 = star_radial(P, i + 1, w + 1, v1, dt);
-        const double varq_inf = c_inf * qs0 * ds0 * v0;
-        const double varq_sup = c_sup * qs1 * ds1 * v1;
-        out[IX(i, j)] = Q5[2] + (varq_inf - varq_sup) * inv_surf;
+        __syncthreads();
+        for (int p = ty; p < TR; p += 4) {
+            const int i = i0 + p;
+            if (col && i < nr) {
+                const double varq_inf = dt * P.dphi * P.Rinf[i] * sQS[p][tx] * sDS[p][tx] * sV[p][tx];
+                const double varq_sup =
+                    dt * P.dphi * P.Rsup[i] * sQS[p + 1][tx] * sDS[p + 1][tx] * sV[p + 1][tx];
+                out[IX(i, j)] = sQ[p + 2][tx] + (varq_inf - varq_sup) * P.InvSurf[i];
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -864,71 +874,111 @@ __device__ __forceinline__ double star_theta(const double *q, double dxtheta, do
     return q[2] - (dxtheta + ksi) * dq;
 }
 
-// QuantitiesAdvection (TransportEuler.cpp:292-304) with VanLeerTheta (:630-664).
-// pass 0: residual velocity v - <v>; inputs = radial-sweep outputs, outputs = fields
-// pass 1: uniform residual per ring, followed by AdvectSHIFT (:238-268): the value
-//         of sector j is written to sector j + Nshift of the ping-pong buffer
-__global__ void __launch_bounds__(256) k_azimuthal_pass(KParams P, int pass)
+// OneWindTheta, TransportEuler.cpp:270-289, in one pass per ring segment:
+// QuantitiesAdvection (:292-304, VanLeerTheta :630-664) with the residual
+// velocity v - <v>, then with the uniform residual of the ring, then
+// AdvectSHIFT (:238-268).  A block owns kAzSeg sectors of one ring and stages
+// them with a halo of 4 sectors in LDS (2 per van Leer pass); the result of
+// sector j is written to sector j + Nshift.  The density is advanced first
+// because the second pass needs its first-pass value as DENSITY_INT; the other
+// quantities only read the density, so their order does not matter.
+constexpr int kAzSeg = 256;
+constexpr int kAzHalo = 4;
+constexpr int kAzExt = kAzSeg + 2 * kAzHalo;
+
+__global__ void __launch_bounds__(256) k_azimuthal_sweep(KParams P)
 {
-    CELL_LOOP_HEAD(0, P.nr)
+    __shared__ double sS0[kAzExt], sV0[kAzExt], sDS0[kAzExt], sS1[kAzExt], sDS1[kAzExt];
+    __shared__ double sQ0[kAzExt], sW[kAzExt], sQS[kAzExt], sQ1[kAzExt];
+    const int ns = P.ns;
+    const int i = blockIdx.y;
+    const int j0 = blockIdx.x * kAzSeg;
+    const int Ls = ns - j0 < kAzSeg ? ns - j0 : kAzSeg;
+    const int next = Ls + 2 * kAzHalo;  // staged positions p: sector j0 - 4 + p
+    const int t = threadIdx.x;
     const double dt = P.s->dt;
-    const double *Sin = pass == 0 ? P.sig_b : P.f[FCPT_SIGMA];
-    int js[5];
-#pragma unroll
-    for (int k = 0; k < 5; ++k) {
-        int jj = j - 2 + k;
-        if (jj < 0) jj += ns;
-        if (jj >= ns) jj -= ns;
-        js[k] = jj;
-    }
-    // velocities at sectors j and j+1
-    double v0, v1;
-    if (pass == 0) {
-        const double *vaz = P.f[FCPT_VAZI];
-        const double vm = P.vmean[i];
-        v0 = vaz[IX(i, js[2])] - vm;
-        v1 = vaz[IX(i, js[3])] - vm;
-    } else {
-        v0 = P.vuni[i];
-        v1 = v0;
-    }
+    const double vm = P.vmean[i];
+    const double vu = P.vuni[i];
+    const int nsh = P.nshift[i];
     const double dxtheta = P.dphi * P.Rmed[i];
     const double invdxtheta = 1.0 / dxtheta;
-    double s5[5];
-#pragma unroll
-    for (int k = 0; k < 5; ++k) s5[k] = Sin[IX(i, js[k])];
-    const double ds0 = star_theta(s5, dxtheta, invdxtheta, v0 * dt);
-    const double ds1 = star_theta(s5 + 1, dxtheta, invdxtheta, v1 * dt);
     const double dxrad = (P.Rsup[i] - P.Rinf[i]) * dt;
     const double invsurf = P.InvSurf[i];
-    int jdst = j;
-    if (pass == 1) {
-        jdst = j + P.nshift[i];
-        while (jdst < 0) jdst += ns;
-        while (jdst >= ns) jdst -= ns;
+    const size_t row = (size_t)i * ns;
+    auto sector = [ns](int j) {
+        if (j < 0) j += ns;
+        if (j >= ns) j -= ns;
+        return j;
+    };
+    auto dst_sector = [ns](int j) {
+        while (j < 0) j += ns;
+        while (j >= ns) j -= ns;
+        return j;
+    };
+    const double *vaz = P.f[FCPT_VAZI];
+    for (int p = t; p < next; p += 256) {
+        const int j = sector(j0 - kAzHalo + p);
+        sS0[p] = P.sig_b[row + j];
+        sV0[p] = vaz[row + j] - vm;
     }
-    const int nq = P.adiabatic ? 6 : 5;
+    __syncthreads();
+    // first pass (residual velocity): DensityStar on interfaces 2 .. next-2
+    for (int p = 2 + t; p <= next - 2; p += 256) sDS0[p] = star_theta(&sS0[p - 2], dxtheta, invdxtheta, sV0[p] * dt);
+    // the density's own first pass
+    for (int p = t; p < next; p += 256) sW[p] = sS0[p] / sS0[p];
+    __syncthreads();
+    for (int p = 2 + t; p <= next - 2; p += 256) sQS[p] = star_theta(&sW[p - 2], dxtheta, invdxtheta, sV0[p] * dt);
+    __syncthreads();
+    for (int p = 2 + t; p < next - 2; p += 256) {
+        double varq = dxrad * sQS[p] * sDS0[p] * sV0[p];
+        varq -= dxrad * sQS[p + 1] * sDS0[p + 1] * sV0[p + 1];
+        sS1[p] = sS0[p] + varq * invsurf;
+    }
+    __syncthreads();
+    // second pass (uniform residual): DensityStar on interfaces 4 .. 4+Ls
+    for (int p = kAzHalo + t; p <= kAzHalo + Ls; p += 256)
+        sDS1[p] = star_theta(&sS1[p - 2], dxtheta, invdxtheta, vu * dt);
+    for (int p = 2 + t; p < next - 2; p += 256) sW[p] = sS1[p] / sS1[p];
+    __syncthreads();
+    for (int p = kAzHalo + t; p <= kAzHalo + Ls; p += 256)
+        sQS[p] = star_theta(&sW[p - 2], dxtheta, invdxtheta, vu * dt);
+    __syncthreads();
+    for (int p = kAzHalo + t; p < kAzHalo + Ls; p += 256) {
+        double varq = dxrad * sQS[p] * sDS1[p] * vu;
+        varq -= dxrad * sQS[p + 1] * sDS1[p + 1] * vu;
+        P.sig_c[row + dst_sector(j0 - kAzHalo + p + nsh)] = sS1[p] + varq * invsurf;
+    }
+    // the other quantities
+    const int nq = P.adiabatic ? 5 : 4;
     for (int q = 0; q < nq; ++q) {
-        const double *Q;
-        double *out;
-        if (q < 4) {
-            Q = pass == 0 ? P.mom_b[q] : P.mom[q];
-            out = pass == 0 ? P.mom[q] : P.mom_b[q];
-        } else if (q == 4 && P.adiabatic) {
-            Q = pass == 0 ? P.en_b : P.f[FCPT_ENERGY];
-            out = pass == 0 ? P.f[FCPT_ENERGY] : P.en_b;
-        } else {
-            Q = Sin;
-            out = pass == 0 ? P.f[FCPT_SIGMA] : P.sig_b;
+        const double *Qin = q < 4 ? P.mom_b[q] : P.en_b;
+        double *out = q < 4 ? P.mom[q] : P.en_c;
+        __syncthreads();
+        for (int p = t; p < next; p += 256) {
+            const double Qv = Qin[row + sector(j0 - kAzHalo + p)];
+            sQ0[p] = Qv;
+            sW[p] = Qv / sS0[p];
         }
-        double w[5];
-#pragma unroll
-        for (int k = 0; k < 5; ++k) w[k] = Q[IX(i, js[k])] / s5[k];
-        const double qs0 = star_theta(w, dxtheta, invdxtheta, v0 * dt);
-        const double qs1 = star_theta(w + 1, dxtheta, invdxtheta, v1 * dt);
-        double varq = dxrad * qs0 * ds0 * v0;
-        varq -= dxrad * qs1 * ds1 * v1;
-        out[IX(i, jdst)] = Q[IX(i, j)] + varq * invsurf;
+        __syncthreads();
+        for (int p = 2 + t; p <= next - 2; p += 256)
+            sQS[p] = star_theta(&sW[p - 2], dxtheta, invdxtheta, sV0[p] * dt);
+        __syncthreads();
+        for (int p = 2 + t; p < next - 2; p += 256) {
+            double varq = dxrad * sQS[p] * sDS0[p] * sV0[p];
+            varq -= dxrad * sQS[p + 1] * sDS0[p + 1] * sV0[p + 1];
+            const double q1 = sQ0[p] + varq * invsurf;
+            sQ1[p] = q1;
+            sW[p] = q1 / sS1[p];
+        }
+        __syncthreads();
+        for (int p = kAzHalo + t; p <= kAzHalo + Ls; p += 256)
+            sQS[p] = star_theta(&sW[p - 2], dxtheta, invdxtheta, vu * dt);
+        __syncthreads();
+        for (int p = kAzHalo + t; p < kAzHalo + Ls; p += 256) {
+            double varq = dxrad * sQS[p] * sDS1[p] * vu;
+            varq -= dxrad * sQS[p + 1] * sDS1[p + 1] * vu;
+            out[row + dst_sector(j0 - kAzHalo + p + nsh)] = sQ1[p] + varq * invsurf;
+        }
     }
 }
 
@@ -938,22 +988,22 @@ __global__ void __launch_bounds__(256) k_azimuthal_pass(KParams P, int pass)
 __global__ void __launch_bounds__(256) k_velocities(KParams P)
 {
     CELL_LOOP_HEAD(0, P.nr)
-    const double *Sb = P.sig_b;
+    const double *Sb = P.sig_c;
     const int jm = j == 0 ? ns - 1 : j - 1;
     if (i == 0) {
         P.f[FCPT_VRAD][IX(i, j)] = 0.0;
     } else {
-        P.f[FCPT_VRAD][IX(i, j)] = (P.mom_b[0][IX(i - 1, j)] + P.mom_b[1][IX(i, j)]) /
+        P.f[FCPT_VRAD][IX(i, j)] = (P.mom[0][IX(i - 1, j)] + P.mom[1][IX(i, j)]) /
                                    (Sb[IX(i - 1, j)] + Sb[IX(i, j)]);
     }
-    P.f[FCPT_VAZI][IX(i, j)] = (P.mom_b[2][IX(i, jm)] + P.mom_b[3][IX(i, j)]) /
+    P.f[FCPT_VAZI][IX(i, j)] = (P.mom[2][IX(i, jm)] + P.mom[3][IX(i, j)]) /
                                    (Sb[IX(i, jm)] + Sb[IX(i, j)]) * P.InvRmed[i] -
                                P.Rmed[i] * P.omega_frame;
     const double floor_value = P.sigma_floor * P.sigma0;
     double sig = Sb[IX(i, j)];
     if (sig < floor_value) sig = floor_value;
     P.f[FCPT_SIGMA][IX(i, j)] = sig;
-    if (P.adiabatic) P.f[FCPT_ENERGY][IX(i, j)] = temperature_floor(P, P.en_b[IX(i, j)], sig);
+    if (P.adiabatic) P.f[FCPT_ENERGY][IX(i, j)] = temperature_floor(P, P.en_c[IX(i, j)], sig);
 }
 
 // ---------------------------------------------------------------------------
@@ -1146,13 +1196,16 @@ void launch_boundaries(const KParams &P, hipStream_t st, int final_call, int adv
                        advance_time);
 }
 
+constexpr int kRadialTile = 8;
 void launch_radial_sweep(const KParams &P, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_radial_sweep, cell_grid(P, P.nr), kBlock, 0, st, P);
+    const dim3 g((unsigned)((P.ns + 63) / 64), (unsigned)((P.nr + kRadialTile - 1) / kRadialTile));
+    hipLaunchKernelGGL(k_radial_sweep<kRadialTile>, g, kBlock, 0, st, P);
 }
-void launch_azimuthal_pass(const KParams &P, hipStream_t st, int pass)
+void launch_azimuthal_sweep(const KParams &P, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_azimuthal_pass, cell_grid(P, P.nr), kBlock, 0, st, P, pass);
+    const dim3 g((unsigned)((P.ns + kAzSeg - 1) / kAzSeg), (unsigned)P.nr);
+    hipLaunchKernelGGL(k_azimuthal_sweep, g, dim3(256), 0, st, P);
 }
 void launch_copy4(const KParams &P, hipStream_t st)
 {
@@ -1170,10 +1223,9 @@ void launch_cfl_cells(const KParams &P, hipStream_t st)
 
 void launch_transport(const KParams &P, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_radial_sweep, cell_grid(P, P.nr), kBlock, 0, st, P);
+    launch_radial_sweep(P, st);
     launch_ring_mean(P, st, 1);
-    hipLaunchKernelGGL(k_azimuthal_pass, cell_grid(P, P.nr), kBlock, 0, st, P, 0);
-    hipLaunchKernelGGL(k_azimuthal_pass, cell_grid(P, P.nr), kBlock, 0, st, P, 1);
+    launch_azimuthal_sweep(P, st);
     hipLaunchKernelGGL(k_velocities, cell_grid(P, P.nr), kBlock, 0, st, P);
 }
 

@@ -25,7 +25,7 @@ void launch_heatcool_cfl(const KParams &P, hipStream_t st);
 void launch_boundaries(const KParams &P, hipStream_t st, int final_call, int advance_time);
 void launch_transport(const KParams &P, hipStream_t st);
 void launch_radial_sweep(const KParams &P, hipStream_t st);
-void launch_azimuthal_pass(const KParams &P, hipStream_t st, int pass);
+void launch_azimuthal_sweep(const KParams &P, hipStream_t st);
 void launch_cfl_cells(const KParams &P, hipStream_t st);
 void launch_copy4(const KParams &P, hipStream_t st);
 void launch_derived(const KParams &P, hipStream_t st);

@@ -103,9 +103,8 @@ def algorithmic_bytes(kernel: str, adiabatic: bool) -> int:
     field it reads once plus each field it writes once."""
     e = 1 if adiabatic else 0
     reads_writes = {
-        "radial_sweep": (6 + e, 5 + e),        # Sigma, v_r, 4 momenta, (E) -> 4 momenta, Sigma, (E)
-        "azimuthal_residual": (6 + e, 5 + e),  # + v_phi for the residual velocity
-        "azimuthal_shift": (5 + e, 5 + e),
+        "radial_sweep": (3 + e, 5 + e),        # Sigma, v_r, v_phi, (E) -> 4 momenta, Sigma, (E)
+        "azimuthal_sweep": (6 + e, 5 + e),     # Sigma, 4 momenta, (E), v_phi -> 5/6, shifted
         "cfl": (4 + 3 * e, 0),                 # v_r, v_phi, c_s, nu, (Q+, Q-, E)
         "stress": (4, 4),                      # v_r, v_phi, nu, Sigma -> div v, tau_rr, tau_pp, tau_rp
         "potential": (1, 1),                   # H -> Phi

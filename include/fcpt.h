@@ -190,9 +190,8 @@ int fcpt_run_steps(fcpt_ctx *ctx, int32_t n_steps, int32_t use_graph);
    and the mean duration of one launch of a single kernel of the step, timed
    the same way over `reps` back-to-back launches on the current state
    (FCPT_KERNEL_* below; the listed kernels do not modify their inputs). */
-enum { FCPT_KERNEL_RADIAL_SWEEP = 0, FCPT_KERNEL_AZIMUTHAL_RESIDUAL = 1,
-       FCPT_KERNEL_AZIMUTHAL_SHIFT = 2, FCPT_KERNEL_CFL = 3, FCPT_KERNEL_STRESS = 4,
-       FCPT_KERNEL_POTENTIAL = 5 };
+enum { FCPT_KERNEL_RADIAL_SWEEP = 0, FCPT_KERNEL_AZIMUTHAL = 1, FCPT_KERNEL_CFL = 3,
+       FCPT_KERNEL_STRESS = 4, FCPT_KERNEL_POTENTIAL = 5 };
 int fcpt_time_steps(fcpt_ctx *ctx, int32_t n_steps, int32_t use_graph, double *ms_total);
 int fcpt_time_kernel(fcpt_ctx *ctx, int32_t kernel, int32_t reps, double *ms_per_launch);
 /* Streaming-copy probe (no counterpart in the reference): copies four
