@@ -24,6 +24,9 @@ VECTOR_FIELDS = {"VRAD", "VRAD0", "TAU_R_PHI"}
 
 ARTVISC = {"none": 0, "tw": 1, "sn": 2}
 KERNELS = {"radial_sweep": 0, "azimuthal_sweep": 1, "cfl": 3, "stress": 4, "potential": 5}
+# storage modes (fcpt_disk_desc.precision) and launch parameters of include/fcpt.h
+PREC_FP64, PREC_SPLIT32 = 0, 1
+PARAM_RADIAL_TILE = 0
 BC = {"reflecting": 0, "zerogradient": 1, "outflow": 2}
 MAX_PLANETS = 8
 
@@ -159,6 +162,7 @@ def load_library(path: str = LIB_PATH):
         "fcpt_time_steps": [P, I, I, dp],
         "fcpt_time_kernel": [P, I, I, dp],
         "fcpt_time_copy": [P, I, dp, dp],
+        "fcpt_set_launch_param": [P, I, I],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
@@ -320,3 +324,7 @@ class Disk:
         ms = C.c_double()
         self._check(self.lib.fcpt_time_kernel(self.h, kernel, reps, C.byref(ms)))
         return ms.value
+
+    def set_radial_tile(self, rings):
+        """Rings walked by one radial-sweep thread (0: size-based default)."""
+        self._check(self.lib.fcpt_set_launch_param(self.h, PARAM_RADIAL_TILE, rings))

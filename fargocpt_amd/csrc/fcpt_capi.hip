@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "fcpt_device.hpp"
@@ -30,6 +31,12 @@ struct fcpt_ctx {
     hipGraphExec_t step_graph = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::string err;
+    // host copies of the per-ring metric (split storage derives its backgrounds from them)
+    std::vector<double> Rinf, Rmed, InvRmed, InvDiffRmed, InvDiffRsup, InvDiffRsupRb;
+    // split storage: host copies of the backgrounds last set from the host, and
+    // a device ring array for re-expressing a field about a new background
+    std::vector<double> hbg[FCPT_NUM_FIELDS];
+    double *bg_tmp = nullptr;
 };
 
 namespace {
@@ -67,16 +74,24 @@ int fail(fcpt_ctx *c, int code, const char *fmt, ...)
             return fail((ctx), FCPT_ERR_DEVICE, "hipSetDevice: %s", hipGetErrorString(e_)); \
     } while (0)
 
+int dev_alloc_bytes(fcpt_ctx *c, void **p, size_t bytes)
+{
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, bytes + 256);
+    if (e != hipSuccess)
+        return fail(c, FCPT_ERR_DEVICE, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    c->allocs.push_back(q);
+    *p = q;
+    return FCPT_OK;
+}
+
 template <typename T>
 int dev_alloc(fcpt_ctx *c, T **p, size_t count)
 {
     void *q = nullptr;
-    hipError_t e = hipMalloc(&q, count * sizeof(T) + 256);
-    if (e != hipSuccess)
-        return fail(c, FCPT_ERR_DEVICE, "hipMalloc(%zu): %s", count * sizeof(T), hipGetErrorString(e));
-    c->allocs.push_back(q);
+    const int rc = dev_alloc_bytes(c, &q, count * sizeof(T));
     *p = static_cast<T *>(q);
-    return FCPT_OK;
+    return rc;
 }
 
 size_t field_rows(int field, int nr)
@@ -120,7 +135,7 @@ int fcpt_create(const fcpt_disk_desc *desc, fcpt_ctx **out)
     *out = nullptr;
     if (desc->nr < 8 || desc->nr > 16384 || desc->nphi < 4 || desc->nphi > 262144 || !desc->radii)
         return FCPT_ERR_ARG;
-    if (desc->precision != FCPT_PREC_FP64) return FCPT_ERR_ARG;
+    if (desc->precision != FCPT_PREC_FP64 && desc->precision != FCPT_PREC_SPLIT32) return FCPT_ERR_ARG;
     for (int i = 0; i + 1 < desc->nr + 2; ++i)
         if (!(desc->radii[i + 1] > desc->radii[i]) || !(desc->radii[i] > 0.0)) return FCPT_ERR_ARG;
     fcpt_ctx *c = new fcpt_ctx();
@@ -135,6 +150,8 @@ int fcpt_create(const fcpt_disk_desc *desc, fcpt_ctx **out)
     KParams &P = c->P;
     P.nr = nr;
     P.ns = ns;
+    P.split = desc->precision == FCPT_PREC_SPLIT32 ? 1 : 0;
+    P.radial_tile = 0;
     // Interpret.cpp:230-231
     P.dphi = 2.0 * M_PI / (double)ns;
     P.invdphi = (double)ns / (2.0 * M_PI);
@@ -210,6 +227,12 @@ int fcpt_create(const fcpt_disk_desc *desc, fcpt_ctx **out)
         InvRinf[i] = 1.0 / Rinf[i];
     }
     for (int i = 1; i < nr + 1; ++i) InvDiffRmed[i] = 1.0 / (Rmed[i] - Rmed[i - 1]);
+    c->Rinf = Rinf;
+    c->Rmed = Rmed;
+    c->InvRmed = InvRmed;
+    c->InvDiffRmed = InvDiffRmed;
+    c->InvDiffRsup = InvDiffRsup;
+    c->InvDiffRsupRb = InvDiffRsupRb;
     for (int j = 0; j < ns; ++j) {
         // cell centres, SideEuler.cpp:57-66
         cosphi[j] = std::cos(P.dphi * (double)j);
@@ -236,13 +259,21 @@ int fcpt_create(const fcpt_disk_desc *desc, fcpt_ctx **out)
         return rc;
     }
     const size_t ncell = (size_t)nr * ns;
-    for (int k = 0; k < FCPT_NUM_FIELDS && !rc; ++k) rc = dev_alloc(c, &P.f[k], field_rows(k, nr) * ns);
-    for (int k = 0; k < 4 && !rc; ++k) rc = dev_alloc(c, &P.mom[k], ncell);
-    for (int k = 0; k < 4 && !rc; ++k) rc = dev_alloc(c, &P.mom_b[k], ncell);
-    if (!rc) rc = dev_alloc(c, &P.sig_b, ncell);
-    if (!rc) rc = dev_alloc(c, &P.en_b, ncell);
-    if (!rc) rc = dev_alloc(c, &P.sig_c, ncell);
-    if (!rc) rc = dev_alloc(c, &P.en_c, ncell);
+    const size_t esz = element_size(P);
+    for (int k = 0; k < FCPT_NUM_FIELDS && !rc; ++k)
+        rc = dev_alloc_bytes(c, &P.f[k], field_rows(k, nr) * ns * esz);
+    for (int k = 0; k < 4 && !rc; ++k) rc = dev_alloc_bytes(c, &P.mom[k], ncell * esz);
+    for (int k = 0; k < 4 && !rc; ++k) rc = dev_alloc_bytes(c, &P.mom_b[k], ncell * esz);
+    if (!rc) rc = dev_alloc_bytes(c, &P.sig_b, ncell * esz);
+    if (!rc) rc = dev_alloc_bytes(c, &P.en_b, ncell * esz);
+    if (!rc) rc = dev_alloc_bytes(c, &P.sig_c, ncell * esz);
+    if (!rc) rc = dev_alloc_bytes(c, &P.en_c, ncell * esz);
+    if (P.split) {
+        for (int k = 0; k < FCPT_NUM_FIELDS && !rc; ++k) rc = dev_alloc(c, &P.bg[k], (size_t)nr + 2);
+        for (int k = 0; k < 4 && !rc; ++k) rc = dev_alloc(c, &P.bgm[k], (size_t)nr + 2);
+        if (!rc) rc = dev_alloc(c, &c->bg_tmp, (size_t)nr + 2);
+        for (int k = 0; k < FCPT_NUM_FIELDS; ++k) c->hbg[k].assign((size_t)nr + 2, 0.0);
+    }
     if (!rc) rc = dev_alloc(c, &P.vmean, (size_t)nr + 1);
     if (!rc) rc = dev_alloc(c, &P.vuni, (size_t)nr + 1);
     if (!rc) rc = dev_alloc(c, &P.nshift, (size_t)nr + 1);
@@ -262,9 +293,14 @@ int fcpt_create(const fcpt_disk_desc *desc, fcpt_ctx **out)
         fcpt_destroy(c);
         return FCPT_ERR_DEVICE;
     }
-    // zero every field (the reference's t_polargrid::set_size clears its arrays)
+    // zero every field (the reference's t_polargrid::set_size clears its arrays);
+    // split storage: zero backgrounds and zero deviations
     for (int k = 0; k < FCPT_NUM_FIELDS; ++k)
-        e = e == hipSuccess ? hipMemset(P.f[k], 0, field_rows(k, nr) * ns * sizeof(double)) : e;
+        e = e == hipSuccess ? hipMemset(P.f[k], 0, field_rows(k, nr) * ns * esz) : e;
+    for (int k = 0; P.split && k < FCPT_NUM_FIELDS; ++k)
+        e = e == hipSuccess ? hipMemset(P.bg[k], 0, ((size_t)nr + 2) * sizeof(double)) : e;
+    for (int k = 0; P.split && k < 4; ++k)
+        e = e == hipSuccess ? hipMemset(P.bgm[k], 0, ((size_t)nr + 2) * sizeof(double)) : e;
     std::memset(c->s_host, 0, sizeof(DevScalars));
     if (e == hipSuccess) e = hipMemcpy(c->s_dev, c->s_host, sizeof(DevScalars), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
@@ -309,10 +345,51 @@ static int read_scalars(fcpt_ctx *ctx)
     return FCPT_OK;
 }
 
+// split storage: the field's background becomes its ring means, the device
+// holds (float)(value - background)
+static int upload_split(fcpt_ctx *ctx, int field, const double *host)
+{
+    const int ns = ctx->P.ns;
+    const size_t rows = field_rows(field, ctx->P.nr);
+    std::vector<double> &bg = ctx->hbg[field];
+    bg.assign((size_t)ctx->P.nr + 2, 0.0);
+    std::vector<float> d(rows * ns);
+    for (size_t i = 0; i < rows; ++i) {
+        const double *row = host + i * ns;
+        double sum = 0.0;
+        for (int j = 0; j < ns; ++j) sum += row[j];
+        bg[i] = sum / (double)ns;
+        for (int j = 0; j < ns; ++j) d[i * ns + j] = (float)(row[j] - bg[i]);
+    }
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->P.bg[field], bg.data(), bg.size() * sizeof(double),
+                                hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->P.f[field], d.data(), d.size() * sizeof(float),
+                                hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return FCPT_OK;
+}
+
+static int download_split(fcpt_ctx *ctx, int field, double *host)
+{
+    const int ns = ctx->P.ns;
+    const size_t rows = field_rows(field, ctx->P.nr);
+    std::vector<double> bg(rows);
+    std::vector<float> d(rows * ns);
+    HIP_TRY(ctx, hipMemcpyAsync(bg.data(), ctx->P.bg[field], rows * sizeof(double),
+                                hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(d.data(), ctx->P.f[field], d.size() * sizeof(float),
+                                hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    for (size_t i = 0; i < rows; ++i)
+        for (int j = 0; j < ns; ++j) host[i * ns + j] = bg[i] + (double)d[i * ns + j];
+    return FCPT_OK;
+}
+
 int fcpt_upload_field(fcpt_ctx *ctx, int32_t field, const double *host)
 {
     CTX_CHECK(ctx);
     if (field < 0 || field >= FCPT_NUM_FIELDS || !host) return fail(ctx, FCPT_ERR_ARG, "bad field");
+    if (ctx->P.split) return upload_split(ctx, field, host);
     const size_t n = field_rows(field, ctx->P.nr) * ctx->P.ns;
     HIP_TRY(ctx, hipMemcpyAsync(ctx->P.f[field], host, n * sizeof(double), hipMemcpyHostToDevice,
                                 ctx->stream));
@@ -324,9 +401,85 @@ int fcpt_download_field(fcpt_ctx *ctx, int32_t field, double *host)
 {
     CTX_CHECK(ctx);
     if (field < 0 || field >= FCPT_NUM_FIELDS || !host) return fail(ctx, FCPT_ERR_ARG, "bad field");
+    if (ctx->P.split) return download_split(ctx, field, host);
     const size_t n = field_rows(field, ctx->P.nr) * ctx->P.ns;
     HIP_TRY(ctx, hipMemcpyAsync(host, ctx->P.f[field], n * sizeof(double), hipMemcpyDeviceToHost,
                                 ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return FCPT_OK;
+}
+
+// split storage: re-express a field about a new host-given background
+static int rebase_field(fcpt_ctx *ctx, int field, const std::vector<double> &newbg)
+{
+    const KParams &P = ctx->P;
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->bg_tmp, newbg.data(), ((size_t)P.nr + 2) * sizeof(double),
+                                hipMemcpyHostToDevice, ctx->stream));
+    launch_rebase(P, ctx->stream, P.f[field], P.bg[field], ctx->bg_tmp, (int)field_rows(field, P.nr));
+    LAUNCH_CHECK(ctx);
+    HIP_TRY(ctx, hipMemcpyAsync(P.bg[field], ctx->bg_tmp, ((size_t)P.nr + 2) * sizeof(double),
+                                hipMemcpyDeviceToDevice, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->hbg[field] = newbg;
+    return FCPT_OK;
+}
+
+// split storage: backgrounds of the derived fields and of the transport
+// momenta, evaluated from the ring backgrounds of Sigma, v_r, v_phi and E with
+// the expressions of the kernels that produce those fields (azimuthal
+// differences vanish on a ring profile)
+static int set_derived_backgrounds(fcpt_ctx *ctx)
+{
+    const KParams &P = ctx->P;
+    const int nr = P.nr;
+    const std::vector<double> &S = ctx->hbg[FCPT_SIGMA], &VR = ctx->hbg[FCPT_VRAD],
+                              &VA = ctx->hbg[FCPT_VAZI], &E = ctx->hbg[FCPT_ENERGY];
+    const size_t n1 = (size_t)nr + 2;
+    std::vector<double> cs(n1, 0.0), H(n1, 0.0), Pr(n1, 0.0), nu(n1, 0.0), T(n1, 0.0), dv(n1, 0.0),
+        trr(n1, 0.0), tpp(n1, 0.0), trp(n1, 0.0), mr0(n1, 0.0), mr1(n1, 0.0), ma(n1, 0.0);
+    const double g = P.gamma;
+    for (int i = 0; i < nr; ++i) {
+        const double R = ctx->Rmed[i];
+        const double omk = std::sqrt(P.G * P.mstar / (R * R * R));
+        double c, h;
+        if (P.adiabatic) {
+            c = std::sqrt(g * (g - 1.0) * E[i] / S[i]);
+            h = c / std::sqrt(g) / omk;
+            Pr[i] = (g - 1.0) * E[i];
+            T[i] = P.mu / P.Rgas * (g - 1.0) * E[i] / S[i];
+        } else {
+            c = P.aspect_ratio * std::pow(R, P.flaring) * std::sqrt(P.G * P.mstar / R);
+            h = c / omk;
+            Pr[i] = S[i] * (c * c);
+            T[i] = P.mu / P.Rgas * Pr[i] / S[i];
+        }
+        cs[i] = c;
+        H[i] = h;
+        nu[i] = P.alpha * h * c;
+        dv[i] = (VR[i + 1] * ctx->Rinf[i + 1] - VR[i] * ctx->Rinf[i]) * ctx->InvDiffRsupRb[i];
+        trr[i] = 2.0 * nu[i] * S[i] * ((VR[i + 1] - VR[i]) * ctx->InvDiffRsup[i] - dv[i] / 3.0);
+        tpp[i] = 2.0 * nu[i] * S[i] * (0.5 * (VR[i + 1] + VR[i]) * ctx->InvRmed[i] - dv[i] / 3.0);
+        if (i >= 1)
+            trp[i] = 0.5 * (nu[i] + nu[i - 1]) * 0.5 * (S[i] + S[i - 1]) * ctx->Rinf[i] *
+                     (VA[i] * ctx->InvRmed[i] - VA[i - 1] * ctx->InvRmed[i - 1]) * ctx->InvDiffRmed[i];
+        mr0[i] = S[i] * VR[i + 1];
+        mr1[i] = S[i] * VR[i];
+        ma[i] = S[i] * (VA[i] + R * P.omega_frame) * R;
+    }
+    if (!std::isfinite(cs[1]) || !std::isfinite(trp[1]))
+        return fail(ctx, FCPT_ERR_ARG, "split storage needs a positive Sigma (and E) on every ring");
+    const std::pair<int, const std::vector<double> *> derived[] = {
+        {FCPT_SOUNDSPEED, &cs}, {FCPT_SCALE_HEIGHT, &H},  {FCPT_PRESSURE, &Pr},
+        {FCPT_VISCOSITY, &nu},  {FCPT_TEMPERATURE, &T},   {FCPT_DIV_V, &dv},
+        {FCPT_TAU_R_R, &trr},   {FCPT_TAU_PHI_PHI, &tpp}, {FCPT_TAU_R_PHI, &trp}};
+    for (const auto &fd : derived) {
+        const int rc = rebase_field(ctx, fd.first, *fd.second);
+        if (rc) return rc;
+    }
+    const std::vector<double> *mom[4] = {&mr0, &mr1, &ma, &ma};
+    for (int k = 0; k < 4; ++k)
+        HIP_TRY(ctx, hipMemcpyAsync(P.bgm[k], mom[k]->data(), n1 * sizeof(double),
+                                    hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     return FCPT_OK;
 }
@@ -351,6 +504,7 @@ int fcpt_upload_state(fcpt_ctx *ctx, const double *sigma, const double *vrad, co
     if ((rc = fcpt_upload_field(ctx, FCPT_VAZI, vazi))) return rc;
     if (energy && (rc = fcpt_upload_field(ctx, FCPT_ENERGY, energy))) return rc;
     if (ctx->P.adiabatic && !energy) return fail(ctx, FCPT_ERR_ARG, "adiabatic disk needs energy");
+    if (ctx->P.split && (rc = set_derived_backgrounds(ctx))) return rc;
     // init_euler recalculation (SourceEuler.cpp:264-284)
     const KParams &P = ctx->P;
     if (!P.adiabatic) {
@@ -657,7 +811,22 @@ int fcpt_time_copy(fcpt_ctx *ctx, int32_t reps, double *ms_per_launch, double *b
     float ms = 0.f;
     HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     if (ms_per_launch) *ms_per_launch = (double)ms / reps;
-    if (bytes_per_launch) *bytes_per_launch = 2.0 * 4.0 * 8.0 * (double)((size_t)ctx->P.nr * ctx->P.ns / 2 * 2);
+    if (bytes_per_launch)
+        *bytes_per_launch = 2.0 * 4.0 * (double)((size_t)ctx->P.nr * ctx->P.ns * element_size(ctx->P) / 16 * 16);
+    return FCPT_OK;
+}
+
+int fcpt_set_launch_param(fcpt_ctx *ctx, int32_t param, int32_t value)
+{
+    CTX_CHECK(ctx);
+    if (param != FCPT_PARAM_RADIAL_TILE) return fail(ctx, FCPT_ERR_ARG, "unknown launch parameter");
+    if (value < 0 || value > 64) return fail(ctx, FCPT_ERR_ARG, "radial tile must be 0 .. 64");
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->P.radial_tile = value;
+    if (ctx->step_graph) {
+        HIP_TRY(ctx, hipGraphExecDestroy(ctx->step_graph));
+        ctx->step_graph = nullptr;
+    }
     return FCPT_OK;
 }
 

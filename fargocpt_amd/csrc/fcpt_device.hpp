@@ -2,7 +2,7 @@
 //
 // One disk = one KParams block passed by value to every kernel: grid sizes,
 // physical scalars, pointers to the per-ring metric arrays and to every field
-// in HBM.  Quantities that change from step to step (dt, time, planets, the
+// in HBM (and, for split storage, to the per-ring backgrounds).  Quantities that change from step to step (dt, time, planets, the
 // CFL minimum) live in a DevScalars block in device memory so that a captured
 // graph of the step can be replayed without host involvement.
 #pragma once
@@ -63,13 +63,20 @@ struct KParams {
         *InvDiffRsup, *InvDiffRsupRb;
     // per-sector tables: cos / sin of dphi * j (cell-centre coordinates)
     const double *cosphi, *sinphi;
-    // fields in HBM (indexed by fcpt_field)
-    double *f[FCPT_NUM_FIELDS];
-    // transport work arrays
-    double *mom[4];      // radial momentum +/-, angular momentum +/-
-    double *mom_b[4];    // ping-pong partners
-    double *sig_b, *en_b;   // radial-sweep outputs
-    double *sig_c, *en_c;   // azimuthal-sweep outputs
+    // storage: 0 = FP64 fields; 1 = FP32 deviations from FP64 per-ring backgrounds
+    int split;
+    int radial_tile;     // rings per thread of the radial sweep
+    // fields in HBM (indexed by fcpt_field): double, or float when split
+    void *f[FCPT_NUM_FIELDS];
+    // split storage: per-ring backgrounds of every field (nr + 2 values each;
+    // null in FP64 mode); the potential's is rewritten by the potential kernel
+    double *bg[FCPT_NUM_FIELDS];
+    // transport work arrays (same element type as the fields)
+    void *mom[4];        // radial momentum +/-, angular momentum +/-
+    void *mom_b[4];      // ping-pong partners
+    void *sig_b, *en_b;  // radial-sweep outputs
+    void *sig_c, *en_c;  // azimuthal-sweep outputs
+    double *bgm[4];      // backgrounds of the four momenta (split storage)
     double *vmean;       // per-ring mean azimuthal velocity
     double *vuni;        // per-ring uniform residual velocity (FARGO)
     int *nshift;         // per-ring integer shift

@@ -1,10 +1,18 @@
-// gfx950 kernels of the FargoCPT hydro step (FP64).
+// gfx950 kernels of the FargoCPT hydro step.
 //
 // Every kernel restates one loop nest of the reference (file:line given above
 // it) with the same expression order; the library is compiled with
 // -ffp-contract=off so that add/mul/div/sqrt round exactly like the
 // reference's host build.  Threads map x -> azimuthal index j (contiguous in
-// HBM, so every wavefront reads 64 consecutive doubles) and y -> ring index i.
+// HBM, so a wavefront reads 64 consecutive values) and y -> ring index i.
+//
+// Storage: every kernel is instantiated twice.  SPLIT = false keeps each field
+// in FP64.  SPLIT = true keeps each 2-D field as an FP32 deviation from an FP64
+// per-ring background (value = bg[ring] + (double)deviation), which halves the
+// HBM traffic; the arithmetic is FP64 in both instantiations and the FP64
+// instantiation performs exactly the operations of the reference.
+#include <type_traits>
+
 #include "fcpt_device.hpp"
 #include "fcpt_kernels.hpp"
 
@@ -24,6 +32,40 @@ __device__ __forceinline__ double omega_kepler(const KParams &P, double r)
 {
     return sqrt(P.G * P.mstar / (r * r * r));  // Theo.cpp:246-249
 }
+
+// field accessor: an FP64 array, or FP32 deviations from an FP64 ring background
+template <bool SPLIT>
+struct Fld {
+    using T = typename std::conditional<SPLIT, float, double>::type;
+    T *p;
+    const double *bg;
+    int ns;
+    __device__ __forceinline__ double operator()(int i, int j) const
+    {
+        const size_t c = (size_t)i * (size_t)ns + (size_t)j;
+        if constexpr (SPLIT) return bg[i] + (double)p[c];
+        else return p[c];
+    }
+    __device__ __forceinline__ void set(int i, int j, double v) const
+    {
+        const size_t c = (size_t)i * (size_t)ns + (size_t)j;
+        if constexpr (SPLIT) p[c] = (float)(v - bg[i]);
+        else p[c] = v;
+    }
+};
+
+template <bool SPLIT>
+__device__ __forceinline__ Fld<SPLIT> fld(const KParams &P, int f)
+{
+    return Fld<SPLIT>{static_cast<typename Fld<SPLIT>::T *>(P.f[f]), P.bg[f], P.ns};
+}
+template <bool SPLIT>
+__device__ __forceinline__ Fld<SPLIT> buf(const KParams &P, void *p, const double *bg)
+{
+    return Fld<SPLIT>{static_cast<typename Fld<SPLIT>::T *>(p), bg, P.ns};
+}
+
+#define FLD(name, id) const Fld<SPLIT> name = fld<SPLIT>(P, id)
 
 // 2-D launch over rows [r0, r1) x all sectors
 #define CELL_LOOP_HEAD(r0, r1)                                   \
@@ -52,14 +94,15 @@ __device__ double block_sum(double v, double *sh)
 // mode 0: CFL ring mean
 // mode 1: transport ring mean + FARGO shift + uniform residual
 //         (TransportEuler.cpp:207-236)
+template <bool SPLIT>
 __global__ void __launch_bounds__(256) k_ring_mean(KParams P, int mode)
 {
     __shared__ double sh[256];
     const int ns = P.ns;
     const int i = blockIdx.x;
-    const double *v = P.f[FCPT_VAZI];
+    FLD(v, FCPT_VAZI);
     double acc = 0.0;
-    for (int j = threadIdx.x; j < ns; j += 256) acc += v[IX(i, j)];
+    for (int j = threadIdx.x; j < ns; j += 256) acc += v(i, j);
     const double sum = block_sum<256>(acc, sh);
     if (threadIdx.x == 0) {
         const double vm = sum / (double)ns;
@@ -79,12 +122,16 @@ __global__ void __launch_bounds__(256) k_ring_mean(KParams P, int mode)
 // walks rows with a grid stride; the block minimum (wavefront shuffles, then
 // LDS) goes to its own slot of P.ring_min and k_cfl_finalize folds the slots,
 // so the result is deterministic and free of atomics.
+template <bool SPLIT>
 __global__ void __launch_bounds__(256) k_cfl_cells(KParams P)
 {
     const int ns = P.ns;
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     double cand = __longlong_as_double(0x7FF0000000000000ll);
-    const double *vr = P.f[FCPT_VRAD], *vaz = P.f[FCPT_VAZI];
+    FLD(vr, FCPT_VRAD);
+    FLD(vaz, FCPT_VAZI);
+    FLD(cs, FCPT_SOUNDSPEED);
+    FLD(nu, FCPT_VISCOSITY);
     const double CFL = P.cfl, dphi = P.dphi;
     const double *vm = P.vmean;
     const double fac2 = P.artvisc_factor * P.artvisc_factor;
@@ -103,35 +150,35 @@ __global__ void __launch_bounds__(256) k_cfl_cells(KParams P)
         const double dxRadial = P.Rsup[i] - P.Rinf[i];
         const double dxAzimuthal = P.Rmed[i] * dphi;
         const double cell_size = std_min(dxRadial, dxAzimuthal);
-        const double v_residual = vaz[IX(i, j)] - vm[i];
-        const double invdt1 = P.f[FCPT_SOUNDSPEED][IX(i, j)] / cell_size;
-        const double invdt2 = vr[IX(i, j)] / dxRadial;
-        const double invdt3 = v_residual / dxAzimuthal;
         const int jn = j == ns - 1 ? 0 : j + 1;
+        const double vr0 = vr(i, j), vr1 = vr(i + 1, j), va0 = vaz(i, j), va1 = vaz(i, jn);
+        const double v_residual = va0 - vm[i];
+        const double invdt1 = cs(i, j) / cell_size;
+        const double invdt2 = vr0 / dxRadial;
+        const double invdt3 = v_residual / dxAzimuthal;
         double invdt4;
         if (P.artvisc == FCPT_ARTVISC_SN) {
-            double dvRadial = vr[IX(i + 1, j)] - vr[IX(i, j)];
-            double dvAzimuthal = vaz[IX(i, jn)] - vaz[IX(i, j)];
+            double dvRadial = vr1 - vr0;
+            double dvAzimuthal = va1 - va0;
             if (dvRadial > 0.0) dvRadial = 0.0;
             else dvRadial = -dvRadial;
             if (dvAzimuthal > 0.0) dvAzimuthal = 0.0;
             else dvAzimuthal = -dvAzimuthal;
             invdt4 = 4.0 * fac2 * std_max(dvRadial / dxRadial, dvAzimuthal / dxAzimuthal) * 1.0;
         } else {
-            const double eps_rr = (vr[IX(i + 1, j)] - vr[IX(i, j)]) * P.InvDiffRsup[i];
-            const double eps_pp = P.InvRmed[i] * ((vaz[IX(i, jn)] - vaz[IX(i, j)]) * P.invdphi +
-                                                  0.5 * (vr[IX(i + 1, j)] + vr[IX(i, j)]));
+            const double eps_rr = (vr1 - vr0) * P.InvDiffRsup[i];
+            const double eps_pp = P.InvRmed[i] * ((va1 - va0) * P.invdphi + 0.5 * (vr1 + vr0));
             const double mdiv_V = -std_min(eps_rr + eps_pp, 0.0);
             invdt4 = 4.0 * fac2 * mdiv_V * 1.0;
         }
-        const double invdt5 = 4.0 * P.f[FCPT_VISCOSITY][IX(i, j)] / (cell_size * cell_size) * 1.0;
+        const double invdt5 = 4.0 * nu(i, j) / (cell_size * cell_size) * 1.0;
         double invdt6 = 0.0;
         if (P.adiabatic) {
+            FLD(qp, FCPT_QPLUS);
+            FLD(qm, FCPT_QMINUS);
+            FLD(E, FCPT_ENERGY);
             const double inv_limit = 1.0 / P.heating_cooling_cfl_limit;
-            invdt6 = inv_limit *
-                     fabs((P.f[FCPT_QPLUS][IX(i, j)] - P.f[FCPT_QMINUS][IX(i, j)]) /
-                          P.f[FCPT_ENERGY][IX(i, j)]) *
-                     1.0;
+            invdt6 = inv_limit * fabs((qp(i, j) - qm(i, j)) / E(i, j)) * 1.0;
         }
         const double dt_cell =
             CFL / sqrt(invdt1 * invdt1 + invdt2 * invdt2 + invdt3 * invdt3 + invdt4 * invdt4 +
@@ -209,13 +256,15 @@ __global__ void k_set_params(KParams P, fcpt_step_params sp)
 
 // ---------------------------------------------------------------------------
 // CalculateNbodyPotential, Pframeforce.cpp:44-85 (+ compute_smoothing, Force.cpp:145-159)
+template <bool SPLIT>
 __global__ void __launch_bounds__(256) k_potential(KParams P)
 {
     CELL_LOOP_HEAD(0, P.nr)
+    FLD(Hf, FCPT_SCALE_HEIGHT);
     const DevScalars *s = P.s;
     const double x = P.Rmed[i] * P.cosphi[j];
     const double y = P.Rmed[i] * P.sinphi[j];
-    const double H = P.f[FCPT_SCALE_HEIGHT][IX(i, j)];
+    const double H = Hf(i, j);
     double acc = 0.0;
     const int np = s->n_planets;
     for (int k = 0; k < np; ++k) {
@@ -233,58 +282,80 @@ __global__ void __launch_bounds__(256) k_potential(KParams P)
         acc += -P.G * s->planet[k].mass / d_smoothed * smooth_factor_klahr;
     }
     acc += -s->indirect_x * x - s->indirect_y * y;
-    P.f[FCPT_POTENTIAL][IX(i, j)] = acc;
+    if constexpr (SPLIT) {
+        // The potential is rewritten completely every step, so its background
+        // is chosen here: every body seen from distance sqrt(R^2 + |r_k|^2)
+        // with the ring's background smoothing.  Each thread evaluates the
+        // same expression; the ring's first sector publishes it.
+        const double R = P.Rmed[i];
+        const double smooth = P.thickness_smoothing * P.bg[FCPT_SCALE_HEIGHT][i];
+        double b = 0.0;
+        for (int k = 0; k < np; ++k) {
+            const double px = s->planet[k].x, py = s->planet[k].y;
+            b += -P.G * s->planet[k].mass / sqrt(R * R + (px * px + py * py) + smooth * smooth);
+        }
+        if (j == 0) P.bg[FCPT_POTENTIAL][i] = b;
+        static_cast<float *>(P.f[FCPT_POTENTIAL])[IX(i, j)] = (float)(acc - b);
+    } else {
+        fld<SPLIT>(P, FCPT_POTENTIAL).set(i, j, acc);
+    }
 }
 
 // ---------------------------------------------------------------------------
 // momentum_update_radial, SourceEuler.cpp:325-372
+template <bool SPLIT>
 __global__ void __launch_bounds__(256) k_src_vr(KParams P)
 {
     CELL_LOOP_HEAD(2, P.nr - 1)
-    const double *S = P.f[FCPT_SIGMA], *Pr = P.f[FCPT_PRESSURE], *Phi = P.f[FCPT_POTENTIAL],
-                 *vaz = P.f[FCPT_VAZI];
-    double *vr = P.f[FCPT_VRAD];
+    FLD(S, FCPT_SIGMA);
+    FLD(Pr, FCPT_PRESSURE);
+    FLD(Phi, FCPT_POTENTIAL);
+    FLD(vaz, FCPT_VAZI);
+    FLD(vr, FCPT_VRAD);
     const double dt = P.s->dt;
-    double gradp = 2.0 / (S[IX(i, j)] + S[IX(i - 1, j)]);
-    gradp *= (Pr[IX(i, j)] - Pr[IX(i - 1, j)]);
+    double gradp = 2.0 / (S(i, j) + S(i - 1, j));
+    gradp *= (Pr(i, j) - Pr(i - 1, j));
     gradp *= P.InvDiffRmed[i];
-    const double gradphi = (Phi[IX(i, j)] - Phi[IX(i - 1, j)]) * P.InvDiffRmed[i];
+    const double gradphi = (Phi(i, j) - Phi(i - 1, j)) * P.InvDiffRmed[i];
     const int jn = j == ns - 1 ? 0 : j + 1;
-    const double vsum = vaz[IX(i, j)] + vaz[IX(i, jn)] + vaz[IX(i - 1, j)] + vaz[IX(i - 1, jn)];
+    const double vsum = vaz(i, j) + vaz(i, jn) + vaz(i - 1, j) + vaz(i - 1, jn);
     const double vt = 0.25 * vsum + P.Rinf[i] * P.omega_frame;
     const double vt2 = vt * vt;
     const double centrifugal_accel = vt2 * P.InvRinf[i];
-    vr[IX(i, j)] += dt * (-gradp - gradphi + centrifugal_accel);
+    vr.set(i, j, vr(i, j) + dt * (-gradp - gradphi + centrifugal_accel));
 }
 
 // momentum_update_azimuthal, SourceEuler.cpp:375-428
+template <bool SPLIT>
 __global__ void __launch_bounds__(256) k_src_vphi(KParams P)
 {
     CELL_LOOP_HEAD(1, P.nr - 1)
-    const double *S = P.f[FCPT_SIGMA], *Pr = P.f[FCPT_PRESSURE], *Phi = P.f[FCPT_POTENTIAL];
-    double *vaz = P.f[FCPT_VAZI];
+    FLD(S, FCPT_SIGMA);
+    FLD(Pr, FCPT_PRESSURE);
+    FLD(Phi, FCPT_POTENTIAL);
+    FLD(vaz, FCPT_VAZI);
     const double dt = P.s->dt;
     const double invdxtheta = 2.0 / (P.dphi * (P.Rsup[i] + P.Rinf[i]));
     const int jp = j == 0 ? ns - 1 : j - 1;
-    const double gradp =
-        2.0 / (S[IX(i, j)] + S[IX(i, jp)]) * (Pr[IX(i, j)] - Pr[IX(i, jp)]) * invdxtheta;
-    const double gradphi = (Phi[IX(i, j)] - Phi[IX(i, jp)]) * invdxtheta;
-    vaz[IX(i, j)] = vaz[IX(i, j)] + dt * (-gradp - gradphi);
+    const double gradp = 2.0 / (S(i, j) + S(i, jp)) * (Pr(i, j) - Pr(i, jp)) * invdxtheta;
+    const double gradphi = (Phi(i, j) - Phi(i, jp)) * invdxtheta;
+    vaz.set(i, j, vaz(i, j) + dt * (-gradp - gradphi));
 }
 
 // compression_heating, SourceEuler.cpp:459-493
+template <bool SPLIT>
 __global__ void __launch_bounds__(256) k_src_energy(KParams P)
 {
     CELL_LOOP_HEAD(0, P.nr - 1)
-    const double *vr = P.f[FCPT_VRAD], *vaz = P.f[FCPT_VAZI];
-    double *E = P.f[FCPT_ENERGY];
+    FLD(vr, FCPT_VRAD);
+    FLD(vaz, FCPT_VAZI);
+    FLD(E, FCPT_ENERGY);
     const double dt = P.s->dt;
     const int jn = j == ns - 1 ? 0 : j + 1;
-    const double DIV_V =
-        (vr[IX(i + 1, j)] * P.Rinf[i + 1] - vr[IX(i, j)] * P.Rinf[i]) * P.InvDiffRsupRb[i] +
-        (vaz[IX(i, jn)] - vaz[IX(i, j)]) * P.invdphi * P.InvRmed[i];
-    const double energy_old = E[IX(i, j)];
-    E[IX(i, j)] = energy_old * exp(-(P.gamma - 1.0) * dt * DIV_V);
+    const double DIV_V = (vr(i + 1, j) * P.Rinf[i + 1] - vr(i, j) * P.Rinf[i]) * P.InvDiffRsupRb[i] +
+                         (vaz(i, jn) - vaz(i, j)) * P.invdphi * P.InvRmed[i];
+    const double energy_old = E(i, j);
+    E.set(i, j, energy_old * exp(-(P.gamma - 1.0) * dt * DIV_V));
 }
 
 // assure_temperature_range for one cell, SourceEuler.cpp:136-202
@@ -297,170 +368,194 @@ __device__ __forceinline__ double temperature_floor(const KParams &P, double e, 
     return e;
 }
 
+template <bool SPLIT>
 __global__ void __launch_bounds__(256) k_temperature_floor(KParams P)
 {
     CELL_LOOP_HEAD(0, P.nr)
-    double *E = P.f[FCPT_ENERGY];
-    E[IX(i, j)] = temperature_floor(P, E[IX(i, j)], P.f[FCPT_SIGMA][IX(i, j)]);
+    FLD(E, FCPT_ENERGY);
+    FLD(S, FCPT_SIGMA);
+    E.set(i, j, temperature_floor(P, E(i, j), S(i, j)));
 }
 
 // ---------------------------------------------------------------------------
 // compute_sound_speed / compute_scale_height / update_viscosity for one cell
 // (SourceEuler.cpp:957-995, 1121-1154; viscosity.cpp:98-137)
-__device__ __forceinline__ void derived_cell(const KParams &P, int i, size_t c, bool with_cs_h,
-                                            bool with_temperature, bool with_pressure)
+template <bool SPLIT>
+__device__ __forceinline__ void derived_cell(const KParams &P, int i, int j, bool with_temperature,
+                                            bool with_pressure)
 {
-    const int ns = P.ns;
-    (void)ns;
-    double *cs = P.f[FCPT_SOUNDSPEED], *H = P.f[FCPT_SCALE_HEIGHT];
+    FLD(cs, FCPT_SOUNDSPEED);
+    FLD(Hf, FCPT_SCALE_HEIGHT);
+    double c_s, h;
     if (P.adiabatic) {
-        const double e = P.f[FCPT_ENERGY][c];
-        const double sig = P.f[FCPT_SIGMA][c];
+        FLD(E, FCPT_ENERGY);
+        FLD(S, FCPT_SIGMA);
+        const double e = E(i, j);
+        const double sig = S(i, j);
         if (with_temperature) {
             const double c_v_inv = P.mu / P.Rgas * (P.gamma - 1.0);
-            P.f[FCPT_TEMPERATURE][c] = c_v_inv * e / sig;
+            fld<SPLIT>(P, FCPT_TEMPERATURE).set(i, j, c_v_inv * e / sig);
         }
-        if (with_cs_h) {
-            const double g = P.gamma;
-            const double c_s = sqrt(g * (g - 1.0) * e / sig);
-            cs[c] = c_s;
-            const double inv_omega_kepler = 1.0 / omega_kepler(P, P.Rmed[i]);
-            H[c] = c_s / (sqrt(g)) * inv_omega_kepler;
+        const double g = P.gamma;
+        c_s = sqrt(g * (g - 1.0) * e / sig);
+        cs.set(i, j, c_s);
+        const double inv_omega_kepler = 1.0 / omega_kepler(P, P.Rmed[i]);
+        h = c_s / (sqrt(g)) * inv_omega_kepler;
+        Hf.set(i, j, h);
+        if (with_pressure) fld<SPLIT>(P, FCPT_PRESSURE).set(i, j, (P.gamma - 1.0) * e);
+    } else {
+        c_s = cs(i, j);
+        h = Hf(i, j);
+        if (with_pressure) {
+            FLD(S, FCPT_SIGMA);
+            fld<SPLIT>(P, FCPT_PRESSURE).set(i, j, S(i, j) * (c_s * c_s));
         }
-        if (with_pressure) P.f[FCPT_PRESSURE][c] = (P.gamma - 1.0) * e;
-    } else if (with_pressure) {
-        const double c_s = cs[c];
-        P.f[FCPT_PRESSURE][c] = P.f[FCPT_SIGMA][c] * (c_s * c_s);
     }
     const double alpha = P.alpha;
-    P.f[FCPT_VISCOSITY][c] = alpha * H[c] * cs[c];
+    fld<SPLIT>(P, FCPT_VISCOSITY).set(i, j, alpha * h * c_s);
 }
 
 // ---------------------------------------------------------------------------
 // update_with_artificial_viscosity_TW, artificial_viscosity.cpp:35-88
+template <bool SPLIT>
 __global__ void __launch_bounds__(256) k_tw_q(KParams P)
 {
     CELL_LOOP_HEAD(0, P.nr)
-    const double *vr = P.f[FCPT_VRAD], *vaz = P.f[FCPT_VAZI], *S = P.f[FCPT_SIGMA];
-    double *E = P.f[FCPT_ENERGY];
+    FLD(vr, FCPT_VRAD);
+    FLD(vaz, FCPT_VAZI);
+    FLD(S, FCPT_SIGMA);
     const double dt = P.s->dt;
     const int jn = j == ns - 1 ? 0 : j + 1;
-    const double eps_rr = (vr[IX(i + 1, j)] - vr[IX(i, j)]) * P.InvDiffRsup[i];
-    const double eps_pp = P.InvRmed[i] * ((vaz[IX(i, jn)] - vaz[IX(i, j)]) * P.invdphi +
-                                          0.5 * (vr[IX(i + 1, j)] + vr[IX(i, j)]));
+    const double vr0 = vr(i, j), vr1 = vr(i + 1, j);
+    const double eps_rr = (vr1 - vr0) * P.InvDiffRsup[i];
+    const double eps_pp = P.InvRmed[i] * ((vaz(i, jn) - vaz(i, j)) * P.invdphi + 0.5 * (vr1 + vr0));
     const double div_V = std_min(eps_rr + eps_pp, 0.0);
     const double Dr = P.Rinf[i + 1] - P.Rinf[i];
     const double rDphi = P.Rmed[i] * P.dphi;
     const double dxm = (ns <= 16) ? std_min(Dr, rDphi) : std_max(Dr, rDphi);
     const double dx_sq = dxm * dxm;
     const double l_sq = (P.artvisc_factor * P.artvisc_factor) * dx_sq;
-    const double sig = S[IX(i, j)];
-    P.f[FCPT_Q_R][IX(i, j)] = l_sq * sig * -div_V * (eps_rr - 1.0 / 3.0 * div_V);
-    P.f[FCPT_Q_PHI][IX(i, j)] = l_sq * sig * -div_V * (eps_pp - 1.0 / 3.0 * div_V);
+    const double sig = S(i, j);
+    fld<SPLIT>(P, FCPT_Q_R).set(i, j, l_sq * sig * -div_V * (eps_rr - 1.0 / 3.0 * div_V));
+    fld<SPLIT>(P, FCPT_Q_PHI).set(i, j, l_sq * sig * -div_V * (eps_pp - 1.0 / 3.0 * div_V));
     if (P.adiabatic && P.artvisc_dissipation) {
-        double e = E[IX(i, j)];
+        FLD(E, FCPT_ENERGY);
+        double e = E(i, j);
         if (i > 1 && i < P.nr - 1) {
             const double d = eps_rr - eps_pp;
-            const double Qplus = -l_sq * div_V * sig * 1.0 / 3.0 *
-                                 (eps_rr * eps_rr + eps_pp * eps_pp + d * d);
+            const double Qplus =
+                -l_sq * div_V * sig * 1.0 / 3.0 * (eps_rr * eps_rr + eps_pp * eps_pp + d * d);
             e += Qplus * dt;
         }
         // SetTemperatureFloorCeilValues after the art. viscosity (artificial_viscosity.cpp:19-21)
-        E[IX(i, j)] = temperature_floor(P, e, sig);
+        E.set(i, j, temperature_floor(P, e, sig));
     }
 }
 
 // artificial_viscosity.cpp:90-139
+template <bool SPLIT>
 __global__ void __launch_bounds__(256) k_tw_vel(KParams P, int with_viscosity)
 {
     CELL_LOOP_HEAD(0, P.nr)
     // in the fused step, recalculate_viscosity (SourceEuler.cpp:205-223), which is
     // pointwise in Sigma and E and follows this kernel, is done here for every ring
-    if (with_viscosity) derived_cell(P, i, IX(i, j), true, false, false);
+    if (with_viscosity) derived_cell<SPLIT>(P, i, j, false, false);
     if (i < 1 || i >= P.nr - 1) return;
-    const double *S = P.f[FCPT_SIGMA], *Qr = P.f[FCPT_Q_R], *Qp = P.f[FCPT_Q_PHI];
+    FLD(S, FCPT_SIGMA);
+    FLD(Qr, FCPT_Q_R);
+    FLD(Qp, FCPT_Q_PHI);
+    FLD(vaz, FCPT_VAZI);
     const double dt = P.s->dt;
     const int jp = j == 0 ? ns - 1 : j - 1;
-    const double sigma_phi_avg = 0.5 * (S[IX(i, j)] + S[IX(i, jp)]);
+    const double sigma_phi_avg = 0.5 * (S(i, j) + S(i, jp));
     const double dVp = 2.0 * dt / ((P.Rsup[i] + P.Rinf[i]) * sigma_phi_avg) *
-                       (Qp[IX(i, j)] - Qp[IX(i, jp)]) * P.invdphi;
-    P.f[FCPT_VAZI][IX(i, j)] += dVp;
+                       (Qp(i, j) - Qp(i, jp)) * P.invdphi;
+    vaz.set(i, j, vaz(i, j) + dVp);
     if (i >= 2) {
-        const double sigma_r_avg = 0.5 * (S[IX(i, j)] + S[IX(i - 1, j)]);
+        FLD(vr, FCPT_VRAD);
+        const double sigma_r_avg = 0.5 * (S(i, j) + S(i - 1, j));
         const double r0 = P.Rmed[i], r1 = P.Rmed[i - 1];
         const double dVr = P.radial_viscosity_factor * dt / sigma_r_avg * 2.0 / (r0 * r0 - r1 * r1) *
-                           ((Qr[IX(i, j)] * r0 - Qr[IX(i - 1, j)] * r1) -
-                            0.5 * (Qp[IX(i, j)] + Qp[IX(i - 1, j)]) * (r0 - r1));
-        P.f[FCPT_VRAD][IX(i, j)] += dVr;
+                           ((Qr(i, j) * r0 - Qr(i - 1, j) * r1) -
+                            0.5 * (Qp(i, j) + Qp(i - 1, j)) * (r0 - r1));
+        vr.set(i, j, vr(i, j) + dVr);
     }
 }
 
 // update_with_artificial_viscosity_SN, artificial_viscosity.cpp:148-211
+template <bool SPLIT>
 __global__ void __launch_bounds__(256) k_sn_q(KParams P)
 {
     CELL_LOOP_HEAD(0, P.nr)
-    const double *vr = P.f[FCPT_VRAD], *vaz = P.f[FCPT_VAZI], *S = P.f[FCPT_SIGMA];
+    FLD(vr, FCPT_VRAD);
+    FLD(vaz, FCPT_VAZI);
+    FLD(S, FCPT_SIGMA);
     const double fac2 = P.artvisc_factor * P.artvisc_factor;
-    const double sig = S[IX(i, j)];
-    const double dv_r = vr[IX(i + 1, j)] - vr[IX(i, j)];
+    const double sig = S(i, j);
+    const double dv_r = vr(i + 1, j) - vr(i, j);
     const double qr = dv_r < 0.0 ? fac2 * sig * (dv_r * dv_r) : 0.0;
     const int jn = j == ns - 1 ? 0 : j + 1;
-    const double dv_phi = vaz[IX(i, jn)] - vaz[IX(i, j)];
+    const double dv_phi = vaz(i, jn) - vaz(i, j);
     const double qp = dv_phi < 0.0 ? fac2 * sig * (dv_phi * dv_phi) : 0.0;
-    P.f[FCPT_Q_R][IX(i, j)] = qr;
-    P.f[FCPT_Q_PHI][IX(i, j)] = qp;
+    fld<SPLIT>(P, FCPT_Q_R).set(i, j, qr);
+    fld<SPLIT>(P, FCPT_Q_PHI).set(i, j, qp);
     if (P.adiabatic && P.artvisc_dissipation) {
-        double *E = P.f[FCPT_ENERGY];
-        double e = E[IX(i, j)];
+        FLD(E, FCPT_ENERGY);
+        double e = E(i, j);
         if (i >= 1 && i < P.nr - 1) {
             const double dt = P.s->dt;
             const double dxtheta = P.dphi * P.Rmed[i];
             const double invdxtheta = 1.0 / dxtheta;
             e = e - dt * qr * dv_r * P.InvDiffRsup[i] - dt * qp * dv_phi * invdxtheta;
         }
-        E[IX(i, j)] = temperature_floor(P, e, sig);
+        E.set(i, j, temperature_floor(P, e, sig));
     }
 }
 
 // artificial_viscosity.cpp:214-248
+template <bool SPLIT>
 __global__ void __launch_bounds__(256) k_sn_vel(KParams P, int with_viscosity)
 {
     CELL_LOOP_HEAD(0, P.nr)
     // in the fused step, recalculate_viscosity (SourceEuler.cpp:205-223), which is
     // pointwise in Sigma and E and follows this kernel, is done here for every ring
-    if (with_viscosity) derived_cell(P, i, IX(i, j), true, false, false);
+    if (with_viscosity) derived_cell<SPLIT>(P, i, j, false, false);
     if (i < 1 || i >= P.nr - 1) return;
-    const double *S = P.f[FCPT_SIGMA], *Qr = P.f[FCPT_Q_R], *Qp = P.f[FCPT_Q_PHI];
+    FLD(S, FCPT_SIGMA);
+    FLD(Qr, FCPT_Q_R);
+    FLD(Qp, FCPT_Q_PHI);
     const double dt = P.s->dt;
     if (i >= 2) {
-        double *vr = P.f[FCPT_VRAD];
-        vr[IX(i, j)] = vr[IX(i, j)] - dt * 2.0 / (S[IX(i, j)] + S[IX(i - 1, j)]) *
-                                          (Qr[IX(i, j)] - Qr[IX(i - 1, j)]) * P.InvDiffRmed[i];
+        FLD(vr, FCPT_VRAD);
+        vr.set(i, j, vr(i, j) - dt * 2.0 / (S(i, j) + S(i - 1, j)) * (Qr(i, j) - Qr(i - 1, j)) *
+                                    P.InvDiffRmed[i]);
     }
     const double dxtheta = P.dphi * P.Rmed[i];
     const double invdxtheta = 1.0 / dxtheta;
     const int jp = j == 0 ? ns - 1 : j - 1;
-    double *vaz = P.f[FCPT_VAZI];
-    vaz[IX(i, j)] = vaz[IX(i, j)] - dt * 2.0 / (S[IX(i, j)] + S[IX(i, jp)]) *
-                                        (Qp[IX(i, j)] - Qp[IX(i, jp)]) * invdxtheta;
+    FLD(vaz, FCPT_VAZI);
+    vaz.set(i, j, vaz(i, j) - dt * 2.0 / (S(i, j) + S(i, jp)) * (Qp(i, j) - Qp(i, jp)) * invdxtheta);
 }
 
 // recalculate_viscosity, SourceEuler.cpp:205-223
+template <bool SPLIT>
 __global__ void __launch_bounds__(256) k_recalc_viscosity(KParams P)
 {
     CELL_LOOP_HEAD(0, P.nr)
-    derived_cell(P, i, IX(i, j), true, false, false);
+    derived_cell<SPLIT>(P, i, j, false, false);
 }
 
 // recalculate_derived_disk_quantities, SourceEuler.cpp:225-249
+template <bool SPLIT>
 __global__ void __launch_bounds__(256) k_derived(KParams P)
 {
     CELL_LOOP_HEAD(0, P.nr)
-    derived_cell(P, i, IX(i, j), true, true, true);
+    derived_cell<SPLIT>(P, i, j, true, true);
 }
 
 // locally isothermal sound speed and scale height from the initial setup
 // (compute_sound_speed_normal / compute_scale_height_old, isothermal branches)
+template <bool SPLIT>
 __global__ void __launch_bounds__(256) k_iso_profile(KParams P)
 {
     CELL_LOOP_HEAD(0, P.nr)
@@ -468,82 +563,88 @@ __global__ void __launch_bounds__(256) k_iso_profile(KParams P)
     const double vK = sqrt(P.G * P.mstar / r);
     const double h = P.aspect_ratio * pow(r, P.flaring);
     const double c_s = h * vK;
-    P.f[FCPT_SOUNDSPEED][IX(i, j)] = c_s;
+    fld<SPLIT>(P, FCPT_SOUNDSPEED).set(i, j, c_s);
     const double inv_omega_kepler = 1.0 / omega_kepler(P, r);
-    P.f[FCPT_SCALE_HEIGHT][IX(i, j)] = c_s * inv_omega_kepler;
-    const double sig = P.f[FCPT_SIGMA][IX(i, j)];
+    fld<SPLIT>(P, FCPT_SCALE_HEIGHT).set(i, j, c_s * inv_omega_kepler);
+    const double sig = fld<SPLIT>(P, FCPT_SIGMA)(i, j);
     const double pr = sig * (c_s * c_s);
-    P.f[FCPT_PRESSURE][IX(i, j)] = pr;
-    P.f[FCPT_TEMPERATURE][IX(i, j)] = P.mu / P.Rgas * pr / sig;
+    fld<SPLIT>(P, FCPT_PRESSURE).set(i, j, pr);
+    fld<SPLIT>(P, FCPT_TEMPERATURE).set(i, j, P.mu / P.Rgas * pr / sig);
 }
 
 // ---------------------------------------------------------------------------
 // compute_viscous_stress_tensor, viscosity.cpp:139-254
+template <bool SPLIT>
 __global__ void __launch_bounds__(256) k_stress(KParams P)
 {
     CELL_LOOP_HEAD(0, P.nr)
-    const double *vr = P.f[FCPT_VRAD], *vaz = P.f[FCPT_VAZI], *nu = P.f[FCPT_VISCOSITY],
-                 *S = P.f[FCPT_SIGMA];
+    FLD(vr, FCPT_VRAD);
+    FLD(vaz, FCPT_VAZI);
+    FLD(nu, FCPT_VISCOSITY);
+    FLD(S, FCPT_SIGMA);
     const int jn = j == ns - 1 ? 0 : j + 1;
-    const double divv =
-        (vr[IX(i + 1, j)] * P.Rinf[i + 1] - vr[IX(i, j)] * P.Rinf[i]) * P.InvDiffRsupRb[i] +
-        (vaz[IX(i, jn)] - vaz[IX(i, j)]) * P.invdphi * P.InvRmed[i];
-    P.f[FCPT_DIV_V][IX(i, j)] = divv;
-    const double n = nu[IX(i, j)], s = S[IX(i, j)];
-    const double drr = (vr[IX(i + 1, j)] - vr[IX(i, j)]) * P.InvDiffRsup[i];
-    P.f[FCPT_TAU_R_R][IX(i, j)] = 2.0 * n * s * (drr - 1.0 / 3.0 * divv);
-    const double dpp = (vaz[IX(i, jn)] - vaz[IX(i, j)]) * P.invdphi * P.InvRmed[i] +
-                       0.5 * (vr[IX(i + 1, j)] + vr[IX(i, j)]) * P.InvRmed[i];
-    P.f[FCPT_TAU_PHI_PHI][IX(i, j)] = 2.0 * n * s * (dpp - 1.0 / 3.0 * divv);
+    const double vr0 = vr(i, j), vr1 = vr(i + 1, j), va0 = vaz(i, j), va1 = vaz(i, jn);
+    const double divv = (vr1 * P.Rinf[i + 1] - vr0 * P.Rinf[i]) * P.InvDiffRsupRb[i] +
+                        (va1 - va0) * P.invdphi * P.InvRmed[i];
+    fld<SPLIT>(P, FCPT_DIV_V).set(i, j, divv);
+    const double n = nu(i, j), s = S(i, j);
+    const double drr = (vr1 - vr0) * P.InvDiffRsup[i];
+    fld<SPLIT>(P, FCPT_TAU_R_R).set(i, j, 2.0 * n * s * (drr - 1.0 / 3.0 * divv));
+    const double dpp = (va1 - va0) * P.invdphi * P.InvRmed[i] + 0.5 * (vr1 + vr0) * P.InvRmed[i];
+    fld<SPLIT>(P, FCPT_TAU_PHI_PHI).set(i, j, 2.0 * n * s * (dpp - 1.0 / 3.0 * divv));
     if (i >= 1) {
         const int jp = j == 0 ? ns - 1 : j - 1;
         const double dvazirdr =
-            (vaz[IX(i, j)] * P.InvRmed[i] - vaz[IX(i - 1, j)] * P.InvRmed[i - 1]) * P.InvDiffRmed[i];
-        const double dvrdphi = (vr[IX(i, j)] - vr[IX(i, jp)]) * P.invdphi;
+            (va0 * P.InvRmed[i] - vaz(i - 1, j) * P.InvRmed[i - 1]) * P.InvDiffRmed[i];
+        const double dvrdphi = (vr0 - vr(i, jp)) * P.invdphi;
         const double drp = P.Rinf[i] * dvazirdr + dvrdphi * P.InvRinf[i];
-        const double n4 = 0.25 * (nu[IX(i, j)] + nu[IX(i - 1, j)] + nu[IX(i, jp)] + nu[IX(i - 1, jp)]);
-        const double s4 = 0.25 * (S[IX(i, j)] + S[IX(i - 1, j)] + S[IX(i, jp)] + S[IX(i - 1, jp)]);
-        P.f[FCPT_TAU_R_PHI][IX(i, j)] = n4 * s4 * drp;
+        const double n4 = 0.25 * (n + nu(i - 1, j) + nu(i, jp) + nu(i - 1, jp));
+        const double s4 = 0.25 * (s + S(i - 1, j) + S(i, jp) + S(i - 1, jp));
+        fld<SPLIT>(P, FCPT_TAU_R_PHI).set(i, j, n4 * s4 * drp);
     }
 }
 
 // update_velocities_with_viscosity, viscosity.cpp:355-426
+template <bool SPLIT>
 __global__ void __launch_bounds__(256) k_visc_update(KParams P)
 {
     CELL_LOOP_HEAD(1, P.nr - 1)
-    const double *S = P.f[FCPT_SIGMA], *trr = P.f[FCPT_TAU_R_R], *tpp = P.f[FCPT_TAU_PHI_PHI],
-                 *trp = P.f[FCPT_TAU_R_PHI];
+    FLD(S, FCPT_SIGMA);
+    FLD(trr, FCPT_TAU_R_R);
+    FLD(tpp, FCPT_TAU_PHI_PHI);
+    FLD(trp, FCPT_TAU_R_PHI);
     const double dt = P.s->dt;
     const int jp = j == 0 ? ns - 1 : j - 1;
     {
-        const double sigma_avg = 0.5 * (S[IX(i, j)] + S[IX(i, jp)]);
+        FLD(vaz, FCPT_VAZI);
+        const double sigma_avg = 0.5 * (S(i, j) + S(i, jp));
         const double a1 = P.Rinf[i + 1] * P.Rinf[i + 1], a0 = P.Rinf[i] * P.Rinf[i];
         const double dVp = dt * P.InvRmed[i] / (sigma_avg) *
-                           ((2.0 / (a1 - a0)) * (a1 * trp[IX(i + 1, j)] - a0 * trp[IX(i, j)]) +
-                            (tpp[IX(i, j)] - tpp[IX(i, jp)]) * P.invdphi);
-        P.f[FCPT_VAZI][IX(i, j)] += dVp;
+                           ((2.0 / (a1 - a0)) * (a1 * trp(i + 1, j) - a0 * trp(i, j)) +
+                            (tpp(i, j) - tpp(i, jp)) * P.invdphi);
+        vaz.set(i, j, vaz(i, j) + dVp);
     }
     if (i >= 2) {
+        FLD(vr, FCPT_VRAD);
         const int jn = j == ns - 1 ? 0 : j + 1;
-        const double sigma_avg = 0.5 * (S[IX(i, j)] + S[IX(i - 1, j)]);
+        const double sigma_avg = 0.5 * (S(i, j) + S(i - 1, j));
         const double dVr = dt / (sigma_avg)*P.radial_viscosity_factor * 2.0 /
                            (P.Rmed[i] + P.Rmed[i - 1]) *
-                           ((P.Rmed[i] * trr[IX(i, j)] - P.Rmed[i - 1] * trr[IX(i - 1, j)]) *
-                                P.InvDiffRmed[i] +
-                            (trp[IX(i, jn)] - trp[IX(i, j)]) * P.invdphi -
-                            0.5 * (tpp[IX(i, j)] + tpp[IX(i - 1, j)]));
-        P.f[FCPT_VRAD][IX(i, j)] += dVr;
+                           ((P.Rmed[i] * trr(i, j) - P.Rmed[i - 1] * trr(i - 1, j)) * P.InvDiffRmed[i] +
+                            (trp(i, jn) - trp(i, j)) * P.invdphi - 0.5 * (tpp(i, j) + tpp(i - 1, j)));
+        vr.set(i, j, vr(i, j) + dVr);
     }
 }
 
 // ---------------------------------------------------------------------------
 // Q- (thermal_relaxation, SourceEuler.cpp:632-690) and Q+ (viscous_heating,
 // SourceEuler.cpp:496-536) for one active cell
+template <bool SPLIT>
 __device__ __forceinline__ void heating_cooling_cell(const KParams &P, int i, int j, double time,
-                                                     double e, double &qplus, double &qminus)
+                                                     double e, double sig, double &qplus,
+                                                     double &qminus)
 {
     const int ns = P.ns;
-    const double sig = P.f[FCPT_SIGMA][IX(i, j)];
     qminus = 0.0;
     if (P.cooling_beta_enabled) {
         const double r = P.Rmed[i];
@@ -556,23 +657,22 @@ __device__ __forceinline__ void heating_cooling_cell(const KParams &P, int i, in
         }
         double delta_E = e;
         if (P.cooling_beta_ref == FCPT_BETA_REF_REFERENCE)
-            delta_E -= P.f[FCPT_ENERGY0][IX(i, j)] / P.f[FCPT_SIGMA0][IX(i, j)] * sig;
+            delta_E -= fld<SPLIT>(P, FCPT_ENERGY0)(i, j) / fld<SPLIT>(P, FCPT_SIGMA0)(i, j) * sig;
         if (P.cooling_beta_ref == FCPT_BETA_REF_FLOOR)
             delta_E -= P.t_min * sig / P.mu * P.Rgas / (P.gamma - 1.0);
         qminus = 0.0 + delta_E * omega_k * beta_inv;
     }
     qplus = 0.0;
     if (P.heating_viscous) {
-        const double nu = P.f[FCPT_VISCOSITY][IX(i, j)];
+        const double nu = fld<SPLIT>(P, FCPT_VISCOSITY)(i, j);
         if (nu != 0.0) {
-            const double *trp = P.f[FCPT_TAU_R_PHI];
+            FLD(trp, FCPT_TAU_R_PHI);
             const int jn = j == ns - 1 ? 0 : j + 1;
-            const double tau_r_phi =
-                0.25 * (trp[IX(i, j)] + trp[IX(i + 1, j)] + trp[IX(i, jn)] + trp[IX(i + 1, jn)]);
-            const double trr = P.f[FCPT_TAU_R_R][IX(i, j)], tpp = P.f[FCPT_TAU_PHI_PHI][IX(i, j)];
-            const double dv = P.f[FCPT_DIV_V][IX(i, j)];
-            double q = 1.0 / (2.0 * nu * sig) *
-                       (trr * trr + 2 * (tau_r_phi * tau_r_phi) + tpp * tpp);
+            const double tau_r_phi = 0.25 * (trp(i, j) + trp(i + 1, j) + trp(i, jn) + trp(i + 1, jn));
+            const double trr = fld<SPLIT>(P, FCPT_TAU_R_R)(i, j);
+            const double tpp = fld<SPLIT>(P, FCPT_TAU_PHI_PHI)(i, j);
+            const double dv = fld<SPLIT>(P, FCPT_DIV_V)(i, j);
+            double q = 1.0 / (2.0 * nu * sig) * (trr * trr + 2 * (tau_r_phi * tau_r_phi) + tpp * tpp);
             q += (2.0 / 9.0) * nu * sig * (dv * dv);
             q *= P.heating_viscous_factor;
             qplus = 0.0 + q;
@@ -587,21 +687,21 @@ __device__ __forceinline__ double rad_alpha(const KParams &P, double h, double s
 }
 
 // SubStep3, SourceEuler.cpp:859-954
+template <bool SPLIT>
 __global__ void __launch_bounds__(256) k_substep3(KParams P)
 {
     CELL_LOOP_HEAD(0, P.nr)
-    const size_t c = IX(i, j);
-    double *E = P.f[FCPT_ENERGY];
-    const double sig = P.f[FCPT_SIGMA][c];
-    double e = E[c];
+    FLD(E, FCPT_ENERGY);
+    const double sig = fld<SPLIT>(P, FCPT_SIGMA)(i, j);
+    double e = E(i, j);
     // compute_temperature (SourceEuler.cpp:1378-1408) before the update
     const double c_v_inv = P.mu / P.Rgas * (P.gamma - 1.0);
-    P.f[FCPT_TEMPERATURE][c] = c_v_inv * e / sig;
+    fld<SPLIT>(P, FCPT_TEMPERATURE).set(i, j, c_v_inv * e / sig);
     double qplus = 0.0, qminus = 0.0;
     if (i >= 1 && i < P.nr - 1) {
         const DevScalars *s = P.s;
-        heating_cooling_cell(P, i, j, s->time, e, qplus, qminus);
-        const double alpha = rad_alpha(P, P.f[FCPT_SCALE_HEIGHT][c], sig, e);
+        heating_cooling_cell<SPLIT>(P, i, j, s->time, e, sig, qplus, qminus);
+        const double alpha = rad_alpha(P, fld<SPLIT>(P, FCPT_SCALE_HEIGHT)(i, j), sig, e);
         qplus /= alpha;
         qminus /= alpha;
         double energy_new = e + s->dt * (qplus - qminus);
@@ -616,32 +716,34 @@ __global__ void __launch_bounds__(256) k_substep3(KParams P)
         }
         e = energy_new;
     }
-    P.f[FCPT_QPLUS][c] = qplus;
-    P.f[FCPT_QMINUS][c] = qminus;
-    E[c] = temperature_floor(P, e, sig);
+    fld<SPLIT>(P, FCPT_QPLUS).set(i, j, qplus);
+    fld<SPLIT>(P, FCPT_QMINUS).set(i, j, qminus);
+    E.set(i, j, temperature_floor(P, e, sig));
 }
 
 // compute_heating_cooling_for_CFL, SourceEuler.cpp:1410-1450 (after update_viscosity
 // and the stress tensor)
+template <bool SPLIT>
 __global__ void __launch_bounds__(256) k_heatcool_cfl(KParams P)
 {
     CELL_LOOP_HEAD(0, P.nr)
-    const size_t c = IX(i, j);
     double qplus = 0.0, qminus = 0.0;
     if (i >= 1 && i < P.nr - 1) {
-        const double e = P.f[FCPT_ENERGY][c];
-        heating_cooling_cell(P, i, j, P.s->time, e, qplus, qminus);
-        const double alpha = rad_alpha(P, P.f[FCPT_SCALE_HEIGHT][c], P.f[FCPT_SIGMA][c], e);
+        const double e = fld<SPLIT>(P, FCPT_ENERGY)(i, j);
+        const double sig = fld<SPLIT>(P, FCPT_SIGMA)(i, j);
+        heating_cooling_cell<SPLIT>(P, i, j, P.s->time, e, sig, qplus, qminus);
+        const double alpha = rad_alpha(P, fld<SPLIT>(P, FCPT_SCALE_HEIGHT)(i, j), sig, e);
         qplus /= alpha;
         qminus /= alpha;
     }
-    P.f[FCPT_QPLUS][c] = qplus;
-    P.f[FCPT_QMINUS][c] = qminus;
+    fld<SPLIT>(P, FCPT_QPLUS).set(i, j, qplus);
+    fld<SPLIT>(P, FCPT_QMINUS).set(i, j, qminus);
 }
 
 // ---------------------------------------------------------------------------
 // damping (boundary_conditions/damping.cpp:311-427, 754-776): block row y maps
 // to an inner-zone ring (y < n_inner) or an outer-zone ring
+template <bool SPLIT>
 __global__ void __launch_bounds__(256) k_damping(KParams P, int n_inner)
 {
     const int ns = P.ns;
@@ -663,14 +765,15 @@ __global__ void __launch_bounds__(256) k_damping(KParams P, int n_inner)
         if (i > P.nr) return;
         inner = false;
     }
-    const double *F0[4] = {P.f[FCPT_VRAD0], P.f[FCPT_VAZI0], P.f[FCPT_SIGMA0], P.f[FCPT_ENERGY0]};
-    double *F[4] = {P.f[FCPT_VRAD], P.f[FCPT_VAZI], P.f[FCPT_SIGMA], P.f[FCPT_ENERGY]};
+    const int F0[4] = {FCPT_VRAD0, FCPT_VAZI0, FCPT_SIGMA0, FCPT_ENERGY0};
+    const int F[4] = {FCPT_VRAD, FCPT_VAZI, FCPT_SIGMA, FCPT_ENERGY};
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const bool vector = q == 0;
         const double *radius = vector ? P.Rinf : P.Rmed;
         const int size_radial = vector ? P.nr + 1 : P.nr;
         if (i >= size_radial) continue;
+        const Fld<SPLIT> X = fld<SPLIT>(P, F[q]), X0 = fld<SPLIT>(P, F0[q]);
         if (inner) {
             if (!((lin > 1.0) && (radius[0] < RMIN * lin))) continue;
             const int limit = vector ? P.damp_ids[1] : P.damp_ids[0];
@@ -678,8 +781,8 @@ __global__ void __launch_bounds__(256) k_damping(KParams P, int n_inner)
             const double tau = P.damping_time_factor * 2.0 * M_PI / omega_kepler(P, RMIN);
             const double a = (radius[i] - RMIN * lin) / (RMIN - RMIN * lin);
             const double exp_factor = exp(-dt * (a * a) / tau);
-            const double x = F[q][IX(i, j)], x0 = F0[q][IX(i, j)];
-            F[q][IX(i, j)] = (x - x0) * exp_factor + x0;
+            const double x = X(i, j), x0 = X0(i, j);
+            X.set(i, j, (x - x0) * exp_factor + x0);
         } else {
             if (!((lout < 1.0) && (radius[size_radial - 1] > RMAX * lout))) continue;
             const int limit = vector ? P.damp_ids[3] : P.damp_ids[2];
@@ -688,14 +791,15 @@ __global__ void __launch_bounds__(256) k_damping(KParams P, int n_inner)
                 P.damping_time_factor * 2.0 * M_PI / omega_kepler(P, P.damping_time_radius_outer);
             const double a = (radius[i] - RMAX * lout) / (RMAX - RMAX * lout);
             const double exp_factor = exp(-dt * (a * a) / tau);
-            const double x = F[q][IX(i, j)], x0 = F0[q][IX(i, j)];
-            F[q][IX(i, j)] = (x - x0) * exp_factor + x0;
+            const double x = X(i, j), x0 = X0(i, j);
+            X.set(i, j, (x - x0) * exp_factor + x0);
         }
     }
 }
 
 // ghost rings: zero-gradient Sigma/E, reflecting / zero-gradient / outflow v_r,
 // Keplerian v_phi (boundary_conditions.cpp:67-97 and the BC files it dispatches to)
+template <bool SPLIT>
 __global__ void __launch_bounds__(256) k_ghost_bc(KParams P, int advance_time)
 {
     const int ns = P.ns;
@@ -711,53 +815,57 @@ __global__ void __launch_bounds__(256) k_ghost_bc(KParams P, int advance_time)
     }
     if (j >= ns) return;
     const int nr = P.nr;
-    double *S = P.f[FCPT_SIGMA], *E = P.f[FCPT_ENERGY], *vr = P.f[FCPT_VRAD],
-           *vaz = P.f[FCPT_VAZI];
-    S[IX(0, j)] = S[IX(1, j)];
-    S[IX(nr - 1, j)] = S[IX(nr - 2, j)];
-    E[IX(0, j)] = E[IX(1, j)];
-    E[IX(nr - 1, j)] = E[IX(nr - 2, j)];
+    FLD(S, FCPT_SIGMA);
+    FLD(E, FCPT_ENERGY);
+    FLD(vr, FCPT_VRAD);
+    FLD(vaz, FCPT_VAZI);
+    S.set(0, j, S(1, j));
+    S.set(nr - 1, j, S(nr - 2, j));
+    E.set(0, j, E(1, j));
+    E.set(nr - 1, j, E(nr - 2, j));
+    // rows 2 and nr-2 are never written here (nr >= 8)
+    const double v2 = vr(2, j), vn2 = vr(nr - 2, j);
     if (P.bc_inner == FCPT_BC_REFLECTING) {
-        vr[IX(0, j)] = -vr[IX(2, j)];
-        vr[IX(1, j)] = 0;
+        vr.set(0, j, -v2);
+        vr.set(1, j, 0);
     } else if (P.bc_inner == FCPT_BC_ZEROGRADIENT) {
-        vr[IX(0, j)] = vr[IX(2, j)];
-        vr[IX(1, j)] = vr[IX(2, j)];
+        vr.set(0, j, v2);
+        vr.set(1, j, v2);
     } else {
-        if (vr[IX(2, j)] > 0.0) {
-            vr[IX(1, j)] = 0.0;
-            vr[IX(0, j)] = 0.0;
+        if (v2 > 0.0) {
+            vr.set(1, j, 0.0);
+            vr.set(0, j, 0.0);
         } else {
-            vr[IX(1, j)] = vr[IX(2, j)];
-            vr[IX(0, j)] = vr[IX(2, j)];
+            vr.set(1, j, v2);
+            vr.set(0, j, v2);
         }
     }
     if (P.bc_outer == FCPT_BC_REFLECTING) {
-        vr[IX(nr, j)] = -vr[IX(nr - 2, j)];
-        vr[IX(nr - 1, j)] = 0;
+        vr.set(nr, j, -vn2);
+        vr.set(nr - 1, j, 0);
     } else if (P.bc_outer == FCPT_BC_ZEROGRADIENT) {
-        vr[IX(nr, j)] = vr[IX(nr - 2, j)];
-        vr[IX(nr - 1, j)] = vr[IX(nr - 2, j)];
+        vr.set(nr, j, vn2);
+        vr.set(nr - 1, j, vn2);
     } else {
-        if (vr[IX(nr - 2, j)] < 0.0) {
-            vr[IX(nr - 1, j)] = 0.0;
-            vr[IX(nr, j)] = 0.0;
+        if (vn2 < 0.0) {
+            vr.set(nr - 1, j, 0.0);
+            vr.set(nr, j, 0.0);
         } else {
-            vr[IX(nr - 1, j)] = vr[IX(nr - 2, j)];
-            vr[IX(nr, j)] = vr[IX(nr - 2, j)];
+            vr.set(nr - 1, j, vn2);
+            vr.set(nr, j, vn2);
         }
     }
     const double vk_in = sqrt(P.G * P.mstar / P.Rmed[0]);
     const double vk_out = sqrt(P.G * P.mstar / P.Rmed[nr - 1]);
-    vaz[IX(0, j)] = 1.0 * vk_in - P.Rmed[0] * P.omega_frame;
-    vaz[IX(nr - 1, j)] = 1.0 * vk_out - P.Rmed[nr - 1] * P.omega_frame;
+    vaz.set(0, j, 1.0 * vk_in - P.Rmed[0] * P.omega_frame);
+    vaz.set(nr - 1, j, 1.0 * vk_out - P.Rmed[nr - 1] * P.omega_frame);
 }
 
 // ---------------------------------------------------------------------------
 // Transport
 
 // van Leer slope of a radial profile at ring k (compute_star_radial first loop,
-// TransportEuler.cpp:355-371); q[] holds rings k-1..k+1
+// TransportEuler.cpp:355-371); arguments are rings k-1..k+1
 __device__ __forceinline__ double dq_radial(const KParams &P, int k, double qm, double q0, double qp)
 {
     if (k == 0 || k == P.nr - 1) return 0.0;
@@ -767,97 +875,119 @@ __device__ __forceinline__ double dq_radial(const KParams &P, int k, double qm, 
 }
 
 // upwind interface value at ring interface i (compute_star_radial second loop,
-// TransportEuler.cpp:376-405); w[0..4] = rings i-2..i+2 of the specific quantity
-__device__ __forceinline__ double star_radial(const KParams &P, int i, const double *w, double v,
-                                              double dt)
+// TransportEuler.cpp:376-405); w0..w3 = rings i-2..i+1 of the specific quantity
+__device__ __forceinline__ double star_radial(const KParams &P, int i, double w0, double w1,
+                                              double w2, double w3, double v, double dt)
 {
     if (i <= 0 || i >= P.nr) return 0.0;
     if (v > 0.0) {
-        const double dq = dq_radial(P, i - 1, w[0], w[1], w[2]);
-        return w[1] + (P.Rmed[i] - P.Rmed[i - 1] - v * dt) * 0.5 * dq;
+        const double dq = dq_radial(P, i - 1, w0, w1, w2);
+        return w1 + (P.Rmed[i] - P.Rmed[i - 1] - v * dt) * 0.5 * dq;
     }
-    const double dq = dq_radial(P, i, w[1], w[2], w[3]);
-    return w[2] - (P.Rmed[i + 1] - P.Rmed[i] + v * dt) * 0.5 * dq;
+    const double dq = dq_radial(P, i, w1, w2, w3);
+    return w2 - (P.Rmed[i + 1] - P.Rmed[i] + v * dt) * 0.5 * dq;
 }
 
-// OneWindRad, TransportEuler.cpp:138-168 and VanLeerRadial :545-619, as one
-// LDS-tiled pass.  Every advected quantity of ring i is updated from rings
-// i-2..i+2 of itself and of the density before the sweep (DENSITY_INT), so a
-// block stages 64 sectors x (TR + 4) rings of Sigma and of each quantity's
-// specific value Q/Sigma in LDS, forms every interface value once, and writes
-// the TR updated rings to separate outputs.  The four cell-centred momenta
-// (compute_momenta_from_velocities, :471-495) are formed on the fly from
-// Sigma, v_r and v_phi with the reference's expressions.
-template <int TR>
-__global__ void __launch_bounds__(256) k_radial_sweep(KParams P)
+// One ring of the radial sweep's window: Sigma, the six advected quantities
+// (four momenta, E, Sigma itself), their specific values Q / Sigma and v_r at
+// the ring's inner interface.  The momenta are formed with the expressions of
+// compute_momenta_from_velocities (TransportEuler.cpp:471-495).  Rings outside
+// the grid hold Sigma = 1 and zero quantities (their interface values are
+// never used: the slopes there are zero and DensityStar vanishes).
+struct RingVals {
+    double s;
+    double q[6];
+    double w[6];
+    double v;
+};
+
+template <bool SPLIT>
+__device__ __forceinline__ RingVals load_ring(const KParams &P, int r, int j, int jn)
 {
-    constexpr int R = TR + 4;  // rings i0-2 .. i0+TR+1
-    __shared__ double sS[R][64], sW[R][64], sQ[R][64];
-    __shared__ double sDS[TR + 1][64], sQS[TR + 1][64], sV[TR + 1][64];
+    // loads go to a clamped ring and the results are selected, which keeps
+    // the window in registers
+    const bool in = r >= 0 && r < P.nr;
+    const int rc = in ? r : 0;
+    const int rv = r < 0 ? 0 : (r > P.nr ? P.nr : r);
+    FLD(S, FCPT_SIGMA);
+    FLD(vr, FCPT_VRAD);
+    FLD(vaz, FCPT_VAZI);
+    const double sig = S(rc, j);
+    const double rr = P.Rmed[rc];
+    const double OmegaF = P.omega_frame;
+    const double v0 = vr(rc, j);
+    double q[6];
+    q[0] = sig * vr(rc + 1, j);
+    q[1] = sig * v0;
+    q[2] = sig * (vaz(rc, jn) + rr * OmegaF) * rr;
+    q[3] = sig * (vaz(rc, j) + rr * OmegaF) * rr;
+    q[4] = P.adiabatic ? fld<SPLIT>(P, FCPT_ENERGY)(rc, j) : 0.0;
+    q[5] = sig;
+    RingVals R;
+    const double vv = vr(rv, j);
+    R.v = (r >= 0 && r <= P.nr) ? vv : 0.0;
+    R.s = in ? sig : 1.0;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        R.q[k] = in ? q[k] : 0.0;
+        R.w[k] = in ? q[k] / sig : 0.0;
+    }
+    return R;
+}
+
+// OneWindRad, TransportEuler.cpp:138-168 and VanLeerRadial :545-619.  Every
+// advected quantity of ring i is updated from rings i-2..i+2 of itself and of
+// the density before the sweep (DENSITY_INT), so all of them are done in one
+// pass that reads the state and writes separate outputs.  A thread owns one
+// sector and walks `tile` consecutive rings outward with a window of four
+// rings in registers: each ring is loaded once and each interface value
+// (DensityStar and every quantity's QStar) is formed once.
+template <bool SPLIT>
+__global__ void __launch_bounds__(64) k_radial_sweep(KParams P, int tile)
+{
     const int ns = P.ns, nr = P.nr;
-    const int tx = threadIdx.x, ty = threadIdx.y;
-    const int j = blockIdx.x * 64 + tx;
-    const int i0 = blockIdx.y * TR;
-    const bool col = j < ns;
+    const int j = blockIdx.x * 64 + threadIdx.x;
+    const int i0 = blockIdx.y * tile;
+    if (j >= ns || i0 >= nr) return;
     const int jn = j == ns - 1 ? 0 : j + 1;
     const double dt = P.s->dt;
-    const double *S = P.f[FCPT_SIGMA], *vr = P.f[FCPT_VRAD], *vaz = P.f[FCPT_VAZI],
-                 *E = P.f[FCPT_ENERGY];
-    const double OmegaF = P.omega_frame;
-    for (int p = ty; p < R; p += 4) {
-        const int r = i0 - 2 + p;
-        sS[p][tx] = (col && r >= 0 && r < nr) ? S[IX(r, j)] : 1.0;
-    }
-    for (int p = ty; p < TR + 1; p += 4) {
-        const int r = i0 + p;
-        sV[p][tx] = (col && r <= nr) ? vr[IX(r, j)] : 0.0;
-    }
-    __syncthreads();
-    // DensityStar at interfaces i0 .. i0+TR (interface nr is never written: 0)
-    for (int p = ty; p < TR + 1; p += 4) {
-        const double w4[4] = {sS[p][tx], sS[p + 1][tx], sS[p + 2][tx], sS[p + 3][tx]};
-        sDS[p][tx] = star_radial(P, i0 + p, w4, sV[p][tx], dt);
-    }
-    const int nq = P.adiabatic ? 6 : 5;
-    for (int q = 0; q < nq; ++q) {
-        double *out;
-        if (q < 4) out = P.mom_b[q];
-        else if (q == 4 && P.adiabatic) out = P.en_b;
-        else out = P.sig_b;
-        // quantity and specific quantity on the staged rings
-        for (int p = ty; p < R; p += 4) {
-            const int r = i0 - 2 + p;
-            double Qv = 0.0, w = 0.0;
-            if (col && r >= 0 && r < nr) {
-                const double sig = sS[p][tx];
-                const double rr = P.Rmed[r];
-                if (q == 0) Qv = sig * vr[IX(r + 1, j)];
-                else if (q == 1) Qv = sig * vr[IX(r, j)];
-                else if (q == 2) Qv = sig * (vaz[IX(r, jn)] + rr * OmegaF) * rr;
-                else if (q == 3) Qv = sig * (vaz[IX(r, j)] + rr * OmegaF) * rr;
-                else if (q == 4 && P.adiabatic) Qv = E[IX(r, j)];
-                else Qv = sig;
-                w = Qv / sig;
-            }
-            sQ[p][tx] = Qv;
-            sW[p][tx] = w;
-        }
-        __syncthreads();
-        for (int p = ty; p < TR + 1; p += 4) {
-            const double w4[4] = {sW[p][tx], sW[p + 1][tx], sW[p + 2][tx], sW[p + 3][tx]};
-            sQS[p][tx] = star_radial(P, i0 + p, w4, sV[p][tx], dt);
-        }
-        __syncthreads();
-        for (int p = ty; p < TR; p += 4) {
-            const int i = i0 + p;
-            if (col && i < nr) {
-                const double varq_inf = dt * P.dphi * P.Rinf[i] * sQS[p][tx] * sDS[p][tx] * sV[p][tx];
-                const double varq_sup =
-                    dt * P.dphi * P.Rsup[i] * sQS[p + 1][tx] * sDS[p + 1][tx] * sV[p + 1][tx];
-                out[IX(i, j)] = sQ[p + 2][tx] + (varq_inf - varq_sup) * P.InvSurf[i];
+    const bool adi = P.adiabatic != 0;
+    void *const outp[6] = {P.mom_b[0], P.mom_b[1], P.mom_b[2], P.mom_b[3], P.en_b, P.sig_b};
+    const double *const outbg[6] = {P.bgm[0], P.bgm[1], P.bgm[2], P.bgm[3], P.bg[FCPT_ENERGY],
+                                    P.bg[FCPT_SIGMA]};
+    // window at interface k: rings k-2 .. k+1
+    RingVals R0 = load_ring<SPLIT>(P, i0 - 2, j, jn);
+    RingVals R1 = load_ring<SPLIT>(P, i0 - 1, j, jn);
+    RingVals R2 = load_ring<SPLIT>(P, i0, j, jn);
+    RingVals R3 = load_ring<SPLIT>(P, i0 + 1, j, jn);
+    double qs_prev[6], ds_prev = 0.0, v_prev = 0.0;
+    const int iend = i0 + tile < nr ? i0 + tile : nr;
+    for (int k = i0;; ++k) {
+        const double v = R2.v;
+        const double ds = star_radial(P, k, R0.s, R1.s, R2.s, R3.s, v, dt);
+        double qs[6];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) qs[q] = star_radial(P, k, R0.w[q], R1.w[q], R2.w[q], R3.w[q], v, dt);
+        if (k > i0) {
+            // ring k-1 lies between interfaces k-1 and k
+            const int i = k - 1;
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                if (q == 4 && !adi) continue;
+                const double varq_inf = dt * P.dphi * P.Rinf[i] * qs_prev[q] * ds_prev * v_prev;
+                const double varq_sup = dt * P.dphi * P.Rsup[i] * qs[q] * ds * v;
+                buf<SPLIT>(P, outp[q], outbg[q]).set(i, j, R1.q[q] + (varq_inf - varq_sup) * P.InvSurf[i]);
             }
         }
-        __syncthreads();
+        if (k == iend) break;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) qs_prev[q] = qs[q];
+        ds_prev = ds;
+        v_prev = v;
+        R0 = R1;
+        R1 = R2;
+        R2 = R3;
+        R3 = load_ring<SPLIT>(P, k + 2, j, jn);
     }
 }
 
@@ -886,6 +1016,7 @@ constexpr int kAzSeg = 256;
 constexpr int kAzHalo = 4;
 constexpr int kAzExt = kAzSeg + 2 * kAzHalo;
 
+template <bool SPLIT>
 __global__ void __launch_bounds__(256) k_azimuthal_sweep(KParams P)
 {
     __shared__ double sS0[kAzExt], sV0[kAzExt], sDS0[kAzExt], sS1[kAzExt], sDS1[kAzExt];
@@ -904,7 +1035,6 @@ __global__ void __launch_bounds__(256) k_azimuthal_sweep(KParams P)
     const double invdxtheta = 1.0 / dxtheta;
     const double dxrad = (P.Rsup[i] - P.Rinf[i]) * dt;
     const double invsurf = P.InvSurf[i];
-    const size_t row = (size_t)i * ns;
     auto sector = [ns](int j) {
         if (j < 0) j += ns;
         if (j >= ns) j -= ns;
@@ -915,19 +1045,22 @@ __global__ void __launch_bounds__(256) k_azimuthal_sweep(KParams P)
         while (j >= ns) j -= ns;
         return j;
     };
-    const double *vaz = P.f[FCPT_VAZI];
+    FLD(vaz, FCPT_VAZI);
+    const Fld<SPLIT> Sin = buf<SPLIT>(P, P.sig_b, P.bg[FCPT_SIGMA]);
     for (int p = t; p < next; p += 256) {
         const int j = sector(j0 - kAzHalo + p);
-        sS0[p] = P.sig_b[row + j];
-        sV0[p] = vaz[row + j] - vm;
+        sS0[p] = Sin(i, j);
+        sV0[p] = vaz(i, j) - vm;
     }
     __syncthreads();
     // first pass (residual velocity): DensityStar on interfaces 2 .. next-2
-    for (int p = 2 + t; p <= next - 2; p += 256) sDS0[p] = star_theta(&sS0[p - 2], dxtheta, invdxtheta, sV0[p] * dt);
+    for (int p = 2 + t; p <= next - 2; p += 256)
+        sDS0[p] = star_theta(&sS0[p - 2], dxtheta, invdxtheta, sV0[p] * dt);
     // the density's own first pass
     for (int p = t; p < next; p += 256) sW[p] = sS0[p] / sS0[p];
     __syncthreads();
-    for (int p = 2 + t; p <= next - 2; p += 256) sQS[p] = star_theta(&sW[p - 2], dxtheta, invdxtheta, sV0[p] * dt);
+    for (int p = 2 + t; p <= next - 2; p += 256)
+        sQS[p] = star_theta(&sW[p - 2], dxtheta, invdxtheta, sV0[p] * dt);
     __syncthreads();
     for (int p = 2 + t; p < next - 2; p += 256) {
         double varq = dxrad * sQS[p] * sDS0[p] * sV0[p];
@@ -943,19 +1076,24 @@ __global__ void __launch_bounds__(256) k_azimuthal_sweep(KParams P)
     for (int p = kAzHalo + t; p <= kAzHalo + Ls; p += 256)
         sQS[p] = star_theta(&sW[p - 2], dxtheta, invdxtheta, vu * dt);
     __syncthreads();
-    for (int p = kAzHalo + t; p < kAzHalo + Ls; p += 256) {
-        double varq = dxrad * sQS[p] * sDS1[p] * vu;
-        varq -= dxrad * sQS[p + 1] * sDS1[p + 1] * vu;
-        P.sig_c[row + dst_sector(j0 - kAzHalo + p + nsh)] = sS1[p] + varq * invsurf;
+    {
+        const Fld<SPLIT> Sout = buf<SPLIT>(P, P.sig_c, P.bg[FCPT_SIGMA]);
+        for (int p = kAzHalo + t; p < kAzHalo + Ls; p += 256) {
+            double varq = dxrad * sQS[p] * sDS1[p] * vu;
+            varq -= dxrad * sQS[p + 1] * sDS1[p + 1] * vu;
+            Sout.set(i, dst_sector(j0 - kAzHalo + p + nsh), sS1[p] + varq * invsurf);
+        }
     }
     // the other quantities
     const int nq = P.adiabatic ? 5 : 4;
     for (int q = 0; q < nq; ++q) {
-        const double *Qin = q < 4 ? P.mom_b[q] : P.en_b;
-        double *out = q < 4 ? P.mom[q] : P.en_c;
+        const Fld<SPLIT> Qin = q < 4 ? buf<SPLIT>(P, P.mom_b[q], P.bgm[q])
+                                     : buf<SPLIT>(P, P.en_b, P.bg[FCPT_ENERGY]);
+        const Fld<SPLIT> out = q < 4 ? buf<SPLIT>(P, P.mom[q], P.bgm[q])
+                                     : buf<SPLIT>(P, P.en_c, P.bg[FCPT_ENERGY]);
         __syncthreads();
         for (int p = t; p < next; p += 256) {
-            const double Qv = Qin[row + sector(j0 - kAzHalo + p)];
+            const double Qv = Qin(i, sector(j0 - kAzHalo + p));
             sQ0[p] = Qv;
             sW[p] = Qv / sS0[p];
         }
@@ -977,7 +1115,7 @@ __global__ void __launch_bounds__(256) k_azimuthal_sweep(KParams P)
         for (int p = kAzHalo + t; p < kAzHalo + Ls; p += 256) {
             double varq = dxrad * sQS[p] * sDS1[p] * vu;
             varq -= dxrad * sQS[p + 1] * sDS1[p + 1] * vu;
-            out[row + dst_sector(j0 - kAzHalo + p + nsh)] = sQ1[p] + varq * invsurf;
+            out.set(i, dst_sector(j0 - kAzHalo + p + nsh), sQ1[p] + varq * invsurf);
         }
     }
 }
@@ -985,25 +1123,33 @@ __global__ void __launch_bounds__(256) k_azimuthal_sweep(KParams P)
 // compute_velocities_from_momenta (TransportEuler.cpp:498-535), then
 // assure_minimum_value(Sigma) (SourceEuler.cpp:102-131) and, adiabatic,
 // SetTemperatureFloorCeilValues (TransportEuler.cpp:127-131)
+template <bool SPLIT>
 __global__ void __launch_bounds__(256) k_velocities(KParams P)
 {
     CELL_LOOP_HEAD(0, P.nr)
-    const double *Sb = P.sig_c;
+    const Fld<SPLIT> Sb = buf<SPLIT>(P, P.sig_c, P.bg[FCPT_SIGMA]);
+    FLD(vr, FCPT_VRAD);
+    FLD(vaz, FCPT_VAZI);
     const int jm = j == 0 ? ns - 1 : j - 1;
+    const double s0 = Sb(i, j);
     if (i == 0) {
-        P.f[FCPT_VRAD][IX(i, j)] = 0.0;
+        vr.set(i, j, 0.0);
     } else {
-        P.f[FCPT_VRAD][IX(i, j)] = (P.mom[0][IX(i - 1, j)] + P.mom[1][IX(i, j)]) /
-                                   (Sb[IX(i - 1, j)] + Sb[IX(i, j)]);
+        const Fld<SPLIT> m0 = buf<SPLIT>(P, P.mom[0], P.bgm[0]);
+        const Fld<SPLIT> m1 = buf<SPLIT>(P, P.mom[1], P.bgm[1]);
+        vr.set(i, j, (m0(i - 1, j) + m1(i, j)) / (Sb(i - 1, j) + s0));
     }
-    P.f[FCPT_VAZI][IX(i, j)] = (P.mom[2][IX(i, jm)] + P.mom[3][IX(i, j)]) /
-                                   (Sb[IX(i, jm)] + Sb[IX(i, j)]) * P.InvRmed[i] -
-                               P.Rmed[i] * P.omega_frame;
+    const Fld<SPLIT> m2 = buf<SPLIT>(P, P.mom[2], P.bgm[2]);
+    const Fld<SPLIT> m3 = buf<SPLIT>(P, P.mom[3], P.bgm[3]);
+    vaz.set(i, j, (m2(i, jm) + m3(i, j)) / (Sb(i, jm) + s0) * P.InvRmed[i] - P.Rmed[i] * P.omega_frame);
     const double floor_value = P.sigma_floor * P.sigma0;
-    double sig = Sb[IX(i, j)];
+    double sig = s0;
     if (sig < floor_value) sig = floor_value;
-    P.f[FCPT_SIGMA][IX(i, j)] = sig;
-    if (P.adiabatic) P.f[FCPT_ENERGY][IX(i, j)] = temperature_floor(P, P.en_c[IX(i, j)], sig);
+    fld<SPLIT>(P, FCPT_SIGMA).set(i, j, sig);
+    if (P.adiabatic) {
+        const Fld<SPLIT> Eb = buf<SPLIT>(P, P.en_c, P.bg[FCPT_ENERGY]);
+        fld<SPLIT>(P, FCPT_ENERGY).set(i, j, temperature_floor(P, Eb(i, j), sig));
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1016,6 +1162,7 @@ __device__ __forceinline__ void neumaier_add(double &s, double &c, double x)
     s = t;
 }
 
+template <bool SPLIT>
 __global__ void __launch_bounds__(64) k_totals_rings(KParams P, double radius, double *ring_out)
 {
     const int ns = P.ns;
@@ -1024,12 +1171,14 @@ __global__ void __launch_bounds__(64) k_totals_rings(KParams P, double radius, d
     double m = 0.0, mc = 0.0, l = 0.0, lc = 0.0;
     const bool active = i >= 1 && i < P.nr - 1 && P.Rmed[i] <= radius;
     if (active) {
-        const double *S = P.f[FCPT_SIGMA], *vaz = P.f[FCPT_VAZI];
+        FLD(S, FCPT_SIGMA);
+        FLD(vaz, FCPT_VAZI);
         for (int j = t; j < ns; j += 64) {
             const int jm = j == 0 ? ns - 1 : j - 1;
-            neumaier_add(m, mc, P.Surf[i] * S[IX(i, j)]);
-            neumaier_add(l, lc, P.Surf[i] * 0.5 * (S[IX(i, j)] + S[IX(i, jm)]) * P.Rmed[i] *
-                                    (vaz[IX(i, j)] + P.omega_frame * P.Rmed[i]));
+            const double s = S(i, j);
+            neumaier_add(m, mc, P.Surf[i] * s);
+            neumaier_add(l, lc, P.Surf[i] * 0.5 * (s + S(i, jm)) * P.Rmed[i] *
+                                    (vaz(i, j) + P.omega_frame * P.Rmed[i]));
         }
     }
     __shared__ double sh[4][64];
@@ -1067,19 +1216,36 @@ __global__ void k_totals_final(KParams P, const double *ring_out, double *out2)
 }
 
 // streaming copy of four work arrays (bandwidth probe), 16-byte accesses
-__global__ void __launch_bounds__(256) k_copy4(KParams P, size_t n2)
+__global__ void __launch_bounds__(256) k_copy4(KParams P, size_t n16)
 {
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     for (int q = 0; q < 4; ++q) {
-        const double2 *src = reinterpret_cast<const double2 *>(P.mom[q]);
-        double2 *dst = reinterpret_cast<double2 *>(P.mom_b[q]);
-        for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n2; k += stride)
+        const double2 *src = static_cast<const double2 *>(P.mom[q]);
+        double2 *dst = static_cast<double2 *>(P.mom_b[q]);
+        for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n16; k += stride)
             dst[k] = src[k];
     }
 }
 
+// split storage: re-express a field's deviations about a new ring background
+__global__ void __launch_bounds__(256) k_rebase(float *d, const double *oldbg, const double *newbg,
+                                                int ns, int rows)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y * blockDim.y + threadIdx.y;
+    if (j >= ns || i >= rows) return;
+    const size_t c = IX(i, j);
+    d[c] = (float)((oldbg[i] + (double)d[c]) - newbg[i]);
+}
+
 // ---------------------------------------------------------------------------
-// launch helpers
+// launch helpers: each dispatches on the storage mode
+
+#define LAUNCH(kern, grid, block, st, ...)                                               \
+    do {                                                                                 \
+        if (P.split) hipLaunchKernelGGL(kern<true>, grid, block, 0, st, __VA_ARGS__);    \
+        else hipLaunchKernelGGL(kern<false>, grid, block, 0, st, __VA_ARGS__);           \
+    } while (0)
 
 static inline dim3 cell_grid(const KParams &P, int rows)
 {
@@ -1087,9 +1253,11 @@ static inline dim3 cell_grid(const KParams &P, int rows)
 }
 static const dim3 kBlock(64, 4);
 
+size_t element_size(const KParams &P) { return P.split ? sizeof(float) : sizeof(double); }
+
 void launch_ring_mean(const KParams &P, hipStream_t st, int mode)
 {
-    hipLaunchKernelGGL(k_ring_mean, dim3(P.nr), dim3(256), 0, st, P, mode);
+    LAUNCH(k_ring_mean, dim3(P.nr), dim3(256), st, P, mode);
 }
 
 // CFL grid: at most kCflMaxBlocks blocks, rows walked with a grid stride
@@ -1108,9 +1276,11 @@ void launch_cfl(const KParams &P, hipStream_t st, int mode)
 {
     launch_ring_mean(P, st, 0);
     const dim3 g = cfl_grid(P);
-    hipLaunchKernelGGL(k_cfl_cells, g, kBlock, 0, st, P);
+    LAUNCH(k_cfl_cells, g, kBlock, st, P);
     hipLaunchKernelGGL(k_cfl_finalize, dim3(1), dim3(256), 0, st, P, mode, (int)(g.x * g.y));
 }
+
+void launch_cfl_cells(const KParams &P, hipStream_t st) { LAUNCH(k_cfl_cells, cfl_grid(P), kBlock, st, P); }
 
 void launch_set_dt(const KParams &P, hipStream_t st, double dt)
 {
@@ -1129,59 +1299,45 @@ void launch_set_params(const KParams &P, hipStream_t st, const fcpt_step_params 
     hipLaunchKernelGGL(k_set_params, dim3(1), dim3(1), 0, st, P, sp);
 }
 
-void launch_potential(const KParams &P, hipStream_t st)
-{
-    hipLaunchKernelGGL(k_potential, cell_grid(P, P.nr), kBlock, 0, st, P);
-}
+void launch_potential(const KParams &P, hipStream_t st) { LAUNCH(k_potential, cell_grid(P, P.nr), kBlock, st, P); }
 
 void launch_sources(const KParams &P, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_src_vr, cell_grid(P, P.nr - 3), kBlock, 0, st, P);
-    hipLaunchKernelGGL(k_src_vphi, cell_grid(P, P.nr - 2), kBlock, 0, st, P);
-    if (P.adiabatic) hipLaunchKernelGGL(k_src_energy, cell_grid(P, P.nr - 1), kBlock, 0, st, P);
+    LAUNCH(k_src_vr, cell_grid(P, P.nr - 3), kBlock, st, P);
+    LAUNCH(k_src_vphi, cell_grid(P, P.nr - 2), kBlock, st, P);
+    if (P.adiabatic) LAUNCH(k_src_energy, cell_grid(P, P.nr - 1), kBlock, st, P);
 }
 
 // returns true when the viscosity recalculation was folded in
 bool launch_artificial_viscosity(const KParams &P, hipStream_t st, int with_viscosity)
 {
     if (P.artvisc == FCPT_ARTVISC_TW) {
-        hipLaunchKernelGGL(k_tw_q, cell_grid(P, P.nr), kBlock, 0, st, P);
-        hipLaunchKernelGGL(k_tw_vel, cell_grid(P, P.nr), kBlock, 0, st, P, with_viscosity);
+        LAUNCH(k_tw_q, cell_grid(P, P.nr), kBlock, st, P);
+        LAUNCH(k_tw_vel, cell_grid(P, P.nr), kBlock, st, P, with_viscosity);
         return with_viscosity != 0;
     }
     if (P.artvisc == FCPT_ARTVISC_SN) {
-        hipLaunchKernelGGL(k_sn_q, cell_grid(P, P.nr), kBlock, 0, st, P);
-        hipLaunchKernelGGL(k_sn_vel, cell_grid(P, P.nr), kBlock, 0, st, P, with_viscosity);
+        LAUNCH(k_sn_q, cell_grid(P, P.nr), kBlock, st, P);
+        LAUNCH(k_sn_vel, cell_grid(P, P.nr), kBlock, st, P, with_viscosity);
         return with_viscosity != 0;
     }
-    if (P.adiabatic && P.artvisc_dissipation)
-        hipLaunchKernelGGL(k_temperature_floor, cell_grid(P, P.nr), kBlock, 0, st, P);
+    if (P.adiabatic && P.artvisc_dissipation) LAUNCH(k_temperature_floor, cell_grid(P, P.nr), kBlock, st, P);
     return false;
 }
 
 void launch_recalc_viscosity(const KParams &P, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_recalc_viscosity, cell_grid(P, P.nr), kBlock, 0, st, P);
+    LAUNCH(k_recalc_viscosity, cell_grid(P, P.nr), kBlock, st, P);
 }
-
-void launch_stress(const KParams &P, hipStream_t st)
-{
-    hipLaunchKernelGGL(k_stress, cell_grid(P, P.nr), kBlock, 0, st, P);
-}
-
+void launch_stress(const KParams &P, hipStream_t st) { LAUNCH(k_stress, cell_grid(P, P.nr), kBlock, st, P); }
 void launch_visc_update(const KParams &P, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_visc_update, cell_grid(P, P.nr - 2), kBlock, 0, st, P);
+    LAUNCH(k_visc_update, cell_grid(P, P.nr - 2), kBlock, st, P);
 }
-
-void launch_substep3(const KParams &P, hipStream_t st)
-{
-    hipLaunchKernelGGL(k_substep3, cell_grid(P, P.nr), kBlock, 0, st, P);
-}
-
+void launch_substep3(const KParams &P, hipStream_t st) { LAUNCH(k_substep3, cell_grid(P, P.nr), kBlock, st, P); }
 void launch_heatcool_cfl(const KParams &P, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_heatcool_cfl, cell_grid(P, P.nr), kBlock, 0, st, P);
+    LAUNCH(k_heatcool_cfl, cell_grid(P, P.nr), kBlock, st, P);
 }
 
 void launch_boundaries(const KParams &P, hipStream_t st, int final_call, int advance_time)
@@ -1190,35 +1346,43 @@ void launch_boundaries(const KParams &P, hipStream_t st, int final_call, int adv
         const int n_inner = (P.damp_ids[0] > P.damp_ids[1] ? P.damp_ids[0] : P.damp_ids[1]) + 1;
         const int o0 = P.damp_ids[2] < P.damp_ids[3] ? P.damp_ids[2] : P.damp_ids[3];
         const int n_outer = P.nr + 1 - o0;
-        hipLaunchKernelGGL(k_damping, cell_grid(P, n_inner + n_outer), kBlock, 0, st, P, n_inner);
+        LAUNCH(k_damping, cell_grid(P, n_inner + n_outer), kBlock, st, P, n_inner);
     }
-    hipLaunchKernelGGL(k_ghost_bc, dim3((unsigned)((P.ns + 255) / 256)), dim3(256), 0, st, P,
-                       advance_time);
+    LAUNCH(k_ghost_bc, dim3((unsigned)((P.ns + 255) / 256)), dim3(256), st, P, advance_time);
 }
 
-constexpr int kRadialTile = 8;
+// rings per radial-sweep thread: the longest walk that still gives every SIMD
+// of the 256 CUs three wavefronts (the kernel's register budget allows about
+// three), so that the four-ring window start is amortised where the grid is
+// large and the chip stays full where it is small
+int radial_tile_for(int nr, int ns, int requested)
+{
+    if (requested > 0) return requested;
+    const long waves_per_band = (long)((ns + 63) / 64);
+    int tile = 32;
+    while (tile > 2 && waves_per_band * ((nr + tile - 1) / tile) < 3072) tile /= 2;
+    return tile;
+}
+
 void launch_radial_sweep(const KParams &P, hipStream_t st)
 {
-    const dim3 g((unsigned)((P.ns + 63) / 64), (unsigned)((P.nr + kRadialTile - 1) / kRadialTile));
-    hipLaunchKernelGGL(k_radial_sweep<kRadialTile>, g, kBlock, 0, st, P);
+    const int tile = P.radial_tile > 0 ? P.radial_tile : radial_tile_for(P.nr, P.ns, 0);
+    const dim3 g((unsigned)((P.ns + 63) / 64), (unsigned)((P.nr + tile - 1) / tile));
+    LAUNCH(k_radial_sweep, g, dim3(64), st, P, tile);
 }
 void launch_azimuthal_sweep(const KParams &P, hipStream_t st)
 {
     const dim3 g((unsigned)((P.ns + kAzSeg - 1) / kAzSeg), (unsigned)P.nr);
-    hipLaunchKernelGGL(k_azimuthal_sweep, g, dim3(256), 0, st, P);
-}
-void launch_copy4(const KParams &P, hipStream_t st)
-{
-    const size_t n2 = (size_t)P.nr * P.ns / 2;
-    size_t blocks = (n2 + 255) / 256;
-    if (blocks > 8192) blocks = 8192;
-    if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL(k_copy4, dim3((unsigned)blocks), dim3(256), 0, st, P, n2);
+    LAUNCH(k_azimuthal_sweep, g, dim3(256), st, P);
 }
 
-void launch_cfl_cells(const KParams &P, hipStream_t st)
+void launch_copy4(const KParams &P, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_cfl_cells, cfl_grid(P), kBlock, 0, st, P);
+    const size_t n16 = (size_t)P.nr * P.ns * element_size(P) / 16;
+    size_t blocks = (n16 + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_copy4, dim3((unsigned)blocks), dim3(256), 0, st, P, n16);
 }
 
 void launch_transport(const KParams &P, hipStream_t st)
@@ -1226,23 +1390,26 @@ void launch_transport(const KParams &P, hipStream_t st)
     launch_radial_sweep(P, st);
     launch_ring_mean(P, st, 1);
     launch_azimuthal_sweep(P, st);
-    hipLaunchKernelGGL(k_velocities, cell_grid(P, P.nr), kBlock, 0, st, P);
+    LAUNCH(k_velocities, cell_grid(P, P.nr), kBlock, st, P);
 }
 
-void launch_derived(const KParams &P, hipStream_t st)
-{
-    hipLaunchKernelGGL(k_derived, cell_grid(P, P.nr), kBlock, 0, st, P);
-}
-
+void launch_derived(const KParams &P, hipStream_t st) { LAUNCH(k_derived, cell_grid(P, P.nr), kBlock, st, P); }
 void launch_iso_profile(const KParams &P, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_iso_profile, cell_grid(P, P.nr), kBlock, 0, st, P);
+    LAUNCH(k_iso_profile, cell_grid(P, P.nr), kBlock, st, P);
+}
+
+void launch_rebase(const KParams &P, hipStream_t st, void *field, const double *oldbg,
+                   const double *newbg, int rows)
+{
+    hipLaunchKernelGGL(k_rebase, cell_grid(P, rows), kBlock, 0, st, static_cast<float *>(field), oldbg,
+                       newbg, P.ns, rows);
 }
 
 void launch_totals(const KParams &P, hipStream_t st, double radius, double *ring_scratch,
                    double *out2)
 {
-    hipLaunchKernelGGL(k_totals_rings, dim3(P.nr), dim3(64), 0, st, P, radius, ring_scratch);
+    LAUNCH(k_totals_rings, dim3(P.nr), dim3(64), st, P, radius, ring_scratch);
     hipLaunchKernelGGL(k_totals_final, dim3(1), dim3(1), 0, st, P, ring_scratch, out2);
 }
 

@@ -32,5 +32,13 @@ void launch_derived(const KParams &P, hipStream_t st);
 void launch_iso_profile(const KParams &P, hipStream_t st);
 void launch_totals(const KParams &P, hipStream_t st, double radius, double *ring_scratch,
                    double *out2);
+// split storage: deviations of `field` (rows x nphi floats) re-expressed from
+// background oldbg to newbg (both device arrays)
+void launch_rebase(const KParams &P, hipStream_t st, void *field, const double *oldbg,
+                   const double *newbg, int rows);
+// bytes of one stored value of a 2-D field (8 in FP64 storage, 4 in split storage)
+size_t element_size(const KParams &P);
+// rings per thread of the radial sweep (requested > 0 overrides the size-based choice)
+int radial_tile_for(int nr, int ns, int requested);
 
 }  // namespace fcpt

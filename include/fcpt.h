@@ -44,8 +44,15 @@ enum { FCPT_ARTVISC_NONE = 0, FCPT_ARTVISC_TW = 1, FCPT_ARTVISC_SN = 2 };
 enum { FCPT_BC_REFLECTING = 0, FCPT_BC_ZEROGRADIENT = 1, FCPT_BC_OUTFLOW = 2 };
 /* beta-cooling reference energy (reference parameters.cpp:443-457) */
 enum { FCPT_BETA_REF_ZERO = 0, FCPT_BETA_REF_REFERENCE = 1, FCPT_BETA_REF_FLOOR = 3 };
-/* arithmetic of the bulk stencil */
-enum { FCPT_PREC_FP64 = 0 };
+/* storage of the 2-D fields.  Arithmetic is FP64 in both modes.
+   FCPT_PREC_FP64:    every field in FP64 (results match the reference's host build).
+   FCPT_PREC_SPLIT32: every 2-D field stored as an FP32 deviation from an FP64
+                      per-ring background (value = background[ring] + deviation),
+                      halving the HBM traffic of the step.  The backgrounds are
+                      the ring means of the state given to fcpt_upload_state /
+                      fcpt_upload_field and the profiles derived from them;
+                      downloads return background + deviation in FP64. */
+enum { FCPT_PREC_FP64 = 0, FCPT_PREC_SPLIT32 = 1 };
 
 /* field identifiers for fcpt_upload_field / fcpt_download_field */
 typedef enum fcpt_field {
@@ -201,6 +208,13 @@ int fcpt_time_kernel(fcpt_ctx *ctx, int32_t kernel, int32_t reps, double *ms_per
    calibrates the profiler's byte counters.  It overwrites transport scratch
    only. */
 int fcpt_time_copy(fcpt_ctx *ctx, int32_t reps, double *ms_per_launch, double *bytes_per_launch);
+
+/* Launch-shape tuning (no counterpart in the reference): rings walked by one
+   thread of the radial sweep (FCPT_PARAM_RADIAL_TILE, 1 .. 64; 0 restores the
+   size-based default).  Results do not depend on it.  Drops a captured step
+   graph so that the next fcpt_run_steps captures the new shape. */
+enum { FCPT_PARAM_RADIAL_TILE = 0 };
+int fcpt_set_launch_param(fcpt_ctx *ctx, int32_t param, int32_t value);
 
 /* Compensated (Neumaier) totals over active rings with Rmed <= radius
    (quantities::gas_total_mass / gas_angular_momentum, quantities.cpp:51-78,242-279). */
