@@ -1,5 +1,6 @@
 #!/usr/bin/env python3
-"""Hydro-step throughput on one MI355X: cell-updates/s of the FP64 path.
+"""Hydro-step throughput on one MI355X: cell-updates/s of the FP64 path and of
+the split-storage path, with the split path's error against FP64.
 
 A step is one iteration of the reference's sim::run (simulation.cpp:515-556):
 CFL time step with the CFLmaxVar limiter and the monitor clip, then the gas
@@ -9,7 +10,10 @@ all resident in HBM and replayed as one captured HIP graph per step.
 
 Default workload: the adiabatic 512 x 1536 disk (gamma 1.4, von
 Neumann-Richtmyer viscosity, alpha 1e-3, planet q 1e-3), seeded initial
-condition, planets on fixed positions.  Prints one JSON line.
+condition, planets on fixed positions.  `value` is the throughput of the
+storage mode chosen with --precision (FP64 by default); the other mode runs
+the same steps from the same state and is reported under "compare".  Prints
+one JSON line.
 """
 from __future__ import annotations
 
@@ -24,6 +28,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "cell-updates/s per hydro step: FP64 vs split-FP32 at stated err; % HBM peak"
 HBM_PEAK_GBS = 8000.0
+PRECISIONS = {"fp64": 0, "split": 1}
+ELEMENT = {0: 8, 1: 4}
 
 
 def cpu_baseline(name: str, seconds: float) -> dict:
@@ -48,13 +54,41 @@ def cpu_baseline(name: str, seconds: float) -> dict:
             "sample": f"{n} steps of {name} on the host ({el:.1f} s), oracle/liboracle.so"}
 
 
+def timed_run(name: str, prec: int, steps: int, warmup: int, use_graph: bool, tune: bool):
+    """Start the workload, run the warm-up steps, pick launch shapes with the
+    library's autotuner (untimed), time `steps` steps."""
+    from fargocpt_amd import workloads as W
+    from fargocpt_amd.capi import Disk
+    disk = Disk(W.disk_config(name, prec))
+    W.start(disk, name)
+    disk.run_steps(warmup, use_graph)
+    disk.synchronize()
+    disk.tuned = disk.autotune(10) if tune else None
+    t0 = time.perf_counter()
+    ms_events = disk.time_steps(steps, use_graph)
+    disk.synchronize()
+    wall = time.perf_counter() - t0
+    return disk, wall, ms_events
+
+
+def rel_err(a, b) -> float:
+    import numpy as np
+    scale = float(np.max(np.abs(b)))
+    diff = float(np.max(np.abs(a - b)))
+    return diff / scale if scale else diff
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="adi_512x1536")
+    ap.add_argument("--precision", choices=sorted(PRECISIONS), default="fp64")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="skip the run of the other storage mode")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--skip-cpu", action="store_true")
     args = ap.parse_args()
@@ -64,30 +98,26 @@ def main() -> int:
         return 2
 
     from fargocpt_amd import workloads as W
-    from fargocpt_amd.capi import KERNELS, Disk
+    from fargocpt_amd.capi import KERNELS
 
     name = args.workload
+    prec = PRECISIONS[args.precision]
     use_graph = not args.no_graph
-    disk = Disk(W.disk_config(name))
-    W.start(disk, name)
-    disk.run_steps(args.warmup, use_graph)
-    disk.synchronize()
-    t0 = time.perf_counter()
-    ms_events = disk.time_steps(args.steps, use_graph)
-    disk.synchronize()
-    wall = time.perf_counter() - t0
     cells = W.cells(name)
+    adi = W.is_adiabatic(name)
+    tune = not args.no_autotune
+    disk, wall, ms_events = timed_run(name, prec, args.steps, args.warmup, use_graph, tune)
     value = cells * args.steps / wall
     t_sim, _, _ = disk.get_time()
 
     # roofline of the dominant kernel, timed live with HIP events on the stream
-    adi = W.is_adiabatic(name)
+    el = ELEMENT[prec]
     kt = {k: disk.time_kernel(v, 20) for k, v in KERNELS.items()}
     dom = max(kt, key=kt.get)
-    achieved = W.algorithmic_bytes(dom, adi) * cells / (kt[dom] * 1e-3) / 1e9
+    achieved = W.algorithmic_bytes(dom, adi, el) * cells / (kt[dom] * 1e-3) / 1e9
     copy_ms, copy_bytes = disk.time_copy(20)
     traffic = None
-    prof = os.path.join(ROOT, "profiles", f"traffic_{name}.json")
+    prof = os.path.join(ROOT, "profiles", f"traffic_{name}_{args.precision}.json")
     if os.path.exists(prof):
         traffic = json.load(open(prof)).get(dom)
 
@@ -104,18 +134,40 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": name, "description": W.DESCRIPTION[name],
-                   "cells": cells, "graph": use_graph, "sim_time_reached": t_sim},
+        "config": {"workload": name, "description": W.DESCRIPTION[name], "cells": cells,
+                   "storage": "f64" if prec == 0 else "f32 deviations on f64 ring backgrounds",
+                   "graph": use_graph, "launch_shapes": disk.tuned, "sim_time_reached": t_sim},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "bytes_per_cell": W.algorithmic_bytes(dom, adi),
+                     "bytes_per_cell": W.algorithmic_bytes(dom, adi, el),
                      "ms_per_launch": kt[dom]},
         "step": {"ms_per_step_events": ms_events / args.steps,
-                 "algorithmic_GBs": W.STEP_BYTES[adi] * value / 1e9,
-                 "frac_of_hbm_peak": W.STEP_BYTES[adi] * value / 1e9 / HBM_PEAK_GBS,
+                 "algorithmic_GBs": W.step_bytes(adi, el) * value / 1e9,
+                 "frac_of_hbm_peak": W.step_bytes(adi, el) * value / 1e9 / HBM_PEAK_GBS,
                  "kernel_ms": kt,
                  "copy_probe_GBs": copy_bytes / (copy_ms * 1e-3) / 1e9},
     }
+    if not args.no_compare:
+        other = 1 - prec
+        disk2, wall2, _ = timed_run(name, other, args.steps, args.warmup, use_graph, tune)
+        value2 = cells * args.steps / wall2
+        fp64, split = (disk, disk2) if prec == 0 else (disk2, disk)
+        fields = ["SIGMA", "VRAD", "VAZI"] + (["ENERGY"] if adi else [])
+        errs = {f: rel_err(split.download_field(f), fp64.download_field(f)) for f in fields}
+        m64, l64 = fp64.totals(1e300)
+        m32, l32 = split.totals(1e300)
+        out["compare"] = {
+            "precision": "split" if other == 1 else "fp64",
+            "value": value2,
+            "ms_per_step": 1e3 * wall2 / args.steps,
+            "split_speedup_vs_fp64": (value2 / value) if other == 1 else (value / value2),
+            "split_rel_err_vs_fp64": errs,
+            "split_total_mass_rel_err": abs(m32 - m64) / abs(m64),
+            "split_total_angular_momentum_rel_err": abs(l32 - l64) / abs(l64),
+            "after_steps": args.warmup + args.steps,
+            "error_definition": "max |split - fp64| / max |fp64| per field, same start, same steps",
+        }
+        disk2.close()
     if not args.skip_cpu:
         out["cpu_baseline"] = cpu_baseline(name, args.cpu_seconds)
     print(json.dumps(out), flush=True)

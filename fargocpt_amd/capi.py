@@ -26,7 +26,7 @@ ARTVISC = {"none": 0, "tw": 1, "sn": 2}
 KERNELS = {"radial_sweep": 0, "azimuthal_sweep": 1, "cfl": 3, "stress": 4, "potential": 5}
 # storage modes (fcpt_disk_desc.precision) and launch parameters of include/fcpt.h
 PREC_FP64, PREC_SPLIT32 = 0, 1
-PARAM_RADIAL_TILE = 0
+PARAM_RADIAL_TILE, PARAM_RADIAL_WAVES = 0, 1
 BC = {"reflecting": 0, "zerogradient": 1, "outflow": 2}
 MAX_PLANETS = 8
 
@@ -163,6 +163,7 @@ def load_library(path: str = LIB_PATH):
         "fcpt_time_kernel": [P, I, I, dp],
         "fcpt_time_copy": [P, I, dp, dp],
         "fcpt_set_launch_param": [P, I, I],
+        "fcpt_autotune": [P, I, C.POINTER(I), C.POINTER(I)],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
@@ -328,3 +329,14 @@ class Disk:
     def set_radial_tile(self, rings):
         """Rings walked by one radial-sweep thread (0: size-based default)."""
         self._check(self.lib.fcpt_set_launch_param(self.h, PARAM_RADIAL_TILE, rings))
+
+    def autotune(self, reps=10):
+        """Time the tunable launch shapes on the current state and keep the
+        fastest; returns {"radial_tile": t, "radial_waves": w}."""
+        t, w = C.c_int32(), C.c_int32()
+        self._check(self.lib.fcpt_autotune(self.h, reps, C.byref(t), C.byref(w)))
+        return {"radial_tile": t.value, "radial_waves": w.value}
+
+    def set_radial_waves(self, waves):
+        """Register budget of the radial sweep in wavefronts per SIMD (2 or 3)."""
+        self._check(self.lib.fcpt_set_launch_param(self.h, PARAM_RADIAL_WAVES, waves))

@@ -152,6 +152,7 @@ int fcpt_create(const fcpt_disk_desc *desc, fcpt_ctx **out)
     P.ns = ns;
     P.split = desc->precision == FCPT_PREC_SPLIT32 ? 1 : 0;
     P.radial_tile = 0;
+    P.radial_waves = 3;
     // Interpret.cpp:230-231
     P.dphi = 2.0 * M_PI / (double)ns;
     P.invdphi = (double)ns / (2.0 * M_PI);
@@ -264,10 +265,12 @@ int fcpt_create(const fcpt_disk_desc *desc, fcpt_ctx **out)
         rc = dev_alloc_bytes(c, &P.f[k], field_rows(k, nr) * ns * esz);
     for (int k = 0; k < 4 && !rc; ++k) rc = dev_alloc_bytes(c, &P.mom[k], ncell * esz);
     for (int k = 0; k < 4 && !rc; ++k) rc = dev_alloc_bytes(c, &P.mom_b[k], ncell * esz);
-    if (!rc) rc = dev_alloc_bytes(c, &P.sig_b, ncell * esz);
-    if (!rc) rc = dev_alloc_bytes(c, &P.en_b, ncell * esz);
-    if (!rc) rc = dev_alloc_bytes(c, &P.sig_c, ncell * esz);
-    if (!rc) rc = dev_alloc_bytes(c, &P.en_c, ncell * esz);
+    // the density and energy between the transport sweeps stay FP64 in both
+    // storage modes: they are rounded once per step, on the final store
+    if (!rc) rc = dev_alloc_bytes(c, &P.sig_b, ncell * sizeof(double));
+    if (!rc) rc = dev_alloc_bytes(c, &P.en_b, ncell * sizeof(double));
+    if (!rc) rc = dev_alloc_bytes(c, &P.sig_c, ncell * sizeof(double));
+    if (!rc) rc = dev_alloc_bytes(c, &P.en_c, ncell * sizeof(double));
     if (P.split) {
         for (int k = 0; k < FCPT_NUM_FIELDS && !rc; ++k) rc = dev_alloc(c, &P.bg[k], (size_t)nr + 2);
         for (int k = 0; k < 4 && !rc; ++k) rc = dev_alloc(c, &P.bgm[k], (size_t)nr + 2);
@@ -819,14 +822,63 @@ int fcpt_time_copy(fcpt_ctx *ctx, int32_t reps, double *ms_per_launch, double *b
 int fcpt_set_launch_param(fcpt_ctx *ctx, int32_t param, int32_t value)
 {
     CTX_CHECK(ctx);
-    if (param != FCPT_PARAM_RADIAL_TILE) return fail(ctx, FCPT_ERR_ARG, "unknown launch parameter");
-    if (value < 0 || value > 64) return fail(ctx, FCPT_ERR_ARG, "radial tile must be 0 .. 64");
+    if (param == FCPT_PARAM_RADIAL_TILE) {
+        if (value < 0 || value > 64) return fail(ctx, FCPT_ERR_ARG, "radial tile must be 0 .. 64");
+    } else if (param == FCPT_PARAM_RADIAL_WAVES) {
+        if (value != 2 && value != 3) return fail(ctx, FCPT_ERR_ARG, "radial waves must be 2 or 3");
+    } else {
+        return fail(ctx, FCPT_ERR_ARG, "unknown launch parameter");
+    }
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    ctx->P.radial_tile = value;
+    if (param == FCPT_PARAM_RADIAL_TILE) ctx->P.radial_tile = value;
+    else ctx->P.radial_waves = value;
     if (ctx->step_graph) {
         HIP_TRY(ctx, hipGraphExecDestroy(ctx->step_graph));
         ctx->step_graph = nullptr;
     }
+    return FCPT_OK;
+}
+
+int fcpt_autotune(fcpt_ctx *ctx, int32_t reps, int32_t *radial_tile, int32_t *radial_waves)
+{
+    CTX_CHECK(ctx);
+    if (reps <= 0) return fail(ctx, FCPT_ERR_ARG, "reps must be positive");
+    int rc = ensure_events(ctx);
+    if (rc) return rc;
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    KParams &P = ctx->P;
+    const int tiles[] = {1, 2, 4, 8, 16, 32};
+    const int waves[] = {3, 2};
+    double best = 1e300;
+    int best_t = radial_tile_for(P.nr, P.ns, 0), best_w = 3;
+    for (int w : waves) {
+        for (int t : tiles) {
+            if (t > P.nr) continue;
+            P.radial_tile = t;
+            P.radial_waves = w;
+            launch_radial_sweep(P, ctx->stream);  // warm-up
+            HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+            for (int k = 0; k < reps; ++k) launch_radial_sweep(P, ctx->stream);
+            HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+            LAUNCH_CHECK(ctx);
+            HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
+            float ms = 0.f;
+            HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+            if (ms < best) {
+                best = ms;
+                best_t = t;
+                best_w = w;
+            }
+        }
+    }
+    P.radial_tile = best_t;
+    P.radial_waves = best_w;
+    if (ctx->step_graph) {
+        HIP_TRY(ctx, hipGraphExecDestroy(ctx->step_graph));
+        ctx->step_graph = nullptr;
+    }
+    if (radial_tile) *radial_tile = best_t;
+    if (radial_waves) *radial_waves = best_w;
     return FCPT_OK;
 }
 

@@ -65,13 +65,15 @@ struct KParams {
     const double *cosphi, *sinphi;
     // storage: 0 = FP64 fields; 1 = FP32 deviations from FP64 per-ring backgrounds
     int split;
-    int radial_tile;     // rings per thread of the radial sweep
+    int radial_tile;     // rings per thread of the radial sweep (0: size-based choice)
+    int radial_waves;    // register budget of the radial sweep: wavefronts per SIMD (2 or 3)
     // fields in HBM (indexed by fcpt_field): double, or float when split
     void *f[FCPT_NUM_FIELDS];
     // split storage: per-ring backgrounds of every field (nr + 2 values each;
     // null in FP64 mode); the potential's is rewritten by the potential kernel
     double *bg[FCPT_NUM_FIELDS];
-    // transport work arrays (same element type as the fields)
+    // transport work arrays: the momenta have the fields' element type, the
+    // density and energy between the sweeps are FP64 in both storage modes
     void *mom[4];        // radial momentum +/-, angular momentum +/-
     void *mom_b[4];      // ping-pong partners
     void *sig_b, *en_b;  // radial-sweep outputs

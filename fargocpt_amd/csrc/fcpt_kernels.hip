@@ -24,13 +24,24 @@ __device__ __forceinline__ double std_min(double a, double b) { return (b < a) ?
 __device__ __forceinline__ double std_max(double a, double b) { return (a < b) ? b : a; }
 __device__ __forceinline__ double van_leer_lim(double a, double b)
 {
-    // TransportEuler.cpp:306-312
-    if (a * b > 0.0) return 2.0 * a * b / (a + b);
-    return 0;
+    // TransportEuler.cpp:306-312, evaluated without a branch (the quotient of
+    // a rejected pair is discarded)
+    const double r = 2.0 * a * b / (a + b);
+    return (a * b > 0.0) ? r : 0.0;
 }
 __device__ __forceinline__ double omega_kepler(const KParams &P, double r)
 {
     return sqrt(P.G * P.mstar / (r * r * r));  // Theo.cpp:246-249
+}
+
+// Per-ring arrays (metric, backgrounds, ring means, shifts) are never written
+// by the kernel that reads them.  Reading them through the constant address
+// space says so: with a wave-uniform index they become scalar loads, which
+// use their own counter and do not wait behind the wavefront's vector loads.
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(4))) T *cst(const T *p)
+{
+    return (const __attribute__((address_space(4))) T *)p;
 }
 
 // field accessor: an FP64 array, or FP32 deviations from an FP64 ring background
@@ -43,14 +54,22 @@ struct Fld {
     __device__ __forceinline__ double operator()(int i, int j) const
     {
         const size_t c = (size_t)i * (size_t)ns + (size_t)j;
-        if constexpr (SPLIT) return bg[i] + (double)p[c];
+        if constexpr (SPLIT) return cst(bg)[i] + (double)p[c];
         else return p[c];
     }
     __device__ __forceinline__ void set(int i, int j, double v) const
     {
         const size_t c = (size_t)i * (size_t)ns + (size_t)j;
-        if constexpr (SPLIT) p[c] = (float)(v - bg[i]);
+        if constexpr (SPLIT) p[c] = (float)(v - cst(bg)[i]);
         else p[c] = v;
+    }
+    // the stored value alone, and its decoding: lets a kernel issue a load
+    // early and pay for it where the value is used
+    __device__ __forceinline__ T raw(int i, int j) const { return p[(size_t)i * (size_t)ns + (size_t)j]; }
+    __device__ __forceinline__ double dec(int i, T x) const
+    {
+        if constexpr (SPLIT) return cst(bg)[i] + (double)x;
+        else return x;
     }
 };
 
@@ -67,11 +86,24 @@ __device__ __forceinline__ Fld<SPLIT> buf(const KParams &P, void *p, const doubl
 
 #define FLD(name, id) const Fld<SPLIT> name = fld<SPLIT>(P, id)
 
+// Kernels read every input before their first store: the compiler must
+// assume that a store may alias any later load, so a load written after a
+// store waits for it and the memory latency is paid once per such phase.
+
+// Cell kernels run 64 x 4 blocks: one wavefront is one 64-sector piece of
+// one ring, so the ring index is the same in every lane.  Reading it through
+// readfirstlane tells the compiler so, and the per-ring metric values and
+// backgrounds become scalar loads.
+__device__ __forceinline__ int wave_ring(int r0)
+{
+    return __builtin_amdgcn_readfirstlane(r0 + (int)(blockIdx.y * blockDim.y + threadIdx.y));
+}
+
 // 2-D launch over rows [r0, r1) x all sectors
 #define CELL_LOOP_HEAD(r0, r1)                                   \
     const int ns = P.ns;                                         \
     const int j = blockIdx.x * blockDim.x + threadIdx.x;         \
-    const int i = (r0) + blockIdx.y * blockDim.y + threadIdx.y;  \
+    const int i = wave_ring(r0);                                 \
     if (j >= ns || i >= (r1)) return;
 
 // ---------------------------------------------------------------------------
@@ -110,10 +142,10 @@ __global__ void __launch_bounds__(256) k_ring_mean(KParams P, int mode)
         if (mode == 1) {
             const double dt = P.s->dt;
             const double invdt = 1.0 / dt;
-            const double Ntilde = vm * P.InvRmed[i] * dt * P.invdphi;
+            const double Ntilde = vm * cst(P.InvRmed)[i] * dt * P.invdphi;
             const double Nround = floor(Ntilde + 0.5);
             P.nshift[i] = (int)Nround;
-            P.vuni[i] = (Ntilde - Nround) * P.Rmed[i] * invdt * P.dphi;
+            P.vuni[i] = (Ntilde - Nround) * cst(P.Rmed)[i] * invdt * P.dphi;
         }
     }
 }
@@ -133,22 +165,21 @@ __global__ void __launch_bounds__(256) k_cfl_cells(KParams P)
     FLD(cs, FCPT_SOUNDSPEED);
     FLD(nu, FCPT_VISCOSITY);
     const double CFL = P.cfl, dphi = P.dphi;
-    const double *vm = P.vmean;
+    const auto vm = cst(P.vmean);
     const double fac2 = P.artvisc_factor * P.artvisc_factor;
-    for (int i = 1 + blockIdx.y * blockDim.y + threadIdx.y; i < P.nr - 1 && j < ns;
-         i += gridDim.y * blockDim.y) {
+    for (int i = wave_ring(1); i < P.nr - 1 && j < ns; i += gridDim.y * blockDim.y) {
         if (j == 0) {
             const double denom =
-                fabs(vm[i] * P.InvRmed[i] - vm[i + 1] * P.InvRmed[i + 1]) + 1.0e-100;
+                fabs(vm[i] * cst(P.InvRmed)[i] - vm[i + 1] * cst(P.InvRmed)[i + 1]) + 1.0e-100;
             cand = std_min(cand, CFL * dphi / denom);
             if (i == 1) {
                 const double denom0 =
-                    fabs(vm[0] * P.InvRmed[0] - vm[1] * P.InvRmed[1]) + 1.0e-100;
+                    fabs(vm[0] * cst(P.InvRmed)[0] - vm[1] * cst(P.InvRmed)[1]) + 1.0e-100;
                 cand = std_min(cand, CFL * dphi / denom0);
             }
         }
-        const double dxRadial = P.Rsup[i] - P.Rinf[i];
-        const double dxAzimuthal = P.Rmed[i] * dphi;
+        const double dxRadial = cst(P.Rsup)[i] - cst(P.Rinf)[i];
+        const double dxAzimuthal = cst(P.Rmed)[i] * dphi;
         const double cell_size = std_min(dxRadial, dxAzimuthal);
         const int jn = j == ns - 1 ? 0 : j + 1;
         const double vr0 = vr(i, j), vr1 = vr(i + 1, j), va0 = vaz(i, j), va1 = vaz(i, jn);
@@ -166,8 +197,8 @@ __global__ void __launch_bounds__(256) k_cfl_cells(KParams P)
             else dvAzimuthal = -dvAzimuthal;
             invdt4 = 4.0 * fac2 * std_max(dvRadial / dxRadial, dvAzimuthal / dxAzimuthal) * 1.0;
         } else {
-            const double eps_rr = (vr1 - vr0) * P.InvDiffRsup[i];
-            const double eps_pp = P.InvRmed[i] * ((va1 - va0) * P.invdphi + 0.5 * (vr1 + vr0));
+            const double eps_rr = (vr1 - vr0) * cst(P.InvDiffRsup)[i];
+            const double eps_pp = cst(P.InvRmed)[i] * ((va1 - va0) * P.invdphi + 0.5 * (vr1 + vr0));
             const double mdiv_V = -std_min(eps_rr + eps_pp, 0.0);
             invdt4 = 4.0 * fac2 * mdiv_V * 1.0;
         }
@@ -255,6 +286,25 @@ __global__ void k_set_params(KParams P, fcpt_step_params sp)
 }
 
 // ---------------------------------------------------------------------------
+// split storage: the potential's ring background, rewritten with the potential
+// every step.  Every body is seen from distance sqrt(R^2 + |r_k|^2) with the
+// ring's background smoothing, which keeps the stored deviations small away
+// from the bodies.
+__global__ void __launch_bounds__(256) k_potential_bg(KParams P)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.nr) return;
+    const DevScalars *s = P.s;
+    const double R = cst(P.Rmed)[i];
+    const double smooth = P.thickness_smoothing * P.bg[FCPT_SCALE_HEIGHT][i];
+    double b = 0.0;
+    for (int k = 0; k < s->n_planets; ++k) {
+        const double px = s->planet[k].x, py = s->planet[k].y;
+        b += -P.G * s->planet[k].mass / sqrt(R * R + (px * px + py * py) + smooth * smooth);
+    }
+    P.bg[FCPT_POTENTIAL][i] = b;
+}
+
 // CalculateNbodyPotential, Pframeforce.cpp:44-85 (+ compute_smoothing, Force.cpp:145-159)
 template <bool SPLIT>
 __global__ void __launch_bounds__(256) k_potential(KParams P)
@@ -262,8 +312,8 @@ __global__ void __launch_bounds__(256) k_potential(KParams P)
     CELL_LOOP_HEAD(0, P.nr)
     FLD(Hf, FCPT_SCALE_HEIGHT);
     const DevScalars *s = P.s;
-    const double x = P.Rmed[i] * P.cosphi[j];
-    const double y = P.Rmed[i] * P.sinphi[j];
+    const double x = cst(P.Rmed)[i] * P.cosphi[j];
+    const double y = cst(P.Rmed)[i] * P.sinphi[j];
     const double H = Hf(i, j);
     double acc = 0.0;
     const int np = s->n_planets;
@@ -282,23 +332,7 @@ __global__ void __launch_bounds__(256) k_potential(KParams P)
         acc += -P.G * s->planet[k].mass / d_smoothed * smooth_factor_klahr;
     }
     acc += -s->indirect_x * x - s->indirect_y * y;
-    if constexpr (SPLIT) {
-        // The potential is rewritten completely every step, so its background
-        // is chosen here: every body seen from distance sqrt(R^2 + |r_k|^2)
-        // with the ring's background smoothing.  Each thread evaluates the
-        // same expression; the ring's first sector publishes it.
-        const double R = P.Rmed[i];
-        const double smooth = P.thickness_smoothing * P.bg[FCPT_SCALE_HEIGHT][i];
-        double b = 0.0;
-        for (int k = 0; k < np; ++k) {
-            const double px = s->planet[k].x, py = s->planet[k].y;
-            b += -P.G * s->planet[k].mass / sqrt(R * R + (px * px + py * py) + smooth * smooth);
-        }
-        if (j == 0) P.bg[FCPT_POTENTIAL][i] = b;
-        static_cast<float *>(P.f[FCPT_POTENTIAL])[IX(i, j)] = (float)(acc - b);
-    } else {
-        fld<SPLIT>(P, FCPT_POTENTIAL).set(i, j, acc);
-    }
+    fld<SPLIT>(P, FCPT_POTENTIAL).set(i, j, acc);
 }
 
 // ---------------------------------------------------------------------------
@@ -315,13 +349,13 @@ __global__ void __launch_bounds__(256) k_src_vr(KParams P)
     const double dt = P.s->dt;
     double gradp = 2.0 / (S(i, j) + S(i - 1, j));
     gradp *= (Pr(i, j) - Pr(i - 1, j));
-    gradp *= P.InvDiffRmed[i];
-    const double gradphi = (Phi(i, j) - Phi(i - 1, j)) * P.InvDiffRmed[i];
+    gradp *= cst(P.InvDiffRmed)[i];
+    const double gradphi = (Phi(i, j) - Phi(i - 1, j)) * cst(P.InvDiffRmed)[i];
     const int jn = j == ns - 1 ? 0 : j + 1;
     const double vsum = vaz(i, j) + vaz(i, jn) + vaz(i - 1, j) + vaz(i - 1, jn);
-    const double vt = 0.25 * vsum + P.Rinf[i] * P.omega_frame;
+    const double vt = 0.25 * vsum + cst(P.Rinf)[i] * P.omega_frame;
     const double vt2 = vt * vt;
-    const double centrifugal_accel = vt2 * P.InvRinf[i];
+    const double centrifugal_accel = vt2 * cst(P.InvRinf)[i];
     vr.set(i, j, vr(i, j) + dt * (-gradp - gradphi + centrifugal_accel));
 }
 
@@ -335,7 +369,7 @@ __global__ void __launch_bounds__(256) k_src_vphi(KParams P)
     FLD(Phi, FCPT_POTENTIAL);
     FLD(vaz, FCPT_VAZI);
     const double dt = P.s->dt;
-    const double invdxtheta = 2.0 / (P.dphi * (P.Rsup[i] + P.Rinf[i]));
+    const double invdxtheta = 2.0 / (P.dphi * (cst(P.Rsup)[i] + cst(P.Rinf)[i]));
     const int jp = j == 0 ? ns - 1 : j - 1;
     const double gradp = 2.0 / (S(i, j) + S(i, jp)) * (Pr(i, j) - Pr(i, jp)) * invdxtheta;
     const double gradphi = (Phi(i, j) - Phi(i, jp)) * invdxtheta;
@@ -352,8 +386,8 @@ __global__ void __launch_bounds__(256) k_src_energy(KParams P)
     FLD(E, FCPT_ENERGY);
     const double dt = P.s->dt;
     const int jn = j == ns - 1 ? 0 : j + 1;
-    const double DIV_V = (vr(i + 1, j) * P.Rinf[i + 1] - vr(i, j) * P.Rinf[i]) * P.InvDiffRsupRb[i] +
-                         (vaz(i, jn) - vaz(i, j)) * P.invdphi * P.InvRmed[i];
+    const double DIV_V = (vr(i + 1, j) * cst(P.Rinf)[i + 1] - vr(i, j) * cst(P.Rinf)[i]) * cst(P.InvDiffRsupRb)[i] +
+                         (vaz(i, jn) - vaz(i, j)) * P.invdphi * cst(P.InvRmed)[i];
     const double energy_old = E(i, j);
     E.set(i, j, energy_old * exp(-(P.gamma - 1.0) * dt * DIV_V));
 }
@@ -379,37 +413,50 @@ __global__ void __launch_bounds__(256) k_temperature_floor(KParams P)
 
 // ---------------------------------------------------------------------------
 // compute_sound_speed / compute_scale_height / update_viscosity for one cell
-// (SourceEuler.cpp:957-995, 1121-1154; viscosity.cpp:98-137)
+// (SourceEuler.cpp:957-995, 1121-1154; viscosity.cpp:98-137), as a load of
+// the inputs and a store of the results
+struct DerivedIn {
+    double e, sig, cs, h;
+};
+
 template <bool SPLIT>
-__device__ __forceinline__ void derived_cell(const KParams &P, int i, int j, bool with_temperature,
-                                            bool with_pressure)
+__device__ __forceinline__ DerivedIn derived_load(const KParams &P, int i, int j, bool with_pressure)
 {
-    FLD(cs, FCPT_SOUNDSPEED);
-    FLD(Hf, FCPT_SCALE_HEIGHT);
+    DerivedIn d{0.0, 0.0, 0.0, 0.0};
+    if (P.adiabatic) {
+        d.e = fld<SPLIT>(P, FCPT_ENERGY)(i, j);
+        d.sig = fld<SPLIT>(P, FCPT_SIGMA)(i, j);
+    } else {
+        d.cs = fld<SPLIT>(P, FCPT_SOUNDSPEED)(i, j);
+        d.h = fld<SPLIT>(P, FCPT_SCALE_HEIGHT)(i, j);
+        if (with_pressure) d.sig = fld<SPLIT>(P, FCPT_SIGMA)(i, j);
+    }
+    return d;
+}
+
+template <bool SPLIT>
+__device__ __forceinline__ void derived_store(const KParams &P, int i, int j, const DerivedIn &d,
+                                             bool with_temperature, bool with_pressure)
+{
     double c_s, h;
     if (P.adiabatic) {
-        FLD(E, FCPT_ENERGY);
-        FLD(S, FCPT_SIGMA);
-        const double e = E(i, j);
-        const double sig = S(i, j);
+        const double e = d.e;
+        const double sig = d.sig;
         if (with_temperature) {
             const double c_v_inv = P.mu / P.Rgas * (P.gamma - 1.0);
             fld<SPLIT>(P, FCPT_TEMPERATURE).set(i, j, c_v_inv * e / sig);
         }
         const double g = P.gamma;
         c_s = sqrt(g * (g - 1.0) * e / sig);
-        cs.set(i, j, c_s);
-        const double inv_omega_kepler = 1.0 / omega_kepler(P, P.Rmed[i]);
+        fld<SPLIT>(P, FCPT_SOUNDSPEED).set(i, j, c_s);
+        const double inv_omega_kepler = 1.0 / omega_kepler(P, cst(P.Rmed)[i]);
         h = c_s / (sqrt(g)) * inv_omega_kepler;
-        Hf.set(i, j, h);
+        fld<SPLIT>(P, FCPT_SCALE_HEIGHT).set(i, j, h);
         if (with_pressure) fld<SPLIT>(P, FCPT_PRESSURE).set(i, j, (P.gamma - 1.0) * e);
     } else {
-        c_s = cs(i, j);
-        h = Hf(i, j);
-        if (with_pressure) {
-            FLD(S, FCPT_SIGMA);
-            fld<SPLIT>(P, FCPT_PRESSURE).set(i, j, S(i, j) * (c_s * c_s));
-        }
+        c_s = d.cs;
+        h = d.h;
+        if (with_pressure) fld<SPLIT>(P, FCPT_PRESSURE).set(i, j, d.sig * (c_s * c_s));
     }
     const double alpha = P.alpha;
     fld<SPLIT>(P, FCPT_VISCOSITY).set(i, j, alpha * h * c_s);
@@ -424,23 +471,25 @@ __global__ void __launch_bounds__(256) k_tw_q(KParams P)
     FLD(vr, FCPT_VRAD);
     FLD(vaz, FCPT_VAZI);
     FLD(S, FCPT_SIGMA);
+    FLD(E, FCPT_ENERGY);
+    const bool diss = P.adiabatic && P.artvisc_dissipation;
     const double dt = P.s->dt;
     const int jn = j == ns - 1 ? 0 : j + 1;
-    const double vr0 = vr(i, j), vr1 = vr(i + 1, j);
-    const double eps_rr = (vr1 - vr0) * P.InvDiffRsup[i];
-    const double eps_pp = P.InvRmed[i] * ((vaz(i, jn) - vaz(i, j)) * P.invdphi + 0.5 * (vr1 + vr0));
+    const double vr0 = vr(i, j), vr1 = vr(i + 1, j), va0 = vaz(i, j), va1 = vaz(i, jn);
+    const double sig = S(i, j);
+    const double e_in = diss ? E(i, j) : 0.0;
+    const double eps_rr = (vr1 - vr0) * cst(P.InvDiffRsup)[i];
+    const double eps_pp = cst(P.InvRmed)[i] * ((va1 - va0) * P.invdphi + 0.5 * (vr1 + vr0));
     const double div_V = std_min(eps_rr + eps_pp, 0.0);
-    const double Dr = P.Rinf[i + 1] - P.Rinf[i];
-    const double rDphi = P.Rmed[i] * P.dphi;
+    const double Dr = cst(P.Rinf)[i + 1] - cst(P.Rinf)[i];
+    const double rDphi = cst(P.Rmed)[i] * P.dphi;
     const double dxm = (ns <= 16) ? std_min(Dr, rDphi) : std_max(Dr, rDphi);
     const double dx_sq = dxm * dxm;
     const double l_sq = (P.artvisc_factor * P.artvisc_factor) * dx_sq;
-    const double sig = S(i, j);
     fld<SPLIT>(P, FCPT_Q_R).set(i, j, l_sq * sig * -div_V * (eps_rr - 1.0 / 3.0 * div_V));
     fld<SPLIT>(P, FCPT_Q_PHI).set(i, j, l_sq * sig * -div_V * (eps_pp - 1.0 / 3.0 * div_V));
-    if (P.adiabatic && P.artvisc_dissipation) {
-        FLD(E, FCPT_ENERGY);
-        double e = E(i, j);
+    if (diss) {
+        double e = e_in;
         if (i > 1 && i < P.nr - 1) {
             const double d = eps_rr - eps_pp;
             const double Qplus =
@@ -457,28 +506,47 @@ template <bool SPLIT>
 __global__ void __launch_bounds__(256) k_tw_vel(KParams P, int with_viscosity)
 {
     CELL_LOOP_HEAD(0, P.nr)
-    // in the fused step, recalculate_viscosity (SourceEuler.cpp:205-223), which is
-    // pointwise in Sigma and E and follows this kernel, is done here for every ring
-    if (with_viscosity) derived_cell<SPLIT>(P, i, j, false, false);
-    if (i < 1 || i >= P.nr - 1) return;
     FLD(S, FCPT_SIGMA);
     FLD(Qr, FCPT_Q_R);
     FLD(Qp, FCPT_Q_PHI);
     FLD(vaz, FCPT_VAZI);
+    FLD(vr, FCPT_VRAD);
     const double dt = P.s->dt;
+    const bool inner = i >= 1 && i < P.nr - 1;
+    const bool radial = i >= 2 && inner;
     const int jp = j == 0 ? ns - 1 : j - 1;
-    const double sigma_phi_avg = 0.5 * (S(i, j) + S(i, jp));
-    const double dVp = 2.0 * dt / ((P.Rsup[i] + P.Rinf[i]) * sigma_phi_avg) *
-                       (Qp(i, j) - Qp(i, jp)) * P.invdphi;
-    vaz.set(i, j, vaz(i, j) + dVp);
-    if (i >= 2) {
-        FLD(vr, FCPT_VRAD);
-        const double sigma_r_avg = 0.5 * (S(i, j) + S(i - 1, j));
-        const double r0 = P.Rmed[i], r1 = P.Rmed[i - 1];
+    const int im = i >= 1 ? i - 1 : 0;
+    // in the fused step, recalculate_viscosity (SourceEuler.cpp:205-223), which is
+    // pointwise in Sigma and E and follows this kernel, is done here for every ring
+    DerivedIn din{0.0, 0.0, 0.0, 0.0};
+    if (with_viscosity) din = derived_load<SPLIT>(P, i, j, false);
+    double s0 = 0.0, sjp = 0.0, sm = 0.0, qp0 = 0.0, qpj = 0.0, qpm = 0.0, qr0 = 0.0, qrm = 0.0;
+    double va = 0.0, v = 0.0;
+    if (inner) {
+        s0 = S(i, j);
+        sjp = S(i, jp);
+        qp0 = Qp(i, j);
+        qpj = Qp(i, jp);
+        va = vaz(i, j);
+    }
+    if (radial) {
+        sm = S(im, j);
+        qr0 = Qr(i, j);
+        qrm = Qr(im, j);
+        qpm = Qp(im, j);
+        v = vr(i, j);
+    }
+    if (with_viscosity) derived_store<SPLIT>(P, i, j, din, false, false);
+    if (!inner) return;
+    const double sigma_phi_avg = 0.5 * (s0 + sjp);
+    const double dVp = 2.0 * dt / ((cst(P.Rsup)[i] + cst(P.Rinf)[i]) * sigma_phi_avg) * (qp0 - qpj) * P.invdphi;
+    vaz.set(i, j, va + dVp);
+    if (radial) {
+        const double sigma_r_avg = 0.5 * (s0 + sm);
+        const double r0 = cst(P.Rmed)[i], r1 = cst(P.Rmed)[i - 1];
         const double dVr = P.radial_viscosity_factor * dt / sigma_r_avg * 2.0 / (r0 * r0 - r1 * r1) *
-                           ((Qr(i, j) * r0 - Qr(i - 1, j) * r1) -
-                            0.5 * (Qp(i, j) + Qp(i - 1, j)) * (r0 - r1));
-        vr.set(i, j, vr(i, j) + dVr);
+                           ((qr0 * r0 - qrm * r1) - 0.5 * (qp0 + qpm) * (r0 - r1));
+        vr.set(i, j, v + dVr);
     }
 }
 
@@ -490,23 +558,26 @@ __global__ void __launch_bounds__(256) k_sn_q(KParams P)
     FLD(vr, FCPT_VRAD);
     FLD(vaz, FCPT_VAZI);
     FLD(S, FCPT_SIGMA);
-    const double fac2 = P.artvisc_factor * P.artvisc_factor;
-    const double sig = S(i, j);
-    const double dv_r = vr(i + 1, j) - vr(i, j);
-    const double qr = dv_r < 0.0 ? fac2 * sig * (dv_r * dv_r) : 0.0;
+    FLD(E, FCPT_ENERGY);
+    const bool diss = P.adiabatic && P.artvisc_dissipation;
+    const double dt = P.s->dt;
     const int jn = j == ns - 1 ? 0 : j + 1;
-    const double dv_phi = vaz(i, jn) - vaz(i, j);
+    const double vr0 = vr(i, j), vr1 = vr(i + 1, j), va0 = vaz(i, j), va1 = vaz(i, jn);
+    const double sig = S(i, j);
+    const double e_in = diss ? E(i, j) : 0.0;
+    const double fac2 = P.artvisc_factor * P.artvisc_factor;
+    const double dv_r = vr1 - vr0;
+    const double qr = dv_r < 0.0 ? fac2 * sig * (dv_r * dv_r) : 0.0;
+    const double dv_phi = va1 - va0;
     const double qp = dv_phi < 0.0 ? fac2 * sig * (dv_phi * dv_phi) : 0.0;
     fld<SPLIT>(P, FCPT_Q_R).set(i, j, qr);
     fld<SPLIT>(P, FCPT_Q_PHI).set(i, j, qp);
-    if (P.adiabatic && P.artvisc_dissipation) {
-        FLD(E, FCPT_ENERGY);
-        double e = E(i, j);
+    if (diss) {
+        double e = e_in;
         if (i >= 1 && i < P.nr - 1) {
-            const double dt = P.s->dt;
-            const double dxtheta = P.dphi * P.Rmed[i];
+            const double dxtheta = P.dphi * cst(P.Rmed)[i];
             const double invdxtheta = 1.0 / dxtheta;
-            e = e - dt * qr * dv_r * P.InvDiffRsup[i] - dt * qp * dv_phi * invdxtheta;
+            e = e - dt * qr * dv_r * cst(P.InvDiffRsup)[i] - dt * qp * dv_phi * invdxtheta;
         }
         E.set(i, j, temperature_floor(P, e, sig));
     }
@@ -517,24 +588,41 @@ template <bool SPLIT>
 __global__ void __launch_bounds__(256) k_sn_vel(KParams P, int with_viscosity)
 {
     CELL_LOOP_HEAD(0, P.nr)
-    // in the fused step, recalculate_viscosity (SourceEuler.cpp:205-223), which is
-    // pointwise in Sigma and E and follows this kernel, is done here for every ring
-    if (with_viscosity) derived_cell<SPLIT>(P, i, j, false, false);
-    if (i < 1 || i >= P.nr - 1) return;
     FLD(S, FCPT_SIGMA);
     FLD(Qr, FCPT_Q_R);
     FLD(Qp, FCPT_Q_PHI);
-    const double dt = P.s->dt;
-    if (i >= 2) {
-        FLD(vr, FCPT_VRAD);
-        vr.set(i, j, vr(i, j) - dt * 2.0 / (S(i, j) + S(i - 1, j)) * (Qr(i, j) - Qr(i - 1, j)) *
-                                    P.InvDiffRmed[i]);
-    }
-    const double dxtheta = P.dphi * P.Rmed[i];
-    const double invdxtheta = 1.0 / dxtheta;
-    const int jp = j == 0 ? ns - 1 : j - 1;
     FLD(vaz, FCPT_VAZI);
-    vaz.set(i, j, vaz(i, j) - dt * 2.0 / (S(i, j) + S(i, jp)) * (Qp(i, j) - Qp(i, jp)) * invdxtheta);
+    FLD(vr, FCPT_VRAD);
+    const double dt = P.s->dt;
+    const bool inner = i >= 1 && i < P.nr - 1;
+    const bool radial = i >= 2 && inner;
+    const int jp = j == 0 ? ns - 1 : j - 1;
+    const int im = i >= 1 ? i - 1 : 0;
+    // in the fused step, recalculate_viscosity (SourceEuler.cpp:205-223), which is
+    // pointwise in Sigma and E and follows this kernel, is done here for every ring
+    DerivedIn din{0.0, 0.0, 0.0, 0.0};
+    if (with_viscosity) din = derived_load<SPLIT>(P, i, j, false);
+    double s0 = 0.0, sjp = 0.0, sm = 0.0, qp0 = 0.0, qpj = 0.0, qr0 = 0.0, qrm = 0.0;
+    double va = 0.0, v = 0.0;
+    if (inner) {
+        s0 = S(i, j);
+        sjp = S(i, jp);
+        qp0 = Qp(i, j);
+        qpj = Qp(i, jp);
+        va = vaz(i, j);
+    }
+    if (radial) {
+        sm = S(im, j);
+        qr0 = Qr(i, j);
+        qrm = Qr(im, j);
+        v = vr(i, j);
+    }
+    if (with_viscosity) derived_store<SPLIT>(P, i, j, din, false, false);
+    if (!inner) return;
+    if (radial) vr.set(i, j, v - dt * 2.0 / (s0 + sm) * (qr0 - qrm) * cst(P.InvDiffRmed)[i]);
+    const double dxtheta = P.dphi * cst(P.Rmed)[i];
+    const double invdxtheta = 1.0 / dxtheta;
+    vaz.set(i, j, va - dt * 2.0 / (s0 + sjp) * (qp0 - qpj) * invdxtheta);
 }
 
 // recalculate_viscosity, SourceEuler.cpp:205-223
@@ -542,7 +630,7 @@ template <bool SPLIT>
 __global__ void __launch_bounds__(256) k_recalc_viscosity(KParams P)
 {
     CELL_LOOP_HEAD(0, P.nr)
-    derived_cell<SPLIT>(P, i, j, false, false);
+    derived_store<SPLIT>(P, i, j, derived_load<SPLIT>(P, i, j, false), false, false);
 }
 
 // recalculate_derived_disk_quantities, SourceEuler.cpp:225-249
@@ -550,7 +638,7 @@ template <bool SPLIT>
 __global__ void __launch_bounds__(256) k_derived(KParams P)
 {
     CELL_LOOP_HEAD(0, P.nr)
-    derived_cell<SPLIT>(P, i, j, true, true);
+    derived_store<SPLIT>(P, i, j, derived_load<SPLIT>(P, i, j, true), true, true);
 }
 
 // locally isothermal sound speed and scale height from the initial setup
@@ -559,7 +647,7 @@ template <bool SPLIT>
 __global__ void __launch_bounds__(256) k_iso_profile(KParams P)
 {
     CELL_LOOP_HEAD(0, P.nr)
-    const double r = P.Rmed[i];
+    const double r = cst(P.Rmed)[i];
     const double vK = sqrt(P.G * P.mstar / r);
     const double h = P.aspect_ratio * pow(r, P.flaring);
     const double c_s = h * vK;
@@ -583,23 +671,26 @@ __global__ void __launch_bounds__(256) k_stress(KParams P)
     FLD(nu, FCPT_VISCOSITY);
     FLD(S, FCPT_SIGMA);
     const int jn = j == ns - 1 ? 0 : j + 1;
+    const int jp = j == 0 ? ns - 1 : j - 1;
+    const int im = i >= 1 ? i - 1 : 0;
     const double vr0 = vr(i, j), vr1 = vr(i + 1, j), va0 = vaz(i, j), va1 = vaz(i, jn);
-    const double divv = (vr1 * P.Rinf[i + 1] - vr0 * P.Rinf[i]) * P.InvDiffRsupRb[i] +
-                        (va1 - va0) * P.invdphi * P.InvRmed[i];
-    fld<SPLIT>(P, FCPT_DIV_V).set(i, j, divv);
     const double n = nu(i, j), s = S(i, j);
-    const double drr = (vr1 - vr0) * P.InvDiffRsup[i];
+    const double vam = vaz(im, j), vrp = vr(i, jp);
+    const double nm = nu(im, j), njp = nu(i, jp), nmp = nu(im, jp);
+    const double sm = S(im, j), sjp = S(i, jp), smp = S(im, jp);
+    const double divv = (vr1 * cst(P.Rinf)[i + 1] - vr0 * cst(P.Rinf)[i]) * cst(P.InvDiffRsupRb)[i] +
+                        (va1 - va0) * P.invdphi * cst(P.InvRmed)[i];
+    fld<SPLIT>(P, FCPT_DIV_V).set(i, j, divv);
+    const double drr = (vr1 - vr0) * cst(P.InvDiffRsup)[i];
     fld<SPLIT>(P, FCPT_TAU_R_R).set(i, j, 2.0 * n * s * (drr - 1.0 / 3.0 * divv));
-    const double dpp = (va1 - va0) * P.invdphi * P.InvRmed[i] + 0.5 * (vr1 + vr0) * P.InvRmed[i];
+    const double dpp = (va1 - va0) * P.invdphi * cst(P.InvRmed)[i] + 0.5 * (vr1 + vr0) * cst(P.InvRmed)[i];
     fld<SPLIT>(P, FCPT_TAU_PHI_PHI).set(i, j, 2.0 * n * s * (dpp - 1.0 / 3.0 * divv));
     if (i >= 1) {
-        const int jp = j == 0 ? ns - 1 : j - 1;
-        const double dvazirdr =
-            (va0 * P.InvRmed[i] - vaz(i - 1, j) * P.InvRmed[i - 1]) * P.InvDiffRmed[i];
-        const double dvrdphi = (vr0 - vr(i, jp)) * P.invdphi;
-        const double drp = P.Rinf[i] * dvazirdr + dvrdphi * P.InvRinf[i];
-        const double n4 = 0.25 * (n + nu(i - 1, j) + nu(i, jp) + nu(i - 1, jp));
-        const double s4 = 0.25 * (s + S(i - 1, j) + S(i, jp) + S(i - 1, jp));
+        const double dvazirdr = (va0 * cst(P.InvRmed)[i] - vam * cst(P.InvRmed)[i - 1]) * cst(P.InvDiffRmed)[i];
+        const double dvrdphi = (vr0 - vrp) * P.invdphi;
+        const double drp = cst(P.Rinf)[i] * dvazirdr + dvrdphi * cst(P.InvRinf)[i];
+        const double n4 = 0.25 * (n + nm + njp + nmp);
+        const double s4 = 0.25 * (s + sm + sjp + smp);
         fld<SPLIT>(P, FCPT_TAU_R_PHI).set(i, j, n4 * s4 * drp);
     }
 }
@@ -613,26 +704,30 @@ __global__ void __launch_bounds__(256) k_visc_update(KParams P)
     FLD(trr, FCPT_TAU_R_R);
     FLD(tpp, FCPT_TAU_PHI_PHI);
     FLD(trp, FCPT_TAU_R_PHI);
+    FLD(vaz, FCPT_VAZI);
+    FLD(vr, FCPT_VRAD);
     const double dt = P.s->dt;
     const int jp = j == 0 ? ns - 1 : j - 1;
+    const int jn = j == ns - 1 ? 0 : j + 1;
+    const double s0 = S(i, j), sjp = S(i, jp), sm = S(i - 1, j);
+    const double trp0 = trp(i, j), trp1 = trp(i + 1, j), trpn = trp(i, jn);
+    const double tpp0 = tpp(i, j), tppj = tpp(i, jp), tppm = tpp(i - 1, j);
+    const double trr0 = trr(i, j), trrm = trr(i - 1, j);
+    const double va = vaz(i, j), v = vr(i, j);
     {
-        FLD(vaz, FCPT_VAZI);
-        const double sigma_avg = 0.5 * (S(i, j) + S(i, jp));
-        const double a1 = P.Rinf[i + 1] * P.Rinf[i + 1], a0 = P.Rinf[i] * P.Rinf[i];
-        const double dVp = dt * P.InvRmed[i] / (sigma_avg) *
-                           ((2.0 / (a1 - a0)) * (a1 * trp(i + 1, j) - a0 * trp(i, j)) +
-                            (tpp(i, j) - tpp(i, jp)) * P.invdphi);
-        vaz.set(i, j, vaz(i, j) + dVp);
+        const double sigma_avg = 0.5 * (s0 + sjp);
+        const double a1 = cst(P.Rinf)[i + 1] * cst(P.Rinf)[i + 1], a0 = cst(P.Rinf)[i] * cst(P.Rinf)[i];
+        const double dVp = dt * cst(P.InvRmed)[i] / (sigma_avg) *
+                           ((2.0 / (a1 - a0)) * (a1 * trp1 - a0 * trp0) + (tpp0 - tppj) * P.invdphi);
+        vaz.set(i, j, va + dVp);
     }
     if (i >= 2) {
-        FLD(vr, FCPT_VRAD);
-        const int jn = j == ns - 1 ? 0 : j + 1;
-        const double sigma_avg = 0.5 * (S(i, j) + S(i - 1, j));
+        const double sigma_avg = 0.5 * (s0 + sm);
         const double dVr = dt / (sigma_avg)*P.radial_viscosity_factor * 2.0 /
-                           (P.Rmed[i] + P.Rmed[i - 1]) *
-                           ((P.Rmed[i] * trr(i, j) - P.Rmed[i - 1] * trr(i - 1, j)) * P.InvDiffRmed[i] +
-                            (trp(i, jn) - trp(i, j)) * P.invdphi - 0.5 * (tpp(i, j) + tpp(i - 1, j)));
-        vr.set(i, j, vr(i, j) + dVr);
+                           (cst(P.Rmed)[i] + cst(P.Rmed)[i - 1]) *
+                           ((cst(P.Rmed)[i] * trr0 - cst(P.Rmed)[i - 1] * trrm) * cst(P.InvDiffRmed)[i] +
+                            (trpn - trp0) * P.invdphi - 0.5 * (tpp0 + tppm));
+        vr.set(i, j, v + dVr);
     }
 }
 
@@ -647,7 +742,7 @@ __device__ __forceinline__ void heating_cooling_cell(const KParams &P, int i, in
     const int ns = P.ns;
     qminus = 0.0;
     if (P.cooling_beta_enabled) {
-        const double r = P.Rmed[i];
+        const double r = cst(P.Rmed)[i];
         const double omega_k = omega_kepler(P, r);
         double beta_inv = 1 / P.cooling_beta;
         if (P.cooling_beta_ramp_up > 0.0) {
@@ -694,17 +789,19 @@ __global__ void __launch_bounds__(256) k_substep3(KParams P)
     FLD(E, FCPT_ENERGY);
     const double sig = fld<SPLIT>(P, FCPT_SIGMA)(i, j);
     double e = E(i, j);
+    const double h = fld<SPLIT>(P, FCPT_SCALE_HEIGHT)(i, j);
+    const DevScalars *s = P.s;
+    const double dt = s->dt, time = s->time;
     // compute_temperature (SourceEuler.cpp:1378-1408) before the update
     const double c_v_inv = P.mu / P.Rgas * (P.gamma - 1.0);
-    fld<SPLIT>(P, FCPT_TEMPERATURE).set(i, j, c_v_inv * e / sig);
+    const double temperature = c_v_inv * e / sig;
     double qplus = 0.0, qminus = 0.0;
     if (i >= 1 && i < P.nr - 1) {
-        const DevScalars *s = P.s;
-        heating_cooling_cell<SPLIT>(P, i, j, s->time, e, sig, qplus, qminus);
-        const double alpha = rad_alpha(P, fld<SPLIT>(P, FCPT_SCALE_HEIGHT)(i, j), sig, e);
+        heating_cooling_cell<SPLIT>(P, i, j, time, e, sig, qplus, qminus);
+        const double alpha = rad_alpha(P, h, sig, e);
         qplus /= alpha;
         qminus /= alpha;
-        double energy_new = e + s->dt * (qplus - qminus);
+        double energy_new = e + dt * (qplus - qminus);
         const double SigmaFloor = 10.0 * P.sigma0 * P.sigma_floor;
         if ((sig < SigmaFloor)) {
             // TAU_EFF is never computed without surface cooling: it holds 0
@@ -716,6 +813,7 @@ __global__ void __launch_bounds__(256) k_substep3(KParams P)
         }
         e = energy_new;
     }
+    fld<SPLIT>(P, FCPT_TEMPERATURE).set(i, j, temperature);
     fld<SPLIT>(P, FCPT_QPLUS).set(i, j, qplus);
     fld<SPLIT>(P, FCPT_QMINUS).set(i, j, qminus);
     E.set(i, j, temperature_floor(P, e, sig));
@@ -748,7 +846,7 @@ __global__ void __launch_bounds__(256) k_damping(KParams P, int n_inner)
 {
     const int ns = P.ns;
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    const int y = wave_ring(0);
     if (j >= ns) return;
     const double dt = P.s->dt;
     const double lin = P.damping_inner_limit, lout = P.damping_outer_limit;
@@ -855,140 +953,207 @@ __global__ void __launch_bounds__(256) k_ghost_bc(KParams P, int advance_time)
             vr.set(nr, j, vn2);
         }
     }
-    const double vk_in = sqrt(P.G * P.mstar / P.Rmed[0]);
-    const double vk_out = sqrt(P.G * P.mstar / P.Rmed[nr - 1]);
-    vaz.set(0, j, 1.0 * vk_in - P.Rmed[0] * P.omega_frame);
-    vaz.set(nr - 1, j, 1.0 * vk_out - P.Rmed[nr - 1] * P.omega_frame);
+    const double vk_in = sqrt(P.G * P.mstar / cst(P.Rmed)[0]);
+    const double vk_out = sqrt(P.G * P.mstar / cst(P.Rmed)[nr - 1]);
+    vaz.set(0, j, 1.0 * vk_in - cst(P.Rmed)[0] * P.omega_frame);
+    vaz.set(nr - 1, j, 1.0 * vk_out - cst(P.Rmed)[nr - 1] * P.omega_frame);
 }
 
 // ---------------------------------------------------------------------------
 // Transport
 
-// van Leer slope of a radial profile at ring k (compute_star_radial first loop,
-// TransportEuler.cpp:355-371); arguments are rings k-1..k+1
-__device__ __forceinline__ double dq_radial(const KParams &P, int k, double qm, double q0, double qp)
-{
-    if (k == 0 || k == P.nr - 1) return 0.0;
-    const double dqm = (q0 - qm) * P.InvDiffRmed[k];
-    const double dqp = (qp - q0) * P.InvDiffRmed[k + 1];
-    return van_leer_lim(dqp, dqm);
-}
-
-// upwind interface value at ring interface i (compute_star_radial second loop,
-// TransportEuler.cpp:376-405); w0..w3 = rings i-2..i+1 of the specific quantity
-__device__ __forceinline__ double star_radial(const KParams &P, int i, double w0, double w1,
-                                              double w2, double w3, double v, double dt)
-{
-    if (i <= 0 || i >= P.nr) return 0.0;
-    if (v > 0.0) {
-        const double dq = dq_radial(P, i - 1, w0, w1, w2);
-        return w1 + (P.Rmed[i] - P.Rmed[i - 1] - v * dt) * 0.5 * dq;
-    }
-    const double dq = dq_radial(P, i, w1, w2, w3);
-    return w2 - (P.Rmed[i + 1] - P.Rmed[i] + v * dt) * 0.5 * dq;
-}
-
-// One ring of the radial sweep's window: Sigma, the six advected quantities
-// (four momenta, E, Sigma itself), their specific values Q / Sigma and v_r at
-// the ring's inner interface.  The momenta are formed with the expressions of
-// compute_momenta_from_velocities (TransportEuler.cpp:471-495).  Rings outside
-// the grid hold Sigma = 1 and zero quantities (their interface values are
-// never used: the slopes there are zero and DensityStar vanishes).
-struct RingVals {
-    double s;
-    double q[6];
-    double w[6];
-    double v;
+// One ring of the radial sweep's window: Sigma, the advected quantities other
+// than the density (four momenta, and E when adiabatic: NQ of them), their
+// specific values Q / Sigma, the van Leer slopes of Sigma and of each
+// specific value, and v_r at the ring's inner interface.  The momenta are
+// formed with the expressions of compute_momenta_from_velocities
+// (TransportEuler.cpp:471-495).  Rings outside the grid hold Sigma = 1 and
+// zero quantities; the reference's interface values never reach them.  The
+// density's own specific value Sigma / Sigma is exactly 1 for the positive
+// Sigma the floor guarantees, so its interface value is 1 on interior
+// interfaces and 0 on the two edge interfaces.
+template <int NQ>
+struct RingW {
+    double s, w[NQ], q[NQ], v;
+    double ds, dw[NQ];
 };
 
-template <bool SPLIT>
-__device__ __forceinline__ RingVals load_ring(const KParams &P, int r, int j, int jn)
+// The ring's stored values, loaded a ring ahead of their use so that the
+// loads are in flight while the previous ring is processed.  Loads go to a
+// clamped ring and the results are selected, so the loop body has no branch.
+template <bool SPLIT, bool ADI>
+struct RingRaw {
+    using T = typename Fld<SPLIT>::T;
+    T sig, va, vaj, e, v0, v1;
+    int r;
+};
+
+template <bool SPLIT, bool ADI>
+__device__ __forceinline__ void raw_ring(const KParams &P, int r, int j, int jn, RingRaw<SPLIT, ADI> &X)
 {
-    // loads go to a clamped ring and the results are selected, which keeps
-    // the window in registers
     const bool in = r >= 0 && r < P.nr;
     const int rc = in ? r : 0;
     const int rv = r < 0 ? 0 : (r > P.nr ? P.nr : r);
-    FLD(S, FCPT_SIGMA);
+    const int rv1 = rv < P.nr ? rv + 1 : P.nr;
     FLD(vr, FCPT_VRAD);
     FLD(vaz, FCPT_VAZI);
-    const double sig = S(rc, j);
-    const double rr = P.Rmed[rc];
+    X.r = r;
+    X.sig = fld<SPLIT>(P, FCPT_SIGMA).raw(rc, j);
+    X.va = vaz.raw(rc, j);
+    X.vaj = vaz.raw(rc, jn);
+    if constexpr (ADI) X.e = fld<SPLIT>(P, FCPT_ENERGY).raw(rc, j);
+    X.v0 = vr.raw(rv, j);
+    X.v1 = vr.raw(rv1, j);
+}
+
+template <bool SPLIT, bool ADI, int NQ>
+__device__ __forceinline__ void decode_ring(const KParams &P, const RingRaw<SPLIT, ADI> &X, RingW<NQ> &R)
+{
+    const int r = X.r;
+    const bool in = r >= 0 && r < P.nr;
+    const int rc = in ? r : 0;
+    const int rv = r < 0 ? 0 : (r > P.nr ? P.nr : r);
+    const int rv1 = rv < P.nr ? rv + 1 : P.nr;
+    FLD(vr, FCPT_VRAD);
+    FLD(vaz, FCPT_VAZI);
+    const double sig = fld<SPLIT>(P, FCPT_SIGMA).dec(rc, X.sig);
+    const double rr = cst(P.Rmed)[rc];
     const double OmegaF = P.omega_frame;
-    const double v0 = vr(rc, j);
-    double q[6];
-    q[0] = sig * vr(rc + 1, j);
+    const double v0 = vr.dec(rv, X.v0);  // v_r(r), r inside the grid or r == nr
+    double q[NQ];
+    q[0] = sig * vr.dec(rv1, X.v1);      // v_r(r + 1) for r inside the grid
     q[1] = sig * v0;
-    q[2] = sig * (vaz(rc, jn) + rr * OmegaF) * rr;
-    q[3] = sig * (vaz(rc, j) + rr * OmegaF) * rr;
-    q[4] = P.adiabatic ? fld<SPLIT>(P, FCPT_ENERGY)(rc, j) : 0.0;
-    q[5] = sig;
-    RingVals R;
-    const double vv = vr(rv, j);
-    R.v = (r >= 0 && r <= P.nr) ? vv : 0.0;
+    q[2] = sig * (vaz.dec(rc, X.vaj) + rr * OmegaF) * rr;
+    q[3] = sig * (vaz.dec(rc, X.va) + rr * OmegaF) * rr;
+    if constexpr (ADI) q[4] = fld<SPLIT>(P, FCPT_ENERGY).dec(rc, X.e);
+    R.v = (r >= 0 && r <= P.nr) ? v0 : 0.0;
     R.s = in ? sig : 1.0;
 #pragma unroll
-    for (int k = 0; k < 6; ++k) {
+    for (int k = 0; k < NQ; ++k) {
         R.q[k] = in ? q[k] : 0.0;
         R.w[k] = in ? q[k] / sig : 0.0;
     }
-    return R;
+}
+
+// van Leer slopes of ring k from rings k-1, k, k+1 (compute_star_radial first
+// loop, TransportEuler.cpp:355-371); zero on the ghost rings
+template <int NQ>
+__device__ __forceinline__ void ring_slopes(const KParams &P, int k, const RingW<NQ> &M, RingW<NQ> &C,
+                                            const RingW<NQ> &N)
+{
+    const bool act = k > 0 && k < P.nr - 1;
+    const int kc = act ? k : 1;
+    const double im = cst(P.InvDiffRmed)[kc], ip = cst(P.InvDiffRmed)[kc + 1];
+    const double ds = van_leer_lim((N.s - C.s) * ip, (C.s - M.s) * im);
+    C.ds = act ? ds : 0.0;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const double dw = van_leer_lim((N.w[q] - C.w[q]) * ip, (C.w[q] - M.w[q]) * im);
+        C.dw[q] = act ? dw : 0.0;
+    }
+}
+
+// Upwind interface values at interface k between ring A = k-1 and ring B = k
+// (compute_star_radial second loop, TransportEuler.cpp:376-405; zero on the
+// edge interfaces k = 0 and k = nr) and the flux dt dphi Rinf[k] Q* Sigma* v_r
+// of each quantity through it (VanLeerRadial, :545-619).  The reference's
+// varq_sup of ring k-1 and varq_inf of ring k are this same product, Rsup[k-1]
+// and Rinf[k] being the same radius.
+template <int NQ>
+__device__ __forceinline__ void interface_flux(const KParams &P, int k, const RingW<NQ> &A,
+                                               const RingW<NQ> &B, double dt, double *f, double &fs)
+{
+    const bool act = k > 0 && k < P.nr;
+    const int kc = act ? k : 1;
+    const double v = B.v;
+    const double fpos = (cst(P.Rmed)[kc] - cst(P.Rmed)[kc - 1] - v * dt) * 0.5;
+    const double fneg = (cst(P.Rmed)[kc + 1] - cst(P.Rmed)[kc] + v * dt) * 0.5;
+    const bool up = v > 0.0;
+    const double ds0 = up ? A.s + fpos * A.ds : B.s - fneg * B.ds;
+    const double ds = act ? ds0 : 0.0;
+    const double dens = act ? 1.0 : 0.0;
+    const double c = dt * P.dphi * cst(P.Rinf)[k];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const double qs0 = up ? A.w[q] + fpos * A.dw[q] : B.w[q] - fneg * B.dw[q];
+        const double qs = act ? qs0 : 0.0;
+        f[q] = c * qs * ds * v;
+    }
+    fs = c * dens * ds * v;
 }
 
 // OneWindRad, TransportEuler.cpp:138-168 and VanLeerRadial :545-619.  Every
 // advected quantity of ring i is updated from rings i-2..i+2 of itself and of
 // the density before the sweep (DENSITY_INT), so all of them are done in one
 // pass that reads the state and writes separate outputs.  A thread owns one
-// sector and walks `tile` consecutive rings outward with a window of four
-// rings in registers: each ring is loaded once and each interface value
-// (DensityStar and every quantity's QStar) is formed once.
-template <bool SPLIT>
-__global__ void __launch_bounds__(64) k_radial_sweep(KParams P, int tile)
+// sector and walks `tile` consecutive rings outward.  Its window holds rings
+// k-1 and k with their slopes and ring k+1: each ring is loaded once, each
+// slope and each interface flux formed once, and the upwind choice is a
+// select, so the loop body is one branch-free block the compiler can schedule
+// as a whole.
+template <bool SPLIT, bool ADI>
+__device__ __forceinline__ void radial_sweep(const KParams &P, int tile)
 {
+    constexpr int NQ = ADI ? 5 : 4;
     const int ns = P.ns, nr = P.nr;
     const int j = blockIdx.x * 64 + threadIdx.x;
     const int i0 = blockIdx.y * tile;
     if (j >= ns || i0 >= nr) return;
     const int jn = j == ns - 1 ? 0 : j + 1;
     const double dt = P.s->dt;
-    const bool adi = P.adiabatic != 0;
-    void *const outp[6] = {P.mom_b[0], P.mom_b[1], P.mom_b[2], P.mom_b[3], P.en_b, P.sig_b};
-    const double *const outbg[6] = {P.bgm[0], P.bgm[1], P.bgm[2], P.bgm[3], P.bg[FCPT_ENERGY],
-                                    P.bg[FCPT_SIGMA]};
-    // window at interface k: rings k-2 .. k+1
-    RingVals R0 = load_ring<SPLIT>(P, i0 - 2, j, jn);
-    RingVals R1 = load_ring<SPLIT>(P, i0 - 1, j, jn);
-    RingVals R2 = load_ring<SPLIT>(P, i0, j, jn);
-    RingVals R3 = load_ring<SPLIT>(P, i0 + 1, j, jn);
-    double qs_prev[6], ds_prev = 0.0, v_prev = 0.0;
-    const int iend = i0 + tile < nr ? i0 + tile : nr;
-    for (int k = i0;; ++k) {
-        const double v = R2.v;
-        const double ds = star_radial(P, k, R0.s, R1.s, R2.s, R3.s, v, dt);
-        double qs[6];
-#pragma unroll
-        for (int q = 0; q < 6; ++q) qs[q] = star_radial(P, k, R0.w[q], R1.w[q], R2.w[q], R3.w[q], v, dt);
-        if (k > i0) {
-            // ring k-1 lies between interfaces k-1 and k
-            const int i = k - 1;
-#pragma unroll
-            for (int q = 0; q < 6; ++q) {
-                if (q == 4 && !adi) continue;
-                const double varq_inf = dt * P.dphi * P.Rinf[i] * qs_prev[q] * ds_prev * v_prev;
-                const double varq_sup = dt * P.dphi * P.Rsup[i] * qs[q] * ds * v;
-                buf<SPLIT>(P, outp[q], outbg[q]).set(i, j, R1.q[q] + (varq_inf - varq_sup) * P.InvSurf[i]);
-            }
-        }
-        if (k == iend) break;
-#pragma unroll
-        for (int q = 0; q < 6; ++q) qs_prev[q] = qs[q];
-        ds_prev = ds;
-        v_prev = v;
-        R0 = R1;
-        R1 = R2;
-        R2 = R3;
-        R3 = load_ring<SPLIT>(P, k + 2, j, jn);
+    RingW<NQ> A, B, C, D;
+    {
+        RingRaw<SPLIT, ADI> X0, X1, X2, X3;
+        raw_ring<SPLIT, ADI>(P, i0 - 2, j, jn, X0);
+        raw_ring<SPLIT, ADI>(P, i0 - 1, j, jn, X1);
+        raw_ring<SPLIT, ADI>(P, i0, j, jn, X2);
+        raw_ring<SPLIT, ADI>(P, i0 + 1, j, jn, X3);
+        decode_ring<SPLIT, ADI, NQ>(P, X0, D);
+        decode_ring<SPLIT, ADI, NQ>(P, X1, A);
+        decode_ring<SPLIT, ADI, NQ>(P, X2, B);
+        decode_ring<SPLIT, ADI, NQ>(P, X3, C);
     }
+    ring_slopes<NQ>(P, i0 - 1, D, A, B);
+    ring_slopes<NQ>(P, i0, A, B, C);
+    double f_prev[NQ], fs_prev;
+    interface_flux<NQ>(P, i0, A, B, dt, f_prev, fs_prev);
+    const int iend = i0 + tile < nr ? i0 + tile : nr;
+    RingRaw<SPLIT, ADI> X;
+    raw_ring<SPLIT, ADI>(P, i0 + 2, j, jn, X);
+    for (int k = i0 + 1; k <= iend; ++k) {
+        decode_ring<SPLIT, ADI, NQ>(P, X, D);      // ring k+1
+        raw_ring<SPLIT, ADI>(P, k + 2, j, jn, X);  // in flight during this iteration
+        ring_slopes<NQ>(P, k, B, C, D);            // ring k
+        A = B;
+        B = C;
+        C = D;
+        double f[NQ], fs;
+        interface_flux<NQ>(P, k, A, B, dt, f, fs);
+        // ring k-1 lies between interfaces k-1 and k
+        const int i = k - 1;
+        const double invs = cst(P.InvSurf)[i];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            buf<SPLIT>(P, P.mom_b[q], P.bgm[q]).set(i, j, A.q[q] + (f_prev[q] - f[q]) * invs);
+        if constexpr (ADI) buf<false>(P, P.en_b, nullptr).set(i, j, A.q[4] + (f_prev[4] - f[4]) * invs);
+        buf<false>(P, P.sig_b, nullptr).set(i, j, A.s + (fs_prev - fs) * invs);
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) f_prev[q] = f[q];
+        fs_prev = fs;
+    }
+}
+
+// the radial sweep under two register budgets: 3 wavefronts per SIMD (168
+// VGPRs) or 2 (256 VGPRs)
+template <bool SPLIT, bool ADI>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_radial_sweep2(KParams P, int tile)
+{
+    radial_sweep<SPLIT, ADI>(P, tile);
+}
+template <bool SPLIT, bool ADI>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_radial_sweep3(KParams P, int tile)
+{
+    radial_sweep<SPLIT, ADI>(P, tile);
 }
 
 // ComputeStarTheta for one interface j (TransportEuler.cpp:416-465);
@@ -1004,119 +1169,139 @@ __device__ __forceinline__ double star_theta(const double *q, double dxtheta, do
     return q[2] - (dxtheta + ksi) * dq;
 }
 
+// van Leer slope of one sector from its two neighbours (the first half of
+// ComputeStarTheta, TransportEuler.cpp:416-465); q points at sector j
+__device__ __forceinline__ double slope_theta(const double *q, double invdxtheta)
+{
+    return 0.5 * van_leer_lim(q[1] - q[0], q[0] - q[-1]) * invdxtheta;
+}
+
+// upwind interface value between sectors j-1 and j from the two sides'
+// slopes (the second half of ComputeStarTheta); q and d point at sector j
+__device__ __forceinline__ double star_theta_sel(const double *q, const double *d, double dxtheta,
+                                                 double ksi)
+{
+    const double lo = q[-1] + (dxtheta - ksi) * d[-1];
+    const double hi = q[0] - (dxtheta + ksi) * d[0];
+    return (ksi > 0.0) ? lo : hi;
+}
+
 // OneWindTheta, TransportEuler.cpp:270-289, in one pass per ring segment:
 // QuantitiesAdvection (:292-304, VanLeerTheta :630-664) with the residual
 // velocity v - <v>, then with the uniform residual of the ring, then
 // AdvectSHIFT (:238-268).  A block owns kAzSeg sectors of one ring and stages
-// them with a halo of 4 sectors in LDS (2 per van Leer pass); the result of
-// sector j is written to sector j + Nshift.  The density is advanced first
-// because the second pass needs its first-pass value as DENSITY_INT; the other
-// quantities only read the density, so their order does not matter.
-constexpr int kAzSeg = 256;
+// them with a halo of 4 sectors in LDS (2 per van Leer pass), one position per
+// thread; the result of sector j is written to sector j + Nshift.  The density
+// is advanced first because the second pass needs its first-pass value as
+// DENSITY_INT; the other quantities only read the density, so their order does
+// not matter.  The density's own specific value is exactly 1 (positive Sigma),
+// so its interface values are 1 and drop out of its flux.  In the first pass
+// the velocity changes sign along the ring, so every sector's slope is formed
+// once and each interface selects its upwind side; in the second pass the
+// velocity is uniform and each interface forms the one slope it needs.
 constexpr int kAzHalo = 4;
-constexpr int kAzExt = kAzSeg + 2 * kAzHalo;
+constexpr int kAzThreads = 256;
+constexpr int kAzSeg = kAzThreads - 2 * kAzHalo;
 
 template <bool SPLIT>
-__global__ void __launch_bounds__(256) k_azimuthal_sweep(KParams P)
+__global__ void __launch_bounds__(kAzThreads) k_azimuthal_sweep(KParams P)
 {
-    __shared__ double sS0[kAzExt], sV0[kAzExt], sDS0[kAzExt], sS1[kAzExt], sDS1[kAzExt];
-    __shared__ double sQ0[kAzExt], sW[kAzExt], sQS[kAzExt], sQ1[kAzExt];
+    __shared__ double sS0[kAzThreads], sV0[kAzThreads], sDS0[kAzThreads], sS1[kAzThreads],
+        sDS1[kAzThreads], sW[kAzThreads], sD[kAzThreads], sQS[kAzThreads];
     const int ns = P.ns;
     const int i = blockIdx.y;
     const int j0 = blockIdx.x * kAzSeg;
     const int Ls = ns - j0 < kAzSeg ? ns - j0 : kAzSeg;
     const int next = Ls + 2 * kAzHalo;  // staged positions p: sector j0 - 4 + p
-    const int t = threadIdx.x;
+    const int p = threadIdx.x;
     const double dt = P.s->dt;
-    const double vm = P.vmean[i];
-    const double vu = P.vuni[i];
-    const int nsh = P.nshift[i];
-    const double dxtheta = P.dphi * P.Rmed[i];
+    const double vm = cst(P.vmean)[i];
+    const double vu = cst(P.vuni)[i];
+    const double ksu = vu * dt;
+    const int nsh = cst(P.nshift)[i];
+    const double dxtheta = P.dphi * cst(P.Rmed)[i];
     const double invdxtheta = 1.0 / dxtheta;
-    const double dxrad = (P.Rsup[i] - P.Rinf[i]) * dt;
-    const double invsurf = P.InvSurf[i];
-    auto sector = [ns](int j) {
-        if (j < 0) j += ns;
-        if (j >= ns) j -= ns;
-        return j;
-    };
-    auto dst_sector = [ns](int j) {
-        while (j < 0) j += ns;
-        while (j >= ns) j -= ns;
-        return j;
-    };
+    const double dxrad = (cst(P.Rsup)[i] - cst(P.Rinf)[i]) * dt;
+    const double invsurf = cst(P.InvSurf)[i];
+    int js = j0 - kAzHalo + p;  // this thread's sector
+    if (js < 0) js += ns;
+    if (js >= ns) js -= ns;
+    int jd = j0 - kAzHalo + p + nsh;  // where its result goes
+    while (jd < 0) jd += ns;
+    while (jd >= ns) jd -= ns;
+    const bool staged = p < next;
+    const bool slope1 = p >= 1 && p < next - 1;     // first-pass slopes
+    const bool face1 = p >= 2 && p <= next - 2;     // first-pass interfaces
+    const bool cell1 = p >= 2 && p < next - 2;      // first-pass updates
+    const bool face2 = p >= kAzHalo && p <= kAzHalo + Ls;  // second-pass interfaces
+    const bool cell2 = p >= kAzHalo && p < kAzHalo + Ls;   // second-pass updates (written)
     FLD(vaz, FCPT_VAZI);
-    const Fld<SPLIT> Sin = buf<SPLIT>(P, P.sig_b, P.bg[FCPT_SIGMA]);
-    for (int p = t; p < next; p += 256) {
-        const int j = sector(j0 - kAzHalo + p);
-        sS0[p] = Sin(i, j);
-        sV0[p] = vaz(i, j) - vm;
+    const int nq = P.adiabatic ? 5 : 4;
+    // every global load of the block is issued here, ahead of the first store
+    double s0 = 1.0, v0 = 0.0, q0v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (staged) {
+        s0 = buf<false>(P, P.sig_b, nullptr)(i, js);
+        v0 = vaz(i, js) - vm;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) q0v[q] = buf<SPLIT>(P, P.mom_b[q], P.bgm[q])(i, js);
+        if (nq == 5) q0v[4] = buf<false>(P, P.en_b, nullptr)(i, js);
+        sS0[p] = s0;
+        sV0[p] = v0;
+    }
+    const double ks0 = v0 * dt;
+    __syncthreads();
+    // density, first pass
+    if (slope1) sD[p] = slope_theta(&sS0[p], invdxtheta);
+    __syncthreads();
+    if (face1) sDS0[p] = star_theta_sel(&sS0[p], &sD[p], dxtheta, ks0);
+    __syncthreads();
+    double s1 = 0.0;
+    if (cell1) {
+        double varq = dxrad * sDS0[p] * v0;
+        varq -= dxrad * sDS0[p + 1] * sV0[p + 1];
+        s1 = s0 + varq * invsurf;
+        sS1[p] = s1;
     }
     __syncthreads();
-    // first pass (residual velocity): DensityStar on interfaces 2 .. next-2
-    for (int p = 2 + t; p <= next - 2; p += 256)
-        sDS0[p] = star_theta(&sS0[p - 2], dxtheta, invdxtheta, sV0[p] * dt);
-    // the density's own first pass
-    for (int p = t; p < next; p += 256) sW[p] = sS0[p] / sS0[p];
+    // density, second pass
+    if (face2) sDS1[p] = star_theta(&sS1[p - 2], dxtheta, invdxtheta, ksu);
     __syncthreads();
-    for (int p = 2 + t; p <= next - 2; p += 256)
-        sQS[p] = star_theta(&sW[p - 2], dxtheta, invdxtheta, sV0[p] * dt);
-    __syncthreads();
-    for (int p = 2 + t; p < next - 2; p += 256) {
-        double varq = dxrad * sQS[p] * sDS0[p] * sV0[p];
-        varq -= dxrad * sQS[p + 1] * sDS0[p + 1] * sV0[p + 1];
-        sS1[p] = sS0[p] + varq * invsurf;
-    }
-    __syncthreads();
-    // second pass (uniform residual): DensityStar on interfaces 4 .. 4+Ls
-    for (int p = kAzHalo + t; p <= kAzHalo + Ls; p += 256)
-        sDS1[p] = star_theta(&sS1[p - 2], dxtheta, invdxtheta, vu * dt);
-    for (int p = 2 + t; p < next - 2; p += 256) sW[p] = sS1[p] / sS1[p];
-    __syncthreads();
-    for (int p = kAzHalo + t; p <= kAzHalo + Ls; p += 256)
-        sQS[p] = star_theta(&sW[p - 2], dxtheta, invdxtheta, vu * dt);
-    __syncthreads();
-    {
-        const Fld<SPLIT> Sout = buf<SPLIT>(P, P.sig_c, P.bg[FCPT_SIGMA]);
-        for (int p = kAzHalo + t; p < kAzHalo + Ls; p += 256) {
-            double varq = dxrad * sQS[p] * sDS1[p] * vu;
-            varq -= dxrad * sQS[p + 1] * sDS1[p + 1] * vu;
-            Sout.set(i, dst_sector(j0 - kAzHalo + p + nsh), sS1[p] + varq * invsurf);
-        }
+    if (cell2) {
+        double varq = dxrad * sDS1[p] * vu;
+        varq -= dxrad * sDS1[p + 1] * vu;
+        buf<false>(P, P.sig_c, nullptr).set(i, jd, s1 + varq * invsurf);
     }
     // the other quantities
-    const int nq = P.adiabatic ? 5 : 4;
-    for (int q = 0; q < nq; ++q) {
-        const Fld<SPLIT> Qin = q < 4 ? buf<SPLIT>(P, P.mom_b[q], P.bgm[q])
-                                     : buf<SPLIT>(P, P.en_b, P.bg[FCPT_ENERGY]);
-        const Fld<SPLIT> out = q < 4 ? buf<SPLIT>(P, P.mom[q], P.bgm[q])
-                                     : buf<SPLIT>(P, P.en_c, P.bg[FCPT_ENERGY]);
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+        if (q >= nq) break;
+        const double q0 = q0v[q];
+        if (staged) sW[p] = q0 / s0;
         __syncthreads();
-        for (int p = t; p < next; p += 256) {
-            const double Qv = Qin(i, sector(j0 - kAzHalo + p));
-            sQ0[p] = Qv;
-            sW[p] = Qv / sS0[p];
-        }
+        if (slope1) sD[p] = slope_theta(&sW[p], invdxtheta);
         __syncthreads();
-        for (int p = 2 + t; p <= next - 2; p += 256)
-            sQS[p] = star_theta(&sW[p - 2], dxtheta, invdxtheta, sV0[p] * dt);
+        if (face1) sQS[p] = star_theta_sel(&sW[p], &sD[p], dxtheta, ks0);
         __syncthreads();
-        for (int p = 2 + t; p < next - 2; p += 256) {
-            double varq = dxrad * sQS[p] * sDS0[p] * sV0[p];
+        // sW is free again: its last readers (the slopes) are two barriers back
+        double q1 = 0.0;
+        if (cell1) {
+            double varq = dxrad * sQS[p] * sDS0[p] * v0;
             varq -= dxrad * sQS[p + 1] * sDS0[p + 1] * sV0[p + 1];
-            const double q1 = sQ0[p] + varq * invsurf;
-            sQ1[p] = q1;
-            sW[p] = q1 / sS1[p];
+            q1 = q0 + varq * invsurf;
+            sW[p] = q1 / s1;
         }
         __syncthreads();
-        for (int p = kAzHalo + t; p <= kAzHalo + Ls; p += 256)
-            sQS[p] = star_theta(&sW[p - 2], dxtheta, invdxtheta, vu * dt);
+        if (face2) sQS[p] = star_theta(&sW[p - 2], dxtheta, invdxtheta, ksu);
         __syncthreads();
-        for (int p = kAzHalo + t; p < kAzHalo + Ls; p += 256) {
+        if (cell2) {
             double varq = dxrad * sQS[p] * sDS1[p] * vu;
             varq -= dxrad * sQS[p + 1] * sDS1[p + 1] * vu;
-            out.set(i, dst_sector(j0 - kAzHalo + p + nsh), sQ1[p] + varq * invsurf);
+            const double val = q1 + varq * invsurf;
+            if (q < 4) buf<SPLIT>(P, P.mom[q], P.bgm[q]).set(i, jd, val);
+            else buf<false>(P, P.en_c, nullptr).set(i, jd, val);
         }
+        // the next quantity rewrites sW and sD first; sQS, read just above,
+        // is rewritten only after two more barriers
     }
 }
 
@@ -1127,29 +1312,30 @@ template <bool SPLIT>
 __global__ void __launch_bounds__(256) k_velocities(KParams P)
 {
     CELL_LOOP_HEAD(0, P.nr)
-    const Fld<SPLIT> Sb = buf<SPLIT>(P, P.sig_c, P.bg[FCPT_SIGMA]);
+    const Fld<false> Sb = buf<false>(P, P.sig_c, nullptr);
+    const Fld<SPLIT> m0 = buf<SPLIT>(P, P.mom[0], P.bgm[0]);
+    const Fld<SPLIT> m1 = buf<SPLIT>(P, P.mom[1], P.bgm[1]);
+    const Fld<SPLIT> m2 = buf<SPLIT>(P, P.mom[2], P.bgm[2]);
+    const Fld<SPLIT> m3 = buf<SPLIT>(P, P.mom[3], P.bgm[3]);
+    const Fld<false> Eb = buf<false>(P, P.en_c, nullptr);
     FLD(vr, FCPT_VRAD);
     FLD(vaz, FCPT_VAZI);
     const int jm = j == 0 ? ns - 1 : j - 1;
-    const double s0 = Sb(i, j);
+    const int im = i >= 1 ? i - 1 : 0;
+    const double s0 = Sb(i, j), sm = Sb(im, j), sjm = Sb(i, jm);
+    const double m0v = m0(im, j), m1v = m1(i, j), m2v = m2(i, jm), m3v = m3(i, j);
+    const double e = P.adiabatic ? Eb(i, j) : 0.0;
     if (i == 0) {
         vr.set(i, j, 0.0);
     } else {
-        const Fld<SPLIT> m0 = buf<SPLIT>(P, P.mom[0], P.bgm[0]);
-        const Fld<SPLIT> m1 = buf<SPLIT>(P, P.mom[1], P.bgm[1]);
-        vr.set(i, j, (m0(i - 1, j) + m1(i, j)) / (Sb(i - 1, j) + s0));
+        vr.set(i, j, (m0v + m1v) / (sm + s0));
     }
-    const Fld<SPLIT> m2 = buf<SPLIT>(P, P.mom[2], P.bgm[2]);
-    const Fld<SPLIT> m3 = buf<SPLIT>(P, P.mom[3], P.bgm[3]);
-    vaz.set(i, j, (m2(i, jm) + m3(i, j)) / (Sb(i, jm) + s0) * P.InvRmed[i] - P.Rmed[i] * P.omega_frame);
+    vaz.set(i, j, (m2v + m3v) / (sjm + s0) * cst(P.InvRmed)[i] - cst(P.Rmed)[i] * P.omega_frame);
     const double floor_value = P.sigma_floor * P.sigma0;
     double sig = s0;
     if (sig < floor_value) sig = floor_value;
     fld<SPLIT>(P, FCPT_SIGMA).set(i, j, sig);
-    if (P.adiabatic) {
-        const Fld<SPLIT> Eb = buf<SPLIT>(P, P.en_c, P.bg[FCPT_ENERGY]);
-        fld<SPLIT>(P, FCPT_ENERGY).set(i, j, temperature_floor(P, Eb(i, j), sig));
-    }
+    if (P.adiabatic) fld<SPLIT>(P, FCPT_ENERGY).set(i, j, temperature_floor(P, e, sig));
 }
 
 // ---------------------------------------------------------------------------
@@ -1169,16 +1355,16 @@ __global__ void __launch_bounds__(64) k_totals_rings(KParams P, double radius, d
     const int i = blockIdx.x;  // ring
     const int t = threadIdx.x;
     double m = 0.0, mc = 0.0, l = 0.0, lc = 0.0;
-    const bool active = i >= 1 && i < P.nr - 1 && P.Rmed[i] <= radius;
+    const bool active = i >= 1 && i < P.nr - 1 && cst(P.Rmed)[i] <= radius;
     if (active) {
         FLD(S, FCPT_SIGMA);
         FLD(vaz, FCPT_VAZI);
         for (int j = t; j < ns; j += 64) {
             const int jm = j == 0 ? ns - 1 : j - 1;
             const double s = S(i, j);
-            neumaier_add(m, mc, P.Surf[i] * s);
-            neumaier_add(l, lc, P.Surf[i] * 0.5 * (s + S(i, jm)) * P.Rmed[i] *
-                                    (vaz(i, j) + P.omega_frame * P.Rmed[i]));
+            neumaier_add(m, mc, cst(P.Surf)[i] * s);
+            neumaier_add(l, lc, cst(P.Surf)[i] * 0.5 * (s + S(i, jm)) * cst(P.Rmed)[i] *
+                                    (vaz(i, j) + P.omega_frame * cst(P.Rmed)[i]));
         }
     }
     __shared__ double sh[4][64];
@@ -1232,7 +1418,7 @@ __global__ void __launch_bounds__(256) k_rebase(float *d, const double *oldbg, c
                                                 int ns, int rows)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = blockIdx.y * blockDim.y + threadIdx.y;
+    const int i = __builtin_amdgcn_readfirstlane((int)(blockIdx.y * blockDim.y + threadIdx.y));
     if (j >= ns || i >= rows) return;
     const size_t c = IX(i, j);
     d[c] = (float)((oldbg[i] + (double)d[c]) - newbg[i]);
@@ -1299,7 +1485,11 @@ void launch_set_params(const KParams &P, hipStream_t st, const fcpt_step_params 
     hipLaunchKernelGGL(k_set_params, dim3(1), dim3(1), 0, st, P, sp);
 }
 
-void launch_potential(const KParams &P, hipStream_t st) { LAUNCH(k_potential, cell_grid(P, P.nr), kBlock, st, P); }
+void launch_potential(const KParams &P, hipStream_t st)
+{
+    if (P.split) hipLaunchKernelGGL(k_potential_bg, dim3((unsigned)((P.nr + 255) / 256)), dim3(256), 0, st, P);
+    LAUNCH(k_potential, cell_grid(P, P.nr), kBlock, st, P);
+}
 
 void launch_sources(const KParams &P, hipStream_t st)
 {
@@ -1368,12 +1558,30 @@ void launch_radial_sweep(const KParams &P, hipStream_t st)
 {
     const int tile = P.radial_tile > 0 ? P.radial_tile : radial_tile_for(P.nr, P.ns, 0);
     const dim3 g((unsigned)((P.ns + 63) / 64), (unsigned)((P.nr + tile - 1) / tile));
-    LAUNCH(k_radial_sweep, g, dim3(64), st, P, tile);
+    // register budget: 3 wavefronts per SIMD (168 VGPRs) or 2 (256 VGPRs)
+    const bool w2 = P.radial_waves == 2;
+    if (P.split) {
+        if (P.adiabatic) {
+            if (w2) hipLaunchKernelGGL((k_radial_sweep2<true, true>), g, dim3(64), 0, st, P, tile);
+            else hipLaunchKernelGGL((k_radial_sweep3<true, true>), g, dim3(64), 0, st, P, tile);
+        } else {
+            if (w2) hipLaunchKernelGGL((k_radial_sweep2<true, false>), g, dim3(64), 0, st, P, tile);
+            else hipLaunchKernelGGL((k_radial_sweep3<true, false>), g, dim3(64), 0, st, P, tile);
+        }
+    } else {
+        if (P.adiabatic) {
+            if (w2) hipLaunchKernelGGL((k_radial_sweep2<false, true>), g, dim3(64), 0, st, P, tile);
+            else hipLaunchKernelGGL((k_radial_sweep3<false, true>), g, dim3(64), 0, st, P, tile);
+        } else {
+            if (w2) hipLaunchKernelGGL((k_radial_sweep2<false, false>), g, dim3(64), 0, st, P, tile);
+            else hipLaunchKernelGGL((k_radial_sweep3<false, false>), g, dim3(64), 0, st, P, tile);
+        }
+    }
 }
 void launch_azimuthal_sweep(const KParams &P, hipStream_t st)
 {
     const dim3 g((unsigned)((P.ns + kAzSeg - 1) / kAzSeg), (unsigned)P.nr);
-    LAUNCH(k_azimuthal_sweep, g, dim3(256), st, P);
+    LAUNCH(k_azimuthal_sweep, g, dim3(kAzThreads), st, P);
 }
 
 void launch_copy4(const KParams &P, hipStream_t st)

@@ -40,11 +40,12 @@ DESCRIPTION = {
 }
 
 
-def disk_config(name: str) -> dict:
+def disk_config(name: str, precision: int = 0) -> dict:
+    """Descriptor fields of a workload; precision 0 = FP64 storage, 1 = split."""
     nr, nphi, rmax, adi, artvisc, beta, _ = WORKLOADS[name]
     return {
         "nr": nr, "nphi": nphi, "radii": log_radii(nr, 0.4, rmax), "rmin": 0.4, "rmax": rmax,
-        "adiabatic": int(adi), "precision": 0,
+        "adiabatic": int(adi), "precision": int(precision),
         "G": CODE_UNITS["G"], "R": CODE_UNITS["R"], "sigma_sb": CODE_UNITS["sigma_sb"],
         "c_light": CODE_UNITS["c_light"], "hydro_center_mass": 1.0, "mu": 2.35, "gamma": 1.4,
         "aspect_ratio": 0.05, "flaring_index": 0.0, "alpha": 1e-3, "thickness_smoothing": 0.6,
@@ -98,9 +99,10 @@ def start(disk, name: str, first_dt: float = 0.1):
     return ic
 
 
-def algorithmic_bytes(kernel: str, adiabatic: bool) -> int:
-    """Compulsory HBM bytes per cell of one launch of a step kernel: each FP64
-    field it reads once plus each field it writes once."""
+def algorithmic_bytes(kernel: str, adiabatic: bool, element: int = 8) -> int:
+    """Compulsory HBM bytes per cell of one launch of a step kernel: each
+    stored field it reads once plus each field it writes once, `element` bytes
+    per value (8 for FP64 storage, 4 for split storage)."""
     e = 1 if adiabatic else 0
     reads_writes = {
         "radial_sweep": (3 + e, 5 + e),        # Sigma, v_r, v_phi, (E) -> 4 momenta, Sigma, (E)
@@ -110,10 +112,15 @@ def algorithmic_bytes(kernel: str, adiabatic: bool) -> int:
         "potential": (1, 1),                   # H -> Phi
     }
     r, w = reads_writes[kernel]
-    return 8 * (r + w)
+    return element * (r + w)
 
 
-STEP_BYTES = {False: 48, True: 64}  # evolved fields read + written once per step (FP64)
+def step_bytes(adiabatic: bool, element: int = 8) -> int:
+    """Evolved fields (Sigma, v_r, v_phi and E) read and written once per step."""
+    return element * 2 * (4 if adiabatic else 3)
+
+
+STEP_BYTES = {False: step_bytes(False), True: step_bytes(True)}
 
 
 def cells(name: str) -> int:
@@ -126,4 +133,4 @@ def is_adiabatic(name: str) -> bool:
 
 
 __all__ = ["CODE_UNITS", "WORKLOADS", "DESCRIPTION", "disk_config", "setup", "planet_inputs",
-           "start", "algorithmic_bytes", "STEP_BYTES", "cells", "is_adiabatic"]
+           "start", "algorithmic_bytes", "step_bytes", "STEP_BYTES", "cells", "is_adiabatic"]

@@ -211,10 +211,17 @@ int fcpt_time_copy(fcpt_ctx *ctx, int32_t reps, double *ms_per_launch, double *b
 
 /* Launch-shape tuning (no counterpart in the reference): rings walked by one
    thread of the radial sweep (FCPT_PARAM_RADIAL_TILE, 1 .. 64; 0 restores the
-   size-based default).  Results do not depend on it.  Drops a captured step
-   graph so that the next fcpt_run_steps captures the new shape. */
-enum { FCPT_PARAM_RADIAL_TILE = 0 };
+   size-based default) and the sweep's register budget in wavefronts per SIMD
+   (FCPT_PARAM_RADIAL_WAVES, 2 or 3; default 3).  Results do not depend on
+   them.  Drops a captured step graph so that the next fcpt_run_steps captures
+   the new shape. */
+enum { FCPT_PARAM_RADIAL_TILE = 0, FCPT_PARAM_RADIAL_WAVES = 1 };
 int fcpt_set_launch_param(fcpt_ctx *ctx, int32_t param, int32_t value);
+/* Autotuner (no counterpart in the reference): times every launch shape of
+   the tunable kernels on the context's current state (`reps` launches each;
+   the timed kernels write transport scratch only), keeps the fastest and
+   returns it.  Drops a captured step graph. */
+int fcpt_autotune(fcpt_ctx *ctx, int32_t reps, int32_t *radial_tile, int32_t *radial_waves);
 
 /* Compensated (Neumaier) totals over active rings with Rmed <= radius
    (quantities::gas_total_mass / gas_angular_momentum, quantities.cpp:51-78,242-279). */
