@@ -71,6 +71,76 @@ def timed_run(name: str, prec: int, steps: int, warmup: int, use_graph: bool, tu
     return disk, wall, ms_events
 
 
+def batch_run(name: str, prec: int, steps: int, warmup: int, use_graph: bool):
+    """The batched sweep: every member disk started, then `steps` batched
+    steps timed after `warmup` untimed ones."""
+    from fargocpt_amd import workloads as W
+    from fargocpt_amd.capi import Batch, Disk
+    members = W.batch_members(name)
+    disks = []
+    for m in members:
+        d = Disk(W.member_config(m, prec))
+        W.member_start(d, m)
+        disks.append(d)
+    batch = Batch(disks)
+    batch.run_steps(warmup, use_graph)
+    batch.synchronize()
+    t0 = time.perf_counter()
+    ms_events = batch.time_steps(steps, use_graph)
+    wall = time.perf_counter() - t0
+    cells = sum(m["nr"] * m["nphi"] for m in members)
+    return batch, disks, cells, wall, ms_events
+
+
+def bench_batch(args) -> int:
+    from fargocpt_amd import workloads as W
+    from fargocpt_amd.capi import KERNELS
+    name = args.workload
+    prec = PRECISIONS[args.precision]
+    use_graph = not args.no_graph
+    batch, disks, cells, wall, ms_events = batch_run(name, prec, args.steps, args.warmup, use_graph)
+    value = cells * args.steps / wall
+    el = ELEMENT[prec]
+    kt = {k: batch.time_kernel(v, 10) for k, v in KERNELS.items()}
+    dom = max(kt, key=kt.get)
+    achieved = W.algorithmic_bytes(dom, False, el) * cells / (kt[dom] * 1e-3) / 1e9
+    n = len(disks)
+    out = {
+        "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": name, "disks": n, "cells": cells,
+                   "description": f"{n} locally isothermal disks, Nr ~ U[128,1024], Nphi = 3 Nr, "
+                                  "q ~ logU[1e-5,1e-3], alpha ~ logU[1e-4,1e-2], h ~ U[0.03,0.07], "
+                                  "one launch per stage over a work table of all disks' blocks",
+                   "storage": "f64" if prec == 0 else "f32 deviations on f64 ring backgrounds",
+                   "graph": use_graph},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_cell": W.algorithmic_bytes(dom, False, el), "ms_per_launch": kt[dom]},
+        "step": {"ms_per_step_events": ms_events / args.steps,
+                 "algorithmic_GBs": W.step_bytes(False, el) * value / 1e9,
+                 "frac_of_hbm_peak": W.step_bytes(False, el) * value / 1e9 / HBM_PEAK_GBS,
+                 "kernel_ms": kt},
+    }
+    if not args.no_compare:
+        other = 1 - prec
+        b2, d2, _, wall2, _ = batch_run(name, other, args.steps, args.warmup, use_graph)
+        value2 = cells * args.steps / wall2
+        fp64, split = (disks, d2) if prec == 0 else (d2, disks)
+        worst = {f: max(rel_err(s.download_field(f), d.download_field(f)) for s, d in zip(split, fp64))
+                 for f in ("SIGMA", "VRAD", "VAZI")}
+        out["compare"] = {"precision": "split" if other == 1 else "fp64", "value": value2,
+                          "ms_per_step": 1e3 * wall2 / args.steps,
+                          "split_speedup_vs_fp64": (value2 / value) if other == 1 else (value / value2),
+                          "split_rel_err_vs_fp64_worst_disk": worst,
+                          "after_steps": args.warmup + args.steps}
+        b2.close()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def rel_err(a, b) -> float:
     import numpy as np
     scale = float(np.max(np.abs(b)))
@@ -100,6 +170,8 @@ def main() -> int:
     from fargocpt_amd import workloads as W
     from fargocpt_amd.capi import KERNELS
 
+    if args.workload in W.BATCHES:
+        return bench_batch(args)
     name = args.workload
     prec = PRECISIONS[args.precision]
     use_graph = not args.no_graph

@@ -164,6 +164,13 @@ def load_library(path: str = LIB_PATH):
         "fcpt_time_copy": [P, I, dp, dp],
         "fcpt_set_launch_param": [P, I, I],
         "fcpt_autotune": [P, I, C.POINTER(I), C.POINTER(I)],
+        "fcpt_batch_create": [C.POINTER(P), I, C.POINTER(P)],
+        "fcpt_batch_destroy": [P],
+        "fcpt_batch_synchronize": [P],
+        "fcpt_batch_set_radial_waves": [P, I],
+        "fcpt_batch_run_steps": [P, I, I],
+        "fcpt_batch_time_steps": [P, I, I, dp],
+        "fcpt_batch_time_kernel": [P, I, I, dp],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
@@ -171,6 +178,8 @@ def load_library(path: str = LIB_PATH):
         fn.restype = C.c_int
     lib.fcpt_last_error.argtypes = [P]
     lib.fcpt_last_error.restype = C.c_char_p
+    lib.fcpt_batch_last_error.argtypes = [P]
+    lib.fcpt_batch_last_error.restype = C.c_char_p
     _lib = lib
     return lib
 
@@ -340,3 +349,54 @@ class Disk:
     def set_radial_waves(self, waves):
         """Register budget of the radial sweep in wavefronts per SIMD (2 or 3)."""
         self._check(self.lib.fcpt_set_launch_param(self.h, PARAM_RADIAL_WAVES, waves))
+
+
+class Batch:
+    """Many Disk objects advanced together, one launch per stage of the step
+    (fcpt_batch_* in include/fcpt.h).  The disks keep their own state; read
+    them with their download/totals methods between batch calls."""
+
+    def __init__(self, disks):
+        self.lib = load_library()
+        self.disks = list(disks)
+        arr = (C.c_void_p * len(self.disks))(*[d.h.value for d in self.disks])
+        h = C.c_void_p()
+        rc = self.lib.fcpt_batch_create(arr, len(self.disks), C.byref(h))
+        if rc != 0:
+            raise RuntimeError(f"fcpt_batch_create failed ({rc})")
+        self.h = h
+
+    def _check(self, rc):
+        if rc != 0:
+            msg = self.lib.fcpt_batch_last_error(self.h)
+            raise RuntimeError(f"fcpt batch call failed ({rc}): {msg.decode() if msg else ''}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.fcpt_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_radial_waves(self, waves):
+        self._check(self.lib.fcpt_batch_set_radial_waves(self.h, waves))
+
+    def synchronize(self):
+        self._check(self.lib.fcpt_batch_synchronize(self.h))
+
+    def run_steps(self, n, use_graph=True):
+        self._check(self.lib.fcpt_batch_run_steps(self.h, n, int(use_graph)))
+
+    def time_steps(self, n, use_graph=True):
+        ms = C.c_double()
+        self._check(self.lib.fcpt_batch_time_steps(self.h, n, int(use_graph), C.byref(ms)))
+        return ms.value
+
+    def time_kernel(self, kernel, reps):
+        ms = C.c_double()
+        self._check(self.lib.fcpt_batch_time_kernel(self.h, kernel, reps, C.byref(ms)))
+        return ms.value

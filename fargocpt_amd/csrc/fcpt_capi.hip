@@ -153,6 +153,7 @@ int fcpt_create(const fcpt_disk_desc *desc, fcpt_ctx **out)
     P.split = desc->precision == FCPT_PREC_SPLIT32 ? 1 : 0;
     P.radial_tile = 0;
     P.radial_waves = 3;
+    init_launch_fields(P);
     // Interpret.cpp:230-231
     P.dphi = 2.0 * M_PI / (double)ns;
     P.invdphi = (double)ns / (2.0 * M_PI);
@@ -892,6 +893,230 @@ int fcpt_totals(fcpt_ctx *ctx, double radius, double *mass, double *angular_mome
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     if (mass) *mass = ctx->totals_host[0];
     if (angular_momentum) *angular_momentum = ctx->totals_host[1];
+    return FCPT_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batches: many disks advanced by one launch per stage (plan_batch_step)
+
+struct fcpt_batch {
+    int device = 0;
+    int split = 0;
+    int radial_waves = 3;
+    std::vector<fcpt_ctx *> disks;
+    KParams *Ps = nullptr;  // device array, one per disk
+    std::vector<BatchStageDev> stages;
+    std::vector<void *> allocs;
+    hipStream_t stream = nullptr;
+    hipGraphExec_t graph = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string err;
+};
+
+namespace {
+int bfail(fcpt_batch *b, int code, const char *msg)
+{
+    if (b) b->err = msg;
+    return code;
+}
+#define BHIP(b, expr)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess) {                                                                \
+            if (b) b->err = std::string(#expr) + ": " + hipGetErrorString(e_);                 \
+            return FCPT_ERR_DEVICE;                                                            \
+        }                                                                                      \
+    } while (0)
+
+void enqueue_batch_step(const fcpt_batch *b, hipStream_t st)
+{
+    for (const BatchStageDev &s : b->stages) launch_batch_stage(b->Ps, s, b->split, b->radial_waves, st);
+}
+}  // namespace
+
+extern "C" {
+
+const char *fcpt_batch_last_error(const fcpt_batch *b) { return b ? b->err.c_str() : "null batch"; }
+
+int fcpt_batch_create(fcpt_ctx *const *disks, int32_t n_disks, fcpt_batch **out)
+{
+    if (!disks || !out || n_disks <= 0) return FCPT_ERR_ARG;
+    *out = nullptr;
+    fcpt_batch *b = new fcpt_batch();
+    b->device = disks[0] ? disks[0]->device : 0;
+    b->split = disks[0] ? disks[0]->P.split : 0;
+    std::vector<KParams> host;
+    for (int k = 0; k < n_disks; ++k) {
+        fcpt_ctx *c = disks[k];
+        if (!c || c->device != b->device || c->P.split != b->split) {
+            delete b;
+            return FCPT_ERR_ARG;  // one device and one storage mode per batch
+        }
+        b->disks.push_back(c);
+        KParams P = c->P;
+        P.radial_tile = radial_tile_for(P.nr, P.ns, P.radial_tile);
+        host.push_back(P);
+    }
+    hipError_t e = hipSetDevice(b->device);
+    if (e != hipSuccess) {
+        delete b;
+        return FCPT_ERR_DEVICE;
+    }
+    auto alloc = [&](size_t bytes) -> void * {
+        void *q = nullptr;
+        if (hipMalloc(&q, bytes ? bytes : 16) != hipSuccess) return nullptr;
+        b->allocs.push_back(q);
+        return q;
+    };
+    b->Ps = static_cast<KParams *>(alloc(host.size() * sizeof(KParams)));
+    int rc = b->Ps ? FCPT_OK : FCPT_ERR_DEVICE;
+    if (!rc && hipMemcpy(b->Ps, host.data(), host.size() * sizeof(KParams), hipMemcpyHostToDevice) != hipSuccess)
+        rc = FCPT_ERR_DEVICE;
+    if (!rc) {
+        for (const BatchStageHost &h : plan_batch_step(host)) {
+            void *m = alloc(h.entries.size() * sizeof(BatchEntry));
+            if (!m || hipMemcpy(m, h.entries.data(), h.entries.size() * sizeof(BatchEntry),
+                                hipMemcpyHostToDevice) != hipSuccess) {
+                rc = FCPT_ERR_DEVICE;
+                break;
+            }
+            b->stages.push_back(BatchStageDev{h.kind, h.arg, static_cast<const BatchEntry *>(m),
+                                              (unsigned)h.entries.size()});
+        }
+    }
+    if (!rc && hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) rc = FCPT_ERR_DEVICE;
+    if (rc) {
+        fcpt_batch_destroy(b);
+        return rc;
+    }
+    *out = b;
+    return FCPT_OK;
+}
+
+int fcpt_batch_destroy(fcpt_batch *b)
+{
+    if (!b) return FCPT_OK;
+    hipSetDevice(b->device);
+    if (b->stream) {
+        hipStreamSynchronize(b->stream);
+        hipStreamDestroy(b->stream);
+    }
+    if (b->graph) hipGraphExecDestroy(b->graph);
+    if (b->ev0) hipEventDestroy(b->ev0);
+    if (b->ev1) hipEventDestroy(b->ev1);
+    for (void *p : b->allocs) hipFree(p);
+    delete b;
+    return FCPT_OK;
+}
+
+int fcpt_batch_set_radial_waves(fcpt_batch *b, int32_t waves)
+{
+    if (!b) return FCPT_ERR_ARG;
+    if (waves != 2 && waves != 3) return bfail(b, FCPT_ERR_ARG, "radial waves must be 2 or 3");
+    BHIP(b, hipSetDevice(b->device));
+    BHIP(b, hipStreamSynchronize(b->stream));
+    b->radial_waves = waves;
+    if (b->graph) {
+        BHIP(b, hipGraphExecDestroy(b->graph));
+        b->graph = nullptr;
+    }
+    return FCPT_OK;
+}
+
+int fcpt_batch_synchronize(fcpt_batch *b)
+{
+    if (!b) return FCPT_ERR_ARG;
+    BHIP(b, hipSetDevice(b->device));
+    BHIP(b, hipStreamSynchronize(b->stream));
+    return FCPT_OK;
+}
+
+int fcpt_batch_run_steps(fcpt_batch *b, int32_t n_steps, int32_t use_graph)
+{
+    if (!b) return FCPT_ERR_ARG;
+    if (n_steps < 0) return bfail(b, FCPT_ERR_ARG, "negative step count");
+    BHIP(b, hipSetDevice(b->device));
+    // the disks' own queued work (uploads, start-up calls) comes first
+    for (fcpt_ctx *c : b->disks) BHIP(b, hipStreamSynchronize(c->stream));
+    if (use_graph) {
+        if (!b->graph) {
+            hipGraph_t g = nullptr;
+            BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            enqueue_batch_step(b, b->stream);
+            hipError_t e = hipStreamEndCapture(b->stream, &g);
+            if (e != hipSuccess) return bfail(b, FCPT_ERR_DEVICE, hipGetErrorString(e));
+            e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+            hipGraphDestroy(g);
+            if (e != hipSuccess) {
+                b->graph = nullptr;
+                return bfail(b, FCPT_ERR_DEVICE, hipGetErrorString(e));
+            }
+        }
+        for (int k = 0; k < n_steps; ++k) BHIP(b, hipGraphLaunch(b->graph, b->stream));
+    } else {
+        for (int k = 0; k < n_steps; ++k) enqueue_batch_step(b, b->stream);
+        BHIP(b, hipGetLastError());
+    }
+    return FCPT_OK;
+}
+
+int fcpt_batch_time_kernel(fcpt_batch *b, int32_t kernel, int32_t reps, double *ms_per_launch)
+{
+    if (!b) return FCPT_ERR_ARG;
+    if (reps <= 0) return bfail(b, FCPT_ERR_ARG, "reps must be positive");
+    int kinds[2] = {-1, -1};
+    switch (kernel) {
+    case FCPT_KERNEL_RADIAL_SWEEP: kinds[0] = B_RADIAL_ISO; kinds[1] = B_RADIAL_ADI; break;
+    case FCPT_KERNEL_AZIMUTHAL: kinds[0] = B_AZIM_ISO; kinds[1] = B_AZIM_ADI; break;
+    case FCPT_KERNEL_CFL: kinds[0] = B_CFL_CELLS; break;
+    case FCPT_KERNEL_STRESS: kinds[0] = B_STRESS; break;
+    case FCPT_KERNEL_POTENTIAL: kinds[0] = B_POTENTIAL; break;
+    default: return bfail(b, FCPT_ERR_ARG, "unknown kernel");
+    }
+    BHIP(b, hipSetDevice(b->device));
+    if (!b->ev0) BHIP(b, hipEventCreate(&b->ev0));
+    if (!b->ev1) BHIP(b, hipEventCreate(&b->ev1));
+    auto one = [&]() {
+        for (const BatchStageDev &s : b->stages)
+            if (s.kind == kinds[0] || s.kind == kinds[1]) launch_batch_stage(b->Ps, s, b->split, b->radial_waves, b->stream);
+    };
+    BHIP(b, hipStreamSynchronize(b->stream));
+    one();
+    BHIP(b, hipEventRecord(b->ev0, b->stream));
+    for (int k = 0; k < reps; ++k) one();
+    BHIP(b, hipEventRecord(b->ev1, b->stream));
+    BHIP(b, hipGetLastError());
+    BHIP(b, hipEventSynchronize(b->ev1));
+    float ms = 0.f;
+    BHIP(b, hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    if (ms_per_launch) *ms_per_launch = (double)ms / reps;
+    return FCPT_OK;
+}
+
+int fcpt_batch_time_steps(fcpt_batch *b, int32_t n_steps, int32_t use_graph, double *ms_total)
+{
+    if (!b) return FCPT_ERR_ARG;
+    BHIP(b, hipSetDevice(b->device));
+    if (!b->ev0) BHIP(b, hipEventCreate(&b->ev0));
+    if (!b->ev1) BHIP(b, hipEventCreate(&b->ev1));
+    int rc = fcpt_batch_run_steps(b, 0, use_graph);  // capture outside the timed region
+    if (rc) return rc;
+    BHIP(b, hipStreamSynchronize(b->stream));
+    BHIP(b, hipEventRecord(b->ev0, b->stream));
+    rc = fcpt_batch_run_steps(b, n_steps, use_graph);
+    if (rc) return rc;
+    BHIP(b, hipEventRecord(b->ev1, b->stream));
+    BHIP(b, hipEventSynchronize(b->ev1));
+    float ms = 0.f;
+    BHIP(b, hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    for (fcpt_ctx *c : b->disks) {
+        DevScalars sc;
+        BHIP(b, hipMemcpy(&sc, c->s_dev, sizeof(DevScalars), hipMemcpyDeviceToHost));
+        if (sc.error_flags & 1) return bfail(b, FCPT_ERR_NONFINITE, "a disk's CFL time step became non-finite");
+    }
+    if (ms_total) *ms_total = (double)ms;
     return FCPT_OK;
 }
 

@@ -12,6 +12,7 @@
 // HBM traffic; the arithmetic is FP64 in both instantiations and the FP64
 // instantiation performs exactly the operations of the reference.
 #include <type_traits>
+#include <vector>
 
 #include "fcpt_device.hpp"
 #include "fcpt_kernels.hpp"
@@ -94,17 +95,61 @@ __device__ __forceinline__ Fld<SPLIT> buf(const KParams &P, void *p, const doubl
 // one ring, so the ring index is the same in every lane.  Reading it through
 // readfirstlane tells the compiler so, and the per-ring metric values and
 // backgrounds become scalar loads.
-__device__ __forceinline__ int wave_ring(int r0)
+__device__ __forceinline__ int wave_ring(int r0, int by)
 {
-    return __builtin_amdgcn_readfirstlane(r0 + (int)(blockIdx.y * blockDim.y + threadIdx.y));
+    return __builtin_amdgcn_readfirstlane(r0 + by * (int)blockDim.y + (int)threadIdx.y);
 }
 
-// 2-D launch over rows [r0, r1) x all sectors
+// 2-D launch over rows [r0, r1) x all sectors; (bx, by) is the block's place
+// in the disk's own grid
 #define CELL_LOOP_HEAD(r0, r1)                                   \
     const int ns = P.ns;                                         \
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;         \
-    const int i = wave_ring(r0);                                 \
+    const int j = bx * (int)blockDim.x + (int)threadIdx.x;       \
+    const int i = wave_ring(r0, by);                             \
     if (j >= ns || i >= (r1)) return;
+
+// Every kernel body takes the disk's parameters and its block coordinates
+// (bx, by) in that disk's grid, and comes in two launch forms: k_NAME runs
+// one disk on its own grid; kb_NAME runs many disks in one launch, each
+// block reading (disk, by << 16 | bx) from a per-stage work table and the
+// disk's parameters from a device array (constant address space: scalar
+// loads).
+__device__ __forceinline__ BatchEntry batch_entry(const BatchEntry *map)
+{
+    const auto *m = cst(map) + blockIdx.x;
+    return BatchEntry{m->disk, m->yx};
+}
+__device__ __forceinline__ const KParams &batch_disk(const KParams *Ps, unsigned d)
+{
+    return *(const KParams *)cst(Ps + d);
+}
+
+#define FCPT_PAIR0(name, BOUNDS)                                                                  \
+    template <bool SPLIT>                                                                         \
+    __global__ void __launch_bounds__(BOUNDS) k_##name(KParams P)                                 \
+    {                                                                                             \
+        b_##name<SPLIT>(P, (int)blockIdx.x, (int)blockIdx.y);                                     \
+    }                                                                                             \
+    template <bool SPLIT>                                                                         \
+    __global__ void __launch_bounds__(BOUNDS) kb_##name(const KParams *Ps, const BatchEntry *map)  \
+    {                                                                                             \
+        const BatchEntry e = batch_entry(map);                                                \
+        b_##name<SPLIT>(batch_disk(Ps, e.disk), (int)(e.yx & 0xffffu), (int)(e.yx >> 16));        \
+    }
+
+#define FCPT_PAIR1(name, BOUNDS, T1, a1)                                                          \
+    template <bool SPLIT>                                                                         \
+    __global__ void __launch_bounds__(BOUNDS) k_##name(KParams P, T1 a1)                          \
+    {                                                                                             \
+        b_##name<SPLIT>(P, (int)blockIdx.x, (int)blockIdx.y, a1);                                 \
+    }                                                                                             \
+    template <bool SPLIT>                                                                         \
+    __global__ void __launch_bounds__(BOUNDS)                                                     \
+        kb_##name(const KParams *Ps, const BatchEntry *map, T1 a1)                                \
+    {                                                                                             \
+        const BatchEntry e = batch_entry(map);                                                \
+        b_##name<SPLIT>(batch_disk(Ps, e.disk), (int)(e.yx & 0xffffu), (int)(e.yx >> 16), a1);    \
+    }
 
 // ---------------------------------------------------------------------------
 // ring sums (cfl.cpp:196-205, TransportEuler.cpp:174-190): one block per ring,
@@ -127,11 +172,11 @@ __device__ double block_sum(double v, double *sh)
 // mode 1: transport ring mean + FARGO shift + uniform residual
 //         (TransportEuler.cpp:207-236)
 template <bool SPLIT>
-__global__ void __launch_bounds__(256) k_ring_mean(KParams P, int mode)
+__device__ __forceinline__ void b_ring_mean(const KParams &P, int bx, int by, int mode)
 {
     __shared__ double sh[256];
     const int ns = P.ns;
-    const int i = blockIdx.x;
+    const int i = bx;
     FLD(v, FCPT_VAZI);
     double acc = 0.0;
     for (int j = threadIdx.x; j < ns; j += 256) acc += v(i, j);
@@ -149,16 +194,17 @@ __global__ void __launch_bounds__(256) k_ring_mean(KParams P, int mode)
         }
     }
 }
+FCPT_PAIR1(ring_mean, 256, int, mode)
 
 // cfl.cpp:210-376, per cell dt and the per-ring shear limit.  Each thread
 // walks rows with a grid stride; the block minimum (wavefront shuffles, then
 // LDS) goes to its own slot of P.ring_min and k_cfl_finalize folds the slots,
 // so the result is deterministic and free of atomics.
 template <bool SPLIT>
-__global__ void __launch_bounds__(256) k_cfl_cells(KParams P)
+__device__ __forceinline__ void b_cfl_cells(const KParams &P, int bx, int by)
 {
     const int ns = P.ns;
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = bx * blockDim.x + threadIdx.x;
     double cand = __longlong_as_double(0x7FF0000000000000ll);
     FLD(vr, FCPT_VRAD);
     FLD(vaz, FCPT_VAZI);
@@ -167,7 +213,7 @@ __global__ void __launch_bounds__(256) k_cfl_cells(KParams P)
     const double CFL = P.cfl, dphi = P.dphi;
     const auto vm = cst(P.vmean);
     const double fac2 = P.artvisc_factor * P.artvisc_factor;
-    for (int i = wave_ring(1); i < P.nr - 1 && j < ns; i += gridDim.y * blockDim.y) {
+    for (int i = wave_ring(1, by); i < P.nr - 1 && j < ns; i += P.cfl_gy * (int)blockDim.y) {
         if (j == 0) {
             const double denom =
                 fabs(vm[i] * cst(P.InvRmed)[i] - vm[i + 1] * cst(P.InvRmed)[i + 1]) + 1.0e-100;
@@ -224,14 +270,16 @@ __global__ void __launch_bounds__(256) k_cfl_cells(KParams P)
     __syncthreads();
     if (t == 0) {
         const double m = std_min(std_min(wmin[0], wmin[1]), std_min(wmin[2], wmin[3]));
-        P.ring_min[blockIdx.y * gridDim.x + blockIdx.x] = m;
+        P.ring_min[by * P.cfl_gx + bx] = m;
     }
 }
+FCPT_PAIR0(cfl_cells, 256)
 
 // mode 0: condition_cfl only; 1: sim::CalculateTimeStep (simulation.cpp:100-118);
 // 2: CalculateTimeStep + the monitor clip of sim::run (simulation.cpp:526-542)
-__global__ void __launch_bounds__(256) k_cfl_finalize(KParams P, int mode, int nblocks)
+__device__ __forceinline__ void b_cfl_finalize(const KParams &P, int mode)
 {
+    const int nblocks = P.cfl_gx * P.cfl_gy;
     __shared__ double sh[256];
     double m = __longlong_as_double(0x7FF0000000000000ll);
     for (int k = threadIdx.x; k < nblocks; k += 256) m = std_min(m, P.ring_min[k]);
@@ -259,6 +307,11 @@ __global__ void __launch_bounds__(256) k_cfl_finalize(KParams P, int mode, int n
             s->dt = (overshoot || almost_there) ? time_left_till_write : rv;
         }
     }
+}
+__global__ void __launch_bounds__(256) k_cfl_finalize(KParams P, int mode) { b_cfl_finalize(P, mode); }
+__global__ void __launch_bounds__(256) kb_cfl_finalize(const KParams *Ps, const BatchEntry *map, int mode)
+{
+    b_cfl_finalize(batch_disk(Ps, batch_entry(map).disk), mode);
 }
 
 __global__ void k_set_dt(KParams P, double dt) { P.s->dt = dt; }
@@ -290,9 +343,9 @@ __global__ void k_set_params(KParams P, fcpt_step_params sp)
 // every step.  Every body is seen from distance sqrt(R^2 + |r_k|^2) with the
 // ring's background smoothing, which keeps the stored deviations small away
 // from the bodies.
-__global__ void __launch_bounds__(256) k_potential_bg(KParams P)
+__device__ __forceinline__ void b_potential_bg(const KParams &P, int bx)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = bx * (int)blockDim.x + (int)threadIdx.x;
     if (i >= P.nr) return;
     const DevScalars *s = P.s;
     const double R = cst(P.Rmed)[i];
@@ -304,10 +357,16 @@ __global__ void __launch_bounds__(256) k_potential_bg(KParams P)
     }
     P.bg[FCPT_POTENTIAL][i] = b;
 }
+__global__ void __launch_bounds__(256) k_potential_bg(KParams P) { b_potential_bg(P, (int)blockIdx.x); }
+__global__ void __launch_bounds__(256) kb_potential_bg(const KParams *Ps, const BatchEntry *map)
+{
+    const BatchEntry e = batch_entry(map);
+    b_potential_bg(batch_disk(Ps, e.disk), (int)(e.yx & 0xffffu));
+}
 
 // CalculateNbodyPotential, Pframeforce.cpp:44-85 (+ compute_smoothing, Force.cpp:145-159)
 template <bool SPLIT>
-__global__ void __launch_bounds__(256) k_potential(KParams P)
+__device__ __forceinline__ void b_potential(const KParams &P, int bx, int by)
 {
     CELL_LOOP_HEAD(0, P.nr)
     FLD(Hf, FCPT_SCALE_HEIGHT);
@@ -334,11 +393,12 @@ __global__ void __launch_bounds__(256) k_potential(KParams P)
     acc += -s->indirect_x * x - s->indirect_y * y;
     fld<SPLIT>(P, FCPT_POTENTIAL).set(i, j, acc);
 }
+FCPT_PAIR0(potential, 256)
 
 // ---------------------------------------------------------------------------
 // momentum_update_radial, SourceEuler.cpp:325-372
 template <bool SPLIT>
-__global__ void __launch_bounds__(256) k_src_vr(KParams P)
+__device__ __forceinline__ void b_src_vr(const KParams &P, int bx, int by)
 {
     CELL_LOOP_HEAD(2, P.nr - 1)
     FLD(S, FCPT_SIGMA);
@@ -358,10 +418,11 @@ __global__ void __launch_bounds__(256) k_src_vr(KParams P)
     const double centrifugal_accel = vt2 * cst(P.InvRinf)[i];
     vr.set(i, j, vr(i, j) + dt * (-gradp - gradphi + centrifugal_accel));
 }
+FCPT_PAIR0(src_vr, 256)
 
 // momentum_update_azimuthal, SourceEuler.cpp:375-428
 template <bool SPLIT>
-__global__ void __launch_bounds__(256) k_src_vphi(KParams P)
+__device__ __forceinline__ void b_src_vphi(const KParams &P, int bx, int by)
 {
     CELL_LOOP_HEAD(1, P.nr - 1)
     FLD(S, FCPT_SIGMA);
@@ -375,10 +436,11 @@ __global__ void __launch_bounds__(256) k_src_vphi(KParams P)
     const double gradphi = (Phi(i, j) - Phi(i, jp)) * invdxtheta;
     vaz.set(i, j, vaz(i, j) + dt * (-gradp - gradphi));
 }
+FCPT_PAIR0(src_vphi, 256)
 
 // compression_heating, SourceEuler.cpp:459-493
 template <bool SPLIT>
-__global__ void __launch_bounds__(256) k_src_energy(KParams P)
+__device__ __forceinline__ void b_src_energy(const KParams &P, int bx, int by)
 {
     CELL_LOOP_HEAD(0, P.nr - 1)
     FLD(vr, FCPT_VRAD);
@@ -391,6 +453,7 @@ __global__ void __launch_bounds__(256) k_src_energy(KParams P)
     const double energy_old = E(i, j);
     E.set(i, j, energy_old * exp(-(P.gamma - 1.0) * dt * DIV_V));
 }
+FCPT_PAIR0(src_energy, 256)
 
 // assure_temperature_range for one cell, SourceEuler.cpp:136-202
 __device__ __forceinline__ double temperature_floor(const KParams &P, double e, double sigma)
@@ -403,13 +466,14 @@ __device__ __forceinline__ double temperature_floor(const KParams &P, double e, 
 }
 
 template <bool SPLIT>
-__global__ void __launch_bounds__(256) k_temperature_floor(KParams P)
+__device__ __forceinline__ void b_temperature_floor(const KParams &P, int bx, int by)
 {
     CELL_LOOP_HEAD(0, P.nr)
     FLD(E, FCPT_ENERGY);
     FLD(S, FCPT_SIGMA);
     E.set(i, j, temperature_floor(P, E(i, j), S(i, j)));
 }
+FCPT_PAIR0(temperature_floor, 256)
 
 // ---------------------------------------------------------------------------
 // compute_sound_speed / compute_scale_height / update_viscosity for one cell
@@ -465,7 +529,7 @@ __device__ __forceinline__ void derived_store(const KParams &P, int i, int j, co
 // ---------------------------------------------------------------------------
 // update_with_artificial_viscosity_TW, artificial_viscosity.cpp:35-88
 template <bool SPLIT>
-__global__ void __launch_bounds__(256) k_tw_q(KParams P)
+__device__ __forceinline__ void b_tw_q(const KParams &P, int bx, int by)
 {
     CELL_LOOP_HEAD(0, P.nr)
     FLD(vr, FCPT_VRAD);
@@ -500,10 +564,11 @@ __global__ void __launch_bounds__(256) k_tw_q(KParams P)
         E.set(i, j, temperature_floor(P, e, sig));
     }
 }
+FCPT_PAIR0(tw_q, 256)
 
 // artificial_viscosity.cpp:90-139
 template <bool SPLIT>
-__global__ void __launch_bounds__(256) k_tw_vel(KParams P, int with_viscosity)
+__device__ __forceinline__ void b_tw_vel(const KParams &P, int bx, int by, int with_viscosity)
 {
     CELL_LOOP_HEAD(0, P.nr)
     FLD(S, FCPT_SIGMA);
@@ -549,10 +614,11 @@ __global__ void __launch_bounds__(256) k_tw_vel(KParams P, int with_viscosity)
         vr.set(i, j, v + dVr);
     }
 }
+FCPT_PAIR1(tw_vel, 256, int, with_viscosity)
 
 // update_with_artificial_viscosity_SN, artificial_viscosity.cpp:148-211
 template <bool SPLIT>
-__global__ void __launch_bounds__(256) k_sn_q(KParams P)
+__device__ __forceinline__ void b_sn_q(const KParams &P, int bx, int by)
 {
     CELL_LOOP_HEAD(0, P.nr)
     FLD(vr, FCPT_VRAD);
@@ -582,10 +648,11 @@ __global__ void __launch_bounds__(256) k_sn_q(KParams P)
         E.set(i, j, temperature_floor(P, e, sig));
     }
 }
+FCPT_PAIR0(sn_q, 256)
 
 // artificial_viscosity.cpp:214-248
 template <bool SPLIT>
-__global__ void __launch_bounds__(256) k_sn_vel(KParams P, int with_viscosity)
+__device__ __forceinline__ void b_sn_vel(const KParams &P, int bx, int by, int with_viscosity)
 {
     CELL_LOOP_HEAD(0, P.nr)
     FLD(S, FCPT_SIGMA);
@@ -624,27 +691,30 @@ __global__ void __launch_bounds__(256) k_sn_vel(KParams P, int with_viscosity)
     const double invdxtheta = 1.0 / dxtheta;
     vaz.set(i, j, va - dt * 2.0 / (s0 + sjp) * (qp0 - qpj) * invdxtheta);
 }
+FCPT_PAIR1(sn_vel, 256, int, with_viscosity)
 
 // recalculate_viscosity, SourceEuler.cpp:205-223
 template <bool SPLIT>
-__global__ void __launch_bounds__(256) k_recalc_viscosity(KParams P)
+__device__ __forceinline__ void b_recalc_viscosity(const KParams &P, int bx, int by)
 {
     CELL_LOOP_HEAD(0, P.nr)
     derived_store<SPLIT>(P, i, j, derived_load<SPLIT>(P, i, j, false), false, false);
 }
+FCPT_PAIR0(recalc_viscosity, 256)
 
 // recalculate_derived_disk_quantities, SourceEuler.cpp:225-249
 template <bool SPLIT>
-__global__ void __launch_bounds__(256) k_derived(KParams P)
+__device__ __forceinline__ void b_derived(const KParams &P, int bx, int by)
 {
     CELL_LOOP_HEAD(0, P.nr)
     derived_store<SPLIT>(P, i, j, derived_load<SPLIT>(P, i, j, true), true, true);
 }
+FCPT_PAIR0(derived, 256)
 
 // locally isothermal sound speed and scale height from the initial setup
 // (compute_sound_speed_normal / compute_scale_height_old, isothermal branches)
 template <bool SPLIT>
-__global__ void __launch_bounds__(256) k_iso_profile(KParams P)
+__device__ __forceinline__ void b_iso_profile(const KParams &P, int bx, int by)
 {
     CELL_LOOP_HEAD(0, P.nr)
     const double r = cst(P.Rmed)[i];
@@ -659,11 +729,12 @@ __global__ void __launch_bounds__(256) k_iso_profile(KParams P)
     fld<SPLIT>(P, FCPT_PRESSURE).set(i, j, pr);
     fld<SPLIT>(P, FCPT_TEMPERATURE).set(i, j, P.mu / P.Rgas * pr / sig);
 }
+FCPT_PAIR0(iso_profile, 256)
 
 // ---------------------------------------------------------------------------
 // compute_viscous_stress_tensor, viscosity.cpp:139-254
 template <bool SPLIT>
-__global__ void __launch_bounds__(256) k_stress(KParams P)
+__device__ __forceinline__ void b_stress(const KParams &P, int bx, int by)
 {
     CELL_LOOP_HEAD(0, P.nr)
     FLD(vr, FCPT_VRAD);
@@ -694,10 +765,11 @@ __global__ void __launch_bounds__(256) k_stress(KParams P)
         fld<SPLIT>(P, FCPT_TAU_R_PHI).set(i, j, n4 * s4 * drp);
     }
 }
+FCPT_PAIR0(stress, 256)
 
 // update_velocities_with_viscosity, viscosity.cpp:355-426
 template <bool SPLIT>
-__global__ void __launch_bounds__(256) k_visc_update(KParams P)
+__device__ __forceinline__ void b_visc_update(const KParams &P, int bx, int by)
 {
     CELL_LOOP_HEAD(1, P.nr - 1)
     FLD(S, FCPT_SIGMA);
@@ -730,6 +802,7 @@ __global__ void __launch_bounds__(256) k_visc_update(KParams P)
         vr.set(i, j, v + dVr);
     }
 }
+FCPT_PAIR0(visc_update, 256)
 
 // ---------------------------------------------------------------------------
 // Q- (thermal_relaxation, SourceEuler.cpp:632-690) and Q+ (viscous_heating,
@@ -783,7 +856,7 @@ __device__ __forceinline__ double rad_alpha(const KParams &P, double h, double s
 
 // SubStep3, SourceEuler.cpp:859-954
 template <bool SPLIT>
-__global__ void __launch_bounds__(256) k_substep3(KParams P)
+__device__ __forceinline__ void b_substep3(const KParams &P, int bx, int by)
 {
     CELL_LOOP_HEAD(0, P.nr)
     FLD(E, FCPT_ENERGY);
@@ -818,11 +891,12 @@ __global__ void __launch_bounds__(256) k_substep3(KParams P)
     fld<SPLIT>(P, FCPT_QMINUS).set(i, j, qminus);
     E.set(i, j, temperature_floor(P, e, sig));
 }
+FCPT_PAIR0(substep3, 256)
 
 // compute_heating_cooling_for_CFL, SourceEuler.cpp:1410-1450 (after update_viscosity
 // and the stress tensor)
 template <bool SPLIT>
-__global__ void __launch_bounds__(256) k_heatcool_cfl(KParams P)
+__device__ __forceinline__ void b_heatcool_cfl(const KParams &P, int bx, int by)
 {
     CELL_LOOP_HEAD(0, P.nr)
     double qplus = 0.0, qminus = 0.0;
@@ -837,16 +911,18 @@ __global__ void __launch_bounds__(256) k_heatcool_cfl(KParams P)
     fld<SPLIT>(P, FCPT_QPLUS).set(i, j, qplus);
     fld<SPLIT>(P, FCPT_QMINUS).set(i, j, qminus);
 }
+FCPT_PAIR0(heatcool_cfl, 256)
 
 // ---------------------------------------------------------------------------
 // damping (boundary_conditions/damping.cpp:311-427, 754-776): block row y maps
 // to an inner-zone ring (y < n_inner) or an outer-zone ring
 template <bool SPLIT>
-__global__ void __launch_bounds__(256) k_damping(KParams P, int n_inner)
+__device__ __forceinline__ void b_damping(const KParams &P, int bx, int by)
 {
+    const int n_inner = (P.damp_ids[0] > P.damp_ids[1] ? P.damp_ids[0] : P.damp_ids[1]) + 1;
     const int ns = P.ns;
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = wave_ring(0);
+    const int j = bx * blockDim.x + threadIdx.x;
+    const int y = wave_ring(0, by);
     if (j >= ns) return;
     const double dt = P.s->dt;
     const double lin = P.damping_inner_limit, lout = P.damping_outer_limit;
@@ -894,14 +970,15 @@ __global__ void __launch_bounds__(256) k_damping(KParams P, int n_inner)
         }
     }
 }
+FCPT_PAIR0(damping, 256)
 
 // ghost rings: zero-gradient Sigma/E, reflecting / zero-gradient / outflow v_r,
 // Keplerian v_phi (boundary_conditions.cpp:67-97 and the BC files it dispatches to)
 template <bool SPLIT>
-__global__ void __launch_bounds__(256) k_ghost_bc(KParams P, int advance_time)
+__device__ __forceinline__ void b_ghost_bc(const KParams &P, int bx, int by, int advance_time)
 {
     const int ns = P.ns;
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = bx * blockDim.x + threadIdx.x;
     if (advance_time && j == 0) {
         // end of a sim::run iteration: time += dt and the monitor counter
         // (simulation.cpp:244, 545-551); nothing else in this kernel reads them
@@ -958,6 +1035,7 @@ __global__ void __launch_bounds__(256) k_ghost_bc(KParams P, int advance_time)
     vaz.set(0, j, 1.0 * vk_in - cst(P.Rmed)[0] * P.omega_frame);
     vaz.set(nr - 1, j, 1.0 * vk_out - cst(P.Rmed)[nr - 1] * P.omega_frame);
 }
+FCPT_PAIR1(ghost_bc, 256, int, advance_time)
 
 // ---------------------------------------------------------------------------
 // Transport
@@ -1092,12 +1170,12 @@ __device__ __forceinline__ void interface_flux(const KParams &P, int k, const Ri
 // select, so the loop body is one branch-free block the compiler can schedule
 // as a whole.
 template <bool SPLIT, bool ADI>
-__device__ __forceinline__ void radial_sweep(const KParams &P, int tile)
+__device__ __forceinline__ void radial_sweep(const KParams &P, int bx, int by, int tile)
 {
     constexpr int NQ = ADI ? 5 : 4;
     const int ns = P.ns, nr = P.nr;
-    const int j = blockIdx.x * 64 + threadIdx.x;
-    const int i0 = blockIdx.y * tile;
+    const int j = bx * 64 + (int)threadIdx.x;
+    const int i0 = by * tile;
     if (j >= ns || i0 >= nr) return;
     const int jn = j == ns - 1 ? 0 : j + 1;
     const double dt = P.s->dt;
@@ -1148,72 +1226,79 @@ __device__ __forceinline__ void radial_sweep(const KParams &P, int tile)
 template <bool SPLIT, bool ADI>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_radial_sweep2(KParams P, int tile)
 {
-    radial_sweep<SPLIT, ADI>(P, tile);
+    radial_sweep<SPLIT, ADI>(P, (int)blockIdx.x, (int)blockIdx.y, tile);
 }
 template <bool SPLIT, bool ADI>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_radial_sweep3(KParams P, int tile)
 {
-    radial_sweep<SPLIT, ADI>(P, tile);
+    radial_sweep<SPLIT, ADI>(P, (int)blockIdx.x, (int)blockIdx.y, tile);
+}
+// batched forms: each disk's tile is its resolved radial_tile
+template <bool SPLIT, bool ADI>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
+    kb_radial_sweep2(const KParams *Ps, const BatchEntry *map)
+{
+    const BatchEntry e = batch_entry(map);
+    const KParams &P = batch_disk(Ps, e.disk);
+    radial_sweep<SPLIT, ADI>(P, (int)(e.yx & 0xffffu), (int)(e.yx >> 16), P.radial_tile);
+}
+template <bool SPLIT, bool ADI>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3)))
+    kb_radial_sweep3(const KParams *Ps, const BatchEntry *map)
+{
+    const BatchEntry e = batch_entry(map);
+    const KParams &P = batch_disk(Ps, e.disk);
+    radial_sweep<SPLIT, ADI>(P, (int)(e.yx & 0xffffu), (int)(e.yx >> 16), P.radial_tile);
 }
 
-// ComputeStarTheta for one interface j (TransportEuler.cpp:416-465);
-// q[0..3] = sectors j-2..j+1
-__device__ __forceinline__ double star_theta(const double *q, double dxtheta, double invdxtheta,
-                                             double ksi)
+// Neighbour values within a wavefront: the value of lane - 1 and of lane + 1
+// (the edge lanes get their own value back; results there are discarded)
+__device__ __forceinline__ double from_left(double x) { return __shfl_up(x, 1, 64); }
+__device__ __forceinline__ double from_right(double x) { return __shfl_down(x, 1, 64); }
+
+// van Leer slope of a sector from its two neighbours (the slope half of
+// ComputeStarTheta, TransportEuler.cpp:416-465)
+__device__ __forceinline__ double slope_theta(double q, double invdxtheta)
 {
-    if (ksi > 0.0) {
-        const double dq = 0.5 * van_leer_lim(q[2] - q[1], q[1] - q[0]) * invdxtheta;
-        return q[1] + (dxtheta - ksi) * dq;
-    }
-    const double dq = 0.5 * van_leer_lim(q[3] - q[2], q[2] - q[1]) * invdxtheta;
-    return q[2] - (dxtheta + ksi) * dq;
+    return 0.5 * van_leer_lim(from_right(q) - q, q - from_left(q)) * invdxtheta;
 }
 
-// van Leer slope of one sector from its two neighbours (the first half of
-// ComputeStarTheta, TransportEuler.cpp:416-465); q points at sector j
-__device__ __forceinline__ double slope_theta(const double *q, double invdxtheta)
+// upwind interface value between sector j-1 and sector j (the second half of
+// ComputeStarTheta): q, dq of sector j in this lane, of sector j-1 in the lane
+// to the left
+__device__ __forceinline__ double star_theta(double q, double dq, double dxtheta, double ksi)
 {
-    return 0.5 * van_leer_lim(q[1] - q[0], q[0] - q[-1]) * invdxtheta;
-}
-
-// upwind interface value between sectors j-1 and j from the two sides'
-// slopes (the second half of ComputeStarTheta); q and d point at sector j
-__device__ __forceinline__ double star_theta_sel(const double *q, const double *d, double dxtheta,
-                                                 double ksi)
-{
-    const double lo = q[-1] + (dxtheta - ksi) * d[-1];
-    const double hi = q[0] - (dxtheta + ksi) * d[0];
+    const double lo = from_left(q) + (dxtheta - ksi) * from_left(dq);
+    const double hi = q - (dxtheta + ksi) * dq;
     return (ksi > 0.0) ? lo : hi;
 }
 
 // OneWindTheta, TransportEuler.cpp:270-289, in one pass per ring segment:
 // QuantitiesAdvection (:292-304, VanLeerTheta :630-664) with the residual
 // velocity v - <v>, then with the uniform residual of the ring, then
-// AdvectSHIFT (:238-268).  A block owns kAzSeg sectors of one ring and stages
-// them with a halo of 4 sectors in LDS (2 per van Leer pass), one position per
-// thread; the result of sector j is written to sector j + Nshift.  The density
-// is advanced first because the second pass needs its first-pass value as
-// DENSITY_INT; the other quantities only read the density, so their order does
-// not matter.  The density's own specific value is exactly 1 (positive Sigma),
-// so its interface values are 1 and drop out of its flux.  In the first pass
-// the velocity changes sign along the ring, so every sector's slope is formed
-// once and each interface selects its upwind side; in the second pass the
-// velocity is uniform and each interface forms the one slope it needs.
+// AdvectSHIFT (:238-268).  Each wavefront owns kAzSegW consecutive sectors of
+// one ring and loads them with kAzHalo sectors on either side, one sector per
+// lane; neighbours come from the adjacent lanes, so the kernel needs neither
+// LDS nor barriers.  Each stencil step spoils one more lane at either edge;
+// after the two van Leer passes lanes kAzHalo .. 63 - kAzHalo hold exact
+// results, and the result of sector j is written to sector j + Nshift.  The
+// density's own specific value is exactly 1 (positive Sigma), so its interface
+// values are 1 and drop out of its flux.  Every slope is formed once per
+// sector and each interface selects its upwind side.
 constexpr int kAzHalo = 4;
-constexpr int kAzThreads = 256;
-constexpr int kAzSeg = kAzThreads - 2 * kAzHalo;
+constexpr int kAzSegW = 64 - 2 * kAzHalo;  // sectors written per wavefront
+constexpr int kAzWaves = 4;                // wavefronts per block
 
-template <bool SPLIT>
-__global__ void __launch_bounds__(kAzThreads) k_azimuthal_sweep(KParams P)
+template <bool SPLIT, bool ADI>
+__device__ __forceinline__ void b_azimuthal_sweep(const KParams &P, int bx, int by)
 {
-    __shared__ double sS0[kAzThreads], sV0[kAzThreads], sDS0[kAzThreads], sS1[kAzThreads],
-        sDS1[kAzThreads], sW[kAzThreads], sD[kAzThreads], sQS[kAzThreads];
+    constexpr int NQ = ADI ? 5 : 4;
     const int ns = P.ns;
-    const int i = blockIdx.y;
-    const int j0 = blockIdx.x * kAzSeg;
-    const int Ls = ns - j0 < kAzSeg ? ns - j0 : kAzSeg;
-    const int next = Ls + 2 * kAzHalo;  // staged positions p: sector j0 - 4 + p
-    const int p = threadIdx.x;
+    const int i = by;
+    const int lane = threadIdx.x & 63;
+    const int j0 = __builtin_amdgcn_readfirstlane(bx * kAzWaves + (int)(threadIdx.x >> 6)) * kAzSegW;
+    if (j0 >= ns) return;  // the whole wavefront
+    const int Ls = ns - j0 < kAzSegW ? ns - j0 : kAzSegW;
     const double dt = P.s->dt;
     const double vm = cst(P.vmean)[i];
     const double vu = cst(P.vuni)[i];
@@ -1223,93 +1308,69 @@ __global__ void __launch_bounds__(kAzThreads) k_azimuthal_sweep(KParams P)
     const double invdxtheta = 1.0 / dxtheta;
     const double dxrad = (cst(P.Rsup)[i] - cst(P.Rinf)[i]) * dt;
     const double invsurf = cst(P.InvSurf)[i];
-    int js = j0 - kAzHalo + p;  // this thread's sector
+    int js = (j0 - kAzHalo + lane) % ns;  // this lane's sector
     if (js < 0) js += ns;
-    if (js >= ns) js -= ns;
-    int jd = j0 - kAzHalo + p + nsh;  // where its result goes
-    while (jd < 0) jd += ns;
-    while (jd >= ns) jd -= ns;
-    const bool staged = p < next;
-    const bool slope1 = p >= 1 && p < next - 1;     // first-pass slopes
-    const bool face1 = p >= 2 && p <= next - 2;     // first-pass interfaces
-    const bool cell1 = p >= 2 && p < next - 2;      // first-pass updates
-    const bool face2 = p >= kAzHalo && p <= kAzHalo + Ls;  // second-pass interfaces
-    const bool cell2 = p >= kAzHalo && p < kAzHalo + Ls;   // second-pass updates (written)
+    int jd = (j0 - kAzHalo + lane + nsh) % ns;  // where its result goes
+    if (jd < 0) jd += ns;
+    const bool write = lane >= kAzHalo && lane < kAzHalo + Ls;
+    // every global load first
     FLD(vaz, FCPT_VAZI);
-    const int nq = P.adiabatic ? 5 : 4;
-    // every global load of the block is issued here, ahead of the first store
-    double s0 = 1.0, v0 = 0.0, q0v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    if (staged) {
-        s0 = buf<false>(P, P.sig_b, nullptr)(i, js);
-        v0 = vaz(i, js) - vm;
+    const double s0 = buf<false>(P, P.sig_b, nullptr)(i, js);
+    const double v0 = vaz(i, js) - vm;
+    double q0[NQ];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) q0v[q] = buf<SPLIT>(P, P.mom_b[q], P.bgm[q])(i, js);
-        if (nq == 5) q0v[4] = buf<false>(P, P.en_b, nullptr)(i, js);
-        sS0[p] = s0;
-        sV0[p] = v0;
-    }
+    for (int q = 0; q < 4; ++q) q0[q] = buf<SPLIT>(P, P.mom_b[q], P.bgm[q])(i, js);
+    if constexpr (ADI) q0[4] = buf<false>(P, P.en_b, nullptr)(i, js);
     const double ks0 = v0 * dt;
-    __syncthreads();
-    // density, first pass
-    if (slope1) sD[p] = slope_theta(&sS0[p], invdxtheta);
-    __syncthreads();
-    if (face1) sDS0[p] = star_theta_sel(&sS0[p], &sD[p], dxtheta, ks0);
-    __syncthreads();
-    double s1 = 0.0;
-    if (cell1) {
-        double varq = dxrad * sDS0[p] * v0;
-        varq -= dxrad * sDS0[p + 1] * sV0[p + 1];
-        s1 = s0 + varq * invsurf;
-        sS1[p] = s1;
-    }
-    __syncthreads();
-    // density, second pass
-    if (face2) sDS1[p] = star_theta(&sS1[p - 2], dxtheta, invdxtheta, ksu);
-    __syncthreads();
-    if (cell2) {
-        double varq = dxrad * sDS1[p] * vu;
-        varq -= dxrad * sDS1[p + 1] * vu;
-        buf<false>(P, P.sig_c, nullptr).set(i, jd, s1 + varq * invsurf);
-    }
+    // density: first pass (residual velocity), then second pass (uniform residual)
+    const double ds0 = star_theta(s0, slope_theta(s0, invdxtheta), dxtheta, ks0);
+    const double fd0 = dxrad * ds0 * v0;
+    double varq = fd0;
+    varq -= from_right(fd0);
+    const double s1 = s0 + varq * invsurf;
+    const double ds1 = star_theta(s1, slope_theta(s1, invdxtheta), dxtheta, ksu);
+    const double fd1 = dxrad * ds1 * vu;
+    varq = fd1;
+    varq -= from_right(fd1);
+    if (write) buf<false>(P, P.sig_c, nullptr).set(i, jd, s1 + varq * invsurf);
     // the other quantities
 #pragma unroll
-    for (int q = 0; q < 5; ++q) {
-        if (q >= nq) break;
-        const double q0 = q0v[q];
-        if (staged) sW[p] = q0 / s0;
-        __syncthreads();
-        if (slope1) sD[p] = slope_theta(&sW[p], invdxtheta);
-        __syncthreads();
-        if (face1) sQS[p] = star_theta_sel(&sW[p], &sD[p], dxtheta, ks0);
-        __syncthreads();
-        // sW is free again: its last readers (the slopes) are two barriers back
-        double q1 = 0.0;
-        if (cell1) {
-            double varq = dxrad * sQS[p] * sDS0[p] * v0;
-            varq -= dxrad * sQS[p + 1] * sDS0[p + 1] * sV0[p + 1];
-            q1 = q0 + varq * invsurf;
-            sW[p] = q1 / s1;
+    for (int q = 0; q < NQ; ++q) {
+        const double w0 = q0[q] / s0;
+        const double qs0 = star_theta(w0, slope_theta(w0, invdxtheta), dxtheta, ks0);
+        const double f0 = dxrad * qs0 * ds0 * v0;
+        double vq = f0;
+        vq -= from_right(f0);
+        const double q1 = q0[q] + vq * invsurf;
+        const double w1 = q1 / s1;
+        const double qs1 = star_theta(w1, slope_theta(w1, invdxtheta), dxtheta, ksu);
+        const double f1 = dxrad * qs1 * ds1 * vu;
+        vq = f1;
+        vq -= from_right(f1);
+        const double q2 = q1 + vq * invsurf;
+        if (write) {
+            if (q < 4) buf<SPLIT>(P, P.mom[q], P.bgm[q]).set(i, jd, q2);
+            else buf<false>(P, P.en_c, nullptr).set(i, jd, q2);
         }
-        __syncthreads();
-        if (face2) sQS[p] = star_theta(&sW[p - 2], dxtheta, invdxtheta, ksu);
-        __syncthreads();
-        if (cell2) {
-            double varq = dxrad * sQS[p] * sDS1[p] * vu;
-            varq -= dxrad * sQS[p + 1] * sDS1[p + 1] * vu;
-            const double val = q1 + varq * invsurf;
-            if (q < 4) buf<SPLIT>(P, P.mom[q], P.bgm[q]).set(i, jd, val);
-            else buf<false>(P, P.en_c, nullptr).set(i, jd, val);
-        }
-        // the next quantity rewrites sW and sD first; sQS, read just above,
-        // is rewritten only after two more barriers
     }
+}
+template <bool SPLIT, bool ADI>
+__global__ void __launch_bounds__(64 * kAzWaves) k_azimuthal_sweep(KParams P)
+{
+    b_azimuthal_sweep<SPLIT, ADI>(P, (int)blockIdx.x, (int)blockIdx.y);
+}
+template <bool SPLIT, bool ADI>
+__global__ void __launch_bounds__(64 * kAzWaves) kb_azimuthal_sweep(const KParams *Ps, const BatchEntry *map)
+{
+    const BatchEntry e = batch_entry(map);
+    b_azimuthal_sweep<SPLIT, ADI>(batch_disk(Ps, e.disk), (int)(e.yx & 0xffffu), (int)(e.yx >> 16));
 }
 
 // compute_velocities_from_momenta (TransportEuler.cpp:498-535), then
 // assure_minimum_value(Sigma) (SourceEuler.cpp:102-131) and, adiabatic,
 // SetTemperatureFloorCeilValues (TransportEuler.cpp:127-131)
 template <bool SPLIT>
-__global__ void __launch_bounds__(256) k_velocities(KParams P)
+__device__ __forceinline__ void b_velocities(const KParams &P, int bx, int by)
 {
     CELL_LOOP_HEAD(0, P.nr)
     const Fld<false> Sb = buf<false>(P, P.sig_c, nullptr);
@@ -1337,6 +1398,7 @@ __global__ void __launch_bounds__(256) k_velocities(KParams P)
     fld<SPLIT>(P, FCPT_SIGMA).set(i, j, sig);
     if (P.adiabatic) fld<SPLIT>(P, FCPT_ENERGY).set(i, j, temperature_floor(P, e, sig));
 }
+FCPT_PAIR0(velocities, 256)
 
 // ---------------------------------------------------------------------------
 // totals with Neumaier compensation: per ring (sum, compensation) pairs, then
@@ -1458,15 +1520,25 @@ static dim3 cfl_grid(const KParams &P)
     return dim3(gx, gy);
 }
 
+void init_launch_fields(KParams &P)
+{
+    const dim3 g = cfl_grid(P);
+    P.cfl_gx = (int)g.x;
+    P.cfl_gy = (int)g.y;
+}
+
 void launch_cfl(const KParams &P, hipStream_t st, int mode)
 {
     launch_ring_mean(P, st, 0);
-    const dim3 g = cfl_grid(P);
+    const dim3 g((unsigned)P.cfl_gx, (unsigned)P.cfl_gy);
     LAUNCH(k_cfl_cells, g, kBlock, st, P);
-    hipLaunchKernelGGL(k_cfl_finalize, dim3(1), dim3(256), 0, st, P, mode, (int)(g.x * g.y));
+    hipLaunchKernelGGL(k_cfl_finalize, dim3(1), dim3(256), 0, st, P, mode);
 }
 
-void launch_cfl_cells(const KParams &P, hipStream_t st) { LAUNCH(k_cfl_cells, cfl_grid(P), kBlock, st, P); }
+void launch_cfl_cells(const KParams &P, hipStream_t st)
+{
+    LAUNCH(k_cfl_cells, dim3((unsigned)P.cfl_gx, (unsigned)P.cfl_gy), kBlock, st, P);
+}
 
 void launch_set_dt(const KParams &P, hipStream_t st, double dt)
 {
@@ -1536,7 +1608,7 @@ void launch_boundaries(const KParams &P, hipStream_t st, int final_call, int adv
         const int n_inner = (P.damp_ids[0] > P.damp_ids[1] ? P.damp_ids[0] : P.damp_ids[1]) + 1;
         const int o0 = P.damp_ids[2] < P.damp_ids[3] ? P.damp_ids[2] : P.damp_ids[3];
         const int n_outer = P.nr + 1 - o0;
-        LAUNCH(k_damping, cell_grid(P, n_inner + n_outer), kBlock, st, P, n_inner);
+        LAUNCH(k_damping, cell_grid(P, n_inner + n_outer), kBlock, st, P);
     }
     LAUNCH(k_ghost_bc, dim3((unsigned)((P.ns + 255) / 256)), dim3(256), st, P, advance_time);
 }
@@ -1580,8 +1652,16 @@ void launch_radial_sweep(const KParams &P, hipStream_t st)
 }
 void launch_azimuthal_sweep(const KParams &P, hipStream_t st)
 {
-    const dim3 g((unsigned)((P.ns + kAzSeg - 1) / kAzSeg), (unsigned)P.nr);
-    LAUNCH(k_azimuthal_sweep, g, dim3(kAzThreads), st, P);
+    const int waves = (P.ns + kAzSegW - 1) / kAzSegW;
+    const dim3 g((unsigned)((waves + kAzWaves - 1) / kAzWaves), (unsigned)P.nr);
+    const dim3 b(64 * kAzWaves);
+    if (P.split) {
+        if (P.adiabatic) hipLaunchKernelGGL((k_azimuthal_sweep<true, true>), g, b, 0, st, P);
+        else hipLaunchKernelGGL((k_azimuthal_sweep<true, false>), g, b, 0, st, P);
+    } else {
+        if (P.adiabatic) hipLaunchKernelGGL((k_azimuthal_sweep<false, true>), g, b, 0, st, P);
+        else hipLaunchKernelGGL((k_azimuthal_sweep<false, false>), g, b, 0, st, P);
+    }
 }
 
 void launch_copy4(const KParams &P, hipStream_t st)
@@ -1619,6 +1699,173 @@ void launch_totals(const KParams &P, hipStream_t st, double radius, double *ring
 {
     LAUNCH(k_totals_rings, dim3(P.nr), dim3(64), st, P, radius, ring_scratch);
     hipLaunchKernelGGL(k_totals_final, dim3(1), dim3(1), 0, st, P, ring_scratch, out2);
+}
+
+// ---------------------------------------------------------------------------
+// Batched step: many disks per launch.  Each stage of one sim::run iteration
+// is one launch over a work table listing, for every disk that takes part in
+// the stage, every block of that disk's own grid.  A disk's blocks do exactly
+// what its single-disk launch does, so a batched disk evolves bit for bit like
+// the same disk stepped alone.
+
+static void add_grid(std::vector<BatchEntry> &v, unsigned d, unsigned gx, unsigned gy)
+{
+    for (unsigned y = 0; y < gy; ++y)
+        for (unsigned x = 0; x < gx; ++x) v.push_back(BatchEntry{d, (y << 16) | x});
+}
+
+static unsigned rows_grid(const KParams &P, int rows) { return (unsigned)((rows + 3) / 4); }
+static unsigned sector_grid(const KParams &P) { return (unsigned)((P.ns + 63) / 64); }
+
+std::vector<BatchStageHost> plan_batch_step(const std::vector<KParams> &D)
+{
+    std::vector<BatchStageHost> st;
+    auto stage = [&](int kind, int arg, auto pred, auto grid) {
+        BatchStageHost s{kind, arg, {}};
+        for (unsigned d = 0; d < D.size(); ++d)
+            if (pred(D[d])) grid(s.entries, d, D[d]);
+        if (!s.entries.empty()) st.push_back(std::move(s));
+    };
+    auto all = [](const KParams &) { return true; };
+    auto adi = [](const KParams &P) { return P.adiabatic != 0; };
+    auto iso = [](const KParams &P) { return P.adiabatic == 0; };
+    auto rows = [](int off) {
+        return [off](std::vector<BatchEntry> &v, unsigned d, const KParams &P) {
+            add_grid(v, d, sector_grid(P), rows_grid(P, P.nr - off));
+        };
+    };
+    auto rings = [](std::vector<BatchEntry> &v, unsigned d, const KParams &P) {
+        add_grid(v, d, (unsigned)P.nr, 1);
+    };
+    auto one = [](std::vector<BatchEntry> &v, unsigned d, const KParams &) { add_grid(v, d, 1, 1); };
+    auto cfl = [](std::vector<BatchEntry> &v, unsigned d, const KParams &P) {
+        add_grid(v, d, (unsigned)P.cfl_gx, (unsigned)P.cfl_gy);
+    };
+    auto ghosts = [](std::vector<BatchEntry> &v, unsigned d, const KParams &P) {
+        add_grid(v, d, (unsigned)((P.ns + 255) / 256), 1);
+    };
+    auto radial = [](std::vector<BatchEntry> &v, unsigned d, const KParams &P) {
+        add_grid(v, d, sector_grid(P), (unsigned)((P.nr + P.radial_tile - 1) / P.radial_tile));
+    };
+    auto azim = [](std::vector<BatchEntry> &v, unsigned d, const KParams &P) {
+        const int waves = (P.ns + kAzSegW - 1) / kAzSegW;
+        add_grid(v, d, (unsigned)((waves + kAzWaves - 1) / kAzWaves), (unsigned)P.nr);
+    };
+    auto damp = [](std::vector<BatchEntry> &v, unsigned d, const KParams &P) {
+        const int n_inner = (P.damp_ids[0] > P.damp_ids[1] ? P.damp_ids[0] : P.damp_ids[1]) + 1;
+        const int o0 = P.damp_ids[2] < P.damp_ids[3] ? P.damp_ids[2] : P.damp_ids[3];
+        add_grid(v, d, sector_grid(P), rows_grid(P, n_inner + P.nr + 1 - o0));
+    };
+    auto pot_bg = [](std::vector<BatchEntry> &v, unsigned d, const KParams &P) {
+        add_grid(v, d, (unsigned)((P.nr + 255) / 256), 1);
+    };
+    // sim::CalculateTimeStep and the monitor clip (simulation.cpp:100-118, 526-542)
+    stage(B_RING_MEAN, 0, all, rings);
+    stage(B_CFL_CELLS, 0, all, cfl);
+    stage(B_CFL_FINAL, 2, all, one);
+    // step_Euler (simulation.cpp:148-267), in the order of enqueue_euler
+    stage(B_POT_BG, 0, [](const KParams &P) { return P.split != 0; }, pot_bg);
+    stage(B_POTENTIAL, 0, all, rows(0));
+    stage(B_SRC_VR, 0, all, rows(3));
+    stage(B_SRC_VPHI, 0, all, rows(2));
+    stage(B_SRC_ENERGY, 0, adi, rows(1));
+    auto tw = [](const KParams &P) { return P.artvisc == FCPT_ARTVISC_TW; };
+    auto sn = [](const KParams &P) { return P.artvisc == FCPT_ARTVISC_SN; };
+    auto none = [](const KParams &P) { return P.artvisc == FCPT_ARTVISC_NONE; };
+    stage(B_TW_Q, 0, tw, rows(0));
+    stage(B_TW_VEL, 1, tw, rows(0));
+    stage(B_SN_Q, 0, sn, rows(0));
+    stage(B_SN_VEL, 1, sn, rows(0));
+    stage(B_TEMP_FLOOR, 0,
+          [](const KParams &P) {
+              return P.artvisc == FCPT_ARTVISC_NONE && P.adiabatic && P.artvisc_dissipation;
+          },
+          rows(0));
+    stage(B_RECALC_VISC, 0, none, rows(0));
+    stage(B_STRESS, 0, all, rows(0));
+    stage(B_VISC_UPDATE, 0, all, rows(2));
+    stage(B_SUBSTEP3, 0, adi, rows(0));
+    stage(B_GHOST, 0, all, ghosts);
+    stage(B_RADIAL_ISO, 0, iso, radial);
+    stage(B_RADIAL_ADI, 0, adi, radial);
+    stage(B_RING_MEAN, 1, all, rings);
+    stage(B_AZIM_ISO, 0, iso, azim);
+    stage(B_AZIM_ADI, 0, adi, azim);
+    stage(B_VELOCITIES, 0, all, rows(0));
+    stage(B_DAMPING, 0, [](const KParams &P) { return P.damping != 0; }, damp);
+    stage(B_GHOST, 2, all, ghosts);
+    stage(B_DERIVED, 0, all, rows(0));
+    return st;
+}
+
+#define LAUNCHB(kern, block, ...)                                                              \
+    do {                                                                                       \
+        if (split) hipLaunchKernelGGL(kern<true>, dim3(S.n), block, 0, st, Ps, S.map, ##__VA_ARGS__);  \
+        else hipLaunchKernelGGL(kern<false>, dim3(S.n), block, 0, st, Ps, S.map, ##__VA_ARGS__);       \
+    } while (0)
+
+void launch_batch_stage(const KParams *Ps, const BatchStageDev &S, int split, int radial_waves,
+                        hipStream_t st)
+{
+    const dim3 b256(256);
+    switch (S.kind) {
+    case B_RING_MEAN: LAUNCHB(kb_ring_mean, b256, S.arg); break;
+    case B_CFL_CELLS: LAUNCHB(kb_cfl_cells, kBlock); break;
+    case B_CFL_FINAL:
+        hipLaunchKernelGGL(kb_cfl_finalize, dim3(S.n), b256, 0, st, Ps, S.map, S.arg);
+        break;
+    case B_POT_BG: hipLaunchKernelGGL(kb_potential_bg, dim3(S.n), b256, 0, st, Ps, S.map); break;
+    case B_POTENTIAL: LAUNCHB(kb_potential, kBlock); break;
+    case B_SRC_VR: LAUNCHB(kb_src_vr, kBlock); break;
+    case B_SRC_VPHI: LAUNCHB(kb_src_vphi, kBlock); break;
+    case B_SRC_ENERGY: LAUNCHB(kb_src_energy, kBlock); break;
+    case B_TW_Q: LAUNCHB(kb_tw_q, kBlock); break;
+    case B_TW_VEL: LAUNCHB(kb_tw_vel, kBlock, S.arg); break;
+    case B_SN_Q: LAUNCHB(kb_sn_q, kBlock); break;
+    case B_SN_VEL: LAUNCHB(kb_sn_vel, kBlock, S.arg); break;
+    case B_TEMP_FLOOR: LAUNCHB(kb_temperature_floor, kBlock); break;
+    case B_RECALC_VISC: LAUNCHB(kb_recalc_viscosity, kBlock); break;
+    case B_STRESS: LAUNCHB(kb_stress, kBlock); break;
+    case B_VISC_UPDATE: LAUNCHB(kb_visc_update, kBlock); break;
+    case B_SUBSTEP3: LAUNCHB(kb_substep3, kBlock); break;
+    case B_GHOST: LAUNCHB(kb_ghost_bc, b256, S.arg); break;
+    case B_VELOCITIES: LAUNCHB(kb_velocities, kBlock); break;
+    case B_DAMPING: LAUNCHB(kb_damping, kBlock); break;
+    case B_DERIVED: LAUNCHB(kb_derived, kBlock); break;
+    case B_RADIAL_ISO:
+    case B_RADIAL_ADI: {
+        const bool a = S.kind == B_RADIAL_ADI, w2 = radial_waves == 2;
+        const dim3 g(S.n), b(64);
+#define RADB(SP, AD)                                                                           \
+    do {                                                                                       \
+        if (w2) hipLaunchKernelGGL((kb_radial_sweep2<SP, AD>), g, b, 0, st, Ps, S.map);        \
+        else hipLaunchKernelGGL((kb_radial_sweep3<SP, AD>), g, b, 0, st, Ps, S.map);           \
+    } while (0)
+        if (split) {
+            if (a) RADB(true, true);
+            else RADB(true, false);
+        } else {
+            if (a) RADB(false, true);
+            else RADB(false, false);
+        }
+#undef RADB
+        break;
+    }
+    case B_AZIM_ISO:
+    case B_AZIM_ADI: {
+        const bool a = S.kind == B_AZIM_ADI;
+        const dim3 g(S.n), b(64 * kAzWaves);
+        if (split) {
+            if (a) hipLaunchKernelGGL((kb_azimuthal_sweep<true, true>), g, b, 0, st, Ps, S.map);
+            else hipLaunchKernelGGL((kb_azimuthal_sweep<true, false>), g, b, 0, st, Ps, S.map);
+        } else {
+            if (a) hipLaunchKernelGGL((kb_azimuthal_sweep<false, true>), g, b, 0, st, Ps, S.map);
+            else hipLaunchKernelGGL((kb_azimuthal_sweep<false, false>), g, b, 0, st, Ps, S.map);
+        }
+        break;
+    }
+    default: break;
+    }
 }
 
 }  // namespace fcpt

@@ -4,6 +4,8 @@
 // captured into a HIP graph and replayed.
 #pragma once
 
+#include <vector>
+
 #include "fcpt_device.hpp"
 
 namespace fcpt {
@@ -40,5 +42,32 @@ void launch_rebase(const KParams &P, hipStream_t st, void *field, const double *
 size_t element_size(const KParams &P);
 // rings per thread of the radial sweep (requested > 0 overrides the size-based choice)
 int radial_tile_for(int nr, int ns, int requested);
+// fills the launch-shape fields of a disk's parameters (the CFL grid)
+void init_launch_fields(KParams &P);
+
+// Batched step: one work-table entry per block of one disk's grid
+struct BatchEntry {
+    unsigned disk, yx;  // yx = by << 16 | bx
+};
+enum BatchKind {
+    B_RING_MEAN, B_CFL_CELLS, B_CFL_FINAL, B_POT_BG, B_POTENTIAL, B_SRC_VR, B_SRC_VPHI,
+    B_SRC_ENERGY, B_TW_Q, B_TW_VEL, B_SN_Q, B_SN_VEL, B_TEMP_FLOOR, B_RECALC_VISC, B_STRESS,
+    B_VISC_UPDATE, B_SUBSTEP3, B_GHOST, B_RADIAL_ISO, B_RADIAL_ADI, B_AZIM_ISO, B_AZIM_ADI,
+    B_VELOCITIES, B_DAMPING, B_DERIVED
+};
+struct BatchStageHost {
+    int kind, arg;
+    std::vector<BatchEntry> entries;
+};
+struct BatchStageDev {
+    int kind, arg;
+    const BatchEntry *map;
+    unsigned n;
+};
+// the stages of one sim::run iteration for a set of disks (parameters as the
+// kernels see them, radial_tile resolved); stages no disk takes part in are left out
+std::vector<BatchStageHost> plan_batch_step(const std::vector<KParams> &disks);
+void launch_batch_stage(const KParams *Ps, const BatchStageDev &s, int split, int radial_waves,
+                        hipStream_t st);
 
 }  // namespace fcpt

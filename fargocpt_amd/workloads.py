@@ -99,6 +99,54 @@ def start(disk, name: str, first_dt: float = 0.1):
     return ic
 
 
+# The batched sweep: n locally isothermal disks (physics of iso_128x384) with
+# Nr uniform in [128, 1024], Nphi = 3 Nr, planet mass ratio log-uniform in
+# [1e-5, 1e-3], alpha log-uniform in [1e-4, 1e-2], h uniform in [0.03, 0.07],
+# drawn from numpy's PCG64 with the given seed; disk k's noise seed is seed + k.
+BATCHES = {"batch64": (64, 1337)}
+
+
+def batch_members(name: str = "batch64") -> list:
+    import numpy as np
+    n, seed = BATCHES[name]
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        nr = int(rng.integers(128, 1025))
+        q = float(10.0 ** rng.uniform(-5.0, -3.0))
+        alpha = float(10.0 ** rng.uniform(-4.0, -2.0))
+        h = float(rng.uniform(0.03, 0.07))
+        out.append({"nr": nr, "nphi": 3 * nr, "q": q, "alpha": alpha, "h": h, "seed": seed + k})
+    return out
+
+
+def member_config(m: dict, precision: int = 0) -> dict:
+    """Descriptor of one batched disk: iso_128x384's physics at the member's
+    size, viscosity and aspect ratio."""
+    cfg = disk_config("iso_128x384", precision)
+    cfg.update({"nr": m["nr"], "nphi": m["nphi"], "radii": log_radii(m["nr"], 0.4, 2.5),
+                "alpha": m["alpha"], "aspect_ratio": m["h"]})
+    return cfg
+
+
+def member_start(disk, m: dict, first_dt: float = 0.1):
+    """Initial state and start-up calls of one batched disk (as start())."""
+    setup_m = DiskSetup(nr=m["nr"], nphi=m["nphi"], rmin=0.4, rmax=2.5, sigma0=CODE_UNITS["sigma0"],
+                        sigma_slope=0.5, aspect_ratio=m["h"], flaring=0.0, G=1.0, mstar=1.0,
+                        omega_frame=1.0, adiabatic=False, gamma=1.4, noise_amp=1e-4,
+                        seed=m["seed"])
+    ic = initial_state(setup_m)
+    disk.upload_reference(ic["sigma"], ic["vrad"], ic["vazi"], ic["energy"])
+    disk.upload_state(ic["sigma"], ic["vrad"], ic["vazi"], ic["energy"])
+    disk.set_time(0.0, first_dt, 0)
+    disk.calculate_timestep()
+    disk.apply_boundary_condition(0.0, False)
+    disk.calculate_timestep()
+    q = m["q"]
+    disk.set_step_params([(0.0, 0.0, 1.0, 0.0), (1.0, 0.0, q, 0.0)], (-CODE_UNITS["G"] * q, 0.0))
+    return ic
+
+
 def algorithmic_bytes(kernel: str, adiabatic: bool, element: int = 8) -> int:
     """Compulsory HBM bytes per cell of one launch of a step kernel: each
     stored field it reads once plus each field it writes once, `element` bytes
@@ -133,4 +181,5 @@ def is_adiabatic(name: str) -> bool:
 
 
 __all__ = ["CODE_UNITS", "WORKLOADS", "DESCRIPTION", "disk_config", "setup", "planet_inputs",
-           "start", "algorithmic_bytes", "step_bytes", "STEP_BYTES", "cells", "is_adiabatic"]
+           "start", "algorithmic_bytes", "step_bytes", "STEP_BYTES", "cells", "is_adiabatic",
+           "BATCHES", "batch_members", "member_config", "member_start"]

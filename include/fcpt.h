@@ -223,6 +223,31 @@ int fcpt_set_launch_param(fcpt_ctx *ctx, int32_t param, int32_t value);
    returns it.  Drops a captured step graph. */
 int fcpt_autotune(fcpt_ctx *ctx, int32_t reps, int32_t *radial_tile, int32_t *radial_waves);
 
+/* ---- batches (no counterpart in the reference) ------------------------- */
+/* Many independent disks advanced together: each stage of a sim::run
+   iteration is one launch over a work table holding every block of every
+   disk taking part in it (disks of any size, physics and planets, on one
+   device and in one storage mode).  A batched disk evolves bit for bit like
+   the same disk under fcpt_run_steps.  The batch uses the disks' parameters
+   and launch shapes as they are at fcpt_batch_create and reads and writes
+   the disks' own device state: between batch calls the disks may be read
+   (fcpt_download_*, fcpt_totals) but not stepped on their own. */
+typedef struct fcpt_batch fcpt_batch;
+int fcpt_batch_create(fcpt_ctx *const *disks, int32_t n_disks, fcpt_batch **out);
+int fcpt_batch_destroy(fcpt_batch *batch);
+const char *fcpt_batch_last_error(const fcpt_batch *batch);
+int fcpt_batch_synchronize(fcpt_batch *batch);
+/* radial-sweep register budget for the whole batch (2 or 3 wavefronts per SIMD) */
+int fcpt_batch_set_radial_waves(fcpt_batch *batch, int32_t waves);
+/* n_steps sim::run iterations of every disk; use_graph != 0 replays one
+   captured HIP graph per iteration.  No host synchronisation. */
+int fcpt_batch_run_steps(fcpt_batch *batch, int32_t n_steps, int32_t use_graph);
+/* as fcpt_batch_run_steps, timed with HIP events on the batch's stream */
+int fcpt_batch_time_steps(fcpt_batch *batch, int32_t n_steps, int32_t use_graph, double *ms_total);
+/* mean duration of one batched launch of one step kernel over all disks
+   (FCPT_KERNEL_*; the listed kernels do not modify their inputs) */
+int fcpt_batch_time_kernel(fcpt_batch *batch, int32_t kernel, int32_t reps, double *ms_per_launch);
+
 /* Compensated (Neumaier) totals over active rings with Rmed <= radius
    (quantities::gas_total_mass / gas_angular_momentum, quantities.cpp:51-78,242-279). */
 int fcpt_totals(fcpt_ctx *ctx, double radius, double *mass, double *angular_momentum);

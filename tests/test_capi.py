@@ -45,3 +45,14 @@ def test_create_rejects_bad_sizes_before_touching_the_device():
     d.nr = 20000  # beyond the reference's MAX1D
     assert lib.fcpt_create(C.byref(d), C.byref(h)) == -1
     assert lib.fcpt_create(None, C.byref(h)) == -1
+
+
+def test_batch_members_are_reproducible():
+    from fargocpt_amd import workloads as W
+    a, b = W.batch_members(), W.batch_members()
+    assert a == b and len(a) == 64
+    for m in a:
+        assert 128 <= m["nr"] <= 1024 and m["nphi"] == 3 * m["nr"]
+        assert 1e-5 <= m["q"] <= 1e-3 and 1e-4 <= m["alpha"] <= 1e-2 and 0.03 <= m["h"] <= 0.07
+    cfg = W.member_config(a[0], 1)
+    assert cfg["nr"] == a[0]["nr"] and cfg["precision"] == 1 and len(cfg["radii"]) == a[0]["nr"] + 2
