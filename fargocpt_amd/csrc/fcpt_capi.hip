@@ -278,6 +278,9 @@ int fcpt_create(const fcpt_disk_desc *desc, fcpt_ctx **out)
         if (!rc) rc = dev_alloc(c, &c->bg_tmp, (size_t)nr + 2);
         for (int k = 0; k < FCPT_NUM_FIELDS; ++k) c->hbg[k].assign((size_t)nr + 2, 0.0);
     }
+    if (!rc) rc = dev_alloc(c, &P.iso_cs, (size_t)nr + 2);
+    if (!rc) rc = dev_alloc(c, &P.iso_h, (size_t)nr + 2);
+    if (!rc) rc = dev_alloc(c, &P.iso_nu, (size_t)nr + 2);
     if (!rc) rc = dev_alloc(c, &P.vmean, (size_t)nr + 1);
     if (!rc) rc = dev_alloc(c, &P.vuni, (size_t)nr + 1);
     if (!rc) rc = dev_alloc(c, &P.nshift, (size_t)nr + 1);
@@ -307,6 +310,11 @@ int fcpt_create(const fcpt_disk_desc *desc, fcpt_ctx **out)
         e = e == hipSuccess ? hipMemset(P.bgm[k], 0, ((size_t)nr + 2) * sizeof(double)) : e;
     std::memset(c->s_host, 0, sizeof(DevScalars));
     if (e == hipSuccess) e = hipMemcpy(c->s_dev, c->s_host, sizeof(DevScalars), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !P.adiabatic) {
+        launch_iso_rings(P, c->stream);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    }
     if (e != hipSuccess) {
         fcpt_destroy(c);
         return FCPT_ERR_DEVICE;

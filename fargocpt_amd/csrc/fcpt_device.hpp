@@ -80,6 +80,7 @@ struct KParams {
     void *sig_b, *en_b;  // radial-sweep outputs
     void *sig_c, *en_c;  // azimuthal-sweep outputs
     double *bgm[4];      // backgrounds of the four momenta (split storage)
+    double *iso_cs, *iso_h, *iso_nu;  // locally isothermal ring profiles of c_s, H, nu
     double *vmean;       // per-ring mean azimuthal velocity
     double *vuni;        // per-ring uniform residual velocity (FARGO)
     int *nshift;         // per-ring integer shift

@@ -30,6 +30,51 @@ __device__ __forceinline__ double van_leer_lim(double a, double b)
     const double r = 2.0 * a * b / (a + b);
     return (a * b > 0.0) ? r : 0.0;
 }
+// Split storage rounds every stored value to an FP32 deviation, so its
+// kernels may form quantities that only scale a correction term in FP32 and
+// replace divisions by a reciprocal; the FP64 instantiations keep the
+// reference's IEEE operations.  van Leer limiter: FP32 in split storage (the
+// slope multiplies the second-order term of the interface value).
+template <bool SPLIT>
+__device__ __forceinline__ double vl_lim(double a, double b)
+{
+    if constexpr (SPLIT) {
+        const float fa = (float)a, fb = (float)b;
+        const float r = __fdividef(2.0f * fa * fb, fa + fb);
+        return (fa * fb > 0.0f) ? (double)r : 0.0;
+    } else {
+        return van_leer_lim(a, b);
+    }
+}
+
+// FP64 reciprocal: one IEEE division, or in split storage the hardware
+// reciprocal refined by two Newton steps (error a few ulp)
+template <bool SPLIT>
+__device__ __forceinline__ double recip(double x)
+{
+    if constexpr (SPLIT) {
+        double y = __builtin_amdgcn_rcp(x);
+        y = y * (2.0 - x * y);
+        return y * (2.0 - x * y);
+    } else {
+        return 1.0 / x;
+    }
+}
+
+// 1 / sqrt(x): IEEE square root and division, or in split storage the
+// hardware reciprocal square root refined by two Newton steps
+template <bool SPLIT>
+__device__ __forceinline__ double inv_sqrt(double x)
+{
+    if constexpr (SPLIT) {
+        double y = __builtin_amdgcn_rsq(x);
+        y = y * (1.5 - 0.5 * x * y * y);
+        return y * (1.5 - 0.5 * x * y * y);
+    } else {
+        return 1.0 / sqrt(x);
+    }
+}
+
 __device__ __forceinline__ double omega_kepler(const KParams &P, double r)
 {
     return sqrt(P.G * P.mstar / (r * r * r));  // Theo.cpp:246-249
@@ -210,6 +255,7 @@ __device__ __forceinline__ void b_cfl_cells(const KParams &P, int bx, int by)
     FLD(vaz, FCPT_VAZI);
     FLD(cs, FCPT_SOUNDSPEED);
     FLD(nu, FCPT_VISCOSITY);
+    const bool iso = !P.adiabatic;  // c_s and nu are ring profiles
     const double CFL = P.cfl, dphi = P.dphi;
     const auto vm = cst(P.vmean);
     const double fac2 = P.artvisc_factor * P.artvisc_factor;
@@ -230,9 +276,15 @@ __device__ __forceinline__ void b_cfl_cells(const KParams &P, int bx, int by)
         const int jn = j == ns - 1 ? 0 : j + 1;
         const double vr0 = vr(i, j), vr1 = vr(i + 1, j), va0 = vaz(i, j), va1 = vaz(i, jn);
         const double v_residual = va0 - vm[i];
-        const double invdt1 = cs(i, j) / cell_size;
-        const double invdt2 = vr0 / dxRadial;
-        const double invdt3 = v_residual / dxAzimuthal;
+        // split storage: divisions by the ring's cell sizes become products
+        const double inv_dr = SPLIT ? cst(P.InvDiffRsup)[i] : 0.0;
+        const double inv_daz = SPLIT ? recip<true>(dxAzimuthal) : 0.0;
+        const double inv_cell = SPLIT ? recip<true>(cell_size) : 0.0;
+        const double csv = iso ? cst(P.iso_cs)[i] : cs(i, j);
+        const double nuv = iso ? cst(P.iso_nu)[i] : nu(i, j);
+        const double invdt1 = SPLIT ? csv * inv_cell : csv / cell_size;
+        const double invdt2 = SPLIT ? vr0 * inv_dr : vr0 / dxRadial;
+        const double invdt3 = SPLIT ? v_residual * inv_daz : v_residual / dxAzimuthal;
         double invdt4;
         if (P.artvisc == FCPT_ARTVISC_SN) {
             double dvRadial = vr1 - vr0;
@@ -241,25 +293,30 @@ __device__ __forceinline__ void b_cfl_cells(const KParams &P, int bx, int by)
             else dvRadial = -dvRadial;
             if (dvAzimuthal > 0.0) dvAzimuthal = 0.0;
             else dvAzimuthal = -dvAzimuthal;
-            invdt4 = 4.0 * fac2 * std_max(dvRadial / dxRadial, dvAzimuthal / dxAzimuthal) * 1.0;
+            if constexpr (SPLIT)
+                invdt4 = 4.0 * fac2 * std_max(dvRadial * inv_dr, dvAzimuthal * inv_daz) * 1.0;
+            else
+                invdt4 = 4.0 * fac2 * std_max(dvRadial / dxRadial, dvAzimuthal / dxAzimuthal) * 1.0;
         } else {
             const double eps_rr = (vr1 - vr0) * cst(P.InvDiffRsup)[i];
             const double eps_pp = cst(P.InvRmed)[i] * ((va1 - va0) * P.invdphi + 0.5 * (vr1 + vr0));
             const double mdiv_V = -std_min(eps_rr + eps_pp, 0.0);
             invdt4 = 4.0 * fac2 * mdiv_V * 1.0;
         }
-        const double invdt5 = 4.0 * nu(i, j) / (cell_size * cell_size) * 1.0;
+        const double invdt5 = SPLIT ? 4.0 * nuv * (inv_cell * inv_cell)
+                                    : 4.0 * nuv / (cell_size * cell_size) * 1.0;
         double invdt6 = 0.0;
         if (P.adiabatic) {
             FLD(qp, FCPT_QPLUS);
             FLD(qm, FCPT_QMINUS);
             FLD(E, FCPT_ENERGY);
             const double inv_limit = 1.0 / P.heating_cooling_cfl_limit;
-            invdt6 = inv_limit * fabs((qp(i, j) - qm(i, j)) / E(i, j)) * 1.0;
+            if constexpr (SPLIT) invdt6 = inv_limit * fabs((qp(i, j) - qm(i, j)) * recip<true>(E(i, j)));
+            else invdt6 = inv_limit * fabs((qp(i, j) - qm(i, j)) / E(i, j)) * 1.0;
         }
-        const double dt_cell =
-            CFL / sqrt(invdt1 * invdt1 + invdt2 * invdt2 + invdt3 * invdt3 + invdt4 * invdt4 +
-                       invdt5 * invdt5 + invdt6 * invdt6);
+        const double sum2 = invdt1 * invdt1 + invdt2 * invdt2 + invdt3 * invdt3 + invdt4 * invdt4 +
+                            invdt5 * invdt5 + invdt6 * invdt6;
+        const double dt_cell = SPLIT ? CFL * inv_sqrt<true>(sum2) : CFL / sqrt(sum2);
         // NaN never wins a comparison, exactly like the reference's "if (dt_cell < dt_core)"
         cand = std_min(cand, dt_cell);
     }
@@ -373,7 +430,7 @@ __device__ __forceinline__ void b_potential(const KParams &P, int bx, int by)
     const DevScalars *s = P.s;
     const double x = cst(P.Rmed)[i] * P.cosphi[j];
     const double y = cst(P.Rmed)[i] * P.sinphi[j];
-    const double H = Hf(i, j);
+    const double H = P.adiabatic ? Hf(i, j) : cst(P.iso_h)[i];
     double acc = 0.0;
     const int np = s->n_planets;
     for (int k = 0; k < np; ++k) {
@@ -381,14 +438,18 @@ __device__ __forceinline__ void b_potential(const KParams &P, int bx, int by)
         const double dx = x - s->planet[k].x;
         const double dy = y - s->planet[k].y;
         const double dist_2 = dx * dx + dy * dy;
-        const double d_smoothed = sqrt(dist_2 + smooth * smooth);
+        const double d2 = dist_2 + smooth * smooth;
+        // split storage: one reciprocal square root instead of sqrt and division
+        const double inv_d = SPLIT ? inv_sqrt<true>(d2) : 0.0;
+        const double d_smoothed = SPLIT ? d2 * inv_d : sqrt(d2);
         double smooth_factor_klahr = 1.0;
         const double r_sm = s->planet[k].r_cubic;
         if (r_sm > 0.0 && d_smoothed < r_sm) {
             const double q = d_smoothed / r_sm;
             smooth_factor_klahr = (pow(q, 4.0) - 2.0 * pow(q, 3.0) + 2.0 * d_smoothed / r_sm);
         }
-        acc += -P.G * s->planet[k].mass / d_smoothed * smooth_factor_klahr;
+        if constexpr (SPLIT) acc += -P.G * s->planet[k].mass * inv_d * smooth_factor_klahr;
+        else acc += -P.G * s->planet[k].mass / d_smoothed * smooth_factor_klahr;
     }
     acc += -s->indirect_x * x - s->indirect_y * y;
     fld<SPLIT>(P, FCPT_POTENTIAL).set(i, j, acc);
@@ -491,8 +552,9 @@ __device__ __forceinline__ DerivedIn derived_load(const KParams &P, int i, int j
         d.e = fld<SPLIT>(P, FCPT_ENERGY)(i, j);
         d.sig = fld<SPLIT>(P, FCPT_SIGMA)(i, j);
     } else {
-        d.cs = fld<SPLIT>(P, FCPT_SOUNDSPEED)(i, j);
-        d.h = fld<SPLIT>(P, FCPT_SCALE_HEIGHT)(i, j);
+        // locally isothermal: c_s and H are the ring profiles of k_iso_rings
+        d.cs = cst(P.iso_cs)[i];
+        d.h = cst(P.iso_h)[i];
         if (with_pressure) d.sig = fld<SPLIT>(P, FCPT_SIGMA)(i, j);
     }
     return d;
@@ -500,7 +562,8 @@ __device__ __forceinline__ DerivedIn derived_load(const KParams &P, int i, int j
 
 template <bool SPLIT>
 __device__ __forceinline__ void derived_store(const KParams &P, int i, int j, const DerivedIn &d,
-                                             bool with_temperature, bool with_pressure)
+                                             bool with_temperature, bool with_pressure,
+                                             bool force_nu = false)
 {
     double c_s, h;
     if (P.adiabatic) {
@@ -521,6 +584,9 @@ __device__ __forceinline__ void derived_store(const KParams &P, int i, int j, co
         c_s = d.cs;
         h = d.h;
         if (with_pressure) fld<SPLIT>(P, FCPT_PRESSURE).set(i, j, d.sig * (c_s * c_s));
+        // locally isothermal: nu = alpha H c_s is the ring profile iso_nu,
+        // already in the field since the state was uploaded
+        if (!force_nu) return;
     }
     const double alpha = P.alpha;
     fld<SPLIT>(P, FCPT_VISCOSITY).set(i, j, alpha * h * c_s);
@@ -584,7 +650,8 @@ __device__ __forceinline__ void b_tw_vel(const KParams &P, int bx, int by, int w
     // in the fused step, recalculate_viscosity (SourceEuler.cpp:205-223), which is
     // pointwise in Sigma and E and follows this kernel, is done here for every ring
     DerivedIn din{0.0, 0.0, 0.0, 0.0};
-    if (with_viscosity) din = derived_load<SPLIT>(P, i, j, false);
+    const bool visc = with_viscosity && P.adiabatic;  // constant ring profile when isothermal
+    if (visc) din = derived_load<SPLIT>(P, i, j, false);
     double s0 = 0.0, sjp = 0.0, sm = 0.0, qp0 = 0.0, qpj = 0.0, qpm = 0.0, qr0 = 0.0, qrm = 0.0;
     double va = 0.0, v = 0.0;
     if (inner) {
@@ -601,7 +668,7 @@ __device__ __forceinline__ void b_tw_vel(const KParams &P, int bx, int by, int w
         qpm = Qp(im, j);
         v = vr(i, j);
     }
-    if (with_viscosity) derived_store<SPLIT>(P, i, j, din, false, false);
+    if (visc) derived_store<SPLIT>(P, i, j, din, false, false);
     if (!inner) return;
     const double sigma_phi_avg = 0.5 * (s0 + sjp);
     const double dVp = 2.0 * dt / ((cst(P.Rsup)[i] + cst(P.Rinf)[i]) * sigma_phi_avg) * (qp0 - qpj) * P.invdphi;
@@ -668,7 +735,8 @@ __device__ __forceinline__ void b_sn_vel(const KParams &P, int bx, int by, int w
     // in the fused step, recalculate_viscosity (SourceEuler.cpp:205-223), which is
     // pointwise in Sigma and E and follows this kernel, is done here for every ring
     DerivedIn din{0.0, 0.0, 0.0, 0.0};
-    if (with_viscosity) din = derived_load<SPLIT>(P, i, j, false);
+    const bool visc = with_viscosity && P.adiabatic;  // constant ring profile when isothermal
+    if (visc) din = derived_load<SPLIT>(P, i, j, false);
     double s0 = 0.0, sjp = 0.0, sm = 0.0, qp0 = 0.0, qpj = 0.0, qr0 = 0.0, qrm = 0.0;
     double va = 0.0, v = 0.0;
     if (inner) {
@@ -684,7 +752,7 @@ __device__ __forceinline__ void b_sn_vel(const KParams &P, int bx, int by, int w
         qrm = Qr(im, j);
         v = vr(i, j);
     }
-    if (with_viscosity) derived_store<SPLIT>(P, i, j, din, false, false);
+    if (visc) derived_store<SPLIT>(P, i, j, din, false, false);
     if (!inner) return;
     if (radial) vr.set(i, j, v - dt * 2.0 / (s0 + sm) * (qr0 - qrm) * cst(P.InvDiffRmed)[i]);
     const double dxtheta = P.dphi * cst(P.Rmed)[i];
@@ -698,7 +766,7 @@ template <bool SPLIT>
 __device__ __forceinline__ void b_recalc_viscosity(const KParams &P, int bx, int by)
 {
     CELL_LOOP_HEAD(0, P.nr)
-    derived_store<SPLIT>(P, i, j, derived_load<SPLIT>(P, i, j, false), false, false);
+    derived_store<SPLIT>(P, i, j, derived_load<SPLIT>(P, i, j, false), false, false, true);
 }
 FCPT_PAIR0(recalc_viscosity, 256)
 
@@ -731,6 +799,24 @@ __device__ __forceinline__ void b_iso_profile(const KParams &P, int bx, int by)
 }
 FCPT_PAIR0(iso_profile, 256)
 
+// the locally isothermal ring profiles c_s, H and nu = alpha H c_s, with the
+// expressions of the 2-D kernels above (k_iso_profile, derived_store)
+__global__ void __launch_bounds__(256) k_iso_rings(KParams P)
+{
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= P.nr) return;
+    const double r = cst(P.Rmed)[i];
+    const double vK = sqrt(P.G * P.mstar / r);
+    const double h = P.aspect_ratio * pow(r, P.flaring);
+    const double c_s = h * vK;
+    const double inv_omega_kepler = 1.0 / omega_kepler(P, r);
+    const double H = c_s * inv_omega_kepler;
+    const double alpha = P.alpha;
+    P.iso_cs[i] = c_s;
+    P.iso_h[i] = H;
+    P.iso_nu[i] = alpha * H * c_s;
+}
+
 // ---------------------------------------------------------------------------
 // compute_viscous_stress_tensor, viscosity.cpp:139-254
 template <bool SPLIT>
@@ -745,9 +831,11 @@ __device__ __forceinline__ void b_stress(const KParams &P, int bx, int by)
     const int jp = j == 0 ? ns - 1 : j - 1;
     const int im = i >= 1 ? i - 1 : 0;
     const double vr0 = vr(i, j), vr1 = vr(i + 1, j), va0 = vaz(i, j), va1 = vaz(i, jn);
-    const double n = nu(i, j), s = S(i, j);
+    const bool iso = !P.adiabatic;  // nu is a ring profile
+    const double n = iso ? cst(P.iso_nu)[i] : nu(i, j), s = S(i, j);
     const double vam = vaz(im, j), vrp = vr(i, jp);
-    const double nm = nu(im, j), njp = nu(i, jp), nmp = nu(im, jp);
+    const double nm = iso ? cst(P.iso_nu)[im] : nu(im, j);
+    const double njp = iso ? n : nu(i, jp), nmp = iso ? nm : nu(im, jp);
     const double sm = S(im, j), sjp = S(i, jp), smp = S(im, jp);
     const double divv = (vr1 * cst(P.Rinf)[i + 1] - vr0 * cst(P.Rinf)[i]) * cst(P.InvDiffRsupRb)[i] +
                         (va1 - va0) * P.invdphi * cst(P.InvRmed)[i];
@@ -1106,27 +1194,29 @@ __device__ __forceinline__ void decode_ring(const KParams &P, const RingRaw<SPLI
     if constexpr (ADI) q[4] = fld<SPLIT>(P, FCPT_ENERGY).dec(rc, X.e);
     R.v = (r >= 0 && r <= P.nr) ? v0 : 0.0;
     R.s = in ? sig : 1.0;
+    const double inv_sig = SPLIT ? recip<true>(sig) : 0.0;
 #pragma unroll
     for (int k = 0; k < NQ; ++k) {
         R.q[k] = in ? q[k] : 0.0;
-        R.w[k] = in ? q[k] / sig : 0.0;
+        const double w = SPLIT ? q[k] * inv_sig : q[k] / sig;
+        R.w[k] = in ? w : 0.0;
     }
 }
 
 // van Leer slopes of ring k from rings k-1, k, k+1 (compute_star_radial first
 // loop, TransportEuler.cpp:355-371); zero on the ghost rings
-template <int NQ>
+template <bool SPLIT, int NQ>
 __device__ __forceinline__ void ring_slopes(const KParams &P, int k, const RingW<NQ> &M, RingW<NQ> &C,
                                             const RingW<NQ> &N)
 {
     const bool act = k > 0 && k < P.nr - 1;
     const int kc = act ? k : 1;
     const double im = cst(P.InvDiffRmed)[kc], ip = cst(P.InvDiffRmed)[kc + 1];
-    const double ds = van_leer_lim((N.s - C.s) * ip, (C.s - M.s) * im);
+    const double ds = vl_lim<SPLIT>((N.s - C.s) * ip, (C.s - M.s) * im);
     C.ds = act ? ds : 0.0;
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
-        const double dw = van_leer_lim((N.w[q] - C.w[q]) * ip, (C.w[q] - M.w[q]) * im);
+        const double dw = vl_lim<SPLIT>((N.w[q] - C.w[q]) * ip, (C.w[q] - M.w[q]) * im);
         C.dw[q] = act ? dw : 0.0;
     }
 }
@@ -1191,8 +1281,8 @@ __device__ __forceinline__ void radial_sweep(const KParams &P, int bx, int by, i
         decode_ring<SPLIT, ADI, NQ>(P, X2, B);
         decode_ring<SPLIT, ADI, NQ>(P, X3, C);
     }
-    ring_slopes<NQ>(P, i0 - 1, D, A, B);
-    ring_slopes<NQ>(P, i0, A, B, C);
+    ring_slopes<SPLIT, NQ>(P, i0 - 1, D, A, B);
+    ring_slopes<SPLIT, NQ>(P, i0, A, B, C);
     double f_prev[NQ], fs_prev;
     interface_flux<NQ>(P, i0, A, B, dt, f_prev, fs_prev);
     const int iend = i0 + tile < nr ? i0 + tile : nr;
@@ -1201,7 +1291,7 @@ __device__ __forceinline__ void radial_sweep(const KParams &P, int bx, int by, i
     for (int k = i0 + 1; k <= iend; ++k) {
         decode_ring<SPLIT, ADI, NQ>(P, X, D);      // ring k+1
         raw_ring<SPLIT, ADI>(P, k + 2, j, jn, X);  // in flight during this iteration
-        ring_slopes<NQ>(P, k, B, C, D);            // ring k
+        ring_slopes<SPLIT, NQ>(P, k, B, C, D);            // ring k
         A = B;
         B = C;
         C = D;
@@ -1258,9 +1348,10 @@ __device__ __forceinline__ double from_right(double x) { return __shfl_down(x, 1
 
 // van Leer slope of a sector from its two neighbours (the slope half of
 // ComputeStarTheta, TransportEuler.cpp:416-465)
+template <bool SPLIT>
 __device__ __forceinline__ double slope_theta(double q, double invdxtheta)
 {
-    return 0.5 * van_leer_lim(from_right(q) - q, q - from_left(q)) * invdxtheta;
+    return 0.5 * vl_lim<SPLIT>(from_right(q) - q, q - from_left(q)) * invdxtheta;
 }
 
 // upwind interface value between sector j-1 and sector j (the second half of
@@ -1323,27 +1414,29 @@ __device__ __forceinline__ void b_azimuthal_sweep(const KParams &P, int bx, int 
     if constexpr (ADI) q0[4] = buf<false>(P, P.en_b, nullptr)(i, js);
     const double ks0 = v0 * dt;
     // density: first pass (residual velocity), then second pass (uniform residual)
-    const double ds0 = star_theta(s0, slope_theta(s0, invdxtheta), dxtheta, ks0);
+    const double ds0 = star_theta(s0, slope_theta<SPLIT>(s0, invdxtheta), dxtheta, ks0);
     const double fd0 = dxrad * ds0 * v0;
     double varq = fd0;
     varq -= from_right(fd0);
     const double s1 = s0 + varq * invsurf;
-    const double ds1 = star_theta(s1, slope_theta(s1, invdxtheta), dxtheta, ksu);
+    const double ds1 = star_theta(s1, slope_theta<SPLIT>(s1, invdxtheta), dxtheta, ksu);
     const double fd1 = dxrad * ds1 * vu;
     varq = fd1;
     varq -= from_right(fd1);
     if (write) buf<false>(P, P.sig_c, nullptr).set(i, jd, s1 + varq * invsurf);
     // the other quantities
+    const double inv_s0 = SPLIT ? recip<true>(s0) : 0.0;
+    const double inv_s1 = SPLIT ? recip<true>(s1) : 0.0;
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
-        const double w0 = q0[q] / s0;
-        const double qs0 = star_theta(w0, slope_theta(w0, invdxtheta), dxtheta, ks0);
+        const double w0 = SPLIT ? q0[q] * inv_s0 : q0[q] / s0;
+        const double qs0 = star_theta(w0, slope_theta<SPLIT>(w0, invdxtheta), dxtheta, ks0);
         const double f0 = dxrad * qs0 * ds0 * v0;
         double vq = f0;
         vq -= from_right(f0);
         const double q1 = q0[q] + vq * invsurf;
-        const double w1 = q1 / s1;
-        const double qs1 = star_theta(w1, slope_theta(w1, invdxtheta), dxtheta, ksu);
+        const double w1 = SPLIT ? q1 * inv_s1 : q1 / s1;
+        const double qs1 = star_theta(w1, slope_theta<SPLIT>(w1, invdxtheta), dxtheta, ksu);
         const double f1 = dxrad * qs1 * ds1 * vu;
         vq = f1;
         vq -= from_right(f1);
@@ -1682,6 +1775,11 @@ void launch_transport(const KParams &P, hipStream_t st)
 }
 
 void launch_derived(const KParams &P, hipStream_t st) { LAUNCH(k_derived, cell_grid(P, P.nr), kBlock, st, P); }
+void launch_iso_rings(const KParams &P, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_iso_rings, dim3((unsigned)((P.nr + 255) / 256)), dim3(256), 0, st, P);
+}
+
 void launch_iso_profile(const KParams &P, hipStream_t st)
 {
     LAUNCH(k_iso_profile, cell_grid(P, P.nr), kBlock, st, P);
