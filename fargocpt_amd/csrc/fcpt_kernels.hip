@@ -1341,10 +1341,19 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3)))
     radial_sweep<SPLIT, ADI>(P, (int)(e.yx & 0xffffu), (int)(e.yx >> 16), P.radial_tile);
 }
 
-// Neighbour values within a wavefront: the value of lane - 1 and of lane + 1
-// (the edge lanes get their own value back; results there are discarded)
-__device__ __forceinline__ double from_left(double x) { return __shfl_up(x, 1, 64); }
-__device__ __forceinline__ double from_right(double x) { return __shfl_down(x, 1, 64); }
+// Neighbour values within a wavefront: the value of lane - 1 and of lane + 1,
+// moved by the DPP wavefront shifts (wave_shr:1, wave_shl:1) on both 32-bit
+// halves; the edge lanes receive 0 and their results are discarded
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double x)
+{
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b & 0xffffffffll), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+__device__ __forceinline__ double from_left(double x) { return dpp_move<0x138>(x); }
+__device__ __forceinline__ double from_right(double x) { return dpp_move<0x130>(x); }
 
 // van Leer slope of a sector from its two neighbours (the slope half of
 // ComputeStarTheta, TransportEuler.cpp:416-465)
