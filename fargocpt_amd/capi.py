@@ -69,6 +69,21 @@ class StepParams(C.Structure):
                 ("indirect_x", C.c_double), ("indirect_y", C.c_double)]
 
 
+class ForceBody(C.Structure):
+    _fields_ = [("x", C.c_double), ("y", C.c_double), ("r", C.c_double), ("r_cubic", C.c_double)]
+
+
+FORCE_NO_STAR_SMOOTHING, FORCE_SMOOTHING_PLANETLOC, FORCE_CORRECT_SELFGRAVITY = 1, 2, 4
+
+
+def make_force_bodies(bodies):
+    """bodies: sequence of (x, y, r, r_cubic) -> ctypes array of ForceBody"""
+    arr = (ForceBody * max(1, len(bodies)))()
+    for k, (x, y, r, rc) in enumerate(bodies):
+        arr[k] = ForceBody(x, y, r, rc)
+    return arr
+
+
 def make_step_params(planets, indirect=(0.0, 0.0)) -> StepParams:
     """planets: iterable of (x, y, mass, r_cubic); body 0 is the central object."""
     sp = StepParams()
@@ -164,6 +179,7 @@ def load_library(path: str = LIB_PATH):
         "fcpt_time_copy": [P, I, dp, dp],
         "fcpt_set_launch_param": [P, I, I],
         "fcpt_autotune": [P, I, C.POINTER(I), C.POINTER(I)],
+        "fcpt_disk_on_planet_accel": [P, C.POINTER(ForceBody), I, I, dp],
         "fcpt_batch_create": [C.POINTER(P), I, C.POINTER(P)],
         "fcpt_batch_destroy": [P],
         "fcpt_batch_synchronize": [P],
@@ -334,6 +350,14 @@ class Disk:
         ms = C.c_double()
         self._check(self.lib.fcpt_time_kernel(self.h, kernel, reps, C.byref(ms)))
         return ms.value
+
+    def disk_on_planet_accel(self, bodies, flags=FORCE_CORRECT_SELFGRAVITY):
+        """ComputeDiskOnPlanetAccel for every body (x, y, r, r_cubic);
+        returns an (n, 2) array of accelerations."""
+        arr = make_force_bodies(bodies)
+        out = np.zeros(2 * max(1, len(bodies)))
+        self._check(self.lib.fcpt_disk_on_planet_accel(self.h, arr, len(bodies), flags, _dptr(out)))
+        return out[:2 * len(bodies)].reshape(-1, 2)
 
     def set_radial_tile(self, rings):
         """Rings walked by one radial-sweep thread (0: size-based default)."""

@@ -28,6 +28,10 @@ struct fcpt_ctx {
     double *totals_ring = nullptr;
     double *totals_out = nullptr;
     double *totals_host = nullptr;
+    double *force_sig1d = nullptr;  // ring means of Sigma for the disk-force reduction
+    double *force_part = nullptr;   // per-block partial sums
+    double *force_out = nullptr;
+    double *force_host = nullptr;
     hipGraphExec_t step_graph = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::string err;
@@ -288,6 +292,9 @@ int fcpt_create(const fcpt_disk_desc *desc, fcpt_ctx **out)
     if (!rc) rc = dev_alloc(c, &c->s_dev, 1);
     if (!rc) rc = dev_alloc(c, &c->totals_ring, 4 * (size_t)nr);
     if (!rc) rc = dev_alloc(c, &c->totals_out, 2);
+    if (!rc) rc = dev_alloc(c, &c->force_sig1d, (size_t)nr + 2);
+    if (!rc) rc = dev_alloc(c, &c->force_part, (size_t)kCflMaxBlocks * 4 * kForceMaxBodies);
+    if (!rc) rc = dev_alloc(c, &c->force_out, (size_t)4 * kForceMaxBodies);
     if (rc) {
         fcpt_destroy(c);
         return rc;
@@ -295,6 +302,8 @@ int fcpt_create(const fcpt_disk_desc *desc, fcpt_ctx **out)
     P.s = c->s_dev;
     e = hipHostMalloc((void **)&c->s_host, sizeof(DevScalars), hipHostMallocDefault);
     if (e == hipSuccess) e = hipHostMalloc((void **)&c->totals_host, 2 * sizeof(double), hipHostMallocDefault);
+    if (e == hipSuccess)
+        e = hipHostMalloc((void **)&c->force_host, 4 * kForceMaxBodies * sizeof(double), hipHostMallocDefault);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         fcpt_destroy(c);
@@ -337,6 +346,7 @@ int fcpt_destroy(fcpt_ctx *ctx)
     for (void *p : ctx->allocs) hipFree(p);
     if (ctx->s_host) hipHostFree(ctx->s_host);
     if (ctx->totals_host) hipHostFree(ctx->totals_host);
+    if (ctx->force_host) hipHostFree(ctx->force_host);
     delete ctx;
     return FCPT_OK;
 }
@@ -888,6 +898,46 @@ int fcpt_autotune(fcpt_ctx *ctx, int32_t reps, int32_t *radial_tile, int32_t *ra
     }
     if (radial_tile) *radial_tile = best_t;
     if (radial_waves) *radial_waves = best_w;
+    return FCPT_OK;
+}
+
+int fcpt_disk_on_planet_accel(fcpt_ctx *ctx, const fcpt_force_body *bodies, int32_t n_bodies,
+                              int32_t flags, double *accel_xy)
+{
+    CTX_CHECK(ctx);
+    if (n_bodies < 0 || n_bodies > kForceMaxBodies || (n_bodies > 0 && (!bodies || !accel_xy)))
+        return fail(ctx, FCPT_ERR_ARG, "1 .. %d bodies", kForceMaxBodies);
+    if (flags & ~(FCPT_FORCE_NO_STAR_SMOOTHING | FCPT_FORCE_SMOOTHING_PLANETLOC |
+                  FCPT_FORCE_CORRECT_SELFGRAVITY))
+        return fail(ctx, FCPT_ERR_ARG, "unknown disk-force flags");
+    if (n_bodies == 0) return FCPT_OK;
+    ForceBodies B{};
+    B.n = n_bodies;
+    B.flags = flags;
+    for (int k = 0; k < n_bodies; ++k) {
+        B.x[k] = bodies[k].x;
+        B.y[k] = bodies[k].y;
+        B.r[k] = bodies[k].r;
+        B.r_cubic[k] = bodies[k].r_cubic;
+        // compute_smoothing (Force.cpp:145-159): fixed per body, or thickness x H(cell)
+        if ((flags & FCPT_FORCE_NO_STAR_SMOOTHING) && k == 0)
+            B.smooth_fixed[k] = 0.0;
+        else if (flags & FCPT_FORCE_SMOOTHING_PLANETLOC)
+            B.smooth_fixed[k] = ctx->desc.thickness_smoothing *
+                                (ctx->desc.aspect_ratio * std::pow(bodies[k].r, 1 + ctx->desc.flaring_index));
+        else
+            B.smooth_fixed[k] = -1.0;
+    }
+    launch_disk_force(ctx->P, ctx->stream, B, ctx->force_sig1d, ctx->force_part, ctx->force_out);
+    LAUNCH_CHECK(ctx);
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->force_host, ctx->force_out, 4 * kForceMaxBodies * sizeof(double),
+                                hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    for (int k = 0; k < n_bodies; ++k) {
+        const double *o = ctx->force_host + 4 * k;
+        accel_xy[2 * k] = o[0] + o[2];      // inner + outer, Force.cpp:118-119
+        accel_xy[2 * k + 1] = o[1] + o[3];
+    }
     return FCPT_OK;
 }
 

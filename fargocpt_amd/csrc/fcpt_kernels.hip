@@ -1809,6 +1809,110 @@ void launch_totals(const KParams &P, hipStream_t st, double radius, double *ring
 }
 
 // ---------------------------------------------------------------------------
+// The disk's gravity on the N bodies (ComputeDiskOnPlanetAccel, Force.cpp:23-122)
+
+// ComputeAverageDensity, Pframeforce.cpp:174-187: ring means of Sigma over the
+// active rings (pairwise tree order), one block per ring
+template <bool SPLIT>
+__global__ void __launch_bounds__(256) k_sigma_ring_mean(KParams P, double *sig1d)
+{
+    __shared__ double sh[256];
+    const int ns = P.ns;
+    const int i = blockIdx.x;
+    FLD(S, FCPT_SIGMA);
+    double acc = 0.0;
+    for (int j = threadIdx.x; j < ns; j += 256) acc += S(i, j);
+    const double sum = block_sum<256>(acc, sh);
+    if (threadIdx.x == 0) sig1d[i] = sum / ns;
+}
+
+// Per cell and body, the reference's expression (the smoothing of
+// compute_smoothing, Force.cpp:145-159, chosen by the flags; the Klahr-Kley
+// factor of the force); inner (Rmed < r_body) and outer sums kept apart as in
+// the reference.  Rows are walked with a grid stride; each block reduces its
+// 4 x n partial sums with wavefront shuffles and LDS into its own slot, and
+// k_disk_force_final folds the slots in order: deterministic, no atomics.
+template <bool SPLIT>
+__global__ void __launch_bounds__(256) k_disk_force(KParams P, ForceBodies B, const double *sig1d,
+                                                    double *part)
+{
+    const int ns = P.ns;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    double acc[kForceMaxBodies][4];
+#pragma unroll
+    for (int k = 0; k < kForceMaxBodies; ++k) acc[k][0] = acc[k][1] = acc[k][2] = acc[k][3] = 0.0;
+    FLD(S, FCPT_SIGMA);
+    FLD(Hf, FCPT_SCALE_HEIGHT);
+    const double G = P.G;
+    const bool correct = (B.flags & FCPT_FORCE_CORRECT_SELFGRAVITY) != 0;
+    for (int i = wave_ring(1, (int)blockIdx.y); i < P.nr - 1 && j < ns; i += P.cfl_gy * (int)blockDim.y) {
+        const double sig = S(i, j);
+        const double H = P.adiabatic ? Hf(i, j) : cst(P.iso_h)[i];
+        const double rmed = cst(P.Rmed)[i];
+        const double xc = rmed * P.cosphi[j];
+        const double yc = rmed * P.sinphi[j];
+        double cell_sigma = sig;
+        if (correct) cell_sigma -= cst(sig1d)[i];
+        const double cellmass = cst(P.Surf)[i] * cell_sigma;
+#pragma unroll
+        for (int k = 0; k < kForceMaxBodies; ++k) {
+            if (k >= B.n) break;
+            const double smooth = B.smooth_fixed[k] >= 0.0 ? B.smooth_fixed[k] : P.thickness_smoothing * H;
+            const double dx = xc - B.x[k];
+            const double dy = yc - B.y[k];
+            const double dist_2 = dx * dx + dy * dy;
+            const double dist_sm_2 = dist_2 + smooth * smooth;
+            const double dist_sm = sqrt(dist_sm_2);
+            const double dist_sm_3 = dist_sm_2 * dist_sm;
+            const double inv_dist_sm_3 = 1.0 / dist_sm_3;
+            double smooth_factor_klahr = 1.0;
+            const double r_sm = B.r_cubic[k];
+            if (r_sm > 0.0 && dist_sm < r_sm)
+                smooth_factor_klahr = -(3.0 * pow(dist_sm / r_sm, 4.0) - 4.0 * pow(dist_sm / r_sm, 3.0));
+            const double fx = G * cellmass * dx * inv_dist_sm_3 * smooth_factor_klahr;
+            const double fy = G * cellmass * dy * inv_dist_sm_3 * smooth_factor_klahr;
+            const int o = rmed < B.r[k] ? 0 : 2;
+            acc[k][o] += fx;
+            acc[k][o + 1] += fy;
+        }
+    }
+    __shared__ double wsum[4][kForceMaxBodies * 4];
+    const int t = threadIdx.y * blockDim.x + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < kForceMaxBodies; ++k) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            double v = acc[k][c];
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+            if ((t & 63) == 0) wsum[t >> 6][4 * k + c] = v;
+        }
+    }
+    __syncthreads();
+    if (t < kForceMaxBodies * 4)
+        part[(size_t)(blockIdx.y * gridDim.x + blockIdx.x) * (kForceMaxBodies * 4) + t] =
+            (wsum[0][t] + wsum[1][t]) + (wsum[2][t] + wsum[3][t]);
+}
+
+__global__ void __launch_bounds__(64) k_disk_force_final(const double *part, int nblocks, double *out)
+{
+    const int t = threadIdx.x;
+    if (t >= kForceMaxBodies * 4) return;
+    double s = 0.0;
+    for (int b = 0; b < nblocks; ++b) s += part[(size_t)b * (kForceMaxBodies * 4) + t];
+    out[t] = s;
+}
+
+void launch_disk_force(const KParams &P, hipStream_t st, const ForceBodies &B, double *sig1d,
+                       double *part, double *out4)
+{
+    if (B.flags & FCPT_FORCE_CORRECT_SELFGRAVITY)
+        LAUNCH(k_sigma_ring_mean, dim3((unsigned)P.nr), dim3(256), st, P, sig1d);
+    const dim3 g((unsigned)P.cfl_gx, (unsigned)P.cfl_gy);
+    LAUNCH(k_disk_force, g, kBlock, st, P, B, sig1d, part);
+    hipLaunchKernelGGL(k_disk_force_final, dim3(1), dim3(64), 0, st, part, (int)(g.x * g.y), out4);
+}
+
+// ---------------------------------------------------------------------------
 // Batched step: many disks per launch.  Each stage of one sim::run iteration
 // is one launch over a work table listing, for every disk that takes part in
 // the stage, every block of that disk's own grid.  A disk's blocks do exactly

@@ -47,6 +47,18 @@ int radial_tile_for(int nr, int ns, int requested);
 // fills the launch-shape fields of a disk's parameters (the CFL grid)
 void init_launch_fields(KParams &P);
 
+// The disk's gravity on up to kForceMaxBodies bodies (ComputeDiskOnPlanetAccel)
+constexpr int kForceMaxBodies = FCPT_MAX_PLANETS;
+struct ForceBodies {
+    int n, flags;
+    double x[kForceMaxBodies], y[kForceMaxBodies], r[kForceMaxBodies], r_cubic[kForceMaxBodies];
+    double smooth_fixed[kForceMaxBodies];  // >= 0: that smoothing length, < 0: thickness x H(cell)
+};
+// sig1d: nr doubles (ring means of Sigma), part: kCflMaxBlocks x 4 kForceMaxBodies
+// doubles, out4: 4 kForceMaxBodies doubles (inner x, inner y, outer x, outer y per body)
+void launch_disk_force(const KParams &P, hipStream_t st, const ForceBodies &B, double *sig1d,
+                       double *part, double *out4);
+
 // Batched step: one work-table entry per block of one disk's grid
 struct BatchEntry {
     unsigned disk, yx;  // yx = by << 16 | bx

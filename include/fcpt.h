@@ -248,6 +248,27 @@ int fcpt_batch_time_steps(fcpt_batch *batch, int32_t n_steps, int32_t use_graph,
    (FCPT_KERNEL_*; the listed kernels do not modify their inputs) */
 int fcpt_batch_time_kernel(fcpt_batch *batch, int32_t kernel, int32_t reps, double *ms_per_launch);
 
+/* ---- the disk's gravity on the N bodies ------------------------------- */
+/* One body as ComputeDiskOnPlanetAccel reads it (Force.cpp:23-122). */
+typedef struct fcpt_force_body {
+    double x, y;        /* position in the hydro frame */
+    double r;           /* planet.get_r(): hypot(x, y) */
+    double r_cubic;     /* Klahr-Kley radius l1 * CubicSmoothingFactor; 0 disables */
+} fcpt_force_body;
+/* compute_smoothing options (Force.cpp:145-159) and CorrectDiskSelfgravity
+   (Force.cpp:64-66; the reference's default without self-gravity) */
+enum { FCPT_FORCE_NO_STAR_SMOOTHING = 1, FCPT_FORCE_SMOOTHING_PLANETLOC = 2,
+       FCPT_FORCE_CORRECT_SELFGRAVITY = 4 };
+/* ComputeDiskOnPlanetAccel(t_data&, unsigned nb) for nb = 0 .. n_bodies-1
+   (Force.h:6, Force.cpp:23-122; with FCPT_FORCE_CORRECT_SELFGRAVITY preceded
+   by ComputeAverageDensity, Pframeforce.cpp:174-187, as in
+   ComputeDiskOnNbodyAccel, Pframeforce.cpp:194-220): the acceleration of each
+   body by the disk over the active rings, accel_xy[2k] = x, accel_xy[2k+1] = y.
+   All bodies in one pass over Sigma.  Sums are formed in a fixed tree order
+   (the reference's OpenMP reduction order depends on its thread count). */
+int fcpt_disk_on_planet_accel(fcpt_ctx *ctx, const fcpt_force_body *bodies, int32_t n_bodies,
+                              int32_t flags, double *accel_xy);
+
 /* Compensated (Neumaier) totals over active rings with Rmed <= radius
    (quantities::gas_total_mass / gas_angular_momentum, quantities.cpp:51-78,242-279). */
 int fcpt_totals(fcpt_ctx *ctx, double radius, double *mass, double *angular_momentum);

@@ -12,7 +12,8 @@ import subprocess
 
 import numpy as np
 
-from fargocpt_amd.capi import (FIELD_ID, DiskDesc, desc_from_dict, field_shape,
+from fargocpt_amd.capi import (FIELD_ID, FORCE_CORRECT_SELFGRAVITY, DiskDesc, ForceBody,
+                               desc_from_dict, field_shape, make_force_bodies,
                                make_step_params, StepParams)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -61,6 +62,7 @@ def load():
         "fo_step_euler": ([P, D], None),
         "fo_run_step": ([P], D),
         "fo_totals": ([P, D, dp, dp], None),
+        "fo_disk_on_planet_accel": ([P, C.POINTER(ForceBody), I, I, dp], None),
     }
     for name, (args, res) in spec.items():
         fn = getattr(lib, name)
@@ -185,6 +187,12 @@ class OracleDisk:
         m, l = C.c_double(), C.c_double()
         self.lib.fo_totals(self.h, radius, C.byref(m), C.byref(l))
         return m.value, l.value
+
+    def disk_on_planet_accel(self, bodies, flags=FORCE_CORRECT_SELFGRAVITY):
+        arr = make_force_bodies(bodies)
+        out = np.zeros(2 * max(1, len(bodies)))
+        self.lib.fo_disk_on_planet_accel(self.h, arr, len(bodies), flags, _dptr(out))
+        return out[:2 * len(bodies)].reshape(-1, 2)
 
     def synchronize(self):
         pass

@@ -1235,3 +1235,70 @@ void fo_totals(fo_disk *d, double radius, double *mass, double *angmom)
     *mass = m + mc;
     *angmom = l + lc;
 }
+
+/* ComputeAverageDensity (Pframeforce.cpp:174-187) into a ring array, then
+   ComputeDiskOnPlanetAccel (Force.cpp:23-122) for every body, with the
+   reference's sequential order of a one-thread run: for each ring of
+   radial_first_active .. radial_active_size-1, each sector in order, the
+   inner (Rmed < a) or outer accumulator; compute_smoothing, Force.cpp:145-159 */
+void fo_disk_on_planet_accel(fo_disk *d, const fcpt_force_body *bodies, int n_bodies, int flags,
+                             double *accel_xy)
+{
+    const int nr = d->nr, ns = d->ns;
+    const double *S = d->f[FCPT_SIGMA], *H = d->f[FCPT_SCALE_HEIGHT];
+    const double G = d->p.G;
+    double *sigma1d = calloc((size_t)nr, sizeof(double));
+    if (flags & FCPT_FORCE_CORRECT_SELFGRAVITY) {
+        for (int i = 1; i < nr - 1; ++i) {
+            double sum = 0;
+            for (int j = 0; j < ns; ++j) sum += S[IDX(i, j)];
+            sigma1d[i] = sum / ns;
+        }
+    }
+    for (int nb = 0; nb < n_bodies; ++nb) {
+        const double x = bodies[nb].x, y = bodies[nb].y, a = bodies[nb].r;
+        const double r_sm = bodies[nb].r_cubic;
+        double axi = 0.0, ayi = 0.0, axo = 0.0, ayo = 0.0;
+        for (int i = 1; i < nr - 1; ++i) {
+            for (int j = 0; j < ns; ++j) {
+                double smooth;
+                if ((flags & FCPT_FORCE_NO_STAR_SMOOTHING) && nb == 0) {
+                    smooth = 0;
+                } else if (flags & FCPT_FORCE_SMOOTHING_PLANETLOC) {
+                    const double scale_height =
+                        d->p.aspect_ratio * pow(a, 1 + d->p.flaring_index);
+                    smooth = d->p.thickness_smoothing * scale_height;
+                } else {
+                    smooth = d->p.thickness_smoothing * H[IDX(i, j)];
+                }
+                const double xc = d->Rmed[i] * d->cosphi[j];
+                const double yc = d->Rmed[i] * d->sinphi[j];
+                double cell_sigma = S[IDX(i, j)];
+                if (flags & FCPT_FORCE_CORRECT_SELFGRAVITY) cell_sigma -= sigma1d[i];
+                const double cellmass = d->Surf[i] * cell_sigma;
+                const double dx = xc - x;
+                const double dy = yc - y;
+                const double dist_2 = pow(dx, 2) + pow(dy, 2);
+                const double dist_sm_2 = dist_2 + pow(smooth, 2);
+                const double dist_sm = sqrt(dist_sm_2);
+                const double dist_sm_3 = dist_sm_2 * dist_sm;
+                const double inv_dist_sm_3 = 1.0 / dist_sm_3;
+                double smooth_factor_klahr = 1.0;
+                if (r_sm > 0.0 && dist_sm < r_sm) {
+                    smooth_factor_klahr =
+                        -(3.0 * pow(dist_sm / r_sm, 4.0) - 4.0 * pow(dist_sm / r_sm, 3.0));
+                }
+                if (d->Rmed[i] < a) {
+                    axi += G * cellmass * dx * inv_dist_sm_3 * smooth_factor_klahr;
+                    ayi += G * cellmass * dy * inv_dist_sm_3 * smooth_factor_klahr;
+                } else {
+                    axo += G * cellmass * dx * inv_dist_sm_3 * smooth_factor_klahr;
+                    ayo += G * cellmass * dy * inv_dist_sm_3 * smooth_factor_klahr;
+                }
+            }
+        }
+        accel_xy[2 * nb] = axi + axo;
+        accel_xy[2 * nb + 1] = ayi + ayo;
+    }
+    free(sigma1d);
+}

@@ -55,6 +55,8 @@ const int32_t *fo_nshift(const fo_disk *d);
 void fo_step_euler(fo_disk *d, double dt);
 double fo_run_step(fo_disk *d);
 void fo_totals(fo_disk *d, double radius, double *mass, double *angmom);
+void fo_disk_on_planet_accel(fo_disk *d, const fcpt_force_body *bodies, int n_bodies, int flags,
+                             double *accel_xy);
 
 #ifdef __cplusplus
 }

@@ -211,6 +211,8 @@ def make_case(name, exe, addr, keep_dir=None):
         npl = meta["n_planets"]
         out["planets"] = load_bin(os.path.join(dump, "planets.bin")).reshape(nsteps, npl, 4)
         out["nshift"] = load_bin(os.path.join(dump, "nshift.bin"), np.int32).reshape(nsteps, nr)
+        # the disk's gravity on every body after each step (x, y, r, Klahr radius, ax, ay)
+        out["diskacc"] = load_bin(os.path.join(dump, "diskacc.bin")).reshape(nsteps, npl, 6)
         shapes = {"sigma": (nr, nphi), "vrad": (nr + 1, nphi), "vazi": (nr, nphi),
                   "energy": (nr, nphi), "qplus": (nr, nphi), "qminus": (nr, nphi)}
         for f in ("sigma0", "vrad0", "vazi0", "energy0"):

@@ -21,6 +21,8 @@
 //   steps.bin               per step: 16 float64 values (see STEP_RECORD)
 //   planets.bin             per step: n_planets x 4 float64 (x, y, m, r_cubic)
 //   nshift.bin              per step: Nr int32 FARGO shifts
+//   diskacc.bin             per step, after it: n_planets x 6 float64 (x, y, r, Klahr
+//                           radius, ax, ay) of ComputeDiskOnPlanetAccel, one thread
 //   state_<step>_<f>.bin    checkpoints (f = sigma, vrad, vazi, energy, qplus, qminus)
 //   stage_<step>_<k>_<name>_<f>.bin   per-stage fields for the first steps
 #include <cstdio>
@@ -30,6 +32,9 @@
 #include <vector>
 #include <set>
 
+#include <omp.h>
+
+#include "Force.h"
 #include "Interpret.h"
 #include "LowTasks.h"
 #include "Pframeforce.h"
@@ -222,6 +227,36 @@ static void euler_step(t_data &data, double dt, unsigned step, bool stages,
     if (stages) dump_stage(data, step, k++, "derived");
 }
 
+// The disk's gravity on every body after a step, as ComputeDiskOnNbodyAccel
+// obtains it (Pframeforce.cpp:194-220), with one OpenMP thread so that the
+// reduction runs in the loop order; the planets' own state is not touched.
+static void dump_disk_accel(t_data &data)
+{
+    const int nt = omp_get_max_threads();
+    omp_set_num_threads(1);
+    if (parameters::correct_disk_selfgravity) {
+	ComputeAverageDensity(data);
+    }
+    std::vector<double> rec;
+    t_planetary_system &psys = data.get_planetary_system();
+    for (unsigned k = 0; k < psys.get_number_of_planets(); ++k) {
+	t_planet &pl = psys.get_planet(k);
+	const double c = pl.get_cubic_smoothing_factor();
+	const double rsm = c > 0.0 ? pl.get_dimensionless_roche_radius() *
+					 pl.get_distance_to_primary() * c
+				   : 0.0;
+	const Pair a = ComputeDiskOnPlanetAccel(data, k);
+	rec.push_back(pl.get_x());
+	rec.push_back(pl.get_y());
+	rec.push_back(pl.get_r());
+	rec.push_back(rsm);
+	rec.push_back(a.x);
+	rec.push_back(a.y);
+    }
+    omp_set_num_threads(nt);
+    append_raw("diskacc.bin", rec.data(), sizeof(double) * rec.size());
+}
+
 int main(int argc, char **argv)
 {
     // refdump <config.yml> <outdir> <icdir|-> <nsteps> <stage_steps> <checkpoints,..> [nshift_addr|0] [simrun]
@@ -351,6 +386,9 @@ int main(int argc, char **argv)
 	fprintf(f, "cooling_beta_reference = %d\ncooling_beta_ramp_up = %.17g\n",
 		parameters::cooling_beta_reference ? 1 : 0, parameters::cooling_beta_ramp_up);
 	fprintf(f, "heating_cooling_cfl_limit = %.17g\n", parameters::HEATING_COOLING_CFL_LIMIT);
+	fprintf(f, "correct_disk_selfgravity = %d\n", (int)parameters::correct_disk_selfgravity);
+	fprintf(f, "compatibility_no_star_smoothing = %d\n", (int)parameters::compatibility_no_star_smoothing);
+	fprintf(f, "compatibility_smoothing_planetloc = %d\n", (int)parameters::compatibility_smoothing_planetloc);
 	fclose(f);
     }
     write_raw("geom_Radii.bin", &Radii[0], sizeof(double) * (Nr + 1));
@@ -420,6 +458,7 @@ int main(int argc, char **argv)
 	rec[9] = towrite ? 1.0 : 0.0;
 	append_raw("steps.bin", rec, sizeof(rec));
 	append_raw("planets.bin", planet_rec.data(), sizeof(double) * planet_rec.size());
+	dump_disk_accel(data);
 	if (checkpoints.count(step)) {
 	    dump_state(data, std::to_string(step));
 	}

@@ -147,3 +147,63 @@ def test_totals_match_naive_sum(oracle_lib):
     l_ref = np.sum((surf * rmed)[act, None] * sl[act] * (vazi[act] + rmed[act, None]))
     assert abs(m - m_ref) / m_ref < 1e-13
     assert abs(l - l_ref) / abs(l_ref) < 1e-13
+
+
+def _force_flags(meta):
+    from fargocpt_amd.capi import (FORCE_CORRECT_SELFGRAVITY, FORCE_NO_STAR_SMOOTHING,
+                                   FORCE_SMOOTHING_PLANETLOC)
+    return ((FORCE_CORRECT_SELFGRAVITY if meta.get("correct_disk_selfgravity", 1) else 0) |
+            (FORCE_NO_STAR_SMOOTHING if meta.get("compatibility_no_star_smoothing", 0) else 0) |
+            (FORCE_SMOOTHING_PLANETLOC if meta.get("compatibility_smoothing_planetloc", 0) else 0))
+
+
+@pytest.mark.parametrize("case", ["c1_small", "c5_small", "mix_small"])
+def test_disk_force_bit_exact(case, oracle_lib):
+    """ComputeDiskOnPlanetAccel after every step of the reference run (one
+    OpenMP thread, loop order), recomputed by the oracle from its own replay."""
+    from fixtures import start_run, step_inputs
+    meta, data = load_case(case)
+    d = OracleDisk(disk_config(meta))
+    start_run(d, meta)
+    acc = data["diskacc"]
+    flags = _force_flags(meta)
+    for s in range(1, meta["nsteps"] + 1):
+        planets, ind = step_inputs(data, s)
+        d.set_step_params(planets, ind)
+        d.run_step()
+        bodies = [tuple(b[:4]) for b in acc[s - 1]]
+        got = d.disk_on_planet_accel(bodies, flags)
+        assert np.array_equal(got, acc[s - 1][:, 4:6]), (s, got, acc[s - 1][:, 4:6])
+
+
+def test_disk_force_closed_forms(oracle_lib):
+    """One massive cell: the point-mass acceleration G m d / (d^2 + eps^2)^1.5;
+    a ring-uniform disk with the self-gravity correction (Force.cpp:64
+    subtracts the ring mean): exactly zero."""
+    from fargocpt_amd.capi import FORCE_CORRECT_SELFGRAVITY
+    meta, d = _small_disk()
+    nr, ns = meta["Nr"], meta["Nphi"]
+    ic = initial_condition(dict(meta))
+    sigma = np.zeros((nr, ns))
+    i0, j0 = nr // 2, ns // 3
+    sigma[i0, j0] = 2.5e-4
+    d.upload_state(sigma, ic["vrad"], ic["vazi"], ic["energy"])
+    H = d.download_field("SCALE_HEIGHT")
+    surf = d.geom("Surf", nr)
+    rmed = d.geom("Rmed", nr)
+    phi = 2 * np.pi * j0 / ns
+    xc, yc = rmed[i0] * np.cos(phi), rmed[i0] * np.sin(phi)
+    body = (0.9, -0.4, float(np.hypot(0.9, -0.4)), 0.0)
+    got = d.disk_on_planet_accel([body], 0)[0]
+    eps = 0.6 * H[i0, j0]
+    dx, dy = xc - body[0], yc - body[1]
+    r3 = (dx * dx + dy * dy + eps * eps) ** 1.5
+    m = surf[i0] * 2.5e-4
+    np.testing.assert_allclose(got, [meta["G"] * m * dx / r3, meta["G"] * m * dy / r3], rtol=1e-13)
+    # sign: the disk pulls the body towards the cell
+    assert got[0] * dx > 0 and got[1] * dy > 0
+    # ring values k * 2^-17: every partial sum of the ring mean is exact
+    ring = np.repeat((1.0 + np.arange(nr))[:, None] * 2.0 ** -17, ns, axis=1)
+    d.upload_state(ring, ic["vrad"], ic["vazi"], ic["energy"])
+    zero = d.disk_on_planet_accel([(0.0, 0.0, 0.0, 0.0), body], FORCE_CORRECT_SELFGRAVITY)
+    assert np.all(zero == 0.0)
