@@ -168,7 +168,7 @@ def main() -> int:
         return 2
 
     from fargocpt_amd import workloads as W
-    from fargocpt_amd.capi import KERNELS
+    from fargocpt_amd.capi import KERNEL_DISK_FORCE, KERNELS
 
     if args.workload in W.BATCHES:
         return bench_batch(args)
@@ -188,6 +188,10 @@ def main() -> int:
     dom = max(kt, key=kt.get)
     achieved = W.algorithmic_bytes(dom, adi, el) * cells / (kt[dom] * 1e-3) / 1e9
     copy_ms, copy_bytes = disk.time_copy(20)
+    # the disk's gravity on star + planet (ComputeDiskOnPlanetAccel), once per step
+    # in the reference's N-body coupling
+    force_ms = disk.time_kernel(KERNEL_DISK_FORCE, 20)
+    force_bytes = W.algorithmic_bytes("disk_force", adi, el) * cells
     traffic = None
     prof = os.path.join(ROOT, "profiles", f"traffic_{name}_{args.precision}.json")
     if os.path.exists(prof):
@@ -218,6 +222,10 @@ def main() -> int:
                  "frac_of_hbm_peak": W.step_bytes(adi, el) * value / 1e9 / HBM_PEAK_GBS,
                  "kernel_ms": kt,
                  "copy_probe_GBs": copy_bytes / (copy_ms * 1e-3) / 1e9},
+        "disk_force": {"bodies": 2, "ms_per_call": force_ms,
+                       "bytes_per_cell": W.algorithmic_bytes("disk_force", adi, el),
+                       "achieved_GBs": force_bytes / (force_ms * 1e-3) / 1e9,
+                       "frac_of_hbm_peak": force_bytes / (force_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
     }
     if not args.no_compare:
         other = 1 - prec

@@ -24,6 +24,7 @@ VECTOR_FIELDS = {"VRAD", "VRAD0", "TAU_R_PHI"}
 
 ARTVISC = {"none": 0, "tw": 1, "sn": 2}
 KERNELS = {"radial_sweep": 0, "azimuthal_sweep": 1, "cfl": 3, "stress": 4, "potential": 5}
+KERNEL_DISK_FORCE = 6  # the disk-on-planet reduction (not part of the hydro step)
 # storage modes (fcpt_disk_desc.precision) and launch parameters of include/fcpt.h
 PREC_FP64, PREC_SPLIT32 = 0, 1
 PARAM_RADIAL_TILE, PARAM_RADIAL_WAVES = 0, 1

@@ -793,8 +793,17 @@ int fcpt_time_kernel(fcpt_ctx *ctx, int32_t kernel, int32_t reps, double *ms_per
     int rc = ensure_events(ctx);
     if (rc) return rc;
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    ForceBodies fb{};
+    fb.n = 2;
+    fb.flags = FCPT_FORCE_CORRECT_SELFGRAVITY;
+    fb.x[1] = fb.r[1] = 1.0;
+    fb.smooth_fixed[0] = fb.smooth_fixed[1] = -1.0;
     auto one = [&]() {
         switch (kernel) {
+        case FCPT_KERNEL_DISK_FORCE:
+            launch_disk_force(ctx->P, ctx->stream, fb, ctx->force_sig1d, ctx->force_part,
+                              ctx->force_out);
+            break;
         case FCPT_KERNEL_RADIAL_SWEEP: launch_radial_sweep(ctx->P, ctx->stream); break;
         case FCPT_KERNEL_AZIMUTHAL: launch_azimuthal_sweep(ctx->P, ctx->stream); break;
         case FCPT_KERNEL_CFL: launch_cfl_cells(ctx->P, ctx->stream); break;
@@ -803,7 +812,7 @@ int fcpt_time_kernel(fcpt_ctx *ctx, int32_t kernel, int32_t reps, double *ms_per
         default: break;
         }
     };
-    if (kernel < 0 || kernel > FCPT_KERNEL_POTENTIAL || kernel == 2)
+    if (kernel < 0 || kernel > FCPT_KERNEL_DISK_FORCE || kernel == 2)
         return fail(ctx, FCPT_ERR_ARG, "unknown kernel");
     one();  // warm-up
     HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));

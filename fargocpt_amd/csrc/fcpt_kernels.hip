@@ -1828,58 +1828,84 @@ __global__ void __launch_bounds__(256) k_sigma_ring_mean(KParams P, double *sig1
 
 // Per cell and body, the reference's expression (the smoothing of
 // compute_smoothing, Force.cpp:145-159, chosen by the flags; the Klahr-Kley
-// factor of the force); inner (Rmed < r_body) and outer sums kept apart as in
-// the reference.  Rows are walked with a grid stride; each block reduces its
-// 4 x n partial sums with wavefront shuffles and LDS into its own slot, and
-// k_disk_force_final folds the slots in order: deterministic, no atomics.
-template <bool SPLIT>
+// factor 4q^3 - 3q^4 inside r_cubic); inner (Rmed < r_body) and outer sums
+// kept apart as in the reference.  NB bodies per launch (a template argument,
+// so only their accumulators live in registers).  A thread owns one sector
+// and walks the rings with a grid stride, loading the next ring's cell before
+// it works on the current one; 1 / d^3 comes from the reciprocal square root
+// with two Newton steps (a few ulp from the reference's sqrt and division).
+// Each block folds its 4 NB partial sums with wavefront shuffles and LDS into
+// its own slot (component-major, slot = block), and k_disk_force_final sums
+// each component's slots in a fixed order: deterministic, no atomics.
+template <bool SPLIT, int NB>
 __global__ void __launch_bounds__(256) k_disk_force(KParams P, ForceBodies B, const double *sig1d,
                                                     double *part)
 {
     const int ns = P.ns;
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    double acc[kForceMaxBodies][4];
+    const int nslots = (int)(gridDim.x * gridDim.y);
+    double acc[NB][4];
 #pragma unroll
-    for (int k = 0; k < kForceMaxBodies; ++k) acc[k][0] = acc[k][1] = acc[k][2] = acc[k][3] = 0.0;
+    for (int k = 0; k < NB; ++k) acc[k][0] = acc[k][1] = acc[k][2] = acc[k][3] = 0.0;
     FLD(S, FCPT_SIGMA);
     FLD(Hf, FCPT_SCALE_HEIGHT);
     const double G = P.G;
+    const double thick = P.thickness_smoothing;
     const bool correct = (B.flags & FCPT_FORCE_CORRECT_SELFGRAVITY) != 0;
-    for (int i = wave_ring(1, (int)blockIdx.y); i < P.nr - 1 && j < ns; i += P.cfl_gy * (int)blockDim.y) {
-        const double sig = S(i, j);
-        const double H = P.adiabatic ? Hf(i, j) : cst(P.iso_h)[i];
-        const double rmed = cst(P.Rmed)[i];
-        const double xc = rmed * P.cosphi[j];
-        const double yc = rmed * P.sinphi[j];
-        double cell_sigma = sig;
-        if (correct) cell_sigma -= cst(sig1d)[i];
-        const double cellmass = cst(P.Surf)[i] * cell_sigma;
+    const int last = P.nr - 1;  // rings 1 .. nr - 2
+    const int stride = P.cfl_gy * (int)blockDim.y;
+    int i = wave_ring(1, (int)blockIdx.y);
+    if (j < ns && i < last) {
+        const double cphi = P.cosphi[j];
+        const double sphi = P.sinphi[j];
+        double sig = S(i, j);
+        double H = P.adiabatic ? Hf(i, j) : 0.0;
+        for (;;) {
+            const int in = i + stride;
+            const int ip = in < last ? in : i;
+            const double sig_n = S(ip, j);
+            const double H_n = P.adiabatic ? Hf(ip, j) : 0.0;
+            const double rmed = cst(P.Rmed)[i];
+            const double Hc = P.adiabatic ? H : cst(P.iso_h)[i];
+            const double xc = rmed * cphi;
+            const double yc = rmed * sphi;
+            const double cell_sigma = correct ? sig - cst(sig1d)[i] : sig;
+            const double gm = G * (cst(P.Surf)[i] * cell_sigma);
 #pragma unroll
-        for (int k = 0; k < kForceMaxBodies; ++k) {
-            if (k >= B.n) break;
-            const double smooth = B.smooth_fixed[k] >= 0.0 ? B.smooth_fixed[k] : P.thickness_smoothing * H;
-            const double dx = xc - B.x[k];
-            const double dy = yc - B.y[k];
-            const double dist_2 = dx * dx + dy * dy;
-            const double dist_sm_2 = dist_2 + smooth * smooth;
-            const double dist_sm = sqrt(dist_sm_2);
-            const double dist_sm_3 = dist_sm_2 * dist_sm;
-            const double inv_dist_sm_3 = 1.0 / dist_sm_3;
-            double smooth_factor_klahr = 1.0;
-            const double r_sm = B.r_cubic[k];
-            if (r_sm > 0.0 && dist_sm < r_sm)
-                smooth_factor_klahr = -(3.0 * pow(dist_sm / r_sm, 4.0) - 4.0 * pow(dist_sm / r_sm, 3.0));
-            const double fx = G * cellmass * dx * inv_dist_sm_3 * smooth_factor_klahr;
-            const double fy = G * cellmass * dy * inv_dist_sm_3 * smooth_factor_klahr;
-            const int o = rmed < B.r[k] ? 0 : 2;
-            acc[k][o] += fx;
-            acc[k][o + 1] += fy;
+            for (int k = 0; k < NB; ++k) {
+                const double smooth = B.smooth_fixed[k] >= 0.0 ? B.smooth_fixed[k] : thick * Hc;
+                const double dx = xc - B.x[k];
+                const double dy = yc - B.y[k];
+                const double d2 = (dx * dx + dy * dy) + smooth * smooth;
+                const double inv_d = inv_sqrt<true>(d2);
+                const double inv_d3 = inv_d * inv_d * inv_d;
+                double fx = gm * dx * inv_d3;
+                double fy = gm * dy * inv_d3;
+                const double r_sm = B.r_cubic[k];
+                if (r_sm > 0.0) {  // uniform: the body's cubic smoothing radius
+                    const double q = (d2 * inv_d) / r_sm;
+                    const double klahr = q < 1.0 ? q * q * q * (4.0 - 3.0 * q) : 1.0;
+                    fx *= klahr;
+                    fy *= klahr;
+                }
+                if (rmed < B.r[k]) {  // uniform: one ring against one body
+                    acc[k][0] += fx;
+                    acc[k][1] += fy;
+                } else {
+                    acc[k][2] += fx;
+                    acc[k][3] += fy;
+                }
+            }
+            if (in >= last) break;
+            i = in;
+            sig = sig_n;
+            H = H_n;
         }
     }
-    __shared__ double wsum[4][kForceMaxBodies * 4];
+    __shared__ double wsum[4][NB * 4];
     const int t = threadIdx.y * blockDim.x + threadIdx.x;
 #pragma unroll
-    for (int k = 0; k < kForceMaxBodies; ++k) {
+    for (int k = 0; k < NB; ++k) {
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
             double v = acc[k][c];
@@ -1888,28 +1914,58 @@ __global__ void __launch_bounds__(256) k_disk_force(KParams P, ForceBodies B, co
         }
     }
     __syncthreads();
-    if (t < kForceMaxBodies * 4)
-        part[(size_t)(blockIdx.y * gridDim.x + blockIdx.x) * (kForceMaxBodies * 4) + t] =
+    if (t < NB * 4)
+        part[(size_t)t * nslots + (blockIdx.y * gridDim.x + blockIdx.x)] =
             (wsum[0][t] + wsum[1][t]) + (wsum[2][t] + wsum[3][t]);
 }
 
-__global__ void __launch_bounds__(64) k_disk_force_final(const double *part, int nblocks, double *out)
+// one block per component: the slots summed with a 256-wide stride, then a
+// pairwise tree over the 256 partial sums
+__global__ void __launch_bounds__(256) k_disk_force_final(const double *part, int nslots, double *out)
 {
-    const int t = threadIdx.x;
-    if (t >= kForceMaxBodies * 4) return;
+    __shared__ double sh[256];
+    const double *p = part + (size_t)blockIdx.x * nslots;
     double s = 0.0;
-    for (int b = 0; b < nblocks; ++b) s += part[(size_t)b * (kForceMaxBodies * 4) + t];
-    out[t] = s;
+    for (int b = threadIdx.x; b < nslots; b += 256) s += p[b];
+    s = block_sum<256>(s, sh);
+    if (threadIdx.x == 0) out[blockIdx.x] = s;
+}
+
+template <bool SPLIT, int NB>
+static void launch_force_nb(const KParams &P, hipStream_t st, const ForceBodies &B, const double *sig1d,
+                            double *part, dim3 g)
+{
+    hipLaunchKernelGGL((k_disk_force<SPLIT, NB>), g, kBlock, 0, st, P, B, sig1d, part);
+}
+
+template <bool SPLIT>
+static void launch_force_split(const KParams &P, hipStream_t st, const ForceBodies &B,
+                               const double *sig1d, double *part, dim3 g)
+{
+    switch (B.n) {
+    case 1: launch_force_nb<SPLIT, 1>(P, st, B, sig1d, part, g); break;
+    case 2: launch_force_nb<SPLIT, 2>(P, st, B, sig1d, part, g); break;
+    case 3: launch_force_nb<SPLIT, 3>(P, st, B, sig1d, part, g); break;
+    case 4: launch_force_nb<SPLIT, 4>(P, st, B, sig1d, part, g); break;
+    case 5: launch_force_nb<SPLIT, 5>(P, st, B, sig1d, part, g); break;
+    case 6: launch_force_nb<SPLIT, 6>(P, st, B, sig1d, part, g); break;
+    case 7: launch_force_nb<SPLIT, 7>(P, st, B, sig1d, part, g); break;
+    default: launch_force_nb<SPLIT, 8>(P, st, B, sig1d, part, g); break;
+    }
 }
 
 void launch_disk_force(const KParams &P, hipStream_t st, const ForceBodies &B, double *sig1d,
                        double *part, double *out4)
 {
+    static_assert(kForceMaxBodies == 8, "launch_force_split dispatches 1 .. 8 bodies");
+    if (B.n < 1 || B.n > kForceMaxBodies) return;
     if (B.flags & FCPT_FORCE_CORRECT_SELFGRAVITY)
         LAUNCH(k_sigma_ring_mean, dim3((unsigned)P.nr), dim3(256), st, P, sig1d);
     const dim3 g((unsigned)P.cfl_gx, (unsigned)P.cfl_gy);
-    LAUNCH(k_disk_force, g, kBlock, st, P, B, sig1d, part);
-    hipLaunchKernelGGL(k_disk_force_final, dim3(1), dim3(64), 0, st, part, (int)(g.x * g.y), out4);
+    if (P.split) launch_force_split<true>(P, st, B, sig1d, part, g);
+    else launch_force_split<false>(P, st, B, sig1d, part, g);
+    hipLaunchKernelGGL(k_disk_force_final, dim3((unsigned)(4 * B.n)), dim3(256), 0, st, part,
+                       (int)(g.x * g.y), out4);
 }
 
 // ---------------------------------------------------------------------------

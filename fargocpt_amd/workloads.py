@@ -158,6 +158,7 @@ def algorithmic_bytes(kernel: str, adiabatic: bool, element: int = 8) -> int:
         "cfl": (4 + 3 * e, 0),                 # v_r, v_phi, c_s, nu, (Q+, Q-, E)
         "stress": (4, 4),                      # v_r, v_phi, nu, Sigma -> div v, tau_rr, tau_pp, tau_rp
         "potential": (1, 1),                   # H -> Phi
+        "disk_force": (2 + e, 0),              # Sigma (ring means, then cells), (H)
     }
     r, w = reads_writes[kernel]
     return element * (r + w)

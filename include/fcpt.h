@@ -196,9 +196,13 @@ int fcpt_run_steps(fcpt_ctx *ctx, int32_t n_steps, int32_t use_graph);
    n_steps fcpt_run_steps iterations, from HIP events on the context's stream;
    and the mean duration of one launch of a single kernel of the step, timed
    the same way over `reps` back-to-back launches on the current state
-   (FCPT_KERNEL_* below; the listed kernels do not modify their inputs). */
+   (FCPT_KERNEL_* below; the listed kernels do not modify their inputs).
+   FCPT_KERNEL_DISK_FORCE times the launches of fcpt_disk_on_planet_accel
+   (ring means, cell reduction, block reduction; no copy to the host) for two
+   bodies, a star at the origin and a planet at (1, 0), with the self-gravity
+   correction. */
 enum { FCPT_KERNEL_RADIAL_SWEEP = 0, FCPT_KERNEL_AZIMUTHAL = 1, FCPT_KERNEL_CFL = 3,
-       FCPT_KERNEL_STRESS = 4, FCPT_KERNEL_POTENTIAL = 5 };
+       FCPT_KERNEL_STRESS = 4, FCPT_KERNEL_POTENTIAL = 5, FCPT_KERNEL_DISK_FORCE = 6 };
 int fcpt_time_steps(fcpt_ctx *ctx, int32_t n_steps, int32_t use_graph, double *ms_total);
 int fcpt_time_kernel(fcpt_ctx *ctx, int32_t kernel, int32_t reps, double *ms_per_launch);
 /* Streaming-copy probe (no counterpart in the reference): copies four
@@ -245,7 +249,8 @@ int fcpt_batch_run_steps(fcpt_batch *batch, int32_t n_steps, int32_t use_graph);
 /* as fcpt_batch_run_steps, timed with HIP events on the batch's stream */
 int fcpt_batch_time_steps(fcpt_batch *batch, int32_t n_steps, int32_t use_graph, double *ms_total);
 /* mean duration of one batched launch of one step kernel over all disks
-   (FCPT_KERNEL_*; the listed kernels do not modify their inputs) */
+   (FCPT_KERNEL_* except FCPT_KERNEL_DISK_FORCE; the listed kernels do not
+   modify their inputs) */
 int fcpt_batch_time_kernel(fcpt_batch *batch, int32_t kernel, int32_t reps, double *ms_per_launch);
 
 /* ---- the disk's gravity on the N bodies ------------------------------- */
