@@ -250,6 +250,15 @@ class Disk:
         arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (sigma0, vrad0, vazi0, energy0)]
         self._check(self.lib.fcpt_upload_reference(self.h, *[_dptr(a) for a in arrs]))
 
+    def download_state(self):
+        """(sigma, vrad, vazi, energy) as FP64 host arrays (energy None when
+        isothermal)."""
+        out = [np.empty(field_shape(f, self.nr, self.nphi)) for f in ("SIGMA", "VRAD", "VAZI")]
+        e = np.empty(field_shape("ENERGY", self.nr, self.nphi)) if self.desc.adiabatic else None
+        self._check(self.lib.fcpt_download_state(self.h, *[_dptr(a) for a in out],
+                                                 _dptr(e) if e is not None else None))
+        return (*out, e)
+
     def upload_field(self, name, arr):
         a = np.ascontiguousarray(arr, dtype=np.float64)
         assert a.shape == field_shape(name, self.nr, self.nphi), (name, a.shape)

@@ -1,0 +1,138 @@
+"""Summarise the rocprofv3 runs of one bench.py configuration into a short
+Markdown table and a JSON of HBM bytes (kernel -> bytes per launch, plus the
+bytes of one whole step), the files bench.py reads back as `traffic`.
+
+Three runs of `bench.py --workload W --precision P --skip-cpu --no-compare
+--no-autotune` are expected under DIR: a kernel trace with statistics
+(`DIR/prof_W_P_kt`, 50 timed + 5 warm-up steps), and two counter runs
+(`DIR/prof_W_P_fetch` with FETCH_SIZE, `DIR/prof_W_P_write` with WRITE_SIZE,
+10 + 2 steps).  Byte counters are calibrated on `k_copy4`, which moves
+4 x element x cells bytes each way per launch.
+
+Usage: python -m fargocpt_amd.profsum W P [DIR] [--out profiles]
+"""
+from __future__ import annotations
+
+import argparse
+import collections
+import csv
+import glob
+import json
+import os
+
+# bench.py times these kernels with fcpt_time_kernel: 3 rounds of 1 warm-up + 20 launches
+TIMED = ("k_radial_sweep", "k_azimuthal_sweep", "k_cfl_cells", "k_stress", "k_potential<")
+TIMING_LAUNCHES = 63
+KT_STEPS = 55
+JSON_NAMES = {"k_radial_sweep": "radial_sweep", "k_azimuthal_sweep": "azimuthal_sweep",
+              "k_cfl_cells": "cfl", "k_stress": "stress", "k_potential<": "potential",
+              "k_disk_force<": "disk_force"}
+
+
+def _short(name: str) -> str:
+    return name.replace("fcpt::", "").split("(")[0]
+
+
+def _per_kernel(files) -> dict:
+    agg = collections.defaultdict(list)
+    for f in files:
+        for r in csv.DictReader(open(f)):
+            agg[r["Kernel_Name"]].append(float(r["Counter_Value"]))
+    return {k: sum(v) / len(v) for k, v in agg.items()}
+
+
+def summarise(workload: str, prec: str, base: str, outdir: str) -> dict:
+    from fargocpt_amd import workloads as W
+    cells = W.cells(workload)
+    esize = 8 if prec == "fp64" else 4
+    tag = f"{workload}_{prec}"
+    kt = glob.glob(f"{base}/prof_{tag}_kt/**/*kernel_stats.csv", recursive=True)
+    fe = glob.glob(f"{base}/prof_{tag}_fetch/**/*counter_collection.csv", recursive=True)
+    wr = glob.glob(f"{base}/prof_{tag}_write/**/*counter_collection.csv", recursive=True)
+    cmd = f"python3 bench.py --workload {workload} --precision {prec}"
+    opts = "--skip-cpu --no-compare --no-autotune"
+    L = [f"# rocprofv3 summary: {workload} ({cells} cells), "
+         f"{'FP64' if prec == 'fp64' else 'split'} storage, graph-replayed step\n",
+         "Commands (repository root, one MI355X; default launch shapes, no autotuning):\n",
+         "```",
+         f"rocprofv3 --kernel-trace --stats --output-format csv -d {base}/prof_{tag}_kt -o kt -- "
+         f"{cmd} --steps 50 --warmup 5 {opts}",
+         f"rocprofv3 --pmc FETCH_SIZE --output-format csv -d {base}/prof_{tag}_fetch -o fetch -- "
+         f"{cmd} --steps 10 --warmup 2 {opts}",
+         f"rocprofv3 --pmc WRITE_SIZE --output-format csv -d {base}/prof_{tag}_write -o write -- "
+         f"{cmd} --steps 10 --warmup 2 {opts}",
+         f"python3 -m fargocpt_amd.profsum {workload} {prec} {base}",
+         "```\n"]
+    rows = list(csv.DictReader(open(kt[0])))
+    L += ["## Kernel time (kernel_stats.csv)\n",
+          "| kernel | calls | launches per step | average µs | % of GPU time |",
+          "|---|---|---|---|---|"]
+    per_step = {}
+    avg_us = {}
+    for r in sorted(rows, key=lambda r: -float(r["TotalDurationNs"])):
+        name = r["Name"]
+        calls = int(r["Calls"])
+        extra = TIMING_LAUNCHES if any(t in name for t in TIMED) else 0
+        n = (calls - extra) // KT_STEPS
+        per_step[_short(name)] = n
+        avg_us[_short(name)] = float(r["AverageNs"]) / 1e3
+        L.append(f"| `{_short(name)}` | {calls} | {n} | {avg_us[_short(name)]:.1f} | "
+                 f"{float(r['Percentage']):.1f} |")
+    F = _per_kernel(fe)
+    Wr = _per_kernel(wr)
+    copy = [k for k in F if "k_copy4" in k][0]
+    exp_kib = 4 * esize * cells / 1024.0
+    ff, fw = exp_kib / F[copy], exp_kib / Wr[copy]
+    L += ["\n## HBM bytes per launch (counter_collection.csv of the --pmc runs, mean per kernel)\n",
+          f"Calibration: `k_copy4` moves {exp_kib:.0f} KiB each way per launch; FETCH_SIZE reads "
+          f"{F[copy]:.0f} KiB and WRITE_SIZE {Wr[copy]:.0f} KiB for it, so fetch is scaled by "
+          f"{ff:.3f} and write by {fw:.3f}.\n",
+          "| kernel | FETCH_SIZE KiB | WRITE_SIZE KiB | scaled HBM bytes / cell | achieved GB/s |",
+          "|---|---|---|---|---|"]
+    traffic = {}
+    step_bytes = 0.0
+    step_us = 0.0
+    for k in sorted(F, key=lambda k: -(F[k] * ff + Wr.get(k, 0) * fw)):
+        b = (F[k] * ff + Wr.get(k, 0) * fw) * 1024
+        s = _short(k)
+        us = avg_us.get(s, 0.0)
+        gbs = b / (us * 1e-6) / 1e9 if us else 0.0
+        L.append(f"| `{s}` | {F[k]:.0f} | {Wr.get(k, 0):.0f} | {b / cells:.1f} | {gbs:.0f} |")
+        traffic[s] = b
+        step_bytes += per_step.get(s, 0) * b
+        step_us += per_step.get(s, 0) * us
+    L += ["\n## One step\n",
+          f"HBM bytes per step (scaled counters x launches per step): {step_bytes / 1e6:.1f} MB "
+          f"= {step_bytes / cells:.0f} B/cell; kernel time per step {step_us:.1f} µs; "
+          f"{step_bytes / (step_us * 1e-6) / 1e9:.0f} GB/s over the kernels "
+          f"({step_bytes / (step_us * 1e-6) / 8e12 * 100:.0f} % of 8 TB/s). "
+          f"Algorithmic bytes (evolved fields read + written once): "
+          f"{W.step_bytes(W.is_adiabatic(workload), esize)} B/cell."]
+    os.makedirs(outdir, exist_ok=True)
+    with open(os.path.join(outdir, f"{tag}_step.md"), "w") as fh:
+        fh.write("\n".join(L) + "\n")
+    tj = {}
+    for s, b in traffic.items():
+        for kk, v in JSON_NAMES.items():
+            if kk in s:
+                tj[v] = b
+    tj["step"] = step_bytes
+    tj["step_kernel_us"] = step_us
+    with open(os.path.join(outdir, f"traffic_{tag}.json"), "w") as fh:
+        json.dump(tj, fh, indent=1)
+    return tj
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("workload")
+    ap.add_argument("precision", choices=("fp64", "split"))
+    ap.add_argument("base", nargs="?", default="gpurun_out")
+    ap.add_argument("--out", default="profiles")
+    a = ap.parse_args()
+    print(json.dumps(summarise(a.workload, a.precision, a.base, a.out)))
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())
