@@ -94,14 +94,14 @@ def batch_run(name: str, prec: int, steps: int, warmup: int, use_graph: bool):
 
 def bench_batch(args) -> int:
     from fargocpt_amd import workloads as W
-    from fargocpt_amd.capi import KERNELS
+    from fargocpt_amd.capi import BATCH_KERNELS
     name = args.workload
     prec = PRECISIONS[args.precision]
     use_graph = not args.no_graph
     batch, disks, cells, wall, ms_events = batch_run(name, prec, args.steps, args.warmup, use_graph)
     value = cells * args.steps / wall
     el = ELEMENT[prec]
-    kt = {k: batch.time_kernel(v, 10) for k, v in KERNELS.items()}
+    kt = {k: batch.time_kernel(v, 10) for k, v in BATCH_KERNELS.items()}
     dom = max(kt, key=kt.get)
     achieved = W.algorithmic_bytes(dom, False, el) * cells / (kt[dom] * 1e-3) / 1e9
     n = len(disks)

@@ -23,11 +23,15 @@ FIELD_ID = {name: i for i, name in enumerate(FIELDS)}
 VECTOR_FIELDS = {"VRAD", "VRAD0", "TAU_R_PHI"}
 
 ARTVISC = {"none": 0, "tw": 1, "sn": 2}
-KERNELS = {"radial_sweep": 0, "azimuthal_sweep": 1, "cfl": 3, "stress": 4, "potential": 5}
+# kernels of the (fused) step that fcpt_time_kernel can time on their own
+KERNELS = {"radial_sweep": 0, "azimuthal_sweep": 1, "cfl": 3, "visc_fused": 7, "potential": 5}
+# the same on a batch (the viscous pass is not timed there)
+BATCH_KERNELS = {"radial_sweep": 0, "azimuthal_sweep": 1, "cfl": 3, "potential": 5}
+KERNEL_STRESS = 4      # the separate stress-tensor kernel of fcpt_viscous_stress_tensor
 KERNEL_DISK_FORCE = 6  # the disk-on-planet reduction (not part of the hydro step)
 # storage modes (fcpt_disk_desc.precision) and launch parameters of include/fcpt.h
 PREC_FP64, PREC_SPLIT32 = 0, 1
-PARAM_RADIAL_TILE, PARAM_RADIAL_WAVES = 0, 1
+PARAM_RADIAL_TILE, PARAM_RADIAL_WAVES, PARAM_FUSED_STEP = 0, 1, 2
 BC = {"reflecting": 0, "zerogradient": 1, "outflow": 2}
 MAX_PLANETS = 8
 
@@ -368,6 +372,11 @@ class Disk:
         out = np.zeros(2 * max(1, len(bodies)))
         self._check(self.lib.fcpt_disk_on_planet_accel(self.h, arr, len(bodies), flags, _dptr(out)))
         return out[:2 * len(bodies)].reshape(-1, 2)
+
+    def set_fused_step(self, fused):
+        """Fused source / artificial-viscosity / viscous passes in the step
+        (default) or the separate kernels of the per-function entry points."""
+        self._check(self.lib.fcpt_set_launch_param(self.h, PARAM_FUSED_STEP, 1 if fused else 0))
 
     def set_radial_tile(self, rings):
         """Rings walked by one radial-sweep thread (0: size-based default)."""

@@ -23,6 +23,7 @@ struct fcpt_ctx {
     hipStream_t stream = nullptr;
     int device = 0;
     std::vector<void *> allocs;
+    bool fused = true;  // fused step kernels (FCPT_PARAM_FUSED_STEP)
     DevScalars *s_dev = nullptr;
     DevScalars *s_host = nullptr;  // pinned mirror for read-back
     double *totals_ring = nullptr;
@@ -268,6 +269,8 @@ int fcpt_create(const fcpt_disk_desc *desc, fcpt_ctx **out)
     const size_t esz = element_size(P);
     for (int k = 0; k < FCPT_NUM_FIELDS && !rc; ++k)
         rc = dev_alloc_bytes(c, &P.f[k], field_rows(k, nr) * ns * esz);
+    if (!rc) rc = dev_alloc_bytes(c, &P.valt[0], field_rows(FCPT_VRAD, nr) * ns * esz);
+    if (!rc) rc = dev_alloc_bytes(c, &P.valt[1], field_rows(FCPT_VAZI, nr) * ns * esz);
     for (int k = 0; k < 4 && !rc; ++k) rc = dev_alloc_bytes(c, &P.mom[k], ncell * esz);
     for (int k = 0; k < 4 && !rc; ++k) rc = dev_alloc_bytes(c, &P.mom_b[k], ncell * esz);
     // the density and energy between the transport sweeps stay FP64 in both
@@ -681,9 +684,37 @@ int fcpt_get_nshift(fcpt_ctx *ctx, int32_t *nshift)
 }
 
 // the gas part of step_Euler (simulation.cpp:148-267); dt already in device memory
-static void enqueue_euler(const KParams &P, hipStream_t st, int count_monitor)
+// One gas step of step_Euler.  The fused form runs the source terms, the
+// artificial viscosity and the viscous stress + update as one pass each; each
+// pass writes the velocities to the partner buffers, so the passes alternate
+// between the two views of the state (P and its swapped view), and the
+// transport's velocity kernel writes the final velocities to the fields.
+static void enqueue_euler(const KParams &P, hipStream_t st, int count_monitor, bool fused)
 {
     launch_potential(P, st);
+    if (fused) {
+        const KParams Q = swapped_view(P);
+        const KParams *cur = &P, *other = &Q;
+        auto flip = [&]() { std::swap(cur, other); };
+        launch_src_fused(*cur, st);
+        flip();
+        if (P.adiabatic) launch_src_energy(*cur, st);
+        if (P.artvisc == FCPT_ARTVISC_TW || P.artvisc == FCPT_ARTVISC_SN) {
+            launch_av_fused(*cur, st, 1);
+            flip();
+        } else {
+            launch_artificial_viscosity(*cur, st, 1);  // temperature floor only
+            launch_recalc_viscosity(*cur, st);
+        }
+        launch_visc_fused(*cur, st);
+        flip();
+        if (P.adiabatic) launch_substep3(*cur, st);
+        launch_boundaries(*cur, st, 0, 0);
+        launch_transport_to(*cur, P, st);
+        launch_boundaries(P, st, 1, count_monitor ? 2 : 1);
+        launch_derived(P, st);
+        return;
+    }
     launch_sources(P, st);
     if (!launch_artificial_viscosity(P, st, 1)) launch_recalc_viscosity(P, st);
     launch_stress(P, st);
@@ -700,7 +731,7 @@ int fcpt_step_euler(fcpt_ctx *ctx, double dt)
 {
     CTX_CHECK(ctx);
     launch_set_dt(ctx->P, ctx->stream, dt);
-    enqueue_euler(ctx->P, ctx->stream, 0);
+    enqueue_euler(ctx->P, ctx->stream, 0, ctx->fused);
     LAUNCH_CHECK(ctx);
     return FCPT_OK;
 }
@@ -710,7 +741,7 @@ int fcpt_run_step(fcpt_ctx *ctx, double *dt_out)
     CTX_CHECK(ctx);
     const KParams &P = ctx->P;
     launch_cfl(P, ctx->stream, 2);
-    enqueue_euler(P, ctx->stream, 1);
+    enqueue_euler(P, ctx->stream, 1, ctx->fused);
     LAUNCH_CHECK(ctx);
     int rc = read_scalars(ctx);
     if (rc) return rc;
@@ -727,7 +758,7 @@ static int ensure_step_graph(fcpt_ctx *ctx)
     hipGraph_t g = nullptr;
     HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
     launch_cfl(ctx->P, ctx->stream, 2);
-    enqueue_euler(ctx->P, ctx->stream, 1);
+    enqueue_euler(ctx->P, ctx->stream, 1, ctx->fused);
     hipError_t e = hipStreamEndCapture(ctx->stream, &g);
     if (e != hipSuccess) return fail(ctx, FCPT_ERR_DEVICE, "graph capture: %s", hipGetErrorString(e));
     e = hipGraphInstantiate(&ctx->step_graph, g, nullptr, nullptr, 0);
@@ -750,7 +781,7 @@ int fcpt_run_steps(fcpt_ctx *ctx, int32_t n_steps, int32_t use_graph)
     } else {
         for (int k = 0; k < n_steps; ++k) {
             launch_cfl(ctx->P, ctx->stream, 2);
-            enqueue_euler(ctx->P, ctx->stream, 1);
+            enqueue_euler(ctx->P, ctx->stream, 1, ctx->fused);
         }
         LAUNCH_CHECK(ctx);
     }
@@ -809,10 +840,11 @@ int fcpt_time_kernel(fcpt_ctx *ctx, int32_t kernel, int32_t reps, double *ms_per
         case FCPT_KERNEL_CFL: launch_cfl_cells(ctx->P, ctx->stream); break;
         case FCPT_KERNEL_STRESS: launch_stress(ctx->P, ctx->stream); break;
         case FCPT_KERNEL_POTENTIAL: launch_potential(ctx->P, ctx->stream); break;
+        case FCPT_KERNEL_VISC_FUSED: launch_visc_fused(ctx->P, ctx->stream); break;
         default: break;
         }
     };
-    if (kernel < 0 || kernel > FCPT_KERNEL_DISK_FORCE || kernel == 2)
+    if (kernel < 0 || kernel > FCPT_KERNEL_VISC_FUSED || kernel == 2)
         return fail(ctx, FCPT_ERR_ARG, "unknown kernel");
     one();  // warm-up
     HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
@@ -854,12 +886,15 @@ int fcpt_set_launch_param(fcpt_ctx *ctx, int32_t param, int32_t value)
         if (value < 0 || value > 64) return fail(ctx, FCPT_ERR_ARG, "radial tile must be 0 .. 64");
     } else if (param == FCPT_PARAM_RADIAL_WAVES) {
         if (value != 2 && value != 3) return fail(ctx, FCPT_ERR_ARG, "radial waves must be 2 or 3");
+    } else if (param == FCPT_PARAM_FUSED_STEP) {
+        if (value != 0 && value != 1) return fail(ctx, FCPT_ERR_ARG, "fused step must be 0 or 1");
     } else {
         return fail(ctx, FCPT_ERR_ARG, "unknown launch parameter");
     }
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     if (param == FCPT_PARAM_RADIAL_TILE) ctx->P.radial_tile = value;
-    else ctx->P.radial_waves = value;
+    else if (param == FCPT_PARAM_RADIAL_WAVES) ctx->P.radial_waves = value;
+    else ctx->fused = value != 0;
     if (ctx->step_graph) {
         HIP_TRY(ctx, hipGraphExecDestroy(ctx->step_graph));
         ctx->step_graph = nullptr;
@@ -973,7 +1008,7 @@ struct fcpt_batch {
     int split = 0;
     int radial_waves = 3;
     std::vector<fcpt_ctx *> disks;
-    KParams *Ps = nullptr;  // device array, one per disk
+    KParams *Ps = nullptr;  // device array: every disk's parameters, then their swapped views
     std::vector<BatchStageDev> stages;
     std::vector<void *> allocs;
     hipStream_t stream = nullptr;
@@ -1037,9 +1072,12 @@ int fcpt_batch_create(fcpt_ctx *const *disks, int32_t n_disks, fcpt_batch **out)
         b->allocs.push_back(q);
         return q;
     };
-    b->Ps = static_cast<KParams *>(alloc(host.size() * sizeof(KParams)));
+    // disk d's parameters at d, its swapped view (fused passes) at n + d
+    std::vector<KParams> views(host);
+    for (const KParams &P : host) views.push_back(swapped_view(P));
+    b->Ps = static_cast<KParams *>(alloc(views.size() * sizeof(KParams)));
     int rc = b->Ps ? FCPT_OK : FCPT_ERR_DEVICE;
-    if (!rc && hipMemcpy(b->Ps, host.data(), host.size() * sizeof(KParams), hipMemcpyHostToDevice) != hipSuccess)
+    if (!rc && hipMemcpy(b->Ps, views.data(), views.size() * sizeof(KParams), hipMemcpyHostToDevice) != hipSuccess)
         rc = FCPT_ERR_DEVICE;
     if (!rc) {
         for (const BatchStageHost &h : plan_batch_step(host)) {
@@ -1138,8 +1176,12 @@ int fcpt_batch_time_kernel(fcpt_batch *b, int32_t kernel, int32_t reps, double *
     case FCPT_KERNEL_RADIAL_SWEEP: kinds[0] = B_RADIAL_ISO; kinds[1] = B_RADIAL_ADI; break;
     case FCPT_KERNEL_AZIMUTHAL: kinds[0] = B_AZIM_ISO; kinds[1] = B_AZIM_ADI; break;
     case FCPT_KERNEL_CFL: kinds[0] = B_CFL_CELLS; break;
-    case FCPT_KERNEL_STRESS: kinds[0] = B_STRESS; break;
     case FCPT_KERNEL_POTENTIAL: kinds[0] = B_POTENTIAL; break;
+    case FCPT_KERNEL_STRESS:
+    case FCPT_KERNEL_VISC_FUSED:
+        // the batched step runs the fused viscous pass, on views that differ
+        // between disks; timed alone it could write the fields
+        return bfail(b, FCPT_ERR_ARG, "the viscous kernels are not timed on a batch");
     default: return bfail(b, FCPT_ERR_ARG, "unknown kernel");
     }
     BHIP(b, hipSetDevice(b->device));

@@ -79,6 +79,7 @@ struct KParams {
     void *mom_b[4];      // ping-pong partners
     void *sig_b, *en_b;  // radial-sweep outputs
     void *sig_c, *en_c;  // azimuthal-sweep outputs
+    void *valt[2];       // partners of v_r and v_phi for the fused step kernels
     double *bgm[4];      // backgrounds of the four momenta (split storage)
     double *iso_cs, *iso_h, *iso_nu;  // locally isothermal ring profiles of c_s, H, nu
     double *vmean;       // per-ring mean azimuthal velocity

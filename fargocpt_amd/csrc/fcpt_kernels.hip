@@ -112,6 +112,13 @@ struct Fld {
     // the stored value alone, and its decoding: lets a kernel issue a load
     // early and pay for it where the value is used
     __device__ __forceinline__ T raw(int i, int j) const { return p[(size_t)i * (size_t)ns + (size_t)j]; }
+    __device__ __forceinline__ void set_raw(int i, int j, T x) const { p[(size_t)i * (size_t)ns + (size_t)j] = x; }
+    // v as a later load of the stored value returns it
+    __device__ __forceinline__ double stored(int i, double v) const
+    {
+        if constexpr (SPLIT) return cst(bg)[i] + (double)(float)(v - cst(bg)[i]);
+        else return v;
+    }
     __device__ __forceinline__ double dec(int i, T x) const
     {
         if constexpr (SPLIT) return cst(bg)[i] + (double)x;
@@ -457,6 +464,35 @@ __device__ __forceinline__ void b_potential(const KParams &P, int bx, int by)
 FCPT_PAIR0(potential, 256)
 
 // ---------------------------------------------------------------------------
+// momentum_update_radial, SourceEuler.cpp:325-372, for one cell: pressure and
+// potential gradients between rings i - 1 and i, and the centrifugal term of
+// the four v_phi around the interface (summed in that order)
+__device__ __forceinline__ double src_vr_new(const KParams &P, int i, double dt, double v, double s0,
+                                             double sm, double p0, double pm, double phi0, double phim,
+                                             double va00, double va01, double va10, double va11)
+{
+    double gradp = 2.0 / (s0 + sm);
+    gradp *= (p0 - pm);
+    gradp *= cst(P.InvDiffRmed)[i];
+    const double gradphi = (phi0 - phim) * cst(P.InvDiffRmed)[i];
+    const double vsum = va00 + va01 + va10 + va11;
+    const double vt = 0.25 * vsum + cst(P.Rinf)[i] * P.omega_frame;
+    const double vt2 = vt * vt;
+    const double centrifugal_accel = vt2 * cst(P.InvRinf)[i];
+    return v + dt * (-gradp - gradphi + centrifugal_accel);
+}
+
+// momentum_update_azimuthal, SourceEuler.cpp:375-428, for one cell
+__device__ __forceinline__ double src_vphi_new(const KParams &P, int i, double dt, double va, double s0,
+                                               double sjp, double p0, double pjp, double phi0,
+                                               double phijp)
+{
+    const double invdxtheta = 2.0 / (P.dphi * (cst(P.Rsup)[i] + cst(P.Rinf)[i]));
+    const double gradp = 2.0 / (s0 + sjp) * (p0 - pjp) * invdxtheta;
+    const double gradphi = (phi0 - phijp) * invdxtheta;
+    return va + dt * (-gradp - gradphi);
+}
+
 // momentum_update_radial, SourceEuler.cpp:325-372
 template <bool SPLIT>
 __device__ __forceinline__ void b_src_vr(const KParams &P, int bx, int by)
@@ -468,16 +504,9 @@ __device__ __forceinline__ void b_src_vr(const KParams &P, int bx, int by)
     FLD(vaz, FCPT_VAZI);
     FLD(vr, FCPT_VRAD);
     const double dt = P.s->dt;
-    double gradp = 2.0 / (S(i, j) + S(i - 1, j));
-    gradp *= (Pr(i, j) - Pr(i - 1, j));
-    gradp *= cst(P.InvDiffRmed)[i];
-    const double gradphi = (Phi(i, j) - Phi(i - 1, j)) * cst(P.InvDiffRmed)[i];
     const int jn = j == ns - 1 ? 0 : j + 1;
-    const double vsum = vaz(i, j) + vaz(i, jn) + vaz(i - 1, j) + vaz(i - 1, jn);
-    const double vt = 0.25 * vsum + cst(P.Rinf)[i] * P.omega_frame;
-    const double vt2 = vt * vt;
-    const double centrifugal_accel = vt2 * cst(P.InvRinf)[i];
-    vr.set(i, j, vr(i, j) + dt * (-gradp - gradphi + centrifugal_accel));
+    vr.set(i, j, src_vr_new(P, i, dt, vr(i, j), S(i, j), S(i - 1, j), Pr(i, j), Pr(i - 1, j), Phi(i, j),
+                            Phi(i - 1, j), vaz(i, j), vaz(i, jn), vaz(i - 1, j), vaz(i - 1, jn)));
 }
 FCPT_PAIR0(src_vr, 256)
 
@@ -491,11 +520,9 @@ __device__ __forceinline__ void b_src_vphi(const KParams &P, int bx, int by)
     FLD(Phi, FCPT_POTENTIAL);
     FLD(vaz, FCPT_VAZI);
     const double dt = P.s->dt;
-    const double invdxtheta = 2.0 / (P.dphi * (cst(P.Rsup)[i] + cst(P.Rinf)[i]));
     const int jp = j == 0 ? ns - 1 : j - 1;
-    const double gradp = 2.0 / (S(i, j) + S(i, jp)) * (Pr(i, j) - Pr(i, jp)) * invdxtheta;
-    const double gradphi = (Phi(i, j) - Phi(i, jp)) * invdxtheta;
-    vaz.set(i, j, vaz(i, j) + dt * (-gradp - gradphi));
+    vaz.set(i, j, src_vphi_new(P, i, dt, vaz(i, j), S(i, j), S(i, jp), Pr(i, j), Pr(i, jp), Phi(i, j),
+                               Phi(i, jp)));
 }
 FCPT_PAIR0(src_vphi, 256)
 
@@ -593,6 +620,44 @@ __device__ __forceinline__ void derived_store(const KParams &P, int i, int j, co
 }
 
 // ---------------------------------------------------------------------------
+// The tensor of the TW artificial viscosity at one cell from its velocities
+// v_r(i), v_r(i + 1), v_phi(j), v_phi(j + 1) (artificial_viscosity.cpp:44-70)
+struct TwQ {
+    double qr, qp, eps_rr, eps_pp, div_V, l_sq;
+};
+
+__device__ __forceinline__ TwQ tw_q_cell(const KParams &P, int i, double vr0, double vr1, double va0,
+                                         double va1, double sig)
+{
+    const int ns = P.ns;
+    TwQ q;
+    q.eps_rr = (vr1 - vr0) * cst(P.InvDiffRsup)[i];
+    q.eps_pp = cst(P.InvRmed)[i] * ((va1 - va0) * P.invdphi + 0.5 * (vr1 + vr0));
+    q.div_V = std_min(q.eps_rr + q.eps_pp, 0.0);
+    const double Dr = cst(P.Rinf)[i + 1] - cst(P.Rinf)[i];
+    const double rDphi = cst(P.Rmed)[i] * P.dphi;
+    const double dxm = (ns <= 16) ? std_min(Dr, rDphi) : std_max(Dr, rDphi);
+    const double dx_sq = dxm * dxm;
+    q.l_sq = (P.artvisc_factor * P.artvisc_factor) * dx_sq;
+    q.qr = q.l_sq * sig * -q.div_V * (q.eps_rr - 1.0 / 3.0 * q.div_V);
+    q.qp = q.l_sq * sig * -q.div_V * (q.eps_pp - 1.0 / 3.0 * q.div_V);
+    return q;
+}
+
+// energy after the TW dissipation and the temperature floor
+// (artificial_viscosity.cpp:72-86, then SetTemperatureFloorCeilValues, :19-21)
+__device__ __forceinline__ double tw_dissipation(const KParams &P, int i, const TwQ &q, double sig,
+                                                 double dt, double e)
+{
+    if (i > 1 && i < P.nr - 1) {
+        const double d = q.eps_rr - q.eps_pp;
+        const double Qplus =
+            -q.l_sq * q.div_V * sig * 1.0 / 3.0 * (q.eps_rr * q.eps_rr + q.eps_pp * q.eps_pp + d * d);
+        e += Qplus * dt;
+    }
+    return temperature_floor(P, e, sig);
+}
+
 // update_with_artificial_viscosity_TW, artificial_viscosity.cpp:35-88
 template <bool SPLIT>
 __device__ __forceinline__ void b_tw_q(const KParams &P, int bx, int by)
@@ -608,31 +673,33 @@ __device__ __forceinline__ void b_tw_q(const KParams &P, int bx, int by)
     const double vr0 = vr(i, j), vr1 = vr(i + 1, j), va0 = vaz(i, j), va1 = vaz(i, jn);
     const double sig = S(i, j);
     const double e_in = diss ? E(i, j) : 0.0;
-    const double eps_rr = (vr1 - vr0) * cst(P.InvDiffRsup)[i];
-    const double eps_pp = cst(P.InvRmed)[i] * ((va1 - va0) * P.invdphi + 0.5 * (vr1 + vr0));
-    const double div_V = std_min(eps_rr + eps_pp, 0.0);
-    const double Dr = cst(P.Rinf)[i + 1] - cst(P.Rinf)[i];
-    const double rDphi = cst(P.Rmed)[i] * P.dphi;
-    const double dxm = (ns <= 16) ? std_min(Dr, rDphi) : std_max(Dr, rDphi);
-    const double dx_sq = dxm * dxm;
-    const double l_sq = (P.artvisc_factor * P.artvisc_factor) * dx_sq;
-    fld<SPLIT>(P, FCPT_Q_R).set(i, j, l_sq * sig * -div_V * (eps_rr - 1.0 / 3.0 * div_V));
-    fld<SPLIT>(P, FCPT_Q_PHI).set(i, j, l_sq * sig * -div_V * (eps_pp - 1.0 / 3.0 * div_V));
-    if (diss) {
-        double e = e_in;
-        if (i > 1 && i < P.nr - 1) {
-            const double d = eps_rr - eps_pp;
-            const double Qplus =
-                -l_sq * div_V * sig * 1.0 / 3.0 * (eps_rr * eps_rr + eps_pp * eps_pp + d * d);
-            e += Qplus * dt;
-        }
-        // SetTemperatureFloorCeilValues after the art. viscosity (artificial_viscosity.cpp:19-21)
-        E.set(i, j, temperature_floor(P, e, sig));
-    }
+    const TwQ q = tw_q_cell(P, i, vr0, vr1, va0, va1, sig);
+    fld<SPLIT>(P, FCPT_Q_R).set(i, j, q.qr);
+    fld<SPLIT>(P, FCPT_Q_PHI).set(i, j, q.qp);
+    if (diss) E.set(i, j, tw_dissipation(P, i, q, sig, dt, e_in));
 }
 FCPT_PAIR0(tw_q, 256)
 
-// artificial_viscosity.cpp:90-139
+// the TW velocity updates of one cell from the tensors around it
+// (artificial_viscosity.cpp:90-139)
+__device__ __forceinline__ double tw_vphi_new(const KParams &P, int i, double dt, double va, double s0,
+                                              double sjp, double qp0, double qpj)
+{
+    const double sigma_phi_avg = 0.5 * (s0 + sjp);
+    const double dVp = 2.0 * dt / ((cst(P.Rsup)[i] + cst(P.Rinf)[i]) * sigma_phi_avg) * (qp0 - qpj) * P.invdphi;
+    return va + dVp;
+}
+
+__device__ __forceinline__ double tw_vr_new(const KParams &P, int i, double dt, double v, double s0,
+                                            double sm, double qr0, double qrm, double qp0, double qpm)
+{
+    const double sigma_r_avg = 0.5 * (s0 + sm);
+    const double r0 = cst(P.Rmed)[i], r1 = cst(P.Rmed)[i - 1];
+    const double dVr = P.radial_viscosity_factor * dt / sigma_r_avg * 2.0 / (r0 * r0 - r1 * r1) *
+                       ((qr0 * r0 - qrm * r1) - 0.5 * (qp0 + qpm) * (r0 - r1));
+    return v + dVr;
+}
+
 template <bool SPLIT>
 __device__ __forceinline__ void b_tw_vel(const KParams &P, int bx, int by, int with_viscosity)
 {
@@ -670,18 +737,40 @@ __device__ __forceinline__ void b_tw_vel(const KParams &P, int bx, int by, int w
     }
     if (visc) derived_store<SPLIT>(P, i, j, din, false, false);
     if (!inner) return;
-    const double sigma_phi_avg = 0.5 * (s0 + sjp);
-    const double dVp = 2.0 * dt / ((cst(P.Rsup)[i] + cst(P.Rinf)[i]) * sigma_phi_avg) * (qp0 - qpj) * P.invdphi;
-    vaz.set(i, j, va + dVp);
-    if (radial) {
-        const double sigma_r_avg = 0.5 * (s0 + sm);
-        const double r0 = cst(P.Rmed)[i], r1 = cst(P.Rmed)[i - 1];
-        const double dVr = P.radial_viscosity_factor * dt / sigma_r_avg * 2.0 / (r0 * r0 - r1 * r1) *
-                           ((qr0 * r0 - qrm * r1) - 0.5 * (qp0 + qpm) * (r0 - r1));
-        vr.set(i, j, v + dVr);
-    }
+    vaz.set(i, j, tw_vphi_new(P, i, dt, va, s0, sjp, qp0, qpj));
+    if (radial) vr.set(i, j, tw_vr_new(P, i, dt, v, s0, sm, qr0, qrm, qp0, qpm));
 }
 FCPT_PAIR1(tw_vel, 256, int, with_viscosity)
+
+// The von Neumann-Richtmyer tensor at one cell (artificial_viscosity.cpp:160-185)
+struct SnQ {
+    double qr, qp, dv_r, dv_phi;
+};
+
+__device__ __forceinline__ SnQ sn_q_cell(const KParams &P, double vr0, double vr1, double va0,
+                                         double va1, double sig)
+{
+    SnQ q;
+    const double fac2 = P.artvisc_factor * P.artvisc_factor;
+    q.dv_r = vr1 - vr0;
+    q.qr = q.dv_r < 0.0 ? fac2 * sig * (q.dv_r * q.dv_r) : 0.0;
+    q.dv_phi = va1 - va0;
+    q.qp = q.dv_phi < 0.0 ? fac2 * sig * (q.dv_phi * q.dv_phi) : 0.0;
+    return q;
+}
+
+// energy after the SN dissipation and the temperature floor
+// (artificial_viscosity.cpp:187-205, then SetTemperatureFloorCeilValues)
+__device__ __forceinline__ double sn_dissipation(const KParams &P, int i, const SnQ &q, double sig,
+                                                 double dt, double e)
+{
+    if (i >= 1 && i < P.nr - 1) {
+        const double dxtheta = P.dphi * cst(P.Rmed)[i];
+        const double invdxtheta = 1.0 / dxtheta;
+        e = e - dt * q.qr * q.dv_r * cst(P.InvDiffRsup)[i] - dt * q.qp * q.dv_phi * invdxtheta;
+    }
+    return temperature_floor(P, e, sig);
+}
 
 // update_with_artificial_viscosity_SN, artificial_viscosity.cpp:148-211
 template <bool SPLIT>
@@ -698,26 +787,28 @@ __device__ __forceinline__ void b_sn_q(const KParams &P, int bx, int by)
     const double vr0 = vr(i, j), vr1 = vr(i + 1, j), va0 = vaz(i, j), va1 = vaz(i, jn);
     const double sig = S(i, j);
     const double e_in = diss ? E(i, j) : 0.0;
-    const double fac2 = P.artvisc_factor * P.artvisc_factor;
-    const double dv_r = vr1 - vr0;
-    const double qr = dv_r < 0.0 ? fac2 * sig * (dv_r * dv_r) : 0.0;
-    const double dv_phi = va1 - va0;
-    const double qp = dv_phi < 0.0 ? fac2 * sig * (dv_phi * dv_phi) : 0.0;
-    fld<SPLIT>(P, FCPT_Q_R).set(i, j, qr);
-    fld<SPLIT>(P, FCPT_Q_PHI).set(i, j, qp);
-    if (diss) {
-        double e = e_in;
-        if (i >= 1 && i < P.nr - 1) {
-            const double dxtheta = P.dphi * cst(P.Rmed)[i];
-            const double invdxtheta = 1.0 / dxtheta;
-            e = e - dt * qr * dv_r * cst(P.InvDiffRsup)[i] - dt * qp * dv_phi * invdxtheta;
-        }
-        E.set(i, j, temperature_floor(P, e, sig));
-    }
+    const SnQ q = sn_q_cell(P, vr0, vr1, va0, va1, sig);
+    fld<SPLIT>(P, FCPT_Q_R).set(i, j, q.qr);
+    fld<SPLIT>(P, FCPT_Q_PHI).set(i, j, q.qp);
+    if (diss) E.set(i, j, sn_dissipation(P, i, q, sig, dt, e_in));
 }
 FCPT_PAIR0(sn_q, 256)
 
-// artificial_viscosity.cpp:214-248
+// the SN velocity updates of one cell (artificial_viscosity.cpp:214-248)
+__device__ __forceinline__ double sn_vr_new(const KParams &P, int i, double dt, double v, double s0,
+                                            double sm, double qr0, double qrm)
+{
+    return v - dt * 2.0 / (s0 + sm) * (qr0 - qrm) * cst(P.InvDiffRmed)[i];
+}
+
+__device__ __forceinline__ double sn_vphi_new(const KParams &P, int i, double dt, double va, double s0,
+                                              double sjp, double qp0, double qpj)
+{
+    const double dxtheta = P.dphi * cst(P.Rmed)[i];
+    const double invdxtheta = 1.0 / dxtheta;
+    return va - dt * 2.0 / (s0 + sjp) * (qp0 - qpj) * invdxtheta;
+}
+
 template <bool SPLIT>
 __device__ __forceinline__ void b_sn_vel(const KParams &P, int bx, int by, int with_viscosity)
 {
@@ -754,10 +845,8 @@ __device__ __forceinline__ void b_sn_vel(const KParams &P, int bx, int by, int w
     }
     if (visc) derived_store<SPLIT>(P, i, j, din, false, false);
     if (!inner) return;
-    if (radial) vr.set(i, j, v - dt * 2.0 / (s0 + sm) * (qr0 - qrm) * cst(P.InvDiffRmed)[i]);
-    const double dxtheta = P.dphi * cst(P.Rmed)[i];
-    const double invdxtheta = 1.0 / dxtheta;
-    vaz.set(i, j, va - dt * 2.0 / (s0 + sjp) * (qp0 - qpj) * invdxtheta);
+    if (radial) vr.set(i, j, sn_vr_new(P, i, dt, v, s0, sm, qr0, qrm));
+    vaz.set(i, j, sn_vphi_new(P, i, dt, va, s0, sjp, qp0, qpj));
 }
 FCPT_PAIR1(sn_vel, 256, int, with_viscosity)
 
@@ -818,7 +907,40 @@ __global__ void __launch_bounds__(256) k_iso_rings(KParams P)
 }
 
 // ---------------------------------------------------------------------------
-// compute_viscous_stress_tensor, viscosity.cpp:139-254
+// compute_viscous_stress_tensor, viscosity.cpp:139-254: the cell-centred
+// divergence and diagonal components at (i, j) from v_r(i), v_r(i + 1),
+// v_phi(j), v_phi(j + 1), nu and Sigma of the cell
+struct StressDiag {
+    double divv, trr, tpp;
+};
+
+__device__ __forceinline__ StressDiag stress_diag(const KParams &P, int i, double vr0, double vr1,
+                                                  double va0, double va1, double n, double s)
+{
+    StressDiag d;
+    d.divv = (vr1 * cst(P.Rinf)[i + 1] - vr0 * cst(P.Rinf)[i]) * cst(P.InvDiffRsupRb)[i] +
+             (va1 - va0) * P.invdphi * cst(P.InvRmed)[i];
+    const double drr = (vr1 - vr0) * cst(P.InvDiffRsup)[i];
+    d.trr = 2.0 * n * s * (drr - 1.0 / 3.0 * d.divv);
+    const double dpp = (va1 - va0) * P.invdphi * cst(P.InvRmed)[i] + 0.5 * (vr1 + vr0) * cst(P.InvRmed)[i];
+    d.tpp = 2.0 * n * s * (dpp - 1.0 / 3.0 * d.divv);
+    return d;
+}
+
+// the corner component tau_r_phi at (i >= 1, j) from v_phi(i), v_phi(i - 1),
+// v_r(j), v_r(j - 1) and nu, Sigma of the four cells around the corner
+__device__ __forceinline__ double stress_rp(const KParams &P, int i, double va0, double vam, double vr0,
+                                            double vrp, double n, double nm, double njp, double nmp,
+                                            double s, double sm, double sjp, double smp)
+{
+    const double dvazirdr = (va0 * cst(P.InvRmed)[i] - vam * cst(P.InvRmed)[i - 1]) * cst(P.InvDiffRmed)[i];
+    const double dvrdphi = (vr0 - vrp) * P.invdphi;
+    const double drp = cst(P.Rinf)[i] * dvazirdr + dvrdphi * cst(P.InvRinf)[i];
+    const double n4 = 0.25 * (n + nm + njp + nmp);
+    const double s4 = 0.25 * (s + sm + sjp + smp);
+    return n4 * s4 * drp;
+}
+
 template <bool SPLIT>
 __device__ __forceinline__ void b_stress(const KParams &P, int bx, int by)
 {
@@ -837,25 +959,39 @@ __device__ __forceinline__ void b_stress(const KParams &P, int bx, int by)
     const double nm = iso ? cst(P.iso_nu)[im] : nu(im, j);
     const double njp = iso ? n : nu(i, jp), nmp = iso ? nm : nu(im, jp);
     const double sm = S(im, j), sjp = S(i, jp), smp = S(im, jp);
-    const double divv = (vr1 * cst(P.Rinf)[i + 1] - vr0 * cst(P.Rinf)[i]) * cst(P.InvDiffRsupRb)[i] +
-                        (va1 - va0) * P.invdphi * cst(P.InvRmed)[i];
-    fld<SPLIT>(P, FCPT_DIV_V).set(i, j, divv);
-    const double drr = (vr1 - vr0) * cst(P.InvDiffRsup)[i];
-    fld<SPLIT>(P, FCPT_TAU_R_R).set(i, j, 2.0 * n * s * (drr - 1.0 / 3.0 * divv));
-    const double dpp = (va1 - va0) * P.invdphi * cst(P.InvRmed)[i] + 0.5 * (vr1 + vr0) * cst(P.InvRmed)[i];
-    fld<SPLIT>(P, FCPT_TAU_PHI_PHI).set(i, j, 2.0 * n * s * (dpp - 1.0 / 3.0 * divv));
-    if (i >= 1) {
-        const double dvazirdr = (va0 * cst(P.InvRmed)[i] - vam * cst(P.InvRmed)[i - 1]) * cst(P.InvDiffRmed)[i];
-        const double dvrdphi = (vr0 - vrp) * P.invdphi;
-        const double drp = cst(P.Rinf)[i] * dvazirdr + dvrdphi * cst(P.InvRinf)[i];
-        const double n4 = 0.25 * (n + nm + njp + nmp);
-        const double s4 = 0.25 * (s + sm + sjp + smp);
-        fld<SPLIT>(P, FCPT_TAU_R_PHI).set(i, j, n4 * s4 * drp);
-    }
+    const StressDiag d = stress_diag(P, i, vr0, vr1, va0, va1, n, s);
+    fld<SPLIT>(P, FCPT_DIV_V).set(i, j, d.divv);
+    fld<SPLIT>(P, FCPT_TAU_R_R).set(i, j, d.trr);
+    fld<SPLIT>(P, FCPT_TAU_PHI_PHI).set(i, j, d.tpp);
+    if (i >= 1)
+        fld<SPLIT>(P, FCPT_TAU_R_PHI).set(i, j, stress_rp(P, i, va0, vam, vr0, vrp, n, nm, njp, nmp, s, sm, sjp, smp));
 }
 FCPT_PAIR0(stress, 256)
 
-// update_velocities_with_viscosity, viscosity.cpp:355-426
+// update_velocities_with_viscosity, viscosity.cpp:355-426, for one cell
+__device__ __forceinline__ double visc_vphi_new(const KParams &P, int i, double dt, double va, double s0,
+                                                double sjp, double trp0, double trp1, double tpp0,
+                                                double tppj)
+{
+    const double sigma_avg = 0.5 * (s0 + sjp);
+    const double a1 = cst(P.Rinf)[i + 1] * cst(P.Rinf)[i + 1], a0 = cst(P.Rinf)[i] * cst(P.Rinf)[i];
+    const double dVp = dt * cst(P.InvRmed)[i] / (sigma_avg) *
+                       ((2.0 / (a1 - a0)) * (a1 * trp1 - a0 * trp0) + (tpp0 - tppj) * P.invdphi);
+    return va + dVp;
+}
+
+__device__ __forceinline__ double visc_vr_new(const KParams &P, int i, double dt, double v, double s0,
+                                              double sm, double trr0, double trrm, double trp0,
+                                              double trpn, double tpp0, double tppm)
+{
+    const double sigma_avg = 0.5 * (s0 + sm);
+    const double dVr = dt / (sigma_avg)*P.radial_viscosity_factor * 2.0 /
+                       (cst(P.Rmed)[i] + cst(P.Rmed)[i - 1]) *
+                       ((cst(P.Rmed)[i] * trr0 - cst(P.Rmed)[i - 1] * trrm) * cst(P.InvDiffRmed)[i] +
+                        (trpn - trp0) * P.invdphi - 0.5 * (tpp0 + tppm));
+    return v + dVr;
+}
+
 template <bool SPLIT>
 __device__ __forceinline__ void b_visc_update(const KParams &P, int bx, int by)
 {
@@ -874,23 +1010,258 @@ __device__ __forceinline__ void b_visc_update(const KParams &P, int bx, int by)
     const double tpp0 = tpp(i, j), tppj = tpp(i, jp), tppm = tpp(i - 1, j);
     const double trr0 = trr(i, j), trrm = trr(i - 1, j);
     const double va = vaz(i, j), v = vr(i, j);
-    {
-        const double sigma_avg = 0.5 * (s0 + sjp);
-        const double a1 = cst(P.Rinf)[i + 1] * cst(P.Rinf)[i + 1], a0 = cst(P.Rinf)[i] * cst(P.Rinf)[i];
-        const double dVp = dt * cst(P.InvRmed)[i] / (sigma_avg) *
-                           ((2.0 / (a1 - a0)) * (a1 * trp1 - a0 * trp0) + (tpp0 - tppj) * P.invdphi);
-        vaz.set(i, j, va + dVp);
-    }
-    if (i >= 2) {
-        const double sigma_avg = 0.5 * (s0 + sm);
-        const double dVr = dt / (sigma_avg)*P.radial_viscosity_factor * 2.0 /
-                           (cst(P.Rmed)[i] + cst(P.Rmed)[i - 1]) *
-                           ((cst(P.Rmed)[i] * trr0 - cst(P.Rmed)[i - 1] * trrm) * cst(P.InvDiffRmed)[i] +
-                            (trpn - trp0) * P.invdphi - 0.5 * (tpp0 + tppm));
-        vr.set(i, j, v + dVr);
-    }
+    vaz.set(i, j, visc_vphi_new(P, i, dt, va, s0, sjp, trp0, trp1, tpp0, tppj));
+    if (i >= 2) vr.set(i, j, visc_vr_new(P, i, dt, v, s0, sm, trr0, trrm, trp0, trpn, tpp0, tppm));
 }
 FCPT_PAIR0(visc_update, 256)
+
+// ---------------------------------------------------------------------------
+// Fused kernels of the step.  Each reads the velocities of its view (P.f[VRAD],
+// P.f[VAZI]) and writes the velocities of every row, updated or passed
+// through, to the view's partner buffers P.valt: a cell's neighbours then
+// always see the old values, exactly as between the separate kernels above,
+// whose per-cell expressions they call, and whose intermediate tensors they
+// recompute where a neighbour needs them instead of storing and reloading
+// them.  The step alternates two views with swapped velocity pointers; the
+// transport's velocity kernel writes the final velocities to the fields.
+template <bool SPLIT>
+__device__ __forceinline__ Fld<SPLIT> alt_vr(const KParams &P)
+{
+    return buf<SPLIT>(P, P.valt[0], P.bg[FCPT_VRAD]);
+}
+template <bool SPLIT>
+__device__ __forceinline__ Fld<SPLIT> alt_vaz(const KParams &P)
+{
+    return buf<SPLIT>(P, P.valt[1], P.bg[FCPT_VAZI]);
+}
+
+// locally isothermal pressure Sigma c_s^2 (the value recalculate_derived
+// stores, SourceEuler.cpp:225-249, with the ring profile of c_s)
+__device__ __forceinline__ double iso_pressure(const KParams &P, int i, double sig)
+{
+    const double c_s = cst(P.iso_cs)[i];
+    return sig * (c_s * c_s);
+}
+
+// update_with_sourceterms' two momentum updates (SourceEuler.cpp:325-428) in
+// one pass over rows 0 .. nr; v_r is updated on rings 2 .. nr-2, v_phi on
+// 1 .. nr-2.  A locally isothermal disk's pressure is formed from Sigma.
+template <bool SPLIT>
+__device__ __forceinline__ void b_src_fused(const KParams &P, int bx, int by)
+{
+    CELL_LOOP_HEAD(0, P.nr + 1)
+    const int nr = P.nr;
+    FLD(vr, FCPT_VRAD);
+    FLD(vaz, FCPT_VAZI);
+    const Fld<SPLIT> vr_o = alt_vr<SPLIT>(P);
+    const Fld<SPLIT> vaz_o = alt_vaz<SPLIT>(P);
+    const bool upd_p = i >= 1 && i < nr - 1;
+    const bool upd_r = i >= 2 && upd_p;
+    if (!upd_p) {
+        vr_o.set_raw(i, j, vr.raw(i, j));
+        if (i < nr) vaz_o.set_raw(i, j, vaz.raw(i, j));
+        return;
+    }
+    FLD(S, FCPT_SIGMA);
+    FLD(Pr, FCPT_PRESSURE);
+    FLD(Phi, FCPT_POTENTIAL);
+    const bool iso = !P.adiabatic;
+    const double dt = P.s->dt;
+    const int jn = j == ns - 1 ? 0 : j + 1;
+    const int jp = j == 0 ? ns - 1 : j - 1;
+    const double s0 = S(i, j), sjp = S(i, jp);
+    const double p0 = iso ? iso_pressure(P, i, s0) : Pr(i, j);
+    const double pjp = iso ? iso_pressure(P, i, sjp) : Pr(i, jp);
+    const double phi0 = Phi(i, j), phijp = Phi(i, jp);
+    const double va0 = vaz(i, j);
+    double v = 0.0, sm = 0.0, pm = 0.0, phim = 0.0, va1 = 0.0, vam0 = 0.0, vam1 = 0.0;
+    if (upd_r) {
+        v = vr(i, j);
+        sm = S(i - 1, j);
+        pm = iso ? iso_pressure(P, i - 1, sm) : Pr(i - 1, j);
+        phim = Phi(i - 1, j);
+        va1 = vaz(i, jn);
+        vam0 = vaz(i - 1, j);
+        vam1 = vaz(i - 1, jn);
+    }
+    const typename Fld<SPLIT>::T vr_raw = vr.raw(i, j);
+    vaz_o.set(i, j, src_vphi_new(P, i, dt, va0, s0, sjp, p0, pjp, phi0, phijp));
+    if (upd_r) vr_o.set(i, j, src_vr_new(P, i, dt, v, s0, sm, p0, pm, phi0, phim, va0, va1, vam0, vam1));
+    else vr_o.set_raw(i, j, vr_raw);
+}
+FCPT_PAIR0(src_fused, 256)
+
+// the TW or SN artificial viscosity (artificial_viscosity.cpp:35-248) in one
+// pass: the tensor of the cell, of its azimuthal neighbour j - 1 and of its
+// radial neighbour i - 1, the dissipation and temperature floor of E, the
+// viscosity recalculation when asked, and the velocity updates
+template <bool SPLIT>
+__device__ __forceinline__ void b_av_fused(const KParams &P, int bx, int by, int with_viscosity)
+{
+    CELL_LOOP_HEAD(0, P.nr + 1)
+    const int nr = P.nr;
+    FLD(vr, FCPT_VRAD);
+    FLD(vaz, FCPT_VAZI);
+    const Fld<SPLIT> vr_o = alt_vr<SPLIT>(P);
+    const Fld<SPLIT> vaz_o = alt_vaz<SPLIT>(P);
+    if (i == nr) {
+        vr_o.set_raw(i, j, vr.raw(i, j));
+        return;
+    }
+    FLD(S, FCPT_SIGMA);
+    FLD(E, FCPT_ENERGY);
+    const bool sn = P.artvisc == FCPT_ARTVISC_SN;
+    const bool diss = P.adiabatic && P.artvisc_dissipation;
+    const bool visc = with_viscosity && P.adiabatic;  // constant ring profile when isothermal
+    const double dt = P.s->dt;
+    const int jn = j == ns - 1 ? 0 : j + 1;
+    const int jp = j == 0 ? ns - 1 : j - 1;
+    const bool inner = i >= 1 && i < nr - 1;
+    const bool radial = i >= 2 && inner;
+    const int im = i >= 1 ? i - 1 : 0;
+    // rows that are not updated are copied as stored (in split storage a
+    // decode / encode round trip would round the deviation)
+    const typename Fld<SPLIT>::T vr_raw = vr.raw(i, j), va_raw = vaz.raw(i, j);
+    const double vr0 = vr.dec(i, vr_raw), vr1 = vr(i + 1, j), va0 = vaz.dec(i, va_raw), va1 = vaz(i, jn);
+    const double sig = S(i, j);
+    const double e_in = (diss || visc) ? E(i, j) : 0.0;
+    double vrj0 = 0.0, vrj1 = 0.0, vaj0 = 0.0, sjp = 0.0;  // cell (i, j - 1)
+    if (inner) {
+        vrj0 = vr(i, jp);
+        vrj1 = vr(i + 1, jp);
+        vaj0 = vaz(i, jp);
+        sjp = S(i, jp);
+    }
+    double vrm0 = 0.0, vam0 = 0.0, vam1 = 0.0, sm = 0.0;  // cell (i - 1, j)
+    if (radial) {
+        vrm0 = vr(im, j);
+        vam0 = vaz(im, j);
+        vam1 = vaz(im, jn);
+        sm = S(im, j);
+    }
+    double qr0, qp0, qpj = 0.0, qrm = 0.0, qpm = 0.0, e = e_in;
+    if (sn) {
+        const SnQ q = sn_q_cell(P, vr0, vr1, va0, va1, sig);
+        qr0 = q.qr;
+        qp0 = q.qp;
+        if (diss) e = sn_dissipation(P, i, q, sig, dt, e_in);
+        if (inner) qpj = sn_q_cell(P, vrj0, vrj1, vaj0, va0, sjp).qp;
+        if (radial) qrm = sn_q_cell(P, vrm0, vr0, vam0, vam1, sm).qr;
+    } else {
+        const TwQ q = tw_q_cell(P, i, vr0, vr1, va0, va1, sig);
+        qr0 = q.qr;
+        qp0 = q.qp;
+        if (diss) e = tw_dissipation(P, i, q, sig, dt, e_in);
+        if (inner) qpj = tw_q_cell(P, i, vrj0, vrj1, vaj0, va0, sjp).qp;
+        if (radial) {
+            const TwQ m = tw_q_cell(P, im, vrm0, vr0, vam0, vam1, sm);
+            qrm = m.qr;
+            qpm = m.qp;
+        }
+    }
+    if (diss) {
+        E.set(i, j, e);
+        e = E.stored(i, e);
+    }
+    if (visc) derived_store<SPLIT>(P, i, j, DerivedIn{e, sig, 0.0, 0.0}, false, false);
+    if (inner)
+        vaz_o.set(i, j, sn ? sn_vphi_new(P, i, dt, va0, sig, sjp, qp0, qpj)
+                           : tw_vphi_new(P, i, dt, va0, sig, sjp, qp0, qpj));
+    else
+        vaz_o.set_raw(i, j, va_raw);
+    if (radial)
+        vr_o.set(i, j, sn ? sn_vr_new(P, i, dt, vr0, sig, sm, qr0, qrm)
+                          : tw_vr_new(P, i, dt, vr0, sig, sm, qr0, qrm, qp0, qpm));
+    else
+        vr_o.set_raw(i, j, vr_raw);
+}
+FCPT_PAIR1(av_fused, 256, int, with_viscosity)
+
+// compute_viscous_stress_tensor and update_velocities_with_viscosity
+// (viscosity.cpp:139-426) in one pass: the stress components a cell's update
+// needs -- tau_rr at (i, j), (i - 1, j); tau_pp at (i, j), (i, j - 1),
+// (i - 1, j); tau_rp at (i, j), (i + 1, j), (i, j + 1) -- from the velocities,
+// nu and Sigma around it.  The tensor and div v of the cell are stored when a
+// later kernel of the step reads them (the viscous heating of SubStep3 in an
+// adiabatic disk).
+template <bool SPLIT>
+__device__ __forceinline__ void b_visc_fused(const KParams &P, int bx, int by)
+{
+    const bool store_tensor = P.adiabatic && P.heating_viscous;
+    CELL_LOOP_HEAD(0, P.nr + 1)
+    const int nr = P.nr;
+    FLD(vr, FCPT_VRAD);
+    FLD(vaz, FCPT_VAZI);
+    const Fld<SPLIT> vr_o = alt_vr<SPLIT>(P);
+    const Fld<SPLIT> vaz_o = alt_vaz<SPLIT>(P);
+    if (i == nr) {
+        vr_o.set_raw(i, j, vr.raw(i, j));
+        return;
+    }
+    FLD(nu, FCPT_VISCOSITY);
+    FLD(S, FCPT_SIGMA);
+    const bool iso = !P.adiabatic;  // nu is a ring profile
+    const double dt = P.s->dt;
+    const int jn = j == ns - 1 ? 0 : j + 1;
+    const int jp = j == 0 ? ns - 1 : j - 1;
+    const int im = i >= 1 ? i - 1 : 0;
+    const bool upd_p = i >= 1 && i < nr - 1;
+    const bool upd_r = i >= 2 && upd_p;
+    const int ip = upd_p ? i + 1 : i;
+    // the cell and the corner (i, j); rows that are not updated are copied as stored
+    const typename Fld<SPLIT>::T vr_raw = vr.raw(i, j), va_raw = vaz.raw(i, j);
+    const double vr0 = vr.dec(i, vr_raw), vr1 = vr(i + 1, j), va0 = vaz.dec(i, va_raw), va1 = vaz(i, jn);
+    const double n = iso ? cst(P.iso_nu)[i] : nu(i, j), s = S(i, j);
+    const double vam = vaz(im, j), vrp = vr(i, jp);
+    const double nm = iso ? cst(P.iso_nu)[im] : nu(im, j);
+    const double njp = iso ? n : nu(i, jp), nmp = iso ? nm : nu(im, jp);
+    const double sm = S(im, j), sjp = S(i, jp), smp = S(im, jp);
+    // corner (i + 1, j) and cell (i, j - 1): the v_phi update
+    double va_p = 0.0, vr1p = 0.0, np_ = 0.0, npj = 0.0, sp = 0.0, spj = 0.0, vaj = 0.0;
+    if (upd_p) {
+        va_p = vaz(ip, j);
+        vr1p = vr(ip, jp);
+        np_ = iso ? cst(P.iso_nu)[ip] : nu(ip, j);
+        npj = iso ? np_ : nu(ip, jp);
+        sp = S(ip, j);
+        spj = S(ip, jp);
+        vaj = vaz(i, jp);
+    }
+    // cell (i - 1, j) and corner (i, j + 1): the v_r update
+    double vrm = 0.0, vam1 = 0.0, vrn = 0.0, nn = 0.0, nmn = 0.0, sn = 0.0, smn = 0.0;
+    if (upd_r) {
+        vrm = vr(im, j);
+        vam1 = vaz(im, jn);
+        vrn = vr(i, jn);
+        nn = iso ? n : nu(i, jn);
+        nmn = iso ? nm : nu(im, jn);
+        sn = S(i, jn);
+        smn = S(im, jn);
+    }
+    const StressDiag d = stress_diag(P, i, vr0, vr1, va0, va1, n, s);
+    const double trp0 = i >= 1 ? stress_rp(P, i, va0, vam, vr0, vrp, n, nm, njp, nmp, s, sm, sjp, smp) : 0.0;
+    if (store_tensor) {
+        fld<SPLIT>(P, FCPT_DIV_V).set(i, j, d.divv);
+        fld<SPLIT>(P, FCPT_TAU_R_R).set(i, j, d.trr);
+        fld<SPLIT>(P, FCPT_TAU_PHI_PHI).set(i, j, d.tpp);
+        if (i >= 1) fld<SPLIT>(P, FCPT_TAU_R_PHI).set(i, j, trp0);
+    }
+    if (upd_p) {
+        const double trp1 = stress_rp(P, ip, va_p, va0, vr1, vr1p, np_, n, npj, njp, sp, s, spj, sjp);
+        const double tppj = stress_diag(P, i, vrp, vr1p, vaj, va0, njp, sjp).tpp;
+        vaz_o.set(i, j, visc_vphi_new(P, i, dt, va0, s, sjp, trp0, trp1, d.tpp, tppj));
+    } else {
+        vaz_o.set_raw(i, j, va_raw);
+    }
+    if (upd_r) {
+        const StressDiag dm = stress_diag(P, im, vrm, vr0, vam, vam1, nm, sm);
+        const double trpn = stress_rp(P, i, va1, vam1, vrn, vr0, nn, nmn, n, nm, sn, smn, s, sm);
+        vr_o.set(i, j, visc_vr_new(P, i, dt, vr0, s, sm, d.trr, dm.trr, trp0, trpn, d.tpp, dm.tpp));
+    } else {
+        vr_o.set_raw(i, j, vr_raw);
+    }
+}
+FCPT_PAIR0(visc_fused, 256)
 
 // ---------------------------------------------------------------------------
 // Q- (thermal_relaxation, SourceEuler.cpp:632-690) and Q+ (viscous_heating,
@@ -1775,12 +2146,43 @@ void launch_copy4(const KParams &P, hipStream_t st)
     hipLaunchKernelGGL(k_copy4, dim3((unsigned)blocks), dim3(256), 0, st, P, n16);
 }
 
-void launch_transport(const KParams &P, hipStream_t st)
+void launch_transport(const KParams &P, hipStream_t st) { launch_transport_to(P, P, st); }
+
+// the sweeps read the velocities of view P; the velocity kernel reads only the
+// transported density and momenta and writes the velocities of view out
+void launch_transport_to(const KParams &P, const KParams &out, hipStream_t st)
 {
     launch_radial_sweep(P, st);
     launch_ring_mean(P, st, 1);
     launch_azimuthal_sweep(P, st);
-    LAUNCH(k_velocities, cell_grid(P, P.nr), kBlock, st, P);
+    LAUNCH(k_velocities, cell_grid(out, out.nr), kBlock, st, out);
+}
+
+KParams swapped_view(const KParams &P)
+{
+    KParams Q = P;
+    Q.f[FCPT_VRAD] = P.valt[0];
+    Q.valt[0] = P.f[FCPT_VRAD];
+    Q.f[FCPT_VAZI] = P.valt[1];
+    Q.valt[1] = P.f[FCPT_VAZI];
+    return Q;
+}
+
+void launch_src_fused(const KParams &P, hipStream_t st)
+{
+    LAUNCH(k_src_fused, cell_grid(P, P.nr + 1), kBlock, st, P);
+}
+void launch_src_energy(const KParams &P, hipStream_t st)
+{
+    LAUNCH(k_src_energy, cell_grid(P, P.nr - 1), kBlock, st, P);
+}
+void launch_av_fused(const KParams &P, hipStream_t st, int with_viscosity)
+{
+    LAUNCH(k_av_fused, cell_grid(P, P.nr + 1), kBlock, st, P, with_viscosity);
+}
+void launch_visc_fused(const KParams &P, hipStream_t st)
+{
+    LAUNCH(k_visc_fused, cell_grid(P, P.nr + 1), kBlock, st, P);
 }
 
 void launch_derived(const KParams &P, hipStream_t st) { LAUNCH(k_derived, cell_grid(P, P.nr), kBlock, st, P); }
@@ -1987,12 +2389,20 @@ static unsigned sector_grid(const KParams &P) { return (unsigned)((P.ns + 63) / 
 std::vector<BatchStageHost> plan_batch_step(const std::vector<KParams> &D)
 {
     std::vector<BatchStageHost> st;
-    auto stage = [&](int kind, int arg, auto pred, auto grid) {
+    // parameter block of disk d: d (the fields hold the velocities) or n + d
+    // (its swapped view: the partner buffers hold them, between fused passes)
+    const unsigned n = (unsigned)D.size();
+    std::vector<unsigned> view(n, 0);
+    auto stage_v = [&](int kind, int arg, auto pred, auto grid, bool flips, bool fields) {
         BatchStageHost s{kind, arg, {}};
-        for (unsigned d = 0; d < D.size(); ++d)
-            if (pred(D[d])) grid(s.entries, d, D[d]);
+        for (unsigned d = 0; d < n; ++d) {
+            if (!pred(D[d])) continue;
+            grid(s.entries, (fields || !view[d]) ? d : n + d, D[d]);
+            if (flips) view[d] ^= 1u;
+        }
         if (!s.entries.empty()) st.push_back(std::move(s));
     };
+    auto stage = [&](int kind, int arg, auto pred, auto grid) { stage_v(kind, arg, pred, grid, false, false); };
     auto all = [](const KParams &) { return true; };
     auto adi = [](const KParams &P) { return P.adiabatic != 0; };
     auto iso = [](const KParams &P) { return P.adiabatic == 0; };
@@ -2033,24 +2443,22 @@ std::vector<BatchStageHost> plan_batch_step(const std::vector<KParams> &D)
     // step_Euler (simulation.cpp:148-267), in the order of enqueue_euler
     stage(B_POT_BG, 0, [](const KParams &P) { return P.split != 0; }, pot_bg);
     stage(B_POTENTIAL, 0, all, rows(0));
-    stage(B_SRC_VR, 0, all, rows(3));
-    stage(B_SRC_VPHI, 0, all, rows(2));
+    // the fused passes of enqueue_euler, each flipping the disk's view
+    auto flip = [&](int kind, int arg, auto pred, auto grid) { stage_v(kind, arg, pred, grid, true, false); };
+    flip(B_SRC_FUSED, 0, all, rows(-1));
     stage(B_SRC_ENERGY, 0, adi, rows(1));
-    auto tw = [](const KParams &P) { return P.artvisc == FCPT_ARTVISC_TW; };
-    auto sn = [](const KParams &P) { return P.artvisc == FCPT_ARTVISC_SN; };
+    auto av = [](const KParams &P) {
+        return P.artvisc == FCPT_ARTVISC_TW || P.artvisc == FCPT_ARTVISC_SN;
+    };
     auto none = [](const KParams &P) { return P.artvisc == FCPT_ARTVISC_NONE; };
-    stage(B_TW_Q, 0, tw, rows(0));
-    stage(B_TW_VEL, 1, tw, rows(0));
-    stage(B_SN_Q, 0, sn, rows(0));
-    stage(B_SN_VEL, 1, sn, rows(0));
+    flip(B_AV_FUSED, 1, av, rows(-1));
     stage(B_TEMP_FLOOR, 0,
           [](const KParams &P) {
               return P.artvisc == FCPT_ARTVISC_NONE && P.adiabatic && P.artvisc_dissipation;
           },
           rows(0));
     stage(B_RECALC_VISC, 0, none, rows(0));
-    stage(B_STRESS, 0, all, rows(0));
-    stage(B_VISC_UPDATE, 0, all, rows(2));
+    flip(B_VISC_FUSED, 0, all, rows(-1));
     stage(B_SUBSTEP3, 0, adi, rows(0));
     stage(B_GHOST, 0, all, ghosts);
     stage(B_RADIAL_ISO, 0, iso, radial);
@@ -2058,7 +2466,9 @@ std::vector<BatchStageHost> plan_batch_step(const std::vector<KParams> &D)
     stage(B_RING_MEAN, 1, all, rings);
     stage(B_AZIM_ISO, 0, iso, azim);
     stage(B_AZIM_ADI, 0, adi, azim);
-    stage(B_VELOCITIES, 0, all, rows(0));
+    // the velocity kernel writes the fields of every disk
+    stage_v(B_VELOCITIES, 0, all, rows(0), false, true);
+    std::fill(view.begin(), view.end(), 0u);
     stage(B_DAMPING, 0, [](const KParams &P) { return P.damping != 0; }, damp);
     stage(B_GHOST, 2, all, ghosts);
     stage(B_DERIVED, 0, all, rows(0));
@@ -2085,6 +2495,9 @@ void launch_batch_stage(const KParams *Ps, const BatchStageDev &S, int split, in
     case B_POTENTIAL: LAUNCHB(kb_potential, kBlock); break;
     case B_SRC_VR: LAUNCHB(kb_src_vr, kBlock); break;
     case B_SRC_VPHI: LAUNCHB(kb_src_vphi, kBlock); break;
+    case B_SRC_FUSED: LAUNCHB(kb_src_fused, kBlock); break;
+    case B_AV_FUSED: LAUNCHB(kb_av_fused, kBlock, S.arg); break;
+    case B_VISC_FUSED: LAUNCHB(kb_visc_fused, kBlock); break;
     case B_SRC_ENERGY: LAUNCHB(kb_src_energy, kBlock); break;
     case B_TW_Q: LAUNCHB(kb_tw_q, kBlock); break;
     case B_TW_VEL: LAUNCHB(kb_tw_vel, kBlock, S.arg); break;

@@ -26,6 +26,14 @@ void launch_substep3(const KParams &P, hipStream_t st);
 void launch_heatcool_cfl(const KParams &P, hipStream_t st);
 void launch_boundaries(const KParams &P, hipStream_t st, int final_call, int advance_time);
 void launch_transport(const KParams &P, hipStream_t st);
+void launch_transport_to(const KParams &P, const KParams &out, hipStream_t st);
+// the fused step kernels write the velocities of view P to P.valt; the
+// swapped view reads those buffers and writes the fields
+KParams swapped_view(const KParams &P);
+void launch_src_fused(const KParams &P, hipStream_t st);
+void launch_src_energy(const KParams &P, hipStream_t st);
+void launch_av_fused(const KParams &P, hipStream_t st, int with_viscosity);
+void launch_visc_fused(const KParams &P, hipStream_t st);
 void launch_radial_sweep(const KParams &P, hipStream_t st);
 void launch_azimuthal_sweep(const KParams &P, hipStream_t st);
 void launch_cfl_cells(const KParams &P, hipStream_t st);
@@ -67,7 +75,7 @@ enum BatchKind {
     B_RING_MEAN, B_CFL_CELLS, B_CFL_FINAL, B_POT_BG, B_POTENTIAL, B_SRC_VR, B_SRC_VPHI,
     B_SRC_ENERGY, B_TW_Q, B_TW_VEL, B_SN_Q, B_SN_VEL, B_TEMP_FLOOR, B_RECALC_VISC, B_STRESS,
     B_VISC_UPDATE, B_SUBSTEP3, B_GHOST, B_RADIAL_ISO, B_RADIAL_ADI, B_AZIM_ISO, B_AZIM_ADI,
-    B_VELOCITIES, B_DAMPING, B_DERIVED
+    B_VELOCITIES, B_DAMPING, B_DERIVED, B_SRC_FUSED, B_AV_FUSED, B_VISC_FUSED
 };
 struct BatchStageHost {
     int kind, arg;

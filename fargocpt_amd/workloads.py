@@ -157,6 +157,8 @@ def algorithmic_bytes(kernel: str, adiabatic: bool, element: int = 8) -> int:
         "azimuthal_sweep": (6 + e, 5 + e),     # Sigma, 4 momenta, (E), v_phi -> 5/6, shifted
         "cfl": (4 + 3 * e, 0),                 # v_r, v_phi, c_s, nu, (Q+, Q-, E)
         "stress": (4, 4),                      # v_r, v_phi, nu, Sigma -> div v, tau_rr, tau_pp, tau_rp
+        # v_r, v_phi, Sigma, (nu) -> v_r, v_phi, (div v and the tensor for the heating)
+        "visc_fused": (3 + e, 2 + 4 * e),
         "potential": (1, 1),                   # H -> Phi
         "disk_force": (2 + e, 0),              # Sigma (ring means, then cells), (H)
     }

@@ -200,9 +200,13 @@ int fcpt_run_steps(fcpt_ctx *ctx, int32_t n_steps, int32_t use_graph);
    FCPT_KERNEL_DISK_FORCE times the launches of fcpt_disk_on_planet_accel
    (ring means, cell reduction, block reduction; no copy to the host) for two
    bodies, a star at the origin and a planet at (1, 0), with the self-gravity
-   correction. */
+   correction.  FCPT_KERNEL_VISC_FUSED is the fused viscous stress + velocity
+   pass of the step (FCPT_PARAM_FUSED_STEP); it writes only the velocities'
+   partner buffers (and, in an adiabatic disk with viscous heating, the
+   stress tensor). */
 enum { FCPT_KERNEL_RADIAL_SWEEP = 0, FCPT_KERNEL_AZIMUTHAL = 1, FCPT_KERNEL_CFL = 3,
-       FCPT_KERNEL_STRESS = 4, FCPT_KERNEL_POTENTIAL = 5, FCPT_KERNEL_DISK_FORCE = 6 };
+       FCPT_KERNEL_STRESS = 4, FCPT_KERNEL_POTENTIAL = 5, FCPT_KERNEL_DISK_FORCE = 6,
+       FCPT_KERNEL_VISC_FUSED = 7 };
 int fcpt_time_steps(fcpt_ctx *ctx, int32_t n_steps, int32_t use_graph, double *ms_total);
 int fcpt_time_kernel(fcpt_ctx *ctx, int32_t kernel, int32_t reps, double *ms_per_launch);
 /* Streaming-copy probe (no counterpart in the reference): copies four
@@ -217,9 +221,16 @@ int fcpt_time_copy(fcpt_ctx *ctx, int32_t reps, double *ms_per_launch, double *b
    thread of the radial sweep (FCPT_PARAM_RADIAL_TILE, 1 .. 64; 0 restores the
    size-based default) and the sweep's register budget in wavefronts per SIMD
    (FCPT_PARAM_RADIAL_WAVES, 2 or 3; default 3).  Results do not depend on
-   them.  Drops a captured step graph so that the next fcpt_run_steps captures
-   the new shape. */
-enum { FCPT_PARAM_RADIAL_TILE = 0, FCPT_PARAM_RADIAL_WAVES = 1 };
+   them.  FCPT_PARAM_FUSED_STEP (default 1) makes fcpt_step_euler,
+   fcpt_run_step and fcpt_run_steps run the source terms, the artificial
+   viscosity and the viscous stress + velocity update as one pass each; the
+   evolved fields are the same bit for bit in FP64 storage, but those steps
+   then leave Q_R and Q_PHI, and (unless the energy equation's viscous heating
+   reads them) TAU_* and DIV_V, as they were; 0 runs the separate kernels of
+   the per-function entry points above, which always store them.  Drops a
+   captured step graph so that the next fcpt_run_steps captures the new
+   shape. */
+enum { FCPT_PARAM_RADIAL_TILE = 0, FCPT_PARAM_RADIAL_WAVES = 1, FCPT_PARAM_FUSED_STEP = 2 };
 int fcpt_set_launch_param(fcpt_ctx *ctx, int32_t param, int32_t value);
 /* Autotuner (no counterpart in the reference): times every launch shape of
    the tunable kernels on the context's current state (`reps` launches each;
@@ -249,7 +260,7 @@ int fcpt_batch_run_steps(fcpt_batch *batch, int32_t n_steps, int32_t use_graph);
 /* as fcpt_batch_run_steps, timed with HIP events on the batch's stream */
 int fcpt_batch_time_steps(fcpt_batch *batch, int32_t n_steps, int32_t use_graph, double *ms_total);
 /* mean duration of one batched launch of one step kernel over all disks
-   (FCPT_KERNEL_* except FCPT_KERNEL_DISK_FORCE; the listed kernels do not
+   (FCPT_KERNEL_RADIAL_SWEEP, _AZIMUTHAL, _CFL, _POTENTIAL; they do not
    modify their inputs) */
 int fcpt_batch_time_kernel(fcpt_batch *batch, int32_t kernel, int32_t reps, double *ms_per_launch);
 

@@ -1,0 +1,80 @@
+"""The fused step kernels (source terms, artificial viscosity, viscous stress
++ update, one pass each) against the separate kernels of the per-function
+entry points, through the C-ABI.
+
+In FP64 storage the fused passes evaluate the same per-cell expressions on
+the same values, so every evolved and derived field must come out identical,
+bit for bit, after whole fixture runs (TW and SN artificial viscosity, none,
+isothermal and adiabatic, beta cooling, damping, outflow boundaries).  In
+split storage the separate kernels round the intermediate tensors (Q, tau) to
+FP32 and the fused ones keep them in FP64, so the two agree to the split
+storage's own tolerance.  The batched step (fused) is checked against single
+disks elsewhere.
+"""
+import numpy as np
+import pytest
+
+from fixtures import disk_config, load_case, rel_err, run_case
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ["SIGMA", "VRAD", "VAZI", "ENERGY", "PRESSURE", "SOUNDSPEED", "SCALE_HEIGHT",
+          "VISCOSITY", "TEMPERATURE", "QPLUS", "QMINUS"]
+CASES = ["c1_small", "c2_small", "c5_small", "mix_small", "none_small", "c1_iso"]
+
+
+def _run(meta, data, precision, fused, nsteps):
+    from fargocpt_amd.capi import Disk
+    cfg = dict(disk_config(meta))
+    cfg["precision"] = precision
+    d = Disk(cfg)
+    d.set_fused_step(fused)
+    run_case(d, meta, data, nsteps=nsteps)
+    out = {f: d.download_field(f) for f in FIELDS}
+    shifts = d.nshift()
+    d.close()
+    return out, shifts
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_fused_step_bit_identical_fp64(case):
+    meta, data = load_case(case)
+    n = min(meta["nsteps"], 20)
+    print(f"fused vs separate, FP64, case={case} steps={n}", flush=True)
+    a, sa = _run(meta, data, 0, True, n)
+    b, sb = _run(meta, data, 0, False, n)
+    assert np.array_equal(sa, sb)
+    for f in FIELDS:
+        assert np.array_equal(a[f], b[f]), (f, rel_err(a[f], b[f]))
+
+
+@pytest.mark.parametrize("case", ["c2_small", "mix_small", "c1_iso"])
+def test_fused_step_split(case):
+    meta, data = load_case(case)
+    n = min(meta["nsteps"], 20)
+    print(f"fused vs separate, split, case={case} steps={n}", flush=True)
+    a, _ = _run(meta, data, 1, True, n)
+    b, _ = _run(meta, data, 1, False, n)
+    for f in ("SIGMA", "VRAD", "VAZI", "ENERGY"):
+        assert rel_err(a[f], b[f]) <= 2e-6, (f, rel_err(a[f], b[f]))
+
+
+def test_fused_graph_matches_stepwise():
+    """Graph replay of the fused step (both views captured) against fused
+    steps launched one by one, on the adiabatic SN disk."""
+    from fargocpt_amd import workloads as W
+    from fargocpt_amd.capi import Disk
+    name = "adi_512x1536"
+    print(f"{name}: fused graph vs run_step", flush=True)
+    a = Disk(W.disk_config(name))
+    b = Disk(W.disk_config(name))
+    for d in (a, b):
+        W.start(d, name)
+    a.run_steps(25, True)
+    a.synchronize()
+    for _ in range(25):
+        b.run_step()
+    for f in ("SIGMA", "VRAD", "VAZI", "ENERGY"):
+        assert np.array_equal(a.download_field(f), b.download_field(f)), f
+    a.close()
+    b.close()
