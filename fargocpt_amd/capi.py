@@ -31,6 +31,7 @@ KERNEL_STRESS = 4      # the separate stress-tensor kernel of fcpt_viscous_stres
 KERNEL_DISK_FORCE = 6  # the disk-on-planet reduction (not part of the hydro step)
 # storage modes (fcpt_disk_desc.precision) and launch parameters of include/fcpt.h
 PREC_FP64, PREC_SPLIT32 = 0, 1
+INTEGRATOR_EULER, INTEGRATOR_LEAPFROG = 0, 1
 PARAM_RADIAL_TILE, PARAM_RADIAL_WAVES, PARAM_FUSED_STEP = 0, 1, 2
 BC = {"reflecting": 0, "zerogradient": 1, "outflow": 2}
 MAX_PLANETS = 8
@@ -60,6 +61,7 @@ class DiskDesc(C.Structure):
         ("damp_inner_scalar", C.c_int32), ("damp_inner_vector", C.c_int32),
         ("damp_outer_scalar", C.c_int32), ("damp_outer_vector", C.c_int32),
         ("omega_frame", C.c_double), ("monitor_timestep", C.c_double),
+        ("integrator", C.c_int32), ("pad_", C.c_int32),
     ]
 
 
@@ -114,7 +116,7 @@ def desc_from_dict(cfg: dict) -> tuple[DiskDesc, np.ndarray]:
         "precision": 0, "heating_cooling_cfl_limit": 1.0, "radial_viscosity_factor": 1.0,
         "cooling_beta_ramp_up": 0.0, "bc_inner": 0, "bc_outer": 0,
         "damp_inner_scalar": -1, "damp_inner_vector": -1, "damp_outer_scalar": -1,
-        "damp_outer_vector": -1,
+        "damp_outer_vector": -1, "integrator": 0, "pad_": 0,
     }
     for name, _ in DiskDesc._fields_:
         if name == "radii":
@@ -162,6 +164,9 @@ def load_library(path: str = LIB_PATH):
         "fcpt_set_time": [P, D, D, I],
         "fcpt_get_time": [P, dp, dp, C.POINTER(I)],
         "fcpt_set_step_params": [P, C.POINTER(StepParams)],
+        "fcpt_set_midstep_params": [P, C.POINTER(StepParams)],
+        "fcpt_compute_pressure": [P],
+        "fcpt_step_leapfrog": [P, D],
         "fcpt_condition_cfl": [P, dp],
         "fcpt_calculate_timestep": [P, dp],
         "fcpt_nbody_potential": [P],
@@ -285,6 +290,14 @@ class Disk:
         sp = make_step_params(planets, indirect)
         self._check(self.lib.fcpt_set_step_params(self.h, C.byref(sp)))
 
+    def set_midstep_params(self, planets, indirect=(0.0, 0.0)):
+        """Bodies and indirect term of a LeapFrog step's second kick."""
+        sp = make_step_params(planets, indirect)
+        self._check(self.lib.fcpt_set_midstep_params(self.h, C.byref(sp)))
+
+    def compute_pressure(self):
+        self._check(self.lib.fcpt_compute_pressure(self.h))
+
     # hydro step functions (names follow the reference)
     def condition_cfl(self):
         v = C.c_double()
@@ -333,6 +346,9 @@ class Disk:
 
     def step_euler(self, dt):
         self._check(self.lib.fcpt_step_euler(self.h, dt))
+
+    def step_leapfrog(self, dt):
+        self._check(self.lib.fcpt_step_leapfrog(self.h, dt))
 
     def run_step(self):
         v = C.c_double()

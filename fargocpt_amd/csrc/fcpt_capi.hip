@@ -141,6 +141,8 @@ int fcpt_create(const fcpt_disk_desc *desc, fcpt_ctx **out)
     if (desc->nr < 8 || desc->nr > 16384 || desc->nphi < 4 || desc->nphi > 262144 || !desc->radii)
         return FCPT_ERR_ARG;
     if (desc->precision != FCPT_PREC_FP64 && desc->precision != FCPT_PREC_SPLIT32) return FCPT_ERR_ARG;
+    if (desc->integrator != FCPT_INTEGRATOR_EULER && desc->integrator != FCPT_INTEGRATOR_LEAPFROG)
+        return FCPT_ERR_ARG;
     for (int i = 0; i + 1 < desc->nr + 2; ++i)
         if (!(desc->radii[i + 1] > desc->radii[i]) || !(desc->radii[i] > 0.0)) return FCPT_ERR_ARG;
     fcpt_ctx *c = new fcpt_ctx();
@@ -165,6 +167,7 @@ int fcpt_create(const fcpt_disk_desc *desc, fcpt_ctx **out)
     P.rmin = desc->rmin;
     P.rmax = desc->rmax;
     P.adiabatic = desc->adiabatic;
+    P.integrator = desc->integrator;
     P.artvisc = desc->artificial_viscosity;
     P.artvisc_dissipation = desc->artvisc_dissipation;
     P.heating_viscous = desc->heating_viscous;
@@ -684,47 +687,86 @@ int fcpt_get_nshift(fcpt_ctx *ctx, int32_t *nshift)
 }
 
 // the gas part of step_Euler (simulation.cpp:148-267); dt already in device memory
-// One gas step of step_Euler.  The fused form runs the source terms, the
+// The source part of a step (potential, sources, artificial viscosity,
+// viscosity recalculation, viscous stress and update, SubStep3) with the dt
+// and time in device memory.  The fused form runs the source terms, the
 // artificial viscosity and the viscous stress + update as one pass each; each
 // pass writes the velocities to the partner buffers, so the passes alternate
-// between the two views of the state (P and its swapped view), and the
-// transport's velocity kernel writes the final velocities to the fields.
-static void enqueue_euler(const KParams &P, hipStream_t st, int count_monitor, bool fused)
+// between the two views of the state (P and its swapped view).  Returns the
+// view that holds the velocities afterwards.
+static const KParams *enqueue_kick(const KParams &P, const KParams &Q, hipStream_t st, bool fused,
+                                   bool pressure_first)
 {
     launch_potential(P, st);
-    if (fused) {
-        const KParams Q = swapped_view(P);
-        const KParams *cur = &P, *other = &Q;
-        auto flip = [&]() { std::swap(cur, other); };
-        launch_src_fused(*cur, st);
-        flip();
-        if (P.adiabatic) launch_src_energy(*cur, st);
-        if (P.artvisc == FCPT_ARTVISC_TW || P.artvisc == FCPT_ARTVISC_SN) {
-            launch_av_fused(*cur, st, 1);
-            flip();
-        } else {
-            launch_artificial_viscosity(*cur, st, 1);  // temperature floor only
-            launch_recalc_viscosity(*cur, st);
-        }
-        launch_visc_fused(*cur, st);
-        flip();
-        if (P.adiabatic) launch_substep3(*cur, st);
-        launch_boundaries(*cur, st, 0, 0);
-        launch_transport_to(*cur, P, st);
-        launch_boundaries(P, st, 1, count_monitor ? 2 : 1);
-        launch_derived(P, st);
-        return;
+    if (!fused) {
+        if (pressure_first) launch_pressure(P, st);
+        launch_sources(P, st);
+        if (!launch_artificial_viscosity(P, st, 1)) launch_recalc_viscosity(P, st);
+        launch_stress(P, st);
+        launch_visc_update(P, st);
+        if (P.adiabatic) launch_substep3(P, st);
+        return &P;
     }
-    launch_sources(P, st);
-    if (!launch_artificial_viscosity(P, st, 1)) launch_recalc_viscosity(P, st);
-    launch_stress(P, st);
-    launch_visc_update(P, st);
-    if (P.adiabatic) launch_substep3(P, st);
-    launch_boundaries(P, st, 0, 0);
-    launch_transport(P, st);
-    // the final boundary call also advances the time (and counts the monitor)
+    const KParams *cur = &P, *other = &Q;
+    auto flip = [&]() { std::swap(cur, other); };
+    // the isothermal fused source pass forms the pressure from Sigma itself
+    if (pressure_first && P.adiabatic) launch_pressure(P, st);
+    launch_src_fused(*cur, st);
+    flip();
+    if (P.adiabatic) launch_src_energy(*cur, st);
+    if (P.artvisc == FCPT_ARTVISC_TW || P.artvisc == FCPT_ARTVISC_SN) {
+        launch_av_fused(*cur, st, 1);
+        flip();
+    } else {
+        launch_artificial_viscosity(*cur, st, 1);  // temperature floor only
+        launch_recalc_viscosity(*cur, st);
+    }
+    launch_visc_fused(*cur, st);
+    flip();
+    if (P.adiabatic) launch_substep3(*cur, st);
+    return cur;
+}
+
+// One gas step of step_Euler; the transport's velocity kernel writes the
+// final velocities to the fields of view P.
+static void enqueue_euler(const KParams &P, hipStream_t st, int count_monitor, bool fused)
+{
+    const KParams Q = swapped_view(P);
+    const KParams *cur = enqueue_kick(P, Q, st, fused, false);
+    launch_boundaries(*cur, st, 0, 0);
+    launch_transport_to(*cur, P, st);
     launch_boundaries(P, st, 1, count_monitor ? 2 : 1);
     launch_derived(P, st);
+}
+
+// One gas step of step_LeapFrog (simulation.cpp:276-459): kick with dt / 2,
+// boundaries and transport over dt, kick with dt / 2 at the midstep time and
+// bodies (compute_pressure first), final boundaries over dt, derived fields.
+// The isothermal unfused path needs the pressure of the transported Sigma,
+// as does the adiabatic one of the transported E.
+static void enqueue_leapfrog(const KParams &P, hipStream_t st, int count_monitor, bool fused)
+{
+    const KParams Q = swapped_view(P);
+    launch_leapfrog_phase(P, st, 0);
+    const KParams *cur = enqueue_kick(P, Q, st, fused, false);
+    launch_leapfrog_phase(P, st, 1);
+    launch_boundaries(*cur, st, 0, 0);
+    launch_transport_to(*cur, P, st);
+    launch_leapfrog_phase(P, st, 2);
+    cur = enqueue_kick(P, Q, st, fused, true);
+    if (cur != &P) {
+        // an odd number of fused passes: the velocities go back to the fields
+        launch_copy_velocities(*cur, st);
+    }
+    launch_leapfrog_phase(P, st, 3);
+    launch_boundaries(P, st, 1, count_monitor ? 2 : 1);
+    launch_derived(P, st);
+}
+
+static void enqueue_step(const KParams &P, hipStream_t st, int count_monitor, bool fused)
+{
+    if (P.integrator == FCPT_INTEGRATOR_LEAPFROG) enqueue_leapfrog(P, st, count_monitor, fused);
+    else enqueue_euler(P, st, count_monitor, fused);
 }
 
 int fcpt_step_euler(fcpt_ctx *ctx, double dt)
@@ -736,12 +778,39 @@ int fcpt_step_euler(fcpt_ctx *ctx, double dt)
     return FCPT_OK;
 }
 
+int fcpt_step_leapfrog(fcpt_ctx *ctx, double dt)
+{
+    CTX_CHECK(ctx);
+    launch_set_dt(ctx->P, ctx->stream, dt);
+    enqueue_leapfrog(ctx->P, ctx->stream, 0, ctx->fused);
+    LAUNCH_CHECK(ctx);
+    return FCPT_OK;
+}
+
+int fcpt_compute_pressure(fcpt_ctx *ctx)
+{
+    CTX_CHECK(ctx);
+    launch_pressure(ctx->P, ctx->stream);
+    LAUNCH_CHECK(ctx);
+    return FCPT_OK;
+}
+
+int fcpt_set_midstep_params(fcpt_ctx *ctx, const fcpt_step_params *params)
+{
+    CTX_CHECK(ctx);
+    if (!params || params->n_planets < 0 || params->n_planets > FCPT_MAX_PLANETS)
+        return fail(ctx, FCPT_ERR_ARG, "bad step parameters");
+    launch_set_mid_params(ctx->P, ctx->stream, *params);
+    LAUNCH_CHECK(ctx);
+    return FCPT_OK;
+}
+
 int fcpt_run_step(fcpt_ctx *ctx, double *dt_out)
 {
     CTX_CHECK(ctx);
     const KParams &P = ctx->P;
     launch_cfl(P, ctx->stream, 2);
-    enqueue_euler(P, ctx->stream, 1, ctx->fused);
+    enqueue_step(P, ctx->stream, 1, ctx->fused);
     LAUNCH_CHECK(ctx);
     int rc = read_scalars(ctx);
     if (rc) return rc;
@@ -758,7 +827,7 @@ static int ensure_step_graph(fcpt_ctx *ctx)
     hipGraph_t g = nullptr;
     HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
     launch_cfl(ctx->P, ctx->stream, 2);
-    enqueue_euler(ctx->P, ctx->stream, 1, ctx->fused);
+    enqueue_step(ctx->P, ctx->stream, 1, ctx->fused);
     hipError_t e = hipStreamEndCapture(ctx->stream, &g);
     if (e != hipSuccess) return fail(ctx, FCPT_ERR_DEVICE, "graph capture: %s", hipGetErrorString(e));
     e = hipGraphInstantiate(&ctx->step_graph, g, nullptr, nullptr, 0);
@@ -781,7 +850,7 @@ int fcpt_run_steps(fcpt_ctx *ctx, int32_t n_steps, int32_t use_graph)
     } else {
         for (int k = 0; k < n_steps; ++k) {
             launch_cfl(ctx->P, ctx->stream, 2);
-            enqueue_euler(ctx->P, ctx->stream, 1, ctx->fused);
+            enqueue_step(ctx->P, ctx->stream, 1, ctx->fused);
         }
         LAUNCH_CHECK(ctx);
     }
@@ -1052,9 +1121,10 @@ int fcpt_batch_create(fcpt_ctx *const *disks, int32_t n_disks, fcpt_batch **out)
     std::vector<KParams> host;
     for (int k = 0; k < n_disks; ++k) {
         fcpt_ctx *c = disks[k];
-        if (!c || c->device != b->device || c->P.split != b->split) {
+        if (!c || c->device != b->device || c->P.split != b->split ||
+            c->P.integrator != FCPT_INTEGRATOR_EULER) {
             delete b;
-            return FCPT_ERR_ARG;  // one device and one storage mode per batch
+            return FCPT_ERR_ARG;  // one device, one storage mode, Euler disks
         }
         b->disks.push_back(c);
         KParams P = c->P;

@@ -33,6 +33,14 @@ struct DevScalars {
     int pad_;
     double indirect_x, indirect_y;
     DevPlanet planet[kMaxPlanets];
+    // LeapFrog: the full step's dt and start time while the kicks run with
+    // dt / 2, the second kick's bodies (has_mid) and the first kick's, kept
+    // while the second kick uses the slots above
+    double dt_step, t_step;
+    int has_mid, n_planets_mid, n_planets_first, pad2_;
+    double indirect_mid_x, indirect_mid_y, indirect_first_x, indirect_first_y;
+    DevPlanet planet_mid[kMaxPlanets];
+    DevPlanet planet_first[kMaxPlanets];
 };
 
 struct KParams {
@@ -41,6 +49,7 @@ struct KParams {
     double rmin, rmax;
     // physics
     int adiabatic;
+    int integrator;      // FCPT_INTEGRATOR_*
     int artvisc;
     int artvisc_dissipation;
     int heating_viscous;

@@ -292,6 +292,9 @@ __device__ __forceinline__ void b_cfl_cells(const KParams &P, int bx, int by)
         const double invdt1 = SPLIT ? csv * inv_cell : csv / cell_size;
         const double invdt2 = SPLIT ? vr0 * inv_dr : vr0 / dxRadial;
         const double invdt3 = SPLIT ? v_residual * inv_daz : v_residual / dxAzimuthal;
+        // the LeapFrog scheme applies the source terms twice with dt / 2 and
+        // relaxes their limits (cfl.cpp:254-261); the Euler factor 1.0 is exact
+        const double lf = P.integrator == FCPT_INTEGRATOR_LEAPFROG ? 0.6 : 1.0;
         double invdt4;
         if (P.artvisc == FCPT_ARTVISC_SN) {
             double dvRadial = vr1 - vr0;
@@ -301,25 +304,25 @@ __device__ __forceinline__ void b_cfl_cells(const KParams &P, int bx, int by)
             if (dvAzimuthal > 0.0) dvAzimuthal = 0.0;
             else dvAzimuthal = -dvAzimuthal;
             if constexpr (SPLIT)
-                invdt4 = 4.0 * fac2 * std_max(dvRadial * inv_dr, dvAzimuthal * inv_daz) * 1.0;
+                invdt4 = 4.0 * fac2 * std_max(dvRadial * inv_dr, dvAzimuthal * inv_daz) * lf;
             else
-                invdt4 = 4.0 * fac2 * std_max(dvRadial / dxRadial, dvAzimuthal / dxAzimuthal) * 1.0;
+                invdt4 = 4.0 * fac2 * std_max(dvRadial / dxRadial, dvAzimuthal / dxAzimuthal) * lf;
         } else {
             const double eps_rr = (vr1 - vr0) * cst(P.InvDiffRsup)[i];
             const double eps_pp = cst(P.InvRmed)[i] * ((va1 - va0) * P.invdphi + 0.5 * (vr1 + vr0));
             const double mdiv_V = -std_min(eps_rr + eps_pp, 0.0);
-            invdt4 = 4.0 * fac2 * mdiv_V * 1.0;
+            invdt4 = 4.0 * fac2 * mdiv_V * lf;
         }
-        const double invdt5 = SPLIT ? 4.0 * nuv * (inv_cell * inv_cell)
-                                    : 4.0 * nuv / (cell_size * cell_size) * 1.0;
+        const double invdt5 = SPLIT ? 4.0 * nuv * (inv_cell * inv_cell) * lf
+                                    : 4.0 * nuv / (cell_size * cell_size) * lf;
         double invdt6 = 0.0;
         if (P.adiabatic) {
             FLD(qp, FCPT_QPLUS);
             FLD(qm, FCPT_QMINUS);
             FLD(E, FCPT_ENERGY);
             const double inv_limit = 1.0 / P.heating_cooling_cfl_limit;
-            if constexpr (SPLIT) invdt6 = inv_limit * fabs((qp(i, j) - qm(i, j)) * recip<true>(E(i, j)));
-            else invdt6 = inv_limit * fabs((qp(i, j) - qm(i, j)) / E(i, j)) * 1.0;
+            if constexpr (SPLIT) invdt6 = inv_limit * fabs((qp(i, j) - qm(i, j)) * recip<true>(E(i, j))) * lf;
+            else invdt6 = inv_limit * fabs((qp(i, j) - qm(i, j)) / E(i, j)) * lf;
         }
         const double sum2 = invdt1 * invdt1 + invdt2 * invdt2 + invdt3 * invdt3 + invdt4 * invdt4 +
                             invdt5 * invdt5 + invdt6 * invdt6;
@@ -400,6 +403,62 @@ __global__ void k_set_params(KParams P, fcpt_step_params sp)
     }
     s->indirect_x = sp.indirect_x;
     s->indirect_y = sp.indirect_y;
+    s->has_mid = 0;
+}
+
+// the bodies of a LeapFrog step's second kick
+__global__ void k_set_mid_params(KParams P, fcpt_step_params sp)
+{
+    DevScalars *s = P.s;
+    s->n_planets_mid = sp.n_planets;
+    for (int k = 0; k < sp.n_planets; ++k) {
+        s->planet_mid[k].x = sp.planet[k].x;
+        s->planet_mid[k].y = sp.planet[k].y;
+        s->planet_mid[k].mass = sp.planet[k].mass;
+        s->planet_mid[k].r_cubic = sp.planet[k].r_cubic;
+    }
+    s->indirect_mid_x = sp.indirect_x;
+    s->indirect_mid_y = sp.indirect_y;
+    s->has_mid = 1;
+}
+
+// step_LeapFrog's clock (simulation.cpp:278-281, 378, 441): phase 0 before
+// the first kick (dt / 2 at the start time), 1 before the drift (dt), 2 before
+// the second kick (dt / 2 at the midstep time, the midstep bodies), 3 before
+// the final boundaries (dt at the start time: they advance the time to
+// start + dt, the reference's end_time)
+__global__ void k_leapfrog_phase(KParams P, int phase)
+{
+    DevScalars *s = P.s;
+    if (phase == 0) {
+        s->dt_step = s->dt;
+        s->t_step = s->time;
+        s->dt = s->dt_step / 2;
+    } else if (phase == 1) {
+        s->dt = s->dt_step;
+    } else if (phase == 2) {
+        s->dt = s->dt_step / 2;
+        s->time = s->t_step + s->dt;
+        if (s->has_mid) {
+            s->n_planets_first = s->n_planets;
+            s->indirect_first_x = s->indirect_x;
+            s->indirect_first_y = s->indirect_y;
+            for (int k = 0; k < kMaxPlanets; ++k) s->planet_first[k] = s->planet[k];
+            s->n_planets = s->n_planets_mid;
+            s->indirect_x = s->indirect_mid_x;
+            s->indirect_y = s->indirect_mid_y;
+            for (int k = 0; k < kMaxPlanets; ++k) s->planet[k] = s->planet_mid[k];
+        }
+    } else {
+        s->dt = s->dt_step;
+        s->time = s->t_step;
+        if (s->has_mid) {
+            s->n_planets = s->n_planets_first;
+            s->indirect_x = s->indirect_first_x;
+            s->indirect_y = s->indirect_first_y;
+            for (int k = 0; k < kMaxPlanets; ++k) s->planet[k] = s->planet_first[k];
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -567,6 +626,14 @@ FCPT_PAIR0(temperature_floor, 256)
 // compute_sound_speed / compute_scale_height / update_viscosity for one cell
 // (SourceEuler.cpp:957-995, 1121-1154; viscosity.cpp:98-137), as a load of
 // the inputs and a store of the results
+// locally isothermal pressure Sigma c_s^2 (the value recalculate_derived
+// stores, SourceEuler.cpp:225-249, with the ring profile of c_s)
+__device__ __forceinline__ double iso_pressure(const KParams &P, int i, double sig)
+{
+    const double c_s = cst(P.iso_cs)[i];
+    return sig * (c_s * c_s);
+}
+
 struct DerivedIn {
     double e, sig, cs, h;
 };
@@ -868,6 +935,18 @@ __device__ __forceinline__ void b_derived(const KParams &P, int bx, int by)
 }
 FCPT_PAIR0(derived, 256)
 
+// compute_pressure (SourceEuler.cpp): (gamma - 1) E, or Sigma c_s^2 with the
+// isothermal ring profile of c_s
+template <bool SPLIT>
+__device__ __forceinline__ void b_pressure(const KParams &P, int bx, int by)
+{
+    CELL_LOOP_HEAD(0, P.nr)
+    const double v = P.adiabatic ? (P.gamma - 1.0) * fld<SPLIT>(P, FCPT_ENERGY)(i, j)
+                                 : iso_pressure(P, i, fld<SPLIT>(P, FCPT_SIGMA)(i, j));
+    fld<SPLIT>(P, FCPT_PRESSURE).set(i, j, v);
+}
+FCPT_PAIR0(pressure, 256)
+
 // locally isothermal sound speed and scale height from the initial setup
 // (compute_sound_speed_normal / compute_scale_height_old, isothermal branches)
 template <bool SPLIT>
@@ -1033,14 +1112,6 @@ template <bool SPLIT>
 __device__ __forceinline__ Fld<SPLIT> alt_vaz(const KParams &P)
 {
     return buf<SPLIT>(P, P.valt[1], P.bg[FCPT_VAZI]);
-}
-
-// locally isothermal pressure Sigma c_s^2 (the value recalculate_derived
-// stores, SourceEuler.cpp:225-249, with the ring profile of c_s)
-__device__ __forceinline__ double iso_pressure(const KParams &P, int i, double sig)
-{
-    const double c_s = cst(P.iso_cs)[i];
-    return sig * (c_s * c_s);
 }
 
 // update_with_sourceterms' two momentum updates (SourceEuler.cpp:325-428) in
@@ -1262,6 +1333,19 @@ __device__ __forceinline__ void b_visc_fused(const KParams &P, int bx, int by)
     }
 }
 FCPT_PAIR0(visc_fused, 256)
+
+// the velocities of view P copied as stored to its partner buffers (a LeapFrog
+// step whose second kick ends on the swapped view hands them back this way)
+template <bool SPLIT>
+__device__ __forceinline__ void b_copy_velocities(const KParams &P, int bx, int by)
+{
+    CELL_LOOP_HEAD(0, P.nr + 1)
+    FLD(vr, FCPT_VRAD);
+    FLD(vaz, FCPT_VAZI);
+    alt_vr<SPLIT>(P).set_raw(i, j, vr.raw(i, j));
+    if (i < P.nr) alt_vaz<SPLIT>(P).set_raw(i, j, vaz.raw(i, j));
+}
+FCPT_PAIR0(copy_velocities, 256)
 
 // ---------------------------------------------------------------------------
 // Q- (thermal_relaxation, SourceEuler.cpp:632-690) and Q+ (viscous_heating,
@@ -2028,6 +2112,19 @@ void launch_set_time(const KParams &P, hipStream_t st, double t, double last_dt,
 void launch_set_params(const KParams &P, hipStream_t st, const fcpt_step_params &sp)
 {
     hipLaunchKernelGGL(k_set_params, dim3(1), dim3(1), 0, st, P, sp);
+}
+void launch_set_mid_params(const KParams &P, hipStream_t st, const fcpt_step_params &sp)
+{
+    hipLaunchKernelGGL(k_set_mid_params, dim3(1), dim3(1), 0, st, P, sp);
+}
+void launch_leapfrog_phase(const KParams &P, hipStream_t st, int phase)
+{
+    hipLaunchKernelGGL(k_leapfrog_phase, dim3(1), dim3(1), 0, st, P, phase);
+}
+void launch_pressure(const KParams &P, hipStream_t st) { LAUNCH(k_pressure, cell_grid(P, P.nr), kBlock, st, P); }
+void launch_copy_velocities(const KParams &P, hipStream_t st)
+{
+    LAUNCH(k_copy_velocities, cell_grid(P, P.nr + 1), kBlock, st, P);
 }
 
 void launch_potential(const KParams &P, hipStream_t st)

@@ -26,6 +26,10 @@ void launch_substep3(const KParams &P, hipStream_t st);
 void launch_heatcool_cfl(const KParams &P, hipStream_t st);
 void launch_boundaries(const KParams &P, hipStream_t st, int final_call, int advance_time);
 void launch_transport(const KParams &P, hipStream_t st);
+void launch_set_mid_params(const KParams &P, hipStream_t st, const fcpt_step_params &sp);
+void launch_leapfrog_phase(const KParams &P, hipStream_t st, int phase);
+void launch_pressure(const KParams &P, hipStream_t st);
+void launch_copy_velocities(const KParams &P, hipStream_t st);
 void launch_transport_to(const KParams &P, const KParams &out, hipStream_t st);
 // the fused step kernels write the velocities of view P to P.valt; the
 // swapped view reads those buffers and writes the fields

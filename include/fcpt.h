@@ -53,6 +53,10 @@ enum { FCPT_BETA_REF_ZERO = 0, FCPT_BETA_REF_REFERENCE = 1, FCPT_BETA_REF_FLOOR 
                       fcpt_upload_field and the profiles derived from them;
                       downloads return background + deviation in FP64. */
 enum { FCPT_PREC_FP64 = 0, FCPT_PREC_SPLIT32 = 1 };
+/* hydro integrator (reference types.h:64-65, Interpret.cpp:173-186): the
+   LeapFrog scheme also scales the artificial-viscosity, viscosity and
+   heating/cooling limits of the CFL condition by 0.6 (cfl.cpp:254-261) */
+enum { FCPT_INTEGRATOR_EULER = 0, FCPT_INTEGRATOR_LEAPFROG = 1 };
 
 /* field identifiers for fcpt_upload_field / fcpt_download_field */
 typedef enum fcpt_field {
@@ -106,6 +110,8 @@ typedef struct fcpt_disk_desc {
     int32_t damp_inner_scalar, damp_inner_vector, damp_outer_scalar, damp_outer_vector;
     double omega_frame;         /* refframe::OmegaFrame (fixed frame rotation) */
     double monitor_timestep;    /* parameters::monitor_timestep for the dt clip */
+    int32_t integrator;         /* FCPT_INTEGRATOR_* (parameters::hydro_integrator) */
+    int32_t pad_;
 } fcpt_disk_desc;
 
 #define FCPT_MAX_PLANETS 8
@@ -145,6 +151,11 @@ int fcpt_download_field(fcpt_ctx *ctx, int32_t field, double *host);
 int fcpt_set_time(fcpt_ctx *ctx, double time, double last_dt, int32_t n_monitor);
 int fcpt_get_time(fcpt_ctx *ctx, double *time, double *last_dt, int32_t *n_monitor);
 int fcpt_set_step_params(fcpt_ctx *ctx, const fcpt_step_params *params);
+/* The bodies and indirect term of a LeapFrog step's second kick (the N-body
+   state at midstep_time, simulation.cpp:347-361), used by fcpt_step_leapfrog,
+   fcpt_run_step and fcpt_run_steps.  fcpt_set_step_params clears them: the
+   second kick then sees the same bodies as the first. */
+int fcpt_set_midstep_params(fcpt_ctx *ctx, const fcpt_step_params *params);
 
 /* ---- the hydro step, one entry point per reference function ------------ */
 /* cfl::condition_cfl(t_data&, double)                      cfl.h:8, cfl.cpp:185-382 */
@@ -173,6 +184,10 @@ int fcpt_apply_boundary_condition(fcpt_ctx *ctx, double dt, int32_t final_call);
 int fcpt_transport(fcpt_ctx *ctx, double dt);
 /* recalculate_derived_disk_quantities(t_data&, double) SourceEuler.h:11, SourceEuler.cpp:225-249 */
 int fcpt_recalculate_derived(fcpt_ctx *ctx);
+/* compute_pressure(t_data&)  SourceEuler.h, SourceEuler.cpp (the pressure from
+   Sigma and c_s, or from E; step_LeapFrog calls it before its second kick,
+   simulation.cpp:374) */
+int fcpt_compute_pressure(fcpt_ctx *ctx);
 /* FARGO integer shifts of the last fcpt_transport call (TransportEuler.cpp:215-220) */
 int fcpt_get_nshift(fcpt_ctx *ctx, int32_t *nshift);
 
@@ -182,8 +197,16 @@ int fcpt_get_nshift(fcpt_ctx *ctx, int32_t *nshift);
    final boundaries and derived fields, with the step parameters last set.
    Advances the context time by dt. */
 int fcpt_step_euler(fcpt_ctx *ctx, double dt);
+/* step_LeapFrog(t_data&, double) (simulation.cpp:276-459) for the gas: kick
+   with dt/2 at the start time (potential of the step parameters, sources,
+   artificial viscosity, viscosity, SubStep3), boundaries, transport over dt,
+   kick with dt/2 at the midstep time (potential of the midstep parameters,
+   compute_pressure first), final boundaries and derived fields.  Advances
+   the context time by dt. */
+int fcpt_step_leapfrog(fcpt_ctx *ctx, double dt);
 /* One iteration of sim::run (simulation.cpp:515-556): CalculateTimeStep, the
-   clip to the next monitor time, step_Euler.  *dt_out receives the step size. */
+   clip to the next monitor time, step_Euler or step_LeapFrog (the descriptor's
+   integrator).  *dt_out receives the step size. */
 int fcpt_run_step(fcpt_ctx *ctx, double *dt_out);
 
 /* n_steps iterations of sim::run entirely on the device: the CFL time step,
@@ -242,8 +265,8 @@ int fcpt_autotune(fcpt_ctx *ctx, int32_t reps, int32_t *radial_tile, int32_t *ra
 /* Many independent disks advanced together: each stage of a sim::run
    iteration is one launch over a work table holding every block of every
    disk taking part in it (disks of any size, physics and planets, on one
-   device and in one storage mode).  A batched disk evolves bit for bit like
-   the same disk under fcpt_run_steps.  The batch uses the disks' parameters
+   device and in one storage mode, Euler integrator).  A batched disk evolves
+   bit for bit like the same disk under fcpt_run_steps.  The batch uses the disks' parameters
    and launch shapes as they are at fcpt_batch_create and reads and writes
    the disks' own device state: between batch calls the disks may be read
    (fcpt_download_*, fcpt_totals) but not stepped on their own. */

@@ -44,6 +44,9 @@ def load():
         "fo_set_time": ([P, D, D, I], None),
         "fo_get_time": ([P, dp, dp, C.POINTER(I)], None),
         "fo_set_step_params": ([P, C.POINTER(StepParams)], None),
+        "fo_set_midstep_params": ([P, C.POINTER(StepParams)], None),
+        "fo_step_leapfrog": ([P, D], None),
+        "fo_compute_pressure": ([P], None),
         "fo_geom": ([P, C.c_char_p], dp),
         "fo_damping_ids": ([P, C.POINTER(I)], I),
         "fo_condition_cfl": ([P], D),
@@ -179,6 +182,16 @@ class OracleDisk:
 
     def step_euler(self, dt):
         self.lib.fo_step_euler(self.h, dt)
+
+    def step_leapfrog(self, dt):
+        self.lib.fo_step_leapfrog(self.h, dt)
+
+    def compute_pressure(self):
+        self.lib.fo_compute_pressure(self.h)
+
+    def set_midstep_params(self, planets, indirect=(0.0, 0.0)):
+        sp = make_step_params(planets, indirect)
+        self.lib.fo_set_midstep_params(self.h, C.byref(sp))
 
     def run_step(self):
         return self.lib.fo_run_step(self.h)

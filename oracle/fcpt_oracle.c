@@ -31,6 +31,8 @@ struct fo_disk {
     double time, last_dt;
     int n_monitor;
     fcpt_step_params sp;
+    fcpt_step_params sp_mid; /* LeapFrog second kick (has_mid) */
+    int has_mid;
 };
 
 static int rows_of(int field, int nr)
@@ -209,7 +211,17 @@ void fo_get_time(const fo_disk *d, double *time, double *last_dt, int *n_monitor
     *n_monitor = d->n_monitor;
 }
 
-void fo_set_step_params(fo_disk *d, const fcpt_step_params *p) { d->sp = *p; }
+void fo_set_step_params(fo_disk *d, const fcpt_step_params *p)
+{
+    d->sp = *p;
+    d->has_mid = 0;
+}
+
+void fo_set_midstep_params(fo_disk *d, const fcpt_step_params *p)
+{
+    d->sp_mid = *p;
+    d->has_mid = 1;
+}
 
 /* ---------------------------------------------------------------------- */
 /* derived fields                                                          */
@@ -400,6 +412,8 @@ double fo_condition_cfl(fo_disk *d)
     const double denom0 = fabs(v_mean[0] * d->InvRmed[0] - v_mean[1] * d->InvRmed[1]) + 1.0e-100;
     double dt_core = CFL * dphi / denom0;
     const double fac2 = pow(d->p.artvisc_factor, 2);
+    /* leapfrog_cfl_factor, cfl.cpp:254-261 */
+    const double lf = d->p.integrator == FCPT_INTEGRATOR_LEAPFROG ? 0.6 : 1.0;
 #pragma omp parallel for reduction(min : dt_core)
     for (int i = 1; i < nr - 1; ++i) {
         const double denom =
@@ -424,20 +438,20 @@ double fo_condition_cfl(fo_disk *d)
                 if (dvAzimuthal > 0.0) dvAzimuthal = 0.0;
                 else dvAzimuthal = -dvAzimuthal;
                 const double a = dvRadial / dxRadial, b = dvAzimuthal / dxAzimuthal;
-                invdt4 = 4.0 * fac2 * std_max(a, b) * 1.0;
+                invdt4 = 4.0 * fac2 * std_max(a, b) * lf;
             } else {
                 const double eps_rr = (vr[IDX(i + 1, j)] - vr[IDX(i, j)]) * d->InvDiffRsup[i];
                 const double eps_pp =
                     d->InvRmed[i] * ((vaz[IDX(i, jn)] - vaz[IDX(i, j)]) * d->invdphi +
                                      0.5 * (vr[IDX(i + 1, j)] + vr[IDX(i, j)]));
                 const double mdiv_V = -std_min(eps_rr + eps_pp, 0.0);
-                invdt4 = 4.0 * fac2 * mdiv_V * 1.0;
+                invdt4 = 4.0 * fac2 * mdiv_V * lf;
             }
-            const double invdt5 = 4.0 * nu[IDX(i, j)] / pow(cell_size, 2) * 1.0;
+            const double invdt5 = 4.0 * nu[IDX(i, j)] / pow(cell_size, 2) * lf;
             double invdt6 = 0.0;
             if (d->p.adiabatic) {
                 const double inv_limit = 1.0 / d->p.heating_cooling_cfl_limit;
-                invdt6 = inv_limit * fabs((Qp[IDX(i, j)] - Qm[IDX(i, j)]) / E[IDX(i, j)]) * 1.0;
+                invdt6 = inv_limit * fabs((Qp[IDX(i, j)] - Qm[IDX(i, j)]) / E[IDX(i, j)]) * lf;
             }
             const double dt_cell =
                 CFL / sqrt(pow(invdt1, 2) + pow(invdt2, 2) + pow(invdt3, 2) + pow(invdt4, 2) +
@@ -1178,6 +1192,43 @@ void fo_recalculate_derived(fo_disk *d)
     update_viscosity(d);
 }
 
+void fo_compute_pressure(fo_disk *d) { compute_pressure(d); }
+
+/* the source part of one LeapFrog kick (simulation.cpp:319-340, 364-385) */
+static void leapfrog_kick(fo_disk *d, double kick_time, double frog_dt)
+{
+    fo_update_with_sourceterms(d, frog_dt);
+    fo_artificial_viscosity(d, frog_dt);
+    fo_recalculate_viscosity(d);
+    fo_viscous_stress_tensor(d);
+    fo_update_velocities_with_viscosity(d, frog_dt);
+    if (d->p.adiabatic) fo_substep3(d, kick_time, frog_dt);
+}
+
+/* step_LeapFrog gas part, simulation.cpp:276-459: kick dt/2 with the bodies
+   of the step parameters, drift (boundaries, transport over dt), kick dt/2
+   with the midstep bodies after compute_pressure, final boundaries, derived */
+void fo_step_leapfrog(fo_disk *d, double step_dt)
+{
+    const double frog_dt = step_dt / 2;
+    const double start_time = d->time;
+    const double midstep_time = d->time + frog_dt;
+    const double end_time = d->time + step_dt;
+    fo_nbody_potential(d);
+    leapfrog_kick(d, start_time, frog_dt);
+    fo_apply_boundary_condition(d, 0.0, 0);
+    fo_transport(d, step_dt);
+    const fcpt_step_params first = d->sp;
+    if (d->has_mid) d->sp = d->sp_mid;
+    fo_nbody_potential(d);
+    d->sp = first;
+    compute_pressure(d);
+    leapfrog_kick(d, midstep_time, frog_dt);
+    d->time = end_time;
+    fo_apply_boundary_condition(d, step_dt, 1);
+    fo_recalculate_derived(d);
+}
+
 /* step_Euler gas part, simulation.cpp:148-267 */
 void fo_step_euler(fo_disk *d, double dt)
 {
@@ -1205,7 +1256,8 @@ double fo_run_step(fo_disk *d)
     const double dt_stretch_factor = 0.05;
     const int almost_there = time_left_till_write < cfl_dt * (1 + dt_stretch_factor);
     const double step_dt = (overshoot || almost_there) ? time_left_till_write : cfl_dt;
-    fo_step_euler(d, step_dt);
+    if (d->p.integrator == FCPT_INTEGRATOR_LEAPFROG) fo_step_leapfrog(d, step_dt);
+    else fo_step_euler(d, step_dt);
     if (fabs(time_next_monitor - d->time) < 1e-6 * cfl_dt) d->n_monitor++;
     return step_dt;
 }

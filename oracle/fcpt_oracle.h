@@ -34,6 +34,7 @@ void fo_set_reference(fo_disk *d, const double *sigma0, const double *vrad0,
 void fo_set_time(fo_disk *d, double time, double last_dt, int n_monitor);
 void fo_get_time(const fo_disk *d, double *time, double *last_dt, int *n_monitor);
 void fo_set_step_params(fo_disk *d, const fcpt_step_params *p);
+void fo_set_midstep_params(fo_disk *d, const fcpt_step_params *p);
 /* geometry: name is one of Radii Rinf Rsup Rmed Surf InvSurf InvRmed InvRinf
    InvDiffRmed InvDiffRsup InvDiffRsupRb TwoDiffRaSq; returns NULL if unknown */
 const double *fo_geom(const fo_disk *d, const char *name);
@@ -53,6 +54,8 @@ void fo_transport(fo_disk *d, double dt);
 void fo_recalculate_derived(fo_disk *d);
 const int32_t *fo_nshift(const fo_disk *d);
 void fo_step_euler(fo_disk *d, double dt);
+void fo_step_leapfrog(fo_disk *d, double dt);
+void fo_compute_pressure(fo_disk *d);
 double fo_run_step(fo_disk *d);
 void fo_totals(fo_disk *d, double radius, double *mass, double *angmom);
 void fo_disk_on_planet_accel(fo_disk *d, const fcpt_force_body *bodies, int n_bodies, int flags,

@@ -76,6 +76,7 @@ def disk_config(meta: dict) -> dict:
         "damp_inner_scalar": -1, "damp_inner_vector": -1,
         "damp_outer_scalar": -1, "damp_outer_vector": -1,
         "omega_frame": meta["OmegaFrame"], "monitor_timestep": meta["monitor_timestep"],
+        "integrator": meta.get("integrator", 0),
     }
 
 
@@ -108,6 +109,15 @@ def step_inputs(data, step: int):
     planets = [tuple(p) for p in pl]
     rec = data["steps"][step - 1]
     return planets, (rec[6], rec[7])
+
+
+def step_inputs_mid(data, step: int):
+    """(planets, indirect) of a LeapFrog step's second kick, or None."""
+    if "planets_mid" not in data.files:
+        return None
+    pl = data["planets_mid"][step - 1]
+    rec = data["steps"][step - 1]
+    return [tuple(p) for p in pl], (rec[10], rec[11])
 
 
 def stage_states(data, meta: dict, step: int):
@@ -224,6 +234,9 @@ def run_case(disk, meta, data, nsteps=None):
     for s in range(1, nsteps + 1):
         planets, ind = step_inputs(data, s)
         disk.set_step_params(planets, ind)
+        mid = step_inputs_mid(data, s)
+        if mid is not None:
+            disk.set_midstep_params(*mid)
         dt = disk.run_step()
         ref_dt = data["steps"][s - 1][3]
         worst_dt = max(worst_dt, abs(dt - ref_dt) / ref_dt)

@@ -12,8 +12,9 @@ Runs only in a container that holds the reference sources at
    units (sigma0, G), write the package's seeded initial condition
    (fargocpt_amd/ic.py), run the driver, and pack its outputs into
    ``tests/golden/<case>.npz`` (float64 arrays, int32 shifts, meta as JSON);
-3. run one case a second time through the reference's own ``sim::run`` and
-   check that the driver's step loop reproduces it bit for bit.
+3. run the Euler case c1_small and the LeapFrog cases a second time through
+   the reference's own ``sim::run`` and check that the driver's step loops
+   reproduce them bit for bit.
 
 Usage:  python tests/golden/make_golden.py [case ...]
 """
@@ -94,6 +95,14 @@ CASES = {
                    PLANET_1, 20, 1, [1, 5, 20], True),
     # adiabatic 512 x 1536, 500 steps: only per-ring moments and a few full rings kept
     "c2_adi": (dict(ADIABATIC, Nrad="512", Naz="1536"), PLANET_1, 500, 0, [500], False),
+    # the LeapFrog integrator (kick dt/2, drift dt, kick dt/2): the isothermal 32 x 64
+    # disk, and an adiabatic one with beta cooling (ramp-up, so SubStep3's time
+    # matters), a cubic-smoothed planet and a planet whose mass ramps up (the
+    # two kicks see different bodies)
+    "lf_iso": ({"Nrad": "32", "Naz": "64", "Integrator": "Leapfrog"}, PLANET_1, 20, 1,
+               [1, 5, 20], True),
+    "lf_adi": (dict(ADIABATIC, Nrad="32", Naz="64", Integrator="Leapfrog", CoolingBetaLocal="yes",
+                    CoolingBetaRampUp="1.0"), PLANETS_MIX, 20, 1, [1, 5, 20], True),
 }
 
 
@@ -209,7 +218,13 @@ def make_case(name, exe, addr, keep_dir=None):
         steps = load_bin(os.path.join(dump, "steps.bin")).reshape(nsteps, 16)
         out["steps"] = steps
         npl = meta["n_planets"]
-        out["planets"] = load_bin(os.path.join(dump, "planets.bin")).reshape(nsteps, npl, 4)
+        if meta.get("integrator", 0) == 1:
+            # LeapFrog: the bodies of the first kick, then of the second
+            pl = load_bin(os.path.join(dump, "planets.bin")).reshape(nsteps, 2, npl, 4)
+            out["planets"] = pl[:, 0]
+            out["planets_mid"] = pl[:, 1]
+        else:
+            out["planets"] = load_bin(os.path.join(dump, "planets.bin")).reshape(nsteps, npl, 4)
         out["nshift"] = load_bin(os.path.join(dump, "nshift.bin"), np.int32).reshape(nsteps, nr)
         # the disk's gravity on every body after each step (x, y, r, Klahr radius, ax, ay)
         out["diskacc"] = load_bin(os.path.join(dump, "diskacc.bin")).reshape(nsteps, npl, 6)
@@ -264,9 +279,8 @@ def make_case(name, exe, addr, keep_dir=None):
         shutil.rmtree(work, ignore_errors=True)
 
 
-def check_simrun(exe, addr):
+def check_simrun(exe, addr, name="c1_small"):
     """The driver's step loop must equal the reference's sim::run bit for bit."""
-    name = "c1_small"
     overrides, planets, nsteps, _, _, _ = CASES[name]
     data = np.load(os.path.join(HERE, f"{name}.npz"))
     meta = json.loads(bytes(data["meta_json"]).decode())
@@ -286,11 +300,11 @@ def check_simrun(exe, addr):
             ic[f].tofile(os.path.join(icdir, f + ".bin"))
         dump = run_driver(exe, addr, cfg, os.path.join(work, "run"), icdir, nsteps, 0, [],
                           simrun=True)
-        for f in ("sigma", "vrad", "vazi"):
+        for f in ("sigma", "vrad", "vazi", "energy"):
             a = load_bin(os.path.join(dump, f"state_simrun_{f}.bin"))
             b = data[f"state_{nsteps}_{f}"].ravel()
-            assert np.array_equal(a, b), f"sim::run differs from driver loop in {f}"
-        print("driver loop == sim::run (bit-exact)")
+            assert np.array_equal(a, b), f"{name}: sim::run differs from driver loop in {f}"
+        print(f"{name}: driver loop == sim::run (bit-exact)")
     finally:
         shutil.rmtree(work, ignore_errors=True)
 
@@ -302,8 +316,9 @@ def main(argv):
     for name in names:
         make_case(name, exe, addr)
         print("wrote", name)
-    if "c1_small" in names:
-        check_simrun(exe, addr)
+    for name in ("c1_small", "lf_iso", "lf_adi"):
+        if name in names:
+            check_simrun(exe, addr, name)
 
 
 if __name__ == "__main__":

@@ -19,7 +19,8 @@
 //   meta.txt                key = value lines (constants, sizes, damping ids)
 //   geom_<name>.bin         1-D radial arrays (float64)
 //   steps.bin               per step: 16 float64 values (see STEP_RECORD)
-//   planets.bin             per step: n_planets x 4 float64 (x, y, m, r_cubic)
+//   planets.bin             per step: n_planets x 4 float64 (x, y, m, r_cubic), twice
+//                           per step for LeapFrog (first kick, then second kick)
 //   nshift.bin              per step: Nr int32 FARGO shifts
 //   diskacc.bin             per step, after it: n_planets x 6 float64 (x, y, r, Klahr
 //                           radius, ax, ay) of ComputeDiskOnPlanetAccel, one thread
@@ -227,6 +228,117 @@ static void euler_step(t_data &data, double dt, unsigned step, bool stages,
     if (stages) dump_stage(data, step, k++, "derived");
 }
 
+// Same order of calls as step_LeapFrog (src/simulation.cpp:276-459) for the
+// same physics: gas kick with dt/2 at the start positions, drift (transport
+// over dt), second kick with dt/2 at the midstep positions, the N-body drift
+// and kick halves around them.  The potential inputs of both kicks are
+// recorded (planet_rec holds the first kick's bodies, then the second's;
+// ind_term[0..1] and ind_term[4..5] the two indirect terms).
+static void leapfrog_step(t_data &data, const double step_dt, unsigned step, bool stages,
+			  std::vector<double> &planet_rec, double *ind_term)
+{
+    int k = 0;
+    const double frog_dt = step_dt / 2;
+    const double start_time = sim::time;
+    const double midstep_time = sim::time + frog_dt;
+    const double end_time = sim::time + step_dt;
+    const unsigned np = data.get_planetary_system().get_number_of_planets();
+    auto record = [&](double *ind) {
+	for (unsigned p = 0; p < np; ++p) {
+	    planet_rec.push_back(g_xpl[p]);
+	    planet_rec.push_back(g_ypl[p]);
+	    planet_rec.push_back(g_mpl[p]);
+	    planet_rec.push_back(g_cubic_smoothing_radius[p]);
+	}
+	ind[0] = refframe::IndirectTerm.x;
+	ind[1] = refframe::IndirectTerm.y;
+    };
+
+    refframe::ComputeIndirectTermNbody(data, start_time, frog_dt);
+    refframe::init_corotation(data);
+    data.get_planetary_system().integrate(start_time, frog_dt);
+    data.get_planetary_system().copy_data_from_rebound();
+    data.get_planetary_system().move_to_hydro_center_and_update_orbital_parameters();
+    if (parameters::disk_feedback) {
+	ComputeDiskOnNbodyAccel(data);
+    }
+    refframe::ComputeIndirectTermDisk(data, frog_dt);
+    refframe::ComputeIndirectTermFully();
+    accretion::AccreteOntoPlanets(data, frog_dt);
+    if (parameters::disk_feedback) {
+	UpdatePlanetVelocitiesWithDiskForce(data, frog_dt);
+    }
+    data.get_planetary_system().apply_indirect_term_on_Nbody(refframe::IndirectTerm, frog_dt);
+    refframe::handle_corotation(data, frog_dt);
+
+    if (stages) dump_stage(data, step, k++, "start");
+    CalculateNbodyPotential(data, start_time);
+    record(&ind_term[0]);
+    if (stages) dump_stage(data, step, k++, "potential");
+    update_with_sourceterms(data, frog_dt);
+    if (stages) dump_stage(data, step, k++, "sources");
+    art_visc::update_with_artificial_viscosity(data, frog_dt);
+    if (stages) dump_stage(data, step, k++, "artvisc");
+    recalculate_viscosity(data, start_time);
+    viscosity::compute_viscous_stress_tensor(data);
+    viscosity::update_velocities_with_viscosity(data, frog_dt);
+    if (stages) dump_stage(data, step, k++, "viscupdate");
+    if (parameters::Adiabatic) {
+	SubStep3(data, start_time, frog_dt);
+	if (stages) dump_stage(data, step, k++, "substep3");
+    }
+    boundary_conditions::apply_boundary_condition(data, start_time, 0.0, false);
+    Transport(data, &data[t_data::SIGMA], &data[t_data::V_RADIAL],
+	      &data[t_data::V_AZIMUTHAL], &data[t_data::ENERGY], step_dt);
+    if (g_nshift) {
+	append_raw("nshift.bin", g_nshift, sizeof(int) * NRadial);
+    }
+    if (stages) dump_stage(data, step, k++, "transport");
+
+    if (parameters::disk_feedback) {
+	ComputeDiskOnNbodyAccel(data);
+    }
+    refframe::ComputeIndirectTermDisk(data, frog_dt);
+    refframe::ComputeIndirectTermNbody(data, midstep_time, frog_dt);
+    refframe::ComputeIndirectTermFully();
+    CalculateNbodyPotential(data, midstep_time);
+    record(&ind_term[4]);
+    if (stages) dump_stage(data, step, k++, "potential2");
+    compute_pressure(data);
+    update_with_sourceterms(data, frog_dt);
+    if (stages) dump_stage(data, step, k++, "sources2");
+    art_visc::update_with_artificial_viscosity(data, frog_dt);
+    if (stages) dump_stage(data, step, k++, "artvisc2");
+    recalculate_viscosity(data, midstep_time);
+    viscosity::compute_viscous_stress_tensor(data);
+    viscosity::update_velocities_with_viscosity(data, frog_dt);
+    if (stages) dump_stage(data, step, k++, "viscupdate2");
+    if (parameters::Adiabatic) {
+	SubStep3(data, midstep_time, frog_dt);
+	if (stages) dump_stage(data, step, k++, "substep3_2");
+    }
+
+    accretion::AccreteOntoPlanets(data, frog_dt);
+    if (parameters::disk_feedback) {
+	UpdatePlanetVelocitiesWithDiskForce(data, frog_dt);
+    }
+    data.get_planetary_system().apply_indirect_term_on_Nbody(refframe::IndirectTerm, frog_dt);
+    refframe::init_corotation(data);
+    data.get_planetary_system().integrate(midstep_time, frog_dt);
+    data.get_planetary_system().copy_data_from_rebound();
+    data.get_planetary_system().move_to_hydro_center_and_update_orbital_parameters();
+    refframe::handle_corotation(data, frog_dt);
+
+    sim::time = end_time;
+    sim::N_hydro_iter = sim::N_hydro_iter + 1;
+    CommunicateBoundaries(&data[t_data::SIGMA], &data[t_data::V_RADIAL],
+			  &data[t_data::V_AZIMUTHAL], &data[t_data::ENERGY]);
+    boundary_conditions::apply_boundary_condition(data, end_time, step_dt, true);
+    if (stages) dump_stage(data, step, k++, "bcfinal");
+    recalculate_derived_disk_quantities(data, end_time);
+    if (stages) dump_stage(data, step, k++, "derived");
+}
+
 // The disk's gravity on every body after a step, as ComputeDiskOnNbodyAccel
 // obtains it (Pframeforce.cpp:194-220), with one OpenMP thread so that the
 // reduction runs in the loop order; the planets' own state is not touched.
@@ -389,6 +501,7 @@ int main(int argc, char **argv)
 	fprintf(f, "correct_disk_selfgravity = %d\n", (int)parameters::correct_disk_selfgravity);
 	fprintf(f, "compatibility_no_star_smoothing = %d\n", (int)parameters::compatibility_no_star_smoothing);
 	fprintf(f, "compatibility_smoothing_planetloc = %d\n", (int)parameters::compatibility_smoothing_planetloc);
+	fprintf(f, "integrator = %d\n", (int)parameters::hydro_integrator);
 	fclose(f);
     }
     write_raw("geom_Radii.bin", &Radii[0], sizeof(double) * (Nr + 1));
@@ -449,7 +562,16 @@ int main(int argc, char **argv)
 	rec[5] = refframe::OmegaFrame;
 
 	std::vector<double> planet_rec;
-	euler_step(data, step_dt, step, step <= stage_steps, planet_rec, &rec[6]);
+	if (parameters::hydro_integrator == LEAPFROG_INTEGRATOR) {
+	    double ind[6] = {0};
+	    leapfrog_step(data, step_dt, step, step <= stage_steps, planet_rec, ind);
+	    rec[6] = ind[0];
+	    rec[7] = ind[1];
+	    rec[10] = ind[4];
+	    rec[11] = ind[5];
+	} else {
+	    euler_step(data, step_dt, step, step <= stage_steps, planet_rec, &rec[6]);
+	}
 	rec[8] = sim::time;
 	const bool towrite = std::fabs(time_next_monitor - sim::time) < 1e-6 * cfl_dt;
 	if (towrite) {

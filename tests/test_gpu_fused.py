@@ -5,7 +5,8 @@ entry points, through the C-ABI.
 In FP64 storage the fused passes evaluate the same per-cell expressions on
 the same values, so every evolved and derived field must come out identical,
 bit for bit, after whole fixture runs (TW and SN artificial viscosity, none,
-isothermal and adiabatic, beta cooling, damping, outflow boundaries).  In
+isothermal and adiabatic, beta cooling, damping, outflow boundaries, Euler
+and LeapFrog).  In
 split storage the separate kernels round the intermediate tensors (Q, tau) to
 FP32 and the fused ones keep them in FP64, so the two agree to the split
 storage's own tolerance.  The batched step (fused) is checked against single
@@ -20,7 +21,8 @@ pytestmark = pytest.mark.gpu
 
 FIELDS = ["SIGMA", "VRAD", "VAZI", "ENERGY", "PRESSURE", "SOUNDSPEED", "SCALE_HEIGHT",
           "VISCOSITY", "TEMPERATURE", "QPLUS", "QMINUS"]
-CASES = ["c1_small", "c2_small", "c5_small", "mix_small", "none_small", "c1_iso"]
+CASES = ["c1_small", "c2_small", "c5_small", "mix_small", "none_small", "c1_iso", "lf_iso",
+         "lf_adi"]
 
 
 def _run(meta, data, precision, fused, nsteps):
@@ -59,15 +61,17 @@ def test_fused_step_split(case):
         assert rel_err(a[f], b[f]) <= 2e-6, (f, rel_err(a[f], b[f]))
 
 
-def test_fused_graph_matches_stepwise():
+@pytest.mark.parametrize("integrator", [0, 1])
+def test_fused_graph_matches_stepwise(integrator):
     """Graph replay of the fused step (both views captured) against fused
-    steps launched one by one, on the adiabatic SN disk."""
+    steps launched one by one, on the adiabatic SN disk, Euler and LeapFrog."""
     from fargocpt_amd import workloads as W
     from fargocpt_amd.capi import Disk
     name = "adi_512x1536"
-    print(f"{name}: fused graph vs run_step", flush=True)
-    a = Disk(W.disk_config(name))
-    b = Disk(W.disk_config(name))
+    print(f"{name}: fused graph vs run_step, integrator {integrator}", flush=True)
+    cfg = dict(W.disk_config(name), integrator=integrator)
+    a = Disk(cfg)
+    b = Disk(cfg)
     for d in (a, b):
         W.start(d, name)
     a.run_steps(25, True)

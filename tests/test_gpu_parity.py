@@ -36,7 +36,7 @@ def test_stages(case, step):
 
 
 @pytest.mark.parametrize("case", ["c1_small", "c2_small", "c5_small", "mix_small", "none_small",
-                                  "c1_iso"])
+                                  "c1_iso", "lf_iso", "lf_adi"])
 def test_full_run(case):
     meta, data = load_case(case)
     print(f"case={case} nsteps={meta['nsteps']}", flush=True)

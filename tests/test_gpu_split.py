@@ -53,7 +53,7 @@ def test_split_roundtrip():
 
 
 @pytest.mark.parametrize("case", ["c1_small", "c2_small", "c5_small", "mix_small", "none_small",
-                                  "c1_iso", "c2_adi"])
+                                  "c1_iso", "c2_adi", "lf_iso", "lf_adi"])
 def test_split_fixture_runs(case):
     from fargocpt_amd.capi import Disk
     meta, data = load_case(case)

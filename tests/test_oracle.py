@@ -40,7 +40,7 @@ def test_stages_bit_exact(case, step, oracle_lib):
     assert nshift_ok
 
 
-@pytest.mark.parametrize("case", SMALL + ["c1_iso"])
+@pytest.mark.parametrize("case", SMALL + ["c1_iso", "lf_iso", "lf_adi"])
 def test_full_run_bit_exact(case, oracle_lib):
     meta, data = load_case(case)
     d = OracleDisk(disk_config(meta))
