@@ -76,6 +76,22 @@ class StepParams(C.Structure):
                 ("indirect_x", C.c_double), ("indirect_y", C.c_double)]
 
 
+class IcDesc(C.Structure):
+    """fcpt_ic_desc: the profile initial condition's own configuration keys."""
+    _fields_ = [("sigma_slope", C.c_double), ("imposed_disk_drift", C.c_double),
+                ("vradial_zero", C.c_int32), ("cutoff_outer", C.c_int32),
+                ("cutoff_inner", C.c_int32), ("pad_", C.c_int32),
+                ("cutoff_outer_point", C.c_double), ("cutoff_outer_width", C.c_double),
+                ("cutoff_inner_point", C.c_double), ("cutoff_inner_width", C.c_double)]
+
+
+def make_ic(sigma_slope, imposed_disk_drift=0.0, vradial_zero=0, cutoff_outer=0,
+            cutoff_outer_point=1e300, cutoff_outer_width=1.0, cutoff_inner=0,
+            cutoff_inner_point=0.0, cutoff_inner_width=1.0) -> IcDesc:
+    return IcDesc(sigma_slope, imposed_disk_drift, vradial_zero, cutoff_outer, cutoff_inner, 0,
+                  cutoff_outer_point, cutoff_outer_width, cutoff_inner_point, cutoff_inner_width)
+
+
 class ForceBody(C.Structure):
     _fields_ = [("x", C.c_double), ("y", C.c_double), ("r", C.c_double), ("r_cubic", C.c_double)]
 
@@ -165,6 +181,7 @@ def load_library(path: str = LIB_PATH):
         "fcpt_get_time": [P, dp, dp, C.POINTER(I)],
         "fcpt_set_step_params": [P, C.POINTER(StepParams)],
         "fcpt_set_midstep_params": [P, C.POINTER(StepParams)],
+        "fcpt_init_profiles": [P, C.POINTER(IcDesc)],
         "fcpt_compute_pressure": [P],
         "fcpt_step_leapfrog": [P, D],
         "fcpt_condition_cfl": [P, dp],
@@ -289,6 +306,12 @@ class Disk:
     def set_step_params(self, planets, indirect=(0.0, 0.0)):
         sp = make_step_params(planets, indirect)
         self._check(self.lib.fcpt_set_step_params(self.h, C.byref(sp)))
+
+    def init_profiles(self, **ic):
+        """The reference's profile initial condition (init_physics), built on
+        the device; keywords as make_ic."""
+        d = make_ic(**ic)
+        self._check(self.lib.fcpt_init_profiles(self.h, C.byref(d)))
 
     def set_midstep_params(self, planets, indirect=(0.0, 0.0)):
         """Bodies and indirect term of a LeapFrog step's second kick."""

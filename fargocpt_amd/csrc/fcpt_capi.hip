@@ -128,6 +128,99 @@ int rinf_id_log(const fcpt_disk_desc &d, double r)
 }
 int clampi(int v, int hi) { return v < 0 ? 0 : (v > hi ? hi : v); }
 
+// The radial profiles of the reference's profile initial condition, one
+// function of R each, in the reference's operand order so that they round
+// the same way (host libm, no contraction): Sigma (init.cpp:946-957 and the
+// cutoffs :1047-1148; get_sigma, viscous_radial_speed.cpp:95-118), the
+// pressure-supported smoothed v_phi (Theo.cpp:131-181), the viscosity of the
+// viscous-speed estimate (get_nu2, viscous_radial_speed.cpp:38-93) and the
+// numerical viscous radial speed (:120-195).
+struct DiskProfile {
+    const fcpt_disk_desc &d;
+    const fcpt_ic_desc &ic;
+
+    static double cut_out(double point, double width, double x) { return 1.0 / (1.0 + std::exp((x - point) / width)); }
+    static double cut_in(double point, double width, double x) { return 1.0 / (1.0 + std::exp((point - x) / width)); }
+    static double max_(double a, double b) { return (a < b) ? b : a; }
+    static double min_(double a, double b) { return (b < a) ? b : a; }
+    double floor_value() const { return d.sigma_floor * d.sigma0; }
+
+    double cutoff(double R) const
+    {
+        double c = 1.0;
+        if (ic.cutoff_outer) c *= cut_out(ic.cutoff_outer_point, ic.cutoff_outer_width, R);
+        if (ic.cutoff_inner) c *= cut_in(ic.cutoff_inner_point, ic.cutoff_inner_width, R);
+        return c;
+    }
+    // the grid's Sigma: power law, floor, then each cutoff with the floor again
+    double sigma_cell(double R) const
+    {
+        double s = max_(d.sigma0 * std::pow(R, -ic.sigma_slope), floor_value());
+        if (ic.cutoff_outer) s = max_(s * cut_out(ic.cutoff_outer_point, ic.cutoff_outer_width, R), floor_value());
+        if (ic.cutoff_inner) s = max_(s * cut_in(ic.cutoff_inner_point, ic.cutoff_inner_width, R), floor_value());
+        return s;
+    }
+    // the viscous-speed estimate's Sigma: both cutoffs, then the floor once
+    double sigma_model(double R) const
+    {
+        double s = d.sigma0 * std::pow(R, -ic.sigma_slope);
+        if (ic.cutoff_outer) s *= cut_out(ic.cutoff_outer_point, ic.cutoff_outer_width, R);
+        if (ic.cutoff_inner) s *= cut_in(ic.cutoff_inner_point, ic.cutoff_inner_width, R);
+        return max_(s, floor_value());
+    }
+    double energy(double R) const
+    {
+        return 1.0 / (d.gamma - 1.0) * d.sigma0 * std::pow(d.aspect_ratio, 2) *
+               std::pow(R, -ic.sigma_slope - 1.0 + 2.0 * d.flaring_index) * d.G * d.hydro_center_mass;
+    }
+    double energy_floor(double sig) const { return d.t_min * sig / d.mu * d.R / (d.gamma - 1.0); }
+    double vphi(double R) const
+    {
+        const double F = d.flaring_index;
+        const double h = d.aspect_ratio * std::pow(R, F);
+        const double he = h * d.thickness_smoothing;
+        const double smoothing = (1.0 + (F + 1.0) * std::pow(he, 2)) / std::pow(std::sqrt(1 + std::pow(he, 2)), 3);
+        const double pressure = (2.0 * F - 1.0 - ic.sigma_slope) * std::pow(h, 2);
+        const double vk_2 = d.G * d.hydro_center_mass / R;
+        return std::sqrt(vk_2 * (smoothing + pressure));
+    }
+    double nu(double R, double sig) const
+    {
+        const double v_k = std::sqrt(d.G * d.hydro_center_mass / R);
+        const double h = d.aspect_ratio * std::pow(R, d.flaring_index);
+        if (!d.adiabatic) return d.alpha * (h * v_k) * (h * R);
+        const double g = d.gamma;
+        double e = cutoff(R) * 1.0 / (g - 1.0) * sig * std::pow(h * v_k, 2);
+        e = max_(e, d.t_min * sig / d.mu * d.R / (g - 1.0));
+        e = min_(e, d.t_max * sig / d.mu * d.R / (g - 1.0));
+        const double cs_adb = std::sqrt(g * (g - 1.0) * e / sig);
+        const double cs_iso = std::sqrt((g - 1.0) * e / sig);
+        return d.alpha * cs_adb * (cs_iso / (v_k / R));
+    }
+    double omega(double r) const { return vphi(r) / r; }
+    double r2_omega(double r) const { return std::pow(r, 2) * omega(r); }
+    // five-point derivative with the step 8e-4 r
+    template <class Fn>
+    static double derivative(double r, Fn f)
+    {
+        const double h = 8.0e-4 * r;
+        const double t = -1.0 * f(r + 2.0 * h) + 8.0 * f(r + h) + -8.0 * f(r - h) + 1.0 * f(r - 2.0 * h);
+        return t / (12.0 * h);
+    }
+    double torque_flux(double r) const  // nu Sigma r^3 d(omega)/dr
+    {
+        const double dw = derivative(r, [this](double x) { return omega(x); });
+        const double sig = sigma_model(r);
+        return nu(r, sig) * sig * std::pow(r, 3) * dw;
+    }
+    double vr_viscous(double r) const
+    {
+        const double num = 1.0 / r * derivative(r, [this](double x) { return torque_flux(x); });
+        const double den = sigma_model(r) * derivative(r, [this](double x) { return r2_omega(x); });
+        return num / den;
+    }
+};
+
 }  // namespace
 
 extern "C" {
@@ -509,6 +602,124 @@ static int set_derived_backgrounds(fcpt_ctx *ctx)
         HIP_TRY(ctx, hipMemcpyAsync(P.bgm[k], mom[k]->data(), n1 * sizeof(double),
                                     hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return FCPT_OK;
+}
+
+// copy_initial_values (damping.cpp:301-309): the damping references take the
+// current fields (split storage: with their backgrounds)
+static int copy_initial_values(fcpt_ctx *ctx)
+{
+    const KParams &P = ctx->P;
+    const int pairs[4][2] = {{FCPT_SIGMA, FCPT_SIGMA0}, {FCPT_VRAD, FCPT_VRAD0},
+                             {FCPT_VAZI, FCPT_VAZI0}, {FCPT_ENERGY, FCPT_ENERGY0}};
+    for (const auto &pr : pairs) {
+        const size_t bytes = field_rows(pr[0], P.nr) * P.ns * element_size(P);
+        HIP_TRY(ctx, hipMemcpyAsync(P.f[pr[1]], P.f[pr[0]], bytes, hipMemcpyDeviceToDevice, ctx->stream));
+        if (P.split) {
+            ctx->hbg[pr[1]] = ctx->hbg[pr[0]];
+            HIP_TRY(ctx, hipMemcpyAsync(P.bg[pr[1]], P.bg[pr[0]], ((size_t)P.nr + 2) * sizeof(double),
+                                        hipMemcpyDeviceToDevice, ctx->stream));
+        }
+    }
+    return FCPT_OK;
+}
+
+int fcpt_init_profiles(fcpt_ctx *ctx, const fcpt_ic_desc *ic)
+{
+    CTX_CHECK(ctx);
+    if (!ic) return fail(ctx, FCPT_ERR_ARG, "null initial-condition description");
+    const KParams &P = ctx->P;
+    const int nr = P.nr, ns = P.ns;
+    const DiskProfile prof{ctx->desc, *ic};
+    // ring profiles, O(nr) on the host
+    std::vector<double> sig((size_t)nr + 2, 0.0), en((size_t)nr + 2, 0.0), vr((size_t)nr + 2, 0.0),
+        vaz((size_t)nr + 2, 0.0), zero((size_t)nr + 2, 0.0);
+    for (int i = 0; i < nr; ++i) {
+        sig[i] = prof.sigma_cell(ctx->Rmed[i]);
+        if (P.adiabatic) en[i] = DiskProfile::max_(prof.energy(ctx->Rmed[i]), prof.energy_floor(sig[i]));
+    }
+    // SigmaMed and SigmaInf (compute_azi_avg_Sigma, Theo.cpp:30-48): the ring
+    // sum of nphi equal values, in order, divided by nphi
+    std::vector<double> smed((size_t)nr, 0.0), sinf((size_t)nr, 0.0);
+    for (int i = 0; i < nr; ++i) {
+        double m = 0.0;
+        for (int j = 0; j < ns; ++j) m += sig[i];
+        smed[i] = m / (double)ns;
+    }
+    sinf[0] = smed[0];
+    for (int i = 1; i < nr; ++i) {
+        const double dr = ctx->Rmed[i] - ctx->Rmed[i - 1];
+        sinf[i] = (smed[i - 1] * (ctx->Rmed[i] - ctx->Rinf[i]) + smed[i] * (ctx->Rinf[i] - ctx->Rmed[i - 1])) / dr;
+    }
+    for (int i = 0; i < nr; ++i) {
+        const double r = ctx->Rmed[i], ri = ctx->Rinf[i];
+        double v = prof.vphi(r);
+        v -= P.omega_frame * r;
+        vaz[i] = v;
+        double u = ic->imposed_disk_drift * P.sigma0 / sinf[i] / ri;
+        if (!ic->vradial_zero) u += prof.vr_viscous(ri);
+        else u = 0.0;
+        vr[i] = u;
+    }
+    // the rings on the device (one allocation per call, freed at the end)
+    double *d_rings = nullptr;
+    const size_t n1 = (size_t)nr + 2;
+    HIP_TRY(ctx, hipMalloc(&d_rings, 5 * n1 * sizeof(double)));
+    double *d_sig = d_rings, *d_en = d_rings + n1, *d_vr = d_rings + 2 * n1, *d_vaz = d_rings + 3 * n1,
+           *d_zero = d_rings + 4 * n1;
+    int rc = FCPT_OK;
+    auto up = [&](double *dst, const std::vector<double> &src) {
+        if (!rc && hipMemcpyAsync(dst, src.data(), n1 * sizeof(double), hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+            rc = fail(ctx, FCPT_ERR_DEVICE, "ring profile upload");
+    };
+    up(d_sig, sig);
+    up(d_en, en);
+    up(d_vr, vr);
+    up(d_vaz, vaz);
+    up(d_zero, zero);
+    // split storage: each field's background is its ring profile, so every
+    // deviation is zero (the velocities' background is zero while init_euler
+    // runs, then their profiles)
+    auto fill = [&](int f, const std::vector<double> &ring, const double *d_ring, int rows) {
+        const double *b = nullptr;
+        if (P.split) {
+            ctx->hbg[f] = ring;
+            if (!rc && hipMemcpyAsync(P.bg[f], ring.data(), n1 * sizeof(double), hipMemcpyHostToDevice,
+                                      ctx->stream) != hipSuccess)
+                rc = fail(ctx, FCPT_ERR_DEVICE, "background upload");
+            b = P.bg[f];
+        }
+        if (!rc) launch_fill_rows(P, ctx->stream, P.f[f], d_ring, b, rows);
+    };
+    fill(FCPT_SIGMA, sig, d_sig, nr);
+    fill(FCPT_ENERGY, en, d_en, nr);
+    // init_euler runs while the velocities are still zero (init.cpp:331-332)
+    fill(FCPT_VRAD, zero, d_zero, nr + 1);
+    fill(FCPT_VAZI, zero, d_zero, nr);
+    if (!rc && P.split) rc = set_derived_backgrounds(ctx);
+    if (!rc) {
+        if (!P.adiabatic) {
+            launch_iso_profile(P, ctx->stream);
+            launch_recalc_viscosity(P, ctx->stream);
+        } else {
+            launch_derived(P, ctx->stream);
+            launch_recalc_viscosity(P, ctx->stream);
+            launch_stress(P, ctx->stream);
+            launch_heatcool_cfl(P, ctx->stream);
+        }
+    }
+    fill(FCPT_VRAD, vr, d_vr, nr + 1);
+    fill(FCPT_VAZI, vaz, d_vaz, nr);
+    if (!rc && P.split) rc = set_derived_backgrounds(ctx);
+    if (!rc) rc = copy_initial_values(ctx);
+    if (!rc) launch_boundaries(P, ctx->stream, 0, 0);
+    if (!rc) rc = copy_initial_values(ctx);
+    const hipError_t le = hipGetLastError();
+    const hipError_t se = hipStreamSynchronize(ctx->stream);
+    hipFree(d_rings);
+    if (rc) return rc;
+    if (le != hipSuccess) return fail(ctx, FCPT_ERR_DEVICE, "launch: %s", hipGetErrorString(le));
+    if (se != hipSuccess) return fail(ctx, FCPT_ERR_DEVICE, "sync: %s", hipGetErrorString(se));
     return FCPT_OK;
 }
 

@@ -2122,6 +2122,28 @@ void launch_leapfrog_phase(const KParams &P, hipStream_t st, int phase)
     hipLaunchKernelGGL(k_leapfrog_phase, dim3(1), dim3(1), 0, st, P, phase);
 }
 void launch_pressure(const KParams &P, hipStream_t st) { LAUNCH(k_pressure, cell_grid(P, P.nr), kBlock, st, P); }
+
+// a 2-D field whose rows are constant: row i = ring[i] (split storage: the
+// deviation from bg[i])
+template <bool SPLIT>
+__global__ void __launch_bounds__(256) k_fill_rows(void *dst, const double *ring, const double *bg,
+                                                   int ns, int rows)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = wave_ring(0, (int)blockIdx.y);
+    if (j >= ns || i >= rows) return;
+    const size_t c = (size_t)i * (size_t)ns + (size_t)j;
+    if constexpr (SPLIT) static_cast<float *>(dst)[c] = (float)(ring[i] - bg[i]);
+    else static_cast<double *>(dst)[c] = ring[i];
+}
+
+void launch_fill_rows(const KParams &P, hipStream_t st, void *dst, const double *ring,
+                      const double *bg, int rows)
+{
+    const dim3 g((unsigned)((P.ns + 63) / 64), (unsigned)((rows + 3) / 4));
+    if (P.split) hipLaunchKernelGGL(k_fill_rows<true>, g, kBlock, 0, st, dst, ring, bg, P.ns, rows);
+    else hipLaunchKernelGGL(k_fill_rows<false>, g, kBlock, 0, st, dst, ring, bg, P.ns, rows);
+}
 void launch_copy_velocities(const KParams &P, hipStream_t st)
 {
     LAUNCH(k_copy_velocities, cell_grid(P, P.nr + 1), kBlock, st, P);

@@ -30,6 +30,8 @@ void launch_set_mid_params(const KParams &P, hipStream_t st, const fcpt_step_par
 void launch_leapfrog_phase(const KParams &P, hipStream_t st, int phase);
 void launch_pressure(const KParams &P, hipStream_t st);
 void launch_copy_velocities(const KParams &P, hipStream_t st);
+void launch_fill_rows(const KParams &P, hipStream_t st, void *dst, const double *ring,
+                      const double *bg, int rows);
 void launch_transport_to(const KParams &P, const KParams &out, hipStream_t st);
 // the fused step kernels write the velocities of view P to P.valt; the
 // swapped view reads those buffers and writes the fields

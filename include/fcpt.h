@@ -130,6 +130,21 @@ typedef struct fcpt_step_params {
     double indirect_x, indirect_y;     /* refframe::IndirectTerm */
 } fcpt_step_params;
 
+/* Initial condition of init_physics (init.cpp:255-343) for the profile
+   setups (SigmaCondition: Profile, EnergyCondition: Profile, no self-gravity,
+   no centrifugal-balance option, no random perturbation): the configuration
+   keys the descriptor does not already hold. */
+typedef struct fcpt_ic_desc {
+    double sigma_slope;          /* SigmaSlope: Sigma = sigma0 r^-slope */
+    double imposed_disk_drift;   /* ImposedDiskDrift */
+    int32_t vradial_zero;        /* InitializeVradialZero */
+    int32_t cutoff_outer;        /* ProfileCutoffOuter */
+    int32_t cutoff_inner;        /* ProfileCutoffInner */
+    int32_t pad_;
+    double cutoff_outer_point, cutoff_outer_width;  /* ProfileCutoffPoint/WidthOuter */
+    double cutoff_inner_point, cutoff_inner_width;  /* ProfileCutoffPoint/WidthInner */
+} fcpt_ic_desc;
+
 /* ---- context ---------------------------------------------------------- */
 int fcpt_create(const fcpt_disk_desc *desc, fcpt_ctx **out);
 int fcpt_destroy(fcpt_ctx *ctx);
@@ -143,6 +158,18 @@ int fcpt_upload_state(fcpt_ctx *ctx, const double *sigma, const double *vrad,
                       const double *vazi, const double *energy);
 int fcpt_download_state(fcpt_ctx *ctx, double *sigma, double *vrad, double *vazi,
                         double *energy);
+/* init_physics (init.cpp:255-343) for a profile setup, computed on the
+   device: Sigma (power law, cutoffs, floor; init_gas_density, :937-1148), E
+   (initial_energy with the temperature floor; init_gas_energy, :1257-1300),
+   the init_euler recalculation of the derived fields and heating/cooling
+   (SourceEuler.cpp:251-284, with the velocities still zero), v_phi (the
+   pressure-supported smoothed Keplerian profile minus the frame rotation) and
+   v_r (ImposedDiskDrift plus the numerical viscous speed, or zero;
+   init_gas_velocities, :1725-1770), then copy_initial_values, the boundary
+   call and copy_initial_values.  The radial profiles are formed once per ring
+   on the host with the reference's expressions (O(nr)), the 2-D fields are
+   filled on the device; no host arrays, no files. */
+int fcpt_init_profiles(fcpt_ctx *ctx, const fcpt_ic_desc *ic);
 /* Damping reference values (reference copy_initial_values, damping.cpp:301-309). */
 int fcpt_upload_reference(fcpt_ctx *ctx, const double *sigma0, const double *vrad0,
                           const double *vazi0, const double *energy0);

@@ -383,6 +383,203 @@ void fo_set_state(fo_disk *d, const double *sigma, const double *vrad, const dou
     heating_cooling_for_cfl(d);
 }
 
+/* ---- initial conditions (init_physics, init.cpp:255-343, profile path) ---- */
+
+/* cutoff_outer / cutoff_inner, util.cpp:69-93 */
+static double ic_cutoff_outer(double point, double width, double x)
+{
+    return 1.0 / (1.0 + exp((x - point) / width));
+}
+static double ic_cutoff_inner(double point, double width, double x)
+{
+    return 1.0 / (1.0 + exp((point - x) / width));
+}
+
+/* support_azi_smoothing_derivative, support_azi_pressure and
+   initial_locally_isothermal_smoothed_v_az, Theo.cpp:131-181 */
+static double ic_vaz(const fo_disk *d, const fcpt_ic_desc *ic, double R)
+{
+    const double h0 = d->p.aspect_ratio, F = d->p.flaring_index, S = ic->sigma_slope;
+    const double eps = d->p.thickness_smoothing;
+    const double h = h0 * pow(R, F);
+    const double smoothing_derivative_2 =
+        (1.0 + (F + 1.0) * pow(h * eps, 2)) / pow(sqrt(1 + pow(h * eps, 2)), 3);
+    const double pressure_support_2 = (2.0 * F - 1.0 - S) * pow(h, 2);
+    const double support = smoothing_derivative_2 + pressure_support_2;
+    const double vk_2 = d->p.G * d->p.hydro_center_mass / R;
+    return sqrt(vk_2 * support);
+}
+
+/* get_sigma, viscous_radial_speed.cpp:95-118 */
+static double ic_sigma(const fo_disk *d, const fcpt_ic_desc *ic, double R)
+{
+    double density = d->p.sigma0 * pow(R, -ic->sigma_slope);
+    const double density_floor = d->p.sigma_floor * d->p.sigma0;
+    if (ic->cutoff_outer) density *= ic_cutoff_outer(ic->cutoff_outer_point, ic->cutoff_outer_width, R);
+    if (ic->cutoff_inner) density *= ic_cutoff_inner(ic->cutoff_inner_point, ic->cutoff_inner_width, R);
+    return std_max(density, density_floor);
+}
+
+/* get_nu2, viscous_radial_speed.cpp:38-93 */
+static double ic_nu(const fo_disk *d, const fcpt_ic_desc *ic, double R, double Sigma)
+{
+    const double M = d->p.hydro_center_mass;
+    const double v_k = sqrt(d->p.G * M / R);
+    const double h = d->p.aspect_ratio * pow(R, d->p.flaring_index);
+    double cutoff = 1.0;
+    if (ic->cutoff_outer) cutoff *= ic_cutoff_outer(ic->cutoff_outer_point, ic->cutoff_outer_width, R);
+    if (ic->cutoff_inner) cutoff *= ic_cutoff_inner(ic->cutoff_inner_point, ic->cutoff_inner_width, R);
+    double cs_adb, H;
+    if (d->p.adiabatic) {
+        const double gamma = d->p.gamma;
+        double energy = cutoff * 1.0 / (gamma - 1.0) * Sigma * pow(h * v_k, 2);
+        const double energy_floor = d->p.t_min * Sigma / d->p.mu * d->p.R / (gamma - 1.0);
+        const double energy_ceil = d->p.t_max * Sigma / d->p.mu * d->p.R / (gamma - 1.0);
+        energy = std_max(energy, energy_floor);
+        energy = std_min(energy, energy_ceil);
+        cs_adb = sqrt(gamma * (gamma - 1.0) * energy / Sigma);
+        const double cs_iso = sqrt((gamma - 1.0) * energy / Sigma);
+        const double omega_k = v_k / R;
+        H = cs_iso / omega_k;
+    } else {
+        cs_adb = h * v_k;
+        H = h * R;
+    }
+    return d->p.alpha * cs_adb * H;
+}
+
+/* the functions viscous_speed differentiates (viscous_radial_speed.cpp:140-172) */
+enum { IC_W, IC_R2W, IC_NU_S_R3_DWDR };
+static double ic_eval(const fo_disk *d, const fcpt_ic_desc *ic, int which, double r);
+
+/* derive, viscous_radial_speed.cpp:120-138: five-point first derivative */
+static double ic_derive(const fo_disk *d, const fcpt_ic_desc *ic, int which, double r)
+{
+    const double x = r;
+    const double h = 8.0e-4 * x;
+    const double f1 = -1.0 * ic_eval(d, ic, which, x + 2.0 * h);
+    const double f2 = 8.0 * ic_eval(d, ic, which, x + h);
+    const double f3 = -8.0 * ic_eval(d, ic, which, x - h);
+    const double f4 = 1.0 * ic_eval(d, ic, which, x - 2.0 * h);
+    return (f1 + f2 + f3 + f4) / (12.0 * h);
+}
+
+static double ic_eval(const fo_disk *d, const fcpt_ic_desc *ic, int which, double r)
+{
+    if (which == IC_W) return ic_vaz(d, ic, r) / r;
+    if (which == IC_R2W) return pow(r, 2) * (ic_vaz(d, ic, r) / r);
+    const double dw_dr = ic_derive(d, ic, IC_W, r);
+    const double Sigma = ic_sigma(d, ic, r);
+    const double nu = ic_nu(d, ic, r, Sigma);
+    return nu * Sigma * pow(r, 3) * dw_dr;
+}
+
+/* get_vr_with_numerical_viscous_speed, viscous_radial_speed.cpp:174-195 */
+static double ic_vr_visc(const fo_disk *d, const fcpt_ic_desc *ic, double r)
+{
+    const double num = 1.0 / r * ic_derive(d, ic, IC_NU_S_R3_DWDR, r);
+    const double Sigma = ic_sigma(d, ic, r);
+    const double den = Sigma * ic_derive(d, ic, IC_R2W, r);
+    return num / den;
+}
+
+void fo_init_profiles(fo_disk *d, const fcpt_ic_desc *ic)
+{
+    const int nr = d->nr, ns = d->ns;
+    double *S = d->f[FCPT_SIGMA], *E = d->f[FCPT_ENERGY], *vr = d->f[FCPT_VRAD],
+           *vaz = d->f[FCPT_VAZI];
+    const double floor_value = d->p.sigma_floor * d->p.sigma0;
+    /* init_gas_density, init.cpp:937-1148 (profile, cutoffs) */
+    for (int i = 0; i < nr; ++i)
+        for (int j = 0; j < ns; ++j) {
+            const double density = d->p.sigma0 * pow(d->Rmed[i], -ic->sigma_slope);
+            S[IDX(i, j)] = std_max(density, floor_value);
+        }
+    if (ic->cutoff_outer)
+        for (int i = 0; i < nr; ++i)
+            for (int j = 0; j < ns; ++j) {
+                const double damped = S[IDX(i, j)] * ic_cutoff_outer(ic->cutoff_outer_point,
+                                                                    ic->cutoff_outer_width, d->Rmed[i]);
+                S[IDX(i, j)] = std_max(damped, floor_value);
+            }
+    if (ic->cutoff_inner)
+        for (int i = 0; i < nr; ++i)
+            for (int j = 0; j < ns; ++j) {
+                const double damped = S[IDX(i, j)] * ic_cutoff_inner(ic->cutoff_inner_point,
+                                                                    ic->cutoff_inner_width, d->Rmed[i]);
+                S[IDX(i, j)] = std_max(damped, floor_value);
+            }
+    /* init_gas_energy, init.cpp:1257-1300 (profile) with initial_energy, Theo.cpp:86-99 */
+    if (d->p.adiabatic) {
+        const double h0 = d->p.aspect_ratio, F = d->p.flaring_index, Sl = ic->sigma_slope;
+        for (int i = 0; i < nr; ++i)
+            for (int j = 0; j < ns; ++j) {
+                const double energy = 1.0 / (d->p.gamma - 1.0) * d->p.sigma0 * pow(h0, 2) *
+                                      pow(d->Rmed[i], -Sl - 1.0 + 2.0 * F) * d->p.G *
+                                      d->p.hydro_center_mass;
+                const double energy_floor =
+                    d->p.t_min * S[IDX(i, j)] / d->p.mu * d->p.R / (d->p.gamma - 1.0);
+                E[IDX(i, j)] = std_max(energy, energy_floor);
+            }
+    }
+    /* renormalize_sigma_and_report without SigmaAdjust: SigmaMed, SigmaInf
+       (compute_azi_avg_Sigma, Theo.cpp:30-48) */
+    double *smed = (double *)calloc((size_t)nr + 1, sizeof(double));
+    double *sinf = (double *)calloc((size_t)nr + 1, sizeof(double));
+    for (int i = 0; i < nr; ++i) {
+        double mean = 0.0;
+        for (int j = 0; j < ns; ++j) mean += S[IDX(i, j)];
+        smed[i] = mean / (double)ns;
+    }
+    sinf[0] = smed[0];
+    for (int i = 1; i < nr; ++i) {
+        const double sigm = smed[i - 1], sigp = smed[i];
+        const double dr = d->Rmed[i] - d->Rmed[i - 1];
+        sinf[i] = (sigm * (d->Rmed[i] - d->Rinf[i]) + sigp * (d->Rinf[i] - d->Rmed[i - 1])) / dr;
+    }
+    /* init_euler (SourceEuler.cpp:251-284) while the velocities are still zero */
+    memset(vr, 0, sizeof(double) * (size_t)(nr + 1) * ns);
+    memset(vaz, 0, sizeof(double) * (size_t)nr * ns);
+    if (!d->p.adiabatic) {
+        compute_sound_speed(d);
+        compute_pressure(d);
+        compute_temperature(d);
+        compute_scale_height(d);
+    } else {
+        compute_temperature(d);
+        compute_sound_speed(d);
+        compute_scale_height(d);
+        compute_pressure(d);
+    }
+    update_viscosity(d);
+    heating_cooling_for_cfl(d);
+    /* init_gas_velocities, init.cpp:1725-1770 (no self-gravity, no centrifugal
+       balance) */
+    for (int i = 0; i < nr; ++i) {
+        const double r = d->Rmed[i], ri = d->Rinf[i];
+        for (int j = 0; j < ns; ++j) {
+            vaz[IDX(i, j)] = ic_vaz(d, ic, r);
+            vaz[IDX(i, j)] -= d->p.omega_frame * r;
+            vr[IDX(i, j)] = ic->imposed_disk_drift * d->p.sigma0 / sinf[i] / ri;
+            if (!ic->vradial_zero) vr[IDX(i, j)] += ic_vr_visc(d, ic, ri);
+            else vr[IDX(i, j)] = 0.0;
+        }
+    }
+    free(smed);
+    free(sinf);
+    /* copy_initial_values, the boundary call, copy_initial_values (init.cpp:337-341) */
+    const size_t n = (size_t)nr * ns;
+    memcpy(d->f[FCPT_SIGMA0], S, n * sizeof(double));
+    memcpy(d->f[FCPT_VRAD0], vr, (n + ns) * sizeof(double));
+    memcpy(d->f[FCPT_VAZI0], vaz, n * sizeof(double));
+    memcpy(d->f[FCPT_ENERGY0], E, n * sizeof(double));
+    fo_apply_boundary_condition(d, 0.0, 0);
+    memcpy(d->f[FCPT_SIGMA0], S, n * sizeof(double));
+    memcpy(d->f[FCPT_VRAD0], vr, (n + ns) * sizeof(double));
+    memcpy(d->f[FCPT_VAZI0], vaz, n * sizeof(double));
+    memcpy(d->f[FCPT_ENERGY0], E, n * sizeof(double));
+}
+
 void fo_set_reference(fo_disk *d, const double *sigma0, const double *vrad0,
                       const double *vazi0, const double *energy0)
 {

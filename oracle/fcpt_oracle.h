@@ -35,6 +35,7 @@ void fo_set_time(fo_disk *d, double time, double last_dt, int n_monitor);
 void fo_get_time(const fo_disk *d, double *time, double *last_dt, int *n_monitor);
 void fo_set_step_params(fo_disk *d, const fcpt_step_params *p);
 void fo_set_midstep_params(fo_disk *d, const fcpt_step_params *p);
+void fo_init_profiles(fo_disk *d, const fcpt_ic_desc *ic);
 /* geometry: name is one of Radii Rinf Rsup Rmed Surf InvSurf InvRmed InvRinf
    InvDiffRmed InvDiffRsup InvDiffRsupRb TwoDiffRaSq; returns NULL if unknown */
 const double *fo_geom(const fo_disk *d, const char *name);

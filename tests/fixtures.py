@@ -111,6 +111,19 @@ def step_inputs(data, step: int):
     return planets, (rec[6], rec[7])
 
 
+IC_FIELDS = [("sigma", "SIGMA"), ("vrad", "VRAD"), ("vazi", "VAZI"), ("energy", "ENERGY"),
+             ("qplus", "QPLUS"), ("qminus", "QMINUS"), ("pressure", "PRESSURE"),
+             ("soundspeed", "SOUNDSPEED"), ("temperature", "TEMPERATURE"),
+             ("viscosity", "VISCOSITY")]
+IC_REFERENCE = [("sigma0", "SIGMA0"), ("vrad0", "VRAD0"), ("vazi0", "VAZI0"),
+                ("energy0", "ENERGY0")]
+
+
+def ic_params(meta: dict) -> dict:
+    """Keywords of init_profiles for an initial-condition fixture."""
+    return dict(meta["ic"], sigma_slope=meta["sigma_slope"])
+
+
 def step_inputs_mid(data, step: int):
     """(planets, indirect) of a LeapFrog step's second kick, or None."""
     if "planets_mid" not in data.files:

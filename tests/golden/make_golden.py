@@ -106,6 +106,67 @@ CASES = {
 }
 
 
+# The reference's own initial-condition generators (init_physics): name ->
+# (overrides, planets).  A 32 x 64 mesh; the fixture holds the state right
+# after init_physics.
+IC_CASES = {
+    # power-law Sigma, pressure-supported smoothed v_phi, numerical viscous v_r
+    "ic_iso": ({"Nrad": "32", "Naz": "64"}, PLANET_1),
+    # adiabatic: energy profile with the temperature floor
+    "ic_adi": (dict(ADIABATIC, Nrad="32", Naz="64", SigmaSlope="1.0", FlaringIndex="0.25"),
+               PLANET_1),
+    # exponential cutoffs at both edges down to a raised floor, an imposed disk
+    # drift, flaring
+    "ic_cut": ({"Nrad": "32", "Naz": "64", "ProfileCutoffOuter": "yes",
+                "ProfileCutoffPointOuter": "2.0", "ProfileCutoffWidthOuter": "0.1",
+                "ProfileCutoffInner": "yes", "ProfileCutoffPointInner": "0.6",
+                "ProfileCutoffWidthInner": "0.05", "SigmaFloor": "1e-2",
+                "ImposedDiskDrift": "1e-5", "FlaringIndex": "0.1"}, PLANET_1),
+    # v_r = 0 (InitializeVradialZero), adiabatic
+    "ic_vr0": (dict(ADIABATIC, Nrad="32", Naz="64", InitializeVradialZero="yes"), PLANET_1),
+}
+
+
+def make_ic_case(name, exe, addr):
+    overrides, planets = IC_CASES[name]
+    work = tempfile.mkdtemp(prefix=f"golden_{name}_")
+    try:
+        cfg = os.path.join(work, f"{name}.yml")
+        write_config(cfg, overrides, planets)
+        shutil.copy(cfg, os.path.join(HERE, "configs", f"{name}.yml"))
+        dump = run_driver(exe, addr, cfg, os.path.join(work, "run"), "-", 0, 0, [])
+        meta = read_meta(os.path.join(dump, "meta.txt"))
+        nr, nphi = meta["Nr"], meta["Nphi"]
+        c = dict(BASE)
+        c.update(overrides)
+        meta["case"] = name
+        meta["sigma_slope"] = float(c["SigmaSlope"])
+        meta["ic"] = {
+            "imposed_disk_drift": float(c.get("ImposedDiskDrift", 0.0)),
+            "vradial_zero": 1 if c.get("InitializeVradialZero", "no") == "yes" else 0,
+            "cutoff_outer": 1 if c.get("ProfileCutoffOuter", "no") == "yes" else 0,
+            "cutoff_outer_point": float(c.get("ProfileCutoffPointOuter", 1e300)),
+            "cutoff_outer_width": float(c.get("ProfileCutoffWidthOuter", 1.0)),
+            "cutoff_inner": 1 if c.get("ProfileCutoffInner", "no") == "yes" else 0,
+            "cutoff_inner_point": float(c.get("ProfileCutoffPointInner", 0.0)),
+            "cutoff_inner_width": float(c.get("ProfileCutoffWidthInner", 1.0)),
+        }
+        meta["bc_inner"] = c["InnerBoundary"].lower()
+        meta["bc_outer"] = c["OuterBoundary"].lower()
+        out = {"meta_json": np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8)}
+        shapes = {"sigma": (nr, nphi), "vrad": (nr + 1, nphi), "vazi": (nr, nphi),
+                  "energy": (nr, nphi), "qplus": (nr, nphi), "qminus": (nr, nphi)}
+        for f, shp in shapes.items():
+            out[f"ic_{f}"] = load_bin(os.path.join(dump, f"state_ic_{f}.bin")).reshape(shp)
+        for f in ("pressure", "soundspeed", "temperature", "viscosity"):
+            out[f"ic_{f}"] = load_bin(os.path.join(dump, f"ic_{f}.bin")).reshape(nr, nphi)
+        for f in ("sigma0", "vrad0", "vazi0", "energy0"):
+            out["init_" + f] = load_bin(os.path.join(dump, f"init_{f}.bin")).reshape(shapes[f[:-1]])
+        np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **out)
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+
+
 def write_config(path: str, overrides: dict, planets) -> None:
     cfg = dict(BASE)
     cfg.update(overrides)
@@ -312,9 +373,12 @@ def check_simrun(exe, addr, name="c1_small"):
 def main(argv):
     os.makedirs(os.path.join(HERE, "configs"), exist_ok=True)
     exe, addr = build()
-    names = argv or list(CASES)
+    names = argv or list(CASES) + list(IC_CASES)
     for name in names:
-        make_case(name, exe, addr)
+        if name in IC_CASES:
+            make_ic_case(name, exe, addr)
+        else:
+            make_case(name, exe, addr)
         print("wrote", name)
     for name in ("c1_small", "lf_iso", "lf_adi"):
         if name in names:

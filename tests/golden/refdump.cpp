@@ -24,7 +24,9 @@
 //   nshift.bin              per step: Nr int32 FARGO shifts
 //   diskacc.bin             per step, after it: n_planets x 6 float64 (x, y, r, Klahr
 //                           radius, ax, ay) of ComputeDiskOnPlanetAccel, one thread
-//   state_<step>_<f>.bin    checkpoints (f = sigma, vrad, vazi, energy, qplus, qminus)
+//   state_<step>_<f>.bin    checkpoints (f = sigma, vrad, vazi, energy, qplus, qminus);
+//                           state_ic_<f> and ic_<derived>.bin: the reference's own
+//                           initial condition when no --ic directory is given
 //   stage_<step>_<k>_<name>_<f>.bin   per-stage fields for the first steps
 #include <cstdio>
 #include <cstdlib>
@@ -424,6 +426,14 @@ int main(int argc, char **argv)
     config::cfg.exit_on_unknown_key();
     boundary_conditions::init(data);
     init_physics(data);
+    if (icdir == "-") {
+	// the reference's own initial condition (init_physics, src/init.cpp:255-343)
+	dump_state(data, "ic");
+	dump_grid("ic_pressure", data[t_data::PRESSURE]);
+	dump_grid("ic_soundspeed", data[t_data::SOUNDSPEED]);
+	dump_grid("ic_temperature", data[t_data::TEMPERATURE]);
+	dump_grid("ic_viscosity", data[t_data::VISCOSITY]);
+    }
 
     const unsigned Nr = NRadial, Ns = NAzimuthal;
     if (icdir != "-") {

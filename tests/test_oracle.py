@@ -207,3 +207,24 @@ def test_disk_force_closed_forms(oracle_lib):
     d.upload_state(ring, ic["vrad"], ic["vazi"], ic["energy"])
     zero = d.disk_on_planet_accel([(0.0, 0.0, 0.0, 0.0), body], FORCE_CORRECT_SELFGRAVITY)
     assert np.all(zero == 0.0)
+
+
+@pytest.mark.parametrize("case", ["ic_iso", "ic_adi", "ic_cut", "ic_vr0"])
+def test_initial_condition_bit_exact(case, oracle_lib):
+    """init_physics for profile setups: Sigma with cutoffs and floor, E with
+    the temperature floor, the derived fields, v_phi and the numerical
+    viscous v_r, the boundary call and the damping references -- the
+    reference's own values after init_physics, bit for bit."""
+    from fixtures import IC_FIELDS, IC_REFERENCE, ic_params
+    meta, data = load_case(case)
+    d = OracleDisk(disk_config(meta))
+    d.init_profiles(**ic_params(meta))
+    for key, name in IC_FIELDS:
+        if not meta["adiabatic"] and key == "energy":
+            continue
+        got = d.download_field(name)
+        assert np.array_equal(got, data["ic_" + key]), (key, np.max(np.abs(got - data["ic_" + key])))
+    for key, name in IC_REFERENCE:
+        if not meta["adiabatic"] and key == "energy0":
+            continue
+        assert np.array_equal(d.download_field(name), data["init_" + key]), key
