@@ -92,6 +92,13 @@ def make_ic(sigma_slope, imposed_disk_drift=0.0, vradial_zero=0, cutoff_outer=0,
                   cutoff_outer_point, cutoff_outer_width, cutoff_inner_point, cutoff_inner_width)
 
 
+class SnapshotInfo(C.Structure):
+    """fcpt_snapshot_info: the misc.bin record of a snapshot."""
+    _fields_ = [("snapshot", C.c_uint32), ("n_monitor", C.c_uint32), ("time", C.c_double),
+                ("omega_frame", C.c_double), ("frame_angle", C.c_double),
+                ("last_dt", C.c_double), ("n_iter", C.c_uint64)]
+
+
 class ForceBody(C.Structure):
     _fields_ = [("x", C.c_double), ("y", C.c_double), ("r", C.c_double), ("r_cubic", C.c_double)]
 
@@ -182,6 +189,10 @@ def load_library(path: str = LIB_PATH):
         "fcpt_set_step_params": [P, C.POINTER(StepParams)],
         "fcpt_set_midstep_params": [P, C.POINTER(StepParams)],
         "fcpt_init_profiles": [P, C.POINTER(IcDesc)],
+        "fcpt_write_snapshot": [P, C.c_char_p, C.POINTER(SnapshotInfo), C.c_uint32],
+        "fcpt_read_snapshot": [P, C.c_char_p, C.POINTER(SnapshotInfo)],
+        "fcpt_read_snapshot_reference": [P, C.c_char_p],
+        "fcpt_write_grid_files": [P, C.c_char_p, C.c_char_p],
         "fcpt_compute_pressure": [P],
         "fcpt_step_leapfrog": [P, D],
         "fcpt_condition_cfl": [P, dp],
@@ -312,6 +323,29 @@ class Disk:
         the device; keywords as make_ic."""
         d = make_ic(**ic)
         self._check(self.lib.fcpt_init_profiles(self.h, C.byref(d)))
+
+    def write_snapshot(self, directory, snapshot=0, omega_frame=0.0, frame_angle=0.0, n_iter=0,
+                       fields=()):
+        """The reference's snapshot files in an existing directory; `fields`
+        lists field names (empty: the restart set)."""
+        info = SnapshotInfo(snapshot, 0, 0.0, omega_frame, frame_angle, 0.0, n_iter)
+        mask = 0
+        for f in fields:
+            mask |= 1 << FIELD_ID[f]
+        self._check(self.lib.fcpt_write_snapshot(self.h, os.fsencode(directory), C.byref(info), mask))
+
+    def read_snapshot(self, directory):
+        """Restart from a snapshot directory; returns its misc.bin record."""
+        info = SnapshotInfo()
+        self._check(self.lib.fcpt_read_snapshot(self.h, os.fsencode(directory), C.byref(info)))
+        return {k: getattr(info, k) for k, _ in SnapshotInfo._fields_}
+
+    def read_snapshot_reference(self, directory):
+        self._check(self.lib.fcpt_read_snapshot_reference(self.h, os.fsencode(directory)))
+
+    def write_grid_files(self, outdir, radial_spacing="Logarithmic"):
+        self._check(self.lib.fcpt_write_grid_files(self.h, os.fsencode(outdir),
+                                                   radial_spacing.encode()))
 
     def set_midstep_params(self, planets, indirect=(0.0, 0.0)):
         """Bodies and indirect term of a LeapFrog step's second kick."""

@@ -1265,6 +1265,153 @@ int fcpt_disk_on_planet_accel(fcpt_ctx *ctx, const fcpt_force_body *bodies, int3
     return FCPT_OK;
 }
 
+// ---- snapshots -------------------------------------------------------------
+
+// the reference's grid names (data.cpp:13-250) of the fields this library holds
+static const char *const kGridName[FCPT_NUM_FIELDS] = {
+    "Sigma", "vrad", "vazi", "energy", "pressure", "soundspeed", "scale_height", "viscosity",
+    "potential", "q_r", "q_phi", "div_v", "tau_r_r", "tau_r_phi", "tau_phi_phi", "Qplus",
+    "Qminus", "Temperature", "dens0", "vrad0", "vazi0", "energy0"};
+
+static std::string grid_path(const char *dir, int field)
+{
+    return std::string(dir) + "/" + kGridName[field] + ".dat";
+}
+
+static int write_file(fcpt_ctx *ctx, const std::string &path, const void *data, size_t bytes)
+{
+    FILE *f = std::fopen(path.c_str(), "wb");
+    if (!f) return fail(ctx, FCPT_ERR_ARG, "cannot write %s", path.c_str());
+    const size_t n = bytes ? std::fwrite(data, 1, bytes, f) : 0;
+    const int closed = std::fclose(f);
+    if (n != bytes || closed != 0) return fail(ctx, FCPT_ERR_ARG, "short write to %s", path.c_str());
+    return FCPT_OK;
+}
+
+// a whole file of exactly `bytes` bytes; 1 if it does not exist
+static int read_file(fcpt_ctx *ctx, const std::string &path, void *data, size_t bytes)
+{
+    FILE *f = std::fopen(path.c_str(), "rb");
+    if (!f) return 1;
+    std::fseek(f, 0, SEEK_END);
+    const long size = std::ftell(f);
+    std::fseek(f, 0, SEEK_SET);
+    if (size < 0 || (size_t)size != bytes) {
+        std::fclose(f);
+        return fail(ctx, FCPT_ERR_ARG, "%s holds %ld bytes, expected %zu", path.c_str(), size, bytes);
+    }
+    const size_t n = std::fread(data, 1, bytes, f);
+    std::fclose(f);
+    if (n != bytes) return fail(ctx, FCPT_ERR_ARG, "short read from %s", path.c_str());
+    return FCPT_OK;
+}
+
+static size_t grid_doubles(const fcpt_ctx *ctx, int field)
+{
+    return field_rows(field, ctx->P.nr) * (size_t)ctx->P.ns;
+}
+
+int fcpt_write_snapshot(fcpt_ctx *ctx, const char *dir, const fcpt_snapshot_info *info, uint32_t fields)
+{
+    CTX_CHECK(ctx);
+    if (!dir || !info) return fail(ctx, FCPT_ERR_ARG, "null snapshot directory or info");
+    if (fields == 0) {
+        fields = (1u << FCPT_SIGMA) | (1u << FCPT_VRAD) | (1u << FCPT_VAZI) | (1u << FCPT_ENERGY);
+        if (ctx->P.adiabatic) fields |= (1u << FCPT_QPLUS) | (1u << FCPT_QMINUS);
+    }
+    if (fields >> FCPT_NUM_FIELDS) return fail(ctx, FCPT_ERR_ARG, "unknown field in the mask");
+    std::vector<double> host;
+    for (int f = 0; f < FCPT_NUM_FIELDS; ++f) {
+        if (!(fields & (1u << f))) continue;
+        host.resize(grid_doubles(ctx, f));
+        int rc = fcpt_download_field(ctx, f, host.data());
+        if (!rc) rc = write_file(ctx, grid_path(dir, f), host.data(), host.size() * sizeof(double));
+        if (rc) return rc;
+    }
+    fcpt_snapshot_info m = *info;
+    double t = 0.0, last_dt = 0.0;
+    int32_t n_monitor = 0;
+    int rc = fcpt_get_time(ctx, &t, &last_dt, &n_monitor);
+    if (rc) return rc;
+    m.time = t;
+    m.last_dt = last_dt;
+    m.n_monitor = (uint32_t)n_monitor;
+    static_assert(sizeof(fcpt_snapshot_info) == 48, "misc.bin record");
+    return write_file(ctx, std::string(dir) + "/misc.bin", &m, sizeof(m));
+}
+
+int fcpt_read_snapshot(fcpt_ctx *ctx, const char *dir, fcpt_snapshot_info *info)
+{
+    CTX_CHECK(ctx);
+    if (!dir) return fail(ctx, FCPT_ERR_ARG, "null snapshot directory");
+    fcpt_snapshot_info m{};
+    int rc = read_file(ctx, std::string(dir) + "/misc.bin", &m, sizeof(m));
+    if (rc == 1) return fail(ctx, FCPT_ERR_ARG, "no misc.bin in %s", dir);
+    if (rc) return rc;
+    const bool adi = ctx->P.adiabatic != 0;
+    std::vector<double> g[4];
+    const int want[4] = {FCPT_SIGMA, FCPT_VRAD, FCPT_VAZI, FCPT_ENERGY};
+    for (int k = 0; k < (adi ? 4 : 3); ++k) {
+        g[k].resize(grid_doubles(ctx, want[k]));
+        rc = read_file(ctx, grid_path(dir, want[k]), g[k].data(), g[k].size() * sizeof(double));
+        if (rc == 1) return fail(ctx, FCPT_ERR_ARG, "no %s", grid_path(dir, want[k]).c_str());
+        if (rc) return rc;
+    }
+    rc = fcpt_upload_state(ctx, g[0].data(), g[1].data(), g[2].data(), adi ? g[3].data() : nullptr);
+    if (rc) return rc;
+    if (adi) {
+        // Qplus / Qminus make the restart bit for bit (restart.cpp:70-86);
+        // without them the upload's heating/cooling estimate stays
+        for (int f : {FCPT_QPLUS, FCPT_QMINUS}) {
+            std::vector<double> q(grid_doubles(ctx, f));
+            rc = read_file(ctx, grid_path(dir, f), q.data(), q.size() * sizeof(double));
+            if (rc == 0) rc = fcpt_upload_field(ctx, f, q.data());
+            else if (rc == 1) rc = FCPT_OK;
+            if (rc) return rc;
+        }
+    }
+    rc = fcpt_set_time(ctx, m.time, m.last_dt, (int32_t)m.n_monitor);
+    if (rc) return rc;
+    if (info) *info = m;
+    return fcpt_synchronize(ctx);
+}
+
+int fcpt_read_snapshot_reference(fcpt_ctx *ctx, const char *dir)
+{
+    CTX_CHECK(ctx);
+    if (!dir) return fail(ctx, FCPT_ERR_ARG, "null snapshot directory");
+    const bool adi = ctx->P.adiabatic != 0;
+    const int src[4] = {FCPT_SIGMA, FCPT_VRAD, FCPT_VAZI, FCPT_ENERGY};
+    const int dst[4] = {FCPT_SIGMA0, FCPT_VRAD0, FCPT_VAZI0, FCPT_ENERGY0};
+    for (int k = 0; k < (adi ? 4 : 3); ++k) {
+        std::vector<double> g(grid_doubles(ctx, src[k]));
+        int rc = read_file(ctx, grid_path(dir, src[k]), g.data(), g.size() * sizeof(double));
+        if (rc == 1) return fail(ctx, FCPT_ERR_ARG, "no %s", grid_path(dir, src[k]).c_str());
+        if (!rc) rc = fcpt_upload_field(ctx, dst[k], g.data());
+        if (rc) return rc;
+    }
+    return FCPT_OK;
+}
+
+int fcpt_write_grid_files(fcpt_ctx *ctx, const char *outdir, const char *radial_spacing)
+{
+    CTX_CHECK(ctx);
+    if (!outdir || !radial_spacing) return fail(ctx, FCPT_ERR_ARG, "null output directory or spacing");
+    std::string text;
+    char line[512];
+    for (int i = 0; i <= ctx->P.nr; ++i) {
+        std::snprintf(line, sizeof(line), "%.18g\n", ctx->Rinf[i]);
+        text += line;
+    }
+    int rc = write_file(ctx, std::string(outdir) + "/used_rad.dat", text.data(), text.size());
+    if (rc) return rc;
+    text = "#RMIN\tRMAX\tPHIMIN\tPHIMAX          \tNRAD\tNAZ\tNGHRAD\tNGHAZ\tRadial_spacing\n";
+    std::snprintf(line, sizeof(line), "%.16g\t%.16g\t%.16g\t%.16g\t%d\t%d\t%d\t%d\t%s\n", ctx->P.rmin,
+                  ctx->P.rmax, 0.0, 2 * M_PI, ctx->P.nr, ctx->P.ns, 1, 1, radial_spacing);
+    text += line;
+    return write_file(ctx, std::string(outdir) + "/dimensions.dat", text.data(), text.size());
+}
+
 int fcpt_totals(fcpt_ctx *ctx, double radius, double *mass, double *angular_momentum)
 {
     CTX_CHECK(ctx);

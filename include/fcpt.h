@@ -314,6 +314,42 @@ int fcpt_batch_time_steps(fcpt_batch *batch, int32_t n_steps, int32_t use_graph,
    modify their inputs) */
 int fcpt_batch_time_kernel(fcpt_batch *batch, int32_t kernel, int32_t reps, double *ms_per_launch);
 
+/* ---- snapshots (output files of the reference) ------------------------ */
+/* The run counters of a snapshot: the layout of output::misc_entry
+   (output.h:16-24), the record misc.bin holds. */
+typedef struct fcpt_snapshot_info {
+    uint32_t snapshot;        /* sim::N_snapshot */
+    uint32_t n_monitor;       /* sim::N_monitor */
+    double time;              /* sim::time */
+    double omega_frame;       /* refframe::OmegaFrame */
+    double frame_angle;       /* refframe::FrameAngle */
+    double last_dt;           /* sim::last_dt */
+    uint64_t n_iter;          /* sim::N_hydro_iter */
+} fcpt_snapshot_info;
+
+/* Writes the snapshot directory `dir` (which must exist) as the reference's
+   write_full_output does for the gas (output.cpp:249-305): one raw native
+   FP64 file <name>.dat per field in `fields` (a mask of 1 << FCPT_*; 0
+   selects Sigma, vrad, vazi, energy, and Qplus, Qminus when adiabatic), the
+   whole grid including the ghost rings, vector fields with nr + 1 rows
+   (t_polargrid::write2D, polargrid.cpp:135-185), and misc.bin
+   (output::write_misc, output.cpp:494-525) with `info`'s counters and the
+   context's time, last dt and monitor counter. */
+int fcpt_write_snapshot(fcpt_ctx *ctx, const char *dir, const fcpt_snapshot_info *info, uint32_t fields);
+/* restart_load for the gas (restart.cpp:19-131): misc.bin into *info and the
+   context's clock, Sigma, vrad, vazi (and energy, and Qplus / Qminus when
+   present, for an adiabatic disk) into the state, then the derived fields.
+   Files of the wrong size are rejected. */
+int fcpt_read_snapshot(fcpt_ctx *ctx, const char *dir, fcpt_snapshot_info *info);
+/* The damping references from a snapshot (the reference's "reference"
+   snapshot, restart.cpp:27-56): Sigma, vrad, vazi (and energy). */
+int fcpt_read_snapshot_reference(fcpt_ctx *ctx, const char *dir);
+/* The grid files of the output directory: used_rad.dat (Radii[0 .. nr],
+   "%.18g" per line; init.cpp:227-247) and dimensions.dat (the grid header
+   of parameters::write_grid_data_to_file, parameters.cpp:1075-1123, with
+   `radial_spacing` e.g. "Logarithmic"). */
+int fcpt_write_grid_files(fcpt_ctx *ctx, const char *outdir, const char *radial_spacing);
+
 /* ---- the disk's gravity on the N bodies ------------------------------- */
 /* One body as ComputeDiskOnPlanetAccel reads it (Force.cpp:23-122). */
 typedef struct fcpt_force_body {

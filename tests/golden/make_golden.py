@@ -221,7 +221,14 @@ def read_meta(path: str) -> dict:
     return meta
 
 
-def run_driver(exe, addr, cfg, workdir, icdir, nsteps, stage_steps, checkpoints, simrun=False):
+# cases whose final state is also written with the reference's snapshot
+# writers (the .dat grids, misc.bin) next to its used_rad.dat and
+# dimensions.dat
+SNAPSHOT_CASES = {"c1_small", "c2_small"}
+
+
+def run_driver(exe, addr, cfg, workdir, icdir, nsteps, stage_steps, checkpoints, simrun=False,
+               snapshot=False):
     out = os.path.join(workdir, "dump")
     os.makedirs(out, exist_ok=True)
     args = [exe, cfg, out, icdir, str(nsteps), str(stage_steps),
@@ -229,6 +236,8 @@ def run_driver(exe, addr, cfg, workdir, icdir, nsteps, stage_steps, checkpoints,
     if simrun:
         args.append("simrun")
     env = dict(os.environ, OMP_NUM_THREADS="8")
+    if snapshot:
+        env["REFDUMP_SNAPSHOT"] = "1"
     subprocess.run(args, cwd=workdir, check=True, stdout=subprocess.DEVNULL, env=env)
     return out
 
@@ -260,7 +269,7 @@ def make_case(name, exe, addr, keep_dir=None):
         for f in ("sigma", "vrad", "vazi", "energy"):
             ic[f].tofile(os.path.join(icdir, f + ".bin"))
         dump = run_driver(exe, addr, cfg, os.path.join(work, "run"), icdir, nsteps,
-                          stage_steps, checkpoints)
+                          stage_steps, checkpoints, snapshot=name in SNAPSHOT_CASES)
         meta = read_meta(os.path.join(dump, "meta.txt"))
         meta["case"] = name
         meta["nsteps"] = nsteps
@@ -332,6 +341,14 @@ def make_case(name, exe, addr, keep_dir=None):
                 if fld not in prev or not np.array_equal(prev[fld], arr):
                     out[key] = arr
                 prev[fld] = arr
+        if name in SNAPSHOT_CASES:
+            # file bytes, as written by the reference
+            snap = os.path.join(dump, "snapshot")
+            for fn in sorted(os.listdir(snap)):
+                out["snap_" + fn] = np.fromfile(os.path.join(snap, fn), dtype=np.uint8)
+            outdir = os.path.join(work, "run", BASE["OutputDir"])
+            for fn in ("used_rad.dat", "dimensions.dat"):
+                out["grid_" + fn] = np.fromfile(os.path.join(outdir, fn), dtype=np.uint8)
         np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **out)
         if keep_dir:
             shutil.copytree(dump, keep_dir, dirs_exist_ok=True)

@@ -28,6 +28,8 @@
 //                           state_ic_<f> and ic_<derived>.bin: the reference's own
 //                           initial condition when no --ic directory is given
 //   stage_<step>_<k>_<name>_<f>.bin   per-stage fields for the first steps
+//   snapshot/               with REFDUMP_SNAPSHOT set: the reference's snapshot
+//                           files (<grid>.dat, misc.bin) of the final state
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -594,6 +596,22 @@ int main(int argc, char **argv)
 	if (checkpoints.count(step)) {
 	    dump_state(data, std::to_string(step));
 	}
+    }
+    if (getenv("REFDUMP_SNAPSHOT")) {
+	// the reference's own snapshot writers on the final state
+	// (t_polargrid::write2D, polargrid.cpp:135-185; output::write_misc,
+	// output.cpp:494-525) with snapshot number 7
+	output::snapshot_dir = g_out + "/snapshot";
+	std::string cmd = "mkdir -p " + output::snapshot_dir;
+	if (system(cmd.c_str()) != 0) return 3;
+	const t_data::t_polargrid_type grids[] = {t_data::SIGMA, t_data::V_RADIAL, t_data::V_AZIMUTHAL,
+						  t_data::ENERGY, t_data::QPLUS, t_data::QMINUS,
+						  t_data::TEMPERATURE, t_data::PRESSURE};
+	for (auto g : grids) {
+	    data[g].write2D(output::snapshot_dir + "/" + std::string(data[g].get_name()) + ".dat");
+	}
+	sim::N_snapshot = 7;
+	output::write_misc();
     }
     return 0;
 }
