@@ -1832,17 +1832,55 @@ __device__ __forceinline__ double star_theta(double q, double dq, double dxtheta
 // QuantitiesAdvection (:292-304, VanLeerTheta :630-664) with the residual
 // velocity v - <v>, then with the uniform residual of the ring, then
 // AdvectSHIFT (:238-268).  Each wavefront owns kAzSegW consecutive sectors of
-// one ring and loads them with kAzHalo sectors on either side, one sector per
-// lane; neighbours come from the adjacent lanes, so the kernel needs neither
-// LDS nor barriers.  Each stencil step spoils one more lane at either edge;
-// after the two van Leer passes lanes kAzHalo .. 63 - kAzHalo hold exact
-// results, and the result of sector j is written to sector j + Nshift.  The
-// density's own specific value is exactly 1 (positive Sigma), so its interface
-// values are 1 and drop out of its flux.  Every slope is formed once per
-// sector and each interface selects its upwind side.
+// one ring, two adjacent sectors per lane, and loads them with kAzHalo
+// sectors on either side; a sector's neighbours are the lane's other sector
+// or the adjacent lane's (one DPP move per neighbour pair), so the kernel
+// needs neither LDS nor barriers.  Each stencil step spoils one more sector
+// at either edge; after the two van Leer passes sectors kAzHalo .. 127 -
+// kAzHalo of the wavefront hold exact results, and the result of sector j is
+// written to sector j + Nshift.  The density's own specific value is exactly
+// 1 (positive Sigma), so its interface values are 1 and drop out of its
+// flux.  Every slope is formed once per sector and each interface selects its
+// upwind side.
 constexpr int kAzHalo = 4;
-constexpr int kAzSegW = 64 - 2 * kAzHalo;  // sectors written per wavefront
-constexpr int kAzWaves = 4;                // wavefronts per block
+constexpr int kAzSegW = 128 - 2 * kAzHalo;  // sectors written per wavefront
+constexpr int kAzWaves = 4;                 // wavefronts per block
+
+// the two sectors of a lane: a = 2 lane, b = 2 lane + 1 (within the segment)
+struct D2 {
+    double a, b;
+};
+__device__ __forceinline__ D2 left2(const D2 &x) { return D2{from_left(x.b), x.a}; }
+__device__ __forceinline__ D2 right2(const D2 &x) { return D2{x.b, from_right(x.a)}; }
+
+template <bool SPLIT>
+__device__ __forceinline__ D2 slope2(const D2 &q, double invdxtheta)
+{
+    const D2 r = right2(q), l = left2(q);
+    return D2{0.5 * vl_lim<SPLIT>(r.a - q.a, q.a - l.a) * invdxtheta,
+              0.5 * vl_lim<SPLIT>(r.b - q.b, q.b - l.b) * invdxtheta};
+}
+
+// upwind interface value between each sector and its left neighbour (the
+// second half of ComputeStarTheta)
+__device__ __forceinline__ D2 star2(const D2 &q, const D2 &dq, double dxtheta, const D2 &ksi)
+{
+    const D2 lq = left2(q), ldq = left2(dq);
+    const double loa = lq.a + (dxtheta - ksi.a) * ldq.a, hia = q.a - (dxtheta + ksi.a) * dq.a;
+    const double lob = lq.b + (dxtheta - ksi.b) * ldq.b, hib = q.b - (dxtheta + ksi.b) * dq.b;
+    return D2{(ksi.a > 0.0) ? loa : hia, (ksi.b > 0.0) ? lob : hib};
+}
+
+// flux into the sector through its left interface minus flux out through the
+// right one
+__device__ __forceinline__ D2 flux_div2(const D2 &f)
+{
+    const D2 r = right2(f);
+    double va = f.a, vb = f.b;
+    va -= r.a;
+    vb -= r.b;
+    return D2{va, vb};
+}
 
 template <bool SPLIT, bool ADI>
 __device__ __forceinline__ void b_azimuthal_sweep(const KParams &P, int bx, int by)
@@ -1863,51 +1901,64 @@ __device__ __forceinline__ void b_azimuthal_sweep(const KParams &P, int bx, int 
     const double invdxtheta = 1.0 / dxtheta;
     const double dxrad = (cst(P.Rsup)[i] - cst(P.Rinf)[i]) * dt;
     const double invsurf = cst(P.InvSurf)[i];
-    int js = (j0 - kAzHalo + lane) % ns;  // this lane's sector
-    if (js < 0) js += ns;
-    int jd = (j0 - kAzHalo + lane + nsh) % ns;  // where its result goes
-    if (jd < 0) jd += ns;
-    const bool write = lane >= kAzHalo && lane < kAzHalo + Ls;
+    // this lane's sectors (offset k = 2 lane, 2 lane + 1 from the segment's
+    // first halo sector) and where their results go
+    const int ka = 2 * lane, kb = 2 * lane + 1;
+    int jsa = (j0 - kAzHalo + ka) % ns;
+    if (jsa < 0) jsa += ns;
+    const int jsb = jsa + 1 == ns ? 0 : jsa + 1;
+    int jda = (j0 - kAzHalo + ka + nsh) % ns;
+    if (jda < 0) jda += ns;
+    const int jdb = jda + 1 == ns ? 0 : jda + 1;
+    const bool wa = ka >= kAzHalo && ka < kAzHalo + Ls;
+    const bool wb = kb >= kAzHalo && kb < kAzHalo + Ls;
     // every global load first
     FLD(vaz, FCPT_VAZI);
-    const double s0 = buf<false>(P, P.sig_b, nullptr)(i, js);
-    const double v0 = vaz(i, js) - vm;
-    double q0[NQ];
+    const Fld<false> Sb = buf<false>(P, P.sig_b, nullptr);
+    const D2 s0{Sb(i, jsa), Sb(i, jsb)};
+    const D2 v0{vaz(i, jsa) - vm, vaz(i, jsb) - vm};
+    D2 q0[NQ];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) q0[q] = buf<SPLIT>(P, P.mom_b[q], P.bgm[q])(i, js);
-    if constexpr (ADI) q0[4] = buf<false>(P, P.en_b, nullptr)(i, js);
-    const double ks0 = v0 * dt;
+    for (int q = 0; q < 4; ++q) {
+        const Fld<SPLIT> m = buf<SPLIT>(P, P.mom_b[q], P.bgm[q]);
+        q0[q] = D2{m(i, jsa), m(i, jsb)};
+    }
+    if constexpr (ADI) {
+        const Fld<false> Eb = buf<false>(P, P.en_b, nullptr);
+        q0[4] = D2{Eb(i, jsa), Eb(i, jsb)};
+    }
+    const D2 ks0{v0.a * dt, v0.b * dt};
+    const D2 ksu2{ksu, ksu};
     // density: first pass (residual velocity), then second pass (uniform residual)
-    const double ds0 = star_theta(s0, slope_theta<SPLIT>(s0, invdxtheta), dxtheta, ks0);
-    const double fd0 = dxrad * ds0 * v0;
-    double varq = fd0;
-    varq -= from_right(fd0);
-    const double s1 = s0 + varq * invsurf;
-    const double ds1 = star_theta(s1, slope_theta<SPLIT>(s1, invdxtheta), dxtheta, ksu);
-    const double fd1 = dxrad * ds1 * vu;
-    varq = fd1;
-    varq -= from_right(fd1);
-    if (write) buf<false>(P, P.sig_c, nullptr).set(i, jd, s1 + varq * invsurf);
+    const D2 ds0 = star2(s0, slope2<SPLIT>(s0, invdxtheta), dxtheta, ks0);
+    const D2 vd0 = flux_div2(D2{dxrad * ds0.a * v0.a, dxrad * ds0.b * v0.b});
+    const D2 s1{s0.a + vd0.a * invsurf, s0.b + vd0.b * invsurf};
+    const D2 ds1 = star2(s1, slope2<SPLIT>(s1, invdxtheta), dxtheta, ksu2);
+    const D2 vd1 = flux_div2(D2{dxrad * ds1.a * vu, dxrad * ds1.b * vu});
+    const Fld<false> Sc = buf<false>(P, P.sig_c, nullptr);
+    if (wa) Sc.set(i, jda, s1.a + vd1.a * invsurf);
+    if (wb) Sc.set(i, jdb, s1.b + vd1.b * invsurf);
     // the other quantities
-    const double inv_s0 = SPLIT ? recip<true>(s0) : 0.0;
-    const double inv_s1 = SPLIT ? recip<true>(s1) : 0.0;
+    const D2 inv_s0{SPLIT ? recip<true>(s0.a) : 0.0, SPLIT ? recip<true>(s0.b) : 0.0};
+    const D2 inv_s1{SPLIT ? recip<true>(s1.a) : 0.0, SPLIT ? recip<true>(s1.b) : 0.0};
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
-        const double w0 = SPLIT ? q0[q] * inv_s0 : q0[q] / s0;
-        const double qs0 = star_theta(w0, slope_theta<SPLIT>(w0, invdxtheta), dxtheta, ks0);
-        const double f0 = dxrad * qs0 * ds0 * v0;
-        double vq = f0;
-        vq -= from_right(f0);
-        const double q1 = q0[q] + vq * invsurf;
-        const double w1 = SPLIT ? q1 * inv_s1 : q1 / s1;
-        const double qs1 = star_theta(w1, slope_theta<SPLIT>(w1, invdxtheta), dxtheta, ksu);
-        const double f1 = dxrad * qs1 * ds1 * vu;
-        vq = f1;
-        vq -= from_right(f1);
-        const double q2 = q1 + vq * invsurf;
-        if (write) {
-            if (q < 4) buf<SPLIT>(P, P.mom[q], P.bgm[q]).set(i, jd, q2);
-            else buf<false>(P, P.en_c, nullptr).set(i, jd, q2);
+        const D2 w0{SPLIT ? q0[q].a * inv_s0.a : q0[q].a / s0.a, SPLIT ? q0[q].b * inv_s0.b : q0[q].b / s0.b};
+        const D2 qs0 = star2(w0, slope2<SPLIT>(w0, invdxtheta), dxtheta, ks0);
+        const D2 vq0 = flux_div2(D2{dxrad * qs0.a * ds0.a * v0.a, dxrad * qs0.b * ds0.b * v0.b});
+        const D2 q1{q0[q].a + vq0.a * invsurf, q0[q].b + vq0.b * invsurf};
+        const D2 w1{SPLIT ? q1.a * inv_s1.a : q1.a / s1.a, SPLIT ? q1.b * inv_s1.b : q1.b / s1.b};
+        const D2 qs1 = star2(w1, slope2<SPLIT>(w1, invdxtheta), dxtheta, ksu2);
+        const D2 vq1 = flux_div2(D2{dxrad * qs1.a * ds1.a * vu, dxrad * qs1.b * ds1.b * vu});
+        const D2 q2{q1.a + vq1.a * invsurf, q1.b + vq1.b * invsurf};
+        if (q < 4) {
+            const Fld<SPLIT> m = buf<SPLIT>(P, P.mom[q], P.bgm[q]);
+            if (wa) m.set(i, jda, q2.a);
+            if (wb) m.set(i, jdb, q2.b);
+        } else {
+            const Fld<false> Ec = buf<false>(P, P.en_c, nullptr);
+            if (wa) Ec.set(i, jda, q2.a);
+            if (wb) Ec.set(i, jdb, q2.b);
         }
     }
 }
