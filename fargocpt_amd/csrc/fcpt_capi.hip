@@ -946,8 +946,7 @@ static void enqueue_euler(const KParams &P, hipStream_t st, int count_monitor, b
     const KParams *cur = enqueue_kick(P, Q, st, fused, false);
     launch_boundaries(*cur, st, 0, 0);
     launch_transport_to(*cur, P, st);
-    launch_boundaries(P, st, 1, count_monitor ? 2 : 1);
-    launch_derived(P, st);
+    launch_final_boundaries_derived(P, st, count_monitor ? 2 : 1);
 }
 
 // One gas step of step_LeapFrog (simulation.cpp:276-459): kick with dt / 2,
@@ -970,8 +969,7 @@ static void enqueue_leapfrog(const KParams &P, hipStream_t st, int count_monitor
         launch_copy_velocities(*cur, st);
     }
     launch_leapfrog_phase(P, st, 3);
-    launch_boundaries(P, st, 1, count_monitor ? 2 : 1);
-    launch_derived(P, st);
+    launch_final_boundaries_derived(P, st, count_monitor ? 2 : 1);
 }
 
 static void enqueue_step(const KParams &P, hipStream_t st, int count_monitor, bool fused)
@@ -1616,8 +1614,10 @@ int fcpt_batch_time_kernel(fcpt_batch *b, int32_t kernel, int32_t reps, double *
     if (!b->ev0) BHIP(b, hipEventCreate(&b->ev0));
     if (!b->ev1) BHIP(b, hipEventCreate(&b->ev1));
     auto one = [&]() {
-        for (const BatchStageDev &s : b->stages)
-            if (s.kind == kinds[0] || s.kind == kinds[1]) launch_batch_stage(b->Ps, s, b->split, b->radial_waves, b->stream);
+        for (const BatchStageDev &s : b->stages) {
+            if (s.kind != kinds[0] && s.kind != kinds[1]) continue;
+            launch_batch_stage(b->Ps, s, b->split, b->radial_waves, b->stream);
+        }
     };
     BHIP(b, hipStreamSynchronize(b->stream));
     one();

@@ -252,6 +252,47 @@ FCPT_PAIR1(ring_mean, 256, int, mode)
 // walks rows with a grid stride; the block minimum (wavefront shuffles, then
 // LDS) goes to its own slot of P.ring_min and k_cfl_finalize folds the slots,
 // so the result is deterministic and free of atomics.
+// mode 0: condition_cfl only; 1: sim::CalculateTimeStep (simulation.cpp:100-118);
+// 2: CalculateTimeStep + the monitor clip of sim::run (simulation.cpp:526-542).
+// One block of 256 threads (thread index t) folds the block minima in a fixed
+// pairwise order.
+__device__ __forceinline__ void b_cfl_fold(const KParams &P, int mode, int t)
+{
+    const int nblocks = P.cfl_gx * P.cfl_gy;
+    __shared__ double sh[256];
+    double m = __longlong_as_double(0x7FF0000000000000ll);
+    for (int k = t; k < nblocks; k += 256) m = std_min(m, P.ring_min[k]);
+    sh[t] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) sh[t] = std_min(sh[t], sh[t + w]);
+        __syncthreads();
+    }
+    if (t != 0) return;
+    DevScalars *s = P.s;
+    const double cfl_dt = sh[0];
+    s->cfl_dt = cfl_dt;
+    if (!(cfl_dt > 0.0) || !isfinite(cfl_dt)) s->error_flags |= 1;
+    if (mode >= 1) {
+        const double rv = std_min(P.cfl_max_var * s->last_dt, cfl_dt);
+        s->last_dt = rv;
+        s->cfl_dt = rv;
+        if (mode == 2) {
+            const double time_next_monitor = (s->n_monitor + 1) * P.monitor_timestep;
+            const double time_left_till_write = time_next_monitor - s->time;
+            const bool overshoot = rv > time_left_till_write;
+            const double dt_stretch_factor = 0.05;
+            const bool almost_there = time_left_till_write < rv * (1 + dt_stretch_factor);
+            s->dt = (overshoot || almost_there) ? time_left_till_write : rv;
+        }
+    }
+}
+__device__ __forceinline__ void b_cfl_finalize(const KParams &P, int mode) { b_cfl_fold(P, mode, (int)threadIdx.x); }
+
+// per-cell CFL limits (cfl.cpp:185-382) and this block's minimum in its slot
+// (k_cfl_finalize folds the slots: a device-wide fold inside this kernel
+// would need a device-scope release per block, which on gfx950 writes back
+// the XCD's L2 and costs more than the separate launch)
 template <bool SPLIT>
 __device__ __forceinline__ void b_cfl_cells(const KParams &P, int bx, int by)
 {
@@ -342,39 +383,6 @@ __device__ __forceinline__ void b_cfl_cells(const KParams &P, int bx, int by)
 }
 FCPT_PAIR0(cfl_cells, 256)
 
-// mode 0: condition_cfl only; 1: sim::CalculateTimeStep (simulation.cpp:100-118);
-// 2: CalculateTimeStep + the monitor clip of sim::run (simulation.cpp:526-542)
-__device__ __forceinline__ void b_cfl_finalize(const KParams &P, int mode)
-{
-    const int nblocks = P.cfl_gx * P.cfl_gy;
-    __shared__ double sh[256];
-    double m = __longlong_as_double(0x7FF0000000000000ll);
-    for (int k = threadIdx.x; k < nblocks; k += 256) m = std_min(m, P.ring_min[k]);
-    sh[threadIdx.x] = m;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) sh[threadIdx.x] = std_min(sh[threadIdx.x], sh[threadIdx.x + w]);
-        __syncthreads();
-    }
-    if (threadIdx.x != 0) return;
-    DevScalars *s = P.s;
-    const double cfl_dt = sh[0];
-    s->cfl_dt = cfl_dt;
-    if (!(cfl_dt > 0.0) || !isfinite(cfl_dt)) s->error_flags |= 1;
-    if (mode >= 1) {
-        const double rv = std_min(P.cfl_max_var * s->last_dt, cfl_dt);
-        s->last_dt = rv;
-        s->cfl_dt = rv;
-        if (mode == 2) {
-            const double time_next_monitor = (s->n_monitor + 1) * P.monitor_timestep;
-            const double time_left_till_write = time_next_monitor - s->time;
-            const bool overshoot = rv > time_left_till_write;
-            const double dt_stretch_factor = 0.05;
-            const bool almost_there = time_left_till_write < rv * (1 + dt_stretch_factor);
-            s->dt = (overshoot || almost_there) ? time_left_till_write : rv;
-        }
-    }
-}
 __global__ void __launch_bounds__(256) k_cfl_finalize(KParams P, int mode) { b_cfl_finalize(P, mode); }
 __global__ void __launch_bounds__(256) kb_cfl_finalize(const KParams *Ps, const BatchEntry *map, int mode)
 {
@@ -934,6 +942,7 @@ __device__ __forceinline__ void b_derived(const KParams &P, int bx, int by)
     derived_store<SPLIT>(P, i, j, derived_load<SPLIT>(P, i, j, true), true, true);
 }
 FCPT_PAIR0(derived, 256)
+
 
 // compute_pressure (SourceEuler.cpp): (gamma - 1) E, or Sigma c_s^2 with the
 // isothermal ring profile of c_s
@@ -1517,32 +1526,29 @@ FCPT_PAIR0(damping, 256)
 
 // ghost rings: zero-gradient Sigma/E, reflecting / zero-gradient / outflow v_r,
 // Keplerian v_phi (boundary_conditions.cpp:67-97 and the BC files it dispatches to)
-template <bool SPLIT>
-__device__ __forceinline__ void b_ghost_bc(const KParams &P, int bx, int by, int advance_time)
+// end of a sim::run iteration: time += dt and the monitor counter
+// (simulation.cpp:244, 545-551); nothing else in the kernels that call it
+// reads them
+__device__ __forceinline__ void advance_clock(const KParams &P, int advance_time)
 {
-    const int ns = P.ns;
-    const int j = bx * blockDim.x + threadIdx.x;
-    if (advance_time && j == 0) {
-        // end of a sim::run iteration: time += dt and the monitor counter
-        // (simulation.cpp:244, 545-551); nothing else in this kernel reads them
-        DevScalars *s = P.s;
-        const double time_next_monitor = (s->n_monitor + 1) * P.monitor_timestep;
-        s->time = s->time + s->dt;
-        if (advance_time == 2 && fabs(time_next_monitor - s->time) < 1e-6 * s->last_dt)
-            s->n_monitor++;
-    }
-    if (j >= ns) return;
-    const int nr = P.nr;
+    DevScalars *s = P.s;
+    const double time_next_monitor = (s->n_monitor + 1) * P.monitor_timestep;
+    s->time = s->time + s->dt;
+    if (advance_time == 2 && fabs(time_next_monitor - s->time) < 1e-6 * s->last_dt) s->n_monitor++;
+}
+
+// the inner / outer ghost rings of sector j (rows 2 and nr-2 are never
+// written here, nr >= 8)
+template <bool SPLIT>
+__device__ __forceinline__ void ghost_inner(const KParams &P, int j)
+{
     FLD(S, FCPT_SIGMA);
     FLD(E, FCPT_ENERGY);
     FLD(vr, FCPT_VRAD);
     FLD(vaz, FCPT_VAZI);
-    S.set(0, j, S(1, j));
-    S.set(nr - 1, j, S(nr - 2, j));
-    E.set(0, j, E(1, j));
-    E.set(nr - 1, j, E(nr - 2, j));
-    // rows 2 and nr-2 are never written here (nr >= 8)
-    const double v2 = vr(2, j), vn2 = vr(nr - 2, j);
+    const double s1 = S(1, j), e1 = E(1, j), v2 = vr(2, j);
+    S.set(0, j, s1);
+    E.set(0, j, e1);
     if (P.bc_inner == FCPT_BC_REFLECTING) {
         vr.set(0, j, -v2);
         vr.set(1, j, 0);
@@ -1558,6 +1564,21 @@ __device__ __forceinline__ void b_ghost_bc(const KParams &P, int bx, int by, int
             vr.set(0, j, v2);
         }
     }
+    const double vk_in = sqrt(P.G * P.mstar / cst(P.Rmed)[0]);
+    vaz.set(0, j, 1.0 * vk_in - cst(P.Rmed)[0] * P.omega_frame);
+}
+
+template <bool SPLIT>
+__device__ __forceinline__ void ghost_outer(const KParams &P, int j)
+{
+    const int nr = P.nr;
+    FLD(S, FCPT_SIGMA);
+    FLD(E, FCPT_ENERGY);
+    FLD(vr, FCPT_VRAD);
+    FLD(vaz, FCPT_VAZI);
+    const double s2 = S(nr - 2, j), e2 = E(nr - 2, j), vn2 = vr(nr - 2, j);
+    S.set(nr - 1, j, s2);
+    E.set(nr - 1, j, e2);
     if (P.bc_outer == FCPT_BC_REFLECTING) {
         vr.set(nr, j, -vn2);
         vr.set(nr - 1, j, 0);
@@ -1573,12 +1594,35 @@ __device__ __forceinline__ void b_ghost_bc(const KParams &P, int bx, int by, int
             vr.set(nr, j, vn2);
         }
     }
-    const double vk_in = sqrt(P.G * P.mstar / cst(P.Rmed)[0]);
     const double vk_out = sqrt(P.G * P.mstar / cst(P.Rmed)[nr - 1]);
-    vaz.set(0, j, 1.0 * vk_in - cst(P.Rmed)[0] * P.omega_frame);
     vaz.set(nr - 1, j, 1.0 * vk_out - cst(P.Rmed)[nr - 1] * P.omega_frame);
 }
+
+template <bool SPLIT>
+__device__ __forceinline__ void b_ghost_bc(const KParams &P, int bx, int by, int advance_time)
+{
+    const int j = bx * blockDim.x + threadIdx.x;
+    if (advance_time && j == 0) advance_clock(P, advance_time);
+    if (j >= P.ns) return;
+    ghost_inner<SPLIT>(P, j);
+    ghost_outer<SPLIT>(P, j);
+}
 FCPT_PAIR1(ghost_bc, 256, int, advance_time)
+
+// the final boundary call's ghost rings (b_ghost_bc, after the damping) and
+// recalculate_derived_disk_quantities in one pass: a ghost-ring cell first
+// takes its neighbour ring's values, then forms its derived fields from what
+// it stored, exactly as the two separate kernels would
+template <bool SPLIT>
+__device__ __forceinline__ void b_final_derived(const KParams &P, int bx, int by, int advance_time)
+{
+    CELL_LOOP_HEAD(0, P.nr)
+    if (advance_time && i == 0 && j == 0) advance_clock(P, advance_time);
+    if (i == 0) ghost_inner<SPLIT>(P, j);
+    if (i == P.nr - 1) ghost_outer<SPLIT>(P, j);
+    derived_store<SPLIT>(P, i, j, derived_load<SPLIT>(P, i, j, true), true, true);
+}
+FCPT_PAIR1(final_derived, 256, int, advance_time)
 
 // ---------------------------------------------------------------------------
 // Transport
@@ -2356,6 +2400,19 @@ void launch_visc_fused(const KParams &P, hipStream_t st)
 }
 
 void launch_derived(const KParams &P, hipStream_t st) { LAUNCH(k_derived, cell_grid(P, P.nr), kBlock, st, P); }
+
+// the final boundary call (damping, ghost rings, the clock) and the derived
+// fields: damping, then one pass for the rest
+void launch_final_boundaries_derived(const KParams &P, hipStream_t st, int advance_time)
+{
+    if (P.damping) {
+        const int n_inner = (P.damp_ids[0] > P.damp_ids[1] ? P.damp_ids[0] : P.damp_ids[1]) + 1;
+        const int o0 = P.damp_ids[2] < P.damp_ids[3] ? P.damp_ids[2] : P.damp_ids[3];
+        const int n_outer = P.nr + 1 - o0;
+        LAUNCH(k_damping, cell_grid(P, n_inner + n_outer), kBlock, st, P);
+    }
+    LAUNCH(k_final_derived, cell_grid(P, P.nr), kBlock, st, P, advance_time);
+}
 void launch_iso_rings(const KParams &P, hipStream_t st)
 {
     hipLaunchKernelGGL(k_iso_rings, dim3((unsigned)((P.nr + 255) / 256)), dim3(256), 0, st, P);
@@ -2584,7 +2641,6 @@ std::vector<BatchStageHost> plan_batch_step(const std::vector<KParams> &D)
     auto rings = [](std::vector<BatchEntry> &v, unsigned d, const KParams &P) {
         add_grid(v, d, (unsigned)P.nr, 1);
     };
-    auto one = [](std::vector<BatchEntry> &v, unsigned d, const KParams &) { add_grid(v, d, 1, 1); };
     auto cfl = [](std::vector<BatchEntry> &v, unsigned d, const KParams &P) {
         add_grid(v, d, (unsigned)P.cfl_gx, (unsigned)P.cfl_gy);
     };
@@ -2609,7 +2665,7 @@ std::vector<BatchStageHost> plan_batch_step(const std::vector<KParams> &D)
     // sim::CalculateTimeStep and the monitor clip (simulation.cpp:100-118, 526-542)
     stage(B_RING_MEAN, 0, all, rings);
     stage(B_CFL_CELLS, 0, all, cfl);
-    stage(B_CFL_FINAL, 2, all, one);
+    stage(B_CFL_FINAL, 2, all, [](std::vector<BatchEntry> &v, unsigned d, const KParams &) { add_grid(v, d, 1, 1); });
     // step_Euler (simulation.cpp:148-267), in the order of enqueue_euler
     stage(B_POT_BG, 0, [](const KParams &P) { return P.split != 0; }, pot_bg);
     stage(B_POTENTIAL, 0, all, rows(0));
@@ -2640,8 +2696,7 @@ std::vector<BatchStageHost> plan_batch_step(const std::vector<KParams> &D)
     stage_v(B_VELOCITIES, 0, all, rows(0), false, true);
     std::fill(view.begin(), view.end(), 0u);
     stage(B_DAMPING, 0, [](const KParams &P) { return P.damping != 0; }, damp);
-    stage(B_GHOST, 2, all, ghosts);
-    stage(B_DERIVED, 0, all, rows(0));
+    stage(B_FINAL_DERIVED, 2, all, rows(0));
     return st;
 }
 
@@ -2682,6 +2737,7 @@ void launch_batch_stage(const KParams *Ps, const BatchStageDev &S, int split, in
     case B_VELOCITIES: LAUNCHB(kb_velocities, kBlock); break;
     case B_DAMPING: LAUNCHB(kb_damping, kBlock); break;
     case B_DERIVED: LAUNCHB(kb_derived, kBlock); break;
+    case B_FINAL_DERIVED: LAUNCHB(kb_final_derived, kBlock, S.arg); break;
     case B_RADIAL_ISO:
     case B_RADIAL_ADI: {
         const bool a = S.kind == B_RADIAL_ADI, w2 = radial_waves == 2;

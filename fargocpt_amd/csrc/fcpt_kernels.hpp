@@ -29,6 +29,7 @@ void launch_transport(const KParams &P, hipStream_t st);
 void launch_set_mid_params(const KParams &P, hipStream_t st, const fcpt_step_params &sp);
 void launch_leapfrog_phase(const KParams &P, hipStream_t st, int phase);
 void launch_pressure(const KParams &P, hipStream_t st);
+void launch_final_boundaries_derived(const KParams &P, hipStream_t st, int advance_time);
 void launch_copy_velocities(const KParams &P, hipStream_t st);
 void launch_fill_rows(const KParams &P, hipStream_t st, void *dst, const double *ring,
                       const double *bg, int rows);
@@ -81,7 +82,7 @@ enum BatchKind {
     B_RING_MEAN, B_CFL_CELLS, B_CFL_FINAL, B_POT_BG, B_POTENTIAL, B_SRC_VR, B_SRC_VPHI,
     B_SRC_ENERGY, B_TW_Q, B_TW_VEL, B_SN_Q, B_SN_VEL, B_TEMP_FLOOR, B_RECALC_VISC, B_STRESS,
     B_VISC_UPDATE, B_SUBSTEP3, B_GHOST, B_RADIAL_ISO, B_RADIAL_ADI, B_AZIM_ISO, B_AZIM_ADI,
-    B_VELOCITIES, B_DAMPING, B_DERIVED, B_SRC_FUSED, B_AV_FUSED, B_VISC_FUSED
+    B_VELOCITIES, B_DAMPING, B_DERIVED, B_SRC_FUSED, B_AV_FUSED, B_VISC_FUSED, B_FINAL_DERIVED
 };
 struct BatchStageHost {
     int kind, arg;
