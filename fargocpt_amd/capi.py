@@ -32,7 +32,7 @@ KERNEL_DISK_FORCE = 6  # the disk-on-planet reduction (not part of the hydro ste
 # storage modes (fcpt_disk_desc.precision) and launch parameters of include/fcpt.h
 PREC_FP64, PREC_SPLIT32 = 0, 1
 INTEGRATOR_EULER, INTEGRATOR_LEAPFROG = 0, 1
-PARAM_RADIAL_TILE, PARAM_RADIAL_WAVES, PARAM_FUSED_STEP = 0, 1, 2
+PARAM_RADIAL_TILE, PARAM_RADIAL_WAVES, PARAM_FUSED_STEP, PARAM_VISC_TILE = 0, 1, 2, 3
 BC = {"reflecting": 0, "zerogradient": 1, "outflow": 2}
 MAX_PLANETS = 8
 
@@ -454,6 +454,11 @@ class Disk:
     def set_radial_tile(self, rings):
         """Rings walked by one radial-sweep thread (0: size-based default)."""
         self._check(self.lib.fcpt_set_launch_param(self.h, PARAM_RADIAL_TILE, rings))
+
+    def set_visc_tile(self, rings):
+        """Rings walked by one thread of the fused viscous pass (0:
+        size-based default)."""
+        self._check(self.lib.fcpt_set_launch_param(self.h, PARAM_VISC_TILE, rings))
 
     def autotune(self, reps=10):
         """Time the tunable launch shapes on the current state and keep the

@@ -253,6 +253,7 @@ int fcpt_create(const fcpt_disk_desc *desc, fcpt_ctx **out)
     P.split = desc->precision == FCPT_PREC_SPLIT32 ? 1 : 0;
     P.radial_tile = 0;
     P.radial_waves = 3;
+    P.visc_tile = 0;
     init_launch_fields(P);
     // Interpret.cpp:230-231
     P.dphi = 2.0 * M_PI / (double)ns;
@@ -1166,12 +1167,15 @@ int fcpt_set_launch_param(fcpt_ctx *ctx, int32_t param, int32_t value)
         if (value != 2 && value != 3) return fail(ctx, FCPT_ERR_ARG, "radial waves must be 2 or 3");
     } else if (param == FCPT_PARAM_FUSED_STEP) {
         if (value != 0 && value != 1) return fail(ctx, FCPT_ERR_ARG, "fused step must be 0 or 1");
+    } else if (param == FCPT_PARAM_VISC_TILE) {
+        if (value < 0 || value > 64) return fail(ctx, FCPT_ERR_ARG, "viscous tile must be 0 .. 64");
     } else {
         return fail(ctx, FCPT_ERR_ARG, "unknown launch parameter");
     }
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     if (param == FCPT_PARAM_RADIAL_TILE) ctx->P.radial_tile = value;
     else if (param == FCPT_PARAM_RADIAL_WAVES) ctx->P.radial_waves = value;
+    else if (param == FCPT_PARAM_VISC_TILE) ctx->P.visc_tile = value;
     else ctx->fused = value != 0;
     if (ctx->step_graph) {
         HIP_TRY(ctx, hipGraphExecDestroy(ctx->step_graph));

@@ -76,6 +76,7 @@ struct KParams {
     int split;
     int radial_tile;     // rings per thread of the radial sweep (0: size-based choice)
     int radial_waves;    // register budget of the radial sweep: wavefronts per SIMD (2 or 3)
+    int visc_tile;       // rings per thread of the fused viscous pass (0: size-based choice)
     int cfl_gx, cfl_gy;  // grid of the CFL cell kernel (its block-minimum slots)
     // fields in HBM (indexed by fcpt_field): double, or float when split
     void *f[FCPT_NUM_FIELDS];

@@ -1343,6 +1343,145 @@ __device__ __forceinline__ void b_visc_fused(const KParams &P, int bx, int by)
 }
 FCPT_PAIR0(visc_fused, 256)
 
+// Neighbour values within a wavefront: the value of lane - 1 and of lane + 1,
+// moved by the DPP wavefront shifts (wave_shr:1, wave_shl:1) on both 32-bit
+// halves; the edge lanes receive 0 and their results are discarded
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double x)
+{
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b & 0xffffffffll), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+__device__ __forceinline__ double from_left(double x) { return dpp_move<0x138>(x); }
+__device__ __forceinline__ double from_right(double x) { return dpp_move<0x130>(x); }
+
+// The fused viscous pass as a radial walk.  A wavefront owns kViscSegW
+// consecutive sectors and `tile` consecutive rings: lane k holds sector
+// j0 - 1 + k (lanes 0 and 63 are the halo) and walks the rings outwards.
+// Each ring is loaded once (v_r, v_phi, Sigma, nu of the lane's own sector;
+// the ring after next is prefetched before the current ring's stores), the
+// neighbouring sectors' values come from the adjacent lanes by DPP moves,
+// and every stress component is formed once per cell: the diagonal of ring
+// i, whose tau_phi_phi at sector j - 1 is the left lane's, and tau_rp of
+// corner (i + 1, j), whose value at sector j + 1 is the right lane's; the
+// diagonal of ring i - 1 and the corners of ring i carry over from the
+// previous step.  Every value goes through the helpers b_visc_fused calls,
+// with the same arguments, so the two are bit-identical.
+constexpr int kViscSegW = 62;  // sectors written per wavefront
+
+struct ViscRing {
+    double vr, va, s, n;  // the lane's sector
+    double vrL, sL, nL;   // sector j - 1
+    double vaR;           // v_phi of sector j + 1
+};
+
+template <bool SPLIT>
+__device__ __forceinline__ void b_visc_walk(const KParams &P, int bx, int by, int tile)
+{
+    const int ns = P.ns, nr = P.nr;
+    const int k = (int)threadIdx.x;  // lane
+    const int i0 = __builtin_amdgcn_readfirstlane((by * (int)blockDim.y + (int)threadIdx.y) * tile);
+    if (i0 >= nr) return;
+    const int i1 = i0 + tile < nr ? i0 + tile : nr;  // rings [i0, i1)
+    const int j0 = bx * kViscSegW;
+    const int Ls = ns - j0 < kViscSegW ? ns - j0 : kViscSegW;
+    const int j = (j0 - 1 + k + ns) % ns;
+    const bool wr = k >= 1 && k <= Ls;  // lanes that write
+    FLD(vr, FCPT_VRAD);
+    FLD(vaz, FCPT_VAZI);
+    FLD(nu, FCPT_VISCOSITY);
+    FLD(S, FCPT_SIGMA);
+    const Fld<SPLIT> vr_o = alt_vr<SPLIT>(P);
+    const Fld<SPLIT> vaz_o = alt_vaz<SPLIT>(P);
+    const bool iso = !P.adiabatic;  // nu is a ring profile
+    const bool store = P.adiabatic && P.heating_viscous;
+    const double dt = P.s->dt;
+    // the lane's values of ring r (r < nr); ring nr holds only v_r
+    struct Raw {
+        double vr, va, s, n;
+    };
+    auto load = [&](int r) {
+        Raw w;
+        w.vr = vr(r, j);
+        if (r < nr) {
+            w.va = vaz(r, j);
+            w.s = S(r, j);
+            w.n = iso ? cst(P.iso_nu)[r] : nu(r, j);
+        } else {
+            w.va = w.s = w.n = 0.0;
+        }
+        return w;
+    };
+    auto ring = [&](const Raw &w) {
+        ViscRing g;
+        g.vr = w.vr;
+        g.va = w.va;
+        g.s = w.s;
+        g.n = w.n;
+        g.vrL = from_left(w.vr);
+        g.sL = from_left(w.s);
+        g.nL = iso ? w.n : from_left(w.n);
+        g.vaR = from_right(w.va);
+        return g;
+    };
+    // tau_rp at corner (r, j) from rings a = r - 1 and b = r
+    auto corner = [&](int r, const ViscRing &a, const ViscRing &b) {
+        return stress_rp(P, r, b.va, a.va, b.vr, b.vrL, b.n, a.n, b.nL, a.nL, b.s, a.s, b.sL, a.sL);
+    };
+    ViscRing A, B;  // rings i, i + 1
+    StressDiag dprev{0.0, 0.0, 0.0};
+    double trp_c = 0.0, s_prev = 0.0;  // tau_rp(i, j), Sigma(i - 1, j)
+    if (i0 >= 1) {
+        const ViscRing Z = ring(load(i0 - 1));
+        A = ring(load(i0));
+        dprev = stress_diag(P, i0 - 1, Z.vr, A.vr, Z.va, Z.vaR, Z.n, Z.s);
+        trp_c = corner(i0, Z, A);
+        s_prev = Z.s;
+    } else {
+        A = ring(load(i0));
+    }
+    B = ring(load(i0 + 1));
+    for (int i = i0; i < i1; ++i) {
+        const Raw Cn = load(i + 2 <= nr ? i + 2 : nr);  // prefetch, before this ring's stores
+        const bool upd_p = i >= 1 && i < nr - 1;
+        const bool upd_r = i >= 2 && upd_p;
+        const StressDiag d = stress_diag(P, i, A.vr, B.vr, A.va, A.vaR, A.n, A.s);
+        const double tppj = from_left(d.tpp);  // diagonal of (i, j - 1)
+        const double trp_n = from_right(trp_c);  // corner (i, j + 1)
+        const double trp1 = i + 1 <= nr - 1 ? corner(i + 1, A, B) : 0.0;
+        if (wr) {
+            if (store) {
+                fld<SPLIT>(P, FCPT_DIV_V).set(i, j, d.divv);
+                fld<SPLIT>(P, FCPT_TAU_R_R).set(i, j, d.trr);
+                fld<SPLIT>(P, FCPT_TAU_PHI_PHI).set(i, j, d.tpp);
+                if (i >= 1) fld<SPLIT>(P, FCPT_TAU_R_PHI).set(i, j, trp_c);
+            }
+            if (upd_p)
+                vaz_o.set(i, j, visc_vphi_new(P, i, dt, A.va, A.s, A.sL, trp_c, trp1, d.tpp, tppj));
+            else
+                vaz_o.set_raw(i, j, vaz.raw(i, j));
+            if (upd_r)
+                vr_o.set(i, j, visc_vr_new(P, i, dt, A.vr, A.s, s_prev, d.trr, dprev.trr, trp_c, trp_n,
+                                           d.tpp, dprev.tpp));
+            else
+                vr_o.set_raw(i, j, vr.raw(i, j));
+        }
+        dprev = d;
+        trp_c = trp1;
+        s_prev = A.s;
+        A = B;
+        B = ring(Cn);
+    }
+    if (i1 == nr && wr) vr_o.set_raw(nr, j, vr.raw(nr, j));  // the outer v_r row, untouched
+}
+template <bool SPLIT>
+__global__ void __launch_bounds__(256) k_visc_walk(KParams P, int tile)
+{
+    b_visc_walk<SPLIT>(P, (int)blockIdx.x, (int)blockIdx.y, tile);
+}
+
 // the velocities of view P copied as stored to its partner buffers (a LeapFrog
 // step whose second kick ends on the swapped view hands them back this way)
 template <bool SPLIT>
@@ -1839,20 +1978,6 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3)))
     const KParams &P = batch_disk(Ps, e.disk);
     radial_sweep<SPLIT, ADI>(P, (int)(e.yx & 0xffffu), (int)(e.yx >> 16), P.radial_tile);
 }
-
-// Neighbour values within a wavefront: the value of lane - 1 and of lane + 1,
-// moved by the DPP wavefront shifts (wave_shr:1, wave_shl:1) on both 32-bit
-// halves; the edge lanes receive 0 and their results are discarded
-template <int CTRL>
-__device__ __forceinline__ double dpp_move(double x)
-{
-    const long long b = __double_as_longlong(x);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b & 0xffffffffll), CTRL, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
-    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
-}
-__device__ __forceinline__ double from_left(double x) { return dpp_move<0x138>(x); }
-__device__ __forceinline__ double from_right(double x) { return dpp_move<0x130>(x); }
 
 // van Leer slope of a sector from its two neighbours (the slope half of
 // ComputeStarTheta, TransportEuler.cpp:416-465)
@@ -2396,7 +2521,20 @@ void launch_av_fused(const KParams &P, hipStream_t st, int with_viscosity)
 }
 void launch_visc_fused(const KParams &P, hipStream_t st)
 {
-    LAUNCH(k_visc_fused, cell_grid(P, P.nr + 1), kBlock, st, P);
+    // rings per thread: 8, or 4, while that gives 3072 wavefronts (the
+    // window start costs two extra rings per walk), else the cell-parallel
+    // form
+    const long segs = (P.ns + kViscSegW - 1) / kViscSegW;
+    int tile = 8;
+    while (tile > 1 && segs * ((P.nr + tile - 1) / tile) < 3072) tile = tile > 4 ? tile / 2 : 1;
+    if (P.visc_tile > 0) tile = P.visc_tile;
+    if (tile == 1) {  // one ring per thread: the cell-parallel form
+        LAUNCH(k_visc_fused, cell_grid(P, P.nr + 1), kBlock, st, P);
+        return;
+    }
+    const int tiles = (P.nr + tile - 1) / tile;
+    const dim3 g((unsigned)segs, (unsigned)((tiles + 3) / 4));
+    LAUNCH(k_visc_walk, g, kBlock, st, P, tile);
 }
 
 void launch_derived(const KParams &P, hipStream_t st) { LAUNCH(k_derived, cell_grid(P, P.nr), kBlock, st, P); }

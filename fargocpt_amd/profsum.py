@@ -21,11 +21,11 @@ import json
 import os
 
 # bench.py times these kernels with fcpt_time_kernel: 3 rounds of 1 warm-up + 20 launches
-TIMED = ("k_radial_sweep", "k_azimuthal_sweep", "k_cfl_cells", "k_visc_fused", "k_potential<")
+TIMED = ("k_radial_sweep", "k_azimuthal_sweep", "k_cfl_cells", "k_visc_walk", "k_potential<")
 TIMING_LAUNCHES = 63
 KT_STEPS = 55
 JSON_NAMES = {"k_radial_sweep": "radial_sweep", "k_azimuthal_sweep": "azimuthal_sweep",
-              "k_cfl_cells": "cfl", "k_visc_fused": "visc_fused", "k_potential<": "potential",
+              "k_cfl_cells": "cfl", "k_visc_walk": "visc_fused", "k_potential<": "potential",
               "k_disk_force<": "disk_force"}
 
 

@@ -61,6 +61,62 @@ def test_fused_step_split(case):
         assert rel_err(a[f], b[f]) <= 2e-6, (f, rel_err(a[f], b[f]))
 
 
+VISC_FIELDS = ("VRAD", "VAZI", "TAU_R_R", "TAU_R_PHI", "TAU_PHI_PHI", "DIV_V")
+
+
+def _check_visc(got, want, precision, what):
+    for f in VISC_FIELDS:
+        if precision == 0:
+            assert np.array_equal(got[f], want[f]), (what, f, rel_err(got[f], want[f]))
+        else:
+            assert rel_err(got[f], want[f]) <= 2e-6, (what, f, rel_err(got[f], want[f]))
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_visc_walk_tiles(precision):
+    """The fused viscous pass walks `tile` rings per thread: every tile,
+    including ragged last tiles, one ring per thread and tiles longer than
+    the grid, gives the velocities and stress tensor of the separate stress +
+    update kernels (FP64: bit for bit; split: up to the separate kernels' FP32
+    tensors), on the adiabatic SN disk with viscous heating at 512 x 1536 (one
+    step) and on the 32-ring fixture disk (5 steps)."""
+    from fargocpt_amd import workloads as W
+    from fargocpt_amd.capi import Disk
+    name = "adi_512x1536"
+    cfg = W.disk_config(name, precision)
+
+    def big(tile, fused):
+        d = Disk(cfg)
+        W.start(d, name)
+        d.set_fused_step(fused)
+        d.set_visc_tile(tile)
+        d.run_step()
+        out = {f: d.download_field(f) for f in VISC_FIELDS}
+        d.close()
+        return out
+
+    want = big(0, False)
+    for tile in (1, 3, 16, 64):
+        print(f"viscous walk {name} tile={tile} precision={precision}", flush=True)
+        _check_visc(big(tile, True), want, precision, tile)
+    meta, data = load_case("c2_small")
+    c = dict(disk_config(meta), precision=precision)
+
+    def small(tile, fused):
+        d = Disk(c)
+        d.set_fused_step(fused)
+        d.set_visc_tile(tile)
+        run_case(d, meta, data, nsteps=5)
+        out = {f: d.download_field(f) for f in VISC_FIELDS}
+        d.close()
+        return out
+
+    want = small(0, False)
+    for tile in (3, 64):
+        print(f"viscous walk c2_small tile={tile} precision={precision}", flush=True)
+        _check_visc(small(tile, True), want, precision, tile)
+
+
 @pytest.mark.parametrize("integrator", [0, 1])
 def test_fused_graph_matches_stepwise(integrator):
     """Graph replay of the fused step (both views captured) against fused
