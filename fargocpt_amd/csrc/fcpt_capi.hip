@@ -933,9 +933,9 @@ static const KParams *enqueue_kick(const KParams &P, const KParams &Q, hipStream
         launch_artificial_viscosity(*cur, st, 1);  // temperature floor only
         launch_recalc_viscosity(*cur, st);
     }
-    launch_visc_fused(*cur, st);
+    if (P.adiabatic) launch_visc_energy(*cur, st);
+    else launch_visc_fused(*cur, st);
     flip();
-    if (P.adiabatic) launch_substep3(*cur, st);
     return cur;
 }
 

@@ -1343,6 +1343,174 @@ __device__ __forceinline__ void b_visc_fused(const KParams &P, int bx, int by)
 }
 FCPT_PAIR0(visc_fused, 256)
 
+// the velocities of view P copied as stored to its partner buffers (a LeapFrog
+// step whose second kick ends on the swapped view hands them back this way)
+template <bool SPLIT>
+__device__ __forceinline__ void b_copy_velocities(const KParams &P, int bx, int by)
+{
+    CELL_LOOP_HEAD(0, P.nr + 1)
+    FLD(vr, FCPT_VRAD);
+    FLD(vaz, FCPT_VAZI);
+    alt_vr<SPLIT>(P).set_raw(i, j, vr.raw(i, j));
+    if (i < P.nr) alt_vaz<SPLIT>(P).set_raw(i, j, vaz.raw(i, j));
+}
+FCPT_PAIR0(copy_velocities, 256)
+
+// ---------------------------------------------------------------------------
+// Q- of the beta cooling (thermal_relaxation, SourceEuler.cpp:632-690) for
+// one active cell
+template <bool SPLIT>
+__device__ __forceinline__ double cooling_beta_cell(const KParams &P, int i, int j, double time, double e,
+                                                    double sig)
+{
+    if (!P.cooling_beta_enabled) return 0.0;
+    const double r = cst(P.Rmed)[i];
+    const double omega_k = omega_kepler(P, r);
+    double beta_inv = 1 / P.cooling_beta;
+    if (P.cooling_beta_ramp_up > 0.0) {
+        const double x = 2 * time / P.cooling_beta_ramp_up;
+        const double ramp_factor = 1 - exp(-(x * x));
+        beta_inv = beta_inv * ramp_factor;
+    }
+    double delta_E = e;
+    if (P.cooling_beta_ref == FCPT_BETA_REF_REFERENCE)
+        delta_E -= fld<SPLIT>(P, FCPT_ENERGY0)(i, j) / fld<SPLIT>(P, FCPT_SIGMA0)(i, j) * sig;
+    if (P.cooling_beta_ref == FCPT_BETA_REF_FLOOR)
+        delta_E -= P.t_min * sig / P.mu * P.Rgas / (P.gamma - 1.0);
+    return 0.0 + delta_E * omega_k * beta_inv;
+}
+
+// Q+ of the viscous heating (viscous_heating, SourceEuler.cpp:496-536) from
+// the cell's stress tensor; tau_r_phi is the mean of the cell's four corners
+__device__ __forceinline__ double viscous_heating_q(const KParams &P, double nu, double sig, double trr,
+                                                    double tpp, double dv, double tau_r_phi)
+{
+    double q = 1.0 / (2.0 * nu * sig) * (trr * trr + 2 * (tau_r_phi * tau_r_phi) + tpp * tpp);
+    q += (2.0 / 9.0) * nu * sig * (dv * dv);
+    q *= P.heating_viscous_factor;
+    return 0.0 + q;
+}
+
+// Q- and Q+ for one active cell, the tensor read from its fields
+template <bool SPLIT>
+__device__ __forceinline__ void heating_cooling_cell(const KParams &P, int i, int j, double time,
+                                                     double e, double sig, double &qplus,
+                                                     double &qminus)
+{
+    const int ns = P.ns;
+    qminus = cooling_beta_cell<SPLIT>(P, i, j, time, e, sig);
+    qplus = 0.0;
+    if (P.heating_viscous) {
+        const double nu = fld<SPLIT>(P, FCPT_VISCOSITY)(i, j);
+        if (nu != 0.0) {
+            FLD(trp, FCPT_TAU_R_PHI);
+            const int jn = j == ns - 1 ? 0 : j + 1;
+            const double tau_r_phi = 0.25 * (trp(i, j) + trp(i + 1, j) + trp(i, jn) + trp(i + 1, jn));
+            qplus = viscous_heating_q(P, nu, sig, fld<SPLIT>(P, FCPT_TAU_R_R)(i, j),
+                                      fld<SPLIT>(P, FCPT_TAU_PHI_PHI)(i, j), fld<SPLIT>(P, FCPT_DIV_V)(i, j),
+                                      tau_r_phi);
+        }
+    }
+}
+
+// x^4 and x^3 by compensated powering (Graillat, Langlois and Louvet,
+// "Algorithms for accurate, validated and fast polynomial evaluation",
+// Japan J. Indust. Appl. Math. 26, 2009): the square is carried as the
+// unevaluated sum p + e (e from an FMA), and the last product's rounding error
+// is added back before the one final rounding, so the result is within 0.5 +
+// 2^-49 ulp of the exact power: the correctly rounded value but for inputs
+// within 2^-49 ulp of a rounding boundary, as the C library's pow (within
+// 0.52 ulp) returns it.  An overflowing power returns the rounded product.
+__device__ __forceinline__ double pow4_comp(double x)
+{
+    const double p = x * x, e = fma(x, x, -p);
+    const double q = p * p, eq = fma(p, p, -q);
+    const double r = q + (eq + 2.0 * p * e);
+    return isfinite(q) ? r : q;
+}
+__device__ __forceinline__ double pow3_comp(double x)
+{
+    const double p = x * x, e = fma(x, x, -p);
+    const double q = p * x, eq = fma(p, x, -q);
+    const double r = q + (eq + e * x);
+    return isfinite(q) ? r : q;
+}
+
+// the radiative correction of SubStep3 (SourceEuler.cpp:920-924; its
+// std::pow(., 4) and std::pow(., 3) by compensated powering)
+__device__ __forceinline__ double rad_alpha(const KParams &P, double h, double sig, double e)
+{
+    const double inv_pow4 = pow4_comp(P.mu * (P.gamma - 1.0) / (P.Rgas * sig));
+    return 1.0 + 2.0 * h * 4.0 * P.sigma_sb / P.c_light * inv_pow4 * pow3_comp(e);
+}
+
+// SubStep3 (SourceEuler.cpp:859-954) at one cell from its Q+ and Q- before
+// the radiative correction; writes T, Q+, Q- and E
+template <bool SPLIT>
+__device__ __forceinline__ void substep3_cell(const KParams &P, int i, int j, double e, double sig, double h,
+                                              double qplus, double qminus)
+{
+    const double dt = P.s->dt;
+    // compute_temperature (SourceEuler.cpp:1378-1408) before the update
+    const double c_v_inv = P.mu / P.Rgas * (P.gamma - 1.0);
+    const double temperature = c_v_inv * e / sig;
+    if (i >= 1 && i < P.nr - 1) {
+        const double alpha = rad_alpha(P, h, sig, e);
+        qplus /= alpha;
+        qminus /= alpha;
+        double energy_new = e + dt * (qplus - qminus);
+        const double SigmaFloor = 10.0 * P.sigma0 * P.sigma_floor;
+        if ((sig < SigmaFloor)) {
+            // TAU_EFF is never computed without surface cooling: it holds 0
+            const double tau_eff = 0.0;
+            const double e4 = qplus * tau_eff / (2.0 * P.sigma_sb);
+            const double constant = (P.Rgas / P.mu * sig / (P.gamma - 1.0));
+            energy_new = sqrt(sqrt(e4)) * constant;  // pow(e4, 1 / 4): 0 here, as tau_eff is 0
+            qminus = qplus;
+        }
+        e = energy_new;
+    } else {
+        qplus = qminus = 0.0;
+    }
+    fld<SPLIT>(P, FCPT_TEMPERATURE).set(i, j, temperature);
+    fld<SPLIT>(P, FCPT_QPLUS).set(i, j, qplus);
+    fld<SPLIT>(P, FCPT_QMINUS).set(i, j, qminus);
+    fld<SPLIT>(P, FCPT_ENERGY).set(i, j, temperature_floor(P, e, sig));
+}
+
+template <bool SPLIT>
+__device__ __forceinline__ void b_substep3(const KParams &P, int bx, int by)
+{
+    CELL_LOOP_HEAD(0, P.nr)
+    const double sig = fld<SPLIT>(P, FCPT_SIGMA)(i, j);
+    const double e = fld<SPLIT>(P, FCPT_ENERGY)(i, j);
+    const double h = fld<SPLIT>(P, FCPT_SCALE_HEIGHT)(i, j);
+    double qplus = 0.0, qminus = 0.0;
+    if (i >= 1 && i < P.nr - 1) heating_cooling_cell<SPLIT>(P, i, j, P.s->time, e, sig, qplus, qminus);
+    substep3_cell<SPLIT>(P, i, j, e, sig, h, qplus, qminus);
+}
+FCPT_PAIR0(substep3, 256)
+
+// compute_heating_cooling_for_CFL, SourceEuler.cpp:1410-1450 (after update_viscosity
+// and the stress tensor)
+template <bool SPLIT>
+__device__ __forceinline__ void b_heatcool_cfl(const KParams &P, int bx, int by)
+{
+    CELL_LOOP_HEAD(0, P.nr)
+    double qplus = 0.0, qminus = 0.0;
+    if (i >= 1 && i < P.nr - 1) {
+        const double e = fld<SPLIT>(P, FCPT_ENERGY)(i, j);
+        const double sig = fld<SPLIT>(P, FCPT_SIGMA)(i, j);
+        heating_cooling_cell<SPLIT>(P, i, j, P.s->time, e, sig, qplus, qminus);
+        const double alpha = rad_alpha(P, fld<SPLIT>(P, FCPT_SCALE_HEIGHT)(i, j), sig, e);
+        qplus /= alpha;
+        qminus /= alpha;
+    }
+    fld<SPLIT>(P, FCPT_QPLUS).set(i, j, qplus);
+    fld<SPLIT>(P, FCPT_QMINUS).set(i, j, qminus);
+}
+FCPT_PAIR0(heatcool_cfl, 256)
+
 // Neighbour values within a wavefront: the value of lane - 1 and of lane + 1,
 // moved by the DPP wavefront shifts (wave_shr:1, wave_shl:1) on both 32-bit
 // halves; the edge lanes receive 0 and their results are discarded
@@ -1368,7 +1536,11 @@ __device__ __forceinline__ double from_right(double x) { return dpp_move<0x130>(
 // corner (i + 1, j), whose value at sector j + 1 is the right lane's; the
 // diagonal of ring i - 1 and the corners of ring i carry over from the
 // previous step.  Every value goes through the helpers b_visc_fused calls,
-// with the same arguments, so the two are bit-identical.
+// with the same arguments, so the two are bit-identical.  With ENERGY the walk
+// also runs SubStep3 on each cell it updates, its viscous heating formed from
+// the tensor in registers (the corner mean takes tau_rp of corner (i + 1,
+// j + 1) from the right lane), exactly as b_substep3 forms it from the stored
+// tensor, which the walk then does not store.
 constexpr int kViscSegW = 62;  // sectors written per wavefront
 
 struct ViscRing {
@@ -1377,7 +1549,7 @@ struct ViscRing {
     double vaR;           // v_phi of sector j + 1
 };
 
-template <bool SPLIT>
+template <bool SPLIT, bool ENERGY>
 __device__ __forceinline__ void b_visc_walk(const KParams &P, int bx, int by, int tile)
 {
     const int ns = P.ns, nr = P.nr;
@@ -1396,7 +1568,7 @@ __device__ __forceinline__ void b_visc_walk(const KParams &P, int bx, int by, in
     const Fld<SPLIT> vr_o = alt_vr<SPLIT>(P);
     const Fld<SPLIT> vaz_o = alt_vaz<SPLIT>(P);
     const bool iso = !P.adiabatic;  // nu is a ring profile
-    const bool store = P.adiabatic && P.heating_viscous;
+    const bool store = !ENERGY && P.adiabatic && P.heating_viscous;
     const double dt = P.s->dt;
     // the lane's values of ring r (r < nr); ring nr holds only v_r
     struct Raw {
@@ -1451,6 +1623,7 @@ __device__ __forceinline__ void b_visc_walk(const KParams &P, int bx, int by, in
         const double tppj = from_left(d.tpp);  // diagonal of (i, j - 1)
         const double trp_n = from_right(trp_c);  // corner (i, j + 1)
         const double trp1 = i + 1 <= nr - 1 ? corner(i + 1, A, B) : 0.0;
+        const double trp1_n = ENERGY ? from_right(trp1) : 0.0;  // corner (i + 1, j + 1)
         if (wr) {
             if (store) {
                 fld<SPLIT>(P, FCPT_DIV_V).set(i, j, d.divv);
@@ -1467,6 +1640,18 @@ __device__ __forceinline__ void b_visc_walk(const KParams &P, int bx, int by, in
                                            d.tpp, dprev.tpp));
             else
                 vr_o.set_raw(i, j, vr.raw(i, j));
+            if constexpr (ENERGY) {
+                const double e = fld<SPLIT>(P, FCPT_ENERGY)(i, j);
+                const double h = fld<SPLIT>(P, FCPT_SCALE_HEIGHT)(i, j);
+                double qplus = 0.0, qminus = 0.0;
+                if (upd_p) {
+                    qminus = cooling_beta_cell<SPLIT>(P, i, j, P.s->time, e, A.s);
+                    if (P.heating_viscous && A.n != 0.0)
+                        qplus = viscous_heating_q(P, A.n, A.s, d.trr, d.tpp, d.divv,
+                                                  0.25 * (trp_c + trp1 + trp_n + trp1_n));
+                }
+                substep3_cell<SPLIT>(P, i, j, e, A.s, h, qplus, qminus);
+            }
         }
         dprev = d;
         trp_c = trp1;
@@ -1476,133 +1661,11 @@ __device__ __forceinline__ void b_visc_walk(const KParams &P, int bx, int by, in
     }
     if (i1 == nr && wr) vr_o.set_raw(nr, j, vr.raw(nr, j));  // the outer v_r row, untouched
 }
-template <bool SPLIT>
+template <bool SPLIT, bool ENERGY>
 __global__ void __launch_bounds__(256) k_visc_walk(KParams P, int tile)
 {
-    b_visc_walk<SPLIT>(P, (int)blockIdx.x, (int)blockIdx.y, tile);
+    b_visc_walk<SPLIT, ENERGY>(P, (int)blockIdx.x, (int)blockIdx.y, tile);
 }
-
-// the velocities of view P copied as stored to its partner buffers (a LeapFrog
-// step whose second kick ends on the swapped view hands them back this way)
-template <bool SPLIT>
-__device__ __forceinline__ void b_copy_velocities(const KParams &P, int bx, int by)
-{
-    CELL_LOOP_HEAD(0, P.nr + 1)
-    FLD(vr, FCPT_VRAD);
-    FLD(vaz, FCPT_VAZI);
-    alt_vr<SPLIT>(P).set_raw(i, j, vr.raw(i, j));
-    if (i < P.nr) alt_vaz<SPLIT>(P).set_raw(i, j, vaz.raw(i, j));
-}
-FCPT_PAIR0(copy_velocities, 256)
-
-// ---------------------------------------------------------------------------
-// Q- (thermal_relaxation, SourceEuler.cpp:632-690) and Q+ (viscous_heating,
-// SourceEuler.cpp:496-536) for one active cell
-template <bool SPLIT>
-__device__ __forceinline__ void heating_cooling_cell(const KParams &P, int i, int j, double time,
-                                                     double e, double sig, double &qplus,
-                                                     double &qminus)
-{
-    const int ns = P.ns;
-    qminus = 0.0;
-    if (P.cooling_beta_enabled) {
-        const double r = cst(P.Rmed)[i];
-        const double omega_k = omega_kepler(P, r);
-        double beta_inv = 1 / P.cooling_beta;
-        if (P.cooling_beta_ramp_up > 0.0) {
-            const double x = 2 * time / P.cooling_beta_ramp_up;
-            const double ramp_factor = 1 - exp(-(x * x));
-            beta_inv = beta_inv * ramp_factor;
-        }
-        double delta_E = e;
-        if (P.cooling_beta_ref == FCPT_BETA_REF_REFERENCE)
-            delta_E -= fld<SPLIT>(P, FCPT_ENERGY0)(i, j) / fld<SPLIT>(P, FCPT_SIGMA0)(i, j) * sig;
-        if (P.cooling_beta_ref == FCPT_BETA_REF_FLOOR)
-            delta_E -= P.t_min * sig / P.mu * P.Rgas / (P.gamma - 1.0);
-        qminus = 0.0 + delta_E * omega_k * beta_inv;
-    }
-    qplus = 0.0;
-    if (P.heating_viscous) {
-        const double nu = fld<SPLIT>(P, FCPT_VISCOSITY)(i, j);
-        if (nu != 0.0) {
-            FLD(trp, FCPT_TAU_R_PHI);
-            const int jn = j == ns - 1 ? 0 : j + 1;
-            const double tau_r_phi = 0.25 * (trp(i, j) + trp(i + 1, j) + trp(i, jn) + trp(i + 1, jn));
-            const double trr = fld<SPLIT>(P, FCPT_TAU_R_R)(i, j);
-            const double tpp = fld<SPLIT>(P, FCPT_TAU_PHI_PHI)(i, j);
-            const double dv = fld<SPLIT>(P, FCPT_DIV_V)(i, j);
-            double q = 1.0 / (2.0 * nu * sig) * (trr * trr + 2 * (tau_r_phi * tau_r_phi) + tpp * tpp);
-            q += (2.0 / 9.0) * nu * sig * (dv * dv);
-            q *= P.heating_viscous_factor;
-            qplus = 0.0 + q;
-        }
-    }
-}
-
-__device__ __forceinline__ double rad_alpha(const KParams &P, double h, double sig, double e)
-{
-    const double inv_pow4 = pow(P.mu * (P.gamma - 1.0) / (P.Rgas * sig), 4.0);
-    return 1.0 + 2.0 * h * 4.0 * P.sigma_sb / P.c_light * inv_pow4 * pow(e, 3.0);
-}
-
-// SubStep3, SourceEuler.cpp:859-954
-template <bool SPLIT>
-__device__ __forceinline__ void b_substep3(const KParams &P, int bx, int by)
-{
-    CELL_LOOP_HEAD(0, P.nr)
-    FLD(E, FCPT_ENERGY);
-    const double sig = fld<SPLIT>(P, FCPT_SIGMA)(i, j);
-    double e = E(i, j);
-    const double h = fld<SPLIT>(P, FCPT_SCALE_HEIGHT)(i, j);
-    const DevScalars *s = P.s;
-    const double dt = s->dt, time = s->time;
-    // compute_temperature (SourceEuler.cpp:1378-1408) before the update
-    const double c_v_inv = P.mu / P.Rgas * (P.gamma - 1.0);
-    const double temperature = c_v_inv * e / sig;
-    double qplus = 0.0, qminus = 0.0;
-    if (i >= 1 && i < P.nr - 1) {
-        heating_cooling_cell<SPLIT>(P, i, j, time, e, sig, qplus, qminus);
-        const double alpha = rad_alpha(P, h, sig, e);
-        qplus /= alpha;
-        qminus /= alpha;
-        double energy_new = e + dt * (qplus - qminus);
-        const double SigmaFloor = 10.0 * P.sigma0 * P.sigma_floor;
-        if ((sig < SigmaFloor)) {
-            // TAU_EFF is never computed without surface cooling: it holds 0
-            const double tau_eff = 0.0;
-            const double e4 = qplus * tau_eff / (2.0 * P.sigma_sb);
-            const double constant = (P.Rgas / P.mu * sig / (P.gamma - 1.0));
-            energy_new = pow(e4, 1.0 / 4.0) * constant;
-            qminus = qplus;
-        }
-        e = energy_new;
-    }
-    fld<SPLIT>(P, FCPT_TEMPERATURE).set(i, j, temperature);
-    fld<SPLIT>(P, FCPT_QPLUS).set(i, j, qplus);
-    fld<SPLIT>(P, FCPT_QMINUS).set(i, j, qminus);
-    E.set(i, j, temperature_floor(P, e, sig));
-}
-FCPT_PAIR0(substep3, 256)
-
-// compute_heating_cooling_for_CFL, SourceEuler.cpp:1410-1450 (after update_viscosity
-// and the stress tensor)
-template <bool SPLIT>
-__device__ __forceinline__ void b_heatcool_cfl(const KParams &P, int bx, int by)
-{
-    CELL_LOOP_HEAD(0, P.nr)
-    double qplus = 0.0, qminus = 0.0;
-    if (i >= 1 && i < P.nr - 1) {
-        const double e = fld<SPLIT>(P, FCPT_ENERGY)(i, j);
-        const double sig = fld<SPLIT>(P, FCPT_SIGMA)(i, j);
-        heating_cooling_cell<SPLIT>(P, i, j, P.s->time, e, sig, qplus, qminus);
-        const double alpha = rad_alpha(P, fld<SPLIT>(P, FCPT_SCALE_HEIGHT)(i, j), sig, e);
-        qplus /= alpha;
-        qminus /= alpha;
-    }
-    fld<SPLIT>(P, FCPT_QPLUS).set(i, j, qplus);
-    fld<SPLIT>(P, FCPT_QMINUS).set(i, j, qminus);
-}
-FCPT_PAIR0(heatcool_cfl, 256)
 
 // ---------------------------------------------------------------------------
 // damping (boundary_conditions/damping.cpp:311-427, 754-776): block row y maps
@@ -2519,22 +2582,47 @@ void launch_av_fused(const KParams &P, hipStream_t st, int with_viscosity)
 {
     LAUNCH(k_av_fused, cell_grid(P, P.nr + 1), kBlock, st, P, with_viscosity);
 }
-void launch_visc_fused(const KParams &P, hipStream_t st)
+// rings per thread of the viscous walk: 8, or 4, while that gives 3072
+// wavefronts (the window start costs two extra rings per walk), else 1 (the
+// cell-parallel form)
+static int visc_tile(const KParams &P)
 {
-    // rings per thread: 8, or 4, while that gives 3072 wavefronts (the
-    // window start costs two extra rings per walk), else the cell-parallel
-    // form
+    if (P.visc_tile > 0) return P.visc_tile;
     const long segs = (P.ns + kViscSegW - 1) / kViscSegW;
     int tile = 8;
     while (tile > 1 && segs * ((P.nr + tile - 1) / tile) < 3072) tile = tile > 4 ? tile / 2 : 1;
-    if (P.visc_tile > 0) tile = P.visc_tile;
-    if (tile == 1) {  // one ring per thread: the cell-parallel form
-        LAUNCH(k_visc_fused, cell_grid(P, P.nr + 1), kBlock, st, P);
-        return;
-    }
+    return tile;
+}
+
+static void launch_visc_walk(const KParams &P, int tile, bool energy, hipStream_t st)
+{
     const int tiles = (P.nr + tile - 1) / tile;
-    const dim3 g((unsigned)segs, (unsigned)((tiles + 3) / 4));
-    LAUNCH(k_visc_walk, g, kBlock, st, P, tile);
+    const dim3 g((unsigned)((P.ns + kViscSegW - 1) / kViscSegW), (unsigned)((tiles + 3) / 4));
+    if (P.split) {
+        if (energy) hipLaunchKernelGGL((k_visc_walk<true, true>), g, kBlock, 0, st, P, tile);
+        else hipLaunchKernelGGL((k_visc_walk<true, false>), g, kBlock, 0, st, P, tile);
+    } else {
+        if (energy) hipLaunchKernelGGL((k_visc_walk<false, true>), g, kBlock, 0, st, P, tile);
+        else hipLaunchKernelGGL((k_visc_walk<false, false>), g, kBlock, 0, st, P, tile);
+    }
+}
+
+void launch_visc_fused(const KParams &P, hipStream_t st)
+{
+    const int tile = visc_tile(P);
+    if (tile == 1) LAUNCH(k_visc_fused, cell_grid(P, P.nr + 1), kBlock, st, P);
+    else launch_visc_walk(P, tile, false, st);
+}
+
+void launch_visc_energy(const KParams &P, hipStream_t st)
+{
+    const int tile = visc_tile(P);
+    if (tile == 1) {
+        LAUNCH(k_visc_fused, cell_grid(P, P.nr + 1), kBlock, st, P);
+        launch_substep3(P, st);
+    } else {
+        launch_visc_walk(P, tile, true, st);
+    }
 }
 
 void launch_derived(const KParams &P, hipStream_t st) { LAUNCH(k_derived, cell_grid(P, P.nr), kBlock, st, P); }

@@ -41,6 +41,9 @@ void launch_src_fused(const KParams &P, hipStream_t st);
 void launch_src_energy(const KParams &P, hipStream_t st);
 void launch_av_fused(const KParams &P, hipStream_t st, int with_viscosity);
 void launch_visc_fused(const KParams &P, hipStream_t st);
+// the fused viscous pass followed by SubStep3 (adiabatic disks); one walk
+// that does not store the stress tensor where the grid is large enough
+void launch_visc_energy(const KParams &P, hipStream_t st);
 void launch_radial_sweep(const KParams &P, hipStream_t st);
 void launch_azimuthal_sweep(const KParams &P, hipStream_t st);
 void launch_cfl_cells(const KParams &P, hipStream_t st);

@@ -61,11 +61,12 @@ def test_fused_step_split(case):
         assert rel_err(a[f], b[f]) <= 2e-6, (f, rel_err(a[f], b[f]))
 
 
-VISC_FIELDS = ("VRAD", "VAZI", "TAU_R_R", "TAU_R_PHI", "TAU_PHI_PHI", "DIV_V")
+STEP_FIELDS = ("SIGMA", "VRAD", "VAZI", "ENERGY", "TEMPERATURE", "QPLUS", "QMINUS")
+TENSOR = ("TAU_R_R", "TAU_R_PHI", "TAU_PHI_PHI", "DIV_V")
 
 
-def _check_visc(got, want, precision, what):
-    for f in VISC_FIELDS:
+def _check(got, want, precision, what):
+    for f in want:
         if precision == 0:
             assert np.array_equal(got[f], want[f]), (what, f, rel_err(got[f], want[f]))
         else:
@@ -74,14 +75,17 @@ def _check_visc(got, want, precision, what):
 
 @pytest.mark.parametrize("precision", [0, 1])
 def test_visc_walk_tiles(precision):
-    """The fused viscous pass walks `tile` rings per thread: every tile,
-    including ragged last tiles, one ring per thread and tiles longer than
-    the grid, gives the velocities and stress tensor of the separate stress +
-    update kernels (FP64: bit for bit; split: up to the separate kernels' FP32
-    tensors), on the adiabatic SN disk with viscous heating at 512 x 1536 (one
-    step) and on the 32-ring fixture disk (5 steps)."""
+    """The fused viscous pass walks `tile` rings per thread (1: the
+    cell-parallel form) and, in an adiabatic disk, runs SubStep3 in the same
+    walk from the tensor in registers.  Every tile, including ragged last
+    tiles and tiles longer than the grid, gives the step of the separate
+    stress, update and SubStep3 kernels (FP64: bit for bit; split: up to the
+    separate kernels' FP32 tensors) on the adiabatic SN disk with viscous
+    heating at 512 x 1536 (one step) and on the 32-ring fixture disk (5
+    steps); the walk without SubStep3 stores the tensor of the separate
+    stress kernel."""
     from fargocpt_amd import workloads as W
-    from fargocpt_amd.capi import Disk
+    from fargocpt_amd.capi import KERNEL_STRESS, KERNELS, Disk
     name = "adi_512x1536"
     cfg = W.disk_config(name, precision)
 
@@ -91,14 +95,28 @@ def test_visc_walk_tiles(precision):
         d.set_fused_step(fused)
         d.set_visc_tile(tile)
         d.run_step()
-        out = {f: d.download_field(f) for f in VISC_FIELDS}
+        out = {f: d.download_field(f) for f in STEP_FIELDS}
+        d.close()
+        return out
+
+    def tensor(tile):
+        d = Disk(cfg)
+        W.start(d, name)
+        d.set_visc_tile(tile)
+        if tile:
+            d.time_kernel(KERNELS["visc_fused"], 1)
+        else:
+            d.time_kernel(KERNEL_STRESS, 1)
+        out = {f: d.download_field(f) for f in TENSOR}
         d.close()
         return out
 
     want = big(0, False)
+    want_t = tensor(0)
     for tile in (1, 3, 16, 64):
         print(f"viscous walk {name} tile={tile} precision={precision}", flush=True)
-        _check_visc(big(tile, True), want, precision, tile)
+        _check(big(tile, True), want, precision, tile)
+        _check(tensor(tile), want_t, precision, tile)
     meta, data = load_case("c2_small")
     c = dict(disk_config(meta), precision=precision)
 
@@ -107,14 +125,14 @@ def test_visc_walk_tiles(precision):
         d.set_fused_step(fused)
         d.set_visc_tile(tile)
         run_case(d, meta, data, nsteps=5)
-        out = {f: d.download_field(f) for f in VISC_FIELDS}
+        out = {f: d.download_field(f) for f in STEP_FIELDS}
         d.close()
         return out
 
     want = small(0, False)
     for tile in (3, 64):
         print(f"viscous walk c2_small tile={tile} precision={precision}", flush=True)
-        _check_visc(small(tile, True), want, precision, tile)
+        _check(small(tile, True), want, precision, tile)
 
 
 @pytest.mark.parametrize("integrator", [0, 1])
