@@ -32,7 +32,7 @@ KERNEL_DISK_FORCE = 6  # the disk-on-planet reduction (not part of the hydro ste
 # storage modes (fcpt_disk_desc.precision) and launch parameters of include/fcpt.h
 PREC_FP64, PREC_SPLIT32 = 0, 1
 INTEGRATOR_EULER, INTEGRATOR_LEAPFROG = 0, 1
-PARAM_RADIAL_TILE, PARAM_RADIAL_WAVES, PARAM_FUSED_STEP, PARAM_VISC_TILE = 0, 1, 2, 3
+PARAM_RADIAL_TILE, PARAM_RADIAL_WAVES, PARAM_FUSED_STEP, PARAM_VISC_TILE, PARAM_KICK_TILE = 0, 1, 2, 3, 4
 BC = {"reflecting": 0, "zerogradient": 1, "outflow": 2}
 MAX_PLANETS = 8
 
@@ -459,6 +459,12 @@ class Disk:
         """Rings walked by one thread of the fused viscous pass (0:
         size-based default)."""
         self._check(self.lib.fcpt_set_launch_param(self.h, PARAM_VISC_TILE, rings))
+
+    def set_kick_tile(self, rings):
+        """Rings walked by one thread of the kick walk (0: size-based
+        default; 1: the sources and artificial viscosity as separate
+        passes)."""
+        self._check(self.lib.fcpt_set_launch_param(self.h, PARAM_KICK_TILE, rings))
 
     def autotune(self, reps=10):
         """Time the tunable launch shapes on the current state and keep the

@@ -254,6 +254,7 @@ int fcpt_create(const fcpt_disk_desc *desc, fcpt_ctx **out)
     P.radial_tile = 0;
     P.radial_waves = 3;
     P.visc_tile = 0;
+    P.kick_tile = 0;
     init_launch_fields(P);
     // Interpret.cpp:230-231
     P.dphi = 2.0 * M_PI / (double)ns;
@@ -909,6 +910,15 @@ int fcpt_get_nshift(fcpt_ctx *ctx, int32_t *nshift)
 static const KParams *enqueue_kick(const KParams &P, const KParams &Q, hipStream_t st, bool fused,
                                    bool pressure_first)
 {
+    if (fused && kick_walk_applies(P)) {
+        // the potential pass and two walks: sources and artificial
+        // viscosity; then the viscous stress and update (and SubStep3)
+        if (pressure_first && P.adiabatic) launch_pressure(P, st);
+        launch_kick(P, st);
+        if (P.adiabatic) launch_visc_energy(Q, st);
+        else launch_visc_fused(Q, st);
+        return &P;
+    }
     launch_potential(P, st);
     if (!fused) {
         if (pressure_first) launch_pressure(P, st);
@@ -1169,6 +1179,8 @@ int fcpt_set_launch_param(fcpt_ctx *ctx, int32_t param, int32_t value)
         if (value != 0 && value != 1) return fail(ctx, FCPT_ERR_ARG, "fused step must be 0 or 1");
     } else if (param == FCPT_PARAM_VISC_TILE) {
         if (value < 0 || value > 64) return fail(ctx, FCPT_ERR_ARG, "viscous tile must be 0 .. 64");
+    } else if (param == FCPT_PARAM_KICK_TILE) {
+        if (value < 0 || value > 64) return fail(ctx, FCPT_ERR_ARG, "kick tile must be 0 .. 64");
     } else {
         return fail(ctx, FCPT_ERR_ARG, "unknown launch parameter");
     }
@@ -1176,6 +1188,7 @@ int fcpt_set_launch_param(fcpt_ctx *ctx, int32_t param, int32_t value)
     if (param == FCPT_PARAM_RADIAL_TILE) ctx->P.radial_tile = value;
     else if (param == FCPT_PARAM_RADIAL_WAVES) ctx->P.radial_waves = value;
     else if (param == FCPT_PARAM_VISC_TILE) ctx->P.visc_tile = value;
+    else if (param == FCPT_PARAM_KICK_TILE) ctx->P.kick_tile = value;
     else ctx->fused = value != 0;
     if (ctx->step_graph) {
         HIP_TRY(ctx, hipGraphExecDestroy(ctx->step_graph));

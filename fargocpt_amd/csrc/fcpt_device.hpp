@@ -77,6 +77,7 @@ struct KParams {
     int radial_tile;     // rings per thread of the radial sweep (0: size-based choice)
     int radial_waves;    // register budget of the radial sweep: wavefronts per SIMD (2 or 3)
     int visc_tile;       // rings per thread of the fused viscous pass (0: size-based choice)
+    int kick_tile;       // rings per thread of the kick walk (0: size-based choice, 1: off)
     int cfl_gx, cfl_gy;  // grid of the CFL cell kernel (its block-minimum slots)
     // fields in HBM (indexed by fcpt_field): double, or float when split
     void *f[FCPT_NUM_FIELDS];

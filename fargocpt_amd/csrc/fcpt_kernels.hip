@@ -139,6 +139,20 @@ __device__ __forceinline__ Fld<SPLIT> buf(const KParams &P, void *p, const doubl
 
 #define FLD(name, id) const Fld<SPLIT> name = fld<SPLIT>(P, id)
 
+// Neighbour values within a wavefront: the value of lane - 1 and of lane + 1,
+// moved by the DPP wavefront shifts (wave_shr:1, wave_shl:1) on both 32-bit
+// halves; the edge lanes receive 0 and their results are discarded
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double x)
+{
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b & 0xffffffffll), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+__device__ __forceinline__ double from_left(double x) { return dpp_move<0x138>(x); }
+__device__ __forceinline__ double from_right(double x) { return dpp_move<0x130>(x); }
+
 // Kernels read every input before their first store: the compiler must
 // assume that a store may alias any later load, so a load written after a
 // store waits for it and the memory latency is paid once per such phase.
@@ -495,16 +509,14 @@ __global__ void __launch_bounds__(256) kb_potential_bg(const KParams *Ps, const 
     b_potential_bg(batch_disk(Ps, e.disk), (int)(e.yx & 0xffffu));
 }
 
-// CalculateNbodyPotential, Pframeforce.cpp:44-85 (+ compute_smoothing, Force.cpp:145-159)
+// CalculateNbodyPotential, Pframeforce.cpp:44-85 (+ compute_smoothing,
+// Force.cpp:145-159), at cell (i, j) with scale height H
 template <bool SPLIT>
-__device__ __forceinline__ void b_potential(const KParams &P, int bx, int by)
+__device__ __forceinline__ double potential_cell(const KParams &P, int i, int j, double H)
 {
-    CELL_LOOP_HEAD(0, P.nr)
-    FLD(Hf, FCPT_SCALE_HEIGHT);
     const DevScalars *s = P.s;
     const double x = cst(P.Rmed)[i] * P.cosphi[j];
     const double y = cst(P.Rmed)[i] * P.sinphi[j];
-    const double H = P.adiabatic ? Hf(i, j) : cst(P.iso_h)[i];
     double acc = 0.0;
     const int np = s->n_planets;
     for (int k = 0; k < np; ++k) {
@@ -526,7 +538,15 @@ __device__ __forceinline__ void b_potential(const KParams &P, int bx, int by)
         else acc += -P.G * s->planet[k].mass / d_smoothed * smooth_factor_klahr;
     }
     acc += -s->indirect_x * x - s->indirect_y * y;
-    fld<SPLIT>(P, FCPT_POTENTIAL).set(i, j, acc);
+    return acc;
+}
+
+template <bool SPLIT>
+__device__ __forceinline__ void b_potential(const KParams &P, int bx, int by)
+{
+    CELL_LOOP_HEAD(0, P.nr)
+    const double H = P.adiabatic ? fld<SPLIT>(P, FCPT_SCALE_HEIGHT)(i, j) : cst(P.iso_h)[i];
+    fld<SPLIT>(P, FCPT_POTENTIAL).set(i, j, potential_cell<SPLIT>(P, i, j, H));
 }
 FCPT_PAIR0(potential, 256)
 
@@ -593,7 +613,16 @@ __device__ __forceinline__ void b_src_vphi(const KParams &P, int bx, int by)
 }
 FCPT_PAIR0(src_vphi, 256)
 
-// compression_heating, SourceEuler.cpp:459-493
+// compression_heating, SourceEuler.cpp:459-493, for one cell (rings 0 .. nr-2)
+// from v_r(i), v_r(i + 1), v_phi(j), v_phi(j + 1)
+__device__ __forceinline__ double src_energy_new(const KParams &P, int i, double dt, double e, double vr0,
+                                                 double vr1, double va0, double va1)
+{
+    const double DIV_V = (vr1 * cst(P.Rinf)[i + 1] - vr0 * cst(P.Rinf)[i]) * cst(P.InvDiffRsupRb)[i] +
+                         (va1 - va0) * P.invdphi * cst(P.InvRmed)[i];
+    return e * exp(-(P.gamma - 1.0) * dt * DIV_V);
+}
+
 template <bool SPLIT>
 __device__ __forceinline__ void b_src_energy(const KParams &P, int bx, int by)
 {
@@ -603,10 +632,7 @@ __device__ __forceinline__ void b_src_energy(const KParams &P, int bx, int by)
     FLD(E, FCPT_ENERGY);
     const double dt = P.s->dt;
     const int jn = j == ns - 1 ? 0 : j + 1;
-    const double DIV_V = (vr(i + 1, j) * cst(P.Rinf)[i + 1] - vr(i, j) * cst(P.Rinf)[i]) * cst(P.InvDiffRsupRb)[i] +
-                         (vaz(i, jn) - vaz(i, j)) * P.invdphi * cst(P.InvRmed)[i];
-    const double energy_old = E(i, j);
-    E.set(i, j, energy_old * exp(-(P.gamma - 1.0) * dt * DIV_V));
+    E.set(i, j, src_energy_new(P, i, dt, E(i, j), vr(i, j), vr(i + 1, j), vaz(i, j), vaz(i, jn)));
 }
 FCPT_PAIR0(src_energy, 256)
 
@@ -662,6 +688,23 @@ __device__ __forceinline__ DerivedIn derived_load(const KParams &P, int i, int j
     return d;
 }
 
+// adiabatic sound speed, scale height and viscosity of one cell
+// (compute_sound_speed, compute_scale_height, compute_viscosity)
+struct AdiDerived {
+    double cs, h, nu;
+};
+__device__ __forceinline__ AdiDerived adi_derived(const KParams &P, int i, double e, double sig)
+{
+    AdiDerived d;
+    const double g = P.gamma;
+    d.cs = sqrt(g * (g - 1.0) * e / sig);
+    const double inv_omega_kepler = 1.0 / omega_kepler(P, cst(P.Rmed)[i]);
+    d.h = d.cs / (sqrt(g)) * inv_omega_kepler;
+    const double alpha = P.alpha;
+    d.nu = alpha * d.h * d.cs;
+    return d;
+}
+
 template <bool SPLIT>
 __device__ __forceinline__ void derived_store(const KParams &P, int i, int j, const DerivedIn &d,
                                              bool with_temperature, bool with_pressure,
@@ -675,11 +718,10 @@ __device__ __forceinline__ void derived_store(const KParams &P, int i, int j, co
             const double c_v_inv = P.mu / P.Rgas * (P.gamma - 1.0);
             fld<SPLIT>(P, FCPT_TEMPERATURE).set(i, j, c_v_inv * e / sig);
         }
-        const double g = P.gamma;
-        c_s = sqrt(g * (g - 1.0) * e / sig);
+        const AdiDerived a = adi_derived(P, i, e, sig);
+        c_s = a.cs;
+        h = a.h;
         fld<SPLIT>(P, FCPT_SOUNDSPEED).set(i, j, c_s);
-        const double inv_omega_kepler = 1.0 / omega_kepler(P, cst(P.Rmed)[i]);
-        h = c_s / (sqrt(g)) * inv_omega_kepler;
         fld<SPLIT>(P, FCPT_SCALE_HEIGHT).set(i, j, h);
         if (with_pressure) fld<SPLIT>(P, FCPT_PRESSURE).set(i, j, (P.gamma - 1.0) * e);
     } else {
@@ -1257,6 +1299,165 @@ __device__ __forceinline__ void b_av_fused(const KParams &P, int bx, int by, int
 }
 FCPT_PAIR1(av_fused, 256, int, with_viscosity)
 
+// The kick's source terms and artificial viscosity as one radial walk:
+// update_with_sourceterms, compression_heating (adiabatic) and the TW or SN
+// artificial viscosity with its dissipation and recalculate_viscosity, the
+// per-cell steps of b_src_fused, b_src_energy and b_av_fused in that order,
+// after the potential pass.  A wavefront owns kKickSegW consecutive sectors
+// and `tile` consecutive rings: lane k holds sector j0 - 2 + k (lanes 0, 1,
+// 62 and 63 are the halo) and walks the rings outwards.  Each ring is loaded
+// once (v_r, v_phi, Sigma, the potential, and the pressure of an adiabatic
+// disk); the source-term velocities of a ring are formed one ring ahead of the
+// artificial viscosity that reads them, the neighbouring sectors' values come
+// from the adjacent lanes by DPP moves (each stage spoils one more lane at
+// either edge), and the tensor of ring i - 1 carries over.  E is read and
+// written only at the lane's own cells, so no wavefront reads what another
+// writes, and the derived fields it stores (c_s, H, nu) are not read here.
+// Every value goes through the helpers
+// of the separate kernels with the same arguments, so FP64 storage gives the
+// same fields bit for bit; in split storage the source-term velocities stay
+// in FP64 between the stages.
+constexpr int kKickSegW = 60;
+
+struct KickRing {
+    double vr, va, s, p, phi;  // the lane's sector (p: pressure, phi: potential)
+    double sL, vaR;            // Sigma of sector j - 1, v_phi of sector j + 1
+};
+
+template <bool SPLIT, bool ADI, bool SN>
+__device__ __forceinline__ void b_kick_walk(const KParams &P, int bx, int by, int tile)
+{
+    const int ns = P.ns, nr = P.nr;
+    const int k = (int)threadIdx.x;  // lane
+    const int i0 = __builtin_amdgcn_readfirstlane((by * (int)blockDim.y + (int)threadIdx.y) * tile);
+    if (i0 >= nr) return;
+    const int i1 = i0 + tile < nr ? i0 + tile : nr;  // rings [i0, i1)
+    const int j0 = bx * kKickSegW;
+    const int Ls = ns - j0 < kKickSegW ? ns - j0 : kKickSegW;
+    const int j = (j0 - 2 + k + 2 * ns) % ns;
+    const bool own = k >= 2 && k < 2 + Ls;  // lanes that write
+    FLD(vr, FCPT_VRAD);
+    FLD(vaz, FCPT_VAZI);
+    FLD(S, FCPT_SIGMA);
+    FLD(E, FCPT_ENERGY);
+    const Fld<SPLIT> vr_o = alt_vr<SPLIT>(P);
+    const Fld<SPLIT> vaz_o = alt_vaz<SPLIT>(P);
+    const double dt = P.s->dt;
+    constexpr bool sn = SN;
+    const bool diss = ADI && P.artvisc_dissipation;
+    auto load = [&](int r) {
+        KickRing w;
+        w.vr = vr(r, j);
+        if (r < nr) {
+            w.va = vaz(r, j);
+            w.s = S(r, j);
+            w.p = ADI ? fld<SPLIT>(P, FCPT_PRESSURE)(r, j) : iso_pressure(P, r, w.s);
+            w.phi = fld<SPLIT>(P, FCPT_POTENTIAL)(r, j);
+        } else {
+            w.va = w.s = w.p = w.phi = 0.0;
+        }
+        w.sL = from_left(w.s);
+        w.vaR = from_right(w.va);
+        return w;
+    };
+    // the source-term velocities of ring r (b) from ring r - 1 (a); rows
+    // that are not updated keep their values
+    auto src = [&](int r, const KickRing &a, const KickRing &b, double &v1r, double &v1a) {
+        const double pjp = from_left(b.p), phijp = from_left(b.phi);
+        const bool upd_p = r >= 1 && r < nr - 1;
+        const bool upd_r = r >= 2 && upd_p;
+        v1a = upd_p ? src_vphi_new(P, r, dt, b.va, b.s, b.sL, b.p, pjp, b.phi, phijp) : b.va;
+        v1r = upd_r ? src_vr_new(P, r, dt, b.vr, b.s, a.s, b.p, a.p, b.phi, a.phi, b.va, b.vaR, a.va, a.vaR)
+                    : b.vr;
+    };
+    // the tensor of cell (r, j) from the source-term velocities
+    struct Q {
+        TwQ tw;
+        SnQ sn;
+        double qr, qp;
+    };
+    auto qcell = [&](int r, double vr0, double vr1, double va0, double va1, double sig) {
+        Q q;
+        if constexpr (SN) {
+            q.sn = sn_q_cell(P, vr0, vr1, va0, va1, sig);
+            q.qr = q.sn.qr;
+            q.qp = q.sn.qp;
+        } else {
+            q.tw = tw_q_cell(P, r, vr0, vr1, va0, va1, sig);
+            q.qr = q.tw.qr;
+            q.qp = q.tw.qp;
+        }
+        return q;
+    };
+    KickRing R0, R1;  // rings i, i + 1
+    double v1r0 = 0.0, v1a0 = 0.0;  // source-term velocities of ring i
+    double qr_m = 0.0, qp_m = 0.0, s_m = 0.0;  // tensor and Sigma of cell (i - 1, j)
+    if (i0 >= 1) {
+        const KickRing Rm2 = load(i0 >= 2 ? i0 - 2 : 0);  // unused by ring 0
+        const KickRing Rm = load(i0 - 1);
+        R0 = load(i0);
+        double v1rm, v1am;
+        src(i0 - 1, Rm2, Rm, v1rm, v1am);
+        src(i0, Rm, R0, v1r0, v1a0);
+        const Q qm = qcell(i0 - 1, v1rm, v1r0, v1am, from_right(v1am), Rm.s);
+        qr_m = qm.qr;
+        qp_m = qm.qp;
+        s_m = Rm.s;
+    } else {
+        R0 = load(i0);
+        src(i0, R0, R0, v1r0, v1a0);  // ring 0 is passed through
+    }
+    for (int i = i0; i < i1; ++i) {
+        R1 = load(i + 1);  // ring nr holds only v_r
+        double v1r1, v1a1;
+        src(i + 1, R0, R1, v1r1, v1a1);
+        const double v1aR = from_right(v1a0);  // v_phi of sector j + 1
+        const Q q = qcell(i, v1r0, v1r1, v1a0, v1aR, R0.s);
+        const double qpj = from_left(q.qp);  // tensor of cell (i, j - 1)
+        const bool inner = i >= 1 && i < nr - 1;
+        const bool radial = i >= 2 && inner;
+        if (own) {
+            if constexpr (ADI) {
+                // compression_heating (rings 0 .. nr-2), then the dissipation
+                double e = E(i, j);
+                if (i < nr - 1) e = E.stored(i, src_energy_new(P, i, dt, e, v1r0, v1r1, v1a0, v1aR));
+                if (diss) e = sn ? sn_dissipation(P, i, q.sn, R0.s, dt, e) : tw_dissipation(P, i, q.tw, R0.s, dt, e);
+                if (i < nr - 1 || diss) {
+                    E.set(i, j, e);
+                    e = E.stored(i, e);
+                }
+                // recalculate_viscosity
+                const AdiDerived d = adi_derived(P, i, e, R0.s);
+                fld<SPLIT>(P, FCPT_SOUNDSPEED).set(i, j, d.cs);
+                fld<SPLIT>(P, FCPT_SCALE_HEIGHT).set(i, j, d.h);
+                fld<SPLIT>(P, FCPT_VISCOSITY).set(i, j, d.nu);
+            }
+            if (inner)
+                vaz_o.set(i, j, sn ? sn_vphi_new(P, i, dt, v1a0, R0.s, R0.sL, q.qp, qpj)
+                                   : tw_vphi_new(P, i, dt, v1a0, R0.s, R0.sL, q.qp, qpj));
+            else
+                vaz_o.set_raw(i, j, vaz.raw(i, j));
+            if (radial)
+                vr_o.set(i, j, sn ? sn_vr_new(P, i, dt, v1r0, R0.s, s_m, q.qr, qr_m)
+                                  : tw_vr_new(P, i, dt, v1r0, R0.s, s_m, q.qr, qr_m, q.qp, qp_m));
+            else
+                vr_o.set_raw(i, j, vr.raw(i, j));
+        }
+        qr_m = q.qr;
+        qp_m = q.qp;
+        s_m = R0.s;
+        R0 = R1;
+        v1r0 = v1r1;
+        v1a0 = v1a1;
+    }
+    if (i1 == nr && own) vr_o.set_raw(nr, j, vr.raw(nr, j));  // the outer v_r row, untouched
+}
+template <bool SPLIT, bool ADI, bool SN>
+__global__ void __launch_bounds__(256) k_kick_walk(KParams P, int tile)
+{
+    b_kick_walk<SPLIT, ADI, SN>(P, (int)blockIdx.x, (int)blockIdx.y, tile);
+}
+
 // compute_viscous_stress_tensor and update_velocities_with_viscosity
 // (viscosity.cpp:139-426) in one pass: the stress components a cell's update
 // needs -- tau_rr at (i, j), (i - 1, j); tau_pp at (i, j), (i, j - 1),
@@ -1510,20 +1711,6 @@ __device__ __forceinline__ void b_heatcool_cfl(const KParams &P, int bx, int by)
     fld<SPLIT>(P, FCPT_QMINUS).set(i, j, qminus);
 }
 FCPT_PAIR0(heatcool_cfl, 256)
-
-// Neighbour values within a wavefront: the value of lane - 1 and of lane + 1,
-// moved by the DPP wavefront shifts (wave_shr:1, wave_shl:1) on both 32-bit
-// halves; the edge lanes receive 0 and their results are discarded
-template <int CTRL>
-__device__ __forceinline__ double dpp_move(double x)
-{
-    const long long b = __double_as_longlong(x);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b & 0xffffffffll), CTRL, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
-    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
-}
-__device__ __forceinline__ double from_left(double x) { return dpp_move<0x138>(x); }
-__device__ __forceinline__ double from_right(double x) { return dpp_move<0x130>(x); }
 
 // The fused viscous pass as a radial walk.  A wavefront owns kViscSegW
 // consecutive sectors and `tile` consecutive rings: lane k holds sector
@@ -2622,6 +2809,42 @@ void launch_visc_energy(const KParams &P, hipStream_t st)
         launch_substep3(P, st);
     } else {
         launch_visc_walk(P, tile, true, st);
+    }
+}
+
+// rings per thread of the kick walk: 8, or 4, while that gives 3072
+// wavefronts, else 1 (the separate passes)
+static int kick_tile(const KParams &P)
+{
+    if (P.kick_tile > 0) return P.kick_tile;
+    const long segs = (P.ns + kKickSegW - 1) / kKickSegW;
+    int tile = 8;
+    while (tile > 1 && segs * ((P.nr + tile - 1) / tile) < 3072) tile = tile > 4 ? tile / 2 : 1;
+    return tile;
+}
+
+bool kick_walk_applies(const KParams &P)
+{
+    return (P.artvisc == FCPT_ARTVISC_TW || P.artvisc == FCPT_ARTVISC_SN) && kick_tile(P) > 1 &&
+           visc_tile(P) > 1;
+}
+
+void launch_kick(const KParams &P, hipStream_t st)
+{
+    const int tile = kick_tile(P);
+    const int tiles = (P.nr + tile - 1) / tile;
+    const dim3 g((unsigned)((P.ns + kKickSegW - 1) / kKickSegW), (unsigned)((tiles + 3) / 4));
+    launch_potential(P, st);
+    const int v = (P.split ? 4 : 0) + (P.adiabatic ? 2 : 0) + (P.artvisc == FCPT_ARTVISC_SN ? 1 : 0);
+    switch (v) {
+    case 0: hipLaunchKernelGGL((k_kick_walk<false, false, false>), g, kBlock, 0, st, P, tile); break;
+    case 1: hipLaunchKernelGGL((k_kick_walk<false, false, true>), g, kBlock, 0, st, P, tile); break;
+    case 2: hipLaunchKernelGGL((k_kick_walk<false, true, false>), g, kBlock, 0, st, P, tile); break;
+    case 3: hipLaunchKernelGGL((k_kick_walk<false, true, true>), g, kBlock, 0, st, P, tile); break;
+    case 4: hipLaunchKernelGGL((k_kick_walk<true, false, false>), g, kBlock, 0, st, P, tile); break;
+    case 5: hipLaunchKernelGGL((k_kick_walk<true, false, true>), g, kBlock, 0, st, P, tile); break;
+    case 6: hipLaunchKernelGGL((k_kick_walk<true, true, false>), g, kBlock, 0, st, P, tile); break;
+    default: hipLaunchKernelGGL((k_kick_walk<true, true, true>), g, kBlock, 0, st, P, tile); break;
     }
 }
 

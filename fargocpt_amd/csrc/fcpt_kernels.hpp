@@ -44,6 +44,12 @@ void launch_visc_fused(const KParams &P, hipStream_t st);
 // the fused viscous pass followed by SubStep3 (adiabatic disks); one walk
 // that does not store the stress tensor where the grid is large enough
 void launch_visc_energy(const KParams &P, hipStream_t st);
+// the kick's potential pass, then its source terms and artificial viscosity as
+// one walk (TW or SN artificial viscosity on a grid large enough for it),
+// velocities to the view's partner buffers; then launch_visc_energy
+// (adiabatic) or launch_visc_fused on the swapped view
+bool kick_walk_applies(const KParams &P);
+void launch_kick(const KParams &P, hipStream_t st);
 void launch_radial_sweep(const KParams &P, hipStream_t st);
 void launch_azimuthal_sweep(const KParams &P, hipStream_t st);
 void launch_cfl_cells(const KParams &P, hipStream_t st);

@@ -272,8 +272,12 @@ int fcpt_time_copy(fcpt_ctx *ctx, int32_t reps, double *ms_per_launch, double *b
    size-based default) and the sweep's register budget in wavefronts per SIMD
    (FCPT_PARAM_RADIAL_WAVES, 2 or 3; default 3), and rings walked by one
    thread of the fused viscous pass (FCPT_PARAM_VISC_TILE, 1 .. 64, where 1
-   is the cell-parallel form; 0 restores the size-based default).  Results
-   do not depend on them.  FCPT_PARAM_FUSED_STEP (default 1) makes fcpt_step_euler,
+   is the cell-parallel form; 0 restores the size-based default), and rings
+   walked by one thread of the kick walk, which runs the source terms and the
+   artificial viscosity of the fused step as one pass (FCPT_PARAM_KICK_TILE,
+   1 .. 64, where 1 runs them as separate passes; 0
+   restores the size-based default; the walk needs the TW or SN artificial
+   viscosity and a viscous tile above 1).  Results do not depend on them.  FCPT_PARAM_FUSED_STEP (default 1) makes fcpt_step_euler,
    fcpt_run_step and fcpt_run_steps run the source terms, the artificial
    viscosity and the viscous stress + velocity update as one pass each; the
    evolved fields are the same bit for bit in FP64 storage, but those steps
@@ -283,7 +287,7 @@ int fcpt_time_copy(fcpt_ctx *ctx, int32_t reps, double *ms_per_launch, double *b
    captured step graph so that the next fcpt_run_steps captures the new
    shape. */
 enum { FCPT_PARAM_RADIAL_TILE = 0, FCPT_PARAM_RADIAL_WAVES = 1, FCPT_PARAM_FUSED_STEP = 2,
-       FCPT_PARAM_VISC_TILE = 3 };
+       FCPT_PARAM_VISC_TILE = 3, FCPT_PARAM_KICK_TILE = 4 };
 int fcpt_set_launch_param(fcpt_ctx *ctx, int32_t param, int32_t value);
 /* Autotuner (no counterpart in the reference): times every launch shape of
    the tunable kernels on the context's current state (`reps` launches each;

@@ -135,6 +135,73 @@ def test_visc_walk_tiles(precision):
         _check(small(tile, True), want, precision, tile)
 
 
+KICK_FIELDS = FIELDS + ["POTENTIAL"]
+
+
+def _run_kick(meta, data, precision, kick_tile, nsteps, visc_tile=4):
+    from fargocpt_amd.capi import Disk
+    d = Disk(dict(disk_config(meta), precision=precision))
+    d.set_visc_tile(visc_tile)
+    d.set_kick_tile(kick_tile)
+    run_case(d, meta, data, nsteps=nsteps)
+    out = {f: d.download_field(f) for f in KICK_FIELDS}
+    d.close()
+    return out
+
+
+@pytest.mark.parametrize("case", ["c1_small", "c2_small", "c5_small", "mix_small", "lf_iso", "lf_adi",
+                                  "c1_iso"])
+def test_kick_walk_fixtures(case):
+    """The kick walk (sources, compression heating, artificial viscosity and
+    its viscosity recalculation in one pass, then the viscous walk) against
+    the same step with those as separate passes: every field bit for bit in FP64 storage after whole
+    fixture runs (TW and SN, isothermal and adiabatic, Euler and LeapFrog),
+    with ragged and single-tile walks."""
+    meta, data = load_case(case)
+    n = min(meta["nsteps"], 20)
+    want = _run_kick(meta, data, 0, 1, n)
+    for tile in (3, 64):
+        print(f"kick walk case={case} tile={tile} steps={n}", flush=True)
+        got = _run_kick(meta, data, 0, tile, n)
+        for f in KICK_FIELDS:
+            assert np.array_equal(got[f], want[f]), (tile, f, rel_err(got[f], want[f]))
+
+
+SPLIT_VS_FP64 = 2e-5  # relative, as for the split-storage fixture runs
+
+
+@pytest.mark.parametrize("name", ["adi_512x1536", "iso_2048x6144"])
+def test_kick_walk_workloads(name):
+    """Two steps of the benchmark disks with the size-based launch shapes
+    against the same steps with the kick as separate passes: FP64 storage
+    bit for bit; in split storage the walk keeps the source-term velocities
+    in FP64 between its stages where the separate passes store them as FP32
+    deviations, so both split runs are held to the split
+    storage's tolerance against the FP64 run."""
+    from fargocpt_amd import workloads as W
+    from fargocpt_amd.capi import Disk
+
+    def run(precision, tile):
+        print(f"kick walk {name} precision={precision} tile={tile}", flush=True)
+        d = Disk(W.disk_config(name, precision))
+        W.start(d, name)
+        d.set_kick_tile(tile)
+        d.run_step()
+        d.run_step()
+        out = {f: d.download_field(f) for f in KICK_FIELDS}
+        d.close()
+        return out
+
+    ref = run(0, 1)
+    got = run(0, 0)
+    for f in KICK_FIELDS:
+        assert np.array_equal(got[f], ref[f]), (f, rel_err(got[f], ref[f]))
+    for tile in (0, 1):
+        got = run(1, tile)
+        for f in ("SIGMA", "VRAD", "VAZI", "ENERGY", "POTENTIAL"):
+            assert rel_err(got[f], ref[f]) <= SPLIT_VS_FP64, (tile, f, rel_err(got[f], ref[f]))
+
+
 @pytest.mark.parametrize("integrator", [0, 1])
 def test_fused_graph_matches_stepwise(integrator):
     """Graph replay of the fused step (both views captured) against fused
