@@ -950,14 +950,16 @@ static const KParams *enqueue_kick(const KParams &P, const KParams &Q, hipStream
 }
 
 // One gas step of step_Euler; the transport's velocity kernel writes the
-// final velocities to the fields of view P.
+// final velocities to the fields of view P and, in the fused step, applies
+// the damping of the final boundary call that follows it.
 static void enqueue_euler(const KParams &P, hipStream_t st, int count_monitor, bool fused)
 {
     const KParams Q = swapped_view(P);
     const KParams *cur = enqueue_kick(P, Q, st, fused, false);
     launch_boundaries(*cur, st, 0, 0);
-    launch_transport_to(*cur, P, st);
-    launch_final_boundaries_derived(P, st, count_monitor ? 2 : 1);
+    if (fused) launch_transport_damped(*cur, P, st);
+    else launch_transport_to(*cur, P, st);
+    launch_final_boundaries_derived(P, st, count_monitor ? 2 : 1, fused);
 }
 
 // One gas step of step_LeapFrog (simulation.cpp:276-459): kick with dt / 2,
@@ -980,7 +982,7 @@ static void enqueue_leapfrog(const KParams &P, hipStream_t st, int count_monitor
         launch_copy_velocities(*cur, st);
     }
     launch_leapfrog_phase(P, st, 3);
-    launch_final_boundaries_derived(P, st, count_monitor ? 2 : 1);
+    launch_final_boundaries_derived(P, st, count_monitor ? 2 : 1, false);
 }
 
 static void enqueue_step(const KParams &P, hipStream_t st, int count_monitor, bool fused)

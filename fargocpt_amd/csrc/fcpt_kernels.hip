@@ -1855,8 +1855,46 @@ __global__ void __launch_bounds__(256) k_visc_walk(KParams P, int tile)
 }
 
 // ---------------------------------------------------------------------------
-// damping (boundary_conditions/damping.cpp:311-427, 754-776): block row y maps
-// to an inner-zone ring (y < n_inner) or an outer-zone ring
+// damping (boundary_conditions/damping.cpp:311-427, 754-776) of one value:
+// whether ring i of a vector (v_r) or scalar field lies in the inner or the
+// outer damping zone, and the value relaxed towards its reference x0
+__device__ __forceinline__ bool damp_in_inner(const KParams &P, bool vector, int i)
+{
+    const double *radius = vector ? P.Rinf : P.Rmed;
+    const int size_radial = vector ? P.nr + 1 : P.nr;
+    const double lin = P.damping_inner_limit;
+    if (i >= size_radial || !((lin > 1.0) && (radius[0] < P.rmin * lin))) return false;
+    return i <= (vector ? P.damp_ids[1] : P.damp_ids[0]);
+}
+__device__ __forceinline__ bool damp_in_outer(const KParams &P, bool vector, int i)
+{
+    const double *radius = vector ? P.Rinf : P.Rmed;
+    const int size_radial = vector ? P.nr + 1 : P.nr;
+    const double lout = P.damping_outer_limit;
+    if (i >= size_radial || !((lout < 1.0) && (radius[size_radial - 1] > P.rmax * lout))) return false;
+    return i >= (vector ? P.damp_ids[3] : P.damp_ids[2]);
+}
+__device__ __forceinline__ double damp_apply(const KParams &P, bool vector, bool inner, int i, double dt,
+                                             double x, double x0)
+{
+    const double *radius = vector ? P.Rinf : P.Rmed;
+    const double RMIN = P.rmin, RMAX = P.rmax;
+    double tau, a;
+    if (inner) {
+        const double lin = P.damping_inner_limit;
+        tau = P.damping_time_factor * 2.0 * M_PI / omega_kepler(P, RMIN);
+        a = (radius[i] - RMIN * lin) / (RMIN - RMIN * lin);
+    } else {
+        const double lout = P.damping_outer_limit;
+        tau = P.damping_time_factor * 2.0 * M_PI / omega_kepler(P, P.damping_time_radius_outer);
+        a = (radius[i] - RMAX * lout) / (RMAX - RMAX * lout);
+    }
+    const double exp_factor = exp(-dt * (a * a) / tau);
+    return (x - x0) * exp_factor + x0;
+}
+
+// damping: block row y maps to an inner-zone ring (y < n_inner) or an
+// outer-zone ring
 template <bool SPLIT>
 __device__ __forceinline__ void b_damping(const KParams &P, int bx, int by)
 {
@@ -1866,8 +1904,6 @@ __device__ __forceinline__ void b_damping(const KParams &P, int bx, int by)
     const int y = wave_ring(0, by);
     if (j >= ns) return;
     const double dt = P.s->dt;
-    const double lin = P.damping_inner_limit, lout = P.damping_outer_limit;
-    const double RMIN = P.rmin, RMAX = P.rmax;
     int i;
     bool inner;
     if (y < n_inner) {
@@ -1885,30 +1921,9 @@ __device__ __forceinline__ void b_damping(const KParams &P, int bx, int by)
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const bool vector = q == 0;
-        const double *radius = vector ? P.Rinf : P.Rmed;
-        const int size_radial = vector ? P.nr + 1 : P.nr;
-        if (i >= size_radial) continue;
+        if (!(inner ? damp_in_inner(P, vector, i) : damp_in_outer(P, vector, i))) continue;
         const Fld<SPLIT> X = fld<SPLIT>(P, F[q]), X0 = fld<SPLIT>(P, F0[q]);
-        if (inner) {
-            if (!((lin > 1.0) && (radius[0] < RMIN * lin))) continue;
-            const int limit = vector ? P.damp_ids[1] : P.damp_ids[0];
-            if (i > limit) continue;
-            const double tau = P.damping_time_factor * 2.0 * M_PI / omega_kepler(P, RMIN);
-            const double a = (radius[i] - RMIN * lin) / (RMIN - RMIN * lin);
-            const double exp_factor = exp(-dt * (a * a) / tau);
-            const double x = X(i, j), x0 = X0(i, j);
-            X.set(i, j, (x - x0) * exp_factor + x0);
-        } else {
-            if (!((lout < 1.0) && (radius[size_radial - 1] > RMAX * lout))) continue;
-            const int limit = vector ? P.damp_ids[3] : P.damp_ids[2];
-            if (i < limit) continue;
-            const double tau =
-                P.damping_time_factor * 2.0 * M_PI / omega_kepler(P, P.damping_time_radius_outer);
-            const double a = (radius[i] - RMAX * lout) / (RMAX - RMAX * lout);
-            const double exp_factor = exp(-dt * (a * a) / tau);
-            const double x = X(i, j), x0 = X0(i, j);
-            X.set(i, j, (x - x0) * exp_factor + x0);
-        }
+        X.set(i, j, damp_apply(P, vector, inner, i, dt, X(i, j), X0(i, j)));
     }
 }
 FCPT_PAIR0(damping, 256)
@@ -2427,6 +2442,62 @@ __device__ __forceinline__ void b_velocities(const KParams &P, int bx, int by)
 }
 FCPT_PAIR0(velocities, 256)
 
+// the velocity kernel with the damping of the final boundary call applied to
+// each value as it is stored (rows 0 .. nr; row nr holds v_r only, which
+// only the damping touches); a value is damped as a later load of it
+// returns it, so the fields are those of the velocity kernel followed by the
+// damping kernel
+template <bool SPLIT>
+__device__ __forceinline__ double damp_stored(const KParams &P, const Fld<SPLIT> &X, int f0, bool vector,
+                                              int i, int j, double dt, double x)
+{
+    const bool in = damp_in_inner(P, vector, i), out = damp_in_outer(P, vector, i);
+    if (!in && !out) return x;
+    x = X.stored(i, x);
+    const double x0 = fld<SPLIT>(P, f0)(i, j);
+    if (in) x = damp_apply(P, vector, true, i, dt, x, x0);
+    if (out) x = damp_apply(P, vector, false, i, dt, x, x0);
+    return x;
+}
+
+template <bool SPLIT>
+__global__ void __launch_bounds__(256) k_velocities_damped(KParams P)
+{
+    const int bx = (int)blockIdx.x, by = (int)blockIdx.y;
+    CELL_LOOP_HEAD(0, P.nr + 1)
+    FLD(vr, FCPT_VRAD);
+    const double dt = P.s->dt;
+    if (i == P.nr) {
+        if (damp_in_outer(P, true, i) || damp_in_inner(P, true, i))
+            vr.set(i, j, damp_stored<SPLIT>(P, vr, FCPT_VRAD0, true, i, j, dt, vr(i, j)));
+        return;
+    }
+    const Fld<false> Sb = buf<false>(P, P.sig_c, nullptr);
+    const Fld<SPLIT> m0 = buf<SPLIT>(P, P.mom[0], P.bgm[0]);
+    const Fld<SPLIT> m1 = buf<SPLIT>(P, P.mom[1], P.bgm[1]);
+    const Fld<SPLIT> m2 = buf<SPLIT>(P, P.mom[2], P.bgm[2]);
+    const Fld<SPLIT> m3 = buf<SPLIT>(P, P.mom[3], P.bgm[3]);
+    const Fld<false> Eb = buf<false>(P, P.en_c, nullptr);
+    FLD(vaz, FCPT_VAZI);
+    FLD(S, FCPT_SIGMA);
+    FLD(E, FCPT_ENERGY);
+    const int jm = j == 0 ? ns - 1 : j - 1;
+    const int im = i >= 1 ? i - 1 : 0;
+    const double s0 = Sb(i, j), sm = Sb(im, j), sjm = Sb(i, jm);
+    const double m0v = m0(im, j), m1v = m1(i, j), m2v = m2(i, jm), m3v = m3(i, j);
+    const double e = P.adiabatic ? Eb(i, j) : 0.0;
+    const double v_r = i == 0 ? 0.0 : (m0v + m1v) / (sm + s0);
+    vr.set(i, j, damp_stored<SPLIT>(P, vr, FCPT_VRAD0, true, i, j, dt, v_r));
+    const double v_phi = (m2v + m3v) / (sjm + s0) * cst(P.InvRmed)[i] - cst(P.Rmed)[i] * P.omega_frame;
+    vaz.set(i, j, damp_stored<SPLIT>(P, vaz, FCPT_VAZI0, false, i, j, dt, v_phi));
+    const double floor_value = P.sigma_floor * P.sigma0;
+    double sig = s0;
+    if (sig < floor_value) sig = floor_value;
+    S.set(i, j, damp_stored<SPLIT>(P, S, FCPT_SIGMA0, false, i, j, dt, sig));
+    if (P.adiabatic)
+        E.set(i, j, damp_stored<SPLIT>(P, E, FCPT_ENERGY0, false, i, j, dt, temperature_floor(P, e, sig)));
+}
+
 // ---------------------------------------------------------------------------
 // totals with Neumaier compensation: per ring (sum, compensation) pairs, then
 // one block folds the rings in order (quantities.cpp:51-78, 242-279)
@@ -2747,6 +2818,18 @@ void launch_transport_to(const KParams &P, const KParams &out, hipStream_t st)
     LAUNCH(k_velocities, cell_grid(out, out.nr), kBlock, st, out);
 }
 
+void launch_transport_damped(const KParams &P, const KParams &out, hipStream_t st)
+{
+    if (!out.damping) {
+        launch_transport_to(P, out, st);
+        return;
+    }
+    launch_radial_sweep(P, st);
+    launch_ring_mean(P, st, 1);
+    launch_azimuthal_sweep(P, st);
+    LAUNCH(k_velocities_damped, cell_grid(out, out.nr + 1), kBlock, st, out);
+}
+
 KParams swapped_view(const KParams &P)
 {
     KParams Q = P;
@@ -2852,9 +2935,9 @@ void launch_derived(const KParams &P, hipStream_t st) { LAUNCH(k_derived, cell_g
 
 // the final boundary call (damping, ghost rings, the clock) and the derived
 // fields: damping, then one pass for the rest
-void launch_final_boundaries_derived(const KParams &P, hipStream_t st, int advance_time)
+void launch_final_boundaries_derived(const KParams &P, hipStream_t st, int advance_time, bool damped)
 {
-    if (P.damping) {
+    if (P.damping && !damped) {
         const int n_inner = (P.damp_ids[0] > P.damp_ids[1] ? P.damp_ids[0] : P.damp_ids[1]) + 1;
         const int o0 = P.damp_ids[2] < P.damp_ids[3] ? P.damp_ids[2] : P.damp_ids[3];
         const int n_outer = P.nr + 1 - o0;

@@ -29,11 +29,15 @@ void launch_transport(const KParams &P, hipStream_t st);
 void launch_set_mid_params(const KParams &P, hipStream_t st, const fcpt_step_params &sp);
 void launch_leapfrog_phase(const KParams &P, hipStream_t st, int phase);
 void launch_pressure(const KParams &P, hipStream_t st);
-void launch_final_boundaries_derived(const KParams &P, hipStream_t st, int advance_time);
+// damped: the damping was applied by launch_transport_damped
+void launch_final_boundaries_derived(const KParams &P, hipStream_t st, int advance_time, bool damped);
 void launch_copy_velocities(const KParams &P, hipStream_t st);
 void launch_fill_rows(const KParams &P, hipStream_t st, void *dst, const double *ring,
                       const double *bg, int rows);
 void launch_transport_to(const KParams &P, const KParams &out, hipStream_t st);
+// the transport with the final boundary call's damping applied by its
+// velocity kernel (the fields those two kernels in turn give)
+void launch_transport_damped(const KParams &P, const KParams &out, hipStream_t st);
 // the fused step kernels write the velocities of view P to P.valt; the
 // swapped view reads those buffers and writes the fields
 KParams swapped_view(const KParams &P);
