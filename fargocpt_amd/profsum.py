@@ -19,14 +19,17 @@ import csv
 import glob
 import json
 import os
+import re
 
-# bench.py times these kernels with fcpt_time_kernel: 3 rounds of 1 warm-up + 20 launches
-TIMED = ("k_radial_sweep", "k_azimuthal_sweep", "k_cfl_cells", "k_visc_walk", "k_potential<")
+# bench.py times these kernels with fcpt_time_kernel: 3 rounds of 1 warm-up + 20
+# launches (the viscous pass timed is the walk without SubStep3)
+TIMED = (r"k_radial_sweep", r"k_azimuthal_sweep", r"k_cfl_cells", r"k_visc_walk<(true|false), false>",
+         r"k_potential<")
 TIMING_LAUNCHES = 63
 KT_STEPS = 55
-JSON_NAMES = {"k_radial_sweep": "radial_sweep", "k_azimuthal_sweep": "azimuthal_sweep",
-              "k_cfl_cells": "cfl", "k_visc_walk": "visc_fused", "k_potential<": "potential",
-              "k_disk_force<": "disk_force"}
+JSON_NAMES = {r"k_radial_sweep": "radial_sweep", r"k_azimuthal_sweep": "azimuthal_sweep",
+              r"k_cfl_cells": "cfl", r"k_visc_walk<(true|false), false>": "visc_fused",
+              r"k_potential<": "potential", r"k_disk_force<": "disk_force"}
 
 
 def _short(name: str) -> str:
@@ -72,7 +75,7 @@ def summarise(workload: str, prec: str, base: str, outdir: str) -> dict:
     for r in sorted(rows, key=lambda r: -float(r["TotalDurationNs"])):
         name = r["Name"]
         calls = int(r["Calls"])
-        extra = TIMING_LAUNCHES if any(t in name for t in TIMED) else 0
+        extra = TIMING_LAUNCHES if any(re.search(t, name) for t in TIMED) else 0
         n = (calls - extra) // KT_STEPS
         per_step[_short(name)] = n
         avg_us[_short(name)] = float(r["AverageNs"]) / 1e3
@@ -114,7 +117,7 @@ def summarise(workload: str, prec: str, base: str, outdir: str) -> dict:
     tj = {}
     for s, b in traffic.items():
         for kk, v in JSON_NAMES.items():
-            if kk in s:
+            if re.search(kk, s):
                 tj[v] = b
     tj["step"] = step_bytes
     tj["step_kernel_us"] = step_us
