@@ -267,23 +267,29 @@ int fcpt_time_kernel(fcpt_ctx *ctx, int32_t kernel, int32_t reps, double *ms_per
    only. */
 int fcpt_time_copy(fcpt_ctx *ctx, int32_t reps, double *ms_per_launch, double *bytes_per_launch);
 
-/* Launch-shape tuning (no counterpart in the reference): rings walked by one
-   thread of the radial sweep (FCPT_PARAM_RADIAL_TILE, 1 .. 64; 0 restores the
-   size-based default) and the sweep's register budget in wavefronts per SIMD
-   (FCPT_PARAM_RADIAL_WAVES, 2 or 3; default 3), and rings walked by one
-   thread of the fused viscous pass (FCPT_PARAM_VISC_TILE, 1 .. 64, where 1
-   is the cell-parallel form; 0 restores the size-based default), and rings
-   walked by one thread of the kick walk, which runs the source terms and the
-   artificial viscosity of the fused step as one pass (FCPT_PARAM_KICK_TILE,
-   1 .. 64, where 1 runs them as separate passes; 0
-   restores the size-based default; the walk needs the TW or SN artificial
-   viscosity and a viscous tile above 1).  Results do not depend on them.  FCPT_PARAM_FUSED_STEP (default 1) makes fcpt_step_euler,
-   fcpt_run_step and fcpt_run_steps run the source terms, the artificial
-   viscosity and the viscous stress + velocity update as one pass each; the
-   evolved fields are the same bit for bit in FP64 storage, but those steps
-   then leave Q_R and Q_PHI, and (unless the energy equation's viscous heating
-   reads them) TAU_* and DIV_V, as they were; 0 runs the separate kernels of
-   the per-function entry points above, which always store them.  Drops a
+/* Launch-shape tuning (no counterpart in the reference).  Results do not
+   depend on the shapes:
+   FCPT_PARAM_RADIAL_TILE   rings walked by one thread of the radial sweep
+                            (1 .. 64; 0 restores the size-based default);
+   FCPT_PARAM_RADIAL_WAVES  the sweep's register budget in wavefronts per
+                            SIMD (2 or 3; default 3);
+   FCPT_PARAM_VISC_TILE     rings walked by one thread of the fused viscous
+                            pass (1 .. 64, where 1 is the cell-parallel form;
+                            0 restores the size-based default);
+   FCPT_PARAM_KICK_TILE     rings walked by one thread of the kick walk, which
+                            runs the source terms and the artificial
+                            viscosity of the fused step as one pass (1 .. 64,
+                            where 1 runs them as separate passes; 0 restores
+                            the size-based default; the walk needs the TW or
+                            SN artificial viscosity and a viscous tile above 1).
+   FCPT_PARAM_FUSED_STEP (default 1) makes fcpt_step_euler, fcpt_step_leapfrog,
+   fcpt_run_step and fcpt_run_steps run the fused passes (the walks above,
+   the damping inside the transport's velocity kernel); the evolved and
+   derived fields are the same bit for bit in FP64 storage, but those steps
+   leave Q_R, Q_PHI, TAU_* and DIV_V unspecified (as they were, or, for an
+   adiabatic disk with viscous heating on a grid below the walks' size, the
+   step's tensor); 0 runs the separate kernels of the per-function entry
+   points above, which always store them.  Every parameter change drops a
    captured step graph so that the next fcpt_run_steps captures the new
    shape. */
 enum { FCPT_PARAM_RADIAL_TILE = 0, FCPT_PARAM_RADIAL_WAVES = 1, FCPT_PARAM_FUSED_STEP = 2,
