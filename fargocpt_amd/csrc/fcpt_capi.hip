@@ -389,7 +389,8 @@ int fcpt_create(const fcpt_disk_desc *desc, fcpt_ctx **out)
     if (!rc) rc = dev_alloc(c, &P.vmean, (size_t)nr + 1);
     if (!rc) rc = dev_alloc(c, &P.vuni, (size_t)nr + 1);
     if (!rc) rc = dev_alloc(c, &P.nshift, (size_t)nr + 1);
-    if (!rc) rc = dev_alloc(c, &P.ring_min, (size_t)kCflMaxBlocks);
+    // CFL slots: per block of k_cfl_cells (batches) or per ring of k_cfl_rings
+    if (!rc) rc = dev_alloc(c, &P.ring_min, (size_t)(nr > (int)kCflMaxBlocks ? nr : (int)kCflMaxBlocks));
     if (!rc) rc = dev_alloc(c, &c->s_dev, 1);
     if (!rc) rc = dev_alloc(c, &c->totals_ring, 4 * (size_t)nr);
     if (!rc) rc = dev_alloc(c, &c->totals_out, 2);

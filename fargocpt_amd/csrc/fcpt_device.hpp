@@ -96,7 +96,7 @@ struct KParams {
     double *vmean;       // per-ring mean azimuthal velocity
     double *vuni;        // per-ring uniform residual velocity (FARGO)
     int *nshift;         // per-ring integer shift
-    double *ring_min;    // per-block CFL minima (kCflMaxBlocks slots)
+    double *ring_min;    // CFL minima: per block (kCflMaxBlocks slots) or per ring (nr)
     DevScalars *s;
 };
 

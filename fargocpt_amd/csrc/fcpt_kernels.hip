@@ -270,12 +270,24 @@ FCPT_PAIR1(ring_mean, 256, int, mode)
 // 2: CalculateTimeStep + the monitor clip of sim::run (simulation.cpp:526-542).
 // One block of 256 threads (thread index t) folds the block minima in a fixed
 // pairwise order.
-__device__ __forceinline__ void b_cfl_fold(const KParams &P, int mode, int t)
+// With `rings` the slots are the per-ring minima of k_cfl_rings (nr of them)
+// and the fold adds the per-ring shear limits from the ring means.
+__device__ __forceinline__ double cfl_shear(const KParams &P, int k)
 {
-    const int nblocks = P.cfl_gx * P.cfl_gy;
+    // the limit of the shear between rings k and k + 1 (cfl.cpp:223-233)
+    const double *vm = P.vmean;
+    const double denom = fabs(vm[k] * P.InvRmed[k] - vm[k + 1] * P.InvRmed[k + 1]) + 1.0e-100;
+    return P.cfl * P.dphi / denom;
+}
+
+__device__ __forceinline__ void b_cfl_fold(const KParams &P, int mode, int t, bool rings = false)
+{
+    const int nblocks = rings ? P.nr : P.cfl_gx * P.cfl_gy;
     __shared__ double sh[256];
     double m = __longlong_as_double(0x7FF0000000000000ll);
     for (int k = t; k < nblocks; k += 256) m = std_min(m, P.ring_min[k]);
+    if (rings)
+        for (int k = t; k < P.nr - 1; k += 256) m = std_min(m, cfl_shear(P, k));
     sh[t] = m;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
@@ -303,6 +315,71 @@ __device__ __forceinline__ void b_cfl_fold(const KParams &P, int mode, int t)
 }
 __device__ __forceinline__ void b_cfl_finalize(const KParams &P, int mode) { b_cfl_fold(P, mode, (int)threadIdx.x); }
 
+// the CFL limit of cell (i, j) (cfl.cpp:235-376), vm_i the ring's mean v_phi
+template <bool SPLIT>
+__device__ __forceinline__ double cfl_cell(const KParams &P, int i, int j, double vm_i)
+{
+    const int ns = P.ns;
+    FLD(vr, FCPT_VRAD);
+    FLD(vaz, FCPT_VAZI);
+    FLD(cs, FCPT_SOUNDSPEED);
+    FLD(nu, FCPT_VISCOSITY);
+    const bool iso = !P.adiabatic;  // c_s and nu are ring profiles
+    const double CFL = P.cfl, dphi = P.dphi;
+    const double fac2 = P.artvisc_factor * P.artvisc_factor;
+    const double dxRadial = cst(P.Rsup)[i] - cst(P.Rinf)[i];
+    const double dxAzimuthal = cst(P.Rmed)[i] * dphi;
+    const double cell_size = std_min(dxRadial, dxAzimuthal);
+    const int jn = j == ns - 1 ? 0 : j + 1;
+    const double vr0 = vr(i, j), vr1 = vr(i + 1, j), va0 = vaz(i, j), va1 = vaz(i, jn);
+    const double v_residual = va0 - vm_i;
+    // split storage: divisions by the ring's cell sizes become products
+    const double inv_dr = SPLIT ? cst(P.InvDiffRsup)[i] : 0.0;
+    const double inv_daz = SPLIT ? recip<true>(dxAzimuthal) : 0.0;
+    const double inv_cell = SPLIT ? recip<true>(cell_size) : 0.0;
+    const double csv = iso ? cst(P.iso_cs)[i] : cs(i, j);
+    const double nuv = iso ? cst(P.iso_nu)[i] : nu(i, j);
+    const double invdt1 = SPLIT ? csv * inv_cell : csv / cell_size;
+    const double invdt2 = SPLIT ? vr0 * inv_dr : vr0 / dxRadial;
+    const double invdt3 = SPLIT ? v_residual * inv_daz : v_residual / dxAzimuthal;
+    // the LeapFrog scheme applies the source terms twice with dt / 2 and
+    // relaxes their limits (cfl.cpp:254-261); the Euler factor 1.0 is exact
+    const double lf = P.integrator == FCPT_INTEGRATOR_LEAPFROG ? 0.6 : 1.0;
+    double invdt4;
+    if (P.artvisc == FCPT_ARTVISC_SN) {
+        double dvRadial = vr1 - vr0;
+        double dvAzimuthal = va1 - va0;
+        if (dvRadial > 0.0) dvRadial = 0.0;
+        else dvRadial = -dvRadial;
+        if (dvAzimuthal > 0.0) dvAzimuthal = 0.0;
+        else dvAzimuthal = -dvAzimuthal;
+        if constexpr (SPLIT)
+            invdt4 = 4.0 * fac2 * std_max(dvRadial * inv_dr, dvAzimuthal * inv_daz) * lf;
+        else
+            invdt4 = 4.0 * fac2 * std_max(dvRadial / dxRadial, dvAzimuthal / dxAzimuthal) * lf;
+    } else {
+        const double eps_rr = (vr1 - vr0) * cst(P.InvDiffRsup)[i];
+        const double eps_pp = cst(P.InvRmed)[i] * ((va1 - va0) * P.invdphi + 0.5 * (vr1 + vr0));
+        const double mdiv_V = -std_min(eps_rr + eps_pp, 0.0);
+        invdt4 = 4.0 * fac2 * mdiv_V * lf;
+    }
+    const double invdt5 = SPLIT ? 4.0 * nuv * (inv_cell * inv_cell) * lf
+                                : 4.0 * nuv / (cell_size * cell_size) * lf;
+    double invdt6 = 0.0;
+    if (P.adiabatic) {
+        FLD(qp, FCPT_QPLUS);
+        FLD(qm, FCPT_QMINUS);
+        FLD(E, FCPT_ENERGY);
+        const double inv_limit = 1.0 / P.heating_cooling_cfl_limit;
+        if constexpr (SPLIT) invdt6 = inv_limit * fabs((qp(i, j) - qm(i, j)) * recip<true>(E(i, j))) * lf;
+        else invdt6 = inv_limit * fabs((qp(i, j) - qm(i, j)) / E(i, j)) * lf;
+    }
+    const double sum2 = invdt1 * invdt1 + invdt2 * invdt2 + invdt3 * invdt3 + invdt4 * invdt4 +
+                        invdt5 * invdt5 + invdt6 * invdt6;
+    const double dt_cell = SPLIT ? CFL * inv_sqrt<true>(sum2) : CFL / sqrt(sum2);
+    return dt_cell;
+}
+
 // per-cell CFL limits (cfl.cpp:185-382) and this block's minimum in its slot
 // (k_cfl_finalize folds the slots: a device-wide fold inside this kernel
 // would need a device-scope release per block, which on gfx950 writes back
@@ -313,77 +390,14 @@ __device__ __forceinline__ void b_cfl_cells(const KParams &P, int bx, int by)
     const int ns = P.ns;
     const int j = bx * blockDim.x + threadIdx.x;
     double cand = __longlong_as_double(0x7FF0000000000000ll);
-    FLD(vr, FCPT_VRAD);
-    FLD(vaz, FCPT_VAZI);
-    FLD(cs, FCPT_SOUNDSPEED);
-    FLD(nu, FCPT_VISCOSITY);
-    const bool iso = !P.adiabatic;  // c_s and nu are ring profiles
-    const double CFL = P.cfl, dphi = P.dphi;
     const auto vm = cst(P.vmean);
-    const double fac2 = P.artvisc_factor * P.artvisc_factor;
     for (int i = wave_ring(1, by); i < P.nr - 1 && j < ns; i += P.cfl_gy * (int)blockDim.y) {
         if (j == 0) {
-            const double denom =
-                fabs(vm[i] * cst(P.InvRmed)[i] - vm[i + 1] * cst(P.InvRmed)[i + 1]) + 1.0e-100;
-            cand = std_min(cand, CFL * dphi / denom);
-            if (i == 1) {
-                const double denom0 =
-                    fabs(vm[0] * cst(P.InvRmed)[0] - vm[1] * cst(P.InvRmed)[1]) + 1.0e-100;
-                cand = std_min(cand, CFL * dphi / denom0);
-            }
+            cand = std_min(cand, cfl_shear(P, i));
+            if (i == 1) cand = std_min(cand, cfl_shear(P, 0));
         }
-        const double dxRadial = cst(P.Rsup)[i] - cst(P.Rinf)[i];
-        const double dxAzimuthal = cst(P.Rmed)[i] * dphi;
-        const double cell_size = std_min(dxRadial, dxAzimuthal);
-        const int jn = j == ns - 1 ? 0 : j + 1;
-        const double vr0 = vr(i, j), vr1 = vr(i + 1, j), va0 = vaz(i, j), va1 = vaz(i, jn);
-        const double v_residual = va0 - vm[i];
-        // split storage: divisions by the ring's cell sizes become products
-        const double inv_dr = SPLIT ? cst(P.InvDiffRsup)[i] : 0.0;
-        const double inv_daz = SPLIT ? recip<true>(dxAzimuthal) : 0.0;
-        const double inv_cell = SPLIT ? recip<true>(cell_size) : 0.0;
-        const double csv = iso ? cst(P.iso_cs)[i] : cs(i, j);
-        const double nuv = iso ? cst(P.iso_nu)[i] : nu(i, j);
-        const double invdt1 = SPLIT ? csv * inv_cell : csv / cell_size;
-        const double invdt2 = SPLIT ? vr0 * inv_dr : vr0 / dxRadial;
-        const double invdt3 = SPLIT ? v_residual * inv_daz : v_residual / dxAzimuthal;
-        // the LeapFrog scheme applies the source terms twice with dt / 2 and
-        // relaxes their limits (cfl.cpp:254-261); the Euler factor 1.0 is exact
-        const double lf = P.integrator == FCPT_INTEGRATOR_LEAPFROG ? 0.6 : 1.0;
-        double invdt4;
-        if (P.artvisc == FCPT_ARTVISC_SN) {
-            double dvRadial = vr1 - vr0;
-            double dvAzimuthal = va1 - va0;
-            if (dvRadial > 0.0) dvRadial = 0.0;
-            else dvRadial = -dvRadial;
-            if (dvAzimuthal > 0.0) dvAzimuthal = 0.0;
-            else dvAzimuthal = -dvAzimuthal;
-            if constexpr (SPLIT)
-                invdt4 = 4.0 * fac2 * std_max(dvRadial * inv_dr, dvAzimuthal * inv_daz) * lf;
-            else
-                invdt4 = 4.0 * fac2 * std_max(dvRadial / dxRadial, dvAzimuthal / dxAzimuthal) * lf;
-        } else {
-            const double eps_rr = (vr1 - vr0) * cst(P.InvDiffRsup)[i];
-            const double eps_pp = cst(P.InvRmed)[i] * ((va1 - va0) * P.invdphi + 0.5 * (vr1 + vr0));
-            const double mdiv_V = -std_min(eps_rr + eps_pp, 0.0);
-            invdt4 = 4.0 * fac2 * mdiv_V * lf;
-        }
-        const double invdt5 = SPLIT ? 4.0 * nuv * (inv_cell * inv_cell) * lf
-                                    : 4.0 * nuv / (cell_size * cell_size) * lf;
-        double invdt6 = 0.0;
-        if (P.adiabatic) {
-            FLD(qp, FCPT_QPLUS);
-            FLD(qm, FCPT_QMINUS);
-            FLD(E, FCPT_ENERGY);
-            const double inv_limit = 1.0 / P.heating_cooling_cfl_limit;
-            if constexpr (SPLIT) invdt6 = inv_limit * fabs((qp(i, j) - qm(i, j)) * recip<true>(E(i, j))) * lf;
-            else invdt6 = inv_limit * fabs((qp(i, j) - qm(i, j)) / E(i, j)) * lf;
-        }
-        const double sum2 = invdt1 * invdt1 + invdt2 * invdt2 + invdt3 * invdt3 + invdt4 * invdt4 +
-                            invdt5 * invdt5 + invdt6 * invdt6;
-        const double dt_cell = SPLIT ? CFL * inv_sqrt<true>(sum2) : CFL / sqrt(sum2);
         // NaN never wins a comparison, exactly like the reference's "if (dt_cell < dt_core)"
-        cand = std_min(cand, dt_cell);
+        cand = std_min(cand, cfl_cell<SPLIT>(P, i, j, vm[i]));
     }
     for (int off = 32; off > 0; off >>= 1) cand = std_min(cand, __shfl_xor(cand, off, 64));
     __shared__ double wmin[4];
@@ -397,7 +411,40 @@ __device__ __forceinline__ void b_cfl_cells(const KParams &P, int bx, int by)
 }
 FCPT_PAIR0(cfl_cells, 256)
 
+// The CFL ring mean and cell limits in one launch, one block of 256 threads
+// per ring: the block forms the ring's mean v_phi exactly as k_ring_mean does
+// (same per-thread sums, same tree) and stores it, then the limits of the
+// ring's cells (rings 1 .. nr-2) with that mean, and puts the block minimum in
+// the ring's slot; the fold (rings = true) adds the shear limits between
+// neighbouring rings' means.  The minimum over the same values as
+// k_ring_mean + k_cfl_cells form.
+template <bool SPLIT>
+__global__ void __launch_bounds__(256) k_cfl_rings(KParams P)
+{
+    __shared__ double sh[256];
+    __shared__ double wmin[4];
+    const int ns = P.ns;
+    const int i = (int)blockIdx.x;
+    const int t = (int)threadIdx.x;
+    FLD(v, FCPT_VAZI);
+    double acc = 0.0;
+    for (int j = t; j < ns; j += 256) acc += v(i, j);
+    const double vm = block_sum<256>(acc, sh) / (double)ns;
+    if (t == 0) P.vmean[i] = vm;
+    double cand = __longlong_as_double(0x7FF0000000000000ll);
+    if (i >= 1 && i < P.nr - 1)
+        for (int j = t; j < ns; j += 256) cand = std_min(cand, cfl_cell<SPLIT>(P, i, j, vm));
+    for (int off = 32; off > 0; off >>= 1) cand = std_min(cand, __shfl_xor(cand, off, 64));
+    if ((t & 63) == 0) wmin[t >> 6] = cand;
+    __syncthreads();
+    if (t == 0) P.ring_min[i] = std_min(std_min(wmin[0], wmin[1]), std_min(wmin[2], wmin[3]));
+}
+
 __global__ void __launch_bounds__(256) k_cfl_finalize(KParams P, int mode) { b_cfl_finalize(P, mode); }
+__global__ void __launch_bounds__(256) k_cfl_finalize_rings(KParams P, int mode)
+{
+    b_cfl_fold(P, mode, (int)threadIdx.x, true);
+}
 __global__ void __launch_bounds__(256) kb_cfl_finalize(const KParams *Ps, const BatchEntry *map, int mode)
 {
     b_cfl_finalize(batch_disk(Ps, batch_entry(map).disk), mode);
@@ -2627,15 +2674,13 @@ void init_launch_fields(KParams &P)
 
 void launch_cfl(const KParams &P, hipStream_t st, int mode)
 {
-    launch_ring_mean(P, st, 0);
-    const dim3 g((unsigned)P.cfl_gx, (unsigned)P.cfl_gy);
-    LAUNCH(k_cfl_cells, g, kBlock, st, P);
-    hipLaunchKernelGGL(k_cfl_finalize, dim3(1), dim3(256), 0, st, P, mode);
+    LAUNCH(k_cfl_rings, dim3((unsigned)P.nr), dim3(256), st, P);
+    hipLaunchKernelGGL(k_cfl_finalize_rings, dim3(1), dim3(256), 0, st, P, mode);
 }
 
 void launch_cfl_cells(const KParams &P, hipStream_t st)
 {
-    LAUNCH(k_cfl_cells, dim3((unsigned)P.cfl_gx, (unsigned)P.cfl_gy), kBlock, st, P);
+    LAUNCH(k_cfl_rings, dim3((unsigned)P.nr), dim3(256), st, P);
 }
 
 void launch_set_dt(const KParams &P, hipStream_t st, double dt)
