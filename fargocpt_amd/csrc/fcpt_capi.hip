@@ -957,9 +957,12 @@ static void enqueue_euler(const KParams &P, hipStream_t st, int count_monitor, b
 {
     const KParams Q = swapped_view(P);
     const KParams *cur = enqueue_kick(P, Q, st, fused, false);
-    launch_boundaries(*cur, st, 0, 0);
-    if (fused) launch_transport_damped(*cur, P, st);
-    else launch_transport_to(*cur, P, st);
+    if (fused) {
+        launch_transport_ghost(*cur, P, st, true);
+    } else {
+        launch_boundaries(*cur, st, 0, 0);
+        launch_transport_to(*cur, P, st);
+    }
     launch_final_boundaries_derived(P, st, count_monitor ? 2 : 1, fused);
 }
 
@@ -974,8 +977,12 @@ static void enqueue_leapfrog(const KParams &P, hipStream_t st, int count_monitor
     launch_leapfrog_phase(P, st, 0);
     const KParams *cur = enqueue_kick(P, Q, st, fused, false);
     launch_leapfrog_phase(P, st, 1);
-    launch_boundaries(*cur, st, 0, 0);
-    launch_transport_to(*cur, P, st);
+    if (fused) {
+        launch_transport_ghost(*cur, P, st, false);
+    } else {
+        launch_boundaries(*cur, st, 0, 0);
+        launch_transport_to(*cur, P, st);
+    }
     launch_leapfrog_phase(P, st, 2);
     cur = enqueue_kick(P, Q, st, fused, true);
     if (cur != &P) {

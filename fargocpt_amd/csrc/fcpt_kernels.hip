@@ -2060,6 +2060,24 @@ __device__ __forceinline__ void b_ghost_bc(const KParams &P, int bx, int by, int
 }
 FCPT_PAIR1(ghost_bc, 256, int, advance_time)
 
+// the boundary call before the transport (ghost rings, b_ghost_bc) and the
+// transport's ring mean with the FARGO shift (b_ring_mean, mode 1) in one
+// launch, one block of 256 threads per ring: blocks 0 and nr-1 first set
+// their ghost ring, each thread the sectors whose v_phi it then sums, so
+// every block sums what the two separate kernels would have it sum.  Only the
+// azimuthal sweep reads the ring means, so they may be formed before the
+// radial sweep.
+template <bool SPLIT>
+__global__ void __launch_bounds__(256) k_ghost_ring_mean(KParams P)
+{
+    const int i = (int)blockIdx.x;
+    if (i == 0)
+        for (int j = (int)threadIdx.x; j < P.ns; j += 256) ghost_inner<SPLIT>(P, j);
+    if (i == P.nr - 1)
+        for (int j = (int)threadIdx.x; j < P.ns; j += 256) ghost_outer<SPLIT>(P, j);
+    b_ring_mean<SPLIT>(P, i, 0, 1);
+}
+
 // the final boundary call's ghost rings (b_ghost_bc, after the damping) and
 // recalculate_derived_disk_quantities in one pass: a ghost-ring cell first
 // takes its neighbour ring's values, then forms its derived fields from what
@@ -2863,16 +2881,13 @@ void launch_transport_to(const KParams &P, const KParams &out, hipStream_t st)
     LAUNCH(k_velocities, cell_grid(out, out.nr), kBlock, st, out);
 }
 
-void launch_transport_damped(const KParams &P, const KParams &out, hipStream_t st)
+void launch_transport_ghost(const KParams &P, const KParams &out, hipStream_t st, bool damp)
 {
-    if (!out.damping) {
-        launch_transport_to(P, out, st);
-        return;
-    }
+    LAUNCH(k_ghost_ring_mean, dim3((unsigned)P.nr), dim3(256), st, P);
     launch_radial_sweep(P, st);
-    launch_ring_mean(P, st, 1);
     launch_azimuthal_sweep(P, st);
-    LAUNCH(k_velocities_damped, cell_grid(out, out.nr + 1), kBlock, st, out);
+    if (damp && out.damping) LAUNCH(k_velocities_damped, cell_grid(out, out.nr + 1), kBlock, st, out);
+    else LAUNCH(k_velocities, cell_grid(out, out.nr), kBlock, st, out);
 }
 
 KParams swapped_view(const KParams &P)

@@ -29,15 +29,16 @@ void launch_transport(const KParams &P, hipStream_t st);
 void launch_set_mid_params(const KParams &P, hipStream_t st, const fcpt_step_params &sp);
 void launch_leapfrog_phase(const KParams &P, hipStream_t st, int phase);
 void launch_pressure(const KParams &P, hipStream_t st);
-// damped: the damping was applied by launch_transport_damped
+// damped: the damping was applied by the transport (launch_transport_ghost)
 void launch_final_boundaries_derived(const KParams &P, hipStream_t st, int advance_time, bool damped);
 void launch_copy_velocities(const KParams &P, hipStream_t st);
 void launch_fill_rows(const KParams &P, hipStream_t st, void *dst, const double *ring,
                       const double *bg, int rows);
 void launch_transport_to(const KParams &P, const KParams &out, hipStream_t st);
-// the transport with the final boundary call's damping applied by its
-// velocity kernel (the fields those two kernels in turn give)
-void launch_transport_damped(const KParams &P, const KParams &out, hipStream_t st);
+// the boundary call before the transport (ghost rings) and the transport, the
+// ghost rings and the FARGO ring mean in one launch; damp: the final boundary
+// call's damping applied by the velocity kernel
+void launch_transport_ghost(const KParams &P, const KParams &out, hipStream_t st, bool damp);
 // the fused step kernels write the velocities of view P to P.valt; the
 // swapped view reads those buffers and writes the fields
 KParams swapped_view(const KParams &P);
