@@ -907,20 +907,22 @@ int fcpt_get_nshift(fcpt_ctx *ctx, int32_t *nshift)
 // artificial viscosity and the viscous stress + update as one pass each; each
 // pass writes the velocities to the partner buffers, so the passes alternate
 // between the two views of the state (P and its swapped view).  Returns the
-// view that holds the velocities afterwards.
+// view that holds the velocities afterwards.  fold_mode >= 0 (fused only):
+// the potential launch also folds the CFL minima of launch_cfl_cells.
 static const KParams *enqueue_kick(const KParams &P, const KParams &Q, hipStream_t st, bool fused,
-                                   bool pressure_first)
+                                   bool pressure_first, int fold_mode = -1)
 {
     if (fused && kick_walk_applies(P)) {
         // the potential pass and two walks: sources and artificial
         // viscosity; then the viscous stress and update (and SubStep3)
         if (pressure_first && P.adiabatic) launch_pressure(P, st);
-        launch_kick(P, st);
+        launch_kick(P, st, fold_mode);
         if (P.adiabatic) launch_visc_energy(Q, st);
         else launch_visc_fused(Q, st);
         return &P;
     }
-    launch_potential(P, st);
+    if (fused) launch_potential_fold(P, st, fold_mode);
+    else launch_potential(P, st);
     if (!fused) {
         if (pressure_first) launch_pressure(P, st);
         launch_sources(P, st);
@@ -953,10 +955,10 @@ static const KParams *enqueue_kick(const KParams &P, const KParams &Q, hipStream
 // One gas step of step_Euler; the transport's velocity kernel writes the
 // final velocities to the fields of view P and, in the fused step, applies
 // the damping of the final boundary call that follows it.
-static void enqueue_euler(const KParams &P, hipStream_t st, int count_monitor, bool fused)
+static void enqueue_euler(const KParams &P, hipStream_t st, int count_monitor, bool fused, int fold_mode)
 {
     const KParams Q = swapped_view(P);
-    const KParams *cur = enqueue_kick(P, Q, st, fused, false);
+    const KParams *cur = enqueue_kick(P, Q, st, fused, false, fused ? fold_mode : -1);
     if (fused) {
         launch_transport_ghost(*cur, P, st, true);
     } else {
@@ -996,14 +998,29 @@ static void enqueue_leapfrog(const KParams &P, hipStream_t st, int count_monitor
 static void enqueue_step(const KParams &P, hipStream_t st, int count_monitor, bool fused)
 {
     if (P.integrator == FCPT_INTEGRATOR_LEAPFROG) enqueue_leapfrog(P, st, count_monitor, fused);
-    else enqueue_euler(P, st, count_monitor, fused);
+    else enqueue_euler(P, st, count_monitor, fused, -1);
+}
+
+// one sim::run iteration: CalculateTimeStep with the monitor clip (mode 2),
+// then the step; the fused Euler step folds the CFL minima inside its
+// potential launch (one launch less)
+static void enqueue_iteration(const fcpt_ctx *ctx)
+{
+    const KParams &P = ctx->P;
+    if (ctx->fused && P.integrator != FCPT_INTEGRATOR_LEAPFROG) {
+        launch_cfl_cells(P, ctx->stream);
+        enqueue_euler(P, ctx->stream, 1, true, 2);
+    } else {
+        launch_cfl(P, ctx->stream, 2);
+        enqueue_step(P, ctx->stream, 1, ctx->fused);
+    }
 }
 
 int fcpt_step_euler(fcpt_ctx *ctx, double dt)
 {
     CTX_CHECK(ctx);
     launch_set_dt(ctx->P, ctx->stream, dt);
-    enqueue_euler(ctx->P, ctx->stream, 0, ctx->fused);
+    enqueue_euler(ctx->P, ctx->stream, 0, ctx->fused, -1);
     LAUNCH_CHECK(ctx);
     return FCPT_OK;
 }
@@ -1038,9 +1055,7 @@ int fcpt_set_midstep_params(fcpt_ctx *ctx, const fcpt_step_params *params)
 int fcpt_run_step(fcpt_ctx *ctx, double *dt_out)
 {
     CTX_CHECK(ctx);
-    const KParams &P = ctx->P;
-    launch_cfl(P, ctx->stream, 2);
-    enqueue_step(P, ctx->stream, 1, ctx->fused);
+    enqueue_iteration(ctx);
     LAUNCH_CHECK(ctx);
     int rc = read_scalars(ctx);
     if (rc) return rc;
@@ -1056,8 +1071,7 @@ static int ensure_step_graph(fcpt_ctx *ctx)
     if (ctx->step_graph) return FCPT_OK;
     hipGraph_t g = nullptr;
     HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    launch_cfl(ctx->P, ctx->stream, 2);
-    enqueue_step(ctx->P, ctx->stream, 1, ctx->fused);
+    enqueue_iteration(ctx);
     hipError_t e = hipStreamEndCapture(ctx->stream, &g);
     if (e != hipSuccess) return fail(ctx, FCPT_ERR_DEVICE, "graph capture: %s", hipGetErrorString(e));
     e = hipGraphInstantiate(&ctx->step_graph, g, nullptr, nullptr, 0);
@@ -1079,8 +1093,7 @@ int fcpt_run_steps(fcpt_ctx *ctx, int32_t n_steps, int32_t use_graph)
         for (int k = 0; k < n_steps; ++k) HIP_TRY(ctx, hipGraphLaunch(ctx->step_graph, ctx->stream));
     } else {
         for (int k = 0; k < n_steps; ++k) {
-            launch_cfl(ctx->P, ctx->stream, 2);
-            enqueue_step(ctx->P, ctx->stream, 1, ctx->fused);
+            enqueue_iteration(ctx);
         }
         LAUNCH_CHECK(ctx);
     }

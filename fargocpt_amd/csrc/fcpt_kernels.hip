@@ -535,19 +535,23 @@ __global__ void k_leapfrog_phase(KParams P, int phase)
 // every step.  Every body is seen from distance sqrt(R^2 + |r_k|^2) with the
 // ring's background smoothing, which keeps the stored deviations small away
 // from the bodies.
-__device__ __forceinline__ void b_potential_bg(const KParams &P, int bx)
+__device__ __forceinline__ double potential_bg_value(const KParams &P, int i)
 {
-    const int i = bx * (int)blockDim.x + (int)threadIdx.x;
-    if (i >= P.nr) return;
     const DevScalars *s = P.s;
-    const double R = cst(P.Rmed)[i];
+    const double R = P.Rmed[i];
     const double smooth = P.thickness_smoothing * P.bg[FCPT_SCALE_HEIGHT][i];
     double b = 0.0;
     for (int k = 0; k < s->n_planets; ++k) {
         const double px = s->planet[k].x, py = s->planet[k].y;
         b += -P.G * s->planet[k].mass / sqrt(R * R + (px * px + py * py) + smooth * smooth);
     }
-    P.bg[FCPT_POTENTIAL][i] = b;
+    return b;
+}
+__device__ __forceinline__ void b_potential_bg(const KParams &P, int bx)
+{
+    const int i = bx * (int)blockDim.x + (int)threadIdx.x;
+    if (i >= P.nr) return;
+    P.bg[FCPT_POTENTIAL][i] = potential_bg_value(P, i);
 }
 __global__ void __launch_bounds__(256) k_potential_bg(KParams P) { b_potential_bg(P, (int)blockIdx.x); }
 __global__ void __launch_bounds__(256) kb_potential_bg(const KParams *Ps, const BatchEntry *map)
@@ -596,6 +600,31 @@ __device__ __forceinline__ void b_potential(const KParams &P, int bx, int by)
     fld<SPLIT>(P, FCPT_POTENTIAL).set(i, j, potential_cell<SPLIT>(P, i, j, H));
 }
 FCPT_PAIR0(potential, 256)
+
+// The potential of the fused Euler step in one launch: in split storage each
+// wavefront forms its ring's background itself (lane 0 of the first column
+// stores it), and one extra block row runs the CFL fold of the step's start
+// (k_cfl_finalize_rings; the potential reads none of the scalars the fold
+// writes, and dt is first read by the kernels after this one).
+template <bool SPLIT>
+__global__ void __launch_bounds__(256) k_potential_fold(KParams P, int fold_mode)
+{
+    const int bx = (int)blockIdx.x, by = (int)blockIdx.y;
+    if (by == (P.nr + 3) / 4) {
+        if (bx == 0) b_cfl_fold(P, fold_mode, (int)(threadIdx.y * blockDim.x + threadIdx.x), true);
+        return;
+    }
+    CELL_LOOP_HEAD(0, P.nr)
+    const double H = P.adiabatic ? fld<SPLIT>(P, FCPT_SCALE_HEIGHT)(i, j) : cst(P.iso_h)[i];
+    const double acc = potential_cell<SPLIT>(P, i, j, H);
+    if constexpr (SPLIT) {
+        const double b = potential_bg_value(P, i);
+        if (bx == 0 && threadIdx.x == 0) P.bg[FCPT_POTENTIAL][i] = b;
+        static_cast<float *>(P.f[FCPT_POTENTIAL])[(size_t)i * (size_t)ns + (size_t)j] = (float)(acc - b);
+    } else {
+        fld<SPLIT>(P, FCPT_POTENTIAL).set(i, j, acc);
+    }
+}
 
 // ---------------------------------------------------------------------------
 // momentum_update_radial, SourceEuler.cpp:325-372, for one cell: pressure and
@@ -2759,6 +2788,16 @@ void launch_potential(const KParams &P, hipStream_t st)
     LAUNCH(k_potential, cell_grid(P, P.nr), kBlock, st, P);
 }
 
+void launch_potential_fold(const KParams &P, hipStream_t st, int fold_mode)
+{
+    if (fold_mode < 0) {
+        launch_potential(P, st);
+        return;
+    }
+    const dim3 g = cell_grid(P, P.nr);
+    LAUNCH(k_potential_fold, dim3(g.x, g.y + 1), kBlock, st, P, fold_mode);
+}
+
 void launch_sources(const KParams &P, hipStream_t st)
 {
     LAUNCH(k_src_vr, cell_grid(P, P.nr - 3), kBlock, st, P);
@@ -2972,12 +3011,12 @@ bool kick_walk_applies(const KParams &P)
            visc_tile(P) > 1;
 }
 
-void launch_kick(const KParams &P, hipStream_t st)
+void launch_kick(const KParams &P, hipStream_t st, int fold_mode)
 {
     const int tile = kick_tile(P);
     const int tiles = (P.nr + tile - 1) / tile;
     const dim3 g((unsigned)((P.ns + kKickSegW - 1) / kKickSegW), (unsigned)((tiles + 3) / 4));
-    launch_potential(P, st);
+    launch_potential_fold(P, st, fold_mode);
     const int v = (P.split ? 4 : 0) + (P.adiabatic ? 2 : 0) + (P.artvisc == FCPT_ARTVISC_SN ? 1 : 0);
     switch (v) {
     case 0: hipLaunchKernelGGL((k_kick_walk<false, false, false>), g, kBlock, 0, st, P, tile); break;

@@ -54,7 +54,10 @@ void launch_visc_energy(const KParams &P, hipStream_t st);
 // velocities to the view's partner buffers; then launch_visc_energy
 // (adiabatic) or launch_visc_fused on the swapped view
 bool kick_walk_applies(const KParams &P);
-void launch_kick(const KParams &P, hipStream_t st);
+void launch_kick(const KParams &P, hipStream_t st, int fold_mode);
+// the potential; fold_mode >= 0: in the same launch, the CFL fold of
+// k_cfl_rings' slots (launch_cfl_cells) with that mode
+void launch_potential_fold(const KParams &P, hipStream_t st, int fold_mode);
 void launch_radial_sweep(const KParams &P, hipStream_t st);
 void launch_azimuthal_sweep(const KParams &P, hipStream_t st);
 void launch_cfl_cells(const KParams &P, hipStream_t st);
