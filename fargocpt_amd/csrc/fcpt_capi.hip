@@ -960,12 +960,14 @@ static void enqueue_euler(const KParams &P, hipStream_t st, int count_monitor, b
     const KParams Q = swapped_view(P);
     const KParams *cur = enqueue_kick(P, Q, st, fused, false, fused ? fold_mode : -1);
     if (fused) {
-        launch_transport_ghost(*cur, P, st, true);
+        // the transport's velocity kernel also runs the final boundary call
+        // and the derived fields
+        launch_transport_ghost(*cur, P, st, count_monitor ? 2 : 1);
     } else {
         launch_boundaries(*cur, st, 0, 0);
         launch_transport_to(*cur, P, st);
+        launch_final_boundaries_derived(P, st, count_monitor ? 2 : 1, false);
     }
-    launch_final_boundaries_derived(P, st, count_monitor ? 2 : 1, fused);
 }
 
 // One gas step of step_LeapFrog (simulation.cpp:276-459): kick with dt / 2,
@@ -980,7 +982,7 @@ static void enqueue_leapfrog(const KParams &P, hipStream_t st, int count_monitor
     const KParams *cur = enqueue_kick(P, Q, st, fused, false);
     launch_leapfrog_phase(P, st, 1);
     if (fused) {
-        launch_transport_ghost(*cur, P, st, false);
+        launch_transport_ghost(*cur, P, st, 0);
     } else {
         launch_boundaries(*cur, st, 0, 0);
         launch_transport_to(*cur, P, st);

@@ -609,8 +609,9 @@ FCPT_PAIR0(potential, 256)
 template <bool SPLIT>
 __global__ void __launch_bounds__(256) k_potential_fold(KParams P, int fold_mode)
 {
-    const int bx = (int)blockIdx.x, by = (int)blockIdx.y;
-    if (by == (P.nr + 3) / 4) {
+    // block row 0 is the fold (dispatched first, it ends under the others)
+    const int bx = (int)blockIdx.x, by = (int)blockIdx.y - 1;
+    if (by < 0) {
         if (bx == 0) b_cfl_fold(P, fold_mode, (int)(threadIdx.y * blockDim.x + threadIdx.x), true);
         return;
     }
@@ -2019,6 +2020,38 @@ __device__ __forceinline__ void advance_clock(const KParams &P, int advance_time
 
 // the inner / outer ghost rings of sector j (rows 2 and nr-2 are never
 // written here, nr >= 8)
+// the ghost values of v_r: rows 0 and 1 from v_r(2), rows nr and nr-1 from
+// v_r(nr-2), and the Keplerian v_phi of a ghost ring
+__device__ __forceinline__ void ghost_vr_inner(const KParams &P, double v2, double &v0, double &v1)
+{
+    if (P.bc_inner == FCPT_BC_REFLECTING) {
+        v0 = -v2;
+        v1 = 0;
+    } else if (P.bc_inner == FCPT_BC_ZEROGRADIENT) {
+        v0 = v2;
+        v1 = v2;
+    } else {
+        v0 = v1 = v2 > 0.0 ? 0.0 : v2;
+    }
+}
+__device__ __forceinline__ void ghost_vr_outer(const KParams &P, double vn2, double &vn, double &vn1)
+{
+    if (P.bc_outer == FCPT_BC_REFLECTING) {
+        vn = -vn2;
+        vn1 = 0;
+    } else if (P.bc_outer == FCPT_BC_ZEROGRADIENT) {
+        vn = vn2;
+        vn1 = vn2;
+    } else {
+        vn = vn1 = vn2 < 0.0 ? 0.0 : vn2;
+    }
+}
+__device__ __forceinline__ double ghost_vaz(const KParams &P, int r)
+{
+    const double vk = sqrt(P.G * P.mstar / cst(P.Rmed)[r]);
+    return 1.0 * vk - cst(P.Rmed)[r] * P.omega_frame;
+}
+
 template <bool SPLIT>
 __device__ __forceinline__ void ghost_inner(const KParams &P, int j)
 {
@@ -2029,23 +2062,11 @@ __device__ __forceinline__ void ghost_inner(const KParams &P, int j)
     const double s1 = S(1, j), e1 = E(1, j), v2 = vr(2, j);
     S.set(0, j, s1);
     E.set(0, j, e1);
-    if (P.bc_inner == FCPT_BC_REFLECTING) {
-        vr.set(0, j, -v2);
-        vr.set(1, j, 0);
-    } else if (P.bc_inner == FCPT_BC_ZEROGRADIENT) {
-        vr.set(0, j, v2);
-        vr.set(1, j, v2);
-    } else {
-        if (v2 > 0.0) {
-            vr.set(1, j, 0.0);
-            vr.set(0, j, 0.0);
-        } else {
-            vr.set(1, j, v2);
-            vr.set(0, j, v2);
-        }
-    }
-    const double vk_in = sqrt(P.G * P.mstar / cst(P.Rmed)[0]);
-    vaz.set(0, j, 1.0 * vk_in - cst(P.Rmed)[0] * P.omega_frame);
+    double v0, v1;
+    ghost_vr_inner(P, v2, v0, v1);
+    vr.set(0, j, v0);
+    vr.set(1, j, v1);
+    vaz.set(0, j, ghost_vaz(P, 0));
 }
 
 template <bool SPLIT>
@@ -2059,23 +2080,11 @@ __device__ __forceinline__ void ghost_outer(const KParams &P, int j)
     const double s2 = S(nr - 2, j), e2 = E(nr - 2, j), vn2 = vr(nr - 2, j);
     S.set(nr - 1, j, s2);
     E.set(nr - 1, j, e2);
-    if (P.bc_outer == FCPT_BC_REFLECTING) {
-        vr.set(nr, j, -vn2);
-        vr.set(nr - 1, j, 0);
-    } else if (P.bc_outer == FCPT_BC_ZEROGRADIENT) {
-        vr.set(nr, j, vn2);
-        vr.set(nr - 1, j, vn2);
-    } else {
-        if (vn2 < 0.0) {
-            vr.set(nr - 1, j, 0.0);
-            vr.set(nr, j, 0.0);
-        } else {
-            vr.set(nr - 1, j, vn2);
-            vr.set(nr, j, vn2);
-        }
-    }
-    const double vk_out = sqrt(P.G * P.mstar / cst(P.Rmed)[nr - 1]);
-    vaz.set(nr - 1, j, 1.0 * vk_out - cst(P.Rmed)[nr - 1] * P.omega_frame);
+    double vn, vn1;
+    ghost_vr_outer(P, vn2, vn, vn1);
+    vr.set(nr, j, vn);
+    vr.set(nr - 1, j, vn1);
+    vaz.set(nr - 1, j, ghost_vaz(P, nr - 1));
 }
 
 template <bool SPLIT>
@@ -2545,6 +2554,7 @@ template <bool SPLIT>
 __device__ __forceinline__ double damp_stored(const KParams &P, const Fld<SPLIT> &X, int f0, bool vector,
                                               int i, int j, double dt, double x)
 {
+    if (!P.damping) return x;
     const bool in = damp_in_inner(P, vector, i), out = damp_in_outer(P, vector, i);
     if (!in && !out) return x;
     x = X.stored(i, x);
@@ -2590,6 +2600,105 @@ __global__ void __launch_bounds__(256) k_velocities_damped(KParams P)
     S.set(i, j, damp_stored<SPLIT>(P, S, FCPT_SIGMA0, false, i, j, dt, sig));
     if (P.adiabatic)
         E.set(i, j, damp_stored<SPLIT>(P, E, FCPT_ENERGY0, false, i, j, dt, temperature_floor(P, e, sig)));
+}
+
+// The end of the fused Euler step in one launch: the velocity kernel, the
+// final boundary call (damping, ghost rings, the clock) and
+// recalculate_derived_disk_quantities.  A cell forms its velocities, Sigma
+// and E as k_velocities_damped stores them and its derived fields from the
+// stored values, as k_final_derived reads them.  The ghost rings copy their
+// neighbour ring, which another thread stores, so a ghost-ring thread forms
+// the neighbour's values itself (the same expressions on the same inputs):
+// ring 0 takes Sigma and E of ring 1 and v_r from v_r(2), ring nr-1 those of
+// ring nr-2; v_r(1) is a ghost value too, so the ring-1 thread stores that
+// one instead of its own, and no cell is stored by two threads.
+template <bool SPLIT>
+__device__ __forceinline__ double vel_final_vr(const KParams &P, int r, int j, double dt)
+{
+    // v_r(r >= 1) as the velocity kernel and the damping pass it to set()
+    const Fld<false> Sb = buf<false>(P, P.sig_c, nullptr);
+    const Fld<SPLIT> m0 = buf<SPLIT>(P, P.mom[0], P.bgm[0]);
+    const Fld<SPLIT> m1 = buf<SPLIT>(P, P.mom[1], P.bgm[1]);
+    FLD(vr, FCPT_VRAD);
+    const double v = (m0(r - 1, j) + m1(r, j)) / (Sb(r - 1, j) + Sb(r, j));
+    return damp_stored<SPLIT>(P, vr, FCPT_VRAD0, true, r, j, dt, v);
+}
+template <bool SPLIT>
+__device__ __forceinline__ void vel_final_sig_e(const KParams &P, int r, int j, double dt, double &sig,
+                                                double &e)
+{
+    // Sigma and (adiabatic) E of ring r as the velocity kernel and the
+    // damping pass them to set(); isothermal E is not touched there and is
+    // the stored value
+    FLD(S, FCPT_SIGMA);
+    FLD(E, FCPT_ENERGY);
+    const double floor_value = P.sigma_floor * P.sigma0;
+    double s0 = buf<false>(P, P.sig_c, nullptr)(r, j);
+    if (s0 < floor_value) s0 = floor_value;
+    sig = damp_stored<SPLIT>(P, S, FCPT_SIGMA0, false, r, j, dt, s0);
+    if (P.adiabatic) {
+        const double eb = buf<false>(P, P.en_c, nullptr)(r, j);
+        e = damp_stored<SPLIT>(P, E, FCPT_ENERGY0, false, r, j, dt, temperature_floor(P, eb, s0));
+    } else {
+        e = E(r, j);
+    }
+}
+
+template <bool SPLIT>
+__global__ void __launch_bounds__(256) k_velocities_final(KParams P, int advance_time)
+{
+    const int bx = (int)blockIdx.x, by = (int)blockIdx.y;
+    CELL_LOOP_HEAD(0, P.nr)
+    const int nr = P.nr;
+    const double dt = P.s->dt;
+    FLD(vr, FCPT_VRAD);
+    FLD(vaz, FCPT_VAZI);
+    FLD(S, FCPT_SIGMA);
+    FLD(E, FCPT_ENERGY);
+    const bool ghost = i == 0 || i == nr - 1;
+    // every value first, every store after (a load after a store waits for it)
+    double sr, er, va = 0.0, vb = 0.0, vphi = 0.0;
+    if (ghost) {
+        const int r = i == 0 ? 1 : nr - 2;  // the ring the ghost ring copies
+        vel_final_sig_e<SPLIT>(P, r, j, dt, sr, er);
+        // the ghost ring takes the neighbour's values as a load returns them
+        sr = S.stored(r, sr);
+        if (P.adiabatic) er = E.stored(r, er);
+        if (i == 0) ghost_vr_inner(P, vr.stored(2, vel_final_vr<SPLIT>(P, 2, j, dt)), va, vb);
+        else ghost_vr_outer(P, vr.stored(nr - 2, vel_final_vr<SPLIT>(P, nr - 2, j, dt)), va, vb);
+        vphi = ghost_vaz(P, i);
+    } else {
+        const Fld<false> Sb = buf<false>(P, P.sig_c, nullptr);
+        const Fld<SPLIT> m2 = buf<SPLIT>(P, P.mom[2], P.bgm[2]);
+        const Fld<SPLIT> m3 = buf<SPLIT>(P, P.mom[3], P.bgm[3]);
+        const int jm = j == 0 ? ns - 1 : j - 1;
+        const double s0 = Sb(i, j), sjm = Sb(i, jm);
+        const double v_phi =
+            (m2(i, jm) + m3(i, j)) / (sjm + s0) * cst(P.InvRmed)[i] - cst(P.Rmed)[i] * P.omega_frame;
+        vphi = damp_stored<SPLIT>(P, vaz, FCPT_VAZI0, false, i, j, dt, v_phi);
+        if (i == 1) ghost_vr_inner(P, vr.stored(2, vel_final_vr<SPLIT>(P, 2, j, dt)), va, vb);
+        else vb = vel_final_vr<SPLIT>(P, i, j, dt);
+        vel_final_sig_e<SPLIT>(P, i, j, dt, sr, er);
+    }
+    // stores: the ghost rings, v_r(1) of the ring-1 thread, or the cell's own
+    S.set(i, j, sr);
+    if (ghost || P.adiabatic) E.set(i, j, er);
+    vaz.set(i, j, vphi);
+    if (i == 0) {
+        vr.set(0, j, va);
+    } else if (i == nr - 1) {
+        vr.set(nr, j, va);
+        vr.set(nr - 1, j, vb);
+    } else {
+        vr.set(i, j, vb);  // v_r(1): the ghost value
+    }
+    if (advance_time && i == 0 && j == 0) advance_clock(P, advance_time);
+    DerivedIn d{P.adiabatic ? E.stored(i, er) : 0.0, S.stored(i, sr), 0.0, 0.0};
+    if (!P.adiabatic) {
+        d.cs = cst(P.iso_cs)[i];
+        d.h = cst(P.iso_h)[i];
+    }
+    derived_store<SPLIT>(P, i, j, d, true, true);
 }
 
 // ---------------------------------------------------------------------------
@@ -2920,12 +3029,12 @@ void launch_transport_to(const KParams &P, const KParams &out, hipStream_t st)
     LAUNCH(k_velocities, cell_grid(out, out.nr), kBlock, st, out);
 }
 
-void launch_transport_ghost(const KParams &P, const KParams &out, hipStream_t st, bool damp)
+void launch_transport_ghost(const KParams &P, const KParams &out, hipStream_t st, int final_advance)
 {
     LAUNCH(k_ghost_ring_mean, dim3((unsigned)P.nr), dim3(256), st, P);
     launch_radial_sweep(P, st);
     launch_azimuthal_sweep(P, st);
-    if (damp && out.damping) LAUNCH(k_velocities_damped, cell_grid(out, out.nr + 1), kBlock, st, out);
+    if (final_advance) LAUNCH(k_velocities_final, cell_grid(out, out.nr), kBlock, st, out, final_advance);
     else LAUNCH(k_velocities, cell_grid(out, out.nr), kBlock, st, out);
 }
 

@@ -36,9 +36,10 @@ void launch_fill_rows(const KParams &P, hipStream_t st, void *dst, const double 
                       const double *bg, int rows);
 void launch_transport_to(const KParams &P, const KParams &out, hipStream_t st);
 // the boundary call before the transport (ghost rings) and the transport, the
-// ghost rings and the FARGO ring mean in one launch; damp: the final boundary
-// call's damping applied by the velocity kernel
-void launch_transport_ghost(const KParams &P, const KParams &out, hipStream_t st, bool damp);
+// ghost rings and the FARGO ring mean in one launch; final_advance > 0: the
+// velocity kernel also runs the final boundary call (damping, ghost rings,
+// the clock with that advance_time) and the derived fields (Euler)
+void launch_transport_ghost(const KParams &P, const KParams &out, hipStream_t st, int final_advance);
 // the fused step kernels write the velocities of view P to P.valt; the
 // swapped view reads those buffers and writes the fields
 KParams swapped_view(const KParams &P);
