@@ -418,26 +418,40 @@ FCPT_PAIR0(cfl_cells, 256)
 // the ring's slot; the fold (rings = true) adds the shear limits between
 // neighbouring rings' means.  The minimum over the same values as
 // k_ring_mean + k_cfl_cells form.
+// 512 threads: the first 256 form the mean (the 256-thread sums and tree of
+// k_ring_mean; the others take part in the barriers only), all 512 the cells.
 template <bool SPLIT>
-__global__ void __launch_bounds__(256) k_cfl_rings(KParams P)
+__global__ void __launch_bounds__(512) k_cfl_rings(KParams P)
 {
     __shared__ double sh[256];
-    __shared__ double wmin[4];
+    __shared__ double wmin[8];
     const int ns = P.ns;
     const int i = (int)blockIdx.x;
     const int t = (int)threadIdx.x;
     FLD(v, FCPT_VAZI);
     double acc = 0.0;
-    for (int j = t; j < ns; j += 256) acc += v(i, j);
-    const double vm = block_sum<256>(acc, sh) / (double)ns;
+    if (t < 256)
+        for (int j = t; j < ns; j += 256) acc += v(i, j);
+    if (t < 256) sh[t] = acc;
+    __syncthreads();
+#pragma unroll
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) sh[t] = sh[t] + sh[t + w];
+        __syncthreads();
+    }
+    const double vm = sh[0] / (double)ns;
     if (t == 0) P.vmean[i] = vm;
     double cand = __longlong_as_double(0x7FF0000000000000ll);
     if (i >= 1 && i < P.nr - 1)
-        for (int j = t; j < ns; j += 256) cand = std_min(cand, cfl_cell<SPLIT>(P, i, j, vm));
+        for (int j = t; j < ns; j += 512) cand = std_min(cand, cfl_cell<SPLIT>(P, i, j, vm));
     for (int off = 32; off > 0; off >>= 1) cand = std_min(cand, __shfl_xor(cand, off, 64));
     if ((t & 63) == 0) wmin[t >> 6] = cand;
     __syncthreads();
-    if (t == 0) P.ring_min[i] = std_min(std_min(wmin[0], wmin[1]), std_min(wmin[2], wmin[3]));
+    if (t == 0) {
+        double m = wmin[0];
+        for (int k = 1; k < 8; ++k) m = std_min(m, wmin[k]);
+        P.ring_min[i] = m;
+    }
 }
 
 __global__ void __launch_bounds__(256) k_cfl_finalize(KParams P, int mode) { b_cfl_finalize(P, mode); }
@@ -2830,13 +2844,13 @@ void init_launch_fields(KParams &P)
 
 void launch_cfl(const KParams &P, hipStream_t st, int mode)
 {
-    LAUNCH(k_cfl_rings, dim3((unsigned)P.nr), dim3(256), st, P);
+    LAUNCH(k_cfl_rings, dim3((unsigned)P.nr), dim3(512), st, P);
     hipLaunchKernelGGL(k_cfl_finalize_rings, dim3(1), dim3(256), 0, st, P, mode);
 }
 
 void launch_cfl_cells(const KParams &P, hipStream_t st)
 {
-    LAUNCH(k_cfl_rings, dim3((unsigned)P.nr), dim3(256), st, P);
+    LAUNCH(k_cfl_rings, dim3((unsigned)P.nr), dim3(512), st, P);
 }
 
 void launch_set_dt(const KParams &P, hipStream_t st, double dt)
