@@ -237,6 +237,21 @@ __device__ double block_sum(double v, double *sh)
 // mode 0: CFL ring mean
 // mode 1: transport ring mean + FARGO shift + uniform residual
 //         (TransportEuler.cpp:207-236)
+__device__ __forceinline__ void ring_mean_store(const KParams &P, int i, double sum, int mode)
+{
+    if (threadIdx.x != 0) return;
+    const double vm = sum / (double)P.ns;
+    P.vmean[i] = vm;
+    if (mode == 1) {
+        const double dt = P.s->dt;
+        const double invdt = 1.0 / dt;
+        const double Ntilde = vm * cst(P.InvRmed)[i] * dt * P.invdphi;
+        const double Nround = floor(Ntilde + 0.5);
+        P.nshift[i] = (int)Nround;
+        P.vuni[i] = (Ntilde - Nround) * cst(P.Rmed)[i] * invdt * P.dphi;
+    }
+}
+
 template <bool SPLIT>
 __device__ __forceinline__ void b_ring_mean(const KParams &P, int bx, int by, int mode)
 {
@@ -246,19 +261,7 @@ __device__ __forceinline__ void b_ring_mean(const KParams &P, int bx, int by, in
     FLD(v, FCPT_VAZI);
     double acc = 0.0;
     for (int j = threadIdx.x; j < ns; j += 256) acc += v(i, j);
-    const double sum = block_sum<256>(acc, sh);
-    if (threadIdx.x == 0) {
-        const double vm = sum / (double)ns;
-        P.vmean[i] = vm;
-        if (mode == 1) {
-            const double dt = P.s->dt;
-            const double invdt = 1.0 / dt;
-            const double Ntilde = vm * cst(P.InvRmed)[i] * dt * P.invdphi;
-            const double Nround = floor(Ntilde + 0.5);
-            P.nshift[i] = (int)Nround;
-            P.vuni[i] = (Ntilde - Nround) * cst(P.Rmed)[i] * invdt * P.dphi;
-        }
-    }
+    ring_mean_store(P, i, block_sum<256>(acc, sh), mode);
 }
 FCPT_PAIR1(ring_mean, 256, int, mode)
 
@@ -2114,19 +2117,32 @@ FCPT_PAIR1(ghost_bc, 256, int, advance_time)
 
 // the boundary call before the transport (ghost rings, b_ghost_bc) and the
 // transport's ring mean with the FARGO shift (b_ring_mean, mode 1) in one
-// launch, one block of 256 threads per ring: blocks 0 and nr-1 first set
-// their ghost ring, each thread the sectors whose v_phi it then sums, so
-// every block sums what the two separate kernels would have it sum.  Only the
-// azimuthal sweep reads the ring means, so they may be formed before the
-// radial sweep.
+// launch: the first blocks set the ghost rings (one thread per sector), the
+// other blocks (one of 256 threads per ring) form the ring means.  The ghost
+// rings' v_phi is one Keplerian value per ring, so their blocks sum that value
+// as stored in place of a load (the same sums, the same tree) and no block
+// reads what another writes.  Only the azimuthal sweep reads the ring means,
+// so they may be formed before the radial sweep.
 template <bool SPLIT>
-__global__ void __launch_bounds__(256) k_ghost_ring_mean(KParams P)
+__global__ void __launch_bounds__(256) k_ghost_ring_mean(KParams P, int ghost_blocks)
 {
-    const int i = (int)blockIdx.x;
-    if (i == 0)
-        for (int j = (int)threadIdx.x; j < P.ns; j += 256) ghost_inner<SPLIT>(P, j);
-    if (i == P.nr - 1)
-        for (int j = (int)threadIdx.x; j < P.ns; j += 256) ghost_outer<SPLIT>(P, j);
+    if ((int)blockIdx.x < ghost_blocks) {
+        const int j = (int)(blockIdx.x * 256 + threadIdx.x);
+        if (j < P.ns) {
+            ghost_inner<SPLIT>(P, j);
+            ghost_outer<SPLIT>(P, j);
+        }
+        return;
+    }
+    const int i = (int)blockIdx.x - ghost_blocks;
+    if (i == 0 || i == P.nr - 1) {
+        __shared__ double sh[256];
+        const double g = fld<SPLIT>(P, FCPT_VAZI).stored(i, ghost_vaz(P, i));
+        double acc = 0.0;
+        for (int j = (int)threadIdx.x; j < P.ns; j += 256) acc += g;
+        ring_mean_store(P, i, block_sum<256>(acc, sh), 1);
+        return;
+    }
     b_ring_mean<SPLIT>(P, i, 0, 1);
 }
 
@@ -3045,7 +3061,8 @@ void launch_transport_to(const KParams &P, const KParams &out, hipStream_t st)
 
 void launch_transport_ghost(const KParams &P, const KParams &out, hipStream_t st, int final_advance)
 {
-    LAUNCH(k_ghost_ring_mean, dim3((unsigned)P.nr), dim3(256), st, P);
+    const int gb = (P.ns + 255) / 256;
+    LAUNCH(k_ghost_ring_mean, dim3((unsigned)(gb + P.nr)), dim3(256), st, P, gb);
     launch_radial_sweep(P, st);
     launch_azimuthal_sweep(P, st);
     if (final_advance) LAUNCH(k_velocities_final, cell_grid(out, out.nr), kBlock, st, out, final_advance);
