@@ -2655,7 +2655,7 @@ __device__ __forceinline__ double vel_final_vr(const KParams &P, int r, int j, d
 }
 template <bool SPLIT>
 __device__ __forceinline__ void vel_final_sig_e(const KParams &P, int r, int j, double dt, double &sig,
-                                                double &e)
+                                                double &e, bool need_e)
 {
     // Sigma and (adiabatic) E of ring r as the velocity kernel and the
     // damping pass them to set(); isothermal E is not touched there and is
@@ -2670,7 +2670,7 @@ __device__ __forceinline__ void vel_final_sig_e(const KParams &P, int r, int j, 
         const double eb = buf<false>(P, P.en_c, nullptr)(r, j);
         e = damp_stored<SPLIT>(P, E, FCPT_ENERGY0, false, r, j, dt, temperature_floor(P, eb, s0));
     } else {
-        e = E(r, j);
+        e = need_e ? E(r, j) : 0.0;
     }
 }
 
@@ -2690,7 +2690,7 @@ __global__ void __launch_bounds__(256) k_velocities_final(KParams P, int advance
     double sr, er, va = 0.0, vb = 0.0, vphi = 0.0;
     if (ghost) {
         const int r = i == 0 ? 1 : nr - 2;  // the ring the ghost ring copies
-        vel_final_sig_e<SPLIT>(P, r, j, dt, sr, er);
+        vel_final_sig_e<SPLIT>(P, r, j, dt, sr, er, true);
         // the ghost ring takes the neighbour's values as a load returns them
         sr = S.stored(r, sr);
         if (P.adiabatic) er = E.stored(r, er);
@@ -2708,7 +2708,7 @@ __global__ void __launch_bounds__(256) k_velocities_final(KParams P, int advance
         vphi = damp_stored<SPLIT>(P, vaz, FCPT_VAZI0, false, i, j, dt, v_phi);
         if (i == 1) ghost_vr_inner(P, vr.stored(2, vel_final_vr<SPLIT>(P, 2, j, dt)), va, vb);
         else vb = vel_final_vr<SPLIT>(P, i, j, dt);
-        vel_final_sig_e<SPLIT>(P, i, j, dt, sr, er);
+        vel_final_sig_e<SPLIT>(P, i, j, dt, sr, er, false);
     }
     // stores: the ghost rings, v_r(1) of the ring-1 thread, or the cell's own
     S.set(i, j, sr);
