@@ -23,12 +23,12 @@ import re
 
 # bench.py times these kernels with fcpt_time_kernel: 3 rounds of 1 warm-up + 20
 # launches (the viscous pass timed is the walk without SubStep3)
-TIMED = (r"k_radial_sweep", r"k_azimuthal_sweep", r"k_cfl_cells", r"k_visc_walk<(true|false), false>",
+TIMED = (r"k_radial_sweep", r"k_azimuthal_sweep", r"k_cfl_rings", r"k_visc_walk<(true|false), false>",
          r"k_potential<")
 TIMING_LAUNCHES = 63
 KT_STEPS = 55
 JSON_NAMES = {r"k_radial_sweep": "radial_sweep", r"k_azimuthal_sweep": "azimuthal_sweep",
-              r"k_cfl_cells": "cfl", r"k_visc_walk<(true|false), false>": "visc_fused",
+              r"k_cfl_rings": "cfl", r"k_visc_walk<(true|false), false>": "visc_fused",
               r"k_potential<": "potential", r"k_disk_force<": "disk_force"}
 
 
