@@ -284,7 +284,9 @@ int fcpt_time_copy(fcpt_ctx *ctx, int32_t reps, double *ms_per_launch, double *b
                             SN artificial viscosity and a viscous tile above 1).
    FCPT_PARAM_FUSED_STEP (default 1) makes fcpt_step_euler, fcpt_step_leapfrog,
    fcpt_run_step and fcpt_run_steps run the fused passes (the walks above,
-   the damping inside the transport's velocity kernel); the evolved and
+   the ghost rings with the FARGO ring means, and for Euler the final
+   boundary call and derived fields inside the transport's velocity
+   kernel); the evolved and
    derived fields are the same bit for bit in FP64 storage, but those steps
    leave Q_R, Q_PHI, TAU_* and DIV_V unspecified (as they were, or, for an
    adiabatic disk with viscous heating on a grid below the walks' size, the
