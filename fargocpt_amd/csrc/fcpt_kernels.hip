@@ -2411,7 +2411,7 @@ __device__ __forceinline__ double star_theta(double q, double dq, double dxtheta
 // upwind side.
 constexpr int kAzHalo = 4;
 constexpr int kAzSegW = 128 - 2 * kAzHalo;  // sectors written per wavefront
-constexpr int kAzWaves = 4;                 // wavefronts per block
+constexpr int kAzWaves = 2;                 // wavefronts per block
 
 // the two sectors of a lane: a = 2 lane, b = 2 lane + 1 (within the segment)
 struct D2 {
