@@ -1527,6 +1527,9 @@ int fcpt_batch_create(fcpt_ctx *const *disks, int32_t n_disks, fcpt_batch **out)
         b->disks.push_back(c);
         KParams P = c->P;
         P.radial_tile = radial_tile_for(P.nr, P.ns, P.radial_tile);
+        // the walks' tiles as the disk's own launches choose them
+        P.kick_tile = resolve_kick_tile(P);
+        P.visc_tile = resolve_visc_tile(P);
         host.push_back(P);
     }
     hipError_t e = hipSetDevice(b->device);
@@ -1643,7 +1646,7 @@ int fcpt_batch_time_kernel(fcpt_batch *b, int32_t kernel, int32_t reps, double *
     switch (kernel) {
     case FCPT_KERNEL_RADIAL_SWEEP: kinds[0] = B_RADIAL_ISO; kinds[1] = B_RADIAL_ADI; break;
     case FCPT_KERNEL_AZIMUTHAL: kinds[0] = B_AZIM_ISO; kinds[1] = B_AZIM_ADI; break;
-    case FCPT_KERNEL_CFL: kinds[0] = B_CFL_CELLS; break;
+    case FCPT_KERNEL_CFL: kinds[0] = B_CFL_RINGS; break;
     case FCPT_KERNEL_POTENTIAL: kinds[0] = B_POTENTIAL; break;
     case FCPT_KERNEL_STRESS:
     case FCPT_KERNEL_VISC_FUSED:

@@ -424,12 +424,11 @@ FCPT_PAIR0(cfl_cells, 256)
 // 512 threads: the first 256 form the mean (the 256-thread sums and tree of
 // k_ring_mean; the others take part in the barriers only), all 512 the cells.
 template <bool SPLIT>
-__global__ void __launch_bounds__(512) k_cfl_rings(KParams P)
+__device__ __forceinline__ void b_cfl_rings(const KParams &P, int i)
 {
     __shared__ double sh[256];
     __shared__ double wmin[8];
     const int ns = P.ns;
-    const int i = (int)blockIdx.x;
     const int t = (int)threadIdx.x;
     FLD(v, FCPT_VAZI);
     double acc = 0.0;
@@ -456,11 +455,26 @@ __global__ void __launch_bounds__(512) k_cfl_rings(KParams P)
         P.ring_min[i] = m;
     }
 }
+template <bool SPLIT>
+__global__ void __launch_bounds__(512) k_cfl_rings(KParams P)
+{
+    b_cfl_rings<SPLIT>(P, (int)blockIdx.x);
+}
+template <bool SPLIT>
+__global__ void __launch_bounds__(512) kb_cfl_rings(const KParams *Ps, const BatchEntry *map)
+{
+    const BatchEntry e = batch_entry(map);
+    b_cfl_rings<SPLIT>(batch_disk(Ps, e.disk), (int)(e.yx & 0xffffu));
+}
 
 __global__ void __launch_bounds__(256) k_cfl_finalize(KParams P, int mode) { b_cfl_finalize(P, mode); }
 __global__ void __launch_bounds__(256) k_cfl_finalize_rings(KParams P, int mode)
 {
     b_cfl_fold(P, mode, (int)threadIdx.x, true);
+}
+__global__ void __launch_bounds__(256) kb_cfl_finalize_rings(const KParams *Ps, const BatchEntry *map, int mode)
+{
+    b_cfl_fold(batch_disk(Ps, batch_entry(map).disk), mode, (int)threadIdx.x, true);
 }
 __global__ void __launch_bounds__(256) kb_cfl_finalize(const KParams *Ps, const BatchEntry *map, int mode)
 {
@@ -1551,6 +1565,14 @@ __global__ void __launch_bounds__(256) k_kick_walk(KParams P, int tile)
 {
     b_kick_walk<SPLIT, ADI, SN>(P, (int)blockIdx.x, (int)blockIdx.y, tile);
 }
+// batched form: each disk's tile is its resolved kick_tile
+template <bool SPLIT, bool ADI, bool SN>
+__global__ void __launch_bounds__(256) kb_kick_walk(const KParams *Ps, const BatchEntry *map)
+{
+    const BatchEntry e = batch_entry(map);
+    const KParams &P = batch_disk(Ps, e.disk);
+    b_kick_walk<SPLIT, ADI, SN>(P, (int)(e.yx & 0xffffu), (int)(e.yx >> 16), P.kick_tile);
+}
 
 // compute_viscous_stress_tensor and update_velocities_with_viscosity
 // (viscosity.cpp:139-426) in one pass: the stress components a cell's update
@@ -1947,6 +1969,14 @@ __global__ void __launch_bounds__(256) k_visc_walk(KParams P, int tile)
 {
     b_visc_walk<SPLIT, ENERGY>(P, (int)blockIdx.x, (int)blockIdx.y, tile);
 }
+// batched form: each disk's tile is its resolved visc_tile
+template <bool SPLIT, bool ENERGY>
+__global__ void __launch_bounds__(256) kb_visc_walk(const KParams *Ps, const BatchEntry *map)
+{
+    const BatchEntry e = batch_entry(map);
+    const KParams &P = batch_disk(Ps, e.disk);
+    b_visc_walk<SPLIT, ENERGY>(P, (int)(e.yx & 0xffffu), (int)(e.yx >> 16), P.visc_tile);
+}
 
 // ---------------------------------------------------------------------------
 // damping (boundary_conditions/damping.cpp:311-427, 754-776) of one value:
@@ -2124,17 +2154,18 @@ FCPT_PAIR1(ghost_bc, 256, int, advance_time)
 // reads what another writes.  Only the azimuthal sweep reads the ring means,
 // so they may be formed before the radial sweep.
 template <bool SPLIT>
-__global__ void __launch_bounds__(256) k_ghost_ring_mean(KParams P, int ghost_blocks)
+__device__ __forceinline__ void b_ghost_ring_mean(const KParams &P, int bx)
 {
-    if ((int)blockIdx.x < ghost_blocks) {
-        const int j = (int)(blockIdx.x * 256 + threadIdx.x);
+    const int ghost_blocks = (P.ns + 255) / 256;
+    if (bx < ghost_blocks) {
+        const int j = bx * 256 + (int)threadIdx.x;
         if (j < P.ns) {
             ghost_inner<SPLIT>(P, j);
             ghost_outer<SPLIT>(P, j);
         }
         return;
     }
-    const int i = (int)blockIdx.x - ghost_blocks;
+    const int i = bx - ghost_blocks;
     if (i == 0 || i == P.nr - 1) {
         __shared__ double sh[256];
         const double g = fld<SPLIT>(P, FCPT_VAZI).stored(i, ghost_vaz(P, i));
@@ -2144,6 +2175,17 @@ __global__ void __launch_bounds__(256) k_ghost_ring_mean(KParams P, int ghost_bl
         return;
     }
     b_ring_mean<SPLIT>(P, i, 0, 1);
+}
+template <bool SPLIT>
+__global__ void __launch_bounds__(256) k_ghost_ring_mean(KParams P)
+{
+    b_ghost_ring_mean<SPLIT>(P, (int)blockIdx.x);
+}
+template <bool SPLIT>
+__global__ void __launch_bounds__(256) kb_ghost_ring_mean(const KParams *Ps, const BatchEntry *map)
+{
+    const BatchEntry e = batch_entry(map);
+    b_ghost_ring_mean<SPLIT>(batch_disk(Ps, e.disk), (int)(e.yx & 0xffffu));
 }
 
 // the final boundary call's ghost rings (b_ghost_bc, after the damping) and
@@ -2675,9 +2717,8 @@ __device__ __forceinline__ void vel_final_sig_e(const KParams &P, int r, int j, 
 }
 
 template <bool SPLIT>
-__global__ void __launch_bounds__(256) k_velocities_final(KParams P, int advance_time)
+__device__ __forceinline__ void b_velocities_final(const KParams &P, int bx, int by, int advance_time)
 {
-    const int bx = (int)blockIdx.x, by = (int)blockIdx.y;
     CELL_LOOP_HEAD(0, P.nr)
     const int nr = P.nr;
     const double dt = P.s->dt;
@@ -2730,6 +2771,7 @@ __global__ void __launch_bounds__(256) k_velocities_final(KParams P, int advance
     }
     derived_store<SPLIT>(P, i, j, d, true, true);
 }
+FCPT_PAIR1(velocities_final, 256, int, advance_time)
 
 // ---------------------------------------------------------------------------
 // totals with Neumaier compensation: per ring (sum, compensation) pairs, then
@@ -3062,7 +3104,7 @@ void launch_transport_to(const KParams &P, const KParams &out, hipStream_t st)
 void launch_transport_ghost(const KParams &P, const KParams &out, hipStream_t st, int final_advance)
 {
     const int gb = (P.ns + 255) / 256;
-    LAUNCH(k_ghost_ring_mean, dim3((unsigned)(gb + P.nr)), dim3(256), st, P, gb);
+    LAUNCH(k_ghost_ring_mean, dim3((unsigned)(gb + P.nr)), dim3(256), st, P);
     launch_radial_sweep(P, st);
     launch_azimuthal_sweep(P, st);
     if (final_advance) LAUNCH(k_velocities_final, cell_grid(out, out.nr), kBlock, st, out, final_advance);
@@ -3144,6 +3186,9 @@ static int kick_tile(const KParams &P)
     while (tile > 1 && segs * ((P.nr + tile - 1) / tile) < 3072) tile = tile > 4 ? tile / 2 : 1;
     return tile;
 }
+
+int resolve_kick_tile(const KParams &P) { return kick_tile(P); }
+int resolve_visc_tile(const KParams &P) { return visc_tile(P); }
 
 bool kick_walk_applies(const KParams &P)
 {
@@ -3433,19 +3478,39 @@ std::vector<BatchStageHost> plan_batch_step(const std::vector<KParams> &D)
     auto pot_bg = [](std::vector<BatchEntry> &v, unsigned d, const KParams &P) {
         add_grid(v, d, (unsigned)((P.nr + 255) / 256), 1);
     };
-    // sim::CalculateTimeStep and the monitor clip (simulation.cpp:100-118, 526-542)
-    stage(B_RING_MEAN, 0, all, rings);
-    stage(B_CFL_CELLS, 0, all, cfl);
-    stage(B_CFL_FINAL, 2, all, [](std::vector<BatchEntry> &v, unsigned d, const KParams &) { add_grid(v, d, 1, 1); });
+    auto one = [](std::vector<BatchEntry> &v, unsigned d, const KParams &) { add_grid(v, d, 1, 1); };
+    // sim::CalculateTimeStep and the monitor clip (simulation.cpp:100-118, 526-542):
+    // the per-ring CFL blocks and the fold, as launch_cfl
+    stage(B_CFL_RINGS, 0, all, rings);
+    stage(B_CFL_FINAL_RINGS, 2, all, one);
     // step_Euler (simulation.cpp:148-267), in the order of enqueue_euler
     stage(B_POT_BG, 0, [](const KParams &P) { return P.split != 0; }, pot_bg);
     stage(B_POTENTIAL, 0, all, rows(0));
-    // the fused passes of enqueue_euler, each flipping the disk's view
+    // the kick: the two walks where the disk's own launch uses them
+    // (kick_walk_applies with the tiles resolved at batch creation), else the
+    // cell-parallel fused passes; each pass flips the disk's view
     auto flip = [&](int kind, int arg, auto pred, auto grid) { stage_v(kind, arg, pred, grid, true, false); };
-    flip(B_SRC_FUSED, 0, all, rows(-1));
-    stage(B_SRC_ENERGY, 0, adi, rows(1));
-    auto av = [](const KParams &P) {
-        return P.artvisc == FCPT_ARTVISC_TW || P.artvisc == FCPT_ARTVISC_SN;
+    auto walk = [](const KParams &P) { return kick_walk_applies(P); };
+    auto kick_grid = [](std::vector<BatchEntry> &v, unsigned d, const KParams &P) {
+        const int tiles = (P.nr + P.kick_tile - 1) / P.kick_tile;
+        add_grid(v, d, (unsigned)((P.ns + kKickSegW - 1) / kKickSegW), (unsigned)((tiles + 3) / 4));
+    };
+    auto visc_grid = [](std::vector<BatchEntry> &v, unsigned d, const KParams &P) {
+        const int tiles = (P.nr + P.visc_tile - 1) / P.visc_tile;
+        add_grid(v, d, (unsigned)((P.ns + kViscSegW - 1) / kViscSegW), (unsigned)((tiles + 3) / 4));
+    };
+    for (int a = 0; a < 2; ++a)
+        for (int sn = 0; sn < 2; ++sn)
+            flip(B_KICK_WALK, 2 * a + sn,
+                 [&](const KParams &P) {
+                     return walk(P) && P.adiabatic == a && (P.artvisc == FCPT_ARTVISC_SN) == (sn != 0);
+                 },
+                 kick_grid);
+    auto cell = [&](const KParams &P) { return !walk(P); };
+    flip(B_SRC_FUSED, 0, cell, rows(-1));
+    stage(B_SRC_ENERGY, 0, [&](const KParams &P) { return cell(P) && P.adiabatic; }, rows(1));
+    auto av = [&](const KParams &P) {
+        return cell(P) && (P.artvisc == FCPT_ARTVISC_TW || P.artvisc == FCPT_ARTVISC_SN);
     };
     auto none = [](const KParams &P) { return P.artvisc == FCPT_ARTVISC_NONE; };
     flip(B_AV_FUSED, 1, av, rows(-1));
@@ -3455,19 +3520,28 @@ std::vector<BatchStageHost> plan_batch_step(const std::vector<KParams> &D)
           },
           rows(0));
     stage(B_RECALC_VISC, 0, none, rows(0));
-    flip(B_VISC_FUSED, 0, all, rows(-1));
-    stage(B_SUBSTEP3, 0, adi, rows(0));
-    stage(B_GHOST, 0, all, ghosts);
+    // the viscous pass: the walk (with SubStep3 when adiabatic) where the
+    // disk's tile is above 1, as launch_visc_fused / launch_visc_energy
+    auto vwalk = [](const KParams &P) { return P.visc_tile > 1; };
+    for (int a = 0; a < 2; ++a)
+        flip(B_VISC_WALK, a, [&](const KParams &P) { return vwalk(P) && P.adiabatic == a; }, visc_grid);
+    flip(B_VISC_FUSED, 0, [&](const KParams &P) { return !vwalk(P); }, rows(-1));
+    stage(B_SUBSTEP3, 0, [&](const KParams &P) { return !vwalk(P) && P.adiabatic; }, rows(0));
+    // the boundary call before the transport with the FARGO ring means
+    stage(B_GHOST_MEAN, 0, all, [](std::vector<BatchEntry> &v, unsigned d, const KParams &P) {
+        add_grid(v, d, (unsigned)((P.ns + 255) / 256 + P.nr), 1);
+    });
     stage(B_RADIAL_ISO, 0, iso, radial);
     stage(B_RADIAL_ADI, 0, adi, radial);
-    stage(B_RING_MEAN, 1, all, rings);
     stage(B_AZIM_ISO, 0, iso, azim);
     stage(B_AZIM_ADI, 0, adi, azim);
-    // the velocity kernel writes the fields of every disk
-    stage_v(B_VELOCITIES, 0, all, rows(0), false, true);
+    // the velocity kernel with the final boundary call and the derived
+    // fields writes the fields of every disk
+    stage_v(B_VEL_FINAL, 2, all, rows(0), false, true);
     std::fill(view.begin(), view.end(), 0u);
-    stage(B_DAMPING, 0, [](const KParams &P) { return P.damping != 0; }, damp);
-    stage(B_FINAL_DERIVED, 2, all, rows(0));
+    (void)ghosts;
+    (void)cfl;
+    (void)damp;
     return st;
 }
 
@@ -3509,6 +3583,38 @@ void launch_batch_stage(const KParams *Ps, const BatchStageDev &S, int split, in
     case B_DAMPING: LAUNCHB(kb_damping, kBlock); break;
     case B_DERIVED: LAUNCHB(kb_derived, kBlock); break;
     case B_FINAL_DERIVED: LAUNCHB(kb_final_derived, kBlock, S.arg); break;
+    case B_CFL_RINGS: LAUNCHB(kb_cfl_rings, dim3(512)); break;
+    case B_CFL_FINAL_RINGS:
+        hipLaunchKernelGGL(kb_cfl_finalize_rings, dim3(S.n), b256, 0, st, Ps, S.map, S.arg);
+        break;
+    case B_GHOST_MEAN: LAUNCHB(kb_ghost_ring_mean, b256); break;
+    case B_VEL_FINAL: LAUNCHB(kb_velocities_final, kBlock, S.arg); break;
+    case B_KICK_WALK: {
+        const dim3 g(S.n);
+#define KWB(SP, AD, SN)                                                                            hipLaunchKernelGGL((kb_kick_walk<SP, AD, SN>), g, kBlock, 0, st, Ps, S.map)
+        const int v = (split ? 4 : 0) + S.arg;
+        if (v == 0) KWB(false, false, false);
+        else if (v == 1) KWB(false, false, true);
+        else if (v == 2) KWB(false, true, false);
+        else if (v == 3) KWB(false, true, true);
+        else if (v == 4) KWB(true, false, false);
+        else if (v == 5) KWB(true, false, true);
+        else if (v == 6) KWB(true, true, false);
+        else KWB(true, true, true);
+#undef KWB
+        break;
+    }
+    case B_VISC_WALK: {
+        const dim3 g(S.n);
+        if (split) {
+            if (S.arg) hipLaunchKernelGGL((kb_visc_walk<true, true>), g, kBlock, 0, st, Ps, S.map);
+            else hipLaunchKernelGGL((kb_visc_walk<true, false>), g, kBlock, 0, st, Ps, S.map);
+        } else {
+            if (S.arg) hipLaunchKernelGGL((kb_visc_walk<false, true>), g, kBlock, 0, st, Ps, S.map);
+            else hipLaunchKernelGGL((kb_visc_walk<false, false>), g, kBlock, 0, st, Ps, S.map);
+        }
+        break;
+    }
     case B_RADIAL_ISO:
     case B_RADIAL_ADI: {
         const bool a = S.kind == B_RADIAL_ADI, w2 = radial_waves == 2;

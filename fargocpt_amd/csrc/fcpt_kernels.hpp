@@ -55,6 +55,9 @@ void launch_visc_energy(const KParams &P, hipStream_t st);
 // velocities to the view's partner buffers; then launch_visc_energy
 // (adiabatic) or launch_visc_fused on the swapped view
 bool kick_walk_applies(const KParams &P);
+// the rings per thread the walks use for this disk (1: the cell-parallel form)
+int resolve_kick_tile(const KParams &P);
+int resolve_visc_tile(const KParams &P);
 void launch_kick(const KParams &P, hipStream_t st, int fold_mode);
 // the potential; fold_mode >= 0: in the same launch, the CFL fold of
 // k_cfl_rings' slots (launch_cfl_cells) with that mode
@@ -100,7 +103,8 @@ enum BatchKind {
     B_RING_MEAN, B_CFL_CELLS, B_CFL_FINAL, B_POT_BG, B_POTENTIAL, B_SRC_VR, B_SRC_VPHI,
     B_SRC_ENERGY, B_TW_Q, B_TW_VEL, B_SN_Q, B_SN_VEL, B_TEMP_FLOOR, B_RECALC_VISC, B_STRESS,
     B_VISC_UPDATE, B_SUBSTEP3, B_GHOST, B_RADIAL_ISO, B_RADIAL_ADI, B_AZIM_ISO, B_AZIM_ADI,
-    B_VELOCITIES, B_DAMPING, B_DERIVED, B_SRC_FUSED, B_AV_FUSED, B_VISC_FUSED, B_FINAL_DERIVED
+    B_VELOCITIES, B_DAMPING, B_DERIVED, B_SRC_FUSED, B_AV_FUSED, B_VISC_FUSED, B_FINAL_DERIVED,
+    B_CFL_RINGS, B_CFL_FINAL_RINGS, B_KICK_WALK, B_VISC_WALK, B_GHOST_MEAN, B_VEL_FINAL
 };
 struct BatchStageHost {
     int kind, arg;

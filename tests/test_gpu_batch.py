@@ -1,7 +1,8 @@
 """Batched disks (fcpt_batch_*): several disks of different sizes and physics
 advanced by one launch per step stage must evolve bit for bit like the same
 disks stepped alone with fcpt_run_steps (the same blocks do the same
-arithmetic), in FP64 and in split storage."""
+arithmetic, with the walks where the disk's own launches use them), in FP64
+and in split storage."""
 import numpy as np
 import pytest
 
@@ -40,6 +41,34 @@ def test_batch_matches_single_disks(prec):
     b = Batch([d for d, _ in batched])
     b.run_steps(5, True)
     b.run_steps(10, False)
+    b.synchronize()
+    for (s, meta), (d, _) in zip(single, batched):
+        for f in FIELDS:
+            if f in ("ENERGY", "QPLUS") and not meta["adiabatic"]:
+                continue
+            assert np.array_equal(s.download_field(f), d.download_field(f)), (meta["Nr"], f)
+        assert s.get_time() == d.get_time()
+    b.close()
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_batch_walks_match_single_disks(prec):
+    """The fixture disks with the walks forced on (3 rings per thread for
+    the kick and viscous walks, ragged last tiles): the batch runs the walks'
+    batched forms where each disk's own launch runs the walks, and every disk
+    evolves bit for bit like it does alone."""
+    from fargocpt_amd.capi import Batch
+    print(f"batch of {len(CASES)} fixture disks with walks, precision {prec}", flush=True)
+    single = _fixture_disks(prec)
+    batched = _fixture_disks(prec)
+    for d, _ in single + batched:
+        d.set_kick_tile(3)
+        d.set_visc_tile(3)
+    for d, _ in single:
+        d.run_steps(12, True)
+        d.synchronize()
+    b = Batch([d for d, _ in batched])
+    b.run_steps(12, True)
     b.synchronize()
     for (s, meta), (d, _) in zip(single, batched):
         for f in FIELDS:
