@@ -966,7 +966,7 @@ static void enqueue_euler(const KParams &P, hipStream_t st, int count_monitor, b
     } else {
         launch_boundaries(*cur, st, 0, 0);
         launch_transport_to(*cur, P, st);
-        launch_final_boundaries_derived(P, st, count_monitor ? 2 : 1, false);
+        launch_final_boundaries_derived(P, st, count_monitor ? 2 : 1);
     }
 }
 
@@ -994,7 +994,7 @@ static void enqueue_leapfrog(const KParams &P, hipStream_t st, int count_monitor
         launch_copy_velocities(*cur, st);
     }
     launch_leapfrog_phase(P, st, 3);
-    launch_final_boundaries_derived(P, st, count_monitor ? 2 : 1, false);
+    launch_final_boundaries_derived(P, st, count_monitor ? 2 : 1);
 }
 
 static void enqueue_step(const KParams &P, hipStream_t st, int count_monitor, bool fused)

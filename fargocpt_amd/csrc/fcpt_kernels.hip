@@ -2617,11 +2617,9 @@ __device__ __forceinline__ void b_velocities(const KParams &P, int bx, int by)
 }
 FCPT_PAIR0(velocities, 256)
 
-// the velocity kernel with the damping of the final boundary call applied to
-// each value as it is stored (rows 0 .. nr; row nr holds v_r only, which
-// only the damping touches); a value is damped as a later load of it
-// returns it, so the fields are those of the velocity kernel followed by the
-// damping kernel
+// the damping of the final boundary call applied to a value the velocity
+// kernel stores: the value is damped as a later load of it returns it, so
+// the field is that of the velocity kernel followed by the damping kernel
 template <bool SPLIT>
 __device__ __forceinline__ double damp_stored(const KParams &P, const Fld<SPLIT> &X, int f0, bool vector,
                                               int i, int j, double dt, double x)
@@ -2636,51 +2634,14 @@ __device__ __forceinline__ double damp_stored(const KParams &P, const Fld<SPLIT>
     return x;
 }
 
-template <bool SPLIT>
-__global__ void __launch_bounds__(256) k_velocities_damped(KParams P)
-{
-    const int bx = (int)blockIdx.x, by = (int)blockIdx.y;
-    CELL_LOOP_HEAD(0, P.nr + 1)
-    FLD(vr, FCPT_VRAD);
-    const double dt = P.s->dt;
-    if (i == P.nr) {
-        if (damp_in_outer(P, true, i) || damp_in_inner(P, true, i))
-            vr.set(i, j, damp_stored<SPLIT>(P, vr, FCPT_VRAD0, true, i, j, dt, vr(i, j)));
-        return;
-    }
-    const Fld<false> Sb = buf<false>(P, P.sig_c, nullptr);
-    const Fld<SPLIT> m0 = buf<SPLIT>(P, P.mom[0], P.bgm[0]);
-    const Fld<SPLIT> m1 = buf<SPLIT>(P, P.mom[1], P.bgm[1]);
-    const Fld<SPLIT> m2 = buf<SPLIT>(P, P.mom[2], P.bgm[2]);
-    const Fld<SPLIT> m3 = buf<SPLIT>(P, P.mom[3], P.bgm[3]);
-    const Fld<false> Eb = buf<false>(P, P.en_c, nullptr);
-    FLD(vaz, FCPT_VAZI);
-    FLD(S, FCPT_SIGMA);
-    FLD(E, FCPT_ENERGY);
-    const int jm = j == 0 ? ns - 1 : j - 1;
-    const int im = i >= 1 ? i - 1 : 0;
-    const double s0 = Sb(i, j), sm = Sb(im, j), sjm = Sb(i, jm);
-    const double m0v = m0(im, j), m1v = m1(i, j), m2v = m2(i, jm), m3v = m3(i, j);
-    const double e = P.adiabatic ? Eb(i, j) : 0.0;
-    const double v_r = i == 0 ? 0.0 : (m0v + m1v) / (sm + s0);
-    vr.set(i, j, damp_stored<SPLIT>(P, vr, FCPT_VRAD0, true, i, j, dt, v_r));
-    const double v_phi = (m2v + m3v) / (sjm + s0) * cst(P.InvRmed)[i] - cst(P.Rmed)[i] * P.omega_frame;
-    vaz.set(i, j, damp_stored<SPLIT>(P, vaz, FCPT_VAZI0, false, i, j, dt, v_phi));
-    const double floor_value = P.sigma_floor * P.sigma0;
-    double sig = s0;
-    if (sig < floor_value) sig = floor_value;
-    S.set(i, j, damp_stored<SPLIT>(P, S, FCPT_SIGMA0, false, i, j, dt, sig));
-    if (P.adiabatic)
-        E.set(i, j, damp_stored<SPLIT>(P, E, FCPT_ENERGY0, false, i, j, dt, temperature_floor(P, e, sig)));
-}
-
 // The end of the fused Euler step in one launch: the velocity kernel, the
 // final boundary call (damping, ghost rings, the clock) and
 // recalculate_derived_disk_quantities.  A cell forms its velocities, Sigma
-// and E as k_velocities_damped stores them and its derived fields from the
-// stored values, as k_final_derived reads them.  The ghost rings copy their
-// neighbour ring, which another thread stores, so a ghost-ring thread forms
-// the neighbour's values itself (the same expressions on the same inputs):
+// and E as the velocity kernel and the damping store them, and its derived
+// fields from the stored values, as k_final_derived reads them.  The ghost
+// rings copy their neighbour ring, which another thread stores, so a
+// ghost-ring thread forms the neighbour's values itself (the same
+// expressions on the same inputs):
 // ring 0 takes Sigma and E of ring 1 and v_r from v_r(2), ring nr-1 those of
 // ring nr-2; v_r(1) is a ghost value too, so the ring-1 thread stores that
 // one instead of its own, and no cell is stored by two threads.
@@ -3219,9 +3180,9 @@ void launch_derived(const KParams &P, hipStream_t st) { LAUNCH(k_derived, cell_g
 
 // the final boundary call (damping, ghost rings, the clock) and the derived
 // fields: damping, then one pass for the rest
-void launch_final_boundaries_derived(const KParams &P, hipStream_t st, int advance_time, bool damped)
+void launch_final_boundaries_derived(const KParams &P, hipStream_t st, int advance_time)
 {
-    if (P.damping && !damped) {
+    if (P.damping) {
         const int n_inner = (P.damp_ids[0] > P.damp_ids[1] ? P.damp_ids[0] : P.damp_ids[1]) + 1;
         const int o0 = P.damp_ids[2] < P.damp_ids[3] ? P.damp_ids[2] : P.damp_ids[3];
         const int n_outer = P.nr + 1 - o0;

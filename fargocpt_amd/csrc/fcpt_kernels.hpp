@@ -29,8 +29,7 @@ void launch_transport(const KParams &P, hipStream_t st);
 void launch_set_mid_params(const KParams &P, hipStream_t st, const fcpt_step_params &sp);
 void launch_leapfrog_phase(const KParams &P, hipStream_t st, int phase);
 void launch_pressure(const KParams &P, hipStream_t st);
-// damped: the damping was applied by the transport (launch_transport_ghost)
-void launch_final_boundaries_derived(const KParams &P, hipStream_t st, int advance_time, bool damped);
+void launch_final_boundaries_derived(const KParams &P, hipStream_t st, int advance_time);
 void launch_copy_velocities(const KParams &P, hipStream_t st);
 void launch_fill_rows(const KParams &P, hipStream_t st, void *dst, const double *ring,
                       const double *bg, int rows);
