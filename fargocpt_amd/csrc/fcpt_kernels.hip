@@ -47,6 +47,41 @@ __device__ __forceinline__ double vl_lim(double a, double b)
     }
 }
 
+// Quotients n / d that share one denominator d.  The compiler's IEEE FP64
+// division on gfx950 is v_div_scale (range scaling of d and n), v_rcp, two
+// Newton steps on the reciprocal, the quotient, one FMA correction
+// (v_div_fmas) and v_div_fixup (zeros, infinities, NaNs, out-of-range
+// results).  For |d| in [2^-250, 2^250) and |n| in [2^-400, 2^250) neither
+// the scaling nor the fix-up changes anything, so the same reciprocal and
+// correction steps written out give the same bits, and the reciprocal part
+// (five instructions) is formed once per denominator: a quotient then costs
+// three.  Other operands (zero aside, which returns the signed zero n * r)
+// take the division itself.
+struct SharedDen {
+    double d, r;
+    bool ok;
+};
+__device__ __forceinline__ SharedDen shared_den(double d)
+{
+    const double r0 = __builtin_amdgcn_rcp(d);
+    const double e0 = fma(-d, r0, 1.0);
+    const double r1 = fma(r0, e0, r0);
+    const double e1 = fma(-d, r1, 1.0);
+    const double r2 = fma(r1, e1, r1);
+    const unsigned ed = (unsigned)((unsigned long long)__double_as_longlong(d) >> 52) & 0x7ffu;
+    return SharedDen{d, r2, ed - 773u < 500u};
+}
+__device__ __forceinline__ double shared_div(double n, const SharedDen &D)
+{
+    const double q0 = n * D.r;
+    const double rem = fma(-D.d, q0, n);
+    double q = fma(rem, D.r, q0);
+    const unsigned en = (unsigned)((unsigned long long)__double_as_longlong(n) >> 52) & 0x7ffu;
+    if (n == 0.0) q = q0;
+    else if (!D.ok || en - 623u >= 650u) q = n / D.d;
+    return q;
+}
+
 // FP64 reciprocal: one IEEE division, or in split storage the hardware
 // reciprocal refined by two Newton steps (error a few ulp)
 template <bool SPLIT>
@@ -2273,10 +2308,11 @@ __device__ __forceinline__ void decode_ring(const KParams &P, const RingRaw<SPLI
     R.v = (r >= 0 && r <= P.nr) ? v0 : 0.0;
     R.s = in ? sig : 1.0;
     const double inv_sig = SPLIT ? recip<true>(sig) : 0.0;
+    const SharedDen dsig = shared_den(sig);
 #pragma unroll
     for (int k = 0; k < NQ; ++k) {
         R.q[k] = in ? q[k] : 0.0;
-        const double w = SPLIT ? q[k] * inv_sig : q[k] / sig;
+        const double w = SPLIT ? q[k] * inv_sig : shared_div(q[k], dsig);
         R.w[k] = in ? w : 0.0;
     }
 }
@@ -2550,13 +2586,16 @@ __device__ __forceinline__ void b_azimuthal_sweep(const KParams &P, int bx, int 
     // the other quantities
     const D2 inv_s0{SPLIT ? recip<true>(s0.a) : 0.0, SPLIT ? recip<true>(s0.b) : 0.0};
     const D2 inv_s1{SPLIT ? recip<true>(s1.a) : 0.0, SPLIT ? recip<true>(s1.b) : 0.0};
+    const SharedDen s0a = shared_den(s0.a), s0b = shared_den(s0.b);
+    const SharedDen s1a = shared_den(s1.a), s1b = shared_den(s1.b);
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
-        const D2 w0{SPLIT ? q0[q].a * inv_s0.a : q0[q].a / s0.a, SPLIT ? q0[q].b * inv_s0.b : q0[q].b / s0.b};
+        const D2 w0{SPLIT ? q0[q].a * inv_s0.a : shared_div(q0[q].a, s0a),
+                    SPLIT ? q0[q].b * inv_s0.b : shared_div(q0[q].b, s0b)};
         const D2 qs0 = star2(w0, slope2<SPLIT>(w0, invdxtheta), dxtheta, ks0);
         const D2 vq0 = flux_div2(D2{dxrad * qs0.a * ds0.a * v0.a, dxrad * qs0.b * ds0.b * v0.b});
         const D2 q1{q0[q].a + vq0.a * invsurf, q0[q].b + vq0.b * invsurf};
-        const D2 w1{SPLIT ? q1.a * inv_s1.a : q1.a / s1.a, SPLIT ? q1.b * inv_s1.b : q1.b / s1.b};
+        const D2 w1{SPLIT ? q1.a * inv_s1.a : shared_div(q1.a, s1a), SPLIT ? q1.b * inv_s1.b : shared_div(q1.b, s1b)};
         const D2 qs1 = star2(w1, slope2<SPLIT>(w1, invdxtheta), dxtheta, ksu2);
         const D2 vq1 = flux_div2(D2{dxrad * qs1.a * ds1.a * vu, dxrad * qs1.b * ds1.b * vu});
         const D2 q2{q1.a + vq1.a * invsurf, q1.b + vq1.b * invsurf};
