@@ -386,6 +386,7 @@ int fcpt_create(const fcpt_disk_desc *desc, fcpt_ctx **out)
     if (!rc) rc = dev_alloc(c, &P.iso_cs, (size_t)nr + 2);
     if (!rc) rc = dev_alloc(c, &P.iso_h, (size_t)nr + 2);
     if (!rc) rc = dev_alloc(c, &P.iso_nu, (size_t)nr + 2);
+    if (!rc) rc = dev_alloc(c, &P.ring_tab, (size_t)kRingTabs * ((size_t)nr + 2));
     if (!rc) rc = dev_alloc(c, &P.vmean, (size_t)nr + 1);
     if (!rc) rc = dev_alloc(c, &P.vuni, (size_t)nr + 1);
     if (!rc) rc = dev_alloc(c, &P.nshift, (size_t)nr + 1);
@@ -421,8 +422,9 @@ int fcpt_create(const fcpt_disk_desc *desc, fcpt_ctx **out)
         e = e == hipSuccess ? hipMemset(P.bgm[k], 0, ((size_t)nr + 2) * sizeof(double)) : e;
     std::memset(c->s_host, 0, sizeof(DevScalars));
     if (e == hipSuccess) e = hipMemcpy(c->s_dev, c->s_host, sizeof(DevScalars), hipMemcpyHostToDevice);
-    if (e == hipSuccess && !P.adiabatic) {
-        launch_iso_rings(P, c->stream);
+    if (e == hipSuccess) {
+        launch_ring_tables(P, c->stream);
+        if (!P.adiabatic) launch_iso_rings(P, c->stream);
         e = hipGetLastError();
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     }

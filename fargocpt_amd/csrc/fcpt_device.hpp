@@ -16,6 +16,8 @@ namespace fcpt {
 
 constexpr int kMaxPlanets = FCPT_MAX_PLANETS;
 constexpr unsigned kCflMaxBlocks = 4096;  // slots of the block-minimum array
+// rows of KParams::ring_tab
+enum { TAB_SRC_INVDXTHETA = 0, TAB_INV_DXTHETA, TAB_INV_OMEGA_K, TAB_OMEGA_K, TAB_VISC_AREA, kRingTabs };
 
 struct DevPlanet {
     double x, y, mass, r_cubic;
@@ -93,6 +95,10 @@ struct KParams {
     void *valt[2];       // partners of v_r and v_phi for the fused step kernels
     double *bgm[4];      // backgrounds of the four momenta (split storage)
     double *iso_cs, *iso_h, *iso_nu;  // locally isothermal ring profiles of c_s, H, nu
+    // ring-uniform quotients of the source terms, the artificial viscosity,
+    // the viscous update and the derived fields (k_ring_tables), kRingTabs
+    // rows of nr + 2 values
+    double *ring_tab;
     double *vmean;       // per-ring mean azimuthal velocity
     double *vuni;        // per-ring uniform residual velocity (FARGO)
     int *nshift;         // per-ring integer shift

@@ -115,6 +115,7 @@ __device__ __forceinline__ double omega_kepler(const KParams &P, double r)
     return sqrt(P.G * P.mstar / (r * r * r));  // Theo.cpp:246-249
 }
 
+
 // Per-ring arrays (metric, backgrounds, ring means, shifts) are never written
 // by the kernel that reads them.  Reading them through the constant address
 // space says so: with a wave-uniform index they become scalar loads, which
@@ -160,6 +161,13 @@ struct Fld {
         else return x;
     }
 };
+
+// a ring-uniform quotient of ring i (row t of KParams::ring_tab, formed once
+// by k_ring_tables with the expression its users had inline: the same bits)
+__device__ __forceinline__ double ring_tab(const KParams &P, int t, int i)
+{
+    return cst(P.ring_tab)[(size_t)t * (size_t)(P.nr + 2) + (size_t)i];
+}
 
 template <bool SPLIT>
 __device__ __forceinline__ Fld<SPLIT> fld(const KParams &P, int f)
@@ -717,7 +725,7 @@ __device__ __forceinline__ double src_vphi_new(const KParams &P, int i, double d
                                                double sjp, double p0, double pjp, double phi0,
                                                double phijp)
 {
-    const double invdxtheta = 2.0 / (P.dphi * (cst(P.Rsup)[i] + cst(P.Rinf)[i]));
+    const double invdxtheta = ring_tab(P, TAB_SRC_INVDXTHETA, i);  // 2 / (dphi (Rsup + Rinf))
     const double gradp = 2.0 / (s0 + sjp) * (p0 - pjp) * invdxtheta;
     const double gradphi = (phi0 - phijp) * invdxtheta;
     return va + dt * (-gradp - gradphi);
@@ -841,7 +849,7 @@ __device__ __forceinline__ AdiDerived adi_derived(const KParams &P, int i, doubl
     AdiDerived d;
     const double g = P.gamma;
     d.cs = sqrt(g * (g - 1.0) * e / sig);
-    const double inv_omega_kepler = 1.0 / omega_kepler(P, cst(P.Rmed)[i]);
+    const double inv_omega_kepler = ring_tab(P, TAB_INV_OMEGA_K, i);  // 1 / Omega_K(Rmed)
     d.h = d.cs / (sqrt(g)) * inv_omega_kepler;
     const double alpha = P.alpha;
     d.nu = alpha * d.h * d.cs;
@@ -1025,8 +1033,7 @@ __device__ __forceinline__ double sn_dissipation(const KParams &P, int i, const 
                                                  double dt, double e)
 {
     if (i >= 1 && i < P.nr - 1) {
-        const double dxtheta = P.dphi * cst(P.Rmed)[i];
-        const double invdxtheta = 1.0 / dxtheta;
+        const double invdxtheta = ring_tab(P, TAB_INV_DXTHETA, i);  // 1 / (dphi Rmed)
         e = e - dt * q.qr * q.dv_r * cst(P.InvDiffRsup)[i] - dt * q.qp * q.dv_phi * invdxtheta;
     }
     return temperature_floor(P, e, sig);
@@ -1064,8 +1071,7 @@ __device__ __forceinline__ double sn_vr_new(const KParams &P, int i, double dt, 
 __device__ __forceinline__ double sn_vphi_new(const KParams &P, int i, double dt, double va, double s0,
                                               double sjp, double qp0, double qpj)
 {
-    const double dxtheta = P.dphi * cst(P.Rmed)[i];
-    const double invdxtheta = 1.0 / dxtheta;
+    const double invdxtheta = ring_tab(P, TAB_INV_DXTHETA, i);  // 1 / (dphi Rmed)
     return va - dt * 2.0 / (s0 + sjp) * (qp0 - qpj) * invdxtheta;
 }
 
@@ -1161,6 +1167,30 @@ __device__ __forceinline__ void b_iso_profile(const KParams &P, int bx, int by)
 }
 FCPT_PAIR0(iso_profile, 256)
 
+// the ring-uniform quotients of KParams::ring_tab, each with the expression
+// its users had inline (the same bits): 2 / (dphi (Rsup + Rinf)) of the
+// azimuthal source term, 1 / (dphi Rmed) of the SN viscosity, 1 / Omega_K
+// and Omega_K at Rmed, and 2 / (Rinf(i+1)^2 - Rinf(i)^2) of the viscous v_phi
+// update
+__global__ void __launch_bounds__(256) k_ring_tables(KParams P, double *tab)
+{
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= P.nr) return;
+    const size_t w = (size_t)P.nr + 2;
+    tab[TAB_SRC_INVDXTHETA * w + i] = 2.0 / (P.dphi * (P.Rsup[i] + P.Rinf[i]));
+    const double dxtheta = P.dphi * P.Rmed[i];
+    tab[TAB_INV_DXTHETA * w + i] = 1.0 / dxtheta;
+    tab[TAB_INV_OMEGA_K * w + i] = 1.0 / omega_kepler(P, P.Rmed[i]);
+    tab[TAB_OMEGA_K * w + i] = omega_kepler(P, P.Rmed[i]);
+    const double a1 = P.Rinf[i + 1] * P.Rinf[i + 1], a0 = P.Rinf[i] * P.Rinf[i];
+    tab[TAB_VISC_AREA * w + i] = 2.0 / (a1 - a0);
+}
+
+void launch_ring_tables(const KParams &P, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_ring_tables, dim3((unsigned)((P.nr + 255) / 256)), dim3(256), 0, st, P, P.ring_tab);
+}
+
 // the locally isothermal ring profiles c_s, H and nu = alpha H c_s, with the
 // expressions of the 2-D kernels above (k_iso_profile, derived_store)
 __global__ void __launch_bounds__(256) k_iso_rings(KParams P)
@@ -1249,7 +1279,7 @@ __device__ __forceinline__ double visc_vphi_new(const KParams &P, int i, double 
     const double sigma_avg = 0.5 * (s0 + sjp);
     const double a1 = cst(P.Rinf)[i + 1] * cst(P.Rinf)[i + 1], a0 = cst(P.Rinf)[i] * cst(P.Rinf)[i];
     const double dVp = dt * cst(P.InvRmed)[i] / (sigma_avg) *
-                       ((2.0 / (a1 - a0)) * (a1 * trp1 - a0 * trp0) + (tpp0 - tppj) * P.invdphi);
+                       (ring_tab(P, TAB_VISC_AREA, i) * (a1 * trp1 - a0 * trp0) + (tpp0 - tppj) * P.invdphi);
     return va + dVp;
 }
 
@@ -1716,8 +1746,7 @@ __device__ __forceinline__ double cooling_beta_cell(const KParams &P, int i, int
                                                     double sig)
 {
     if (!P.cooling_beta_enabled) return 0.0;
-    const double r = cst(P.Rmed)[i];
-    const double omega_k = omega_kepler(P, r);
+    const double omega_k = ring_tab(P, TAB_OMEGA_K, i);  // Omega_K(Rmed)
     double beta_inv = 1 / P.cooling_beta;
     if (P.cooling_beta_ramp_up > 0.0) {
         const double x = 2 * time / P.cooling_beta_ramp_up;

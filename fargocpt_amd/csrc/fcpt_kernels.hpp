@@ -69,6 +69,8 @@ void launch_derived(const KParams &P, hipStream_t st);
 void launch_iso_profile(const KParams &P, hipStream_t st);
 // the locally isothermal ring profiles of c_s, H and nu (geometry and parameters only)
 void launch_iso_rings(const KParams &P, hipStream_t st);
+// KParams::ring_tab from the metric arrays (once, at context creation)
+void launch_ring_tables(const KParams &P, hipStream_t st);
 void launch_totals(const KParams &P, hipStream_t st, double radius, double *ring_scratch,
                    double *out2);
 // split storage: deviations of `field` (rows x nphi floats) re-expressed from
