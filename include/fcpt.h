@@ -250,10 +250,13 @@ int fcpt_run_steps(fcpt_ctx *ctx, int32_t n_steps, int32_t use_graph);
    FCPT_KERNEL_DISK_FORCE times the launches of fcpt_disk_on_planet_accel
    (ring means, cell reduction, block reduction; no copy to the host) for two
    bodies, a star at the origin and a planet at (1, 0), with the self-gravity
-   correction.  FCPT_KERNEL_VISC_FUSED is the fused viscous stress + velocity
-   pass of the step (FCPT_PARAM_FUSED_STEP); it writes only the velocities'
-   partner buffers (and, in an adiabatic disk with viscous heating, the
-   stress tensor). */
+   correction.  FCPT_KERNEL_CFL is the per-ring CFL kernel (ring means of
+   v_phi and the cells' limits; the fold of the ring minima is not timed).
+   FCPT_KERNEL_VISC_FUSED is the fused viscous stress + velocity pass of the
+   step (FCPT_PARAM_FUSED_STEP) without SubStep3: the viscous walk, or the
+   cell-parallel pass where the viscous tile is 1; it writes only the
+   velocities' partner buffers (and, in an adiabatic disk with viscous
+   heating, the stress tensor). */
 enum { FCPT_KERNEL_RADIAL_SWEEP = 0, FCPT_KERNEL_AZIMUTHAL = 1, FCPT_KERNEL_CFL = 3,
        FCPT_KERNEL_STRESS = 4, FCPT_KERNEL_POTENTIAL = 5, FCPT_KERNEL_DISK_FORCE = 6,
        FCPT_KERNEL_VISC_FUSED = 7 };
