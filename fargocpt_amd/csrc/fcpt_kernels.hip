@@ -636,6 +636,27 @@ __global__ void __launch_bounds__(256) kb_potential_bg(const KParams *Ps, const 
 
 // CalculateNbodyPotential, Pframeforce.cpp:44-85 (+ compute_smoothing,
 // Force.cpp:145-159), at cell (i, j) with scale height H
+// one body's term at (x, y) with smoothing length `smooth`
+template <bool SPLIT>
+__device__ __forceinline__ double potential_body(const KParams &P, double x, double y, double smooth,
+                                                 double px, double py, double mass, double r_sm)
+{
+    const double dx = x - px;
+    const double dy = y - py;
+    const double dist_2 = dx * dx + dy * dy;
+    const double d2 = dist_2 + smooth * smooth;
+    // split storage: one reciprocal square root instead of sqrt and division
+    const double inv_d = SPLIT ? inv_sqrt<true>(d2) : 0.0;
+    const double d_smoothed = SPLIT ? d2 * inv_d : sqrt(d2);
+    double smooth_factor_klahr = 1.0;
+    if (r_sm > 0.0 && d_smoothed < r_sm) {
+        const double q = d_smoothed / r_sm;
+        smooth_factor_klahr = (pow(q, 4.0) - 2.0 * pow(q, 3.0) + 2.0 * d_smoothed / r_sm);
+    }
+    if constexpr (SPLIT) return -P.G * mass * inv_d * smooth_factor_klahr;
+    else return -P.G * mass / d_smoothed * smooth_factor_klahr;
+}
+
 template <bool SPLIT>
 __device__ __forceinline__ double potential_cell(const KParams &P, int i, int j, double H)
 {
@@ -646,24 +667,52 @@ __device__ __forceinline__ double potential_cell(const KParams &P, int i, int j,
     const int np = s->n_planets;
     for (int k = 0; k < np; ++k) {
         const double smooth = P.thickness_smoothing * H;
-        const double dx = x - s->planet[k].x;
-        const double dy = y - s->planet[k].y;
-        const double dist_2 = dx * dx + dy * dy;
-        const double d2 = dist_2 + smooth * smooth;
-        // split storage: one reciprocal square root instead of sqrt and division
-        const double inv_d = SPLIT ? inv_sqrt<true>(d2) : 0.0;
-        const double d_smoothed = SPLIT ? d2 * inv_d : sqrt(d2);
-        double smooth_factor_klahr = 1.0;
-        const double r_sm = s->planet[k].r_cubic;
-        if (r_sm > 0.0 && d_smoothed < r_sm) {
-            const double q = d_smoothed / r_sm;
-            smooth_factor_klahr = (pow(q, 4.0) - 2.0 * pow(q, 3.0) + 2.0 * d_smoothed / r_sm);
-        }
-        if constexpr (SPLIT) acc += -P.G * s->planet[k].mass * inv_d * smooth_factor_klahr;
-        else acc += -P.G * s->planet[k].mass / d_smoothed * smooth_factor_klahr;
+        acc += potential_body<SPLIT>(P, x, y, smooth, s->planet[k].x, s->planet[k].y, s->planet[k].mass,
+                                     s->planet[k].r_cubic);
     }
     acc += -s->indirect_x * x - s->indirect_y * y;
     return acc;
+}
+
+// The same for kPotCells cells of ring i, sectors j0 + 64 c (those below ns),
+// with the bodies' loop outside the cells': every cell still adds the bodies
+// in order and then the indirect term, and a body's parameters are loaded
+// once per kPotCells cells.  Split storage: the deviations from the ring
+// background b.
+constexpr int kPotCells = 4;
+template <bool SPLIT>
+__device__ __forceinline__ void potential_ring_cells(const KParams &P, int i, int j0, double b)
+{
+    const DevScalars *s = P.s;
+    const int ns = P.ns;
+    double x[kPotCells], y[kPotCells], sm[kPotCells], acc[kPotCells];
+#pragma unroll
+    for (int c = 0; c < kPotCells; ++c) {
+        const int j = min(j0 + 64 * c, ns - 1);  // past ns: formed, not stored
+        x[c] = cst(P.Rmed)[i] * P.cosphi[j];
+        y[c] = cst(P.Rmed)[i] * P.sinphi[j];
+        const double H = P.adiabatic ? fld<SPLIT>(P, FCPT_SCALE_HEIGHT)(i, j) : cst(P.iso_h)[i];
+        sm[c] = P.thickness_smoothing * H;
+        acc[c] = 0.0;
+    }
+    const int np = s->n_planets;
+    for (int k = 0; k < np; ++k) {
+        const double px = s->planet[k].x, py = s->planet[k].y;
+        const double mass = s->planet[k].mass, r_sm = s->planet[k].r_cubic;
+#pragma unroll
+        for (int c = 0; c < kPotCells; ++c) acc[c] += potential_body<SPLIT>(P, x[c], y[c], sm[c], px, py, mass, r_sm);
+    }
+    const double ix = s->indirect_x, iy = s->indirect_y;
+#pragma unroll
+    for (int c = 0; c < kPotCells; ++c) {
+        const int j = j0 + 64 * c;
+        if (j >= ns) break;
+        const double a = acc[c] + (-ix * x[c] - iy * y[c]);
+        if constexpr (SPLIT)
+            static_cast<float *>(P.f[FCPT_POTENTIAL])[(size_t)i * (size_t)ns + (size_t)j] = (float)(a - b);
+        else
+            fld<SPLIT>(P, FCPT_POTENTIAL).set(i, j, a);
+    }
 }
 
 template <bool SPLIT>
@@ -675,30 +724,28 @@ __device__ __forceinline__ void b_potential(const KParams &P, int bx, int by)
 }
 FCPT_PAIR0(potential, 256)
 
-// The potential of the fused Euler step in one launch: in split storage each
-// wavefront forms its ring's background itself (lane 0 of the first column
-// stores it), and one extra block row runs the CFL fold of the step's start
-// (k_cfl_finalize_rings; the potential reads none of the scalars the fold
-// writes, and dt is first read by the kernels after this one).
+// The potential of a single disk in one launch, kPotCells cells per thread:
+// in split storage each wavefront forms its ring's background itself (lane 0
+// of the first column stores it); fold_mode >= 0: one extra block row runs
+// the CFL fold of the step's start (k_cfl_finalize_rings; the potential reads
+// none of the scalars the fold writes, and dt is first read by the kernels
+// after this one).
 template <bool SPLIT>
 __global__ void __launch_bounds__(256) k_potential_fold(KParams P, int fold_mode)
 {
     // block row 0 is the fold (dispatched first, it ends under the others)
     const int bx = (int)blockIdx.x, by = (int)blockIdx.y - 1;
     if (by < 0) {
-        if (bx == 0) b_cfl_fold(P, fold_mode, (int)(threadIdx.y * blockDim.x + threadIdx.x), true);
+        if (bx == 0 && fold_mode >= 0)
+            b_cfl_fold(P, fold_mode, (int)(threadIdx.y * blockDim.x + threadIdx.x), true);
         return;
     }
-    CELL_LOOP_HEAD(0, P.nr)
-    const double H = P.adiabatic ? fld<SPLIT>(P, FCPT_SCALE_HEIGHT)(i, j) : cst(P.iso_h)[i];
-    const double acc = potential_cell<SPLIT>(P, i, j, H);
-    if constexpr (SPLIT) {
-        const double b = potential_bg_value(P, i);
-        if (bx == 0 && threadIdx.x == 0) P.bg[FCPT_POTENTIAL][i] = b;
-        static_cast<float *>(P.f[FCPT_POTENTIAL])[(size_t)i * (size_t)ns + (size_t)j] = (float)(acc - b);
-    } else {
-        fld<SPLIT>(P, FCPT_POTENTIAL).set(i, j, acc);
-    }
+    const int i = wave_ring(0, by);
+    const int j0 = bx * 64 * kPotCells + (int)threadIdx.x;
+    if (j0 >= P.ns || i >= P.nr) return;
+    const double b = SPLIT ? potential_bg_value(P, i) : 0.0;
+    if (SPLIT && bx == 0 && threadIdx.x == 0) P.bg[FCPT_POTENTIAL][i] = b;
+    potential_ring_cells<SPLIT>(P, i, j0, b);
 }
 
 // ---------------------------------------------------------------------------
@@ -2992,20 +3039,13 @@ void launch_copy_velocities(const KParams &P, hipStream_t st)
     LAUNCH(k_copy_velocities, cell_grid(P, P.nr + 1), kBlock, st, P);
 }
 
-void launch_potential(const KParams &P, hipStream_t st)
-{
-    if (P.split) hipLaunchKernelGGL(k_potential_bg, dim3((unsigned)((P.nr + 255) / 256)), dim3(256), 0, st, P);
-    LAUNCH(k_potential, cell_grid(P, P.nr), kBlock, st, P);
-}
+void launch_potential(const KParams &P, hipStream_t st) { launch_potential_fold(P, st, -1); }
 
 void launch_potential_fold(const KParams &P, hipStream_t st, int fold_mode)
 {
-    if (fold_mode < 0) {
-        launch_potential(P, st);
-        return;
-    }
     const dim3 g = cell_grid(P, P.nr);
-    LAUNCH(k_potential_fold, dim3(g.x, g.y + 1), kBlock, st, P, fold_mode);
+    const unsigned gx = (unsigned)((P.ns + 64 * kPotCells - 1) / (64 * kPotCells));
+    LAUNCH(k_potential_fold, dim3(gx, g.y + 1), kBlock, st, P, fold_mode);
 }
 
 void launch_sources(const KParams &P, hipStream_t st)

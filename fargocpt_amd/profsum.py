@@ -24,12 +24,12 @@ import re
 # bench.py times these kernels with fcpt_time_kernel: 3 rounds of 1 warm-up + 20
 # launches (the viscous pass timed is the walk without SubStep3)
 TIMED = (r"k_radial_sweep", r"k_azimuthal_sweep", r"k_cfl_rings", r"k_visc_walk<(true|false), false>",
-         r"k_potential<")
+         r"k_potential_fold<")
 TIMING_LAUNCHES = 63
 KT_STEPS = 55
 JSON_NAMES = {r"k_radial_sweep": "radial_sweep", r"k_azimuthal_sweep": "azimuthal_sweep",
               r"k_cfl_rings": "cfl", r"k_visc_walk<(true|false), false>": "visc_fused",
-              r"k_potential<": "potential", r"k_disk_force<": "disk_force"}
+              r"k_potential_fold<": "potential", r"k_disk_force<": "disk_force"}
 
 
 def _short(name: str) -> str:
