@@ -636,6 +636,29 @@ __global__ void __launch_bounds__(256) kb_potential_bg(const KParams *Ps, const 
 
 // CalculateNbodyPotential, Pframeforce.cpp:44-85 (+ compute_smoothing,
 // Force.cpp:145-159), at cell (i, j) with scale height H
+// x^4 and x^3 by compensated powering (Graillat, Langlois and Louvet,
+// "Algorithms for accurate, validated and fast polynomial evaluation",
+// Japan J. Indust. Appl. Math. 26, 2009): the square is carried as the
+// unevaluated sum p + e (e from an FMA), and the last product's rounding error
+// is added back before the one final rounding, so the result is within 0.5 +
+// 2^-49 ulp of the exact power: the correctly rounded value but for inputs
+// within 2^-49 ulp of a rounding boundary, as the C library's pow (within
+// 0.52 ulp) returns it.  An overflowing power returns the rounded product.
+__device__ __forceinline__ double pow4_comp(double x)
+{
+    const double p = x * x, e = fma(x, x, -p);
+    const double q = p * p, eq = fma(p, p, -q);
+    const double r = q + (eq + 2.0 * p * e);
+    return isfinite(q) ? r : q;
+}
+__device__ __forceinline__ double pow3_comp(double x)
+{
+    const double p = x * x, e = fma(x, x, -p);
+    const double q = p * x, eq = fma(p, x, -q);
+    const double r = q + (eq + e * x);
+    return isfinite(q) ? r : q;
+}
+
 // one body's term at (x, y) with smoothing length `smooth`
 template <bool SPLIT>
 __device__ __forceinline__ double potential_body(const KParams &P, double x, double y, double smooth,
@@ -651,7 +674,7 @@ __device__ __forceinline__ double potential_body(const KParams &P, double x, dou
     double smooth_factor_klahr = 1.0;
     if (r_sm > 0.0 && d_smoothed < r_sm) {
         const double q = d_smoothed / r_sm;
-        smooth_factor_klahr = (pow(q, 4.0) - 2.0 * pow(q, 3.0) + 2.0 * d_smoothed / r_sm);
+        smooth_factor_klahr = (pow4_comp(q) - 2.0 * pow3_comp(q) + 2.0 * d_smoothed / r_sm);
     }
     if constexpr (SPLIT) return -P.G * mass * inv_d * smooth_factor_klahr;
     else return -P.G * mass / d_smoothed * smooth_factor_klahr;
@@ -1841,28 +1864,6 @@ __device__ __forceinline__ void heating_cooling_cell(const KParams &P, int i, in
     }
 }
 
-// x^4 and x^3 by compensated powering (Graillat, Langlois and Louvet,
-// "Algorithms for accurate, validated and fast polynomial evaluation",
-// Japan J. Indust. Appl. Math. 26, 2009): the square is carried as the
-// unevaluated sum p + e (e from an FMA), and the last product's rounding error
-// is added back before the one final rounding, so the result is within 0.5 +
-// 2^-49 ulp of the exact power: the correctly rounded value but for inputs
-// within 2^-49 ulp of a rounding boundary, as the C library's pow (within
-// 0.52 ulp) returns it.  An overflowing power returns the rounded product.
-__device__ __forceinline__ double pow4_comp(double x)
-{
-    const double p = x * x, e = fma(x, x, -p);
-    const double q = p * p, eq = fma(p, p, -q);
-    const double r = q + (eq + 2.0 * p * e);
-    return isfinite(q) ? r : q;
-}
-__device__ __forceinline__ double pow3_comp(double x)
-{
-    const double p = x * x, e = fma(x, x, -p);
-    const double q = p * x, eq = fma(p, x, -q);
-    const double r = q + (eq + e * x);
-    return isfinite(q) ? r : q;
-}
 
 // the radiative correction of SubStep3 (SourceEuler.cpp:920-924; its
 // std::pow(., 4) and std::pow(., 3) by compensated powering)
