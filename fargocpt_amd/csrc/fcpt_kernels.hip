@@ -621,18 +621,6 @@ __device__ __forceinline__ double potential_bg_value(const KParams &P, int i)
     }
     return b;
 }
-__device__ __forceinline__ void b_potential_bg(const KParams &P, int bx)
-{
-    const int i = bx * (int)blockDim.x + (int)threadIdx.x;
-    if (i >= P.nr) return;
-    P.bg[FCPT_POTENTIAL][i] = potential_bg_value(P, i);
-}
-__global__ void __launch_bounds__(256) k_potential_bg(KParams P) { b_potential_bg(P, (int)blockIdx.x); }
-__global__ void __launch_bounds__(256) kb_potential_bg(const KParams *Ps, const BatchEntry *map)
-{
-    const BatchEntry e = batch_entry(map);
-    b_potential_bg(batch_disk(Ps, e.disk), (int)(e.yx & 0xffffu));
-}
 
 // CalculateNbodyPotential, Pframeforce.cpp:44-85 (+ compute_smoothing,
 // Force.cpp:145-159), at cell (i, j) with scale height H
@@ -738,21 +726,25 @@ __device__ __forceinline__ void potential_ring_cells(const KParams &P, int i, in
     }
 }
 
+// the potential, blocks of 64 x kPotCells sectors by 4 rings; in split
+// storage each wavefront forms its ring's background itself (lane 0 of the
+// first column stores it)
 template <bool SPLIT>
 __device__ __forceinline__ void b_potential(const KParams &P, int bx, int by)
 {
-    CELL_LOOP_HEAD(0, P.nr)
-    const double H = P.adiabatic ? fld<SPLIT>(P, FCPT_SCALE_HEIGHT)(i, j) : cst(P.iso_h)[i];
-    fld<SPLIT>(P, FCPT_POTENTIAL).set(i, j, potential_cell<SPLIT>(P, i, j, H));
+    const int i = wave_ring(0, by);
+    const int j0 = bx * 64 * kPotCells + (int)threadIdx.x;
+    if (j0 >= P.ns || i >= P.nr) return;
+    const double b = SPLIT ? potential_bg_value(P, i) : 0.0;
+    if (SPLIT && bx == 0 && threadIdx.x == 0) P.bg[FCPT_POTENTIAL][i] = b;
+    potential_ring_cells<SPLIT>(P, i, j0, b);
 }
 FCPT_PAIR0(potential, 256)
 
-// The potential of a single disk in one launch, kPotCells cells per thread:
-// in split storage each wavefront forms its ring's background itself (lane 0
-// of the first column stores it); fold_mode >= 0: one extra block row runs
-// the CFL fold of the step's start (k_cfl_finalize_rings; the potential reads
-// none of the scalars the fold writes, and dt is first read by the kernels
-// after this one).
+// The potential of a single disk in one launch (b_potential); fold_mode >= 0:
+// one extra block row runs the CFL fold of the step's start
+// (k_cfl_finalize_rings; the potential reads none of the scalars the fold
+// writes, and dt is first read by the kernels after this one).
 template <bool SPLIT>
 __global__ void __launch_bounds__(256) k_potential_fold(KParams P, int fold_mode)
 {
@@ -763,12 +755,7 @@ __global__ void __launch_bounds__(256) k_potential_fold(KParams P, int fold_mode
             b_cfl_fold(P, fold_mode, (int)(threadIdx.y * blockDim.x + threadIdx.x), true);
         return;
     }
-    const int i = wave_ring(0, by);
-    const int j0 = bx * 64 * kPotCells + (int)threadIdx.x;
-    if (j0 >= P.ns || i >= P.nr) return;
-    const double b = SPLIT ? potential_bg_value(P, i) : 0.0;
-    if (SPLIT && bx == 0 && threadIdx.x == 0) P.bg[FCPT_POTENTIAL][i] = b;
-    potential_ring_cells<SPLIT>(P, i, j0, b);
+    b_potential<SPLIT>(P, bx, by);
 }
 
 // ---------------------------------------------------------------------------
@@ -3545,17 +3532,15 @@ std::vector<BatchStageHost> plan_batch_step(const std::vector<KParams> &D)
         const int o0 = P.damp_ids[2] < P.damp_ids[3] ? P.damp_ids[2] : P.damp_ids[3];
         add_grid(v, d, sector_grid(P), rows_grid(P, n_inner + P.nr + 1 - o0));
     };
-    auto pot_bg = [](std::vector<BatchEntry> &v, unsigned d, const KParams &P) {
-        add_grid(v, d, (unsigned)((P.nr + 255) / 256), 1);
-    };
     auto one = [](std::vector<BatchEntry> &v, unsigned d, const KParams &) { add_grid(v, d, 1, 1); };
     // sim::CalculateTimeStep and the monitor clip (simulation.cpp:100-118, 526-542):
     // the per-ring CFL blocks and the fold, as launch_cfl
     stage(B_CFL_RINGS, 0, all, rings);
     stage(B_CFL_FINAL_RINGS, 2, all, one);
     // step_Euler (simulation.cpp:148-267), in the order of enqueue_euler
-    stage(B_POT_BG, 0, [](const KParams &P) { return P.split != 0; }, pot_bg);
-    stage(B_POTENTIAL, 0, all, rows(0));
+    stage(B_POTENTIAL, 0, all, [](std::vector<BatchEntry> &v, unsigned d, const KParams &P) {
+        add_grid(v, d, (unsigned)((P.ns + 64 * kPotCells - 1) / (64 * kPotCells)), rows_grid(P, P.nr));
+    });
     // the kick: the two walks where the disk's own launch uses them
     // (kick_walk_applies with the tiles resolved at batch creation), else the
     // cell-parallel fused passes; each pass flips the disk's view
@@ -3631,7 +3616,6 @@ void launch_batch_stage(const KParams *Ps, const BatchStageDev &S, int split, in
     case B_CFL_FINAL:
         hipLaunchKernelGGL(kb_cfl_finalize, dim3(S.n), b256, 0, st, Ps, S.map, S.arg);
         break;
-    case B_POT_BG: hipLaunchKernelGGL(kb_potential_bg, dim3(S.n), b256, 0, st, Ps, S.map); break;
     case B_POTENTIAL: LAUNCHB(kb_potential, kBlock); break;
     case B_SRC_VR: LAUNCHB(kb_src_vr, kBlock); break;
     case B_SRC_VPHI: LAUNCHB(kb_src_vphi, kBlock); break;

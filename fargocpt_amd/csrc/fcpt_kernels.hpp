@@ -101,7 +101,7 @@ struct BatchEntry {
     unsigned disk, yx;  // yx = by << 16 | bx
 };
 enum BatchKind {
-    B_RING_MEAN, B_CFL_CELLS, B_CFL_FINAL, B_POT_BG, B_POTENTIAL, B_SRC_VR, B_SRC_VPHI,
+    B_RING_MEAN, B_CFL_CELLS, B_CFL_FINAL, B_POTENTIAL, B_SRC_VR, B_SRC_VPHI,
     B_SRC_ENERGY, B_TW_Q, B_TW_VEL, B_SN_Q, B_SN_VEL, B_TEMP_FLOOR, B_RECALC_VISC, B_STRESS,
     B_VISC_UPDATE, B_SUBSTEP3, B_GHOST, B_RADIAL_ISO, B_RADIAL_ADI, B_AZIM_ISO, B_AZIM_ADI,
     B_VELOCITIES, B_DAMPING, B_DERIVED, B_SRC_FUSED, B_AV_FUSED, B_VISC_FUSED, B_FINAL_DERIVED,
