@@ -77,6 +77,8 @@ def disk_config(meta: dict) -> dict:
         "damp_outer_scalar": -1, "damp_outer_vector": -1,
         "omega_frame": meta["OmegaFrame"], "monitor_timestep": meta["monitor_timestep"],
         "integrator": meta.get("integrator", 0),
+        "potential_smoothing": ((1 if meta.get("compatibility_no_star_smoothing", 0) else 0) |
+                                (2 if meta.get("compatibility_smoothing_planetloc", 0) else 0)),
     }
 
 
@@ -234,11 +236,14 @@ def compare_state(disk, data, tag: str) -> dict:
     return out
 
 
-def run_case(disk, meta, data, nsteps=None):
+def run_case(disk, meta, data, nsteps=None, totals=False):
     """Replay a whole reference run: start-up calls, then sim::run iterations
     with the reference's per-step planet inputs.  Returns
     (max relative dt error, number of steps whose FARGO shifts differ,
-     {checkpoint: errors})."""
+     {checkpoint: errors}); with `totals`, also the relative errors of the
+    disk's mass and angular momentum after the last step against the
+    reference's gas_total_mass / gas_angular_momentum (whole disk and up to
+    the middle ring's radius) under the key "totals"."""
     start_run(disk, meta)
     nsteps = nsteps or meta["nsteps"]
     res = {"0": compare_state(disk, data, "0")}
@@ -257,4 +262,33 @@ def run_case(disk, meta, data, nsteps=None):
             bad_shift += 1
         if f"state_{s}_sigma" in data.files or f"state_{s}_sigma_ringsum" in data.files:
             res[str(s)] = compare_state(disk, data, str(s))
+    if totals:
+        ref = data["totals"][nsteps - 1]
+        m, l = disk.totals(1e300)
+        mc, lc = disk.totals(ref[4])
+        res["totals"] = {"mass": abs(m - ref[0]) / abs(ref[0]),
+                         "angmom": abs(l - ref[1]) / abs(ref[1]),
+                         "mass_cut": abs(mc - ref[2]) / abs(ref[2]),
+                         "angmom_cut": abs(lc - ref[3]) / abs(ref[3])}
     return worst_dt, bad_shift, res
+
+
+def totals_magnitudes(sigma, vazi, surf, rmed, omega_frame, radius, nr):
+    """Sums of |term| of the mass and angular-momentum totals over the active
+    rings with Rmed <= radius (quantities.cpp:51-78, 242-279) and the number
+    of terms: the scale of the rounding bound between summation orders."""
+    rows = [i for i in range(1, nr - 1) if rmed[i] <= radius]
+    if not rows:
+        return 0.0, 0.0, 0
+    r = np.array(rows)
+    s = sigma[r]
+    sm = 0.5 * (s + np.roll(s, 1, axis=1))
+    dm = surf[r, None] * s
+    dl = surf[r, None] * sm * rmed[r, None] * (vazi[r] + omega_frame * rmed[r, None])
+    return float(np.sum(np.abs(dm))), float(np.sum(np.abs(dl))), dm.size
+
+
+def totals_bound(n, magnitude):
+    """|compensated - left-to-right| for n terms: (n - 1) u + 2 u of sum |term|,
+    u = 2^-53, with a margin for the bound's own rounding."""
+    return (n + 4) * 2.0 ** -53 * magnitude * 1.01

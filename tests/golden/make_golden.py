@@ -93,6 +93,14 @@ CASES = {
     "none_small": (dict(ArtificialViscosity="none", Nrad="32", Naz="64", Damping="No",
                         InnerBoundary="ZeroGradient", OuterBoundary="ZeroGradient"),
                    PLANET_1, 20, 1, [1, 5, 20], True),
+    # the potential smoothing switches of compute_smoothing (Force.cpp:145-159): both
+    # (unsmoothed star, planet smoothing fixed at its location), isothermal 32 x 64; and
+    # the unsmoothed star alone on an adiabatic 32 x 64 disk (the planet's smoothing
+    # from the 2-D scale height)
+    "smooth_small": ({"Nrad": "32", "Naz": "64", "CompatibilityNoStarSmoothing": "yes",
+                      "CompatibilitySmoothingPlanetLoc": "yes"}, PLANET_1, 20, 1, [1, 5, 20], True),
+    "nostar_small": (dict(ADIABATIC, Nrad="32", Naz="64", CompatibilityNoStarSmoothing="yes"),
+                     PLANET_1, 20, 1, [1, 5, 20], True),
     # adiabatic 512 x 1536, 500 steps: only per-ring moments and a few full rings kept
     "c2_adi": (dict(ADIABATIC, Nrad="512", Naz="1536"), PLANET_1, 500, 0, [500], False),
     # the LeapFrog integrator (kick dt/2, drift dt, kick dt/2): the isothermal 32 x 64
@@ -298,6 +306,9 @@ def make_case(name, exe, addr, keep_dir=None):
         out["nshift"] = load_bin(os.path.join(dump, "nshift.bin"), np.int32).reshape(nsteps, nr)
         # the disk's gravity on every body after each step (x, y, r, Klahr radius, ax, ay)
         out["diskacc"] = load_bin(os.path.join(dump, "diskacc.bin")).reshape(nsteps, npl, 6)
+        # the reference's gas totals after each step, one thread (mass, angular
+        # momentum; the same up to the radius in column 4)
+        out["totals"] = load_bin(os.path.join(dump, "totals.bin")).reshape(nsteps, 5)
         shapes = {"sigma": (nr, nphi), "vrad": (nr + 1, nphi), "vazi": (nr, nphi),
                   "energy": (nr, nphi), "qplus": (nr, nphi), "qminus": (nr, nphi)}
         for f in ("sigma0", "vrad0", "vazi0", "energy0"):

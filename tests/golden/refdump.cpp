@@ -24,6 +24,9 @@
 //   nshift.bin              per step: Nr int32 FARGO shifts
 //   diskacc.bin             per step, after it: n_planets x 6 float64 (x, y, r, Klahr
 //                           radius, ax, ay) of ComputeDiskOnPlanetAccel, one thread
+//   totals.bin              per step, after it: 5 float64 (gas_total_mass and
+//                           gas_angular_momentum over the whole disk, the same two
+//                           up to the radius Rmed[Nr/2], and that radius), one thread
 //   state_<step>_<f>.bin    checkpoints (f = sigma, vrad, vazi, energy, qplus, qminus);
 //                           state_ic_<f> and ic_<derived>.bin: the reference's own
 //                           initial condition when no --ic directory is given
@@ -373,6 +376,26 @@ static void dump_disk_accel(t_data &data)
     append_raw("diskacc.bin", rec.data(), sizeof(double) * rec.size());
 }
 
+// The gas totals after a step (quantities::gas_total_mass,
+// quantities.cpp:51-78, and quantities::gas_angular_momentum, :242-279), with
+// one OpenMP thread so that each sum runs in the loop order, over the whole
+// disk and up to the radius of the middle ring (Rmed <= radius includes that
+// ring itself).
+static void dump_totals(t_data &data)
+{
+    const int nt = omp_get_max_threads();
+    omp_set_num_threads(1);
+    const double r_cut = Rmed[data[t_data::SIGMA].get_size_radial() / 2];
+    double rec[5];
+    rec[0] = quantities::gas_total_mass(data, 1e300);
+    rec[1] = quantities::gas_angular_momentum(data, 1e300);
+    rec[2] = quantities::gas_total_mass(data, r_cut);
+    rec[3] = quantities::gas_angular_momentum(data, r_cut);
+    rec[4] = r_cut;
+    omp_set_num_threads(nt);
+    append_raw("totals.bin", rec, sizeof(rec));
+}
+
 int main(int argc, char **argv)
 {
     // refdump <config.yml> <outdir> <icdir|-> <nsteps> <stage_steps> <checkpoints,..> [nshift_addr|0] [simrun]
@@ -593,6 +616,7 @@ int main(int argc, char **argv)
 	append_raw("steps.bin", rec, sizeof(rec));
 	append_raw("planets.bin", planet_rec.data(), sizeof(double) * planet_rec.size());
 	dump_disk_accel(data);
+	dump_totals(data);
 	if (checkpoints.count(step)) {
 	    dump_state(data, std::to_string(step));
 	}

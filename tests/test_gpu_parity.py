@@ -2,7 +2,9 @@
 FargoCPT fixtures and against the CPU oracle.
 
 Tolerance: 1e-11 relative per field after N steps (max |gpu - ref| / max |ref|),
-1e-12 per single stage; FARGO integer shifts must be identical at every step.
+1e-12 per single stage; FARGO integer shifts must be identical at every step;
+the disk's total mass and angular momentum (fcpt_totals) within 1e-11 of the
+reference's gas_total_mass / gas_angular_momentum after the run.
 The GPU path is not bit-exact: exp, pow and the pairwise ring sums round
 differently from the host's libm and sequential sums."""
 import numpy as np
@@ -24,7 +26,8 @@ def gpu_disk(meta):
 
 @pytest.mark.parametrize("case,step", [("c1_small", 1), ("c1_small", 2), ("c2_small", 1),
                                        ("c2_small", 2), ("c5_small", 1), ("mix_small", 1),
-                                       ("none_small", 1)])
+                                       ("none_small", 1), ("smooth_small", 1),
+                                       ("nostar_small", 1)])
 def test_stages(case, step):
     meta, data = load_case(case)
     print(f"case={case} step={step}", flush=True)
@@ -36,12 +39,12 @@ def test_stages(case, step):
 
 
 @pytest.mark.parametrize("case", ["c1_small", "c2_small", "c5_small", "mix_small", "none_small",
-                                  "c1_iso", "lf_iso", "lf_adi"])
+                                  "c1_iso", "lf_iso", "lf_adi", "smooth_small", "nostar_small"])
 def test_full_run(case):
     meta, data = load_case(case)
     print(f"case={case} nsteps={meta['nsteps']}", flush=True)
     d = gpu_disk(meta)
-    worst_dt, bad_shift, res = run_case(d, meta, data)
+    worst_dt, bad_shift, res = run_case(d, meta, data, totals=True)
     assert bad_shift == 0
     assert worst_dt < RUN_TOL
     for tag, errs in res.items():
@@ -53,7 +56,7 @@ def test_adiabatic_512x1536_500_steps():
     meta, data = load_case("c2_adi")
     print("case=c2_adi 512x1536 500 steps", flush=True)
     d = gpu_disk(meta)
-    worst_dt, bad_shift, res = run_case(d, meta, data)
+    worst_dt, bad_shift, res = run_case(d, meta, data, totals=True)
     assert bad_shift == 0
     assert worst_dt < RUN_TOL
     for tag, errs in res.items():

@@ -100,6 +100,7 @@ def test_split_vs_fp64_2048x6144_1000_steps():
     assert errs["SIGMA"] <= SIGMA_TOL_1000
     assert errs["VRAD"] <= VEL_TOL_1000 and errs["VAZI"] <= VEL_TOL_1000
     assert abs(m32 - m64) / m64 <= 1e-7
+    assert abs(l32 - l64) / abs(l64) <= 1e-6
     assert abs(t32 - t64) / t64 <= 1e-6
 
 

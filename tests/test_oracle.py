@@ -5,10 +5,11 @@ import numpy as np
 import pytest
 
 from fixtures import (check_stages, compare_state, disk_config, initial_condition, load_case,
-                      run_case)
+                      run_case, totals_bound, totals_magnitudes)
 from oracle import OracleDisk
 
-SMALL = ["c1_small", "c2_small", "c5_small", "mix_small", "none_small"]
+SMALL = ["c1_small", "c2_small", "c5_small", "mix_small", "none_small", "smooth_small",
+         "nostar_small"]
 
 
 @pytest.mark.parametrize("case", ["c1_small", "c1_iso", "c5_small"])
@@ -131,22 +132,73 @@ def test_potential_closed_form(oracle_lib):
     np.testing.assert_allclose(d.download_field("POTENTIAL"), want, rtol=1e-13, atol=0)
 
 
-def test_totals_match_naive_sum(oracle_lib):
-    meta, data = load_case("c1_iso")
+TOTALS_CASES = ["c1_small", "c2_small", "c5_small", "mix_small", "none_small", "c1_iso",
+                "lf_iso", "lf_adi", "smooth_small", "nostar_small"]
+
+
+@pytest.mark.parametrize("case", TOTALS_CASES)
+def test_totals_match_reference(case, oracle_lib):
+    """fo_totals against the reference's own quantities::gas_total_mass and
+    gas_angular_momentum (one thread, loop order) at every checkpoint the
+    fixture holds whole, over the whole disk and up to the middle ring's
+    radius.  Bound: the compensated sum is within 2u of the exact sum and the
+    reference's sequential sum within (n - 1) u sum |term| of it."""
+    meta, data = load_case(case)
     d = OracleDisk(disk_config(meta))
-    sigma = data["state_100_sigma"]
-    vazi = data["state_100_vazi"]
-    d.upload_state(sigma, data["state_100_vrad"], vazi, None)
-    m, l = d.totals(meta["RMAX"])
     nr = meta["Nr"]
-    surf = d.geom("Surf", nr)
-    rmed = d.geom("Rmed", nr)
-    act = slice(1, nr - 1)
-    m_ref = np.sum(surf[act, None] * sigma[act])
-    sl = 0.5 * (sigma + np.roll(sigma, 1, axis=1))
-    l_ref = np.sum((surf * rmed)[act, None] * sl[act] * (vazi[act] + rmed[act, None]))
-    assert abs(m - m_ref) / m_ref < 1e-13
-    assert abs(l - l_ref) / abs(l_ref) < 1e-13
+    geom = {g: data["geom_" + g] for g in ("Surf", "Rmed")}
+    tags = [k[len("state_"):-len("_sigma")] for k in data.files
+            if k.startswith("state_") and k.endswith("_sigma") and k[6:-6].isdigit()]
+    checked = 0
+    for tag in tags:
+        step = int(tag)
+        if step == 0:
+            continue
+        sig, vaz = data[f"state_{tag}_sigma"], data[f"state_{tag}_vazi"]
+        d.upload_state(sig, data[f"state_{tag}_vrad"], vaz, None)
+        ref = data["totals"][step - 1]
+        for radius, (m_ref, l_ref) in ((1e300, ref[0:2]), (ref[4], ref[2:4])):
+            m, l = d.totals(radius)
+            am, al, n = totals_magnitudes(sig, vaz, geom["Surf"], geom["Rmed"], meta["OmegaFrame"],
+                                          radius, nr)
+            assert abs(m - m_ref) <= totals_bound(n, am), (tag, radius, m, m_ref)
+            assert abs(l - l_ref) <= totals_bound(n, al), (tag, radius, l, l_ref)
+            checked += 1
+    assert checked >= 2
+
+
+def test_totals_closed_forms(oracle_lib):
+    """Closed forms that a dropped ring, a wrong frame term or a wrong
+    azimuthal average fail: uniform Sigma gives M = Sigma pi (Rsup[nr-2]^2 -
+    Rinf[1]^2) (the ring areas telescope); v_phi = -Omega_F R gives L = 0
+    exactly; rigid rotation v_phi + Omega_F R = w R with uniform Sigma gives
+    L = Sigma w sum_i Nphi Surf_i Rmed_i^2; the radius cut includes the ring
+    whose Rmed equals it."""
+    meta, data = load_case("c1_small")
+    d = OracleDisk(disk_config(meta))
+    nr, ns = meta["Nr"], meta["Nphi"]
+    rinf, rsup = data["geom_Rinf"], data["geom_Rsup"]
+    rmed, surf = data["geom_Rmed"][:nr], data["geom_Surf"]
+    om = meta["OmegaFrame"]
+    s0 = 3.0e-5
+    sig = np.full((nr, ns), s0)
+    vrad = np.zeros((nr + 1, ns))
+    vaz = np.repeat((-(om * rmed))[:, None], ns, axis=1)
+    d.upload_state(sig, vrad, vaz, None)
+    m, l = d.totals(1e300)
+    m_closed = s0 * np.pi * (rsup[nr - 2] ** 2 - rinf[1] ** 2)
+    assert abs(m - m_closed) <= 1e-13 * m_closed
+    assert l == 0.0
+    w = 0.7
+    vaz = np.repeat((w * rmed - om * rmed)[:, None], ns, axis=1)
+    d.upload_state(sig, vrad, vaz, None)
+    m, l = d.totals(1e300)
+    l_closed = s0 * w * float(np.sum(ns * surf[1:nr - 1] * rmed[1:nr - 1] ** 2))
+    assert abs(l - l_closed) <= 1e-13 * l_closed
+    k = nr // 2
+    m_in, _ = d.totals(rmed[k])
+    m_out, _ = d.totals(np.nextafter(rmed[k], 0.0))
+    assert abs((m_in - m_out) - s0 * ns * surf[k]) <= 1e-12 * m_in
 
 
 def _force_flags(meta):
@@ -157,7 +209,7 @@ def _force_flags(meta):
             (FORCE_SMOOTHING_PLANETLOC if meta.get("compatibility_smoothing_planetloc", 0) else 0))
 
 
-@pytest.mark.parametrize("case", ["c1_small", "c5_small", "mix_small"])
+@pytest.mark.parametrize("case", ["c1_small", "c5_small", "mix_small", "smooth_small", "nostar_small"])
 def test_disk_force_bit_exact(case, oracle_lib):
     """ComputeDiskOnPlanetAccel after every step of the reference run (one
     OpenMP thread, loop order), recomputed by the oracle from its own replay."""
