@@ -61,7 +61,7 @@ class DiskDesc(C.Structure):
         ("damp_inner_scalar", C.c_int32), ("damp_inner_vector", C.c_int32),
         ("damp_outer_scalar", C.c_int32), ("damp_outer_vector", C.c_int32),
         ("omega_frame", C.c_double), ("monitor_timestep", C.c_double),
-        ("integrator", C.c_int32), ("pad_", C.c_int32),
+        ("integrator", C.c_int32), ("potential_smoothing", C.c_int32),
     ]
 
 
@@ -139,7 +139,7 @@ def desc_from_dict(cfg: dict) -> tuple[DiskDesc, np.ndarray]:
         "precision": 0, "heating_cooling_cfl_limit": 1.0, "radial_viscosity_factor": 1.0,
         "cooling_beta_ramp_up": 0.0, "bc_inner": 0, "bc_outer": 0,
         "damp_inner_scalar": -1, "damp_inner_vector": -1, "damp_outer_scalar": -1,
-        "damp_outer_vector": -1, "integrator": 0, "pad_": 0,
+        "damp_outer_vector": -1, "integrator": 0, "potential_smoothing": 0,
     }
     for name, _ in DiskDesc._fields_:
         if name == "radii":

@@ -236,6 +236,7 @@ int fcpt_create(const fcpt_disk_desc *desc, fcpt_ctx **out)
     if (desc->precision != FCPT_PREC_FP64 && desc->precision != FCPT_PREC_SPLIT32) return FCPT_ERR_ARG;
     if (desc->integrator != FCPT_INTEGRATOR_EULER && desc->integrator != FCPT_INTEGRATOR_LEAPFROG)
         return FCPT_ERR_ARG;
+    if (desc->potential_smoothing & ~(FCPT_SMOOTHING_NO_STAR | FCPT_SMOOTHING_PLANETLOC)) return FCPT_ERR_ARG;
     for (int i = 0; i + 1 < desc->nr + 2; ++i)
         if (!(desc->radii[i + 1] > desc->radii[i]) || !(desc->radii[i] > 0.0)) return FCPT_ERR_ARG;
     fcpt_ctx *c = new fcpt_ctx();
@@ -469,6 +470,21 @@ static int read_scalars(fcpt_ctx *ctx)
                                 ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     return FCPT_OK;
+}
+
+// the non-finite-dt flag is reported once: a call that reports it clears it
+// on the device, as do a new clock and a new state
+static int clear_error_flags(fcpt_ctx *ctx)
+{
+    HIP_TRY(ctx, hipMemsetAsync(&ctx->s_dev->error_flags, 0, sizeof(int), ctx->stream));
+    return FCPT_OK;
+}
+static bool take_nonfinite(fcpt_ctx *ctx)
+{
+    if (!(ctx->s_host->error_flags & 1)) return false;
+    ctx->s_host->error_flags &= ~1;
+    clear_error_flags(ctx);
+    return true;
 }
 
 // split storage: the field's background becomes its ring means, the device
@@ -762,6 +778,7 @@ int fcpt_upload_state(fcpt_ctx *ctx, const double *sigma, const double *vrad, co
         launch_heatcool_cfl(P, ctx->stream);
     }
     LAUNCH_CHECK(ctx);
+    if ((rc = clear_error_flags(ctx))) return rc;
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     return FCPT_OK;
 }
@@ -781,7 +798,7 @@ int fcpt_set_time(fcpt_ctx *ctx, double time, double last_dt, int32_t n_monitor)
     CTX_CHECK(ctx);
     launch_set_time(ctx->P, ctx->stream, time, last_dt, n_monitor);
     LAUNCH_CHECK(ctx);
-    return FCPT_OK;
+    return clear_error_flags(ctx);
 }
 
 int fcpt_get_time(fcpt_ctx *ctx, double *time, double *last_dt, int32_t *n_monitor)
@@ -795,12 +812,34 @@ int fcpt_get_time(fcpt_ctx *ctx, double *time, double *last_dt, int32_t *n_monit
     return FCPT_OK;
 }
 
+// compute_smoothing (Force.cpp:145-159) per body, on the host with the
+// reference's expressions (compute_smoothing_iso_planet :132-142, the body's
+// r = hypot(x, y) as planet.cpp:159 keeps it); < 0 marks the default,
+// ThicknessSmoothing times the scale height of the cell
+static BodySmoothing body_smoothing(const fcpt_ctx *ctx, const fcpt_step_params &sp)
+{
+    BodySmoothing b{};
+    const fcpt_disk_desc &d = ctx->desc;
+    for (int k = 0; k < kMaxPlanets; ++k) {
+        b.s[k] = -1.0;
+        if (k >= sp.n_planets) continue;
+        if ((d.potential_smoothing & FCPT_SMOOTHING_NO_STAR) && k == 0) {
+            b.s[k] = 0;
+        } else if (d.potential_smoothing & FCPT_SMOOTHING_PLANETLOC) {
+            const double a = std::hypot(sp.planet[k].x, sp.planet[k].y);
+            const double scale_height = d.aspect_ratio * std::pow(a, 1 + d.flaring_index);
+            b.s[k] = d.thickness_smoothing * scale_height;
+        }
+    }
+    return b;
+}
+
 int fcpt_set_step_params(fcpt_ctx *ctx, const fcpt_step_params *params)
 {
     CTX_CHECK(ctx);
     if (!params || params->n_planets < 0 || params->n_planets > FCPT_MAX_PLANETS)
         return fail(ctx, FCPT_ERR_ARG, "bad step parameters");
-    launch_set_params(ctx->P, ctx->stream, *params);
+    launch_set_params(ctx->P, ctx->stream, *params, body_smoothing(ctx, *params));
     LAUNCH_CHECK(ctx);
     return FCPT_OK;
 }
@@ -812,12 +851,9 @@ static int cfl_common(fcpt_ctx *ctx, int mode, double *dt_out)
     LAUNCH_CHECK(ctx);
     int rc = read_scalars(ctx);
     if (rc) return rc;
-    if (ctx->s_host->error_flags & 1) {
-        // clear the flag for the next call
-        HIP_TRY(ctx, hipMemsetAsync(&ctx->s_dev->error_flags, 0, sizeof(int), ctx->stream));
+    if (take_nonfinite(ctx))
         return fail(ctx, FCPT_ERR_NONFINITE, "CFL time step is not finite and positive (%g)",
                     ctx->s_host->cfl_dt);
-    }
     if (dt_out) *dt_out = ctx->s_host->cfl_dt;
     return FCPT_OK;
 }
@@ -1051,7 +1087,7 @@ int fcpt_set_midstep_params(fcpt_ctx *ctx, const fcpt_step_params *params)
     CTX_CHECK(ctx);
     if (!params || params->n_planets < 0 || params->n_planets > FCPT_MAX_PLANETS)
         return fail(ctx, FCPT_ERR_ARG, "bad step parameters");
-    launch_set_mid_params(ctx->P, ctx->stream, *params);
+    launch_set_mid_params(ctx->P, ctx->stream, *params, body_smoothing(ctx, *params));
     LAUNCH_CHECK(ctx);
     return FCPT_OK;
 }
@@ -1063,7 +1099,7 @@ int fcpt_run_step(fcpt_ctx *ctx, double *dt_out)
     LAUNCH_CHECK(ctx);
     int rc = read_scalars(ctx);
     if (rc) return rc;
-    if (ctx->s_host->error_flags & 1)
+    if (take_nonfinite(ctx))
         return fail(ctx, FCPT_ERR_NONFINITE, "CFL time step is not finite and positive");
     if (dt_out) *dt_out = ctx->s_host->dt;
     return FCPT_OK;
@@ -1127,7 +1163,7 @@ int fcpt_time_steps(fcpt_ctx *ctx, int32_t n_steps, int32_t use_graph, double *m
     HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     rc = read_scalars(ctx);
     if (rc) return rc;
-    if (ctx->s_host->error_flags & 1)
+    if (take_nonfinite(ctx))
         return fail(ctx, FCPT_ERR_NONFINITE, "CFL time step became non-finite during the run");
     if (ms_total) *ms_total = (double)ms;
     return FCPT_OK;
@@ -1483,6 +1519,7 @@ struct fcpt_batch {
     hipStream_t stream = nullptr;
     hipGraphExec_t graph = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t done = nullptr;  // end of the last batched run; every disk's stream waits for it
     std::string err;
 };
 
@@ -1565,6 +1602,7 @@ int fcpt_batch_create(fcpt_ctx *const *disks, int32_t n_disks, fcpt_batch **out)
         }
     }
     if (!rc && hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) rc = FCPT_ERR_DEVICE;
+    if (!rc && hipEventCreateWithFlags(&b->done, hipEventDisableTiming) != hipSuccess) rc = FCPT_ERR_DEVICE;
     if (rc) {
         fcpt_batch_destroy(b);
         return rc;
@@ -1584,6 +1622,7 @@ int fcpt_batch_destroy(fcpt_batch *b)
     if (b->graph) hipGraphExecDestroy(b->graph);
     if (b->ev0) hipEventDestroy(b->ev0);
     if (b->ev1) hipEventDestroy(b->ev1);
+    if (b->done) hipEventDestroy(b->done);
     for (void *p : b->allocs) hipFree(p);
     delete b;
     return FCPT_OK;
@@ -1637,6 +1676,10 @@ int fcpt_batch_run_steps(fcpt_batch *b, int32_t n_steps, int32_t use_graph)
         for (int k = 0; k < n_steps; ++k) enqueue_batch_step(b, b->stream);
         BHIP(b, hipGetLastError());
     }
+    // a disk call made after this one (a download, the totals) is ordered
+    // after the batched steps on the disk's own stream
+    BHIP(b, hipEventRecord(b->done, b->stream));
+    for (fcpt_ctx *c : b->disks) BHIP(b, hipStreamWaitEvent(c->stream, b->done, 0));
     return FCPT_OK;
 }
 
@@ -1698,7 +1741,11 @@ int fcpt_batch_time_steps(fcpt_batch *b, int32_t n_steps, int32_t use_graph, dou
     for (fcpt_ctx *c : b->disks) {
         DevScalars sc;
         BHIP(b, hipMemcpy(&sc, c->s_dev, sizeof(DevScalars), hipMemcpyDeviceToHost));
-        if (sc.error_flags & 1) return bfail(b, FCPT_ERR_NONFINITE, "a disk's CFL time step became non-finite");
+        if (sc.error_flags & 1) {
+            // reported once, like a disk's own calls report it
+            BHIP(b, hipMemset(&c->s_dev->error_flags, 0, sizeof(int)));
+            return bfail(b, FCPT_ERR_NONFINITE, "a disk's CFL time step became non-finite");
+        }
     }
     if (ms_total) *ms_total = (double)ms;
     return FCPT_OK;

@@ -21,6 +21,12 @@ enum { TAB_SRC_INVDXTHETA = 0, TAB_INV_DXTHETA, TAB_INV_OMEGA_K, TAB_OMEGA_K, TA
 
 struct DevPlanet {
     double x, y, mass, r_cubic;
+    double smooth;  // fixed potential smoothing length, or < 0: ThicknessSmoothing * H of the cell
+};
+
+// per-body potential smoothing of one step, formed on the host (compute_smoothing)
+struct BodySmoothing {
+    double s[kMaxPlanets];
 };
 
 // Step-varying scalars, resident in device memory.

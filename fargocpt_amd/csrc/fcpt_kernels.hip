@@ -277,34 +277,85 @@ __device__ double block_sum(double v, double *sh)
     return sh[0];
 }
 
+// The ring sum and the sum of the magnitudes of its terms in one tree: the
+// sum's additions are exactly those of block_sum<BS>.
+template <int BS>
+__device__ double2 block_sum_abs(double v, double a, double2 *sh)
+{
+    const int t = threadIdx.x;
+    sh[t] = double2{v, a};
+    __syncthreads();
+#pragma unroll
+    for (int s = BS / 2; s > 0; s >>= 1) {
+        if (t < s) sh[t] = double2{sh[t].x + sh[t + s].x, sh[t].y + sh[t + s].y};
+        __syncthreads();
+    }
+    return sh[0];
+}
+
+// FARGO shift of ring i from a ring sum (TransportEuler.cpp:215-220)
+__device__ __forceinline__ double fargo_ntilde(const KParams &P, int i, double sum, double dt)
+{
+    const double vm = sum / (double)P.ns;
+    return vm * cst(P.InvRmed)[i] * dt * P.invdphi;
+}
+
 // mode 0: CFL ring mean
 // mode 1: transport ring mean + FARGO shift + uniform residual
 //         (TransportEuler.cpp:207-236)
-__device__ __forceinline__ void ring_mean_store(const KParams &P, int i, double sum, int mode)
+// The reference sums a ring left to right (TransportEuler.cpp:178-183); the
+// block's tree sums the same terms in another order.  Any two summation
+// orders of n terms differ by at most 2 gamma_(n-1) sum |term| (gamma_k =
+// k u / (1 - k u), u = 2^-53), and the shift floor(Ntilde + 0.5) is a
+// monotone function of the sum (a division by ns and products with positive
+// ring constants, each rounded monotonically).  So when both ends of the
+// interval sum -/+ that bound give the same shift, the left-to-right sum
+// gives it too; otherwise thread 0 forms the left-to-right sum itself
+// (seq_sum: the ring's terms in order) and the ring's mean, shift and
+// residual come from it, as in the reference.
+template <typename SeqSum>
+__device__ __forceinline__ void ring_mean_store(const KParams &P, int i, double sum, double asum, int mode,
+                                                SeqSum seq_sum)
 {
     if (threadIdx.x != 0) return;
-    const double vm = sum / (double)P.ns;
-    P.vmean[i] = vm;
     if (mode == 1) {
         const double dt = P.s->dt;
+        const double nsd = (double)P.ns;
+        const double bound = asum * (2.02 * nsd * 0x1p-53) + fabs(sum) * 0x1p-50;
+        const double lo = floor(fargo_ntilde(P, i, sum - bound, dt) + 0.5);
+        const double hi = floor(fargo_ntilde(P, i, sum + bound, dt) + 0.5);
+        if (!(lo == hi)) sum = seq_sum();
+        const double vm = sum / nsd;
+        P.vmean[i] = vm;
         const double invdt = 1.0 / dt;
         const double Ntilde = vm * cst(P.InvRmed)[i] * dt * P.invdphi;
         const double Nround = floor(Ntilde + 0.5);
         P.nshift[i] = (int)Nround;
         P.vuni[i] = (Ntilde - Nround) * cst(P.Rmed)[i] * invdt * P.dphi;
+    } else {
+        P.vmean[i] = sum / (double)P.ns;
     }
 }
 
 template <bool SPLIT>
 __device__ __forceinline__ void b_ring_mean(const KParams &P, int bx, int by, int mode)
 {
-    __shared__ double sh[256];
+    __shared__ double2 sh[256];
     const int ns = P.ns;
     const int i = bx;
     FLD(v, FCPT_VAZI);
-    double acc = 0.0;
-    for (int j = threadIdx.x; j < ns; j += 256) acc += v(i, j);
-    ring_mean_store(P, i, block_sum<256>(acc, sh), mode);
+    double acc = 0.0, aacc = 0.0;
+    for (int j = threadIdx.x; j < ns; j += 256) {
+        const double x = v(i, j);
+        acc += x;
+        aacc += fabs(x);
+    }
+    const double2 r = block_sum_abs<256>(acc, aacc, sh);
+    ring_mean_store(P, i, r.x, r.y, mode, [&]() {
+        double seq = 0.0;
+        for (int j = 0; j < ns; ++j) seq += v(i, j);
+        return seq;
+    });
 }
 FCPT_PAIR1(ring_mean, 256, int, mode)
 
@@ -534,7 +585,7 @@ __global__ void k_set_time(KParams P, double time, double last_dt, int n_monitor
     P.s->n_monitor = n_monitor;
 }
 
-__global__ void k_set_params(KParams P, fcpt_step_params sp)
+__global__ void k_set_params(KParams P, fcpt_step_params sp, BodySmoothing sm)
 {
     DevScalars *s = P.s;
     s->n_planets = sp.n_planets;
@@ -543,6 +594,7 @@ __global__ void k_set_params(KParams P, fcpt_step_params sp)
         s->planet[k].y = sp.planet[k].y;
         s->planet[k].mass = sp.planet[k].mass;
         s->planet[k].r_cubic = sp.planet[k].r_cubic;
+        s->planet[k].smooth = sm.s[k];
     }
     s->indirect_x = sp.indirect_x;
     s->indirect_y = sp.indirect_y;
@@ -550,7 +602,7 @@ __global__ void k_set_params(KParams P, fcpt_step_params sp)
 }
 
 // the bodies of a LeapFrog step's second kick
-__global__ void k_set_mid_params(KParams P, fcpt_step_params sp)
+__global__ void k_set_mid_params(KParams P, fcpt_step_params sp, BodySmoothing sm)
 {
     DevScalars *s = P.s;
     s->n_planets_mid = sp.n_planets;
@@ -559,6 +611,7 @@ __global__ void k_set_mid_params(KParams P, fcpt_step_params sp)
         s->planet_mid[k].y = sp.planet[k].y;
         s->planet_mid[k].mass = sp.planet[k].mass;
         s->planet_mid[k].r_cubic = sp.planet[k].r_cubic;
+        s->planet_mid[k].smooth = sm.s[k];
     }
     s->indirect_mid_x = sp.indirect_x;
     s->indirect_mid_y = sp.indirect_y;
@@ -613,18 +666,17 @@ __device__ __forceinline__ double potential_bg_value(const KParams &P, int i)
 {
     const DevScalars *s = P.s;
     const double R = P.Rmed[i];
-    const double smooth = P.thickness_smoothing * P.bg[FCPT_SCALE_HEIGHT][i];
+    const double smooth_h = P.thickness_smoothing * P.bg[FCPT_SCALE_HEIGHT][i];
     double b = 0.0;
     for (int k = 0; k < s->n_planets; ++k) {
         const double px = s->planet[k].x, py = s->planet[k].y;
+        const double smooth = s->planet[k].smooth >= 0.0 ? s->planet[k].smooth : smooth_h;
         b += -P.G * s->planet[k].mass / sqrt(R * R + (px * px + py * py) + smooth * smooth);
     }
     return b;
 }
 
-// CalculateNbodyPotential, Pframeforce.cpp:44-85 (+ compute_smoothing,
-// Force.cpp:145-159), at cell (i, j) with scale height H
-// x^4 and x^3 by compensated powering (Graillat, Langlois and Louvet,
+// x^4 and x^3 of the Klahr-Kley factor by compensated powering (Graillat, Langlois and Louvet,
 // "Algorithms for accurate, validated and fast polynomial evaluation",
 // Japan J. Indust. Appl. Math. 26, 2009): the square is carried as the
 // unevaluated sum p + e (e from an FMA), and the last product's rounding error
@@ -668,25 +720,10 @@ __device__ __forceinline__ double potential_body(const KParams &P, double x, dou
     else return -P.G * mass / d_smoothed * smooth_factor_klahr;
 }
 
-template <bool SPLIT>
-__device__ __forceinline__ double potential_cell(const KParams &P, int i, int j, double H)
-{
-    const DevScalars *s = P.s;
-    const double x = cst(P.Rmed)[i] * P.cosphi[j];
-    const double y = cst(P.Rmed)[i] * P.sinphi[j];
-    double acc = 0.0;
-    const int np = s->n_planets;
-    for (int k = 0; k < np; ++k) {
-        const double smooth = P.thickness_smoothing * H;
-        acc += potential_body<SPLIT>(P, x, y, smooth, s->planet[k].x, s->planet[k].y, s->planet[k].mass,
-                                     s->planet[k].r_cubic);
-    }
-    acc += -s->indirect_x * x - s->indirect_y * y;
-    return acc;
-}
-
-// The same for kPotCells cells of ring i, sectors j0 + 64 c (those below ns),
-// with the bodies' loop outside the cells': every cell still adds the bodies
+// CalculateNbodyPotential, Pframeforce.cpp:44-85 (+ compute_smoothing,
+// Force.cpp:145-159: a body's fixed smoothing length, or ThicknessSmoothing *
+// H of the cell), for kPotCells cells of ring i, sectors j0 + 64 c (those
+// below ns), with the bodies' loop outside the cells': every cell still adds the bodies
 // in order and then the indirect term, and a body's parameters are loaded
 // once per kPotCells cells.  Split storage: the deviations from the ring
 // background b.
@@ -710,8 +747,10 @@ __device__ __forceinline__ void potential_ring_cells(const KParams &P, int i, in
     for (int k = 0; k < np; ++k) {
         const double px = s->planet[k].x, py = s->planet[k].y;
         const double mass = s->planet[k].mass, r_sm = s->planet[k].r_cubic;
+        const double smk = s->planet[k].smooth;
 #pragma unroll
-        for (int c = 0; c < kPotCells; ++c) acc[c] += potential_body<SPLIT>(P, x[c], y[c], sm[c], px, py, mass, r_sm);
+        for (int c = 0; c < kPotCells; ++c)
+            acc[c] += potential_body<SPLIT>(P, x[c], y[c], smk >= 0.0 ? smk : sm[c], px, py, mass, r_sm);
     }
     const double ix = s->indirect_x, iy = s->indirect_y;
 #pragma unroll
@@ -2266,11 +2305,19 @@ __device__ __forceinline__ void b_ghost_ring_mean(const KParams &P, int bx)
     }
     const int i = bx - ghost_blocks;
     if (i == 0 || i == P.nr - 1) {
-        __shared__ double sh[256];
+        __shared__ double2 sh[256];
         const double g = fld<SPLIT>(P, FCPT_VAZI).stored(i, ghost_vaz(P, i));
-        double acc = 0.0;
-        for (int j = (int)threadIdx.x; j < P.ns; j += 256) acc += g;
-        ring_mean_store(P, i, block_sum<256>(acc, sh), 1);
+        double acc = 0.0, aacc = 0.0;
+        for (int j = (int)threadIdx.x; j < P.ns; j += 256) {
+            acc += g;
+            aacc += fabs(g);
+        }
+        const double2 r = block_sum_abs<256>(acc, aacc, sh);
+        ring_mean_store(P, i, r.x, r.y, 1, [&]() {
+            double seq = 0.0;
+            for (int j = 0; j < P.ns; ++j) seq += g;
+            return seq;
+        });
         return;
     }
     b_ring_mean<SPLIT>(P, i, 0, 1);
@@ -2987,13 +3034,15 @@ void launch_set_time(const KParams &P, hipStream_t st, double t, double last_dt,
 {
     hipLaunchKernelGGL(k_set_time, dim3(1), dim3(1), 0, st, P, t, last_dt, n_monitor);
 }
-void launch_set_params(const KParams &P, hipStream_t st, const fcpt_step_params &sp)
+void launch_set_params(const KParams &P, hipStream_t st, const fcpt_step_params &sp,
+                       const BodySmoothing &sm)
 {
-    hipLaunchKernelGGL(k_set_params, dim3(1), dim3(1), 0, st, P, sp);
+    hipLaunchKernelGGL(k_set_params, dim3(1), dim3(1), 0, st, P, sp, sm);
 }
-void launch_set_mid_params(const KParams &P, hipStream_t st, const fcpt_step_params &sp)
+void launch_set_mid_params(const KParams &P, hipStream_t st, const fcpt_step_params &sp,
+                           const BodySmoothing &sm)
 {
-    hipLaunchKernelGGL(k_set_mid_params, dim3(1), dim3(1), 0, st, P, sp);
+    hipLaunchKernelGGL(k_set_mid_params, dim3(1), dim3(1), 0, st, P, sp, sm);
 }
 void launch_leapfrog_phase(const KParams &P, hipStream_t st, int phase)
 {

@@ -15,7 +15,8 @@ void launch_cfl(const KParams &P, hipStream_t st, int mode);
 void launch_set_dt(const KParams &P, hipStream_t st, double dt);
 void launch_set_clock(const KParams &P, hipStream_t st, double t);
 void launch_set_time(const KParams &P, hipStream_t st, double t, double last_dt, int n_monitor);
-void launch_set_params(const KParams &P, hipStream_t st, const fcpt_step_params &sp);
+void launch_set_params(const KParams &P, hipStream_t st, const fcpt_step_params &sp,
+                       const BodySmoothing &sm);
 void launch_potential(const KParams &P, hipStream_t st);
 void launch_sources(const KParams &P, hipStream_t st);
 bool launch_artificial_viscosity(const KParams &P, hipStream_t st, int with_viscosity);
@@ -26,7 +27,8 @@ void launch_substep3(const KParams &P, hipStream_t st);
 void launch_heatcool_cfl(const KParams &P, hipStream_t st);
 void launch_boundaries(const KParams &P, hipStream_t st, int final_call, int advance_time);
 void launch_transport(const KParams &P, hipStream_t st);
-void launch_set_mid_params(const KParams &P, hipStream_t st, const fcpt_step_params &sp);
+void launch_set_mid_params(const KParams &P, hipStream_t st, const fcpt_step_params &sp,
+                           const BodySmoothing &sm);
 void launch_leapfrog_phase(const KParams &P, hipStream_t st, int phase);
 void launch_pressure(const KParams &P, hipStream_t st);
 void launch_final_boundaries_derived(const KParams &P, hipStream_t st, int advance_time);

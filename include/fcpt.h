@@ -35,7 +35,9 @@ typedef enum fcpt_status {
     FCPT_ERR_ARG = -1,      /* invalid argument or size */
     FCPT_ERR_DEVICE = -2,   /* HIP runtime error */
     FCPT_ERR_STATE = -3,    /* call order / missing state */
-    FCPT_ERR_NONFINITE = -4 /* CFL reduction met a NaN or non-positive dt */
+    FCPT_ERR_NONFINITE = -4 /* CFL reduction met a NaN or non-positive dt; reported by the
+                               first call that reads it back, which clears it (as do
+                               fcpt_set_time, fcpt_upload_state and fcpt_read_snapshot) */
 } fcpt_status;
 
 /* artificial viscosity (reference parameters.h:156-161) */
@@ -57,6 +59,13 @@ enum { FCPT_PREC_FP64 = 0, FCPT_PREC_SPLIT32 = 1 };
    LeapFrog scheme also scales the artificial-viscosity, viscosity and
    heating/cooling limits of the CFL condition by 0.6 (cfl.cpp:254-261) */
 enum { FCPT_INTEGRATOR_EULER = 0, FCPT_INTEGRATOR_LEAPFROG = 1 };
+/* potential smoothing switches of compute_smoothing (Force.cpp:145-159):
+   CompatibilityNoStarSmoothing (body 0 unsmoothed) and
+   CompatibilitySmoothingPlanetLoc (each body's smoothing fixed at
+   ThicknessSmoothing * AspectRatio * r_k^(1 + FlaringIndex), r_k = hypot(x_k, y_k),
+   instead of ThicknessSmoothing * H of the cell); the same bit values as the
+   disk-force flags FCPT_FORCE_NO_STAR_SMOOTHING / FCPT_FORCE_SMOOTHING_PLANETLOC */
+enum { FCPT_SMOOTHING_NO_STAR = 1, FCPT_SMOOTHING_PLANETLOC = 2 };
 
 /* field identifiers for fcpt_upload_field / fcpt_download_field */
 typedef enum fcpt_field {
@@ -111,7 +120,8 @@ typedef struct fcpt_disk_desc {
     double omega_frame;         /* refframe::OmegaFrame (fixed frame rotation) */
     double monitor_timestep;    /* parameters::monitor_timestep for the dt clip */
     int32_t integrator;         /* FCPT_INTEGRATOR_* (parameters::hydro_integrator) */
-    int32_t pad_;
+    int32_t potential_smoothing; /* FCPT_SMOOTHING_* mask of the gas potential
+                                    (CalculateNbodyPotential); 0 is the reference default */
 } fcpt_disk_desc;
 
 #define FCPT_MAX_PLANETS 8
@@ -314,7 +324,10 @@ int fcpt_autotune(fcpt_ctx *ctx, int32_t reps, int32_t *radial_tile, int32_t *ra
    bit for bit like the same disk under fcpt_run_steps.  The batch uses the disks' parameters
    and launch shapes as they are at fcpt_batch_create and reads and writes
    the disks' own device state: between batch calls the disks may be read
-   (fcpt_download_*, fcpt_totals) but not stepped on their own. */
+   (fcpt_download_*, fcpt_totals) but not stepped on their own.  Every batch
+   run ends with an event that each member disk's stream waits for, so such a
+   read, issued right after fcpt_batch_run_steps returns, sees the stepped
+   state without an explicit fcpt_batch_synchronize. */
 typedef struct fcpt_batch fcpt_batch;
 int fcpt_batch_create(fcpt_ctx *const *disks, int32_t n_disks, fcpt_batch **out);
 int fcpt_batch_destroy(fcpt_batch *batch);

@@ -37,6 +37,8 @@ def load():
     spec = {
         "fo_create": ([C.POINTER(DiskDesc)], P),
         "fo_destroy": ([P], None),
+        "fo_set_threads": ([I], None),
+        "fo_threads": ([], I),
         "fo_field": ([P, I], dp),
         "fo_field_rows": ([P, I], I),
         "fo_set_state": ([P, dp, dp, dp, dp], None),
@@ -214,3 +216,9 @@ class OracleDisk:
 
     def synchronize(self):
         pass
+
+    def set_threads(self, n: int):
+        self.lib.fo_set_threads(n)
+
+    def threads(self) -> int:
+        return self.lib.fo_threads()

@@ -8,6 +8,7 @@
 #include "fcpt_oracle.h"
 
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -163,6 +164,10 @@ fo_disk *fo_create(const fcpt_disk_desc *desc)
     memcpy(d->damp_ids, ids, sizeof(ids));
     return d;
 }
+
+/* OpenMP threads of the oracle's loops (the timed CPU baseline states them) */
+void fo_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int fo_threads(void) { return omp_get_max_threads(); }
 
 void fo_destroy(fo_disk *d)
 {
@@ -669,6 +674,21 @@ double fo_calculate_timestep(fo_disk *d)
 }
 
 /* CalculateNbodyPotential, Pframeforce.cpp:21-86 (+ compute_smoothing, Force.cpp:145-159) */
+/* compute_smoothing, Force.cpp:145-159 (compute_smoothing_scaleheight :123-130,
+   compute_smoothing_iso_planet :132-142 with planet.get_r() = hypot(x, y),
+   planet.cpp:159, and aspectratio_ref = AspectRatio, Interpret.cpp:191) */
+static double compute_smoothing(const fo_disk *d, int k, double H)
+{
+    const int flags = d->p.potential_smoothing;
+    if ((flags & FCPT_SMOOTHING_NO_STAR) && k == 0) return 0;
+    if (flags & FCPT_SMOOTHING_PLANETLOC) {
+        const double a = hypot(d->sp.planet[k].x, d->sp.planet[k].y);
+        const double scale_height = d->p.aspect_ratio * pow(a, 1 + d->p.flaring_index);
+        return d->p.thickness_smoothing * scale_height;
+    }
+    return d->p.thickness_smoothing * H;
+}
+
 void fo_nbody_potential(fo_disk *d)
 {
     const int nr = d->nr, ns = d->ns;
@@ -683,7 +703,7 @@ void fo_nbody_potential(fo_disk *d)
             const double y = d->Rmed[i] * d->sinphi[j];
             double acc = 0.0;
             for (int k = 0; k < sp->n_planets; ++k) {
-                const double smooth = d->p.thickness_smoothing * H[IDX(i, j)];
+                const double smooth = compute_smoothing(d, k, H[IDX(i, j)]);
                 const double dx = x - sp->planet[k].x;
                 const double dy = y - sp->planet[k].y;
                 const double dist_2 = pow(dx, 2) + pow(dy, 2);

@@ -24,6 +24,8 @@ typedef struct fo_disk fo_disk;
 
 fo_disk *fo_create(const fcpt_disk_desc *desc);
 void fo_destroy(fo_disk *d);
+void fo_set_threads(int n);
+int fo_threads(void);
 /* direct access to a field array in the reference layout */
 double *fo_field(fo_disk *d, int field);
 int fo_field_rows(const fo_disk *d, int field);
