@@ -236,7 +236,7 @@ SNAPSHOT_CASES = {"c1_small", "c2_small"}
 
 
 def run_driver(exe, addr, cfg, workdir, icdir, nsteps, stage_steps, checkpoints, simrun=False,
-               snapshot=False):
+               snapshot=False, env_extra=None):
     out = os.path.join(workdir, "dump")
     os.makedirs(out, exist_ok=True)
     args = [exe, cfg, out, icdir, str(nsteps), str(stage_steps),
@@ -244,6 +244,7 @@ def run_driver(exe, addr, cfg, workdir, icdir, nsteps, stage_steps, checkpoints,
     if simrun:
         args.append("simrun")
     env = dict(os.environ, OMP_NUM_THREADS="8")
+    env.update(env_extra or {})
     if snapshot:
         env["REFDUMP_SNAPSHOT"] = "1"
     subprocess.run(args, cwd=workdir, check=True, stdout=subprocess.DEVNULL, env=env)
@@ -254,28 +255,35 @@ def load_bin(path, dtype=np.float64):
     return np.fromfile(path, dtype=dtype)
 
 
+def case_inputs(name, exe, addr, work):
+    """The config file of a case and the package's seeded initial condition
+    for it (code units from a first run of the driver), written under work."""
+    overrides, planets = CASES[name][:2]
+    cfg = os.path.join(work, f"{name}.yml")
+    write_config(cfg, overrides, planets)
+    # pass 1: code units from the reference's own unit system
+    probe = run_driver(exe, addr, cfg, os.path.join(work, "probe"), "-", 0, 0, [])
+    meta = read_meta(os.path.join(probe, "meta.txt"))
+    setup = DiskSetup(nr=meta["Nr"], nphi=meta["Nphi"], rmin=meta["RMIN"], rmax=meta["RMAX"],
+                      sigma0=meta["sigma0"], sigma_slope=float(BASE["SigmaSlope"]),
+                      aspect_ratio=meta["aspectratio"], flaring=meta["flaring"],
+                      G=meta["G"], mstar=meta["hydro_center_mass"],
+                      omega_frame=meta["OmegaFrame"], adiabatic=bool(meta["adiabatic"]),
+                      gamma=meta["gamma"], noise_amp=1e-4, seed=1337)
+    ic = initial_state(setup)
+    icdir = os.path.join(work, "ic")
+    os.makedirs(icdir)
+    for f in ("sigma", "vrad", "vazi", "energy"):
+        ic[f].tofile(os.path.join(icdir, f + ".bin"))
+    return cfg, icdir
+
+
 def make_case(name, exe, addr, keep_dir=None):
     overrides, planets, nsteps, stage_steps, checkpoints, full = CASES[name]
     work = tempfile.mkdtemp(prefix=f"golden_{name}_")
     try:
-        cfg = os.path.join(work, f"{name}.yml")
-        write_config(cfg, overrides, planets)
+        cfg, icdir = case_inputs(name, exe, addr, work)
         shutil.copy(cfg, os.path.join(HERE, "configs", f"{name}.yml"))
-        # pass 1: code units from the reference's own unit system
-        probe = run_driver(exe, addr, cfg, os.path.join(work, "probe"), "-", 0, 0, [])
-        meta = read_meta(os.path.join(probe, "meta.txt"))
-        nr, nphi = meta["Nr"], meta["Nphi"]
-        setup = DiskSetup(nr=nr, nphi=nphi, rmin=meta["RMIN"], rmax=meta["RMAX"],
-                          sigma0=meta["sigma0"], sigma_slope=float(BASE["SigmaSlope"]),
-                          aspect_ratio=meta["aspectratio"], flaring=meta["flaring"],
-                          G=meta["G"], mstar=meta["hydro_center_mass"],
-                          omega_frame=meta["OmegaFrame"], adiabatic=bool(meta["adiabatic"]),
-                          gamma=meta["gamma"], noise_amp=1e-4, seed=1337)
-        ic = initial_state(setup)
-        icdir = os.path.join(work, "ic")
-        os.makedirs(icdir)
-        for f in ("sigma", "vrad", "vazi", "energy"):
-            ic[f].tofile(os.path.join(icdir, f + ".bin"))
         dump = run_driver(exe, addr, cfg, os.path.join(work, "run"), icdir, nsteps,
                           stage_steps, checkpoints, snapshot=name in SNAPSHOT_CASES)
         meta = read_meta(os.path.join(dump, "meta.txt"))

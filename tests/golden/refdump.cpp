@@ -33,6 +33,10 @@
 //   stage_<step>_<k>_<name>_<f>.bin   per-stage fields for the first steps
 //   snapshot/               with REFDUMP_SNAPSHOT set: the reference's snapshot
 //                           files (<grid>.dat, misc.bin) of the final state
+// With REFDUMP_TIME set the step loop writes nothing per step and meta.txt gets
+// loop_seconds, the wall time of the nsteps sim::run iterations (the CPU
+// baseline's fairness check times the reference this way).
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -580,6 +584,8 @@ int main(int argc, char **argv)
     dump_state(data, "0");
 
     // sim::run loop (src/simulation.cpp:505-558)
+    const bool timing = getenv("REFDUMP_TIME") != nullptr;
+    const auto t_loop = std::chrono::steady_clock::now();
     for (unsigned step = 1; step <= nsteps; ++step) {
 	double rec[16] = {0};
 	rec[0] = sim::time;
@@ -613,6 +619,7 @@ int main(int argc, char **argv)
 	    sim::N_monitor++;
 	}
 	rec[9] = towrite ? 1.0 : 0.0;
+	if (timing) continue;
 	append_raw("steps.bin", rec, sizeof(rec));
 	append_raw("planets.bin", planet_rec.data(), sizeof(double) * planet_rec.size());
 	dump_disk_accel(data);
@@ -620,6 +627,12 @@ int main(int argc, char **argv)
 	if (checkpoints.count(step)) {
 	    dump_state(data, std::to_string(step));
 	}
+    }
+    if (timing) {
+	const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_loop).count();
+	FILE *f = fopen((g_out + "/meta.txt").c_str(), "a");
+	fprintf(f, "loop_seconds = %.9g\n", sec);
+	fclose(f);
     }
     if (getenv("REFDUMP_SNAPSHOT")) {
 	// the reference's own snapshot writers on the final state
