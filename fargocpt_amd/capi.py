@@ -33,6 +33,7 @@ KERNEL_DISK_FORCE = 6  # the disk-on-planet reduction (not part of the hydro ste
 PREC_FP64, PREC_SPLIT32 = 0, 1
 INTEGRATOR_EULER, INTEGRATOR_LEAPFROG = 0, 1
 PARAM_RADIAL_TILE, PARAM_RADIAL_WAVES, PARAM_FUSED_STEP, PARAM_VISC_TILE, PARAM_KICK_TILE = 0, 1, 2, 3, 4
+PARAM_SPLIT_ROUNDING = 5
 BC = {"reflecting": 0, "zerogradient": 1, "outflow": 2}
 MAX_PLANETS = 8
 
@@ -465,6 +466,11 @@ class Disk:
         default; 1: the sources and artificial viscosity as separate
         passes)."""
         self._check(self.lib.fcpt_set_launch_param(self.h, PARAM_KICK_TILE, rings))
+
+    def set_split_rounding(self, stochastic):
+        """Split storage: round deviations stochastically (True, the
+        default) or to nearest (False)."""
+        self._check(self.lib.fcpt_set_launch_param(self.h, PARAM_SPLIT_ROUNDING, 1 if stochastic else 0))
 
     def autotune(self, reps=10):
         """Time the tunable launch shapes on the current state and keep the

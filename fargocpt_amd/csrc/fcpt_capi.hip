@@ -252,6 +252,7 @@ int fcpt_create(const fcpt_disk_desc *desc, fcpt_ctx **out)
     P.nr = nr;
     P.ns = ns;
     P.split = desc->precision == FCPT_PREC_SPLIT32 ? 1 : 0;
+    P.split_rounding = P.split;  // stochastic rounding of the split deviations by default
     P.radial_tile = 0;
     P.radial_waves = 3;
     P.visc_tile = 0;
@@ -367,7 +368,7 @@ int fcpt_create(const fcpt_disk_desc *desc, fcpt_ctx **out)
     const size_t ncell = (size_t)nr * ns;
     const size_t esz = element_size(P);
     for (int k = 0; k < FCPT_NUM_FIELDS && !rc; ++k)
-        rc = dev_alloc_bytes(c, &P.f[k], field_rows(k, nr) * ns * esz);
+        rc = dev_alloc_bytes(c, &P.f[k], field_rows(k, nr) * ns * field_element_size(P, k));
     if (!rc) rc = dev_alloc_bytes(c, &P.valt[0], field_rows(FCPT_VRAD, nr) * ns * esz);
     if (!rc) rc = dev_alloc_bytes(c, &P.valt[1], field_rows(FCPT_VAZI, nr) * ns * esz);
     for (int k = 0; k < 4 && !rc; ++k) rc = dev_alloc_bytes(c, &P.mom[k], ncell * esz);
@@ -416,7 +417,7 @@ int fcpt_create(const fcpt_disk_desc *desc, fcpt_ctx **out)
     // zero every field (the reference's t_polargrid::set_size clears its arrays);
     // split storage: zero backgrounds and zero deviations
     for (int k = 0; k < FCPT_NUM_FIELDS; ++k)
-        e = e == hipSuccess ? hipMemset(P.f[k], 0, field_rows(k, nr) * ns * esz) : e;
+        e = e == hipSuccess ? hipMemset(P.f[k], 0, field_rows(k, nr) * ns * field_element_size(P, k)) : e;
     for (int k = 0; P.split && k < FCPT_NUM_FIELDS; ++k)
         e = e == hipSuccess ? hipMemset(P.bg[k], 0, ((size_t)nr + 2) * sizeof(double)) : e;
     for (int k = 0; P.split && k < 4; ++k)
@@ -531,7 +532,7 @@ int fcpt_upload_field(fcpt_ctx *ctx, int32_t field, const double *host)
 {
     CTX_CHECK(ctx);
     if (field < 0 || field >= FCPT_NUM_FIELDS || !host) return fail(ctx, FCPT_ERR_ARG, "bad field");
-    if (ctx->P.split) return upload_split(ctx, field, host);
+    if (ctx->P.split && field != FCPT_POTENTIAL) return upload_split(ctx, field, host);
     const size_t n = field_rows(field, ctx->P.nr) * ctx->P.ns;
     HIP_TRY(ctx, hipMemcpyAsync(ctx->P.f[field], host, n * sizeof(double), hipMemcpyHostToDevice,
                                 ctx->stream));
@@ -543,7 +544,7 @@ int fcpt_download_field(fcpt_ctx *ctx, int32_t field, double *host)
 {
     CTX_CHECK(ctx);
     if (field < 0 || field >= FCPT_NUM_FIELDS || !host) return fail(ctx, FCPT_ERR_ARG, "bad field");
-    if (ctx->P.split) return download_split(ctx, field, host);
+    if (ctx->P.split && field != FCPT_POTENTIAL) return download_split(ctx, field, host);
     const size_t n = field_rows(field, ctx->P.nr) * ctx->P.ns;
     HIP_TRY(ctx, hipMemcpyAsync(host, ctx->P.f[field], n * sizeof(double), hipMemcpyDeviceToHost,
                                 ctx->stream));
@@ -1244,6 +1245,8 @@ int fcpt_set_launch_param(fcpt_ctx *ctx, int32_t param, int32_t value)
         if (value < 0 || value > 64) return fail(ctx, FCPT_ERR_ARG, "viscous tile must be 0 .. 64");
     } else if (param == FCPT_PARAM_KICK_TILE) {
         if (value < 0 || value > 64) return fail(ctx, FCPT_ERR_ARG, "kick tile must be 0 .. 64");
+    } else if (param == FCPT_PARAM_SPLIT_ROUNDING) {
+        if (value != 0 && value != 1) return fail(ctx, FCPT_ERR_ARG, "split rounding must be 0 or 1");
     } else {
         return fail(ctx, FCPT_ERR_ARG, "unknown launch parameter");
     }
@@ -1252,6 +1255,7 @@ int fcpt_set_launch_param(fcpt_ctx *ctx, int32_t param, int32_t value)
     else if (param == FCPT_PARAM_RADIAL_WAVES) ctx->P.radial_waves = value;
     else if (param == FCPT_PARAM_VISC_TILE) ctx->P.visc_tile = value;
     else if (param == FCPT_PARAM_KICK_TILE) ctx->P.kick_tile = value;
+    else if (param == FCPT_PARAM_SPLIT_ROUNDING) ctx->P.split_rounding = ctx->P.split ? value : 0;
     else ctx->fused = value != 0;
     if (ctx->step_graph) {
         HIP_TRY(ctx, hipGraphExecDestroy(ctx->step_graph));

@@ -38,7 +38,7 @@ struct DevScalars {
     int n_monitor;
     int error_flags;      // bit 0: non-finite or non-positive dt
     int n_planets;
-    int pad_;
+    unsigned step_count;  // CFL reductions so far: part of the split storage's rounding key
     double indirect_x, indirect_y;
     DevPlanet planet[kMaxPlanets];
     // LeapFrog: the full step's dt and start time while the kicks run with
@@ -82,6 +82,7 @@ struct KParams {
     const double *cosphi, *sinphi;
     // storage: 0 = FP64 fields; 1 = FP32 deviations from FP64 per-ring backgrounds
     int split;
+    int split_rounding;  // split storage: 0 rounds deviations to nearest, 1 stochastically
     int radial_tile;     // rings per thread of the radial sweep (0: size-based choice)
     int radial_waves;    // register budget of the radial sweep: wavefronts per SIMD (2 or 3)
     int visc_tile;       // rings per thread of the fused viscous pass (0: size-based choice)

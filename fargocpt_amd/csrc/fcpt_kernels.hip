@@ -126,6 +126,44 @@ __device__ __forceinline__ const __attribute__((address_space(4))) T *cst(const 
     return (const __attribute__((address_space(4))) T *)p;
 }
 
+// Split storage rounds each deviation to FP32 either to nearest or
+// stochastically: up or down with probabilities proportional to the distances
+// to the two neighbouring FP32 values (the rounding is unbiased, so the
+// per-step roundings of a slowly changing value do not add up in one
+// direction: an increment below half an FP32 unit of the deviation survives
+// on average instead of being lost every step).  The uniform variate is a hash
+// (Wellons' lowbias32, public domain) of the deviation's low 32 bits, the
+// cell, and a key formed from the field and the step counter, so a value and
+// its cell always round the same way within one kernel (a thread can form
+// what another thread stores) and differently from step to step.
+__device__ __forceinline__ unsigned sr_hash(unsigned x)
+{
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+__device__ __forceinline__ float round_dev(double d, unsigned key, size_t c)
+{
+    if (!key) return (float)d;
+    const unsigned long long b = (unsigned long long)__double_as_longlong(d);
+    const unsigned long long mag = b & 0x7fffffffffffffffull;
+    // NaN and infinities, and deviations the FP32 range cannot hold at full
+    // precision (below 2^-126 or from 2^128): to nearest
+    if (mag >= 0x47f0000000000000ull || mag < 0x3810000000000000ull) return (float)d;
+    const unsigned u = sr_hash((unsigned)b ^ key ^ ((unsigned)c * 0x9e3779b1u)) >> 3;  // 29 bits
+    // add u below the FP32 unit of the magnitude, then drop those 29 bits:
+    // rounded up with probability (dropped bits) / 2^29
+    const unsigned long long m = (mag + (unsigned long long)u) & ~0x1fffffffull;
+    return (float)__longlong_as_double((long long)((b & 0x8000000000000000ull) | m));
+}
+__device__ __forceinline__ unsigned sr_key(int id, unsigned step)
+{
+    return sr_hash(step * 0x85ebca77u + (unsigned)id * 0xc2b2ae3du) | 1u;
+}
+
 // field accessor: an FP64 array, or FP32 deviations from an FP64 ring background
 template <bool SPLIT>
 struct Fld {
@@ -133,6 +171,7 @@ struct Fld {
     T *p;
     const double *bg;
     int ns;
+    unsigned key;  // split storage: 0 rounds to nearest, else the stochastic rounding's key
     __device__ __forceinline__ double operator()(int i, int j) const
     {
         const size_t c = (size_t)i * (size_t)ns + (size_t)j;
@@ -142,17 +181,18 @@ struct Fld {
     __device__ __forceinline__ void set(int i, int j, double v) const
     {
         const size_t c = (size_t)i * (size_t)ns + (size_t)j;
-        if constexpr (SPLIT) p[c] = (float)(v - cst(bg)[i]);
+        if constexpr (SPLIT) p[c] = round_dev(v - cst(bg)[i], key, c);
         else p[c] = v;
     }
     // the stored value alone, and its decoding: lets a kernel issue a load
     // early and pay for it where the value is used
     __device__ __forceinline__ T raw(int i, int j) const { return p[(size_t)i * (size_t)ns + (size_t)j]; }
     __device__ __forceinline__ void set_raw(int i, int j, T x) const { p[(size_t)i * (size_t)ns + (size_t)j] = x; }
-    // v as a later load of the stored value returns it
-    __device__ __forceinline__ double stored(int i, double v) const
+    // v as a later load of the value stored at (i, j) returns it
+    __device__ __forceinline__ double stored(int i, int j, double v) const
     {
-        if constexpr (SPLIT) return cst(bg)[i] + (double)(float)(v - cst(bg)[i]);
+        const size_t c = (size_t)i * (size_t)ns + (size_t)j;
+        if constexpr (SPLIT) return cst(bg)[i] + (double)round_dev(v - cst(bg)[i], key, c);
         else return v;
     }
     __device__ __forceinline__ double dec(int i, T x) const
@@ -169,18 +209,37 @@ __device__ __forceinline__ double ring_tab(const KParams &P, int t, int i)
     return cst(P.ring_tab)[(size_t)t * (size_t)(P.nr + 2) + (size_t)i];
 }
 
+// the rounding key of split-storage stores of field (or buffer) `id` in the
+// current step; 0 (to nearest) in FP64 storage or with stochastic rounding off
+template <bool SPLIT>
+__device__ __forceinline__ unsigned store_key(const KParams &P, int id)
+{
+    if constexpr (SPLIT) return P.split_rounding ? sr_key(id, cst(&P.s->step_count)[0]) : 0u;
+    else return 0u;
+}
 template <bool SPLIT>
 __device__ __forceinline__ Fld<SPLIT> fld(const KParams &P, int f)
 {
-    return Fld<SPLIT>{static_cast<typename Fld<SPLIT>::T *>(P.f[f]), P.bg[f], P.ns};
+    return Fld<SPLIT>{static_cast<typename Fld<SPLIT>::T *>(P.f[f]), P.bg[f], P.ns, store_key<SPLIT>(P, f)};
 }
+// a work buffer; id names it for the rounding key (the momenta: kBufMom + q)
+constexpr int kBufMom = 64;
 template <bool SPLIT>
-__device__ __forceinline__ Fld<SPLIT> buf(const KParams &P, void *p, const double *bg)
+__device__ __forceinline__ Fld<SPLIT> buf(const KParams &P, void *p, const double *bg, int id = -1)
 {
-    return Fld<SPLIT>{static_cast<typename Fld<SPLIT>::T *>(p), bg, P.ns};
+    return Fld<SPLIT>{static_cast<typename Fld<SPLIT>::T *>(p), bg, P.ns, id >= 0 ? store_key<SPLIT>(P, id) : 0u};
 }
 
 #define FLD(name, id) const Fld<SPLIT> name = fld<SPLIT>(P, id)
+
+// The gas potential is stored in FP64 in both storage modes: its radial and
+// azimuthal differences drive the source terms, and near a body the
+// potential's deviation from any ring profile is large, so an FP32 deviation
+// would round the gradient there by about 2^-24 |dev| / |difference|.
+__device__ __forceinline__ Fld<false> phi_fld(const KParams &P)
+{
+    return Fld<false>{static_cast<double *>(P.f[FCPT_POTENTIAL]), nullptr, P.ns, 0u};
+}
 
 // Neighbour values within a wavefront: the value of lane - 1 and of lane + 1,
 // moved by the DPP wavefront shifts (wave_shr:1, wave_shl:1) on both 32-bit
@@ -395,6 +454,7 @@ __device__ __forceinline__ void b_cfl_fold(const KParams &P, int mode, int t, bo
     DevScalars *s = P.s;
     const double cfl_dt = sh[0];
     s->cfl_dt = cfl_dt;
+    s->step_count += 1u;
     if (!(cfl_dt > 0.0) || !isfinite(cfl_dt)) s->error_flags |= 1;
     if (mode >= 1) {
         const double rv = std_min(P.cfl_max_var * s->last_dt, cfl_dt);
@@ -658,24 +718,6 @@ __global__ void k_leapfrog_phase(KParams P, int phase)
 }
 
 // ---------------------------------------------------------------------------
-// split storage: the potential's ring background, rewritten with the potential
-// every step.  Every body is seen from distance sqrt(R^2 + |r_k|^2) with the
-// ring's background smoothing, which keeps the stored deviations small away
-// from the bodies.
-__device__ __forceinline__ double potential_bg_value(const KParams &P, int i)
-{
-    const DevScalars *s = P.s;
-    const double R = P.Rmed[i];
-    const double smooth_h = P.thickness_smoothing * P.bg[FCPT_SCALE_HEIGHT][i];
-    double b = 0.0;
-    for (int k = 0; k < s->n_planets; ++k) {
-        const double px = s->planet[k].x, py = s->planet[k].y;
-        const double smooth = s->planet[k].smooth >= 0.0 ? s->planet[k].smooth : smooth_h;
-        b += -P.G * s->planet[k].mass / sqrt(R * R + (px * px + py * py) + smooth * smooth);
-    }
-    return b;
-}
-
 // x^4 and x^3 of the Klahr-Kley factor by compensated powering (Graillat, Langlois and Louvet,
 // "Algorithms for accurate, validated and fast polynomial evaluation",
 // Japan J. Indust. Appl. Math. 26, 2009): the square is carried as the
@@ -725,11 +767,11 @@ __device__ __forceinline__ double potential_body(const KParams &P, double x, dou
 // H of the cell), for kPotCells cells of ring i, sectors j0 + 64 c (those
 // below ns), with the bodies' loop outside the cells': every cell still adds the bodies
 // in order and then the indirect term, and a body's parameters are loaded
-// once per kPotCells cells.  Split storage: the deviations from the ring
-// background b.
+// once per kPotCells cells.  The potential is stored in FP64 in both storage
+// modes.
 constexpr int kPotCells = 4;
 template <bool SPLIT>
-__device__ __forceinline__ void potential_ring_cells(const KParams &P, int i, int j0, double b)
+__device__ __forceinline__ void potential_ring_cells(const KParams &P, int i, int j0)
 {
     const DevScalars *s = P.s;
     const int ns = P.ns;
@@ -757,26 +799,18 @@ __device__ __forceinline__ void potential_ring_cells(const KParams &P, int i, in
     for (int c = 0; c < kPotCells; ++c) {
         const int j = j0 + 64 * c;
         if (j >= ns) break;
-        const double a = acc[c] + (-ix * x[c] - iy * y[c]);
-        if constexpr (SPLIT)
-            static_cast<float *>(P.f[FCPT_POTENTIAL])[(size_t)i * (size_t)ns + (size_t)j] = (float)(a - b);
-        else
-            fld<SPLIT>(P, FCPT_POTENTIAL).set(i, j, a);
+        phi_fld(P).set(i, j, acc[c] + (-ix * x[c] - iy * y[c]));
     }
 }
 
-// the potential, blocks of 64 x kPotCells sectors by 4 rings; in split
-// storage each wavefront forms its ring's background itself (lane 0 of the
-// first column stores it)
+// the potential, blocks of 64 x kPotCells sectors by 4 rings
 template <bool SPLIT>
 __device__ __forceinline__ void b_potential(const KParams &P, int bx, int by)
 {
     const int i = wave_ring(0, by);
     const int j0 = bx * 64 * kPotCells + (int)threadIdx.x;
     if (j0 >= P.ns || i >= P.nr) return;
-    const double b = SPLIT ? potential_bg_value(P, i) : 0.0;
-    if (SPLIT && bx == 0 && threadIdx.x == 0) P.bg[FCPT_POTENTIAL][i] = b;
-    potential_ring_cells<SPLIT>(P, i, j0, b);
+    potential_ring_cells<SPLIT>(P, i, j0);
 }
 FCPT_PAIR0(potential, 256)
 
@@ -834,7 +868,7 @@ __device__ __forceinline__ void b_src_vr(const KParams &P, int bx, int by)
     CELL_LOOP_HEAD(2, P.nr - 1)
     FLD(S, FCPT_SIGMA);
     FLD(Pr, FCPT_PRESSURE);
-    FLD(Phi, FCPT_POTENTIAL);
+    const Fld<false> Phi = phi_fld(P);
     FLD(vaz, FCPT_VAZI);
     FLD(vr, FCPT_VRAD);
     const double dt = P.s->dt;
@@ -851,7 +885,7 @@ __device__ __forceinline__ void b_src_vphi(const KParams &P, int bx, int by)
     CELL_LOOP_HEAD(1, P.nr - 1)
     FLD(S, FCPT_SIGMA);
     FLD(Pr, FCPT_PRESSURE);
-    FLD(Phi, FCPT_POTENTIAL);
+    const Fld<false> Phi = phi_fld(P);
     FLD(vaz, FCPT_VAZI);
     const double dt = P.s->dt;
     const int jp = j == 0 ? ns - 1 : j - 1;
@@ -1426,12 +1460,12 @@ FCPT_PAIR0(visc_update, 256)
 template <bool SPLIT>
 __device__ __forceinline__ Fld<SPLIT> alt_vr(const KParams &P)
 {
-    return buf<SPLIT>(P, P.valt[0], P.bg[FCPT_VRAD]);
+    return buf<SPLIT>(P, P.valt[0], P.bg[FCPT_VRAD], FCPT_VRAD);
 }
 template <bool SPLIT>
 __device__ __forceinline__ Fld<SPLIT> alt_vaz(const KParams &P)
 {
-    return buf<SPLIT>(P, P.valt[1], P.bg[FCPT_VAZI]);
+    return buf<SPLIT>(P, P.valt[1], P.bg[FCPT_VAZI], FCPT_VAZI);
 }
 
 // update_with_sourceterms' two momentum updates (SourceEuler.cpp:325-428) in
@@ -1455,7 +1489,7 @@ __device__ __forceinline__ void b_src_fused(const KParams &P, int bx, int by)
     }
     FLD(S, FCPT_SIGMA);
     FLD(Pr, FCPT_PRESSURE);
-    FLD(Phi, FCPT_POTENTIAL);
+    const Fld<false> Phi = phi_fld(P);
     const bool iso = !P.adiabatic;
     const double dt = P.s->dt;
     const int jn = j == ns - 1 ? 0 : j + 1;
@@ -1552,7 +1586,7 @@ __device__ __forceinline__ void b_av_fused(const KParams &P, int bx, int by, int
     }
     if (diss) {
         E.set(i, j, e);
-        e = E.stored(i, e);
+        e = E.stored(i, j, e);
     }
     if (visc) derived_store<SPLIT>(P, i, j, DerivedIn{e, sig, 0.0, 0.0}, false, false);
     if (inner)
@@ -1621,7 +1655,7 @@ __device__ __forceinline__ void b_kick_walk(const KParams &P, int bx, int by, in
             w.va = vaz(r, j);
             w.s = S(r, j);
             w.p = ADI ? fld<SPLIT>(P, FCPT_PRESSURE)(r, j) : iso_pressure(P, r, w.s);
-            w.phi = fld<SPLIT>(P, FCPT_POTENTIAL)(r, j);
+            w.phi = phi_fld(P)(r, j);
         } else {
             w.va = w.s = w.p = w.phi = 0.0;
         }
@@ -1689,11 +1723,11 @@ __device__ __forceinline__ void b_kick_walk(const KParams &P, int bx, int by, in
             if constexpr (ADI) {
                 // compression_heating (rings 0 .. nr-2), then the dissipation
                 double e = E(i, j);
-                if (i < nr - 1) e = E.stored(i, src_energy_new(P, i, dt, e, v1r0, v1r1, v1a0, v1aR));
+                if (i < nr - 1) e = E.stored(i, j, src_energy_new(P, i, dt, e, v1r0, v1r1, v1a0, v1aR));
                 if (diss) e = sn ? sn_dissipation(P, i, q.sn, R0.s, dt, e) : tw_dissipation(P, i, q.tw, R0.s, dt, e);
                 if (i < nr - 1 || diss) {
                     E.set(i, j, e);
-                    e = E.stored(i, e);
+                    e = E.stored(i, j, e);
                 }
                 // recalculate_viscosity
                 const AdiDerived d = adi_derived(P, i, e, R0.s);
@@ -2306,16 +2340,20 @@ __device__ __forceinline__ void b_ghost_ring_mean(const KParams &P, int bx)
     const int i = bx - ghost_blocks;
     if (i == 0 || i == P.nr - 1) {
         __shared__ double2 sh[256];
-        const double g = fld<SPLIT>(P, FCPT_VAZI).stored(i, ghost_vaz(P, i));
+        // the ghost ring's v_phi as stored by ghost_inner / ghost_outer in
+        // this launch (split storage with stochastic rounding: per sector)
+        const Fld<SPLIT> vaz = fld<SPLIT>(P, FCPT_VAZI);
+        const double g = ghost_vaz(P, i);
         double acc = 0.0, aacc = 0.0;
         for (int j = (int)threadIdx.x; j < P.ns; j += 256) {
-            acc += g;
-            aacc += fabs(g);
+            const double gj = vaz.stored(i, j, g);
+            acc += gj;
+            aacc += fabs(gj);
         }
         const double2 r = block_sum_abs<256>(acc, aacc, sh);
         ring_mean_store(P, i, r.x, r.y, 1, [&]() {
             double seq = 0.0;
-            for (int j = 0; j < P.ns; ++j) seq += g;
+            for (int j = 0; j < P.ns; ++j) seq += vaz.stored(i, j, g);
             return seq;
         });
         return;
@@ -2527,7 +2565,7 @@ __device__ __forceinline__ void radial_sweep(const KParams &P, int bx, int by, i
         const double invs = cst(P.InvSurf)[i];
 #pragma unroll
         for (int q = 0; q < 4; ++q)
-            buf<SPLIT>(P, P.mom_b[q], P.bgm[q]).set(i, j, A.q[q] + (f_prev[q] - f[q]) * invs);
+            buf<SPLIT>(P, P.mom_b[q], P.bgm[q], kBufMom + q).set(i, j, A.q[q] + (f_prev[q] - f[q]) * invs);
         if constexpr (ADI) buf<false>(P, P.en_b, nullptr).set(i, j, A.q[4] + (f_prev[4] - f[4]) * invs);
         buf<false>(P, P.sig_b, nullptr).set(i, j, A.s + (fs_prev - fs) * invs);
 #pragma unroll
@@ -2638,36 +2676,26 @@ __device__ __forceinline__ D2 flux_div2(const D2 &f)
     return D2{va, vb};
 }
 
+// The results of both FARGO passes for this lane's two sectors of ring i:
+// the transported density and, per quantity, the transported momentum (and
+// energy), before the cyclic shift; sources at sectors jsa, jsb = jsa + 1.
+template <bool ADI>
+struct AzOut {
+    D2 s;
+    D2 q[ADI ? 5 : 4];
+};
+
 template <bool SPLIT, bool ADI>
-__device__ __forceinline__ void b_azimuthal_sweep(const KParams &P, int bx, int by)
+__device__ __forceinline__ void az_ring(const KParams &P, int i, int jsa, int jsb, double dt, AzOut<ADI> &o)
 {
     constexpr int NQ = ADI ? 5 : 4;
-    const int ns = P.ns;
-    const int i = by;
-    const int lane = threadIdx.x & 63;
-    const int j0 = __builtin_amdgcn_readfirstlane(bx * kAzWaves + (int)(threadIdx.x >> 6)) * kAzSegW;
-    if (j0 >= ns) return;  // the whole wavefront
-    const int Ls = ns - j0 < kAzSegW ? ns - j0 : kAzSegW;
-    const double dt = P.s->dt;
     const double vm = cst(P.vmean)[i];
     const double vu = cst(P.vuni)[i];
     const double ksu = vu * dt;
-    const int nsh = cst(P.nshift)[i];
     const double dxtheta = P.dphi * cst(P.Rmed)[i];
     const double invdxtheta = 1.0 / dxtheta;
     const double dxrad = (cst(P.Rsup)[i] - cst(P.Rinf)[i]) * dt;
     const double invsurf = cst(P.InvSurf)[i];
-    // this lane's sectors (offset k = 2 lane, 2 lane + 1 from the segment's
-    // first halo sector) and where their results go
-    const int ka = 2 * lane, kb = 2 * lane + 1;
-    int jsa = (j0 - kAzHalo + ka) % ns;
-    if (jsa < 0) jsa += ns;
-    const int jsb = jsa + 1 == ns ? 0 : jsa + 1;
-    int jda = (j0 - kAzHalo + ka + nsh) % ns;
-    if (jda < 0) jda += ns;
-    const int jdb = jda + 1 == ns ? 0 : jda + 1;
-    const bool wa = ka >= kAzHalo && ka < kAzHalo + Ls;
-    const bool wb = kb >= kAzHalo && kb < kAzHalo + Ls;
     // every global load first
     FLD(vaz, FCPT_VAZI);
     const Fld<false> Sb = buf<false>(P, P.sig_b, nullptr);
@@ -2676,7 +2704,7 @@ __device__ __forceinline__ void b_azimuthal_sweep(const KParams &P, int bx, int 
     D2 q0[NQ];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        const Fld<SPLIT> m = buf<SPLIT>(P, P.mom_b[q], P.bgm[q]);
+        const Fld<SPLIT> m = buf<SPLIT>(P, P.mom_b[q], P.bgm[q], kBufMom + q);
         q0[q] = D2{m(i, jsa), m(i, jsb)};
     }
     if constexpr (ADI) {
@@ -2691,9 +2719,7 @@ __device__ __forceinline__ void b_azimuthal_sweep(const KParams &P, int bx, int 
     const D2 s1{s0.a + vd0.a * invsurf, s0.b + vd0.b * invsurf};
     const D2 ds1 = star2(s1, slope2<SPLIT>(s1, invdxtheta), dxtheta, ksu2);
     const D2 vd1 = flux_div2(D2{dxrad * ds1.a * vu, dxrad * ds1.b * vu});
-    const Fld<false> Sc = buf<false>(P, P.sig_c, nullptr);
-    if (wa) Sc.set(i, jda, s1.a + vd1.a * invsurf);
-    if (wb) Sc.set(i, jdb, s1.b + vd1.b * invsurf);
+    o.s = D2{s1.a + vd1.a * invsurf, s1.b + vd1.b * invsurf};
     // the other quantities
     const D2 inv_s0{SPLIT ? recip<true>(s0.a) : 0.0, SPLIT ? recip<true>(s0.b) : 0.0};
     const D2 inv_s1{SPLIT ? recip<true>(s1.a) : 0.0, SPLIT ? recip<true>(s1.b) : 0.0};
@@ -2709,15 +2735,48 @@ __device__ __forceinline__ void b_azimuthal_sweep(const KParams &P, int bx, int 
         const D2 w1{SPLIT ? q1.a * inv_s1.a : shared_div(q1.a, s1a), SPLIT ? q1.b * inv_s1.b : shared_div(q1.b, s1b)};
         const D2 qs1 = star2(w1, slope2<SPLIT>(w1, invdxtheta), dxtheta, ksu2);
         const D2 vq1 = flux_div2(D2{dxrad * qs1.a * ds1.a * vu, dxrad * qs1.b * ds1.b * vu});
-        const D2 q2{q1.a + vq1.a * invsurf, q1.b + vq1.b * invsurf};
+        o.q[q] = D2{q1.a + vq1.a * invsurf, q1.b + vq1.b * invsurf};
+    }
+}
+
+template <bool SPLIT, bool ADI>
+__device__ __forceinline__ void b_azimuthal_sweep(const KParams &P, int bx, int by)
+{
+    constexpr int NQ = ADI ? 5 : 4;
+    const int ns = P.ns;
+    const int i = by;
+    const int lane = threadIdx.x & 63;
+    const int j0 = __builtin_amdgcn_readfirstlane(bx * kAzWaves + (int)(threadIdx.x >> 6)) * kAzSegW;
+    if (j0 >= ns) return;  // the whole wavefront
+    const int Ls = ns - j0 < kAzSegW ? ns - j0 : kAzSegW;
+    const double dt = P.s->dt;
+    const int nsh = cst(P.nshift)[i];
+    // this lane's sectors (offset k = 2 lane, 2 lane + 1 from the segment's
+    // first halo sector) and where their results go
+    const int ka = 2 * lane, kb = 2 * lane + 1;
+    int jsa = (j0 - kAzHalo + ka) % ns;
+    if (jsa < 0) jsa += ns;
+    const int jsb = jsa + 1 == ns ? 0 : jsa + 1;
+    int jda = (j0 - kAzHalo + ka + nsh) % ns;
+    if (jda < 0) jda += ns;
+    const int jdb = jda + 1 == ns ? 0 : jda + 1;
+    const bool wa = ka >= kAzHalo && ka < kAzHalo + Ls;
+    const bool wb = kb >= kAzHalo && kb < kAzHalo + Ls;
+    AzOut<ADI> o;
+    az_ring<SPLIT, ADI>(P, i, jsa, jsb, dt, o);
+    const Fld<false> Sc = buf<false>(P, P.sig_c, nullptr);
+    if (wa) Sc.set(i, jda, o.s.a);
+    if (wb) Sc.set(i, jdb, o.s.b);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
         if (q < 4) {
-            const Fld<SPLIT> m = buf<SPLIT>(P, P.mom[q], P.bgm[q]);
-            if (wa) m.set(i, jda, q2.a);
-            if (wb) m.set(i, jdb, q2.b);
+            const Fld<SPLIT> m = buf<SPLIT>(P, P.mom[q], P.bgm[q], kBufMom + q);
+            if (wa) m.set(i, jda, o.q[q].a);
+            if (wb) m.set(i, jdb, o.q[q].b);
         } else {
             const Fld<false> Ec = buf<false>(P, P.en_c, nullptr);
-            if (wa) Ec.set(i, jda, q2.a);
-            if (wb) Ec.set(i, jdb, q2.b);
+            if (wa) Ec.set(i, jda, o.q[q].a);
+            if (wb) Ec.set(i, jdb, o.q[q].b);
         }
     }
 }
@@ -2741,10 +2800,10 @@ __device__ __forceinline__ void b_velocities(const KParams &P, int bx, int by)
 {
     CELL_LOOP_HEAD(0, P.nr)
     const Fld<false> Sb = buf<false>(P, P.sig_c, nullptr);
-    const Fld<SPLIT> m0 = buf<SPLIT>(P, P.mom[0], P.bgm[0]);
-    const Fld<SPLIT> m1 = buf<SPLIT>(P, P.mom[1], P.bgm[1]);
-    const Fld<SPLIT> m2 = buf<SPLIT>(P, P.mom[2], P.bgm[2]);
-    const Fld<SPLIT> m3 = buf<SPLIT>(P, P.mom[3], P.bgm[3]);
+    const Fld<SPLIT> m0 = buf<SPLIT>(P, P.mom[0], P.bgm[0], kBufMom + 0);
+    const Fld<SPLIT> m1 = buf<SPLIT>(P, P.mom[1], P.bgm[1], kBufMom + 1);
+    const Fld<SPLIT> m2 = buf<SPLIT>(P, P.mom[2], P.bgm[2], kBufMom + 2);
+    const Fld<SPLIT> m3 = buf<SPLIT>(P, P.mom[3], P.bgm[3], kBufMom + 3);
     const Fld<false> Eb = buf<false>(P, P.en_c, nullptr);
     FLD(vr, FCPT_VRAD);
     FLD(vaz, FCPT_VAZI);
@@ -2777,7 +2836,7 @@ __device__ __forceinline__ double damp_stored(const KParams &P, const Fld<SPLIT>
     if (!P.damping) return x;
     const bool in = damp_in_inner(P, vector, i), out = damp_in_outer(P, vector, i);
     if (!in && !out) return x;
-    x = X.stored(i, x);
+    x = X.stored(i, j, x);
     const double x0 = fld<SPLIT>(P, f0)(i, j);
     if (in) x = damp_apply(P, vector, true, i, dt, x, x0);
     if (out) x = damp_apply(P, vector, false, i, dt, x, x0);
@@ -2800,8 +2859,8 @@ __device__ __forceinline__ double vel_final_vr(const KParams &P, int r, int j, d
 {
     // v_r(r >= 1) as the velocity kernel and the damping pass it to set()
     const Fld<false> Sb = buf<false>(P, P.sig_c, nullptr);
-    const Fld<SPLIT> m0 = buf<SPLIT>(P, P.mom[0], P.bgm[0]);
-    const Fld<SPLIT> m1 = buf<SPLIT>(P, P.mom[1], P.bgm[1]);
+    const Fld<SPLIT> m0 = buf<SPLIT>(P, P.mom[0], P.bgm[0], kBufMom + 0);
+    const Fld<SPLIT> m1 = buf<SPLIT>(P, P.mom[1], P.bgm[1], kBufMom + 1);
     FLD(vr, FCPT_VRAD);
     const double v = (m0(r - 1, j) + m1(r, j)) / (Sb(r - 1, j) + Sb(r, j));
     return damp_stored<SPLIT>(P, vr, FCPT_VRAD0, true, r, j, dt, v);
@@ -2844,21 +2903,21 @@ __device__ __forceinline__ void b_velocities_final(const KParams &P, int bx, int
         const int r = i == 0 ? 1 : nr - 2;  // the ring the ghost ring copies
         vel_final_sig_e<SPLIT>(P, r, j, dt, sr, er, true);
         // the ghost ring takes the neighbour's values as a load returns them
-        sr = S.stored(r, sr);
-        if (P.adiabatic) er = E.stored(r, er);
-        if (i == 0) ghost_vr_inner(P, vr.stored(2, vel_final_vr<SPLIT>(P, 2, j, dt)), va, vb);
-        else ghost_vr_outer(P, vr.stored(nr - 2, vel_final_vr<SPLIT>(P, nr - 2, j, dt)), va, vb);
+        sr = S.stored(r, j, sr);
+        if (P.adiabatic) er = E.stored(r, j, er);
+        if (i == 0) ghost_vr_inner(P, vr.stored(2, j, vel_final_vr<SPLIT>(P, 2, j, dt)), va, vb);
+        else ghost_vr_outer(P, vr.stored(nr - 2, j, vel_final_vr<SPLIT>(P, nr - 2, j, dt)), va, vb);
         vphi = ghost_vaz(P, i);
     } else {
         const Fld<false> Sb = buf<false>(P, P.sig_c, nullptr);
-        const Fld<SPLIT> m2 = buf<SPLIT>(P, P.mom[2], P.bgm[2]);
-        const Fld<SPLIT> m3 = buf<SPLIT>(P, P.mom[3], P.bgm[3]);
+        const Fld<SPLIT> m2 = buf<SPLIT>(P, P.mom[2], P.bgm[2], kBufMom + 2);
+        const Fld<SPLIT> m3 = buf<SPLIT>(P, P.mom[3], P.bgm[3], kBufMom + 3);
         const int jm = j == 0 ? ns - 1 : j - 1;
         const double s0 = Sb(i, j), sjm = Sb(i, jm);
         const double v_phi =
             (m2(i, jm) + m3(i, j)) / (sjm + s0) * cst(P.InvRmed)[i] - cst(P.Rmed)[i] * P.omega_frame;
         vphi = damp_stored<SPLIT>(P, vaz, FCPT_VAZI0, false, i, j, dt, v_phi);
-        if (i == 1) ghost_vr_inner(P, vr.stored(2, vel_final_vr<SPLIT>(P, 2, j, dt)), va, vb);
+        if (i == 1) ghost_vr_inner(P, vr.stored(2, j, vel_final_vr<SPLIT>(P, 2, j, dt)), va, vb);
         else vb = vel_final_vr<SPLIT>(P, i, j, dt);
         vel_final_sig_e<SPLIT>(P, i, j, dt, sr, er, false);
     }
@@ -2875,7 +2934,7 @@ __device__ __forceinline__ void b_velocities_final(const KParams &P, int bx, int
         vr.set(i, j, vb);  // v_r(1): the ghost value
     }
     if (advance_time && i == 0 && j == 0) advance_clock(P, advance_time);
-    DerivedIn d{P.adiabatic ? E.stored(i, er) : 0.0, S.stored(i, sr), 0.0, 0.0};
+    DerivedIn d{P.adiabatic ? E.stored(i, j, er) : 0.0, S.stored(i, j, sr), 0.0, 0.0};
     if (!P.adiabatic) {
         d.cs = cst(P.iso_cs)[i];
         d.h = cst(P.iso_h)[i];
@@ -2986,6 +3045,10 @@ static inline dim3 cell_grid(const KParams &P, int rows)
 static const dim3 kBlock(64, 4);
 
 size_t element_size(const KParams &P) { return P.split ? sizeof(float) : sizeof(double); }
+size_t field_element_size(const KParams &P, int field)
+{
+    return field == FCPT_POTENTIAL ? sizeof(double) : element_size(P);
+}
 
 void launch_ring_mean(const KParams &P, hipStream_t st, int mode)
 {

@@ -81,6 +81,8 @@ void launch_rebase(const KParams &P, hipStream_t st, void *field, const double *
                    const double *newbg, int rows);
 // bytes of one stored value of a 2-D field (8 in FP64 storage, 4 in split storage)
 size_t element_size(const KParams &P);
+// bytes per value of one field: the potential is FP64 in both storage modes
+size_t field_element_size(const KParams &P, int field);
 // rings per thread of the radial sweep (requested > 0 overrides the size-based choice)
 int radial_tile_for(int nr, int ns, int requested);
 // fills the launch-shape fields of a disk's parameters (the CFL grid)

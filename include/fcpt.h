@@ -48,12 +48,15 @@ enum { FCPT_BC_REFLECTING = 0, FCPT_BC_ZEROGRADIENT = 1, FCPT_BC_OUTFLOW = 2 };
 enum { FCPT_BETA_REF_ZERO = 0, FCPT_BETA_REF_REFERENCE = 1, FCPT_BETA_REF_FLOOR = 3 };
 /* storage of the 2-D fields.  Arithmetic is FP64 in both modes.
    FCPT_PREC_FP64:    every field in FP64 (results match the reference's host build).
-   FCPT_PREC_SPLIT32: every 2-D field stored as an FP32 deviation from an FP64
-                      per-ring background (value = background[ring] + deviation),
-                      halving the HBM traffic of the step.  The backgrounds are
+   FCPT_PREC_SPLIT32: every 2-D field but the gas potential stored as an FP32
+                      deviation from an FP64 per-ring background (value =
+                      background[ring] + deviation), halving the HBM traffic of
+                      the step; the potential stays FP64.  The backgrounds are
                       the ring means of the state given to fcpt_upload_state /
                       fcpt_upload_field and the profiles derived from them;
-                      downloads return background + deviation in FP64. */
+                      downloads return background + deviation in FP64.  Stored
+                      deviations are rounded stochastically by default
+                      (FCPT_PARAM_SPLIT_ROUNDING). */
 enum { FCPT_PREC_FP64 = 0, FCPT_PREC_SPLIT32 = 1 };
 /* hydro integrator (reference types.h:64-65, Interpret.cpp:173-186): the
    LeapFrog scheme also scales the artificial-viscosity, viscosity and
@@ -307,8 +310,14 @@ int fcpt_time_copy(fcpt_ctx *ctx, int32_t reps, double *ms_per_launch, double *b
    points above, which always store them.  Every parameter change drops a
    captured step graph so that the next fcpt_run_steps captures the new
    shape. */
+/* FCPT_PARAM_SPLIT_ROUNDING (split storage only; default 1) chooses how a
+   stored value's FP32 deviation is rounded: 1 stochastically (up or down with
+   probabilities given by the distances to the two neighbouring FP32 values, a
+   deterministic hash of the value, the cell, the field and the step count
+   drawing the variate: unbiased, so a slowly changing value does not lose its
+   sub-unit increments every step), 0 to nearest. */
 enum { FCPT_PARAM_RADIAL_TILE = 0, FCPT_PARAM_RADIAL_WAVES = 1, FCPT_PARAM_FUSED_STEP = 2,
-       FCPT_PARAM_VISC_TILE = 3, FCPT_PARAM_KICK_TILE = 4 };
+       FCPT_PARAM_VISC_TILE = 3, FCPT_PARAM_KICK_TILE = 4, FCPT_PARAM_SPLIT_ROUNDING = 5 };
 int fcpt_set_launch_param(fcpt_ctx *ctx, int32_t param, int32_t value);
 /* Autotuner (no counterpart in the reference): times every launch shape of
    the tunable kernels on the context's current state (`reps` launches each;

@@ -104,6 +104,35 @@ def test_split_vs_fp64_2048x6144_1000_steps():
     assert abs(t32 - t64) / t64 <= 1e-6
 
 
+def test_split_vs_fp64_4096x12288_beta_1000_steps():
+    """The large adiabatic disk with beta cooling and three planets: split
+    storage against FP64 from the same start after 1000 steps, every evolved
+    field's error stated, Sigma held to 1e-5 relative (max |d| / max |Sigma|
+    and per cell)."""
+    name = "adi_beta_4096x12288"
+    print(f"{name}: 1000 steps FP64 and split", flush=True)
+    fp64, split = _pair(name, 1000)
+    errs, cell = {}, {}
+    for f in ("SIGMA", "VRAD", "VAZI", "ENERGY"):
+        a, b = split.download_field(f), fp64.download_field(f)
+        errs[f] = rel_err(a, b)
+        if f in ("SIGMA", "ENERGY"):
+            cell[f] = float(np.max(np.abs(a - b) / np.abs(b)))
+    m64, l64 = fp64.totals(1e300)
+    m32, l32 = split.totals(1e300)
+    t64, t32 = fp64.get_time()[0], split.get_time()[0]
+    print(f"errors {errs} per-cell {cell} mass {abs(m32 - m64) / m64:.2e} "
+          f"angmom {abs(l32 - l64) / abs(l64):.2e} t {t64} {t32}", flush=True)
+    fp64.close()
+    split.close()
+    assert errs["SIGMA"] <= SIGMA_TOL_1000 and cell["SIGMA"] <= SIGMA_TOL_1000
+    assert errs["ENERGY"] <= SIGMA_TOL_1000
+    assert errs["VRAD"] <= VEL_TOL_1000 and errs["VAZI"] <= VEL_TOL_1000
+    assert abs(m32 - m64) / m64 <= 1e-7
+    assert abs(l32 - l64) / abs(l64) <= 1e-6
+    assert abs(t32 - t64) / t64 <= 1e-6
+
+
 def test_split_graph_matches_stepwise():
     from fargocpt_amd import workloads as W
     from fargocpt_amd.capi import Disk
