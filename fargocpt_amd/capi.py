@@ -212,6 +212,8 @@ def load_library(path: str = LIB_PATH):
         "fcpt_step_euler": [P, D],
         "fcpt_run_step": [P, dp],
         "fcpt_totals": [P, D, dp, dp],
+        "fcpt_get_ring_geometry": [P, dp, dp],
+        "fcpt_totals_noalloc": [I, I, dp, dp, P, P, D, D, P, dp, dp],
         "fcpt_run_steps": [P, I, I],
         "fcpt_time_steps": [P, I, I, dp],
         "fcpt_time_kernel": [P, I, I, dp],
@@ -235,8 +237,24 @@ def load_library(path: str = LIB_PATH):
     lib.fcpt_last_error.restype = C.c_char_p
     lib.fcpt_batch_last_error.argtypes = [P]
     lib.fcpt_batch_last_error.restype = C.c_char_p
+    lib.fcpt_noalloc_last_error.argtypes = []
+    lib.fcpt_noalloc_last_error.restype = C.c_char_p
     _lib = lib
     return lib
+
+
+def totals_noalloc(nr, nphi, rmed, surf, sigma_dev, vazi_dev, omega_frame, radius, stream=None):
+    """fcpt_totals_noalloc over caller-owned FP64 device arrays (device
+    addresses as ints); returns (mass, angular momentum)."""
+    lib = load_library()
+    rmed = np.ascontiguousarray(rmed, dtype=np.float64)
+    surf = np.ascontiguousarray(surf, dtype=np.float64)
+    m, l = C.c_double(), C.c_double()
+    rc = lib.fcpt_totals_noalloc(nr, nphi, _dptr(rmed), _dptr(surf), sigma_dev, vazi_dev,
+                                 omega_frame, radius, stream, C.byref(m), C.byref(l))
+    if rc != 0:
+        raise RuntimeError(f"fcpt_totals_noalloc -> {rc}: {lib.fcpt_noalloc_last_error().decode()}")
+    return m.value, l.value
 
 
 def exported_symbols() -> list[str]:
@@ -417,6 +435,12 @@ class Disk:
         m, l = C.c_double(), C.c_double()
         self._check(self.lib.fcpt_totals(self.h, radius, C.byref(m), C.byref(l)))
         return m.value, l.value
+
+    def ring_geometry(self):
+        """Rmed and Surf of rings 0 .. nr-1 as the context computed them."""
+        rmed, surf = np.empty(self.nr), np.empty(self.nr)
+        self._check(self.lib.fcpt_get_ring_geometry(self.h, _dptr(rmed), _dptr(surf)))
+        return rmed, surf
 
     def synchronize(self):
         self._check(self.lib.fcpt_synchronize(self.h))

@@ -8,6 +8,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -37,7 +38,7 @@ struct fcpt_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::string err;
     // host copies of the per-ring metric (split storage derives its backgrounds from them)
-    std::vector<double> Rinf, Rmed, InvRmed, InvDiffRmed, InvDiffRsup, InvDiffRsupRb;
+    std::vector<double> Rinf, Rmed, Surf, InvRmed, InvDiffRmed, InvDiffRsup, InvDiffRsupRb;
     // split storage: host copies of the backgrounds last set from the host, and
     // a device ring array for re-expressing a field about a new background
     std::vector<double> hbg[FCPT_NUM_FIELDS];
@@ -336,6 +337,7 @@ int fcpt_create(const fcpt_disk_desc *desc, fcpt_ctx **out)
     for (int i = 1; i < nr + 1; ++i) InvDiffRmed[i] = 1.0 / (Rmed[i] - Rmed[i - 1]);
     c->Rinf = Rinf;
     c->Rmed = Rmed;
+    c->Surf = Surf;
     c->InvRmed = InvRmed;
     c->InvDiffRmed = InvDiffRmed;
     c->InvDiffRsup = InvDiffRsup;
@@ -1504,6 +1506,115 @@ int fcpt_totals(fcpt_ctx *ctx, double radius, double *mass, double *angular_mome
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     if (mass) *mass = ctx->totals_host[0];
     if (angular_momentum) *angular_momentum = ctx->totals_host[1];
+    return FCPT_OK;
+}
+
+int fcpt_get_ring_geometry(fcpt_ctx *ctx, double *rmed, double *surf)
+{
+    if (!ctx) return FCPT_ERR_ARG;
+    const size_t n = (size_t)ctx->P.nr;
+    if (rmed) memcpy(rmed, ctx->Rmed.data(), n * sizeof(double));
+    if (surf) memcpy(surf, ctx->Surf.data(), n * sizeof(double));
+    return FCPT_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Totals that allocate nothing.  The ring geometry, the per-ring partial sums
+// and the two results live in `scratch`, a file-scope device array: every GPU
+// the library runs on holds its own copy until the process ends, at its own
+// address, which is looked up once per device with hipGetSymbolAddress; host
+// code uses only that address (the array's name is a host-side stand-in).
+// Layout for nr rings: Rmed [0, nr), Surf [nr, 2 nr), ring partials
+// [2 nr, 6 nr) (four per ring), results [6 nr, 6 nr + 2).  Every call writes
+// each slot it later reads (the ring kernel writes all 4 nr partials, the
+// fold both results), so nothing has to start at zero.  Calls on one GPU
+// share the array: calls that overlap on one GPU race.
+constexpr int kNoAllocMaxRings = FCPT_NOALLOC_MAX_RINGS;
+static __device__ __align__(256) double scratch[6 * kNoAllocMaxRings + 2];
+
+namespace {
+constexpr int kNoAllocMaxDevices = 64;
+
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kNoAllocMaxDevices];
+thread_local std::string g_noalloc_err;
+
+int noalloc_fail(int code, const char *fmt, ...)
+{
+    char buf[256];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_noalloc_err = buf;
+    return code;
+}
+
+// the current device's copy of `scratch`
+int scratch_address(double **out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return noalloc_fail(FCPT_ERR_DEVICE, "hipGetDevice: %s", hipGetErrorString(e));
+    if (dev < 0 || dev >= kNoAllocMaxDevices)
+        return noalloc_fail(FCPT_ERR_DEVICE, "device %d beyond the %d devices served", dev, kNoAllocMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return noalloc_fail(FCPT_ERR_DEVICE, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+        g_scratch_ptr[dev] = static_cast<double *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return FCPT_OK;
+}
+}  // namespace
+
+extern "C" {
+
+const char *fcpt_noalloc_last_error(void) { return g_noalloc_err.c_str(); }
+
+int fcpt_totals_noalloc(int32_t nr, int32_t nphi, const double *rmed, const double *surf,
+                        const double *sigma, const double *vazi, double omega_frame, double radius,
+                        void *stream, double *mass, double *angular_momentum)
+{
+    g_noalloc_err.clear();
+    if (!rmed || !surf || !sigma || !vazi || nr < 3 || nphi < 1)
+        return noalloc_fail(FCPT_ERR_ARG, "null pointer or nr < 3 / nphi < 1");
+    if (nr > kNoAllocMaxRings)
+        return noalloc_fail(FCPT_ERR_ARG, "nr = %d exceeds the %d rings of the static scratch", nr,
+                            kNoAllocMaxRings);
+    double *s = nullptr;
+    int rc = scratch_address(&s);
+    if (rc) return rc;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    double *g_rmed = s, *g_surf = s + nr, *ring = s + 2 * (size_t)nr, *out = s + 6 * (size_t)nr;
+    hipError_t e = hipMemcpyAsync(g_rmed, rmed, (size_t)nr * sizeof(double), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(g_surf, surf, (size_t)nr * sizeof(double), hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) return noalloc_fail(FCPT_ERR_DEVICE, "geometry upload: %s", hipGetErrorString(e));
+    // only the fields k_totals_rings reads: sizes, Rmed, Surf, the frame, Sigma, v_phi
+    KParams P{};
+    P.nr = nr;
+    P.ns = nphi;
+    P.Rmed = g_rmed;
+    P.Surf = g_surf;
+    P.omega_frame = omega_frame;
+    P.split = 0;
+    P.f[FCPT_SIGMA] = const_cast<double *>(sigma);
+    P.f[FCPT_VAZI] = const_cast<double *>(vazi);
+    launch_totals(P, st, radius, ring, out);
+    e = hipGetLastError();
+    if (e != hipSuccess) return noalloc_fail(FCPT_ERR_DEVICE, "kernel launch: %s", hipGetErrorString(e));
+    double res[2];
+    e = hipMemcpyAsync(res, out, 2 * sizeof(double), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return noalloc_fail(FCPT_ERR_DEVICE, "result read-back: %s", hipGetErrorString(e));
+    if (mass) *mass = res[0];
+    if (angular_momentum) *angular_momentum = res[1];
     return FCPT_OK;
 }
 

@@ -415,6 +415,29 @@ int fcpt_disk_on_planet_accel(fcpt_ctx *ctx, const fcpt_force_body *bodies, int3
    (quantities::gas_total_mass / gas_angular_momentum, quantities.cpp:51-78,242-279). */
 int fcpt_totals(fcpt_ctx *ctx, double radius, double *mass, double *angular_momentum);
 
+/* The ring geometry the context computed at fcpt_create (init_radialarrays,
+   init.cpp:188-225): Rmed[0 .. nr-1] and Surf[0 .. nr-1] into host arrays of
+   nr values (either may be NULL). */
+int fcpt_get_ring_geometry(fcpt_ctx *ctx, double *rmed, double *surf);
+
+/* No counterpart in the reference.  fcpt_totals for callers that cannot
+   allocate device memory (code under HIP graph capture, tight serving loops,
+   harnesses that forbid hipMalloc): the same kernels and summation order over
+   caller-owned FP64 device arrays sigma and vazi (nr x nphi, the reference
+   layout) with the host geometry rmed, surf (nr values each, e.g. from
+   fcpt_get_ring_geometry), with no context.  The ring geometry, the per-ring
+   partial sums and the results go through a static device array of each GPU
+   (6 FCPT_NOALLOC_MAX_RINGS + 2 doubles, held until the process ends); the
+   call allocates nothing.  Runs on `stream` (a hipStream_t, NULL for the null
+   stream) and waits for it.  nr above FCPT_NOALLOC_MAX_RINGS is rejected with
+   FCPT_ERR_ARG; the message is read by fcpt_noalloc_last_error().  Calls that
+   overlap on one GPU share that array and race. */
+#define FCPT_NOALLOC_MAX_RINGS 16384
+int fcpt_totals_noalloc(int32_t nr, int32_t nphi, const double *rmed, const double *surf,
+                        const double *sigma, const double *vazi, double omega_frame, double radius,
+                        void *stream, double *mass, double *angular_momentum);
+const char *fcpt_noalloc_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -56,3 +56,17 @@ def test_batch_members_are_reproducible():
         assert 1e-5 <= m["q"] <= 1e-3 and 1e-4 <= m["alpha"] <= 1e-2 and 0.03 <= m["h"] <= 0.07
     cfg = W.member_config(a[0], 1)
     assert cfg["nr"] == a[0]["nr"] and cfg["precision"] == 1 and len(cfg["radii"]) == a[0]["nr"] + 2
+
+
+def test_totals_noalloc_rejects_bad_sizes_before_touching_the_device():
+    lib = capi.load_library()
+    geo = np.ones(20000)
+    m, l = C.c_double(), C.c_double()
+    dp = geo.ctypes.data_as(C.POINTER(C.c_double))
+    fake = C.c_void_p(0x1000)  # never dereferenced: the size check comes first
+    rc = lib.fcpt_totals_noalloc(20000, 64, dp, dp, fake, fake, 1.0, 1e300, None,
+                                 C.byref(m), C.byref(l))
+    assert rc == -1
+    assert b"exceeds" in lib.fcpt_noalloc_last_error()
+    rc = lib.fcpt_totals_noalloc(128, 64, dp, dp, None, fake, 1.0, 1e300, None, C.byref(m), C.byref(l))
+    assert rc == -1
