@@ -115,7 +115,10 @@ def timed_run(name: str, prec: int, steps: int, warmup: int, use_graph: bool, tu
     W.start(disk, name)
     disk.run_steps(warmup, use_graph)
     disk.synchronize()
-    disk.tuned = disk.autotune(10) if tune else None
+    disk.tuned = disk.autotune(10) if tune else {}
+    from fargocpt_amd.capi import PARAM_KICK_TILE, PARAM_VISC_TILE
+    disk.tuned.update({"kick_tile": disk.launch_param(PARAM_KICK_TILE),
+                       "visc_tile": disk.launch_param(PARAM_VISC_TILE)})
     # the autotuner drops the captured graph: capture it again outside the timing
     disk.run_steps(1, use_graph)
     disk.synchronize()

@@ -219,6 +219,7 @@ def load_library(path: str = LIB_PATH):
         "fcpt_time_kernel": [P, I, I, dp],
         "fcpt_time_copy": [P, I, dp, dp],
         "fcpt_set_launch_param": [P, I, I],
+        "fcpt_get_launch_param": [P, I, C.POINTER(I)],
         "fcpt_autotune": [P, I, C.POINTER(I), C.POINTER(I)],
         "fcpt_disk_on_planet_accel": [P, C.POINTER(ForceBody), I, I, dp],
         "fcpt_batch_create": [C.POINTER(P), I, C.POINTER(P)],
@@ -491,10 +492,17 @@ class Disk:
         passes)."""
         self._check(self.lib.fcpt_set_launch_param(self.h, PARAM_KICK_TILE, rings))
 
-    def set_split_rounding(self, stochastic):
-        """Split storage: round deviations stochastically (True, the
-        default) or to nearest (False)."""
-        self._check(self.lib.fcpt_set_launch_param(self.h, PARAM_SPLIT_ROUNDING, 1 if stochastic else 0))
+    def set_split_rounding(self, mode):
+        """Split storage: 1 (default) rounds the evolved fields' deviations
+        stochastically and the derived fields' to nearest, 2 every field
+        stochastically, 0 every field to nearest."""
+        self._check(self.lib.fcpt_set_launch_param(self.h, PARAM_SPLIT_ROUNDING, int(mode)))
+
+    def launch_param(self, param):
+        """The value launch parameter `param` (PARAM_*) takes in the next step."""
+        v = C.c_int32()
+        self._check(self.lib.fcpt_get_launch_param(self.h, param, C.byref(v)))
+        return v.value
 
     def autotune(self, reps=10):
         """Time the tunable launch shapes on the current state and keep the

@@ -1248,7 +1248,7 @@ int fcpt_set_launch_param(fcpt_ctx *ctx, int32_t param, int32_t value)
     } else if (param == FCPT_PARAM_KICK_TILE) {
         if (value < 0 || value > 64) return fail(ctx, FCPT_ERR_ARG, "kick tile must be 0 .. 64");
     } else if (param == FCPT_PARAM_SPLIT_ROUNDING) {
-        if (value != 0 && value != 1) return fail(ctx, FCPT_ERR_ARG, "split rounding must be 0 or 1");
+        if (value < 0 || value > 2) return fail(ctx, FCPT_ERR_ARG, "split rounding must be 0, 1 or 2");
     } else {
         return fail(ctx, FCPT_ERR_ARG, "unknown launch parameter");
     }
@@ -1262,6 +1262,22 @@ int fcpt_set_launch_param(fcpt_ctx *ctx, int32_t param, int32_t value)
     if (ctx->step_graph) {
         HIP_TRY(ctx, hipGraphExecDestroy(ctx->step_graph));
         ctx->step_graph = nullptr;
+    }
+    return FCPT_OK;
+}
+
+int fcpt_get_launch_param(fcpt_ctx *ctx, int32_t param, int32_t *value)
+{
+    if (!ctx || !value) return FCPT_ERR_ARG;
+    const KParams &P = ctx->P;
+    switch (param) {
+    case FCPT_PARAM_RADIAL_TILE: *value = radial_tile_for(P.nr, P.ns, P.radial_tile); break;
+    case FCPT_PARAM_RADIAL_WAVES: *value = P.radial_waves; break;
+    case FCPT_PARAM_FUSED_STEP: *value = ctx->fused ? 1 : 0; break;
+    case FCPT_PARAM_VISC_TILE: *value = resolve_visc_tile(P); break;
+    case FCPT_PARAM_KICK_TILE: *value = resolve_kick_tile(P); break;
+    case FCPT_PARAM_SPLIT_ROUNDING: *value = P.split_rounding; break;
+    default: return fail(ctx, FCPT_ERR_ARG, "unknown launch parameter");
     }
     return FCPT_OK;
 }

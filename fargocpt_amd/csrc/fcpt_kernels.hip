@@ -210,12 +210,24 @@ __device__ __forceinline__ double ring_tab(const KParams &P, int t, int i)
 }
 
 // the rounding key of split-storage stores of field (or buffer) `id` in the
-// current step; 0 (to nearest) in FP64 storage or with stochastic rounding off
+// current step; 0 (to nearest) in FP64 storage, with stochastic rounding off,
+// and (split_rounding 1) for the fields derived afresh from the state every
+// step, whose roundings do not accumulate: stochastic rounding then applies
+// to the evolved fields Sigma, v_r, v_phi, E and the transported momenta
+constexpr int kBufMom = 64;
+__device__ __forceinline__ bool carries_state(int id)
+{
+    return id == FCPT_SIGMA || id == FCPT_VRAD || id == FCPT_VAZI || id == FCPT_ENERGY || id >= kBufMom;
+}
 template <bool SPLIT>
 __device__ __forceinline__ unsigned store_key(const KParams &P, int id)
 {
-    if constexpr (SPLIT) return P.split_rounding ? sr_key(id, cst(&P.s->step_count)[0]) : 0u;
-    else return 0u;
+    if constexpr (SPLIT) {
+        const int mode = P.split_rounding;
+        return (mode == 2 || (mode == 1 && carries_state(id))) ? sr_key(id, cst(&P.s->step_count)[0]) : 0u;
+    } else {
+        return 0u;
+    }
 }
 template <bool SPLIT>
 __device__ __forceinline__ Fld<SPLIT> fld(const KParams &P, int f)
@@ -223,7 +235,6 @@ __device__ __forceinline__ Fld<SPLIT> fld(const KParams &P, int f)
     return Fld<SPLIT>{static_cast<typename Fld<SPLIT>::T *>(P.f[f]), P.bg[f], P.ns, store_key<SPLIT>(P, f)};
 }
 // a work buffer; id names it for the rounding key (the momenta: kBufMom + q)
-constexpr int kBufMom = 64;
 template <bool SPLIT>
 __device__ __forceinline__ Fld<SPLIT> buf(const KParams &P, void *p, const double *bg, int id = -1)
 {

@@ -311,14 +311,19 @@ int fcpt_time_copy(fcpt_ctx *ctx, int32_t reps, double *ms_per_launch, double *b
    captured step graph so that the next fcpt_run_steps captures the new
    shape. */
 /* FCPT_PARAM_SPLIT_ROUNDING (split storage only; default 1) chooses how a
-   stored value's FP32 deviation is rounded: 1 stochastically (up or down with
+   stored value's FP32 deviation is rounded: stochastically (up or down with
    probabilities given by the distances to the two neighbouring FP32 values, a
    deterministic hash of the value, the cell, the field and the step count
    drawing the variate: unbiased, so a slowly changing value does not lose its
-   sub-unit increments every step), 0 to nearest. */
+   sub-unit increments every step) for the evolved fields Sigma, v_r, v_phi, E
+   and the transported momenta, to nearest for the fields derived afresh every
+   step (1); stochastically for every field (2); to nearest everywhere (0). */
 enum { FCPT_PARAM_RADIAL_TILE = 0, FCPT_PARAM_RADIAL_WAVES = 1, FCPT_PARAM_FUSED_STEP = 2,
        FCPT_PARAM_VISC_TILE = 3, FCPT_PARAM_KICK_TILE = 4, FCPT_PARAM_SPLIT_ROUNDING = 5 };
 int fcpt_set_launch_param(fcpt_ctx *ctx, int32_t param, int32_t value);
+/* The value a launch parameter takes in the next step: the size-based
+   choice where it was left at 0 (tiles), else the value set. */
+int fcpt_get_launch_param(fcpt_ctx *ctx, int32_t param, int32_t *value);
 /* Autotuner (no counterpart in the reference): times every launch shape of
    the tunable kernels on the context's current state (`reps` launches each;
    the timed kernels write transport scratch only), keeps the fastest and
