@@ -43,6 +43,8 @@ struct fcpt_ctx {
     // a device ring array for re-expressing a field about a new background
     std::vector<double> hbg[FCPT_NUM_FIELDS];
     double *bg_tmp = nullptr;
+    // fused FP64 adiabatic steps leave c_s, nu and T to be formed (lazy_derived)
+    bool derived_stale = false;
 };
 
 namespace {
@@ -78,6 +80,38 @@ int fail(fcpt_ctx *c, int code, const char *fmt, ...)
         hipError_t e_ = hipSetDevice((ctx)->device);     \
         if (e_ != hipSuccess)                            \
             return fail((ctx), FCPT_ERR_DEVICE, "hipSetDevice: %s", hipGetErrorString(e_)); \
+    } while (0)
+
+// the fused FP64 adiabatic Euler step forms c_s, nu and T where it needs them
+// (KParams::lazy_derived) when the kick and viscous walks run; afterwards the
+// fields are stale until sync_derived stores them (the same values: the same
+// expressions on the stored E and Sigma)
+static bool lazy_applies(const fcpt_ctx *ctx)
+{
+    const KParams &P = ctx->P;
+    return ctx->fused && !P.split && P.adiabatic && P.integrator == FCPT_INTEGRATOR_EULER &&
+           kick_walk_applies(P);
+}
+static KParams step_view(const fcpt_ctx *ctx)
+{
+    KParams P = ctx->P;
+    P.lazy_derived = lazy_applies(ctx) ? 1 : 0;
+    return P;
+}
+static int sync_derived(fcpt_ctx *ctx)
+{
+    if (!ctx->derived_stale) return FCPT_OK;
+    launch_derived(ctx->P, ctx->stream);
+    ctx->derived_stale = false;
+    return FCPT_OK;
+}
+// entry points that read or expose the fields: the context, with the derived
+// fields stored
+#define CTX_SYNCED(ctx)                        \
+    do {                                       \
+        CTX_CHECK(ctx);                        \
+        const int rc_sync_ = sync_derived(ctx); \
+        if (rc_sync_) return rc_sync_;         \
     } while (0)
 
 int dev_alloc_bytes(fcpt_ctx *c, void **p, size_t bytes)
@@ -258,6 +292,7 @@ int fcpt_create(const fcpt_disk_desc *desc, fcpt_ctx **out)
     P.radial_waves = 3;
     P.visc_tile = 0;
     P.kick_tile = 0;
+    P.lazy_derived = 0;
     init_launch_fields(P);
     // Interpret.cpp:230-231
     P.dphi = 2.0 * M_PI / (double)ns;
@@ -532,7 +567,7 @@ static int download_split(fcpt_ctx *ctx, int field, double *host)
 
 int fcpt_upload_field(fcpt_ctx *ctx, int32_t field, const double *host)
 {
-    CTX_CHECK(ctx);
+    CTX_SYNCED(ctx);
     if (field < 0 || field >= FCPT_NUM_FIELDS || !host) return fail(ctx, FCPT_ERR_ARG, "bad field");
     if (ctx->P.split && field != FCPT_POTENTIAL) return upload_split(ctx, field, host);
     const size_t n = field_rows(field, ctx->P.nr) * ctx->P.ns;
@@ -544,7 +579,7 @@ int fcpt_upload_field(fcpt_ctx *ctx, int32_t field, const double *host)
 
 int fcpt_download_field(fcpt_ctx *ctx, int32_t field, double *host)
 {
-    CTX_CHECK(ctx);
+    CTX_SYNCED(ctx);
     if (field < 0 || field >= FCPT_NUM_FIELDS || !host) return fail(ctx, FCPT_ERR_ARG, "bad field");
     if (ctx->P.split && field != FCPT_POTENTIAL) return download_split(ctx, field, host);
     const size_t n = field_rows(field, ctx->P.nr) * ctx->P.ns;
@@ -650,7 +685,7 @@ static int copy_initial_values(fcpt_ctx *ctx)
 
 int fcpt_init_profiles(fcpt_ctx *ctx, const fcpt_ic_desc *ic)
 {
-    CTX_CHECK(ctx);
+    CTX_SYNCED(ctx);
     if (!ic) return fail(ctx, FCPT_ERR_ARG, "null initial-condition description");
     const KParams &P = ctx->P;
     const int nr = P.nr, ns = P.ns;
@@ -849,7 +884,7 @@ int fcpt_set_step_params(fcpt_ctx *ctx, const fcpt_step_params *params)
 
 static int cfl_common(fcpt_ctx *ctx, int mode, double *dt_out)
 {
-    CTX_CHECK(ctx);
+    CTX_SYNCED(ctx);
     launch_cfl(ctx->P, ctx->stream, mode);
     LAUNCH_CHECK(ctx);
     int rc = read_scalars(ctx);
@@ -866,7 +901,7 @@ int fcpt_calculate_timestep(fcpt_ctx *ctx, double *dt_out) { return cfl_common(c
 
 int fcpt_nbody_potential(fcpt_ctx *ctx)
 {
-    CTX_CHECK(ctx);
+    CTX_SYNCED(ctx);
     launch_potential(ctx->P, ctx->stream);
     LAUNCH_CHECK(ctx);
     return FCPT_OK;
@@ -875,7 +910,7 @@ int fcpt_nbody_potential(fcpt_ctx *ctx)
 #define DT_ENTRY(name, body)                         \
     int name(fcpt_ctx *ctx, double dt)               \
     {                                                \
-        CTX_CHECK(ctx);                              \
+        CTX_SYNCED(ctx);                             \
         launch_set_dt(ctx->P, ctx->stream, dt);      \
         body;                                        \
         LAUNCH_CHECK(ctx);                           \
@@ -889,7 +924,7 @@ DT_ENTRY(fcpt_transport, launch_transport(ctx->P, ctx->stream))
 
 int fcpt_recalculate_viscosity(fcpt_ctx *ctx)
 {
-    CTX_CHECK(ctx);
+    CTX_SYNCED(ctx);
     launch_recalc_viscosity(ctx->P, ctx->stream);
     LAUNCH_CHECK(ctx);
     return FCPT_OK;
@@ -897,7 +932,7 @@ int fcpt_recalculate_viscosity(fcpt_ctx *ctx)
 
 int fcpt_viscous_stress_tensor(fcpt_ctx *ctx)
 {
-    CTX_CHECK(ctx);
+    CTX_SYNCED(ctx);
     launch_stress(ctx->P, ctx->stream);
     LAUNCH_CHECK(ctx);
     return FCPT_OK;
@@ -905,7 +940,7 @@ int fcpt_viscous_stress_tensor(fcpt_ctx *ctx)
 
 int fcpt_substep3(fcpt_ctx *ctx, double time, double dt)
 {
-    CTX_CHECK(ctx);
+    CTX_SYNCED(ctx);
     if (!ctx->P.adiabatic) return FCPT_OK;
     launch_set_clock(ctx->P, ctx->stream, time);
     launch_set_dt(ctx->P, ctx->stream, dt);
@@ -916,7 +951,7 @@ int fcpt_substep3(fcpt_ctx *ctx, double time, double dt)
 
 int fcpt_apply_boundary_condition(fcpt_ctx *ctx, double dt, int32_t final_call)
 {
-    CTX_CHECK(ctx);
+    CTX_SYNCED(ctx);
     launch_set_dt(ctx->P, ctx->stream, dt);
     launch_boundaries(ctx->P, ctx->stream, final_call, 0);
     LAUNCH_CHECK(ctx);
@@ -925,7 +960,7 @@ int fcpt_apply_boundary_condition(fcpt_ctx *ctx, double dt, int32_t final_call)
 
 int fcpt_recalculate_derived(fcpt_ctx *ctx)
 {
-    CTX_CHECK(ctx);
+    CTX_SYNCED(ctx);
     launch_derived(ctx->P, ctx->stream);
     LAUNCH_CHECK(ctx);
     return FCPT_OK;
@@ -1051,8 +1086,9 @@ static void enqueue_iteration(const fcpt_ctx *ctx)
 {
     const KParams &P = ctx->P;
     if (ctx->fused && P.integrator != FCPT_INTEGRATOR_LEAPFROG) {
-        launch_cfl_cells(P, ctx->stream);
-        enqueue_euler(P, ctx->stream, 1, true, 2);
+        const KParams V = step_view(ctx);
+        launch_cfl_cells(V, ctx->stream);
+        enqueue_euler(V, ctx->stream, 1, true, 2);
     } else {
         launch_cfl(P, ctx->stream, 2);
         enqueue_step(P, ctx->stream, 1, ctx->fused);
@@ -1061,16 +1097,17 @@ static void enqueue_iteration(const fcpt_ctx *ctx)
 
 int fcpt_step_euler(fcpt_ctx *ctx, double dt)
 {
-    CTX_CHECK(ctx);
+    CTX_SYNCED(ctx);
     launch_set_dt(ctx->P, ctx->stream, dt);
-    enqueue_euler(ctx->P, ctx->stream, 0, ctx->fused, -1);
+    enqueue_euler(step_view(ctx), ctx->stream, 0, ctx->fused, -1);
     LAUNCH_CHECK(ctx);
+    ctx->derived_stale = lazy_applies(ctx);
     return FCPT_OK;
 }
 
 int fcpt_step_leapfrog(fcpt_ctx *ctx, double dt)
 {
-    CTX_CHECK(ctx);
+    CTX_SYNCED(ctx);
     launch_set_dt(ctx->P, ctx->stream, dt);
     enqueue_leapfrog(ctx->P, ctx->stream, 0, ctx->fused);
     LAUNCH_CHECK(ctx);
@@ -1079,7 +1116,7 @@ int fcpt_step_leapfrog(fcpt_ctx *ctx, double dt)
 
 int fcpt_compute_pressure(fcpt_ctx *ctx)
 {
-    CTX_CHECK(ctx);
+    CTX_SYNCED(ctx);
     launch_pressure(ctx->P, ctx->stream);
     LAUNCH_CHECK(ctx);
     return FCPT_OK;
@@ -1100,6 +1137,7 @@ int fcpt_run_step(fcpt_ctx *ctx, double *dt_out)
     CTX_CHECK(ctx);
     enqueue_iteration(ctx);
     LAUNCH_CHECK(ctx);
+    if (lazy_applies(ctx)) ctx->derived_stale = true;
     int rc = read_scalars(ctx);
     if (rc) return rc;
     if (take_nonfinite(ctx))
@@ -1140,6 +1178,7 @@ int fcpt_run_steps(fcpt_ctx *ctx, int32_t n_steps, int32_t use_graph)
         }
         LAUNCH_CHECK(ctx);
     }
+    if (n_steps > 0 && lazy_applies(ctx)) ctx->derived_stale = true;
     return FCPT_OK;
 }
 
@@ -1174,7 +1213,7 @@ int fcpt_time_steps(fcpt_ctx *ctx, int32_t n_steps, int32_t use_graph, double *m
 
 int fcpt_time_kernel(fcpt_ctx *ctx, int32_t kernel, int32_t reps, double *ms_per_launch)
 {
-    CTX_CHECK(ctx);
+    CTX_SYNCED(ctx);
     if (reps <= 0) return fail(ctx, FCPT_ERR_ARG, "reps must be positive");
     int rc = ensure_events(ctx);
     if (rc) return rc;
@@ -1284,7 +1323,7 @@ int fcpt_get_launch_param(fcpt_ctx *ctx, int32_t param, int32_t *value)
 
 int fcpt_autotune(fcpt_ctx *ctx, int32_t reps, int32_t *radial_tile, int32_t *radial_waves)
 {
-    CTX_CHECK(ctx);
+    CTX_SYNCED(ctx);
     if (reps <= 0) return fail(ctx, FCPT_ERR_ARG, "reps must be positive");
     int rc = ensure_events(ctx);
     if (rc) return rc;
@@ -1328,7 +1367,7 @@ int fcpt_autotune(fcpt_ctx *ctx, int32_t reps, int32_t *radial_tile, int32_t *ra
 int fcpt_disk_on_planet_accel(fcpt_ctx *ctx, const fcpt_force_body *bodies, int32_t n_bodies,
                               int32_t flags, double *accel_xy)
 {
-    CTX_CHECK(ctx);
+    CTX_SYNCED(ctx);
     if (n_bodies < 0 || n_bodies > kForceMaxBodies || (n_bodies > 0 && (!bodies || !accel_xy)))
         return fail(ctx, FCPT_ERR_ARG, "1 .. %d bodies", kForceMaxBodies);
     if (flags & ~(FCPT_FORCE_NO_STAR_SMOOTHING | FCPT_FORCE_SMOOTHING_PLANETLOC |
@@ -1413,7 +1452,7 @@ static size_t grid_doubles(const fcpt_ctx *ctx, int field)
 
 int fcpt_write_snapshot(fcpt_ctx *ctx, const char *dir, const fcpt_snapshot_info *info, uint32_t fields)
 {
-    CTX_CHECK(ctx);
+    CTX_SYNCED(ctx);
     if (!dir || !info) return fail(ctx, FCPT_ERR_ARG, "null snapshot directory or info");
     if (fields == 0) {
         fields = (1u << FCPT_SIGMA) | (1u << FCPT_VRAD) | (1u << FCPT_VAZI) | (1u << FCPT_ENERGY);
@@ -1442,7 +1481,7 @@ int fcpt_write_snapshot(fcpt_ctx *ctx, const char *dir, const fcpt_snapshot_info
 
 int fcpt_read_snapshot(fcpt_ctx *ctx, const char *dir, fcpt_snapshot_info *info)
 {
-    CTX_CHECK(ctx);
+    CTX_SYNCED(ctx);
     if (!dir) return fail(ctx, FCPT_ERR_ARG, "null snapshot directory");
     fcpt_snapshot_info m{};
     int rc = read_file(ctx, std::string(dir) + "/misc.bin", &m, sizeof(m));
@@ -1644,6 +1683,7 @@ struct fcpt_batch {
     int split = 0;
     int radial_waves = 3;
     std::vector<fcpt_ctx *> disks;
+    std::vector<bool> lazy;  // the disk's batched step forms c_s, nu and T where needed
     KParams *Ps = nullptr;  // device array: every disk's parameters, then their swapped views
     std::vector<BatchStageDev> stages;
     std::vector<void *> allocs;
@@ -1695,11 +1735,23 @@ int fcpt_batch_create(fcpt_ctx *const *disks, int32_t n_disks, fcpt_batch **out)
             return FCPT_ERR_ARG;  // one device, one storage mode, Euler disks
         }
         b->disks.push_back(c);
+        // the fields the batched kernels read, stored
+        if (c->derived_stale) {
+            if (hipSetDevice(c->device) != hipSuccess || sync_derived(c) ||
+                hipStreamSynchronize(c->stream) != hipSuccess) {
+                delete b;
+                return FCPT_ERR_DEVICE;
+            }
+        }
         KParams P = c->P;
         P.radial_tile = radial_tile_for(P.nr, P.ns, P.radial_tile);
         // the walks' tiles as the disk's own launches choose them
         P.kick_tile = resolve_kick_tile(P);
         P.visc_tile = resolve_visc_tile(P);
+        // the batch runs the fused step: c_s, nu and T formed where needed
+        // as in the disk's own fused step
+        P.lazy_derived = (!P.split && P.adiabatic && kick_walk_applies(P)) ? 1 : 0;
+        b->lazy.push_back(P.lazy_derived != 0);
         host.push_back(P);
     }
     hipError_t e = hipSetDevice(b->device);
@@ -1811,6 +1863,8 @@ int fcpt_batch_run_steps(fcpt_batch *b, int32_t n_steps, int32_t use_graph)
     // after the batched steps on the disk's own stream
     BHIP(b, hipEventRecord(b->done, b->stream));
     for (fcpt_ctx *c : b->disks) BHIP(b, hipStreamWaitEvent(c->stream, b->done, 0));
+    for (size_t k = 0; k < b->disks.size(); ++k)
+        if (n_steps > 0 && b->lazy[k]) b->disks[k]->derived_stale = true;
     return FCPT_OK;
 }
 

@@ -483,6 +483,31 @@ __device__ __forceinline__ void b_cfl_fold(const KParams &P, int mode, int t, bo
 }
 __device__ __forceinline__ void b_cfl_finalize(const KParams &P, int mode) { b_cfl_fold(P, mode, (int)threadIdx.x); }
 
+// adiabatic sound speed, scale height and viscosity of one cell
+// (compute_sound_speed, compute_scale_height, compute_viscosity)
+struct AdiDerived {
+    double cs, h, nu;
+};
+__device__ __forceinline__ AdiDerived adi_derived(const KParams &P, int i, double e, double sig)
+{
+    AdiDerived d;
+    const double g = P.gamma;
+    d.cs = sqrt(g * (g - 1.0) * e / sig);
+    const double inv_omega_kepler = ring_tab(P, TAB_INV_OMEGA_K, i);  // 1 / Omega_K(Rmed)
+    d.h = d.cs / (sqrt(g)) * inv_omega_kepler;
+    const double alpha = P.alpha;
+    d.nu = alpha * d.h * d.cs;
+    return d;
+}
+
+// true where the fused FP64 step forms c_s, nu and T instead of storing them
+template <bool SPLIT>
+__device__ __forceinline__ bool lazy_derived(const KParams &P)
+{
+    if constexpr (SPLIT) return false;
+    else return P.lazy_derived != 0 && P.adiabatic;
+}
+
 // the CFL limit of cell (i, j) (cfl.cpp:235-376), vm_i the ring's mean v_phi
 template <bool SPLIT>
 __device__ __forceinline__ double cfl_cell(const KParams &P, int i, int j, double vm_i)
@@ -505,8 +530,19 @@ __device__ __forceinline__ double cfl_cell(const KParams &P, int i, int j, doubl
     const double inv_dr = SPLIT ? cst(P.InvDiffRsup)[i] : 0.0;
     const double inv_daz = SPLIT ? recip<true>(dxAzimuthal) : 0.0;
     const double inv_cell = SPLIT ? recip<true>(cell_size) : 0.0;
-    const double csv = iso ? cst(P.iso_cs)[i] : cs(i, j);
-    const double nuv = iso ? cst(P.iso_nu)[i] : nu(i, j);
+    double csv, nuv;
+    if (iso) {
+        csv = cst(P.iso_cs)[i];
+        nuv = cst(P.iso_nu)[i];
+    } else if (lazy_derived<SPLIT>(P)) {
+        // as the end of the last step formed them from the same E and Sigma
+        const AdiDerived a = adi_derived(P, i, fld<SPLIT>(P, FCPT_ENERGY)(i, j), fld<SPLIT>(P, FCPT_SIGMA)(i, j));
+        csv = a.cs;
+        nuv = a.nu;
+    } else {
+        csv = cs(i, j);
+        nuv = nu(i, j);
+    }
     const double invdt1 = SPLIT ? csv * inv_cell : csv / cell_size;
     const double invdt2 = SPLIT ? vr0 * inv_dr : vr0 / dxRadial;
     const double invdt3 = SPLIT ? v_residual * inv_daz : v_residual / dxAzimuthal;
@@ -980,29 +1016,19 @@ __device__ __forceinline__ DerivedIn derived_load(const KParams &P, int i, int j
     return d;
 }
 
-// adiabatic sound speed, scale height and viscosity of one cell
-// (compute_sound_speed, compute_scale_height, compute_viscosity)
-struct AdiDerived {
-    double cs, h, nu;
-};
-__device__ __forceinline__ AdiDerived adi_derived(const KParams &P, int i, double e, double sig)
-{
-    AdiDerived d;
-    const double g = P.gamma;
-    d.cs = sqrt(g * (g - 1.0) * e / sig);
-    const double inv_omega_kepler = ring_tab(P, TAB_INV_OMEGA_K, i);  // 1 / Omega_K(Rmed)
-    d.h = d.cs / (sqrt(g)) * inv_omega_kepler;
-    const double alpha = P.alpha;
-    d.nu = alpha * d.h * d.cs;
-    return d;
-}
-
 template <bool SPLIT>
 __device__ __forceinline__ void derived_store(const KParams &P, int i, int j, const DerivedIn &d,
                                              bool with_temperature, bool with_pressure,
                                              bool force_nu = false)
 {
     double c_s, h;
+    if (lazy_derived<SPLIT>(P)) {
+        // the fields the step reads stored: H (the potential's smoothing) and P
+        const AdiDerived a = adi_derived(P, i, d.e, d.sig);
+        fld<SPLIT>(P, FCPT_SCALE_HEIGHT).set(i, j, a.h);
+        if (with_pressure) fld<SPLIT>(P, FCPT_PRESSURE).set(i, j, (P.gamma - 1.0) * d.e);
+        return;
+    }
     if (P.adiabatic) {
         const double e = d.e;
         const double sig = d.sig;
@@ -1742,8 +1768,10 @@ __device__ __forceinline__ void b_kick_walk(const KParams &P, int bx, int by, in
                 }
                 // recalculate_viscosity
                 const AdiDerived d = adi_derived(P, i, e, R0.s);
-                fld<SPLIT>(P, FCPT_SOUNDSPEED).set(i, j, d.cs);
-                fld<SPLIT>(P, FCPT_SCALE_HEIGHT).set(i, j, d.h);
+                if (!lazy_derived<SPLIT>(P)) {
+                    fld<SPLIT>(P, FCPT_SOUNDSPEED).set(i, j, d.cs);
+                    fld<SPLIT>(P, FCPT_SCALE_HEIGHT).set(i, j, d.h);
+                }
                 fld<SPLIT>(P, FCPT_VISCOSITY).set(i, j, d.nu);
             }
             if (inner)
@@ -1972,7 +2000,7 @@ __device__ __forceinline__ void substep3_cell(const KParams &P, int i, int j, do
     } else {
         qplus = qminus = 0.0;
     }
-    fld<SPLIT>(P, FCPT_TEMPERATURE).set(i, j, temperature);
+    if (!lazy_derived<SPLIT>(P)) fld<SPLIT>(P, FCPT_TEMPERATURE).set(i, j, temperature);
     fld<SPLIT>(P, FCPT_QPLUS).set(i, j, qplus);
     fld<SPLIT>(P, FCPT_QMINUS).set(i, j, qminus);
     fld<SPLIT>(P, FCPT_ENERGY).set(i, j, temperature_floor(P, e, sig));
@@ -2128,7 +2156,9 @@ __device__ __forceinline__ void b_visc_walk(const KParams &P, int bx, int by, in
                 vr_o.set_raw(i, j, vr.raw(i, j));
             if constexpr (ENERGY) {
                 const double e = fld<SPLIT>(P, FCPT_ENERGY)(i, j);
-                const double h = fld<SPLIT>(P, FCPT_SCALE_HEIGHT)(i, j);
+                // H as the kick walk formed it from the same E and Sigma
+                const double h = lazy_derived<SPLIT>(P) ? adi_derived(P, i, e, A.s).h
+                                                         : fld<SPLIT>(P, FCPT_SCALE_HEIGHT)(i, j);
                 double qplus = 0.0, qminus = 0.0;
                 if (upd_p) {
                     qminus = cooling_beta_cell<SPLIT>(P, i, j, P.s->time, e, A.s);
