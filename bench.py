@@ -116,6 +116,8 @@ def timed_run(name: str, prec: int, steps: int, warmup: int, use_graph: bool, tu
     disk.run_steps(warmup, use_graph)
     disk.synchronize()
     disk.tuned = disk.autotune(10) if tune else {}
+    if tune:
+        disk.tuned.update(disk.autotune_walks(3))
     from fargocpt_amd.capi import PARAM_KICK_TILE, PARAM_VISC_TILE
     disk.tuned.update({"kick_tile": disk.launch_param(PARAM_KICK_TILE),
                        "visc_tile": disk.launch_param(PARAM_VISC_TILE)})

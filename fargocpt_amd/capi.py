@@ -221,6 +221,7 @@ def load_library(path: str = LIB_PATH):
         "fcpt_set_launch_param": [P, I, I],
         "fcpt_get_launch_param": [P, I, C.POINTER(I)],
         "fcpt_autotune": [P, I, C.POINTER(I), C.POINTER(I)],
+        "fcpt_autotune_walks": [P, I, C.POINTER(I), C.POINTER(I)],
         "fcpt_disk_on_planet_accel": [P, C.POINTER(ForceBody), I, I, dp],
         "fcpt_batch_create": [C.POINTER(P), I, C.POINTER(P)],
         "fcpt_batch_destroy": [P],
@@ -510,6 +511,14 @@ class Disk:
         t, w = C.c_int32(), C.c_int32()
         self._check(self.lib.fcpt_autotune(self.h, reps, C.byref(t), C.byref(w)))
         return {"radial_tile": t.value, "radial_waves": w.value}
+
+    def autotune_walks(self, reps=5):
+        """Time the kick and viscous walks' tiles over whole steps (the state
+        is put back after each trial) and keep the fastest; returns
+        {"kick_tile": k, "visc_tile": v}."""
+        k, v = C.c_int32(), C.c_int32()
+        self._check(self.lib.fcpt_autotune_walks(self.h, reps, C.byref(k), C.byref(v)))
+        return {"kick_tile": k.value, "visc_tile": v.value}
 
     def set_radial_waves(self, waves):
         """Register budget of the radial sweep in wavefronts per SIMD (2 or 3)."""

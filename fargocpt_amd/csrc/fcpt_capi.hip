@@ -1305,6 +1305,108 @@ int fcpt_set_launch_param(fcpt_ctx *ctx, int32_t param, int32_t value)
     return FCPT_OK;
 }
 
+// The walks' tiles by whole-step time: each candidate runs `reps` graph-replayed
+// sim::run iterations from the same state, which is saved first (every field
+// and the device scalars) and put back after each trial and at the end, so
+// the context's state and clock come out as they went in.  Tiles 2 .. 16
+// for each walk that applies, the kick walk's first, then the viscous walk's.
+int fcpt_autotune_walks(fcpt_ctx *ctx, int32_t reps, int32_t *kick_tile, int32_t *visc_tile)
+{
+    CTX_SYNCED(ctx);
+    if (reps <= 0) return fail(ctx, FCPT_ERR_ARG, "reps must be positive");
+    int rc = ensure_events(ctx);
+    if (rc) return rc;
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    KParams &P = ctx->P;
+    const int kick0 = resolve_kick_tile(P), visc0 = resolve_visc_tile(P);
+    if (!ctx->fused || P.integrator != FCPT_INTEGRATOR_EULER || visc0 <= 1) {
+        if (kick_tile) *kick_tile = kick0;
+        if (visc_tile) *visc_tile = visc0;
+        return FCPT_OK;  // the walks do not run in this context's step
+    }
+    // the saved state: every field and the scalars
+    size_t total = 0;
+    std::vector<size_t> off(FCPT_NUM_FIELDS);
+    for (int k = 0; k < FCPT_NUM_FIELDS; ++k) {
+        off[k] = total;
+        total += (field_rows(k, P.nr) * P.ns * field_element_size(P, k) + 255) / 256 * 256;
+    }
+    char *save = nullptr;
+    HIP_TRY(ctx, hipMalloc((void **)&save, total));
+    DevScalars s0;
+    auto copy_fields = [&](bool to_save) -> hipError_t {
+        for (int k = 0; k < FCPT_NUM_FIELDS; ++k) {
+            const size_t bytes = field_rows(k, P.nr) * P.ns * field_element_size(P, k);
+            hipError_t e = to_save ? hipMemcpyAsync(save + off[k], P.f[k], bytes, hipMemcpyDeviceToDevice, ctx->stream)
+                                   : hipMemcpyAsync(P.f[k], save + off[k], bytes, hipMemcpyDeviceToDevice, ctx->stream);
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+    };
+    auto restore = [&]() -> hipError_t {
+        hipError_t e = copy_fields(false);
+        if (e == hipSuccess) e = hipMemcpyAsync(ctx->s_dev, &s0, sizeof(DevScalars), hipMemcpyHostToDevice, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        ctx->derived_stale = false;
+        return e;
+    };
+    hipError_t e = copy_fields(true);
+    if (e == hipSuccess) e = hipMemcpy(&s0, ctx->s_dev, sizeof(DevScalars), hipMemcpyDeviceToHost);
+    auto trial = [&](double &ms_out) -> int {
+        if (ctx->step_graph) {
+            hipGraphExecDestroy(ctx->step_graph);
+            ctx->step_graph = nullptr;
+        }
+        int r = fcpt_run_steps(ctx, 1, 1);  // capture + warm-up
+        if (!r) r = fcpt_time_steps(ctx, reps, 1, &ms_out);
+        const hipError_t er = restore();
+        if (!r && er != hipSuccess) r = fail(ctx, FCPT_ERR_DEVICE, "state restore: %s", hipGetErrorString(er));
+        return r;
+    };
+    const int cand[] = {2, 4, 8, 16};
+    int best_k = kick0, best_v = visc0;
+    rc = e == hipSuccess ? FCPT_OK : fail(ctx, FCPT_ERR_DEVICE, "state save: %s", hipGetErrorString(e));
+    if (!rc && kick_walk_applies(P)) {
+        double best = 1e300;
+        for (int t : cand) {
+            if (t > P.nr) continue;
+            P.kick_tile = t;
+            if (!kick_walk_applies(P)) continue;
+            double ms = 0.0;
+            if ((rc = trial(ms))) break;
+            if (ms < best) {
+                best = ms;
+                best_k = t;
+            }
+        }
+        P.kick_tile = best_k;
+    }
+    if (!rc) {
+        double best = 1e300;
+        for (int t : cand) {
+            if (t > P.nr) continue;
+            P.visc_tile = t;
+            if (P.kick_tile > 1 && !kick_walk_applies(P)) continue;
+            double ms = 0.0;
+            if ((rc = trial(ms))) break;
+            if (ms < best) {
+                best = ms;
+                best_v = t;
+            }
+        }
+        P.visc_tile = best_v;
+    }
+    hipFree(save);
+    if (ctx->step_graph) {
+        hipGraphExecDestroy(ctx->step_graph);
+        ctx->step_graph = nullptr;
+    }
+    if (rc) return rc;
+    if (kick_tile) *kick_tile = best_k;
+    if (visc_tile) *visc_tile = best_v;
+    return FCPT_OK;
+}
+
 int fcpt_get_launch_param(fcpt_ctx *ctx, int32_t param, int32_t *value)
 {
     if (!ctx || !value) return FCPT_ERR_ARG;

@@ -329,6 +329,13 @@ int fcpt_get_launch_param(fcpt_ctx *ctx, int32_t param, int32_t *value);
    the timed kernels write transport scratch only), keeps the fastest and
    returns it.  Drops a captured step graph. */
 int fcpt_autotune(fcpt_ctx *ctx, int32_t reps, int32_t *radial_tile, int32_t *radial_waves);
+/* Autotuner of the fused step's walks (no counterpart in the reference): the
+   kick walk's and the viscous walk's tiles (2 .. 16 rings) by the time of
+   `reps` whole graph-replayed iterations each, every trial from the same
+   state, which the call saves first and puts back (fields and clock as they
+   were); keeps the fastest and returns the tiles.  A context whose step does
+   not run the walks keeps its tiles.  Drops a captured step graph. */
+int fcpt_autotune_walks(fcpt_ctx *ctx, int32_t reps, int32_t *kick_tile, int32_t *visc_tile);
 
 /* ---- batches (no counterpart in the reference) ------------------------- */
 /* Many independent disks advanced together: each stage of a sim::run

@@ -223,3 +223,32 @@ def test_fused_graph_matches_stepwise(integrator):
         assert np.array_equal(a.download_field(f), b.download_field(f)), f
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_autotune_walks_keeps_state(precision):
+    """fcpt_autotune_walks times whole steps per tile from the same state and
+    puts the state and clock back: a tuned disk then steps exactly like an
+    untuned one (the walks' tiles do not change the result in FP64; in split
+    storage the same holds, the tiles only regroup the same per-cell work)."""
+    from fargocpt_amd import workloads as W
+    from fargocpt_amd.capi import Disk
+    name = "adi_512x1536"
+    print(f"autotune_walks {name} precision {precision}", flush=True)
+    a, b = Disk(W.disk_config(name, precision)), Disk(W.disk_config(name, precision))
+    for d in (a, b):
+        W.start(d, name)
+        d.run_steps(3, True)
+        d.synchronize()
+    t0 = a.get_time()
+    tuned = a.autotune_walks(2)
+    assert a.get_time() == t0
+    assert tuned["kick_tile"] >= 1 and tuned["visc_tile"] >= 1
+    for d in (a, b):
+        d.run_steps(4, True)
+        d.synchronize()
+    assert a.get_time() == b.get_time()
+    for f in ("SIGMA", "VRAD", "VAZI", "ENERGY", "SOUNDSPEED", "QPLUS"):
+        assert np.array_equal(a.download_field(f), b.download_field(f)), f
+    a.close()
+    b.close()
