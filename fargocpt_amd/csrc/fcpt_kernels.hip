@@ -131,11 +131,14 @@ __device__ __forceinline__ const __attribute__((address_space(4))) T *cst(const 
 // to the two neighbouring FP32 values (the rounding is unbiased, so the
 // per-step roundings of a slowly changing value do not add up in one
 // direction: an increment below half an FP32 unit of the deviation survives
-// on average instead of being lost every step).  The uniform variate is a hash
-// (Wellons' lowbias32, public domain) of the deviation's low 32 bits, the
-// cell, and a key formed from the field and the step counter, so a value and
-// its cell always round the same way within one kernel (a thread can form
-// what another thread stores) and differently from step to step.
+// on average instead of being lost every step).  The uniform variate of a
+// store is Marsaglia's xorshift32 of the cell index, the deviation's high
+// word (sign, exponent and leading mantissa bits: independent of the dropped
+// bits, and different for the several values one field of one cell takes in
+// a step) and a key drawn per field and step (sr_key, Wellons' lowbias32,
+// public domain): a value always rounds the same way at its cell within one
+// kernel (a thread can form what another thread stores) and differently from
+// step to step.
 __device__ __forceinline__ unsigned sr_hash(unsigned x)
 {
     x ^= x >> 16;
@@ -145,19 +148,25 @@ __device__ __forceinline__ unsigned sr_hash(unsigned x)
     x ^= x >> 16;
     return x;
 }
+__device__ __forceinline__ unsigned sr_bits(unsigned x)
+{
+    x ^= x << 13;
+    x ^= x >> 17;
+    x ^= x << 5;
+    return x;
+}
 __device__ __forceinline__ float round_dev(double d, unsigned key, size_t c)
 {
     if (!key) return (float)d;
-    const unsigned long long b = (unsigned long long)__double_as_longlong(d);
-    const unsigned long long mag = b & 0x7fffffffffffffffull;
-    // NaN and infinities, and deviations the FP32 range cannot hold at full
-    // precision (below 2^-126 or from 2^128): to nearest
-    if (mag >= 0x47f0000000000000ull || mag < 0x3810000000000000ull) return (float)d;
-    const unsigned u = sr_hash((unsigned)b ^ key ^ ((unsigned)c * 0x9e3779b1u)) >> 3;  // 29 bits
-    // add u below the FP32 unit of the magnitude, then drop those 29 bits:
-    // rounded up with probability (dropped bits) / 2^29
-    const unsigned long long m = (mag + (unsigned long long)u) & ~0x1fffffffull;
-    return (float)__longlong_as_double((long long)((b & 0x8000000000000000ull) | m));
+    // add a 29-bit variate below the FP32 unit of the FP64 deviation and drop
+    // those 29 bits: rounded away from zero with probability (dropped
+    // bits) / 2^29.  Exact for |d| in [2^-126, 2^128) (the sign bit is never
+    // reached); smaller deviations are rounded once more by the conversion,
+    // infinities stay infinite.
+    const unsigned long long b0 = (unsigned long long)__double_as_longlong(d);
+    const unsigned u = sr_bits((unsigned)c ^ key ^ (unsigned)(b0 >> 32)) >> 3;
+    const unsigned long long b = b0 + (unsigned long long)u;
+    return (float)__longlong_as_double((long long)(b & ~0x1fffffffull));
 }
 __device__ __forceinline__ unsigned sr_key(int id, unsigned step)
 {
