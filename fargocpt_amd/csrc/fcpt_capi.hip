@@ -82,20 +82,26 @@ int fail(fcpt_ctx *c, int code, const char *fmt, ...)
             return fail((ctx), FCPT_ERR_DEVICE, "hipSetDevice: %s", hipGetErrorString(e_)); \
     } while (0)
 
-// the fused FP64 adiabatic Euler step forms c_s, nu and T where it needs them
-// (KParams::lazy_derived) when the kick and viscous walks run; afterwards the
-// fields are stale until sync_derived stores them (the same values: the same
-// expressions on the stored E and Sigma)
+// When the fused Euler step of an adiabatic disk runs the kick and viscous
+// walks, in FP64 storage the step forms c_s, nu and T where it needs them
+// (KParams::lazy_derived); in split storage the kick walk forms the pressure
+// from E and passes E to the viscous walk through its partner buffer
+// (KParams::e_partner), so the end of the step does not store the pressure
+// (which saves a decode and an encode per cell where the split kernels are
+// bound by their arithmetic; in FP64 storage, bound by the bytes, measured
+// 0.5 % slower, so not used there).  Afterwards those fields are stale until
+// sync_derived stores them (the same values: the same expressions on the
+// stored E and Sigma).
 static bool lazy_applies(const fcpt_ctx *ctx)
 {
     const KParams &P = ctx->P;
-    return ctx->fused && !P.split && P.adiabatic && P.integrator == FCPT_INTEGRATOR_EULER &&
-           kick_walk_applies(P);
+    return ctx->fused && P.adiabatic && P.integrator == FCPT_INTEGRATOR_EULER && kick_walk_applies(P);
 }
 static KParams step_view(const fcpt_ctx *ctx)
 {
     KParams P = ctx->P;
-    P.lazy_derived = lazy_applies(ctx) ? 1 : 0;
+    P.e_partner = lazy_applies(ctx) && P.split ? 1 : 0;
+    P.lazy_derived = lazy_applies(ctx) && !P.split ? 1 : 0;
     return P;
 }
 static int sync_derived(fcpt_ctx *ctx)
@@ -293,6 +299,8 @@ int fcpt_create(const fcpt_disk_desc *desc, fcpt_ctx **out)
     P.visc_tile = 0;
     P.kick_tile = 0;
     P.lazy_derived = 0;
+    P.e_partner = 0;
+    P.ealt = nullptr;
     init_launch_fields(P);
     // Interpret.cpp:230-231
     P.dphi = 2.0 * M_PI / (double)ns;
@@ -408,6 +416,7 @@ int fcpt_create(const fcpt_disk_desc *desc, fcpt_ctx **out)
         rc = dev_alloc_bytes(c, &P.f[k], field_rows(k, nr) * ns * field_element_size(P, k));
     if (!rc) rc = dev_alloc_bytes(c, &P.valt[0], field_rows(FCPT_VRAD, nr) * ns * esz);
     if (!rc) rc = dev_alloc_bytes(c, &P.valt[1], field_rows(FCPT_VAZI, nr) * ns * esz);
+    if (!rc && P.adiabatic && P.split) rc = dev_alloc_bytes(c, &P.ealt, field_rows(FCPT_ENERGY, nr) * ns * esz);
     for (int k = 0; k < 4 && !rc; ++k) rc = dev_alloc_bytes(c, &P.mom[k], ncell * esz);
     for (int k = 0; k < 4 && !rc; ++k) rc = dev_alloc_bytes(c, &P.mom_b[k], ncell * esz);
     // the density and energy between the transport sweeps stay FP64 in both
@@ -1852,8 +1861,10 @@ int fcpt_batch_create(fcpt_ctx *const *disks, int32_t n_disks, fcpt_batch **out)
         P.visc_tile = resolve_visc_tile(P);
         // the batch runs the fused step: c_s, nu and T formed where needed
         // as in the disk's own fused step
-        P.lazy_derived = (!P.split && P.adiabatic && kick_walk_applies(P)) ? 1 : 0;
-        b->lazy.push_back(P.lazy_derived != 0);
+        const bool walks = P.adiabatic && kick_walk_applies(P);
+        P.e_partner = walks && P.split ? 1 : 0;
+        P.lazy_derived = walks && !P.split ? 1 : 0;
+        b->lazy.push_back(walks);
         host.push_back(P);
     }
     hipError_t e = hipSetDevice(b->device);

@@ -92,6 +92,11 @@ struct KParams {
     // step needs them instead of stored (the kick walk stores nu only); the
     // host stores them all (k_derived) before anything else reads the fields
     int lazy_derived;
+    // the fused step's walks of an adiabatic disk (split storage): the kick walk
+    // forms the pressure from E and writes the updated E to the partner buffer
+    // ealt, which the viscous walk reads (its SubStep3 writes E back); the end
+    // of the step does not store the pressure
+    int e_partner;
     int cfl_gx, cfl_gy;  // grid of the CFL cell kernel (its block-minimum slots)
     // fields in HBM (indexed by fcpt_field): double, or float when split
     void *f[FCPT_NUM_FIELDS];
@@ -105,6 +110,7 @@ struct KParams {
     void *sig_b, *en_b;  // radial-sweep outputs
     void *sig_c, *en_c;  // azimuthal-sweep outputs
     void *valt[2];       // partners of v_r and v_phi for the fused step kernels
+    void *ealt;          // partner of E (adiabatic disks in split storage; e_partner)
     double *bgm[4];      // backgrounds of the four momenta (split storage)
     double *iso_cs, *iso_h, *iso_nu;  // locally isothermal ring profiles of c_s, H, nu
     // ring-uniform quotients of the source terms, the artificial viscosity,

@@ -1031,8 +1031,10 @@ __device__ __forceinline__ void derived_store(const KParams &P, int i, int j, co
                                              bool force_nu = false)
 {
     double c_s, h;
+    // the fused step's kick walk forms the pressure from E itself
+    if (P.e_partner) with_pressure = false;
     if (lazy_derived<SPLIT>(P)) {
-        // the fields the step reads stored: H (the potential's smoothing) and P
+        // the field the step reads stored: H (the potential's smoothing)
         const AdiDerived a = adi_derived(P, i, d.e, d.sig);
         fld<SPLIT>(P, FCPT_SCALE_HEIGHT).set(i, j, a.h);
         if (with_pressure) fld<SPLIT>(P, FCPT_PRESSURE).set(i, j, (P.gamma - 1.0) * d.e);
@@ -1669,8 +1671,8 @@ FCPT_PAIR1(av_fused, 256, int, with_viscosity)
 constexpr int kKickSegW = 60;
 
 struct KickRing {
-    double vr, va, s, p, phi;  // the lane's sector (p: pressure, phi: potential)
-    double sL, vaR;            // Sigma of sector j - 1, v_phi of sector j + 1
+    double vr, va, s, p, phi, e;  // the lane's sector (p: pressure, phi: potential, e: E)
+    double sL, vaR;               // Sigma of sector j - 1, v_phi of sector j + 1
 };
 
 template <bool SPLIT, bool ADI, bool SN>
@@ -1700,10 +1702,14 @@ __device__ __forceinline__ void b_kick_walk(const KParams &P, int bx, int by, in
         if (r < nr) {
             w.va = vaz(r, j);
             w.s = S(r, j);
-            w.p = ADI ? fld<SPLIT>(P, FCPT_PRESSURE)(r, j) : iso_pressure(P, r, w.s);
+            // P = (gamma - 1) E as the last step stored it (compute_pressure):
+            // with the partner buffer no wavefront writes E during the walk
+            w.e = ADI ? E(r, j) : 0.0;
+            w.p = ADI ? (P.e_partner ? (P.gamma - 1.0) * w.e : fld<SPLIT>(P, FCPT_PRESSURE)(r, j))
+                      : iso_pressure(P, r, w.s);
             w.phi = phi_fld(P)(r, j);
         } else {
-            w.va = w.s = w.p = w.phi = 0.0;
+            w.va = w.s = w.p = w.phi = w.e = 0.0;
         }
         w.sL = from_left(w.s);
         w.vaR = from_right(w.va);
@@ -1767,13 +1773,17 @@ __device__ __forceinline__ void b_kick_walk(const KParams &P, int bx, int by, in
         const bool radial = i >= 2 && inner;
         if (own) {
             if constexpr (ADI) {
-                // compression_heating (rings 0 .. nr-2), then the dissipation
-                double e = E(i, j);
+                // compression_heating (rings 0 .. nr-2), then the dissipation;
+                // the result to E or (e_partner) to its partner buffer, every cell
+                const Fld<SPLIT> Eo = P.e_partner ? buf<SPLIT>(P, P.ealt, P.bg[FCPT_ENERGY], FCPT_ENERGY) : E;
+                double e = R0.e;
                 if (i < nr - 1) e = E.stored(i, j, src_energy_new(P, i, dt, e, v1r0, v1r1, v1a0, v1aR));
                 if (diss) e = sn ? sn_dissipation(P, i, q.sn, R0.s, dt, e) : tw_dissipation(P, i, q.tw, R0.s, dt, e);
                 if (i < nr - 1 || diss) {
-                    E.set(i, j, e);
+                    Eo.set(i, j, e);
                     e = E.stored(i, j, e);
+                } else if (P.e_partner) {
+                    Eo.set_raw(i, j, E.raw(i, j));
                 }
                 // recalculate_viscosity
                 const AdiDerived d = adi_derived(P, i, e, R0.s);
@@ -2164,7 +2174,8 @@ __device__ __forceinline__ void b_visc_walk(const KParams &P, int bx, int by, in
             else
                 vr_o.set_raw(i, j, vr.raw(i, j));
             if constexpr (ENERGY) {
-                const double e = fld<SPLIT>(P, FCPT_ENERGY)(i, j);
+                const double e = P.e_partner ? buf<SPLIT>(P, P.ealt, P.bg[FCPT_ENERGY])(i, j)
+                                             : fld<SPLIT>(P, FCPT_ENERGY)(i, j);
                 // H as the kick walk formed it from the same E and Sigma
                 const double h = lazy_derived<SPLIT>(P) ? adi_derived(P, i, e, A.s).h
                                                          : fld<SPLIT>(P, FCPT_SCALE_HEIGHT)(i, j);
