@@ -288,6 +288,27 @@ __device__ __forceinline__ int wave_ring(int r0, int by)
     return __builtin_amdgcn_readfirstlane(r0 + by * (int)blockDim.y + (int)threadIdx.y);
 }
 
+// XCD-aware block order.  The dispatcher hands workgroup b of a launch to XCD
+// b mod 8 (each XCD has its own L2), so blocks adjacent in the grid, which
+// share stencil halos (sector halo lanes, cache lines at segment edges, the
+// rings a walk reloads), would run on different XCDs and fetch the shared
+// lines once per XCD.  xcd_block gives each XCD a contiguous run of the grid
+// instead: the workgroups XCD x receives (b = x, x + 8, ...) take the x-th
+// run of linear indices in order, so neighbours share an L2 and run close in
+// time.  A bijection of [0, gx gy); the block's work is unchanged.
+struct BlockXY {
+    int x, y;
+};
+__device__ __forceinline__ BlockXY xcd_block()
+{
+    constexpr unsigned kXcds = 8;
+    const unsigned gx = gridDim.x, G = gridDim.x * gridDim.y;
+    const unsigned b = blockIdx.y * gx + blockIdx.x;
+    const unsigned xcd = b % kXcds, idx = b / kXcds, q = G / kXcds, r = G % kXcds;
+    const unsigned L = xcd < r ? xcd * (q + 1) + idx : r * (q + 1) + (xcd - r) * q + idx;
+    return BlockXY{(int)(L % gx), (int)(L / gx)};
+}
+
 // 2-D launch over rows [r0, r1) x all sectors; (bx, by) is the block's place
 // in the disk's own grid
 #define CELL_LOOP_HEAD(r0, r1)                                   \
@@ -1816,7 +1837,8 @@ __device__ __forceinline__ void b_kick_walk(const KParams &P, int bx, int by, in
 template <bool SPLIT, bool ADI, bool SN>
 __global__ void __launch_bounds__(256) k_kick_walk(KParams P, int tile)
 {
-    b_kick_walk<SPLIT, ADI, SN>(P, (int)blockIdx.x, (int)blockIdx.y, tile);
+    const BlockXY bb = xcd_block();
+    b_kick_walk<SPLIT, ADI, SN>(P, bb.x, bb.y, tile);
 }
 // batched form: each disk's tile is its resolved kick_tile
 template <bool SPLIT, bool ADI, bool SN>
@@ -2200,7 +2222,8 @@ __device__ __forceinline__ void b_visc_walk(const KParams &P, int bx, int by, in
 template <bool SPLIT, bool ENERGY>
 __global__ void __launch_bounds__(256) k_visc_walk(KParams P, int tile)
 {
-    b_visc_walk<SPLIT, ENERGY>(P, (int)blockIdx.x, (int)blockIdx.y, tile);
+    const BlockXY bb = xcd_block();
+    b_visc_walk<SPLIT, ENERGY>(P, bb.x, bb.y, tile);
 }
 // batched form: each disk's tile is its resolved visc_tile
 template <bool SPLIT, bool ENERGY>
@@ -2640,12 +2663,14 @@ __device__ __forceinline__ void radial_sweep(const KParams &P, int bx, int by, i
 template <bool SPLIT, bool ADI>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_radial_sweep2(KParams P, int tile)
 {
-    radial_sweep<SPLIT, ADI>(P, (int)blockIdx.x, (int)blockIdx.y, tile);
+    const BlockXY bb = xcd_block();
+    radial_sweep<SPLIT, ADI>(P, bb.x, bb.y, tile);
 }
 template <bool SPLIT, bool ADI>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_radial_sweep3(KParams P, int tile)
 {
-    radial_sweep<SPLIT, ADI>(P, (int)blockIdx.x, (int)blockIdx.y, tile);
+    const BlockXY bb = xcd_block();
+    radial_sweep<SPLIT, ADI>(P, bb.x, bb.y, tile);
 }
 // batched forms: each disk's tile is its resolved radial_tile
 template <bool SPLIT, bool ADI>
@@ -2844,7 +2869,8 @@ __device__ __forceinline__ void b_azimuthal_sweep(const KParams &P, int bx, int 
 template <bool SPLIT, bool ADI>
 __global__ void __launch_bounds__(64 * kAzWaves) k_azimuthal_sweep(KParams P)
 {
-    b_azimuthal_sweep<SPLIT, ADI>(P, (int)blockIdx.x, (int)blockIdx.y);
+    const BlockXY bb = xcd_block();
+    b_azimuthal_sweep<SPLIT, ADI>(P, bb.x, bb.y);
 }
 template <bool SPLIT, bool ADI>
 __global__ void __launch_bounds__(64 * kAzWaves) kb_azimuthal_sweep(const KParams *Ps, const BatchEntry *map)
