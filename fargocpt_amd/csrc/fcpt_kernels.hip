@@ -132,13 +132,13 @@ __device__ __forceinline__ const __attribute__((address_space(4))) T *cst(const 
 // per-step roundings of a slowly changing value do not add up in one
 // direction: an increment below half an FP32 unit of the deviation survives
 // on average instead of being lost every step).  The uniform variate of a
-// store is Marsaglia's xorshift32 of the cell index, the deviation's high
-// word (sign, exponent and leading mantissa bits: independent of the dropped
-// bits, and different for the several values one field of one cell takes in
-// a step) and a key drawn per field and step (sr_key, Wellons' lowbias32,
-// public domain): a value always rounds the same way at its cell within one
-// kernel (a thread can form what another thread stores) and differently from
-// step to step.
+// store is one multiply-xorshift round (Wellons' hash prospector, public
+// domain) of the cell index, the deviation's high word (sign, exponent and
+// leading mantissa bits: independent of the dropped bits, and different for
+// the several values one field of one cell takes in a step) and a key drawn
+// per field and step (sr_key, Wellons' lowbias32): a value always rounds the
+// same way at its cell within one kernel (a thread can form what another
+// thread stores) and differently from step to step.
 __device__ __forceinline__ unsigned sr_hash(unsigned x)
 {
     x ^= x >> 16;
@@ -148,11 +148,12 @@ __device__ __forceinline__ unsigned sr_hash(unsigned x)
     x ^= x >> 16;
     return x;
 }
+// one multiply-xorshift round (the first half of lowbias32)
 __device__ __forceinline__ unsigned sr_bits(unsigned x)
 {
-    x ^= x << 13;
-    x ^= x >> 17;
-    x ^= x << 5;
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
     return x;
 }
 __device__ __forceinline__ float round_dev(double d, unsigned key, size_t c)
