@@ -33,7 +33,7 @@ KERNEL_DISK_FORCE = 6  # the disk-on-planet reduction (not part of the hydro ste
 PREC_FP64, PREC_SPLIT32 = 0, 1
 INTEGRATOR_EULER, INTEGRATOR_LEAPFROG = 0, 1
 PARAM_RADIAL_TILE, PARAM_RADIAL_WAVES, PARAM_FUSED_STEP, PARAM_VISC_TILE, PARAM_KICK_TILE = 0, 1, 2, 3, 4
-PARAM_SPLIT_ROUNDING = 5
+PARAM_SPLIT_ROUNDING, PARAM_REBASE_INTERVAL = 5, 6
 BC = {"reflecting": 0, "zerogradient": 1, "outflow": 2}
 MAX_PLANETS = 8
 
@@ -220,6 +220,7 @@ def load_library(path: str = LIB_PATH):
         "fcpt_time_copy": [P, I, dp, dp],
         "fcpt_set_launch_param": [P, I, I],
         "fcpt_get_launch_param": [P, I, C.POINTER(I)],
+        "fcpt_rebase_backgrounds": [P],
         "fcpt_autotune": [P, I, C.POINTER(I), C.POINTER(I)],
         "fcpt_autotune_walks": [P, I, C.POINTER(I), C.POINTER(I)],
         "fcpt_disk_on_planet_accel": [P, C.POINTER(ForceBody), I, I, dp],
@@ -498,6 +499,16 @@ class Disk:
         stochastically and the derived fields' to nearest, 2 every field
         stochastically, 0 every field to nearest."""
         self._check(self.lib.fcpt_set_launch_param(self.h, PARAM_SPLIT_ROUNDING, int(mode)))
+
+    def set_rebase_interval(self, steps):
+        """Split storage: move the ring backgrounds to the current ring means
+        after every `steps` steps of run_steps (default 1000; 0: only on
+        request)."""
+        self._check(self.lib.fcpt_set_launch_param(self.h, PARAM_REBASE_INTERVAL, int(steps)))
+
+    def rebase_backgrounds(self):
+        """Split storage: move the ring backgrounds to the current ring means."""
+        self._check(self.lib.fcpt_rebase_backgrounds(self.h))
 
     def launch_param(self, param):
         """The value launch parameter `param` (PARAM_*) takes in the next step."""

@@ -4,6 +4,7 @@
 // priority); functions that hand a value back to the host synchronise it.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -17,6 +18,10 @@
 #include "fcpt_kernels.hpp"
 
 using namespace fcpt;
+
+// split storage: steps of fcpt_run_steps between two moves of the ring
+// backgrounds unless the caller sets another interval
+constexpr int kDefaultRebaseInterval = 1000;
 
 struct fcpt_ctx {
     fcpt_disk_desc desc{};
@@ -45,6 +50,10 @@ struct fcpt_ctx {
     double *bg_tmp = nullptr;
     // fused FP64 adiabatic steps leave c_s, nu and T to be formed (lazy_derived)
     bool derived_stale = false;
+    // split storage: steps between background moves (FCPT_PARAM_REBASE_INTERVAL;
+    // 0 = only on request) and steps since the last one
+    int rebase_interval = 0;
+    int steps_since_rebase = 0;
 };
 
 namespace {
@@ -294,6 +303,7 @@ int fcpt_create(const fcpt_disk_desc *desc, fcpt_ctx **out)
     P.ns = ns;
     P.split = desc->precision == FCPT_PREC_SPLIT32 ? 1 : 0;
     P.split_rounding = P.split;  // stochastic rounding of the split deviations by default
+    c->rebase_interval = P.split ? kDefaultRebaseInterval : 0;
     P.radial_tile = 0;
     P.radial_waves = 3;
     P.visc_tile = 0;
@@ -671,6 +681,37 @@ static int set_derived_backgrounds(fcpt_ctx *ctx)
                                     hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     return FCPT_OK;
+}
+
+// split storage: the backgrounds of the evolved fields move to their current
+// ring means (each deviation re-expressed about the new background, rounded
+// to nearest once) and the derived fields' and momenta's backgrounds follow
+static int rebase_backgrounds(fcpt_ctx *ctx)
+{
+    const KParams &P = ctx->P;
+    ctx->steps_since_rebase = 0;
+    if (!P.split) return FCPT_OK;
+    int rc = sync_derived(ctx);
+    if (rc) return rc;
+    const int state[4] = {FCPT_SIGMA, FCPT_VRAD, FCPT_VAZI, FCPT_ENERGY};
+    for (int k = 0; k < (P.adiabatic ? 4 : 3); ++k) {
+        const int f = state[k];
+        const int rows = (int)field_rows(f, P.nr);
+        std::vector<double> bg((size_t)P.nr + 2, 0.0);
+        launch_split_ring_mean(P, ctx->stream, P.f[f], P.bg[f], rows, ctx->bg_tmp);
+        LAUNCH_CHECK(ctx);
+        HIP_TRY(ctx, hipMemcpyAsync(bg.data(), ctx->bg_tmp, (size_t)rows * sizeof(double),
+                                    hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        if ((rc = rebase_field(ctx, f, bg))) return rc;
+    }
+    return set_derived_backgrounds(ctx);
+}
+
+int fcpt_rebase_backgrounds(fcpt_ctx *ctx)
+{
+    CTX_CHECK(ctx);
+    return rebase_backgrounds(ctx);
 }
 
 // copy_initial_values (damping.cpp:301-309): the damping references take the
@@ -1173,6 +1214,9 @@ static int ensure_step_graph(fcpt_ctx *ctx)
     return FCPT_OK;
 }
 
+// split storage with a rebase interval: the steps before the next background move
+static bool periodic_rebase(const fcpt_ctx *ctx) { return ctx->P.split && ctx->rebase_interval > 0; }
+
 int fcpt_run_steps(fcpt_ctx *ctx, int32_t n_steps, int32_t use_graph)
 {
     CTX_CHECK(ctx);
@@ -1180,14 +1224,23 @@ int fcpt_run_steps(fcpt_ctx *ctx, int32_t n_steps, int32_t use_graph)
     if (use_graph) {
         int rc = ensure_step_graph(ctx);
         if (rc) return rc;
-        for (int k = 0; k < n_steps; ++k) HIP_TRY(ctx, hipGraphLaunch(ctx->step_graph, ctx->stream));
-    } else {
-        for (int k = 0; k < n_steps; ++k) {
-            enqueue_iteration(ctx);
-        }
-        LAUNCH_CHECK(ctx);
     }
-    if (n_steps > 0 && lazy_applies(ctx)) ctx->derived_stale = true;
+    for (int left = n_steps; left > 0;) {
+        int m = left;
+        if (periodic_rebase(ctx)) m = std::min(m, ctx->rebase_interval - ctx->steps_since_rebase);
+        if (use_graph) {
+            for (int k = 0; k < m; ++k) HIP_TRY(ctx, hipGraphLaunch(ctx->step_graph, ctx->stream));
+        } else {
+            for (int k = 0; k < m; ++k) enqueue_iteration(ctx);
+            LAUNCH_CHECK(ctx);
+        }
+        left -= m;
+        if (lazy_applies(ctx)) ctx->derived_stale = true;
+        if (periodic_rebase(ctx) && (ctx->steps_since_rebase += m) >= ctx->rebase_interval) {
+            const int rc = rebase_backgrounds(ctx);
+            if (rc) return rc;
+        }
+    }
     return FCPT_OK;
 }
 
@@ -1297,6 +1350,11 @@ int fcpt_set_launch_param(fcpt_ctx *ctx, int32_t param, int32_t value)
         if (value < 0 || value > 64) return fail(ctx, FCPT_ERR_ARG, "kick tile must be 0 .. 64");
     } else if (param == FCPT_PARAM_SPLIT_ROUNDING) {
         if (value < 0 || value > 2) return fail(ctx, FCPT_ERR_ARG, "split rounding must be 0, 1 or 2");
+    } else if (param == FCPT_PARAM_REBASE_INTERVAL) {
+        if (value < 0) return fail(ctx, FCPT_ERR_ARG, "rebase interval must be 0 or positive");
+        ctx->rebase_interval = ctx->P.split ? value : 0;
+        ctx->steps_since_rebase = 0;
+        return FCPT_OK;
     } else {
         return fail(ctx, FCPT_ERR_ARG, "unknown launch parameter");
     }
@@ -1333,6 +1391,18 @@ int fcpt_autotune_walks(fcpt_ctx *ctx, int32_t reps, int32_t *kick_tile, int32_t
         if (visc_tile) *visc_tile = visc0;
         return FCPT_OK;  // the walks do not run in this context's step
     }
+    // the trials' steps neither move the split backgrounds (the saved
+    // deviations go back about the same backgrounds) nor count toward the next move
+    struct RebaseHold {
+        fcpt_ctx *c;
+        int interval, since;
+        ~RebaseHold()
+        {
+            c->rebase_interval = interval;
+            c->steps_since_rebase = since;
+        }
+    } hold{ctx, ctx->rebase_interval, ctx->steps_since_rebase};
+    ctx->rebase_interval = 0;
     // the saved state: every field and the scalars
     size_t total = 0;
     std::vector<size_t> off(FCPT_NUM_FIELDS);
@@ -1427,6 +1497,7 @@ int fcpt_get_launch_param(fcpt_ctx *ctx, int32_t param, int32_t *value)
     case FCPT_PARAM_VISC_TILE: *value = resolve_visc_tile(P); break;
     case FCPT_PARAM_KICK_TILE: *value = resolve_kick_tile(P); break;
     case FCPT_PARAM_SPLIT_ROUNDING: *value = P.split_rounding; break;
+    case FCPT_PARAM_REBASE_INTERVAL: *value = ctx->rebase_interval; break;
     default: return fail(ctx, FCPT_ERR_ARG, "unknown launch parameter");
     }
     return FCPT_OK;
@@ -1953,31 +2024,42 @@ int fcpt_batch_run_steps(fcpt_batch *b, int32_t n_steps, int32_t use_graph)
     BHIP(b, hipSetDevice(b->device));
     // the disks' own queued work (uploads, start-up calls) comes first
     for (fcpt_ctx *c : b->disks) BHIP(b, hipStreamSynchronize(c->stream));
-    if (use_graph) {
-        if (!b->graph) {
-            hipGraph_t g = nullptr;
-            BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-            enqueue_batch_step(b, b->stream);
-            hipError_t e = hipStreamEndCapture(b->stream, &g);
-            if (e != hipSuccess) return bfail(b, FCPT_ERR_DEVICE, hipGetErrorString(e));
-            e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
-            hipGraphDestroy(g);
-            if (e != hipSuccess) {
-                b->graph = nullptr;
-                return bfail(b, FCPT_ERR_DEVICE, hipGetErrorString(e));
-            }
+    if (use_graph && !b->graph) {
+        hipGraph_t g = nullptr;
+        BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        enqueue_batch_step(b, b->stream);
+        hipError_t e = hipStreamEndCapture(b->stream, &g);
+        if (e != hipSuccess) return bfail(b, FCPT_ERR_DEVICE, hipGetErrorString(e));
+        e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            b->graph = nullptr;
+            return bfail(b, FCPT_ERR_DEVICE, hipGetErrorString(e));
         }
-        for (int k = 0; k < n_steps; ++k) BHIP(b, hipGraphLaunch(b->graph, b->stream));
-    } else {
-        for (int k = 0; k < n_steps; ++k) enqueue_batch_step(b, b->stream);
-        BHIP(b, hipGetLastError());
     }
-    // a disk call made after this one (a download, the totals) is ordered
-    // after the batched steps on the disk's own stream
-    BHIP(b, hipEventRecord(b->done, b->stream));
-    for (fcpt_ctx *c : b->disks) BHIP(b, hipStreamWaitEvent(c->stream, b->done, 0));
-    for (size_t k = 0; k < b->disks.size(); ++k)
-        if (n_steps > 0 && b->lazy[k]) b->disks[k]->derived_stale = true;
+    for (int left = n_steps; left > 0;) {
+        // up to the next background move of a member in split storage
+        int m = left;
+        for (fcpt_ctx *c : b->disks)
+            if (periodic_rebase(c)) m = std::min(m, c->rebase_interval - c->steps_since_rebase);
+        if (use_graph) {
+            for (int k = 0; k < m; ++k) BHIP(b, hipGraphLaunch(b->graph, b->stream));
+        } else {
+            for (int k = 0; k < m; ++k) enqueue_batch_step(b, b->stream);
+            BHIP(b, hipGetLastError());
+        }
+        left -= m;
+        // a disk call made after this one (a download, the totals) is ordered
+        // after the batched steps on the disk's own stream
+        BHIP(b, hipEventRecord(b->done, b->stream));
+        for (fcpt_ctx *c : b->disks) BHIP(b, hipStreamWaitEvent(c->stream, b->done, 0));
+        for (size_t k = 0; k < b->disks.size(); ++k)
+            if (b->lazy[k]) b->disks[k]->derived_stale = true;
+        for (fcpt_ctx *c : b->disks) {
+            if (!periodic_rebase(c) || (c->steps_since_rebase += m) < c->rebase_interval) continue;
+            if (rebase_backgrounds(c)) return bfail(b, FCPT_ERR_DEVICE, c->err.c_str());
+        }
+    }
     return FCPT_OK;
 }
 

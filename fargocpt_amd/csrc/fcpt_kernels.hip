@@ -3117,6 +3117,20 @@ __global__ void __launch_bounds__(256) k_rebase(float *d, const double *oldbg, c
     d[c] = (float)((oldbg[i] + (double)d[c]) - newbg[i]);
 }
 
+// split storage: the ring means of a field, background + mean deviation (one
+// block per row; the deviations summed in FP64, pairwise tree order)
+__global__ void __launch_bounds__(256) k_split_ring_mean(const float *d, const double *bg, int ns,
+                                                         double *out)
+{
+    __shared__ double sh[256];
+    const int i = blockIdx.x;
+    const float *row = d + (size_t)i * (size_t)ns;
+    double acc = 0.0;
+    for (int j = threadIdx.x; j < ns; j += 256) acc += (double)row[j];
+    const double sum = block_sum<256>(acc, sh);
+    if (threadIdx.x == 0) out[i] = bg[i] + sum / ns;
+}
+
 // ---------------------------------------------------------------------------
 // launch helpers: each dispatches on the storage mode
 
@@ -3501,6 +3515,13 @@ void launch_rebase(const KParams &P, hipStream_t st, void *field, const double *
 {
     hipLaunchKernelGGL(k_rebase, cell_grid(P, rows), kBlock, 0, st, static_cast<float *>(field), oldbg,
                        newbg, P.ns, rows);
+}
+
+void launch_split_ring_mean(const KParams &P, hipStream_t st, const void *field, const double *bg,
+                            int rows, double *out)
+{
+    hipLaunchKernelGGL(k_split_ring_mean, dim3((unsigned)rows), dim3(256), 0, st,
+                       static_cast<const float *>(field), bg, P.ns, out);
 }
 
 void launch_totals(const KParams &P, hipStream_t st, double radius, double *ring_scratch,

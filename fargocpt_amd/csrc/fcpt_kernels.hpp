@@ -79,6 +79,9 @@ void launch_totals(const KParams &P, hipStream_t st, double radius, double *ring
 // background oldbg to newbg (both device arrays)
 void launch_rebase(const KParams &P, hipStream_t st, void *field, const double *oldbg,
                    const double *newbg, int rows);
+// split storage: out[i] = ring mean of the field's row i (rows rows)
+void launch_split_ring_mean(const KParams &P, hipStream_t st, const void *field, const double *bg,
+                            int rows, double *out);
 // bytes of one stored value of a 2-D field (8 in FP64 storage, 4 in split storage)
 size_t element_size(const KParams &P);
 // bytes per value of one field: the potential is FP64 in both storage modes

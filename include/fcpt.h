@@ -318,12 +318,31 @@ int fcpt_time_copy(fcpt_ctx *ctx, int32_t reps, double *ms_per_launch, double *b
    sub-unit increments every step) for the evolved fields Sigma, v_r, v_phi, E
    and the transported momenta, to nearest for the fields derived afresh every
    step (1); stochastically for every field (2); to nearest everywhere (0). */
+/* FCPT_PARAM_REBASE_INTERVAL (split storage only; default 1000) moves the
+   backgrounds to the current ring means (fcpt_rebase_backgrounds) after every
+   that many steps of fcpt_run_steps, fcpt_time_steps and fcpt_batch_run_steps
+   (counted across calls; setting it restarts the count), so that a disk whose
+   ring profiles drift (a gap opening, viscous spreading) keeps its stored
+   deviations, and with them their FP32 rounding, small.  Each move reads the
+   state once, copies 8 (nr + 2) bytes per evolved field to the host and
+   synchronizes the stream.  0 moves them only on request.  The per-function
+   step entry points (fcpt_run_step, fcpt_step_euler, ...) neither move them
+   nor count; fcpt_autotune_walks' trial steps do not count either. */
 enum { FCPT_PARAM_RADIAL_TILE = 0, FCPT_PARAM_RADIAL_WAVES = 1, FCPT_PARAM_FUSED_STEP = 2,
-       FCPT_PARAM_VISC_TILE = 3, FCPT_PARAM_KICK_TILE = 4, FCPT_PARAM_SPLIT_ROUNDING = 5 };
+       FCPT_PARAM_VISC_TILE = 3, FCPT_PARAM_KICK_TILE = 4, FCPT_PARAM_SPLIT_ROUNDING = 5,
+       FCPT_PARAM_REBASE_INTERVAL = 6 };
 int fcpt_set_launch_param(fcpt_ctx *ctx, int32_t param, int32_t value);
 /* The value a launch parameter takes in the next step: the size-based
    choice where it was left at 0 (tiles), else the value set. */
 int fcpt_get_launch_param(fcpt_ctx *ctx, int32_t param, int32_t *value);
+/* Split storage (no counterpart in the reference, whose fields are FP64):
+   moves the per-ring backgrounds of Sigma, v_r, v_phi and (adiabatic) E to
+   the fields' current ring means, re-expressing every deviation about its new
+   background (rounded to nearest once), and re-derives the backgrounds of the
+   derived fields and transport momenta from them, as an upload does.  The
+   decoded values change by at most half an FP32 unit of their new deviation.
+   Synchronizes the stream; FP64 storage: nothing to do. */
+int fcpt_rebase_backgrounds(fcpt_ctx *ctx);
 /* Autotuner (no counterpart in the reference): times every launch shape of
    the tunable kernels on the context's current state (`reps` launches each;
    the timed kernels write transport scratch only), keeps the fastest and

@@ -14,7 +14,11 @@ Checks:
   the FP64 path from the same start: Sigma (and E) within 1e-5 relative
   (max |d| / max |Sigma|), v_r, v_phi within the stated bounds, and the
   compensated totals;
-- graph replay and stepwise calls give the same split result.
+- graph replay and stepwise calls give the same split result;
+- a background move (fcpt_rebase_backgrounds) changes the decoded fields by
+  at most half an FP32 unit of their new deviations; runs that move the
+  backgrounds every few steps stay within the split bounds of the FP64 run,
+  and a batch with the same interval evolves bit for bit like the disk alone.
 """
 import numpy as np
 import pytest
@@ -159,3 +163,64 @@ def test_split_graph_matches_stepwise():
         assert np.array_equal(a.download_field(f), b.download_field(f)), f
     a.close()
     b.close()
+
+
+def test_split_rebase_keeps_values():
+    from fargocpt_amd import workloads as W
+    from fargocpt_amd.capi import PARAM_REBASE_INTERVAL, Disk
+    name = "adi_512x1536"
+    print(f"{name}: split background move after 20 steps", flush=True)
+    d = Disk(W.disk_config(name, 1))
+    assert d.launch_param(PARAM_REBASE_INTERVAL) == 1000
+    W.start(d, name)
+    d.run_steps(20, True)
+    d.synchronize()
+    fields = ("SIGMA", "VRAD", "VAZI", "ENERGY")
+    before = {f: d.download_field(f) for f in fields}
+    d.rebase_backgrounds()
+    for f in fields:
+        x, y = before[f], d.download_field(f)
+        dev = np.abs(x - x.mean(axis=1, keepdims=True))
+        bound = 2.0 ** -24 * dev + 2.0 ** -50 * np.abs(x) + 1e-300
+        assert np.all(np.abs(y - x) <= 1.01 * bound), f
+    d.set_rebase_interval(7)
+    assert d.launch_param(PARAM_REBASE_INTERVAL) == 7
+    d.close()
+    fp64 = Disk(W.disk_config(name, 0))
+    fp64.set_rebase_interval(7)
+    assert fp64.launch_param(PARAM_REBASE_INTERVAL) == 0   # nothing to move in FP64 storage
+    fp64.close()
+
+
+def test_split_periodic_rebase_run():
+    from fargocpt_amd import workloads as W
+    from fargocpt_amd.capi import Batch, Disk
+    name = "adi_512x1536"
+    steps = 100
+    print(f"{name}: {steps} steps, split with a background move every 10 steps", flush=True)
+    ref = Disk(W.disk_config(name, 0))
+    W.start(ref, name)
+    ref.run_steps(steps, True)
+    ref.synchronize()
+    alone = Disk(W.disk_config(name, 1))
+    W.start(alone, name)
+    alone.set_rebase_interval(10)
+    alone.run_steps(33, True)      # chunks that end between moves
+    alone.run_steps(steps - 33, True)
+    alone.synchronize()
+    batched = Disk(W.disk_config(name, 1))
+    W.start(batched, name)
+    batched.set_rebase_interval(10)
+    b = Batch([batched])
+    b.run_steps(steps, True)
+    b.synchronize()
+    for f in ("SIGMA", "VRAD", "VAZI", "ENERGY"):
+        a = alone.download_field(f)
+        err = rel_err(a, ref.download_field(f))
+        print(f"{f}: split with moves vs FP64 {err:.2e}", flush=True)
+        assert err <= SPLIT_FIXTURE_TOL, (f, err)
+        assert np.array_equal(batched.download_field(f), a), f
+    assert batched.get_time() == alone.get_time()
+    b.close()
+    for d in (ref, alone, batched):
+        d.close()
