@@ -55,8 +55,8 @@ __device__ __forceinline__ double vl_lim(double a, double b)
 // the scaling nor the fix-up changes anything, so the same reciprocal and
 // correction steps written out give the same bits, and the reciprocal part
 // (five instructions) is formed once per denominator: a quotient then costs
-// three.  Other operands (zero aside, which returns the signed zero n * r)
-// take the division itself.
+// three.  Other operands (zeros, infinities and NaNs among them) take the
+// division itself; the range tests are two magnitude compares each.
 struct SharedDen {
     double d, r;
     bool ok;
@@ -68,17 +68,21 @@ __device__ __forceinline__ SharedDen shared_den(double d)
     const double r1 = fma(r0, e0, r0);
     const double e1 = fma(-d, r1, 1.0);
     const double r2 = fma(r1, e1, r1);
-    const unsigned ed = (unsigned)((unsigned long long)__double_as_longlong(d) >> 52) & 0x7ffu;
-    return SharedDen{d, r2, ed - 773u < 500u};
+    const double ad = fabs(d);
+    return SharedDen{d, r2, ad >= 0x1p-250 && ad < 0x1p250};
 }
 __device__ __forceinline__ double shared_div(double n, const SharedDen &D)
 {
     const double q0 = n * D.r;
     const double rem = fma(-D.d, q0, n);
     double q = fma(rem, D.r, q0);
-    const unsigned en = (unsigned)((unsigned long long)__double_as_longlong(n) >> 52) & 0x7ffu;
-    if (n == 0.0) q = q0;
-    else if (!D.ok || en - 623u >= 650u) q = n / D.d;
+    const double an = fabs(n);
+    if (!(D.ok && an >= 0x1p-400 && an < 0x1p250)) {
+        // an empty volatile statement keeps the division on its own branch
+        // (taken by the lanes that need it), never computed for every lane
+        asm volatile("");
+        q = n / D.d;
+    }
     return q;
 }
 
@@ -264,13 +268,14 @@ __device__ __forceinline__ Fld<false> phi_fld(const KParams &P)
 
 // Neighbour values within a wavefront: the value of lane - 1 and of lane + 1,
 // moved by the DPP wavefront shifts (wave_shr:1, wave_shl:1) on both 32-bit
-// halves; the edge lanes receive 0 and their results are discarded
+// halves; the edge lanes, which have no source lane, receive 0 (bound_ctrl,
+// so no old value has to be set first) and their results are discarded
 template <int CTRL>
 __device__ __forceinline__ double dpp_move(double x)
 {
     const long long b = __double_as_longlong(x);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b & 0xffffffffll), CTRL, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)(b & 0xffffffffll), CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, true);
     return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
 __device__ __forceinline__ double from_left(double x) { return dpp_move<0x138>(x); }
@@ -2763,17 +2768,13 @@ __device__ __forceinline__ D2 flux_div2(const D2 &f)
     return D2{va, vb};
 }
 
-// The results of both FARGO passes for this lane's two sectors of ring i:
-// the transported density and, per quantity, the transported momentum (and
-// energy), before the cyclic shift; sources at sectors jsa, jsb = jsa + 1.
-template <bool ADI>
-struct AzOut {
-    D2 s;
-    D2 q[ADI ? 5 : 4];
-};
-
-template <bool SPLIT, bool ADI>
-__device__ __forceinline__ void az_ring(const KParams &P, int i, int jsa, int jsb, double dt, AzOut<ADI> &o)
+// Both FARGO passes for this lane's two sectors of ring i (sources at
+// sectors jsa, jsb = jsa + 1): put(-1, .) receives the transported density
+// and put(q, .) each quantity's transported momentum (q = 0 .. 3) and energy
+// (q = 4), before the cyclic shift, as soon as it is formed (so its
+// registers are free for the next quantity).
+template <bool SPLIT, bool ADI, class Put>
+__device__ __forceinline__ void az_ring(const KParams &P, int i, int jsa, int jsb, double dt, Put &&put)
 {
     constexpr int NQ = ADI ? 5 : 4;
     const double vm = cst(P.vmean)[i];
@@ -2806,7 +2807,7 @@ __device__ __forceinline__ void az_ring(const KParams &P, int i, int jsa, int js
     const D2 s1{s0.a + vd0.a * invsurf, s0.b + vd0.b * invsurf};
     const D2 ds1 = star2(s1, slope2<SPLIT>(s1, invdxtheta), dxtheta, ksu2);
     const D2 vd1 = flux_div2(D2{dxrad * ds1.a * vu, dxrad * ds1.b * vu});
-    o.s = D2{s1.a + vd1.a * invsurf, s1.b + vd1.b * invsurf};
+    put(-1, D2{s1.a + vd1.a * invsurf, s1.b + vd1.b * invsurf});
     // the other quantities
     const D2 inv_s0{SPLIT ? recip<true>(s0.a) : 0.0, SPLIT ? recip<true>(s0.b) : 0.0};
     const D2 inv_s1{SPLIT ? recip<true>(s1.a) : 0.0, SPLIT ? recip<true>(s1.b) : 0.0};
@@ -2822,7 +2823,7 @@ __device__ __forceinline__ void az_ring(const KParams &P, int i, int jsa, int js
         const D2 w1{SPLIT ? q1.a * inv_s1.a : shared_div(q1.a, s1a), SPLIT ? q1.b * inv_s1.b : shared_div(q1.b, s1b)};
         const D2 qs1 = star2(w1, slope2<SPLIT>(w1, invdxtheta), dxtheta, ksu2);
         const D2 vq1 = flux_div2(D2{dxrad * qs1.a * ds1.a * vu, dxrad * qs1.b * ds1.b * vu});
-        o.q[q] = D2{q1.a + vq1.a * invsurf, q1.b + vq1.b * invsurf};
+        put(q, D2{q1.a + vq1.a * invsurf, q1.b + vq1.b * invsurf});
     }
 }
 
@@ -2849,23 +2850,21 @@ __device__ __forceinline__ void b_azimuthal_sweep(const KParams &P, int bx, int 
     const int jdb = jda + 1 == ns ? 0 : jda + 1;
     const bool wa = ka >= kAzHalo && ka < kAzHalo + Ls;
     const bool wb = kb >= kAzHalo && kb < kAzHalo + Ls;
-    AzOut<ADI> o;
-    az_ring<SPLIT, ADI>(P, i, jsa, jsb, dt, o);
-    const Fld<false> Sc = buf<false>(P, P.sig_c, nullptr);
-    if (wa) Sc.set(i, jda, o.s.a);
-    if (wb) Sc.set(i, jdb, o.s.b);
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-        if (q < 4) {
+    az_ring<SPLIT, ADI>(P, i, jsa, jsb, dt, [&](int q, const D2 &v) {
+        if (q < 0) {
+            const Fld<false> Sc = buf<false>(P, P.sig_c, nullptr);
+            if (wa) Sc.set(i, jda, v.a);
+            if (wb) Sc.set(i, jdb, v.b);
+        } else if (q < 4) {
             const Fld<SPLIT> m = buf<SPLIT>(P, P.mom[q], P.bgm[q], kBufMom + q);
-            if (wa) m.set(i, jda, o.q[q].a);
-            if (wb) m.set(i, jdb, o.q[q].b);
+            if (wa) m.set(i, jda, v.a);
+            if (wb) m.set(i, jdb, v.b);
         } else {
             const Fld<false> Ec = buf<false>(P, P.en_c, nullptr);
-            if (wa) Ec.set(i, jda, o.q[q].a);
-            if (wb) Ec.set(i, jdb, o.q[q].b);
+            if (wa) Ec.set(i, jda, v.a);
+            if (wb) Ec.set(i, jdb, v.b);
         }
-    }
+    });
 }
 template <bool SPLIT, bool ADI>
 __global__ void __launch_bounds__(64 * kAzWaves) k_azimuthal_sweep(KParams P)
