@@ -2757,6 +2757,23 @@ __device__ __forceinline__ D2 star2(const D2 &q, const D2 &dq, double dxtheta, c
     return D2{(ksi.a > 0.0) ? loa : hia, (ksi.b > 0.0) ? lob : hib};
 }
 
+// star2 for a velocity of one sign on the whole ring (the uniform residual):
+// UP (ksi > 0) takes the left sector's side for both sectors, else the own
+__device__ __forceinline__ D2 star2_up(const D2 &q, const D2 &dq, double dxtheta, double ksi)
+{
+    const D2 lq = left2(q), ldq = left2(dq);
+    return D2{lq.a + (dxtheta - ksi) * ldq.a, lq.b + (dxtheta - ksi) * ldq.b};
+}
+__device__ __forceinline__ D2 star2_down(const D2 &q, const D2 &dq, double dxtheta, double ksi)
+{
+    return D2{q.a - (dxtheta + ksi) * dq.a, q.b - (dxtheta + ksi) * dq.b};
+}
+template <bool UP>
+__device__ __forceinline__ D2 star2_uniform(const D2 &q, const D2 &dq, double dxtheta, double ksi)
+{
+    return UP ? star2_up(q, dq, dxtheta, ksi) : star2_down(q, dq, dxtheta, ksi);
+}
+
 // flux into the sector through its left interface minus flux out through the
 // right one
 __device__ __forceinline__ D2 flux_div2(const D2 &f)
@@ -2772,8 +2789,10 @@ __device__ __forceinline__ D2 flux_div2(const D2 &f)
 // sectors jsa, jsb = jsa + 1): put(-1, .) receives the transported density
 // and put(q, .) each quantity's transported momentum (q = 0 .. 3) and energy
 // (q = 4), before the cyclic shift, as soon as it is formed (so its
-// registers are free for the next quantity).
-template <bool SPLIT, bool ADI, class Put>
+// registers are free for the next quantity).  The second pass's velocity is
+// the ring's uniform residual, so its upwind side is the same for every
+// sector: UP is its sign (ksi > 0), chosen once per ring by the caller.
+template <bool SPLIT, bool ADI, bool UP, class Put>
 __device__ __forceinline__ void az_ring(const KParams &P, int i, int jsa, int jsb, double dt, Put &&put)
 {
     constexpr int NQ = ADI ? 5 : 4;
@@ -2800,12 +2819,11 @@ __device__ __forceinline__ void az_ring(const KParams &P, int i, int jsa, int js
         q0[4] = D2{Eb(i, jsa), Eb(i, jsb)};
     }
     const D2 ks0{v0.a * dt, v0.b * dt};
-    const D2 ksu2{ksu, ksu};
     // density: first pass (residual velocity), then second pass (uniform residual)
     const D2 ds0 = star2(s0, slope2<SPLIT>(s0, invdxtheta), dxtheta, ks0);
     const D2 vd0 = flux_div2(D2{dxrad * ds0.a * v0.a, dxrad * ds0.b * v0.b});
     const D2 s1{s0.a + vd0.a * invsurf, s0.b + vd0.b * invsurf};
-    const D2 ds1 = star2(s1, slope2<SPLIT>(s1, invdxtheta), dxtheta, ksu2);
+    const D2 ds1 = star2_uniform<UP>(s1, slope2<SPLIT>(s1, invdxtheta), dxtheta, ksu);
     const D2 vd1 = flux_div2(D2{dxrad * ds1.a * vu, dxrad * ds1.b * vu});
     put(-1, D2{s1.a + vd1.a * invsurf, s1.b + vd1.b * invsurf});
     // the other quantities
@@ -2821,7 +2839,7 @@ __device__ __forceinline__ void az_ring(const KParams &P, int i, int jsa, int js
         const D2 vq0 = flux_div2(D2{dxrad * qs0.a * ds0.a * v0.a, dxrad * qs0.b * ds0.b * v0.b});
         const D2 q1{q0[q].a + vq0.a * invsurf, q0[q].b + vq0.b * invsurf};
         const D2 w1{SPLIT ? q1.a * inv_s1.a : shared_div(q1.a, s1a), SPLIT ? q1.b * inv_s1.b : shared_div(q1.b, s1b)};
-        const D2 qs1 = star2(w1, slope2<SPLIT>(w1, invdxtheta), dxtheta, ksu2);
+        const D2 qs1 = star2_uniform<UP>(w1, slope2<SPLIT>(w1, invdxtheta), dxtheta, ksu);
         const D2 vq1 = flux_div2(D2{dxrad * qs1.a * ds1.a * vu, dxrad * qs1.b * ds1.b * vu});
         put(q, D2{q1.a + vq1.a * invsurf, q1.b + vq1.b * invsurf});
     }
@@ -2850,7 +2868,7 @@ __device__ __forceinline__ void b_azimuthal_sweep(const KParams &P, int bx, int 
     const int jdb = jda + 1 == ns ? 0 : jda + 1;
     const bool wa = ka >= kAzHalo && ka < kAzHalo + Ls;
     const bool wb = kb >= kAzHalo && kb < kAzHalo + Ls;
-    az_ring<SPLIT, ADI>(P, i, jsa, jsb, dt, [&](int q, const D2 &v) {
+    auto put = [&](int q, const D2 &v) {
         if (q < 0) {
             const Fld<false> Sc = buf<false>(P, P.sig_c, nullptr);
             if (wa) Sc.set(i, jda, v.a);
@@ -2864,7 +2882,9 @@ __device__ __forceinline__ void b_azimuthal_sweep(const KParams &P, int bx, int 
             if (wa) Ec.set(i, jda, v.a);
             if (wb) Ec.set(i, jdb, v.b);
         }
-    });
+    };
+    if (cst(P.vuni)[i] * dt > 0.0) az_ring<SPLIT, ADI, true>(P, i, jsa, jsb, dt, put);
+    else az_ring<SPLIT, ADI, false>(P, i, jsa, jsb, dt, put);
 }
 template <bool SPLIT, bool ADI>
 __global__ void __launch_bounds__(64 * kAzWaves) k_azimuthal_sweep(KParams P)
