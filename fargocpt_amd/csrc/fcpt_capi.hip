@@ -92,15 +92,14 @@ int fail(fcpt_ctx *c, int code, const char *fmt, ...)
     } while (0)
 
 // When the fused Euler step of an adiabatic disk runs the kick and viscous
-// walks, in FP64 storage the step forms c_s, nu and T where it needs them
-// (KParams::lazy_derived); in split storage the kick walk forms the pressure
-// from E and passes E to the viscous walk through its partner buffer
+// walks, the step forms c_s, nu and T where it needs them
+// (KParams::lazy_derived); in split storage the kick walk also forms the
+// pressure from E and passes E to the viscous walk through its partner buffer
 // (KParams::e_partner), so the end of the step does not store the pressure
 // (which saves a decode and an encode per cell where the split kernels are
 // bound by their arithmetic; in FP64 storage, bound by the bytes, measured
 // 0.5 % slower, so not used there).  Afterwards those fields are stale until
-// sync_derived stores them (the same values: the same expressions on the
-// stored E and Sigma).
+// sync_derived stores them (the same expressions on the stored E and Sigma).
 static bool lazy_applies(const fcpt_ctx *ctx)
 {
     const KParams &P = ctx->P;
@@ -110,7 +109,7 @@ static KParams step_view(const fcpt_ctx *ctx)
 {
     KParams P = ctx->P;
     P.e_partner = lazy_applies(ctx) && P.split ? 1 : 0;
-    P.lazy_derived = lazy_applies(ctx) && !P.split ? 1 : 0;
+    P.lazy_derived = lazy_applies(ctx) ? 1 : 0;
     return P;
 }
 static int sync_derived(fcpt_ctx *ctx)
@@ -1934,7 +1933,7 @@ int fcpt_batch_create(fcpt_ctx *const *disks, int32_t n_disks, fcpt_batch **out)
         // as in the disk's own fused step
         const bool walks = P.adiabatic && kick_walk_applies(P);
         P.e_partner = walks && P.split ? 1 : 0;
-        P.lazy_derived = walks && !P.split ? 1 : 0;
+        P.lazy_derived = walks ? 1 : 0;
         b->lazy.push_back(walks);
         host.push_back(P);
     }

@@ -87,7 +87,7 @@ struct KParams {
     int radial_waves;    // register budget of the radial sweep: wavefronts per SIMD (2 or 3)
     int visc_tile;       // rings per thread of the fused viscous pass (0: size-based choice)
     int kick_tile;       // rings per thread of the kick walk (0: size-based choice, 1: off)
-    // the fused Euler step of an adiabatic disk in FP64 storage: the sound
+    // the fused Euler step of an adiabatic disk (either storage): the sound
     // speed, viscosity and temperature are formed from E and Sigma where the
     // step needs them instead of stored (the kick walk stores nu only); the
     // host stores them all (k_derived) before anything else reads the fields

@@ -536,12 +536,11 @@ __device__ __forceinline__ AdiDerived adi_derived(const KParams &P, int i, doubl
     return d;
 }
 
-// true where the fused FP64 step forms c_s, nu and T instead of storing them
+// true where the fused step forms c_s, nu and T instead of storing them
 template <bool SPLIT>
 __device__ __forceinline__ bool lazy_derived(const KParams &P)
 {
-    if constexpr (SPLIT) return false;
-    else return P.lazy_derived != 0 && P.adiabatic;
+    return P.lazy_derived != 0 && P.adiabatic;
 }
 
 // the CFL limit of cell (i, j) (cfl.cpp:235-376), vm_i the ring's mean v_phi
