@@ -68,21 +68,17 @@ __device__ __forceinline__ SharedDen shared_den(double d)
     const double r1 = fma(r0, e0, r0);
     const double e1 = fma(-d, r1, 1.0);
     const double r2 = fma(r1, e1, r1);
-    const double ad = fabs(d);
-    return SharedDen{d, r2, ad >= 0x1p-250 && ad < 0x1p250};
+    const unsigned ed = (unsigned)((unsigned long long)__double_as_longlong(d) >> 52) & 0x7ffu;
+    return SharedDen{d, r2, ed - 773u < 500u};
 }
 __device__ __forceinline__ double shared_div(double n, const SharedDen &D)
 {
     const double q0 = n * D.r;
     const double rem = fma(-D.d, q0, n);
     double q = fma(rem, D.r, q0);
-    const double an = fabs(n);
-    if (!(D.ok && an >= 0x1p-400 && an < 0x1p250)) {
-        // an empty volatile statement keeps the division on its own branch
-        // (taken by the lanes that need it), never computed for every lane
-        asm volatile("");
-        q = n / D.d;
-    }
+    const unsigned en = (unsigned)((unsigned long long)__double_as_longlong(n) >> 52) & 0x7ffu;
+    if (n == 0.0) q = q0;
+    else if (!D.ok || en - 623u >= 650u) q = n / D.d;
     return q;
 }
 
