@@ -55,8 +55,8 @@ __device__ __forceinline__ double vl_lim(double a, double b)
 // the scaling nor the fix-up changes anything, so the same reciprocal and
 // correction steps written out give the same bits, and the reciprocal part
 // (five instructions) is formed once per denominator: a quotient then costs
-// three.  Other operands (zeros, infinities and NaNs among them) take the
-// division itself; the range tests are two magnitude compares each.
+// three.  Other operands (zero aside, which returns the signed zero n * r)
+// take the division itself.
 struct SharedDen {
     double d, r;
     bool ok;
