@@ -330,9 +330,17 @@ __device__ __forceinline__ BatchEntry batch_entry(const BatchEntry *map)
     const auto *m = cst(map) + blockIdx.x;
     return BatchEntry{m->disk, m->yx};
 }
-__device__ __forceinline__ const KParams &batch_disk(const KParams *Ps, unsigned d)
+__device__ __forceinline__ KParams batch_disk(const KParams *Ps, unsigned d)
 {
-    return *(const KParams *)cst(Ps + d);
+    // a copy through the constant address space, word by word: the fields a
+    // kernel uses become scalar loads at its start, as kernel arguments do
+    static_assert(sizeof(KParams) % 8 == 0, "KParams is copied in 8-byte words");
+    KParams P;
+    const auto src = cst(reinterpret_cast<const unsigned long long *>(Ps + d));
+    unsigned long long *dst = reinterpret_cast<unsigned long long *>(&P);
+#pragma unroll
+    for (unsigned k = 0; k < sizeof(KParams) / 8; ++k) dst[k] = src[k];
+    return P;
 }
 
 #define FCPT_PAIR0(name, BOUNDS)                                                                  \
