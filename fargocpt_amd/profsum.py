@@ -68,6 +68,11 @@ def summarise(workload: str, prec: str, base: str, outdir: str) -> dict:
          "```\n"]
     rows = list(csv.DictReader(open(kt[0])))
     L += ["## Kernel time (kernel_stats.csv)\n",
+          f"Calls include the {TIMING_LAUNCHES} launches with which bench.py times each of the "
+          "radial sweep, azimuthal sweep, CFL rings, potential and the viscous walk without "
+          "SubStep3 (an adiabatic step runs the walk with SubStep3, so there this one shows 0 "
+          "launches per step); launches per "
+          f"step excludes them and divides by the {KT_STEPS} traced steps.\n",
           "| kernel | calls | launches per step | average µs | % of GPU time |",
           "|---|---|---|---|---|"]
     per_step = {}
