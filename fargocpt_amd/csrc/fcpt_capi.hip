@@ -1925,7 +1925,7 @@ int fcpt_batch_create(fcpt_ctx *const *disks, int32_t n_disks, fcpt_batch **out)
             }
         }
         KParams P = c->P;
-        P.radial_tile = radial_tile_for(P.nr, P.ns, P.radial_tile);
+        // the radial tile is chosen below, over the whole batch
         // the walks' tiles as the disk's own launches choose them
         P.kick_tile = resolve_kick_tile(P);
         P.visc_tile = resolve_visc_tile(P);
@@ -1936,6 +1936,25 @@ int fcpt_batch_create(fcpt_ctx *const *disks, int32_t n_disks, fcpt_batch **out)
         P.lazy_derived = walks ? 1 : 0;
         b->lazy.push_back(walks);
         host.push_back(P);
+    }
+    // The radial sweep's tile of the members that leave it to the size rule.
+    // A single disk's rule shrinks the tile until that disk alone fills the
+    // device with wavefronts; in a batch all members' blocks share one
+    // launch, so the tile is the largest that fills the device with the
+    // batch's wavefronts (fewer tiles, hence fewer window reloads at tile
+    // starts; the sweep's results do not depend on the tile).
+    {
+        auto waves = [&](int t) {
+            long w = 0;
+            for (const KParams &P : host)
+                w += (long)((P.ns + 63) / 64) * ((P.nr + (P.radial_tile > 0 ? P.radial_tile : t) - 1) /
+                                                  (P.radial_tile > 0 ? P.radial_tile : t));
+            return w;
+        };
+        int t = 32;
+        while (t > 2 && waves(t) < 3072) t /= 2;
+        for (KParams &P : host)
+            if (P.radial_tile <= 0) P.radial_tile = t;
     }
     hipError_t e = hipSetDevice(b->device);
     if (e != hipSuccess) {
