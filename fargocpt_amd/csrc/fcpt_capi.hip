@@ -1925,17 +1925,39 @@ int fcpt_batch_create(fcpt_ctx *const *disks, int32_t n_disks, fcpt_batch **out)
             }
         }
         KParams P = c->P;
-        // the radial tile is chosen below, over the whole batch
-        // the walks' tiles as the disk's own launches choose them
-        P.kick_tile = resolve_kick_tile(P);
-        P.visc_tile = resolve_visc_tile(P);
+        // the radial tile is chosen below, over the whole batch; in split
+        // storage the walks' tiles as the disk's own launches choose them
+        // (whether the walks run decides where split storage rounds)
+        if (P.split) {
+            P.kick_tile = resolve_kick_tile(P);
+            P.visc_tile = resolve_visc_tile(P);
+        }
+        host.push_back(P);
+    }
+    // FP64 storage: the walks' tiles of the members that leave them to the
+    // size rule, chosen over the batch's wavefronts as the radial tile below
+    // (a member too small to fill the device alone still runs the walks;
+    // FP64 results do not depend on the tiles or on whether the walks run)
+    if (!b->split) {
+        auto waves = [&](int t) {
+            long w = 0;
+            for (const KParams &P : host) w += walk_waves(P, P.visc_tile > 0 ? P.visc_tile : t);
+            return w;
+        };
+        int t = 8;
+        while (t > 2 && waves(t) < 3072) t /= 2;
+        for (KParams &P : host) {
+            if (P.kick_tile <= 0) P.kick_tile = t;
+            if (P.visc_tile <= 0) P.visc_tile = t;
+        }
+    }
+    for (KParams &P : host) {
         // the batch runs the fused step: c_s, nu and T formed where needed
         // as in the disk's own fused step
         const bool walks = P.adiabatic && kick_walk_applies(P);
         P.e_partner = walks && P.split ? 1 : 0;
         P.lazy_derived = walks ? 1 : 0;
         b->lazy.push_back(walks);
-        host.push_back(P);
     }
     // The radial sweep's tile of the members that leave it to the size rule.
     // A single disk's rule shrinks the tile until that disk alone fills the

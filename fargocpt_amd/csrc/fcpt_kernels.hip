@@ -3482,6 +3482,11 @@ static int kick_tile(const KParams &P)
 
 int resolve_kick_tile(const KParams &P) { return kick_tile(P); }
 int resolve_visc_tile(const KParams &P) { return visc_tile(P); }
+long walk_waves(const KParams &P, int tile)
+{
+    const long rings = (P.nr + tile - 1) / tile;
+    return rings * ((P.ns + kKickSegW - 1) / kKickSegW + (P.ns + kViscSegW - 1) / kViscSegW) / 2;
+}
 
 bool kick_walk_applies(const KParams &P)
 {

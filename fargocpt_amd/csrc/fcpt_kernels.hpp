@@ -59,6 +59,9 @@ bool kick_walk_applies(const KParams &P);
 // the rings per thread the walks use for this disk (1: the cell-parallel form)
 int resolve_kick_tile(const KParams &P);
 int resolve_visc_tile(const KParams &P);
+// wavefronts of one kick-walk or viscous-walk launch of a disk at `tile`
+// rings per thread (the mean of the two walks' segment counts)
+long walk_waves(const KParams &P, int tile);
 void launch_kick(const KParams &P, hipStream_t st, int fold_mode);
 // the potential; fold_mode >= 0: in the same launch, the CFL fold of
 // k_cfl_rings' slots (launch_cfl_cells) with that mode
