@@ -131,9 +131,10 @@ def timed_run(name: str, prec: int, steps: int, warmup: int, use_graph: bool, tu
     return disk, wall, ms_events, warmup + 1 + steps
 
 
-def batch_run(name: str, prec: int, steps: int, warmup: int, use_graph: bool):
+def batch_run(name: str, prec: int, steps: int, warmup: int, use_graph: bool, tune: bool = False):
     """The batched sweep: every member disk started, then `steps` batched
-    steps timed after `warmup` untimed ones."""
+    steps timed after `warmup` untimed ones (and, with `tune`, the batch's
+    radial-sweep autotuner)."""
     from fargocpt_amd import workloads as W
     from fargocpt_amd.capi import Batch, Disk
     members = W.batch_members(name)
@@ -144,6 +145,9 @@ def batch_run(name: str, prec: int, steps: int, warmup: int, use_graph: bool):
         disks.append(d)
     batch = Batch(disks)
     batch.run_steps(warmup, use_graph)
+    batch.synchronize()
+    batch.shapes = batch.autotune(5) if tune else None
+    batch.run_steps(1, use_graph)  # captures the graph of the tuned plan
     batch.synchronize()
     t0 = time.perf_counter()
     ms_events = batch.time_steps(steps, use_graph)
@@ -158,12 +162,16 @@ def bench_batch(args) -> int:
     name = args.workload
     prec = PRECISIONS[args.precision]
     use_graph = not args.no_graph
-    batch, disks, cells, wall, ms_events = batch_run(name, prec, args.steps, args.warmup, use_graph)
+    tune = not args.no_autotune
+    batch, disks, cells, wall, ms_events = batch_run(name, prec, args.steps, args.warmup, use_graph, tune)
     value = cells * args.steps / wall
     el = ELEMENT[prec]
     kt = {k: batch.time_kernel(v, 10) for k, v in BATCH_KERNELS.items()}
     dom = max(kt, key=kt.get)
-    achieved = W.algorithmic_bytes(dom, False, el) * cells / (kt[dom] * 1e-3) / 1e9
+    # the step's algorithmic bytes per cell (evolved fields read + written
+    # once), as for the single-disk line; the kernel's own field count apart
+    achieved = W.step_bytes(False, el) * cells / (kt[dom] * 1e-3) / 1e9
+    kernel_bytes = W.algorithmic_bytes(dom, False, el)
     n = len(disks)
     out = {
         "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1,
@@ -175,10 +183,13 @@ def bench_batch(args) -> int:
                                   "q ~ logU[1e-5,1e-3], alpha ~ logU[1e-4,1e-2], h ~ U[0.03,0.07], "
                                   "one launch per stage over a work table of all disks' blocks",
                    "storage": "f64" if prec == 0 else "f32 deviations on f64 ring backgrounds",
-                   "graph": use_graph},
+                   "graph": use_graph, "launch_shapes": batch.shapes},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "bytes_per_cell": W.algorithmic_bytes(dom, False, el), "ms_per_launch": kt[dom]},
+                     "bytes_per_cell": W.step_bytes(False, el), "ms_per_launch": kt[dom],
+                     "bytes_basis": "evolved fields read + written once per cell-update",
+                     "kernel_bytes_per_cell": kernel_bytes,
+                     "kernel_achieved_GBs": kernel_bytes * cells / (kt[dom] * 1e-3) / 1e9},
         "step": {"ms_per_step_events": ms_events / args.steps,
                  "algorithmic_GBs": W.step_bytes(False, el) * value / 1e9,
                  "frac_of_hbm_peak": W.step_bytes(False, el) * value / 1e9 / HBM_PEAK_GBS,
@@ -186,7 +197,7 @@ def bench_batch(args) -> int:
     }
     if not args.no_compare:
         other = 1 - prec
-        b2, d2, _, wall2, _ = batch_run(name, other, args.steps, args.warmup, use_graph)
+        b2, d2, _, wall2, _ = batch_run(name, other, args.steps, args.warmup, use_graph, tune)
         value2 = cells * args.steps / wall2
         fp64, split = (disks, d2) if prec == 0 else (d2, disks)
         worst = {f: max(rel_err(s.download_field(f), d.download_field(f)) for s, d in zip(split, fp64))
@@ -195,7 +206,7 @@ def bench_batch(args) -> int:
                           "ms_per_step": 1e3 * wall2 / args.steps,
                           "split_speedup_vs_fp64": (value2 / value) if other == 1 else (value / value2),
                           "split_rel_err_vs_fp64_worst_disk": worst,
-                          "after_steps": args.warmup + args.steps}
+                          "after_steps": args.warmup + 1 + args.steps}
         b2.close()
     print(json.dumps(out), flush=True)
     return 0

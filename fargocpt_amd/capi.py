@@ -228,6 +228,7 @@ def load_library(path: str = LIB_PATH):
         "fcpt_batch_destroy": [P],
         "fcpt_batch_synchronize": [P],
         "fcpt_batch_set_radial_waves": [P, I],
+        "fcpt_batch_autotune": [P, I, C.POINTER(I), C.POINTER(I)],
         "fcpt_batch_run_steps": [P, I, I],
         "fcpt_batch_time_steps": [P, I, I, dp],
         "fcpt_batch_time_kernel": [P, I, I, dp],
@@ -566,6 +567,14 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+    def autotune(self, reps=10):
+        """Time the batched radial sweep's tiles and register budgets on the
+        current state and keep the fastest; returns {"radial_tile": t,
+        "radial_waves": w}."""
+        t, w = C.c_int32(), C.c_int32()
+        self._check(self.lib.fcpt_batch_autotune(self.h, reps, C.byref(t), C.byref(w)))
+        return {"radial_tile": t.value, "radial_waves": w.value}
 
     def set_radial_waves(self, waves):
         self._check(self.lib.fcpt_batch_set_radial_waves(self.h, waves))

@@ -1867,12 +1867,16 @@ struct fcpt_batch {
     std::vector<bool> lazy;  // the disk's batched step forms c_s, nu and T where needed
     KParams *Ps = nullptr;  // device array: every disk's parameters, then their swapped views
     std::vector<BatchStageDev> stages;
-    std::vector<void *> allocs;
     hipStream_t stream = nullptr;
     hipGraphExec_t graph = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipEvent_t done = nullptr;  // end of the last batched run; every disk's stream waits for it
     std::string err;
+    // the parameters the batched kernels see, and which members' radial tile
+    // the batch chooses (left to the size rule by the disk)
+    std::vector<KParams> host;
+    std::vector<bool> auto_radial;
+    std::vector<void *> plan_allocs;  // Ps and the stage tables of the current plan
 };
 
 namespace {
@@ -1899,6 +1903,41 @@ void enqueue_batch_step(const fcpt_batch *b, hipStream_t st)
 extern "C" {
 
 const char *fcpt_batch_last_error(const fcpt_batch *b) { return b ? b->err.c_str() : "null batch"; }
+
+// the device side of the batch's plan from b->host: every disk's parameters
+// (disk d at d, its swapped view for the fused passes at n + d) and the work
+// table of every stage; replaces the previous plan's arrays
+static int upload_batch_plan(fcpt_batch *b)
+{
+    for (void *p : b->plan_allocs) hipFree(p);
+    b->plan_allocs.clear();
+    b->stages.clear();
+    b->Ps = nullptr;
+    auto alloc = [&](size_t bytes) -> void * {
+        void *q = nullptr;
+        if (hipMalloc(&q, bytes ? bytes : 16) != hipSuccess) return nullptr;
+        b->plan_allocs.push_back(q);
+        return q;
+    };
+    std::vector<KParams> views(b->host);
+    for (const KParams &P : b->host) views.push_back(swapped_view(P));
+    b->Ps = static_cast<KParams *>(alloc(views.size() * sizeof(KParams)));
+    if (!b->Ps || hipMemcpy(b->Ps, views.data(), views.size() * sizeof(KParams), hipMemcpyHostToDevice) != hipSuccess)
+        return bfail(b, FCPT_ERR_DEVICE, "batch parameters upload");
+    for (const BatchStageHost &h : plan_batch_step(b->host)) {
+        void *m = alloc(h.entries.size() * sizeof(BatchEntry));
+        if (!m || hipMemcpy(m, h.entries.data(), h.entries.size() * sizeof(BatchEntry), hipMemcpyHostToDevice) !=
+                      hipSuccess)
+            return bfail(b, FCPT_ERR_DEVICE, "batch work table upload");
+        b->stages.push_back(
+            BatchStageDev{h.kind, h.arg, static_cast<const BatchEntry *>(m), (unsigned)h.entries.size()});
+    }
+    if (b->graph) {
+        hipGraphExecDestroy(b->graph);
+        b->graph = nullptr;
+    }
+    return FCPT_OK;
+}
 
 int fcpt_batch_create(fcpt_ctx *const *disks, int32_t n_disks, fcpt_batch **out)
 {
@@ -1965,6 +2004,7 @@ int fcpt_batch_create(fcpt_ctx *const *disks, int32_t n_disks, fcpt_batch **out)
     // launch, so the tile is the largest that fills the device with the
     // batch's wavefronts (fewer tiles, hence fewer window reloads at tile
     // starts; the sweep's results do not depend on the tile).
+    for (const KParams &P : host) b->auto_radial.push_back(P.radial_tile <= 0);
     {
         auto waves = [&](int t) {
             long w = 0;
@@ -1978,36 +2018,9 @@ int fcpt_batch_create(fcpt_ctx *const *disks, int32_t n_disks, fcpt_batch **out)
         for (KParams &P : host)
             if (P.radial_tile <= 0) P.radial_tile = t;
     }
-    hipError_t e = hipSetDevice(b->device);
-    if (e != hipSuccess) {
-        delete b;
-        return FCPT_ERR_DEVICE;
-    }
-    auto alloc = [&](size_t bytes) -> void * {
-        void *q = nullptr;
-        if (hipMalloc(&q, bytes ? bytes : 16) != hipSuccess) return nullptr;
-        b->allocs.push_back(q);
-        return q;
-    };
-    // disk d's parameters at d, its swapped view (fused passes) at n + d
-    std::vector<KParams> views(host);
-    for (const KParams &P : host) views.push_back(swapped_view(P));
-    b->Ps = static_cast<KParams *>(alloc(views.size() * sizeof(KParams)));
-    int rc = b->Ps ? FCPT_OK : FCPT_ERR_DEVICE;
-    if (!rc && hipMemcpy(b->Ps, views.data(), views.size() * sizeof(KParams), hipMemcpyHostToDevice) != hipSuccess)
-        rc = FCPT_ERR_DEVICE;
-    if (!rc) {
-        for (const BatchStageHost &h : plan_batch_step(host)) {
-            void *m = alloc(h.entries.size() * sizeof(BatchEntry));
-            if (!m || hipMemcpy(m, h.entries.data(), h.entries.size() * sizeof(BatchEntry),
-                                hipMemcpyHostToDevice) != hipSuccess) {
-                rc = FCPT_ERR_DEVICE;
-                break;
-            }
-            b->stages.push_back(BatchStageDev{h.kind, h.arg, static_cast<const BatchEntry *>(m),
-                                              (unsigned)h.entries.size()});
-        }
-    }
+    b->host = host;
+    int rc = hipSetDevice(b->device) == hipSuccess ? FCPT_OK : FCPT_ERR_DEVICE;
+    if (!rc) rc = upload_batch_plan(b);
     if (!rc && hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) rc = FCPT_ERR_DEVICE;
     if (!rc && hipEventCreateWithFlags(&b->done, hipEventDisableTiming) != hipSuccess) rc = FCPT_ERR_DEVICE;
     if (rc) {
@@ -2030,7 +2043,7 @@ int fcpt_batch_destroy(fcpt_batch *b)
     if (b->ev0) hipEventDestroy(b->ev0);
     if (b->ev1) hipEventDestroy(b->ev1);
     if (b->done) hipEventDestroy(b->done);
-    for (void *p : b->allocs) hipFree(p);
+    for (void *p : b->plan_allocs) hipFree(p);
     delete b;
     return FCPT_OK;
 }
@@ -2139,6 +2152,49 @@ int fcpt_batch_time_kernel(fcpt_batch *b, int32_t kernel, int32_t reps, double *
     float ms = 0.f;
     BHIP(b, hipEventElapsedTime(&ms, b->ev0, b->ev1));
     if (ms_per_launch) *ms_per_launch = (double)ms / reps;
+    return FCPT_OK;
+}
+
+// The batch's radial tile (for the members that leave it to the size rule)
+// and register budget by the radial stage's time: each candidate's plan is
+// uploaded and its radial stage timed over `reps` launches, which write the
+// transport's scratch arrays only, so the disks' state is untouched.
+int fcpt_batch_autotune(fcpt_batch *b, int32_t reps, int32_t *radial_tile, int32_t *radial_waves)
+{
+    if (!b) return FCPT_ERR_ARG;
+    if (reps <= 0) return bfail(b, FCPT_ERR_ARG, "reps must be positive");
+    BHIP(b, hipSetDevice(b->device));
+    BHIP(b, hipStreamSynchronize(b->stream));
+    int cur_t = 0;
+    for (size_t k = 0; k < b->host.size(); ++k)
+        if (b->auto_radial[k]) cur_t = b->host[k].radial_tile;
+    int best_t = cur_t, best_w = b->radial_waves;
+    double best = 1e300;
+    const int tiles[] = {4, 8, 16, 32};
+    for (int w : {3, 2}) {
+        for (int t : tiles) {
+            if (cur_t == 0 && t != tiles[0]) break;  // no member's tile is the batch's to choose
+            for (size_t k = 0; k < b->host.size(); ++k)
+                if (b->auto_radial[k]) b->host[k].radial_tile = t;
+            int rc = upload_batch_plan(b);
+            b->radial_waves = w;
+            double ms = 0.0;
+            if (!rc) rc = fcpt_batch_time_kernel(b, FCPT_KERNEL_RADIAL_SWEEP, reps, &ms);
+            if (rc) return rc;
+            if (ms < best) {
+                best = ms;
+                best_t = t;
+                best_w = w;
+            }
+        }
+    }
+    for (size_t k = 0; k < b->host.size(); ++k)
+        if (b->auto_radial[k]) b->host[k].radial_tile = best_t;
+    b->radial_waves = best_w;
+    const int rc = upload_batch_plan(b);
+    if (rc) return rc;
+    if (radial_tile) *radial_tile = best_t;
+    if (radial_waves) *radial_waves = best_w;
     return FCPT_OK;
 }
 

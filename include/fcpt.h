@@ -375,6 +375,13 @@ const char *fcpt_batch_last_error(const fcpt_batch *batch);
 int fcpt_batch_synchronize(fcpt_batch *batch);
 /* radial-sweep register budget for the whole batch (2 or 3 wavefronts per SIMD) */
 int fcpt_batch_set_radial_waves(fcpt_batch *batch, int32_t waves);
+/* Autotuner (no counterpart in the reference): the radial sweep's tile of the
+   members that left it to the size rule (one tile for all of them, 4 .. 32
+   rings) and the batch's register budget, by the time of the batched radial
+   stage over `reps` launches on the current state (the timed launches write
+   the transport's scratch arrays only).  Keeps and returns the fastest;
+   drops a captured batch graph.  Results do not depend on the choice. */
+int fcpt_batch_autotune(fcpt_batch *batch, int32_t reps, int32_t *radial_tile, int32_t *radial_waves);
 /* n_steps sim::run iterations of every disk; use_graph != 0 replays one
    captured HIP graph per iteration.  No host synchronisation. */
 int fcpt_batch_run_steps(fcpt_batch *batch, int32_t n_steps, int32_t use_graph);

@@ -102,3 +102,30 @@ def test_batch_members_uneven_sizes():
         m0, _ = s.totals(1e300)
         assert np.isfinite(m0) and m0 > 0
     b.close()
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_batch_autotune_keeps_results(prec):
+    """The batch's radial-sweep autotuner changes launch shapes only: after
+    it, every disk still evolves bit for bit as alone."""
+    from fargocpt_amd.capi import Batch
+    print(f"batch autotune, precision {prec}", flush=True)
+    single = _fixture_disks(prec)
+    batched = _fixture_disks(prec)
+    for d, _ in single:
+        d.run_steps(8, True)
+        d.synchronize()
+    b = Batch([d for d, _ in batched])
+    b.run_steps(3, True)
+    b.synchronize()
+    shapes = b.autotune(2)
+    assert shapes["radial_tile"] in (4, 8, 16, 32) and shapes["radial_waves"] in (2, 3)
+    b.run_steps(5, True)
+    b.synchronize()
+    for (s, meta), (d, _) in zip(single, batched):
+        for f in FIELDS:
+            if f in ("ENERGY", "QPLUS") and not meta["adiabatic"]:
+                continue
+            assert np.array_equal(s.download_field(f), d.download_field(f)), (meta["Nr"], f)
+        assert s.get_time() == d.get_time()
+    b.close()
