@@ -245,9 +245,9 @@ def main() -> int:
     ap.add_argument("--skip-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: every GPU box of this build has one MI355X and the "
-              "whole disk lives on it; runs on more than one GPU are not part of this build "
-              "(the sharding design is written up, not built)", file=sys.stderr)
+        print("bench.py measures one GPU: the whole disk lives on one MI355X, and runs on "
+              "more than one GPU are not implemented (the sharding design is written up, "
+              "not built)", file=sys.stderr)
         return 2
 
     from fargocpt_amd import workloads as W

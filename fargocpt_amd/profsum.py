@@ -135,7 +135,7 @@ def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("workload")
     ap.add_argument("precision", choices=("fp64", "split"))
-    ap.add_argument("base", nargs="?", default="gpurun_out")
+    ap.add_argument("base", help="folder holding the rocprofv3 output folders")
     ap.add_argument("--out", default="profiles")
     a = ap.parse_args()
     print(json.dumps(summarise(a.workload, a.precision, a.base, a.out)))

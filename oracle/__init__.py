@@ -1,7 +1,8 @@
 """ctypes binding of the CPU oracle (liboracle.so) -- TEST INFRASTRUCTURE ONLY.
 
 Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg may use
-this module, as the checker or the timed CPU baseline.  It exposes the same
+this module, as the reference the GPU results are compared with or as the
+timed CPU baseline.  It exposes the same
 method names as fargocpt_amd.capi.Disk so tests can drive both the same way.
 """
 from __future__ import annotations
