@@ -198,6 +198,20 @@ struct Fld {
     // early and pay for it where the value is used
     __device__ __forceinline__ T raw(int i, int j) const { return p[(size_t)i * (size_t)ns + (size_t)j]; }
     __device__ __forceinline__ void set_raw(int i, int j, T x) const { p[(size_t)i * (size_t)ns + (size_t)j] = x; }
+    // store v at (i, j) and return it as a later load of it returns it
+    __device__ __forceinline__ double set_get(int i, int j, double v) const
+    {
+        const size_t c = (size_t)i * (size_t)ns + (size_t)j;
+        if constexpr (SPLIT) {
+            const double b = cst(bg)[i];
+            const float x = round_dev(v - b, key, c);
+            p[c] = x;
+            return b + (double)x;
+        } else {
+            p[c] = v;
+            return v;
+        }
+    }
     // v as a later load of the value stored at (i, j) returns it
     __device__ __forceinline__ double stored(int i, int j, double v) const
     {
@@ -3032,8 +3046,9 @@ __device__ __forceinline__ void b_velocities_final(const KParams &P, int bx, int
         vel_final_sig_e<SPLIT>(P, i, j, dt, sr, er, false);
     }
     // stores: the ghost rings, v_r(1) of the ring-1 thread, or the cell's own
-    S.set(i, j, sr);
-    if (ghost || P.adiabatic) E.set(i, j, er);
+    const double s_stored = S.set_get(i, j, sr);
+    double e_stored = 0.0;
+    if (ghost || P.adiabatic) e_stored = E.set_get(i, j, er);
     vaz.set(i, j, vphi);
     if (i == 0) {
         vr.set(0, j, va);
@@ -3044,7 +3059,7 @@ __device__ __forceinline__ void b_velocities_final(const KParams &P, int bx, int
         vr.set(i, j, vb);  // v_r(1): the ghost value
     }
     if (advance_time && i == 0 && j == 0) advance_clock(P, advance_time);
-    DerivedIn d{P.adiabatic ? E.stored(i, j, er) : 0.0, S.stored(i, j, sr), 0.0, 0.0};
+    DerivedIn d{P.adiabatic ? e_stored : 0.0, s_stored, 0.0, 0.0};
     if (!P.adiabatic) {
         d.cs = cst(P.iso_cs)[i];
         d.h = cst(P.iso_h)[i];
