@@ -1676,10 +1676,7 @@ __device__ __forceinline__ void b_av_fused(const KParams &P, int bx, int by, int
             qpm = m.qp;
         }
     }
-    if (diss) {
-        E.set(i, j, e);
-        e = E.stored(i, j, e);
-    }
+    if (diss) e = E.set_get(i, j, e);
     if (visc) derived_store<SPLIT>(P, i, j, DerivedIn{e, sig, 0.0, 0.0}, false, false);
     if (inner)
         vaz_o.set(i, j, sn ? sn_vphi_new(P, i, dt, va0, sig, sjp, qp0, qpj)
@@ -1824,8 +1821,7 @@ __device__ __forceinline__ void b_kick_walk(const KParams &P, int bx, int by, in
                 if (i < nr - 1) e = E.stored(i, j, src_energy_new(P, i, dt, e, v1r0, v1r1, v1a0, v1aR));
                 if (diss) e = sn ? sn_dissipation(P, i, q.sn, R0.s, dt, e) : tw_dissipation(P, i, q.tw, R0.s, dt, e);
                 if (i < nr - 1 || diss) {
-                    Eo.set(i, j, e);
-                    e = E.stored(i, j, e);
+                    e = Eo.set_get(i, j, e);  // the partner has E's background and key
                 } else if (P.e_partner) {
                     Eo.set_raw(i, j, E.raw(i, j));
                 }
