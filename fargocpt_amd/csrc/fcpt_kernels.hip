@@ -82,6 +82,30 @@ __device__ __forceinline__ double shared_div(double n, const SharedDen &D)
     return q;
 }
 
+// IEEE FP64 square root.  The compiler's sqrt on gfx950 scales x by 2^256
+// below 2^-767, refines v_rsq by one Goldschmidt step and two Newton
+// corrections, scales back, and passes zeros and +inf through a class test.
+// For x in [2^-767, inf) the scalings are by 2^0 and the class test passes
+// the refined value, so the refinement written out gives the same bits;
+// other operands take sqrt itself.
+__device__ __forceinline__ double sqrt_ieee(double x)
+{
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = y * 0.5;
+    const double r = fma(-h, g, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    double e = fma(-g, g, x);
+    g = fma(e, h, g);
+    e = fma(-g, g, x);
+    g = fma(e, h, g);
+    if (!(x >= 0x1p-767 && x < __builtin_inf())) {
+        __builtin_amdgcn_s_sleep(0);  // a branch: not both roots formed and selected
+        g = sqrt(x);
+    }
+    return g;
+}
+
 // FP64 reciprocal: one IEEE division, or in split storage the hardware
 // reciprocal refined by two Newton steps (error a few ulp)
 template <bool SPLIT>
@@ -852,14 +876,14 @@ __device__ __forceinline__ double potential_body(const KParams &P, double x, dou
     const double d2 = dist_2 + smooth * smooth;
     // split storage: one reciprocal square root instead of sqrt and division
     const double inv_d = SPLIT ? inv_sqrt<true>(d2) : 0.0;
-    const double d_smoothed = SPLIT ? d2 * inv_d : sqrt(d2);
+    const double d_smoothed = SPLIT ? d2 * inv_d : sqrt_ieee(d2);
     double smooth_factor_klahr = 1.0;
     if (r_sm > 0.0 && d_smoothed < r_sm) {
         const double q = d_smoothed / r_sm;
         smooth_factor_klahr = (pow4_comp(q) - 2.0 * pow3_comp(q) + 2.0 * d_smoothed / r_sm);
     }
     if constexpr (SPLIT) return -P.G * mass * inv_d * smooth_factor_klahr;
-    else return -P.G * mass / d_smoothed * smooth_factor_klahr;
+    else return shared_div(-P.G * mass, shared_den(d_smoothed)) * smooth_factor_klahr;
 }
 
 // CalculateNbodyPotential, Pframeforce.cpp:44-85 (+ compute_smoothing,
@@ -885,16 +909,18 @@ __device__ __forceinline__ void potential_ring_cells(const KParams &P, int i, in
         sm[c] = P.thickness_smoothing * H;
         acc[c] = 0.0;
     }
-    const int np = s->n_planets;
+    // the bodies through scalar loads (this launch writes none of them)
+    const auto bodies = cst(s->planet);
+    const int np = *cst(&s->n_planets);
     for (int k = 0; k < np; ++k) {
-        const double px = s->planet[k].x, py = s->planet[k].y;
-        const double mass = s->planet[k].mass, r_sm = s->planet[k].r_cubic;
-        const double smk = s->planet[k].smooth;
+        const double px = bodies[k].x, py = bodies[k].y;
+        const double mass = bodies[k].mass, r_sm = bodies[k].r_cubic;
+        const double smk = bodies[k].smooth;
 #pragma unroll
         for (int c = 0; c < kPotCells; ++c)
             acc[c] += potential_body<SPLIT>(P, x[c], y[c], smk >= 0.0 ? smk : sm[c], px, py, mass, r_sm);
     }
-    const double ix = s->indirect_x, iy = s->indirect_y;
+    const double ix = *cst(&s->indirect_x), iy = *cst(&s->indirect_y);
 #pragma unroll
     for (int c = 0; c < kPotCells; ++c) {
         const int j = j0 + 64 * c;
