@@ -566,11 +566,13 @@ __device__ __forceinline__ void b_cfl_finalize(const KParams &P, int mode) { b_c
 struct AdiDerived {
     double cs, h, nu;
 };
+template <bool SPLIT>
 __device__ __forceinline__ AdiDerived adi_derived(const KParams &P, int i, double e, double sig)
 {
     AdiDerived d;
     const double g = P.gamma;
-    d.cs = sqrt(g * (g - 1.0) * e / sig);
+    const double cs2 = g * (g - 1.0) * e / sig;
+    d.cs = SPLIT ? sqrt(cs2) : sqrt_ieee(cs2);  // the same bits (the split walks run faster with sqrt)
     const double inv_omega_kepler = ring_tab(P, TAB_INV_OMEGA_K, i);  // 1 / Omega_K(Rmed)
     d.h = d.cs / (sqrt(g)) * inv_omega_kepler;
     const double alpha = P.alpha;
@@ -613,7 +615,7 @@ __device__ __forceinline__ double cfl_cell(const KParams &P, int i, int j, doubl
         nuv = cst(P.iso_nu)[i];
     } else if (lazy_derived<SPLIT>(P)) {
         // as the end of the last step formed them from the same E and Sigma
-        const AdiDerived a = adi_derived(P, i, fld<SPLIT>(P, FCPT_ENERGY)(i, j), fld<SPLIT>(P, FCPT_SIGMA)(i, j));
+        const AdiDerived a = adi_derived<SPLIT>(P, i, fld<SPLIT>(P, FCPT_ENERGY)(i, j), fld<SPLIT>(P, FCPT_SIGMA)(i, j));
         csv = a.cs;
         nuv = a.nu;
     } else {
@@ -1043,13 +1045,18 @@ __device__ __forceinline__ void b_src_energy(const KParams &P, int bx, int by)
 }
 FCPT_PAIR0(src_energy, 256)
 
+// the energy of temperature t_min, t_min sigma / mu R / (gamma - 1)
+__device__ __forceinline__ double energy_floor(const KParams &P, double sigma)
+{
+    return P.t_min * sigma / P.mu * P.Rgas / (P.gamma - 1.0);
+}
 // assure_temperature_range for one cell, SourceEuler.cpp:136-202
 __device__ __forceinline__ double temperature_floor(const KParams &P, double e, double sigma)
 {
-    const double minimum_energy = P.t_min * sigma / P.mu * P.Rgas / (P.gamma - 1.0);
+    const double minimum_energy = energy_floor(P, sigma);
     const double maximum_energy = P.t_max * sigma / P.mu * P.Rgas / (P.gamma - 1.0);
-    if (!(e > minimum_energy)) e = P.t_min * sigma / P.mu * P.Rgas / (P.gamma - 1.0);
-    if (!(e < maximum_energy)) e = P.t_max * sigma / P.mu * P.Rgas / (P.gamma - 1.0);
+    if (!(e > minimum_energy)) e = minimum_energy;
+    if (!(e < maximum_energy)) e = maximum_energy;
     return e;
 }
 
@@ -1105,7 +1112,7 @@ __device__ __forceinline__ void derived_store(const KParams &P, int i, int j, co
     if (P.e_partner) with_pressure = false;
     if (lazy_derived<SPLIT>(P)) {
         // the field the step reads stored: H (the potential's smoothing)
-        const AdiDerived a = adi_derived(P, i, d.e, d.sig);
+        const AdiDerived a = adi_derived<SPLIT>(P, i, d.e, d.sig);
         fld<SPLIT>(P, FCPT_SCALE_HEIGHT).set(i, j, a.h);
         if (with_pressure) fld<SPLIT>(P, FCPT_PRESSURE).set(i, j, (P.gamma - 1.0) * d.e);
         return;
@@ -1117,7 +1124,7 @@ __device__ __forceinline__ void derived_store(const KParams &P, int i, int j, co
             const double c_v_inv = P.mu / P.Rgas * (P.gamma - 1.0);
             fld<SPLIT>(P, FCPT_TEMPERATURE).set(i, j, c_v_inv * e / sig);
         }
-        const AdiDerived a = adi_derived(P, i, e, sig);
+        const AdiDerived a = adi_derived<SPLIT>(P, i, e, sig);
         c_s = a.cs;
         h = a.h;
         fld<SPLIT>(P, FCPT_SOUNDSPEED).set(i, j, c_s);
@@ -1852,7 +1859,7 @@ __device__ __forceinline__ void b_kick_walk(const KParams &P, int bx, int by, in
                     Eo.set_raw(i, j, E.raw(i, j));
                 }
                 // recalculate_viscosity
-                const AdiDerived d = adi_derived(P, i, e, R0.s);
+                const AdiDerived d = adi_derived<SPLIT>(P, i, e, R0.s);
                 if (!lazy_derived<SPLIT>(P)) {
                     fld<SPLIT>(P, FCPT_SOUNDSPEED).set(i, j, d.cs);
                     fld<SPLIT>(P, FCPT_SCALE_HEIGHT).set(i, j, d.h);
@@ -2012,7 +2019,7 @@ __device__ __forceinline__ double cooling_beta_cell(const KParams &P, int i, int
     if (P.cooling_beta_ref == FCPT_BETA_REF_REFERENCE)
         delta_E -= fld<SPLIT>(P, FCPT_ENERGY0)(i, j) / fld<SPLIT>(P, FCPT_SIGMA0)(i, j) * sig;
     if (P.cooling_beta_ref == FCPT_BETA_REF_FLOOR)
-        delta_E -= P.t_min * sig / P.mu * P.Rgas / (P.gamma - 1.0);
+        delta_E -= energy_floor(P, sig);
     return 0.0 + delta_E * omega_k * beta_inv;
 }
 
@@ -2070,8 +2077,14 @@ __device__ __forceinline__ void substep3_cell(const KParams &P, int i, int j, do
     const double temperature = c_v_inv * e / sig;
     if (i >= 1 && i < P.nr - 1) {
         const double alpha = rad_alpha(P, h, sig, e);
-        qplus /= alpha;
-        qminus /= alpha;
+        if constexpr (SPLIT) {
+            qplus /= alpha;
+            qminus /= alpha;
+        } else {  // the same bits through one reciprocal
+            const SharedDen A = shared_den(alpha);
+            qplus = shared_div(qplus, A);
+            qminus = shared_div(qminus, A);
+        }
         double energy_new = e + dt * (qplus - qminus);
         const double SigmaFloor = 10.0 * P.sigma0 * P.sigma_floor;
         if ((sig < SigmaFloor)) {
@@ -2244,7 +2257,7 @@ __device__ __forceinline__ void b_visc_walk(const KParams &P, int bx, int by, in
                 const double e = P.e_partner ? buf<SPLIT>(P, P.ealt, P.bg[FCPT_ENERGY])(i, j)
                                              : fld<SPLIT>(P, FCPT_ENERGY)(i, j);
                 // H as the kick walk formed it from the same E and Sigma
-                const double h = lazy_derived<SPLIT>(P) ? adi_derived(P, i, e, A.s).h
+                const double h = lazy_derived<SPLIT>(P) ? adi_derived<SPLIT>(P, i, e, A.s).h
                                                          : fld<SPLIT>(P, FCPT_SCALE_HEIGHT)(i, j);
                 double qplus = 0.0, qminus = 0.0;
                 if (upd_p) {
