@@ -131,13 +131,90 @@ def summarise(workload: str, prec: str, base: str, outdir: str) -> dict:
     return tj
 
 
+# instruction counters of the issue summary: two rocprofv3 passes (at most 8
+# SQ counters each)
+ISSUE_PASSES = {
+    "issue": ("SQ_WAVES", "SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_INSTS_VMEM_RD", "SQ_INSTS_VMEM_WR",
+              "SQ_ACTIVE_INST_VALU", "SQ_WAVE_CYCLES", "SQ_BUSY_CYCLES"),
+    "mix": ("SQ_INSTS_VALU_ADD_F64", "SQ_INSTS_VALU_MUL_F64", "SQ_INSTS_VALU_FMA_F64",
+            "SQ_INSTS_VALU_TRANS_F64", "SQ_INSTS_VALU_INT32", "SQ_INSTS_VALU_CVT", "SQ_INSTS_VALU_INT64",
+            "SQ_INSTS_SMEM"),
+}
+STEP_KERNELS = ("k_azimuthal_sweep", "k_radial_sweep3", "k_visc_walk<(true|false), true>", "k_velocities_final",
+                "k_kick_walk", "k_cfl_rings", "k_potential_fold", "k_ghost_ring_mean")
+SIMDS, CLOCK_HZ = 1024, 2.4e9  # MI355X: 256 CUs x 4 SIMDs, 2.4 GHz peak engine clock
+
+
+def _counters(files) -> dict:
+    agg = collections.defaultdict(lambda: collections.defaultdict(list))
+    for f in files:
+        for r in csv.DictReader(open(f)):
+            agg[_short(r["Kernel_Name"])][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    return {k: {c: sum(v) / len(v) for c, v in d.items()} for k, d in agg.items()}
+
+
+def summarise_issue(workload: str, base: str, outdir: str) -> None:
+    """VALU issue of the step kernels against their measured time, both storages.
+
+    Expects DIR/prof_W_P_issue and DIR/prof_W_P_mix (--pmc runs of
+    ISSUE_PASSES, 3 + 1 steps) and DIR/prof_W_P_kt (the kernel trace of
+    summarise) for P in fp64, split.  The issue time is the VALU instructions
+    at one wavefront instruction per 4 cycles per SIMD over all SIMDs at the
+    peak clock: the time the kernel would take if VALU issue were all it did.
+    """
+    cmd = f"python3 bench.py --workload {workload} --precision P --steps 3 --warmup 1 " \
+          "--skip-cpu --no-compare --no-autotune"
+    L = [f"# Instruction issue of the step kernels: {workload}\n",
+         "Commands (repository root, one MI355X; P = fp64, then split):\n", "```"]
+    for name, ctrs in ISSUE_PASSES.items():
+        L.append(f"rocprofv3 --pmc {' '.join(ctrs)} --output-format csv -d {base}/prof_{workload}_P_{name} "
+                 f"-o p -- {cmd}")
+    L += [f"python3 -m fargocpt_amd.profsum {workload} issue {base}", "```\n",
+          "Per wavefront (counter ÷ SQ_WAVES). VALU issue µs = SQ_INSTS_VALU x 4 cycles ÷ "
+          f"({SIMDS} SIMDs x {CLOCK_HZ / 1e9:.1f} GHz); measured µs from the kernel trace "
+          f"(`prof_{workload}_P_kt`). A kernel whose issue time is near its measured time is bound by "
+          "its VALU instructions; well below it, by its bytes or its latency.\n",
+          "| kernel | storage | waves | VALU/wave | F64 add+mul+fma | F64 trans | int32 | cvt | SALU/wave "
+          "| SMEM/wave | VALU issue µs | measured µs | issue ÷ measured |",
+          "|---|---|---|---|---|---|---|---|---|---|---|---|---|"]
+    for prec in ("fp64", "split"):
+        C = {}
+        for name in ISSUE_PASSES:
+            files = glob.glob(f"{base}/prof_{workload}_{prec}_{name}/**/*counter_collection.csv", recursive=True)
+            for k, d in _counters(files).items():
+                C.setdefault(k, {}).update(d)
+        kt = glob.glob(f"{base}/prof_{workload}_{prec}_kt/**/*kernel_stats.csv", recursive=True)
+        avg_us = {_short(r["Name"]): float(r["AverageNs"]) / 1e3 for r in csv.DictReader(open(kt[0]))}
+        for pat in STEP_KERNELS:
+            for k in sorted(C):
+                if not re.search(pat, k):
+                    continue
+                c = C[k]
+                w = c["SQ_WAVES"]
+                valu = c["SQ_INSTS_VALU"]  # over the launch
+                issue_us = valu * 4 / SIMDS / CLOCK_HZ * 1e6
+                arith = c["SQ_INSTS_VALU_ADD_F64"] + c["SQ_INSTS_VALU_MUL_F64"] + c["SQ_INSTS_VALU_FMA_F64"]
+                us = avg_us.get(k, float("nan"))
+                L.append(f"| `{k}` | {prec} | {w:.0f} | {valu / w:.0f} | {arith / w:.0f} | "
+                         f"{c['SQ_INSTS_VALU_TRANS_F64'] / w:.0f} | {c['SQ_INSTS_VALU_INT32'] / w:.0f} | "
+                         f"{c['SQ_INSTS_VALU_CVT'] / w:.0f} | {c['SQ_INSTS_SALU'] / w:.0f} | "
+                         f"{c['SQ_INSTS_SMEM'] / w:.0f} | {issue_us:.0f} | {us:.0f} | {issue_us / us:.2f} |")
+    os.makedirs(outdir, exist_ok=True)
+    with open(os.path.join(outdir, f"{workload}_issue.md"), "w") as fh:
+        fh.write("\n".join(L) + "\n")
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("workload")
-    ap.add_argument("precision", choices=("fp64", "split"))
+    ap.add_argument("precision", choices=("fp64", "split", "issue"),
+                    help="storage of the traffic summary, or issue: the instruction summary of both")
     ap.add_argument("base", help="folder holding the rocprofv3 output folders")
     ap.add_argument("--out", default="profiles")
     a = ap.parse_args()
+    if a.precision == "issue":
+        summarise_issue(a.workload, a.base, a.out)
+        return 0
     print(json.dumps(summarise(a.workload, a.precision, a.base, a.out)))
     return 0
 
