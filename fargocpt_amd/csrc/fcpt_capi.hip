@@ -1330,7 +1330,7 @@ int fcpt_time_copy(fcpt_ctx *ctx, int32_t reps, double *ms_per_launch, double *b
     HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     if (ms_per_launch) *ms_per_launch = (double)ms / reps;
     if (bytes_per_launch)
-        *bytes_per_launch = 2.0 * 4.0 * (double)((size_t)ctx->P.nr * ctx->P.ns * element_size(ctx->P) / 16 * 16);
+        *bytes_per_launch = 2.0 * 4.0 * (double)((size_t)ctx->P.nr * ctx->P.ns * element_size(ctx->P) / 8 * 8);
     return FCPT_OK;
 }
 

