@@ -485,6 +485,26 @@ __device__ __forceinline__ void ring_mean_store(const KParams &P, int i, double 
     }
 }
 
+// One thread's share of a ring sum: the sectors t, t + STRIDE, t + 2 STRIDE,
+// ... below ns, added in that order.  Eight sectors' loads are issued before
+// their additions (a loop of one load and one addition keeps one load in
+// flight per thread and waits for each): the same additions in the same
+// order, so the same bits.
+template <int STRIDE, class Load, class Add>
+__device__ __forceinline__ void strided_row(int ns, int t, Load &&load, Add &&add)
+{
+    constexpr int U = 8;
+    int j = t;
+    for (; j + STRIDE * (U - 1) < ns; j += STRIDE * U) {
+        double x[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) x[u] = load(j + STRIDE * u);
+#pragma unroll
+        for (int u = 0; u < U; ++u) add(x[u]);
+    }
+    for (; j < ns; j += STRIDE) add(load(j));
+}
+
 template <bool SPLIT>
 __device__ __forceinline__ void b_ring_mean(const KParams &P, int bx, int by, int mode)
 {
@@ -493,11 +513,10 @@ __device__ __forceinline__ void b_ring_mean(const KParams &P, int bx, int by, in
     const int i = bx;
     FLD(v, FCPT_VAZI);
     double acc = 0.0, aacc = 0.0;
-    for (int j = threadIdx.x; j < ns; j += 256) {
-        const double x = v(i, j);
+    strided_row<256>(ns, (int)threadIdx.x, [&](int j) { return v(i, j); }, [&](double x) {
         acc += x;
         aacc += fabs(x);
-    }
+    });
     const double2 r = block_sum_abs<256>(acc, aacc, sh);
     ring_mean_store(P, i, r.x, r.y, mode, [&]() {
         double seq = 0.0;
@@ -712,8 +731,7 @@ __device__ __forceinline__ void b_cfl_rings(const KParams &P, int i)
     const int t = (int)threadIdx.x;
     FLD(v, FCPT_VAZI);
     double acc = 0.0;
-    if (t < 256)
-        for (int j = t; j < ns; j += 256) acc += v(i, j);
+    if (t < 256) strided_row<256>(ns, t, [&](int j) { return v(i, j); }, [&](double x) { acc += x; });
     if (t < 256) sh[t] = acc;
     __syncthreads();
 #pragma unroll
@@ -3202,7 +3220,7 @@ __global__ void __launch_bounds__(256) k_split_ring_mean(const float *d, const d
     const int i = blockIdx.x;
     const float *row = d + (size_t)i * (size_t)ns;
     double acc = 0.0;
-    for (int j = threadIdx.x; j < ns; j += 256) acc += (double)row[j];
+    strided_row<256>(ns, (int)threadIdx.x, [&](int j) { return (double)row[j]; }, [&](double x) { acc += x; });
     const double sum = block_sum<256>(acc, sh);
     if (threadIdx.x == 0) out[i] = bg[i] + sum / ns;
 }
@@ -3624,7 +3642,7 @@ __global__ void __launch_bounds__(256) k_sigma_ring_mean(KParams P, double *sig1
     const int i = blockIdx.x;
     FLD(S, FCPT_SIGMA);
     double acc = 0.0;
-    for (int j = threadIdx.x; j < ns; j += 256) acc += S(i, j);
+    strided_row<256>(ns, (int)threadIdx.x, [&](int j) { return S(i, j); }, [&](double x) { acc += x; });
     const double sum = block_sum<256>(acc, sh);
     if (threadIdx.x == 0) sig1d[i] = sum / ns;
 }
