@@ -426,7 +426,7 @@ int fcpt_create(const fcpt_disk_desc *desc, fcpt_ctx **out)
     if (!rc) rc = dev_alloc_bytes(c, &P.valt[0], field_rows(FCPT_VRAD, nr) * ns * esz);
     if (!rc) rc = dev_alloc_bytes(c, &P.valt[1], field_rows(FCPT_VAZI, nr) * ns * esz);
     if (!rc && P.adiabatic && P.split) rc = dev_alloc_bytes(c, &P.ealt, field_rows(FCPT_ENERGY, nr) * ns * esz);
-    for (int k = 0; k < 4 && !rc; ++k) rc = dev_alloc_bytes(c, &P.mom[k], ncell * esz);
+    for (int k = 0; k < 4 && !rc; ++k) rc = dev_alloc_bytes(c, &P.mom[k], ncell * sizeof(double));  // FP64 in both modes
     for (int k = 0; k < 4 && !rc; ++k) rc = dev_alloc_bytes(c, &P.mom_b[k], ncell * esz);
     // the density and energy between the transport sweeps stay FP64 in both
     // storage modes: they are rounded once per step, on the final store

@@ -34,13 +34,15 @@ __device__ __forceinline__ double van_leer_lim(double a, double b)
 // kernels may form quantities that only scale a correction term in FP32 and
 // replace divisions by a reciprocal; the FP64 instantiations keep the
 // reference's IEEE operations.  van Leer limiter: FP32 in split storage (the
-// slope multiplies the second-order term of the interface value).
+// slope multiplies the second-order term of the interface value), its
+// quotient through the hardware reciprocal (1 ulp; an FP32 division without
+// fast-math is the IEEE sequence of about ten instructions).
 template <bool SPLIT>
 __device__ __forceinline__ double vl_lim(double a, double b)
 {
     if constexpr (SPLIT) {
         const float fa = (float)a, fb = (float)b;
-        const float r = __fdividef(2.0f * fa * fb, fa + fb);
+        const float r = 2.0f * fa * fb * __builtin_amdgcn_rcpf(fa + fb);
         return (fa * fb > 0.0f) ? (double)r : 0.0;
     } else {
         return van_leer_lim(a, b);
@@ -290,6 +292,16 @@ __device__ __forceinline__ Fld<SPLIT> buf(const KParams &P, void *p, const doubl
 }
 
 #define FLD(name, id) const Fld<SPLIT> name = fld<SPLIT>(P, id)
+
+// The transported momenta after the azimuthal sweep, read by the end of the
+// step: FP64 in both storage modes (in split storage their FP32 rounding,
+// once per step, was the largest part of the error near the planets, and
+// the sweep that stores them is bound by its instructions, which the
+// stochastic rounding of four stores per cell adds to)
+__device__ __forceinline__ Fld<false> mom_out(const KParams &P, int q)
+{
+    return Fld<false>{static_cast<double *>(P.mom[q]), nullptr, P.ns, 0u};
+}
 
 // The gas potential is stored in FP64 in both storage modes: its radial and
 // azimuthal differences drive the source terms, and near a body the
@@ -2944,7 +2956,7 @@ __device__ __forceinline__ void b_azimuthal_sweep(const KParams &P, int bx, int 
             if (wa) Sc.set(i, jda, v.a);
             if (wb) Sc.set(i, jdb, v.b);
         } else if (q < 4) {
-            const Fld<SPLIT> m = buf<SPLIT>(P, P.mom[q], P.bgm[q], kBufMom + q);
+            const Fld<false> m = mom_out(P, q);
             if (wa) m.set(i, jda, v.a);
             if (wb) m.set(i, jdb, v.b);
         } else {
@@ -2977,10 +2989,10 @@ __device__ __forceinline__ void b_velocities(const KParams &P, int bx, int by)
 {
     CELL_LOOP_HEAD(0, P.nr)
     const Fld<false> Sb = buf<false>(P, P.sig_c, nullptr);
-    const Fld<SPLIT> m0 = buf<SPLIT>(P, P.mom[0], P.bgm[0], kBufMom + 0);
-    const Fld<SPLIT> m1 = buf<SPLIT>(P, P.mom[1], P.bgm[1], kBufMom + 1);
-    const Fld<SPLIT> m2 = buf<SPLIT>(P, P.mom[2], P.bgm[2], kBufMom + 2);
-    const Fld<SPLIT> m3 = buf<SPLIT>(P, P.mom[3], P.bgm[3], kBufMom + 3);
+    const Fld<false> m0 = mom_out(P, 0);
+    const Fld<false> m1 = mom_out(P, 1);
+    const Fld<false> m2 = mom_out(P, 2);
+    const Fld<false> m3 = mom_out(P, 3);
     const Fld<false> Eb = buf<false>(P, P.en_c, nullptr);
     FLD(vr, FCPT_VRAD);
     FLD(vaz, FCPT_VAZI);
@@ -3036,8 +3048,8 @@ __device__ __forceinline__ double vel_final_vr(const KParams &P, int r, int j, d
 {
     // v_r(r >= 1) as the velocity kernel and the damping pass it to set()
     const Fld<false> Sb = buf<false>(P, P.sig_c, nullptr);
-    const Fld<SPLIT> m0 = buf<SPLIT>(P, P.mom[0], P.bgm[0], kBufMom + 0);
-    const Fld<SPLIT> m1 = buf<SPLIT>(P, P.mom[1], P.bgm[1], kBufMom + 1);
+    const Fld<false> m0 = mom_out(P, 0);
+    const Fld<false> m1 = mom_out(P, 1);
     FLD(vr, FCPT_VRAD);
     const double v = (m0(r - 1, j) + m1(r, j)) / (Sb(r - 1, j) + Sb(r, j));
     return damp_stored<SPLIT>(P, vr, FCPT_VRAD0, true, r, j, dt, v);
@@ -3087,8 +3099,8 @@ __device__ __forceinline__ void b_velocities_final(const KParams &P, int bx, int
         vphi = ghost_vaz(P, i);
     } else {
         const Fld<false> Sb = buf<false>(P, P.sig_c, nullptr);
-        const Fld<SPLIT> m2 = buf<SPLIT>(P, P.mom[2], P.bgm[2], kBufMom + 2);
-        const Fld<SPLIT> m3 = buf<SPLIT>(P, P.mom[3], P.bgm[3], kBufMom + 3);
+        const Fld<false> m2 = mom_out(P, 2);
+        const Fld<false> m3 = mom_out(P, 3);
         const int jm = j == 0 ? ns - 1 : j - 1;
         const double s0 = Sb(i, j), sjm = Sb(i, jm);
         const double v_phi =
