@@ -103,10 +103,12 @@ struct KParams {
     // split storage: per-ring backgrounds of every field (nr + 2 values each;
     // null in FP64 mode); the potential's is rewritten by the potential kernel
     double *bg[FCPT_NUM_FIELDS];
-    // transport work arrays: the momenta have the fields' element type, the
-    // density and energy between the sweeps are FP64 in both storage modes
-    void *mom[4];        // radial momentum +/-, angular momentum +/-
-    void *mom_b[4];      // ping-pong partners
+    // transport work arrays: the momenta between the sweeps (mom_b) have the
+    // fields' element type; the azimuthal sweep's momenta (mom, read by the
+    // end of the step) and the density and energy between the sweeps are
+    // FP64 in both storage modes
+    void *mom[4];        // radial momentum +/-, angular momentum +/- (FP64)
+    void *mom_b[4];      // the radial sweep's momenta
     void *sig_b, *en_b;  // radial-sweep outputs
     void *sig_c, *en_c;  // azimuthal-sweep outputs
     void *valt[2];       // partners of v_r and v_phi for the fused step kernels
