@@ -602,10 +602,14 @@ __device__ __forceinline__ AdiDerived adi_derived(const KParams &P, int i, doubl
 {
     AdiDerived d;
     const double g = P.gamma;
-    const double cs2 = g * (g - 1.0) * e / sig;
-    d.cs = SPLIT ? sqrt(cs2) : sqrt_ieee(cs2);  // the same bits (the split walks run faster with sqrt)
     const double inv_omega_kepler = ring_tab(P, TAB_INV_OMEGA_K, i);  // 1 / Omega_K(Rmed)
-    d.h = d.cs / (sqrt(g)) * inv_omega_kepler;
+    if constexpr (SPLIT) {  // products with reciprocals (the 1 / sqrt(gamma) is launch-wide)
+        d.cs = sqrt(g * (g - 1.0) * e * recip<true>(sig));
+        d.h = d.cs * inv_sqrt<true>(g) * inv_omega_kepler;
+    } else {
+        d.cs = sqrt_ieee(g * (g - 1.0) * e / sig);
+        d.h = d.cs / (sqrt(g)) * inv_omega_kepler;
+    }
     const double alpha = P.alpha;
     d.nu = alpha * d.h * d.cs;
     return d;
@@ -1075,16 +1079,25 @@ __device__ __forceinline__ void b_src_energy(const KParams &P, int bx, int by)
 }
 FCPT_PAIR0(src_energy, 256)
 
-// the energy of temperature t_min, t_min sigma / mu R / (gamma - 1)
+// the energy of temperature t, t sigma / mu R / (gamma - 1) (split storage:
+// one product with the launch-wide R / (mu (gamma - 1)))
+template <bool SPLIT = false>
+__device__ __forceinline__ double energy_at(const KParams &P, double t, double sigma)
+{
+    if constexpr (SPLIT) return t * sigma * (P.Rgas * recip<true>(P.mu * (P.gamma - 1.0)));
+    else return t * sigma / P.mu * P.Rgas / (P.gamma - 1.0);
+}
+template <bool SPLIT = false>
 __device__ __forceinline__ double energy_floor(const KParams &P, double sigma)
 {
-    return P.t_min * sigma / P.mu * P.Rgas / (P.gamma - 1.0);
+    return energy_at<SPLIT>(P, P.t_min, sigma);
 }
 // assure_temperature_range for one cell, SourceEuler.cpp:136-202
+template <bool SPLIT = false>
 __device__ __forceinline__ double temperature_floor(const KParams &P, double e, double sigma)
 {
-    const double minimum_energy = energy_floor(P, sigma);
-    const double maximum_energy = P.t_max * sigma / P.mu * P.Rgas / (P.gamma - 1.0);
+    const double minimum_energy = energy_floor<SPLIT>(P, sigma);
+    const double maximum_energy = energy_at<SPLIT>(P, P.t_max, sigma);
     if (!(e > minimum_energy)) e = minimum_energy;
     if (!(e < maximum_energy)) e = maximum_energy;
     return e;
@@ -1096,7 +1109,7 @@ __device__ __forceinline__ void b_temperature_floor(const KParams &P, int bx, in
     CELL_LOOP_HEAD(0, P.nr)
     FLD(E, FCPT_ENERGY);
     FLD(S, FCPT_SIGMA);
-    E.set(i, j, temperature_floor(P, E(i, j), S(i, j)));
+    E.set(i, j, temperature_floor<SPLIT>(P, E(i, j), S(i, j)));
 }
 FCPT_PAIR0(temperature_floor, 256)
 
@@ -2049,7 +2062,7 @@ __device__ __forceinline__ double cooling_beta_cell(const KParams &P, int i, int
     if (P.cooling_beta_ref == FCPT_BETA_REF_REFERENCE)
         delta_E -= fld<SPLIT>(P, FCPT_ENERGY0)(i, j) / fld<SPLIT>(P, FCPT_SIGMA0)(i, j) * sig;
     if (P.cooling_beta_ref == FCPT_BETA_REF_FLOOR)
-        delta_E -= energy_floor(P, sig);
+        delta_E -= energy_floor<SPLIT>(P, sig);
     return 0.0 + delta_E * omega_k * beta_inv;
 }
 
@@ -2132,7 +2145,7 @@ __device__ __forceinline__ void substep3_cell(const KParams &P, int i, int j, do
     if (!lazy_derived<SPLIT>(P)) fld<SPLIT>(P, FCPT_TEMPERATURE).set(i, j, temperature);
     fld<SPLIT>(P, FCPT_QPLUS).set(i, j, qplus);
     fld<SPLIT>(P, FCPT_QMINUS).set(i, j, qminus);
-    fld<SPLIT>(P, FCPT_ENERGY).set(i, j, temperature_floor(P, e, sig));
+    fld<SPLIT>(P, FCPT_ENERGY).set(i, j, temperature_floor<SPLIT>(P, e, sig));
 }
 
 template <bool SPLIT>
@@ -3011,7 +3024,7 @@ __device__ __forceinline__ void b_velocities(const KParams &P, int bx, int by)
     double sig = s0;
     if (sig < floor_value) sig = floor_value;
     fld<SPLIT>(P, FCPT_SIGMA).set(i, j, sig);
-    if (P.adiabatic) fld<SPLIT>(P, FCPT_ENERGY).set(i, j, temperature_floor(P, e, sig));
+    if (P.adiabatic) fld<SPLIT>(P, FCPT_ENERGY).set(i, j, temperature_floor<SPLIT>(P, e, sig));
 }
 FCPT_PAIR0(velocities, 256)
 
@@ -3051,7 +3064,8 @@ __device__ __forceinline__ double vel_final_vr(const KParams &P, int r, int j, d
     const Fld<false> m0 = mom_out(P, 0);
     const Fld<false> m1 = mom_out(P, 1);
     FLD(vr, FCPT_VRAD);
-    const double v = (m0(r - 1, j) + m1(r, j)) / (Sb(r - 1, j) + Sb(r, j));
+    const double den = Sb(r - 1, j) + Sb(r, j);
+    const double v = SPLIT ? (m0(r - 1, j) + m1(r, j)) * recip<true>(den) : (m0(r - 1, j) + m1(r, j)) / den;
     return damp_stored<SPLIT>(P, vr, FCPT_VRAD0, true, r, j, dt, v);
 }
 template <bool SPLIT>
@@ -3069,7 +3083,7 @@ __device__ __forceinline__ void vel_final_sig_e(const KParams &P, int r, int j, 
     sig = damp_stored<SPLIT>(P, S, FCPT_SIGMA0, false, r, j, dt, s0);
     if (P.adiabatic) {
         const double eb = buf<false>(P, P.en_c, nullptr)(r, j);
-        e = damp_stored<SPLIT>(P, E, FCPT_ENERGY0, false, r, j, dt, temperature_floor(P, eb, s0));
+        e = damp_stored<SPLIT>(P, E, FCPT_ENERGY0, false, r, j, dt, temperature_floor<SPLIT>(P, eb, s0));
     } else {
         e = need_e ? E(r, j) : 0.0;
     }
@@ -3103,8 +3117,9 @@ __device__ __forceinline__ void b_velocities_final(const KParams &P, int bx, int
         const Fld<false> m3 = mom_out(P, 3);
         const int jm = j == 0 ? ns - 1 : j - 1;
         const double s0 = Sb(i, j), sjm = Sb(i, jm);
-        const double v_phi =
-            (m2(i, jm) + m3(i, j)) / (sjm + s0) * cst(P.InvRmed)[i] - cst(P.Rmed)[i] * P.omega_frame;
+        const double msum = m2(i, jm) + m3(i, j);
+        const double v_phi = (SPLIT ? msum * recip<true>(sjm + s0) : msum / (sjm + s0)) * cst(P.InvRmed)[i] -
+                             cst(P.Rmed)[i] * P.omega_frame;
         vphi = damp_stored<SPLIT>(P, vaz, FCPT_VAZI0, false, i, j, dt, v_phi);
         if (i == 1) ghost_vr_inner(P, vr.stored(2, j, vel_final_vr<SPLIT>(P, 2, j, dt)), va, vb);
         else vb = vel_final_vr<SPLIT>(P, i, j, dt);
