@@ -136,6 +136,15 @@ __device__ __forceinline__ double inv_sqrt(double x)
     }
 }
 
+// n / d: the IEEE quotient, or in split storage the product of n with the
+// refined reciprocal of d
+template <bool SPLIT>
+__device__ __forceinline__ double quot(double n, double d)
+{
+    if constexpr (SPLIT) return n * recip<true>(d);
+    else return n / d;
+}
+
 __device__ __forceinline__ double omega_kepler(const KParams &P, double r)
 {
     return sqrt(P.G * P.mstar / (r * r * r));  // Theo.cpp:246-249
@@ -997,11 +1006,12 @@ __global__ void __launch_bounds__(256) k_potential_fold(KParams P, int fold_mode
 // momentum_update_radial, SourceEuler.cpp:325-372, for one cell: pressure and
 // potential gradients between rings i - 1 and i, and the centrifugal term of
 // the four v_phi around the interface (summed in that order)
+template <bool SPLIT = false>
 __device__ __forceinline__ double src_vr_new(const KParams &P, int i, double dt, double v, double s0,
                                              double sm, double p0, double pm, double phi0, double phim,
                                              double va00, double va01, double va10, double va11)
 {
-    double gradp = 2.0 / (s0 + sm);
+    double gradp = quot<SPLIT>(2.0, s0 + sm);
     gradp *= (p0 - pm);
     gradp *= cst(P.InvDiffRmed)[i];
     const double gradphi = (phi0 - phim) * cst(P.InvDiffRmed)[i];
@@ -1013,12 +1023,13 @@ __device__ __forceinline__ double src_vr_new(const KParams &P, int i, double dt,
 }
 
 // momentum_update_azimuthal, SourceEuler.cpp:375-428, for one cell
+template <bool SPLIT = false>
 __device__ __forceinline__ double src_vphi_new(const KParams &P, int i, double dt, double va, double s0,
                                                double sjp, double p0, double pjp, double phi0,
                                                double phijp)
 {
     const double invdxtheta = ring_tab(P, TAB_SRC_INVDXTHETA, i);  // 2 / (dphi (Rsup + Rinf))
-    const double gradp = 2.0 / (s0 + sjp) * (p0 - pjp) * invdxtheta;
+    const double gradp = quot<SPLIT>(2.0, s0 + sjp) * (p0 - pjp) * invdxtheta;
     const double gradphi = (phi0 - phijp) * invdxtheta;
     return va + dt * (-gradp - gradphi);
 }
@@ -1035,7 +1046,7 @@ __device__ __forceinline__ void b_src_vr(const KParams &P, int bx, int by)
     FLD(vr, FCPT_VRAD);
     const double dt = P.s->dt;
     const int jn = j == ns - 1 ? 0 : j + 1;
-    vr.set(i, j, src_vr_new(P, i, dt, vr(i, j), S(i, j), S(i - 1, j), Pr(i, j), Pr(i - 1, j), Phi(i, j),
+    vr.set(i, j, src_vr_new<SPLIT>(P, i, dt, vr(i, j), S(i, j), S(i - 1, j), Pr(i, j), Pr(i - 1, j), Phi(i, j),
                             Phi(i - 1, j), vaz(i, j), vaz(i, jn), vaz(i - 1, j), vaz(i - 1, jn)));
 }
 FCPT_PAIR0(src_vr, 256)
@@ -1051,7 +1062,7 @@ __device__ __forceinline__ void b_src_vphi(const KParams &P, int bx, int by)
     FLD(vaz, FCPT_VAZI);
     const double dt = P.s->dt;
     const int jp = j == 0 ? ns - 1 : j - 1;
-    vaz.set(i, j, src_vphi_new(P, i, dt, vaz(i, j), S(i, j), S(i, jp), Pr(i, j), Pr(i, jp), Phi(i, j),
+    vaz.set(i, j, src_vphi_new<SPLIT>(P, i, dt, vaz(i, j), S(i, j), S(i, jp), Pr(i, j), Pr(i, jp), Phi(i, j),
                                Phi(i, jp)));
 }
 FCPT_PAIR0(src_vphi, 256)
@@ -1248,20 +1259,23 @@ FCPT_PAIR0(tw_q, 256)
 
 // the TW velocity updates of one cell from the tensors around it
 // (artificial_viscosity.cpp:90-139)
+template <bool SPLIT = false>
 __device__ __forceinline__ double tw_vphi_new(const KParams &P, int i, double dt, double va, double s0,
                                               double sjp, double qp0, double qpj)
 {
     const double sigma_phi_avg = 0.5 * (s0 + sjp);
-    const double dVp = 2.0 * dt / ((cst(P.Rsup)[i] + cst(P.Rinf)[i]) * sigma_phi_avg) * (qp0 - qpj) * P.invdphi;
+    const double dVp =
+        quot<SPLIT>(2.0 * dt, (cst(P.Rsup)[i] + cst(P.Rinf)[i]) * sigma_phi_avg) * (qp0 - qpj) * P.invdphi;
     return va + dVp;
 }
 
+template <bool SPLIT = false>
 __device__ __forceinline__ double tw_vr_new(const KParams &P, int i, double dt, double v, double s0,
                                             double sm, double qr0, double qrm, double qp0, double qpm)
 {
     const double sigma_r_avg = 0.5 * (s0 + sm);
     const double r0 = cst(P.Rmed)[i], r1 = cst(P.Rmed)[i - 1];
-    const double dVr = P.radial_viscosity_factor * dt / sigma_r_avg * 2.0 / (r0 * r0 - r1 * r1) *
+    const double dVr = quot<SPLIT>(quot<SPLIT>(P.radial_viscosity_factor * dt, sigma_r_avg) * 2.0, r0 * r0 - r1 * r1) *
                        ((qr0 * r0 - qrm * r1) - 0.5 * (qp0 + qpm) * (r0 - r1));
     return v + dVr;
 }
@@ -1303,8 +1317,8 @@ __device__ __forceinline__ void b_tw_vel(const KParams &P, int bx, int by, int w
     }
     if (visc) derived_store<SPLIT>(P, i, j, din, false, false);
     if (!inner) return;
-    vaz.set(i, j, tw_vphi_new(P, i, dt, va, s0, sjp, qp0, qpj));
-    if (radial) vr.set(i, j, tw_vr_new(P, i, dt, v, s0, sm, qr0, qrm, qp0, qpm));
+    vaz.set(i, j, tw_vphi_new<SPLIT>(P, i, dt, va, s0, sjp, qp0, qpj));
+    if (radial) vr.set(i, j, tw_vr_new<SPLIT>(P, i, dt, v, s0, sm, qr0, qrm, qp0, qpm));
 }
 FCPT_PAIR1(tw_vel, 256, int, with_viscosity)
 
@@ -1360,17 +1374,19 @@ __device__ __forceinline__ void b_sn_q(const KParams &P, int bx, int by)
 FCPT_PAIR0(sn_q, 256)
 
 // the SN velocity updates of one cell (artificial_viscosity.cpp:214-248)
+template <bool SPLIT = false>
 __device__ __forceinline__ double sn_vr_new(const KParams &P, int i, double dt, double v, double s0,
                                             double sm, double qr0, double qrm)
 {
-    return v - dt * 2.0 / (s0 + sm) * (qr0 - qrm) * cst(P.InvDiffRmed)[i];
+    return v - quot<SPLIT>(dt * 2.0, s0 + sm) * (qr0 - qrm) * cst(P.InvDiffRmed)[i];
 }
 
+template <bool SPLIT = false>
 __device__ __forceinline__ double sn_vphi_new(const KParams &P, int i, double dt, double va, double s0,
                                               double sjp, double qp0, double qpj)
 {
     const double invdxtheta = ring_tab(P, TAB_INV_DXTHETA, i);  // 1 / (dphi Rmed)
-    return va - dt * 2.0 / (s0 + sjp) * (qp0 - qpj) * invdxtheta;
+    return va - quot<SPLIT>(dt * 2.0, s0 + sjp) * (qp0 - qpj) * invdxtheta;
 }
 
 template <bool SPLIT>
@@ -1409,8 +1425,8 @@ __device__ __forceinline__ void b_sn_vel(const KParams &P, int bx, int by, int w
     }
     if (visc) derived_store<SPLIT>(P, i, j, din, false, false);
     if (!inner) return;
-    if (radial) vr.set(i, j, sn_vr_new(P, i, dt, v, s0, sm, qr0, qrm));
-    vaz.set(i, j, sn_vphi_new(P, i, dt, va, s0, sjp, qp0, qpj));
+    if (radial) vr.set(i, j, sn_vr_new<SPLIT>(P, i, dt, v, s0, sm, qr0, qrm));
+    vaz.set(i, j, sn_vphi_new<SPLIT>(P, i, dt, va, s0, sjp, qp0, qpj));
 }
 FCPT_PAIR1(sn_vel, 256, int, with_viscosity)
 
@@ -1570,24 +1586,26 @@ __device__ __forceinline__ void b_stress(const KParams &P, int bx, int by)
 FCPT_PAIR0(stress, 256)
 
 // update_velocities_with_viscosity, viscosity.cpp:355-426, for one cell
+template <bool SPLIT = false>
 __device__ __forceinline__ double visc_vphi_new(const KParams &P, int i, double dt, double va, double s0,
                                                 double sjp, double trp0, double trp1, double tpp0,
                                                 double tppj)
 {
     const double sigma_avg = 0.5 * (s0 + sjp);
     const double a1 = cst(P.Rinf)[i + 1] * cst(P.Rinf)[i + 1], a0 = cst(P.Rinf)[i] * cst(P.Rinf)[i];
-    const double dVp = dt * cst(P.InvRmed)[i] / (sigma_avg) *
+    const double dVp = quot<SPLIT>(dt * cst(P.InvRmed)[i], sigma_avg) *
                        (ring_tab(P, TAB_VISC_AREA, i) * (a1 * trp1 - a0 * trp0) + (tpp0 - tppj) * P.invdphi);
     return va + dVp;
 }
 
+template <bool SPLIT = false>
 __device__ __forceinline__ double visc_vr_new(const KParams &P, int i, double dt, double v, double s0,
                                               double sm, double trr0, double trrm, double trp0,
                                               double trpn, double tpp0, double tppm)
 {
     const double sigma_avg = 0.5 * (s0 + sm);
-    const double dVr = dt / (sigma_avg)*P.radial_viscosity_factor * 2.0 /
-                       (cst(P.Rmed)[i] + cst(P.Rmed)[i - 1]) *
+    const double dVr = quot<SPLIT>(quot<SPLIT>(dt, sigma_avg) * P.radial_viscosity_factor * 2.0,
+                                   cst(P.Rmed)[i] + cst(P.Rmed)[i - 1]) *
                        ((cst(P.Rmed)[i] * trr0 - cst(P.Rmed)[i - 1] * trrm) * cst(P.InvDiffRmed)[i] +
                         (trpn - trp0) * P.invdphi - 0.5 * (tpp0 + tppm));
     return v + dVr;
@@ -1611,8 +1629,8 @@ __device__ __forceinline__ void b_visc_update(const KParams &P, int bx, int by)
     const double tpp0 = tpp(i, j), tppj = tpp(i, jp), tppm = tpp(i - 1, j);
     const double trr0 = trr(i, j), trrm = trr(i - 1, j);
     const double va = vaz(i, j), v = vr(i, j);
-    vaz.set(i, j, visc_vphi_new(P, i, dt, va, s0, sjp, trp0, trp1, tpp0, tppj));
-    if (i >= 2) vr.set(i, j, visc_vr_new(P, i, dt, v, s0, sm, trr0, trrm, trp0, trpn, tpp0, tppm));
+    vaz.set(i, j, visc_vphi_new<SPLIT>(P, i, dt, va, s0, sjp, trp0, trp1, tpp0, tppj));
+    if (i >= 2) vr.set(i, j, visc_vr_new<SPLIT>(P, i, dt, v, s0, sm, trr0, trrm, trp0, trpn, tpp0, tppm));
 }
 FCPT_PAIR0(visc_update, 256)
 
@@ -1678,8 +1696,8 @@ __device__ __forceinline__ void b_src_fused(const KParams &P, int bx, int by)
         vam1 = vaz(i - 1, jn);
     }
     const typename Fld<SPLIT>::T vr_raw = vr.raw(i, j);
-    vaz_o.set(i, j, src_vphi_new(P, i, dt, va0, s0, sjp, p0, pjp, phi0, phijp));
-    if (upd_r) vr_o.set(i, j, src_vr_new(P, i, dt, v, s0, sm, p0, pm, phi0, phim, va0, va1, vam0, vam1));
+    vaz_o.set(i, j, src_vphi_new<SPLIT>(P, i, dt, va0, s0, sjp, p0, pjp, phi0, phijp));
+    if (upd_r) vr_o.set(i, j, src_vr_new<SPLIT>(P, i, dt, v, s0, sm, p0, pm, phi0, phim, va0, va1, vam0, vam1));
     else vr_o.set_raw(i, j, vr_raw);
 }
 FCPT_PAIR0(src_fused, 256)
@@ -1755,13 +1773,13 @@ __device__ __forceinline__ void b_av_fused(const KParams &P, int bx, int by, int
     if (diss) e = E.set_get(i, j, e);
     if (visc) derived_store<SPLIT>(P, i, j, DerivedIn{e, sig, 0.0, 0.0}, false, false);
     if (inner)
-        vaz_o.set(i, j, sn ? sn_vphi_new(P, i, dt, va0, sig, sjp, qp0, qpj)
-                           : tw_vphi_new(P, i, dt, va0, sig, sjp, qp0, qpj));
+        vaz_o.set(i, j, sn ? sn_vphi_new<SPLIT>(P, i, dt, va0, sig, sjp, qp0, qpj)
+                           : tw_vphi_new<SPLIT>(P, i, dt, va0, sig, sjp, qp0, qpj));
     else
         vaz_o.set_raw(i, j, va_raw);
     if (radial)
-        vr_o.set(i, j, sn ? sn_vr_new(P, i, dt, vr0, sig, sm, qr0, qrm)
-                          : tw_vr_new(P, i, dt, vr0, sig, sm, qr0, qrm, qp0, qpm));
+        vr_o.set(i, j, sn ? sn_vr_new<SPLIT>(P, i, dt, vr0, sig, sm, qr0, qrm)
+                          : tw_vr_new<SPLIT>(P, i, dt, vr0, sig, sm, qr0, qrm, qp0, qpm));
     else
         vr_o.set_raw(i, j, vr_raw);
 }
@@ -1838,8 +1856,8 @@ __device__ __forceinline__ void b_kick_walk(const KParams &P, int bx, int by, in
         const double pjp = from_left(b.p), phijp = from_left(b.phi);
         const bool upd_p = r >= 1 && r < nr - 1;
         const bool upd_r = r >= 2 && upd_p;
-        v1a = upd_p ? src_vphi_new(P, r, dt, b.va, b.s, b.sL, b.p, pjp, b.phi, phijp) : b.va;
-        v1r = upd_r ? src_vr_new(P, r, dt, b.vr, b.s, a.s, b.p, a.p, b.phi, a.phi, b.va, b.vaR, a.va, a.vaR)
+        v1a = upd_p ? src_vphi_new<SPLIT>(P, r, dt, b.va, b.s, b.sL, b.p, pjp, b.phi, phijp) : b.va;
+        v1r = upd_r ? src_vr_new<SPLIT>(P, r, dt, b.vr, b.s, a.s, b.p, a.p, b.phi, a.phi, b.va, b.vaR, a.va, a.vaR)
                     : b.vr;
     };
     // the tensor of cell (r, j) from the source-term velocities
@@ -1910,13 +1928,13 @@ __device__ __forceinline__ void b_kick_walk(const KParams &P, int bx, int by, in
                 fld<SPLIT>(P, FCPT_VISCOSITY).set(i, j, d.nu);
             }
             if (inner)
-                vaz_o.set(i, j, sn ? sn_vphi_new(P, i, dt, v1a0, R0.s, R0.sL, q.qp, qpj)
-                                   : tw_vphi_new(P, i, dt, v1a0, R0.s, R0.sL, q.qp, qpj));
+                vaz_o.set(i, j, sn ? sn_vphi_new<SPLIT>(P, i, dt, v1a0, R0.s, R0.sL, q.qp, qpj)
+                                   : tw_vphi_new<SPLIT>(P, i, dt, v1a0, R0.s, R0.sL, q.qp, qpj));
             else
                 vaz_o.set_raw(i, j, vaz.raw(i, j));
             if (radial)
-                vr_o.set(i, j, sn ? sn_vr_new(P, i, dt, v1r0, R0.s, s_m, q.qr, qr_m)
-                                  : tw_vr_new(P, i, dt, v1r0, R0.s, s_m, q.qr, qr_m, q.qp, qp_m));
+                vr_o.set(i, j, sn ? sn_vr_new<SPLIT>(P, i, dt, v1r0, R0.s, s_m, q.qr, qr_m)
+                                  : tw_vr_new<SPLIT>(P, i, dt, v1r0, R0.s, s_m, q.qr, qr_m, q.qp, qp_m));
             else
                 vr_o.set_raw(i, j, vr.raw(i, j));
         }
@@ -2016,14 +2034,14 @@ __device__ __forceinline__ void b_visc_fused(const KParams &P, int bx, int by)
     if (upd_p) {
         const double trp1 = stress_rp(P, ip, va_p, va0, vr1, vr1p, np_, n, npj, njp, sp, s, spj, sjp);
         const double tppj = stress_diag(P, i, vrp, vr1p, vaj, va0, njp, sjp).tpp;
-        vaz_o.set(i, j, visc_vphi_new(P, i, dt, va0, s, sjp, trp0, trp1, d.tpp, tppj));
+        vaz_o.set(i, j, visc_vphi_new<SPLIT>(P, i, dt, va0, s, sjp, trp0, trp1, d.tpp, tppj));
     } else {
         vaz_o.set_raw(i, j, va_raw);
     }
     if (upd_r) {
         const StressDiag dm = stress_diag(P, im, vrm, vr0, vam, vam1, nm, sm);
         const double trpn = stress_rp(P, i, va1, vam1, vrn, vr0, nn, nmn, n, nm, sn, smn, s, sm);
-        vr_o.set(i, j, visc_vr_new(P, i, dt, vr0, s, sm, d.trr, dm.trr, trp0, trpn, d.tpp, dm.tpp));
+        vr_o.set(i, j, visc_vr_new<SPLIT>(P, i, dt, vr0, s, sm, d.trr, dm.trr, trp0, trpn, d.tpp, dm.tpp));
     } else {
         vr_o.set_raw(i, j, vr_raw);
     }
@@ -2068,10 +2086,11 @@ __device__ __forceinline__ double cooling_beta_cell(const KParams &P, int i, int
 
 // Q+ of the viscous heating (viscous_heating, SourceEuler.cpp:496-536) from
 // the cell's stress tensor; tau_r_phi is the mean of the cell's four corners
+template <bool SPLIT = false>
 __device__ __forceinline__ double viscous_heating_q(const KParams &P, double nu, double sig, double trr,
                                                     double tpp, double dv, double tau_r_phi)
 {
-    double q = 1.0 / (2.0 * nu * sig) * (trr * trr + 2 * (tau_r_phi * tau_r_phi) + tpp * tpp);
+    double q = quot<SPLIT>(1.0, 2.0 * nu * sig) * (trr * trr + 2 * (tau_r_phi * tau_r_phi) + tpp * tpp);
     q += (2.0 / 9.0) * nu * sig * (dv * dv);
     q *= P.heating_viscous_factor;
     return 0.0 + q;
@@ -2092,7 +2111,7 @@ __device__ __forceinline__ void heating_cooling_cell(const KParams &P, int i, in
             FLD(trp, FCPT_TAU_R_PHI);
             const int jn = j == ns - 1 ? 0 : j + 1;
             const double tau_r_phi = 0.25 * (trp(i, j) + trp(i + 1, j) + trp(i, jn) + trp(i + 1, jn));
-            qplus = viscous_heating_q(P, nu, sig, fld<SPLIT>(P, FCPT_TAU_R_R)(i, j),
+            qplus = viscous_heating_q<SPLIT>(P, nu, sig, fld<SPLIT>(P, FCPT_TAU_R_R)(i, j),
                                       fld<SPLIT>(P, FCPT_TAU_PHI_PHI)(i, j), fld<SPLIT>(P, FCPT_DIV_V)(i, j),
                                       tau_r_phi);
         }
@@ -2102,9 +2121,10 @@ __device__ __forceinline__ void heating_cooling_cell(const KParams &P, int i, in
 
 // the radiative correction of SubStep3 (SourceEuler.cpp:920-924; its
 // std::pow(., 4) and std::pow(., 3) by compensated powering)
+template <bool SPLIT = false>
 __device__ __forceinline__ double rad_alpha(const KParams &P, double h, double sig, double e)
 {
-    const double inv_pow4 = pow4_comp(P.mu * (P.gamma - 1.0) / (P.Rgas * sig));
+    const double inv_pow4 = pow4_comp(quot<SPLIT>(P.mu * (P.gamma - 1.0), P.Rgas * sig));
     return 1.0 + 2.0 * h * 4.0 * P.sigma_sb / P.c_light * inv_pow4 * pow3_comp(e);
 }
 
@@ -2119,10 +2139,11 @@ __device__ __forceinline__ void substep3_cell(const KParams &P, int i, int j, do
     const double c_v_inv = P.mu / P.Rgas * (P.gamma - 1.0);
     const double temperature = c_v_inv * e / sig;
     if (i >= 1 && i < P.nr - 1) {
-        const double alpha = rad_alpha(P, h, sig, e);
+        const double alpha = rad_alpha<SPLIT>(P, h, sig, e);
         if constexpr (SPLIT) {
-            qplus /= alpha;
-            qminus /= alpha;
+            const double ra = recip<true>(alpha);
+            qplus *= ra;
+            qminus *= ra;
         } else {  // the same bits through one reciprocal
             const SharedDen A = shared_den(alpha);
             qplus = shared_div(qplus, A);
@@ -2172,7 +2193,7 @@ __device__ __forceinline__ void b_heatcool_cfl(const KParams &P, int bx, int by)
         const double e = fld<SPLIT>(P, FCPT_ENERGY)(i, j);
         const double sig = fld<SPLIT>(P, FCPT_SIGMA)(i, j);
         heating_cooling_cell<SPLIT>(P, i, j, P.s->time, e, sig, qplus, qminus);
-        const double alpha = rad_alpha(P, fld<SPLIT>(P, FCPT_SCALE_HEIGHT)(i, j), sig, e);
+        const double alpha = rad_alpha<SPLIT>(P, fld<SPLIT>(P, FCPT_SCALE_HEIGHT)(i, j), sig, e);
         qplus /= alpha;
         qminus /= alpha;
     }
@@ -2288,11 +2309,11 @@ __device__ __forceinline__ void b_visc_walk(const KParams &P, int bx, int by, in
                 if (i >= 1) fld<SPLIT>(P, FCPT_TAU_R_PHI).set(i, j, trp_c);
             }
             if (upd_p)
-                vaz_o.set(i, j, visc_vphi_new(P, i, dt, A.va, A.s, A.sL, trp_c, trp1, d.tpp, tppj));
+                vaz_o.set(i, j, visc_vphi_new<SPLIT>(P, i, dt, A.va, A.s, A.sL, trp_c, trp1, d.tpp, tppj));
             else
                 vaz_o.set_raw(i, j, vaz.raw(i, j));
             if (upd_r)
-                vr_o.set(i, j, visc_vr_new(P, i, dt, A.vr, A.s, s_prev, d.trr, dprev.trr, trp_c, trp_n,
+                vr_o.set(i, j, visc_vr_new<SPLIT>(P, i, dt, A.vr, A.s, s_prev, d.trr, dprev.trr, trp_c, trp_n,
                                            d.tpp, dprev.tpp));
             else
                 vr_o.set_raw(i, j, vr.raw(i, j));
@@ -2306,7 +2327,7 @@ __device__ __forceinline__ void b_visc_walk(const KParams &P, int bx, int by, in
                 if (upd_p) {
                     qminus = cooling_beta_cell<SPLIT>(P, i, j, P.s->time, e, A.s);
                     if (P.heating_viscous && A.n != 0.0)
-                        qplus = viscous_heating_q(P, A.n, A.s, d.trr, d.tpp, d.divv,
+                        qplus = viscous_heating_q<SPLIT>(P, A.n, A.s, d.trr, d.tpp, d.divv,
                                                   0.25 * (trp_c + trp1 + trp_n + trp1_n));
                 }
                 substep3_cell<SPLIT>(P, i, j, e, A.s, h, qplus, qminus);
