@@ -316,7 +316,8 @@ int fcpt_time_copy(fcpt_ctx *ctx, int32_t reps, double *ms_per_launch, double *b
    deterministic hash of the value, the cell, the field and the step count
    drawing the variate: unbiased, so a slowly changing value does not lose its
    sub-unit increments every step) for the evolved fields Sigma, v_r, v_phi, E
-   and the transported momenta, to nearest for the fields derived afresh every
+   and the transported momenta between the two sweeps (after the second sweep
+   they are FP64), to nearest for the fields derived afresh every
    step (1); stochastically for every field (2); to nearest everywhere (0). */
 /* FCPT_PARAM_REBASE_INTERVAL (split storage only; default 1000) moves the
    backgrounds to the current ring means (fcpt_rebase_backgrounds) after every
